@@ -1,0 +1,221 @@
+/*
+ * sspt.h -- C ABI of the MI355X-native SeeSharp path tracer (libsspt.so).
+ *
+ * The library is the GPU replacement for the CPU hot path of SeeSharp's
+ * unidirectional `PathTracer` and of the TinyEmbree ray tracer underneath it.
+ * Every entry point below names the reference interface it replaces.
+ *
+ * Conventions
+ *   - Every call returns an sspt_status (0 = OK); nothing throws across the ABI.
+ *     The message of the last failure is per context: sspt_last_error().
+ *   - Host arrays passed in are owned by the caller and are copied; device
+ *     memory is owned by the context / scene handles.
+ *   - One HIP stream per context. A context is not thread-safe, mirroring
+ *     `Integrator.Render`, which renders one scene at a time
+ *     (SeeSharp/Integrators/PathTracer.cs:196-200).
+ *   - Matrices use System.Numerics.Matrix4x4 layout: row-major M11..M44 with
+ *     row vectors (v' = v * M), as SeeSharp/Cameras/Camera.cs:34-42 stores them.
+ */
+#ifndef SSPT_H
+#define SSPT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SSPT_ABI_VERSION 1
+
+typedef enum sspt_status {
+    SSPT_OK = 0,
+    SSPT_ERR_INVALID_ARGUMENT = 1,
+    /* Scene.Prepare on an invalid scene (SeeSharp/Scene.cs:104-105,170-197) */
+    SSPT_ERR_INVALID_SCENE = 2,
+    SSPT_ERR_DEVICE = 3,
+    SSPT_ERR_OUT_OF_MEMORY = 4,
+    SSPT_ERR_UNSUPPORTED = 5,
+    /* Triangle with zero area used as an emitter (SeeSharp/Geometry/Triangle.cs:32-33) */
+    SSPT_ERR_ZERO_AREA_EMITTER = 6
+} sspt_status;
+
+typedef struct sspt_ctx sspt_ctx;
+typedef struct sspt_scene sspt_scene;
+
+/* One `Mesh` (SeeSharp/Geometry/Mesh.cs:6-30). PrimId stays mesh-local. */
+typedef struct sspt_mesh_desc {
+    uint32_t num_vertices;
+    const float* positions;   /* 3 * num_vertices */
+    const float* normals;     /* 3 * num_vertices, or NULL: shading normal = face normal */
+    const float* uvs;         /* 2 * num_vertices, or NULL: texture coordinates (0,0) */
+    uint32_t num_triangles;
+    const int32_t* indices;   /* 3 * num_triangles */
+    int32_t material;         /* index into sspt_scene_desc.materials, -1 = none (invalid) */
+} sspt_mesh_desc;
+
+typedef enum sspt_material_kind {
+    SSPT_MATERIAL_DIFFUSE = 0, /* DiffuseMaterial (SeeSharp/Shading/Materials/DiffuseMaterial.cs:6) */
+    SSPT_MATERIAL_GENERIC = 1  /* GenericMaterial (SeeSharp/Shading/Materials/GenericMaterial.cs:3) */
+} sspt_material_kind;
+
+/* Tagged union of DiffuseMaterial.Parameters (DiffuseMaterial.cs:193) and
+ * GenericMaterial.Parameters (GenericMaterial.cs:22-60). Texture ids index
+ * sspt_scene_desc.textures; -1 = the constant value. */
+typedef struct sspt_material_desc {
+    int32_t kind;
+    float base_color[3];
+    int32_t base_color_texture;     /* RGB texture or -1 */
+    float roughness;
+    int32_t roughness_texture;      /* mono texture or -1 */
+    float metallic;
+    float specular_tint_strength;
+    float anisotropic;
+    float specular_transmittance;
+    float index_of_refraction;
+    int32_t transmitter;            /* DiffuseMaterial.Parameters.Transmitter */
+} sspt_material_desc;
+
+/* TextureRgb / TextureMono image (SeeSharp/Images/ImageTexture.cs:6-50). */
+typedef struct sspt_texture_desc {
+    uint32_t width, height;
+    uint32_t channels;     /* 3 (RGB) or 1 (mono) */
+    int32_t border;        /* 0 = Repeat (default), 1 = Clamp */
+    const float* texels;   /* row-major, channels floats per texel */
+} sspt_texture_desc;
+
+/* DiffuseEmitter on one triangle (SeeSharp/Shading/Emitters/DiffuseEmitter.cs:6-37),
+ * listed in Scene.Emitters order: next-event estimation indexes that order. */
+typedef struct sspt_emitter_desc {
+    uint32_t mesh;
+    uint32_t face;
+    float radiance[3];
+} sspt_emitter_desc;
+
+/* EnvironmentMap lat-long image (SeeSharp/Shading/Background/EnvironmentMap.cs:13-189). */
+typedef struct sspt_envmap_desc {
+    uint32_t width, height;
+    const float* rgb;      /* row-major RGB */
+} sspt_envmap_desc;
+
+/* PerspectiveCamera without depth of field (SeeSharp/Cameras/PerspectiveCamera.cs:6-52). */
+typedef struct sspt_camera_desc {
+    float world_to_camera[16];
+    float vertical_fov_degrees;
+} sspt_camera_desc;
+
+/* The `Scene` (SeeSharp/Scene.cs:6-60), flattened in Meshes / Emitters order. */
+typedef struct sspt_scene_desc {
+    uint32_t num_meshes;
+    const sspt_mesh_desc* meshes;
+    uint32_t num_materials;
+    const sspt_material_desc* materials;
+    uint32_t num_textures;
+    const sspt_texture_desc* textures;
+    uint32_t num_emitters;
+    const sspt_emitter_desc* emitters;
+    const sspt_envmap_desc* background;  /* NULL = no background */
+    sspt_camera_desc camera;
+    uint32_t width, height;              /* FrameBuffer resolution */
+} sspt_scene_desc;
+
+/* Public fields of PathTracerBase (SeeSharp/Integrators/PathTracer.cs:16-49) and
+ * Integrator (SeeSharp/Integrators/Integrator.cs:10-16). */
+typedef struct sspt_pt_settings {
+    uint32_t base_seed;          /* BaseSeed, default 0xC030114 */
+    int32_t total_spp;           /* TotalSpp */
+    int32_t max_depth;           /* MaxDepth */
+    int32_t min_depth;           /* MinDepth */
+    int32_t num_shadow_rays;     /* NumShadowRays */
+    int32_t enable_bsdf_di;      /* EnableBsdfDI */
+    int64_t max_render_time_ms;  /* MaximumRenderTimeMs, < 0 = unlimited */
+} sspt_pt_settings;
+
+/* TinyEmbree RayTracerStats as stored by PathTracer.cs:257. */
+typedef struct sspt_ray_stats {
+    uint64_t num_rays;
+    uint64_t num_shadow_rays;
+    uint64_t num_occluded;
+    uint64_t num_ray_hits;
+} sspt_ray_stats;
+
+/* Closest-hit record of TinyEmbree `Hit` for the trace entry points. */
+typedef struct sspt_hit {
+    float t;             /* Hit.Distance, +inf on a miss */
+    float u, v;          /* Hit.BarycentricCoords (weights of vertex 1 and 2) */
+    int32_t prim;        /* global triangle id (mesh offset + PrimId), -1 on a miss */
+} sspt_hit;
+
+/* ---- context ---------------------------------------------------------- */
+
+int32_t sspt_abi_version(void);
+
+/* Creates a context on HIP device `device` with its own stream. */
+sspt_status sspt_ctx_create(int32_t device, sspt_ctx** out_ctx);
+void sspt_ctx_destroy(sspt_ctx* ctx);
+/* Message of the last failed call on this context ("" if none). */
+const char* sspt_last_error(const sspt_ctx* ctx);
+
+/* ---- scene: replaces Scene.Prepare (SeeSharp/Scene.cs:103-164), i.e. the
+ *      TinyEmbree Raytracer.AddMesh/CommitScene BVH build (:108-113), the
+ *      emitter maps (:147-163) and the environment-map sampling grid
+ *      (SeeSharp/Shading/Background/EnvironmentMap.cs:139-156). --------- */
+sspt_status sspt_scene_create(sspt_ctx* ctx, const sspt_scene_desc* desc,
+                              sspt_scene** out_scene);
+void sspt_scene_destroy(sspt_scene* scene);
+/* Triangle count, emitter count, BVH node count and device bytes held. */
+sspt_status sspt_scene_info(const sspt_scene* scene, uint64_t* num_triangles,
+                            uint64_t* num_emitters, uint64_t* num_bvh_nodes,
+                            uint64_t* device_bytes);
+
+/* ---- framebuffer: the scene's FrameBuffer.Image running mean
+ *      (SeeSharp/Images/FrameBuffer.cs:89-160). ------------------------- */
+/* Zeroes the device image (FrameBuffer with CurIteration = 0). */
+sspt_status sspt_framebuffer_clear(sspt_ctx* ctx, sspt_scene* scene);
+/* Copies the W*H*3 fp32 running mean to host memory. */
+sspt_status sspt_framebuffer_read(sspt_ctx* ctx, sspt_scene* scene, float* host_rgb);
+/* Number of non-finite path estimates skipped so far (FrameBuffer.NaNWarnings count). */
+sspt_status sspt_framebuffer_nan_count(sspt_ctx* ctx, sspt_scene* scene, uint64_t* count);
+
+/* ---- rendering: replaces PathTracerBase.Render's iteration loop
+ *      (SeeSharp/Integrators/PathTracer.cs:220-252), RenderPixel (:287-313)
+ *      and EstimateIncidentRadiance (:315-382). -------------------------
+ * Renders iterations first_iteration .. first_iteration+num_iterations-1
+ * (one sample per pixel each, RNG seeded by (base_seed, row*W+col, iteration))
+ * into the scene framebuffer with FrameBuffer running-mean semantics: at
+ * iteration n (1-based) the image is scaled by (n-1)/n and the path estimate
+ * divided by n is added. `stats` (may be NULL) receives the ray counters of
+ * this call. Blocks until the work is done. */
+sspt_status sspt_render(sspt_ctx* ctx, sspt_scene* scene, const sspt_pt_settings* settings,
+                        uint32_t first_iteration, uint32_t num_iterations,
+                        sspt_ray_stats* stats);
+
+/* ---- ray queries: TinyEmbree Raytracer.Trace (PathTracer.cs:319) and
+ *      IsOccluded (:467) over the scene BVH, for callers and parity tests.
+ * rays: n records of 8 floats (ox,oy,oz,tmin, dx,dy,dz,tmax). --------- */
+sspt_status sspt_trace_closest(sspt_ctx* ctx, sspt_scene* scene, const float* host_rays,
+                               uint64_t n, sspt_hit* host_hits);
+sspt_status sspt_trace_occluded(sspt_ctx* ctx, sspt_scene* scene, const float* host_rays,
+                                uint64_t n, uint8_t* host_occluded);
+
+/* Primary rays of one iteration: RNG jitter + PerspectiveCamera.GenerateRay
+ * (PathTracer.cs:289-291, PerspectiveCamera.cs:64-102) followed by the closest
+ * hit. rays_out: W*H*8 floats as above (may be NULL); hits_out: W*H (may be NULL). */
+sspt_status sspt_primary_hits(sspt_ctx* ctx, sspt_scene* scene, uint32_t base_seed,
+                              uint32_t iteration, float* host_rays_out,
+                              sspt_hit* host_hits_out);
+
+/* ---- measurement ------------------------------------------------------- */
+/* Enables HIP-event timing of every kernel launch on the context stream. */
+sspt_status sspt_set_kernel_timing(sspt_ctx* ctx, int32_t enable);
+/* Accumulated launches and milliseconds of kernel `name` since the last reset
+ * ("extend", "shadow", "shade", "generate", "accumulate"). */
+sspt_status sspt_get_kernel_timing(sspt_ctx* ctx, const char* name, uint64_t* launches,
+                                   double* total_ms);
+sspt_status sspt_reset_kernel_timing(sspt_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SSPT_H */

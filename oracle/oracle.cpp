@@ -1,0 +1,1755 @@
+// oracle.cpp -- CPU restatement of SeeSharp's unidirectional PathTracer (fp32,
+// reference operation order, no contraction, no fast-math).
+//
+// TEST INFRASTRUCTURE ONLY: the checker for the GPU path and the CPU baseline
+// timed by bench.py. The product never links, loads or calls this file.
+//
+// Rules this restatement fixes where the reference delegates to packages that
+// are not in the reference tree (TinyEmbree 1.2.0 / Embree, SimpleImageIO 1.8.3,
+// .NET System.Numerics); the GPU library follows the same rules:
+//   * vector sums are evaluated left to right, x*x + y*y + z*z, without FMA;
+//   * Vector3.Normalize(v) = v / sqrt(dot(v, v)) per component;
+//   * Vector3.Transform(v, M) = ((x*M1 + y*M2) + z*M3) + M4 (row vectors);
+//   * ray/triangle: watertight test (Woop, Benthin, Wald, JCGT 2013) with the
+//     double-precision fallback on zero edge functions; ties in t go to the
+//     lower global triangle id;
+//   * hit position / shading normal / uv = u*a1 + v*a2 + (1-u-v)*a0, the order
+//     of Mesh.ComputeErrorOffset (SeeSharp/Geometry/Mesh.cs:50-60);
+//   * SpawnRay: origin = P + sign(dot(d, Ng)) * ErrorOffset * Ng, tmin = ErrorOffset;
+//     IsOccluded: both end points offset the same way towards each other,
+//     d = p1 - p0, t in (1e-5, 1 - 1e-5); LeavesScene: SpawnRay to t = inf;
+//   * RgbColor.Luminance = 0.2126 r + 0.7152 g + 0.0722 b; image lookups clamp
+//     the pixel index into the image;
+//   * MathF.ReciprocalSqrtEstimate is replaced by 1 / sqrt.
+#include "oracle.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cfloat>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+namespace {
+
+constexpr float kPi = 3.14159265358979323846f;
+
+// ---------------------------------------------------------------- vectors --
+struct V3 {
+    float x, y, z;
+};
+struct V2 {
+    float x, y;
+};
+
+inline V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+inline V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3 operator-(V3 a) { return {-a.x, -a.y, -a.z}; }
+inline V3 operator*(V3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+inline V3 operator*(float s, V3 a) { return {s * a.x, s * a.y, s * a.z}; }
+inline V3 operator*(V3 a, V3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+inline V3 operator/(V3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+inline V3 operator/(V3 a, V3 b) { return {a.x / b.x, a.y / b.y, a.z / b.z}; }
+inline bool operator==(V3 a, V3 b) { return a.x == b.x && a.y == b.y && a.z == b.z; }
+inline bool operator!=(V3 a, V3 b) { return !(a == b); }
+inline float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V3 cross(V3 a, V3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+inline float length(V3 a) { return std::sqrt(dot(a, a)); }
+inline V3 normalize(V3 a) { return a / length(a); }
+inline V3 vabs(V3 a) { return {std::fabs(a.x), std::fabs(a.y), std::fabs(a.z)}; }
+inline float get(V3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
+
+// RgbColor (SimpleImageIO): component-wise float3.
+using Rgb = V3;
+inline Rgb rgb(float v) { return {v, v, v}; }
+inline float average(Rgb c) { return (c.x + c.y + c.z) / 3.0f; }
+inline float luminance(Rgb c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
+inline Rgb lerp(float w, Rgb a, Rgb b) { return (1.0f - w) * a + w * b; }
+inline bool is_black(Rgb c) { return c.x == 0.0f && c.y == 0.0f && c.z == 0.0f; }
+
+// C# (int)float on x86: truncation, out-of-range and NaN -> int.MinValue.
+inline int to_int(float f) {
+    if (!(f > -2147483904.0f && f < 2147483648.0f)) return INT_MIN;
+    return (int)f;
+}
+
+// ---------------------------------------------------------------- RNG ------
+// SeeSharp/Sampling/RNG.cs:90-118
+inline uint32_t pcg_hash(uint32_t input) {
+    uint32_t state = input * 747796405u + 2891336453u;
+    uint32_t word = ((state >> ((state >> 28) + 4)) ^ state) * 277803737u;
+    return (word >> 22) ^ word;
+}
+inline uint32_t fnv_hash(uint32_t hash, uint32_t data) {
+    const uint32_t prime = 16777619u;
+    hash = (hash * prime) ^ (data & 0xFF);
+    hash = (hash * prime) ^ ((data >> 8) & 0xFF);
+    hash = (hash * prime) ^ ((data >> 16) & 0xFF);
+    hash = (hash * prime) ^ ((data >> 24) & 0xFF);
+    return hash;
+}
+inline uint32_t hash_seed(uint32_t base, uint32_t chain, uint32_t sample) {
+    uint32_t h = fnv_hash(2166136261u, pcg_hash(base));
+    h = fnv_hash(h, pcg_hash(chain));
+    h = fnv_hash(h, pcg_hash(sample));
+    return pcg_hash(h);
+}
+
+struct Rng {
+    uint32_t state;
+    // RNG.Next (RNG.cs:90-94)
+    uint32_t next() {
+        uint32_t word = ((state >> ((state >> 28) + 4)) ^ state) * 277803737u;
+        state = state * 747796405u + 2891336453u;
+        return (word >> 22) ^ word;
+    }
+    // RNG.NextFloat (RNG.cs:43-45): u32 -> f32 (round to nearest) / 2^32.
+    float next_float() { return (float)next() / 4294967296.0f; }
+    V2 next_float2() {
+        float a = next_float();
+        float b = next_float();
+        return {a, b};
+    }
+    // RNG.NextInt(uint) (RNG.cs:56-70), Lemire's multiply-shift with rejection.
+    uint32_t next_int(uint32_t max) {
+        uint32_t x = next();
+        uint64_t m = (uint64_t)x * max;
+        uint32_t l = (uint32_t)m;
+        if (l < max) {
+            uint32_t t = (0u - max) % max;
+            while (l < t) {
+                x = next();
+                m = (uint64_t)x * max;
+                l = (uint32_t)m;
+            }
+        }
+        return (uint32_t)(m >> 32);
+    }
+};
+
+// ---------------------------------------------------------------- warps ----
+// SampleWarp.ToUniformTriangle (SampleWarp.cs:9-14)
+inline V2 to_uniform_triangle(V2 p) {
+    float s = std::sqrt(p.x);
+    return {1.0f - s, p.y * s};
+}
+// SampleWarp.SphericalToCartesian (SampleWarp.cs:89-96)
+inline V3 spherical_to_cartesian(float sintheta, float costheta, float phi) {
+    float sinphi = std::sin(phi), cosphi = std::cos(phi);
+    return {sintheta * cosphi, sintheta * sinphi, costheta};
+}
+struct DirSample {
+    V3 dir;
+    float pdf;
+};
+// SampleWarp.ToCosHemisphere (SampleWarp.cs:130-137)
+inline DirSample to_cos_hemisphere(V2 p) {
+    V3 d = spherical_to_cartesian(std::sqrt(1.0f - p.y), std::sqrt(p.y), 2.0f * kPi * p.x);
+    return {d, d.z / kPi};
+}
+
+// ------------------------------------------------------- piecewise const ---
+// PiecewiseConstantPDF (SeeSharp/Sampling/PiecewiseConstant.cs:12-70)
+struct Pwc {
+    std::vector<float> cdf;
+    void build(const float* w, int n) {
+        cdf.resize(n);
+        float sum = 0.0f;
+        for (int i = 0; i < n; ++i) cdf[i] = (sum += w[i]);
+        float total = cdf[n - 1];
+        for (int i = 0; i < n && total > 0.0f; ++i) cdf[i] /= total;
+        cdf[n - 1] = 1.0f;
+    }
+    // List.BinarySearch + walk back to the first equal element == lower_bound.
+    void sample(float u, int* idx_out, float* rel_out) const {
+        int idx = (int)(std::lower_bound(cdf.begin(), cdf.end(), u) - cdf.begin());
+        float lo = idx == 0 ? 0.0f : cdf[idx - 1];
+        float delta = cdf[idx] - lo;
+        *idx_out = idx;
+        *rel_out = (u - lo) / delta;
+    }
+    float probability(int idx) const { return idx > 0 ? cdf[idx] - cdf[idx - 1] : cdf[idx]; }
+};
+
+// RegularGrid2d (SeeSharp/Sampling/RegularGrid2d.cs:20-81), Normalize path.
+struct Grid2d {
+    int cols = 0, rows = 0;
+    std::vector<float> density, marginals;
+    Pwc row_dist;
+    std::vector<Pwc> col_dist;
+    std::vector<char> col_valid;
+    void init(int c, int r) {
+        cols = c;
+        rows = r;
+        density.assign((size_t)c * r, 0.0f);
+        marginals.assign(r, 0.0f);
+    }
+    void splat(int col, int row, float v) {
+        density[(size_t)row * cols + col] += v;
+        marginals[row] += v;
+    }
+    void normalize() {
+        row_dist.build(marginals.data(), rows);
+        col_dist.resize(rows);
+        col_valid.assign(rows, 0);
+        for (int i = 0; i < rows; ++i) {
+            if (marginals[i] == 0.0f) continue;
+            col_dist[i].build(&density[(size_t)i * cols], cols);
+            col_valid[i] = 1;
+        }
+    }
+    V2 sample(V2 p) const {
+        int ri, ci;
+        float rr, rc;
+        row_dist.sample(p.y, &ri, &rr);
+        col_dist[ri].sample(p.x, &ci, &rc);
+        return {((float)ci + rc) / (float)cols, ((float)ri + rr) / (float)rows};
+    }
+    float pdf(V2 pos) const {
+        int row = std::min(std::max(to_int(pos.y * (float)rows), 0), rows - 1);
+        int col = std::min(std::max(to_int(pos.x * (float)cols), 0), cols - 1);
+        float p = row_dist.probability(row);
+        if (p == 0.0f) return 0.0f;
+        p *= col_dist[row].probability(col);
+        return p * (float)(rows * cols);
+    }
+};
+
+// ---------------------------------------------------------------- camera ---
+// System.Numerics.Matrix4x4, row-major, row vectors.
+struct M4 {
+    float m[16];
+    float at(int r, int c) const { return m[r * 4 + c]; }
+};
+
+// General inverse by cofactors in double, rounded to float. The reference uses
+// Matrix4x4.Invert (Camera.cs:38, PerspectiveCamera.cs:48), whose last-ulp
+// behaviour is not pinned; the GPU library uses this same rule.
+bool invert(const M4& a, M4* out) {
+    double m[16], inv[16];
+    for (int i = 0; i < 16; ++i) m[i] = a.m[i];
+    inv[0] = m[5] * m[10] * m[15] - m[5] * m[11] * m[14] - m[9] * m[6] * m[15] +
+             m[9] * m[7] * m[14] + m[13] * m[6] * m[11] - m[13] * m[7] * m[10];
+    inv[4] = -m[4] * m[10] * m[15] + m[4] * m[11] * m[14] + m[8] * m[6] * m[15] -
+             m[8] * m[7] * m[14] - m[12] * m[6] * m[11] + m[12] * m[7] * m[10];
+    inv[8] = m[4] * m[9] * m[15] - m[4] * m[11] * m[13] - m[8] * m[5] * m[15] +
+             m[8] * m[7] * m[13] + m[12] * m[5] * m[11] - m[12] * m[7] * m[9];
+    inv[12] = -m[4] * m[9] * m[14] + m[4] * m[10] * m[13] + m[8] * m[5] * m[14] -
+              m[8] * m[6] * m[13] - m[12] * m[5] * m[10] + m[12] * m[6] * m[9];
+    inv[1] = -m[1] * m[10] * m[15] + m[1] * m[11] * m[14] + m[9] * m[2] * m[15] -
+             m[9] * m[3] * m[14] - m[13] * m[2] * m[11] + m[13] * m[3] * m[10];
+    inv[5] = m[0] * m[10] * m[15] - m[0] * m[11] * m[14] - m[8] * m[2] * m[15] +
+             m[8] * m[3] * m[14] + m[12] * m[2] * m[11] - m[12] * m[3] * m[10];
+    inv[9] = -m[0] * m[9] * m[15] + m[0] * m[11] * m[13] + m[8] * m[1] * m[15] -
+             m[8] * m[3] * m[13] - m[12] * m[1] * m[11] + m[12] * m[3] * m[9];
+    inv[13] = m[0] * m[9] * m[14] - m[0] * m[10] * m[13] - m[8] * m[1] * m[14] +
+              m[8] * m[2] * m[13] + m[12] * m[1] * m[10] - m[12] * m[2] * m[9];
+    inv[2] = m[1] * m[6] * m[15] - m[1] * m[7] * m[14] - m[5] * m[2] * m[15] +
+             m[5] * m[3] * m[14] + m[13] * m[2] * m[7] - m[13] * m[3] * m[6];
+    inv[6] = -m[0] * m[6] * m[15] + m[0] * m[7] * m[14] + m[4] * m[2] * m[15] -
+             m[4] * m[3] * m[14] - m[12] * m[2] * m[7] + m[12] * m[3] * m[6];
+    inv[10] = m[0] * m[5] * m[15] - m[0] * m[7] * m[13] - m[4] * m[1] * m[15] +
+              m[4] * m[3] * m[13] + m[12] * m[1] * m[7] - m[12] * m[3] * m[5];
+    inv[14] = -m[0] * m[5] * m[14] + m[0] * m[6] * m[13] + m[4] * m[1] * m[14] -
+              m[4] * m[2] * m[13] - m[12] * m[1] * m[6] + m[12] * m[2] * m[5];
+    inv[3] = -m[1] * m[6] * m[11] + m[1] * m[7] * m[10] + m[5] * m[2] * m[11] -
+             m[5] * m[3] * m[10] - m[9] * m[2] * m[7] + m[9] * m[3] * m[6];
+    inv[7] = m[0] * m[6] * m[11] - m[0] * m[7] * m[10] - m[4] * m[2] * m[11] +
+             m[4] * m[3] * m[10] + m[8] * m[2] * m[7] - m[8] * m[3] * m[6];
+    inv[11] = -m[0] * m[5] * m[11] + m[0] * m[7] * m[9] + m[4] * m[1] * m[11] -
+              m[4] * m[3] * m[9] - m[8] * m[1] * m[7] + m[8] * m[3] * m[5];
+    inv[15] = m[0] * m[5] * m[10] - m[0] * m[6] * m[9] - m[4] * m[1] * m[10] +
+              m[4] * m[2] * m[9] + m[8] * m[1] * m[6] - m[8] * m[2] * m[5];
+    double det = m[0] * inv[0] + m[1] * inv[4] + m[2] * inv[8] + m[3] * inv[12];
+    if (det == 0.0) return false;
+    for (int i = 0; i < 16; ++i) out->m[i] = (float)(inv[i] / det);
+    return true;
+}
+
+// Vector3.Transform(position, M): row vector (x, y, z, 1).
+inline V3 transform_point(V3 p, const M4& M) {
+    return {p.x * M.at(0, 0) + p.y * M.at(1, 0) + p.z * M.at(2, 0) + M.at(3, 0),
+            p.x * M.at(0, 1) + p.y * M.at(1, 1) + p.z * M.at(2, 1) + M.at(3, 1),
+            p.x * M.at(0, 2) + p.y * M.at(1, 2) + p.z * M.at(2, 2) + M.at(3, 2)};
+}
+// Vector4.Transform((d, 0), M).xyz
+inline V3 transform_dir(V3 d, const M4& M) {
+    return {d.x * M.at(0, 0) + d.y * M.at(1, 0) + d.z * M.at(2, 0),
+            d.x * M.at(0, 1) + d.y * M.at(1, 1) + d.z * M.at(2, 1),
+            d.x * M.at(0, 2) + d.y * M.at(1, 2) + d.z * M.at(2, 2)};
+}
+
+// Matrix4x4.CreatePerspectiveFieldOfView (right-handed), values in fp32.
+M4 perspective_fov(float fov, float aspect, float n, float f) {
+    M4 r;
+    std::memset(r.m, 0, sizeof(r.m));
+    float y_scale = 1.0f / std::tan(fov * 0.5f);
+    float x_scale = y_scale / aspect;
+    r.m[0] = x_scale;
+    r.m[5] = y_scale;
+    float range = f / (n - f);
+    r.m[10] = range;
+    r.m[11] = -1.0f;
+    r.m[14] = range * n;
+    return r;
+}
+
+struct Camera {
+    M4 world_to_camera, camera_to_world, view_to_camera;
+    int width = 0, height = 0;
+    V3 position;
+    // Camera.WorldToCamera setter (Camera.cs:34-42) + PerspectiveCamera ctor and
+    // UpdateResolution (PerspectiveCamera.cs:29-52).
+    bool init(const float* w2c, float fov_deg, int w, int h) {
+        std::memcpy(world_to_camera.m, w2c, sizeof(float) * 16);
+        if (!invert(world_to_camera, &camera_to_world)) return false;
+        float fov_rad = fov_deg * kPi / 180.0f;
+        width = w;
+        height = h;
+        float aspect = (float)width / (float)height;
+        M4 camera_to_view = perspective_fov(fov_rad, aspect, 0.001f, 1000.0f);
+        if (!invert(camera_to_view, &view_to_camera)) return false;
+        position = transform_point(v3(0, 0, 0), camera_to_world);
+        return true;
+    }
+    // PerspectiveCamera.GenerateRay (PerspectiveCamera.cs:64-102), lensRadius = 0.
+    void generate_ray(float fx, float fy, V3* org, V3* dir) const {
+        V3 view = v3(2.0f * fx / (float)width - 1.0f, 1.0f - 2.0f * fy / (float)height, 0.0f);
+        V3 local = transform_point(view, view_to_camera);
+        V3 d = transform_dir(local, camera_to_world);
+        *org = position;
+        *dir = normalize(d);
+    }
+};
+
+// ----------------------------------------------------------- shading space -
+// ShadingSpace.ComputeBasisVectors (ShadingSpace.cs:14-23)
+inline void basis(V3 n, V3* t, V3* b) {
+    if (std::fabs(n.x) > std::fabs(n.y)) {
+        float denom = std::sqrt(n.x * n.x + n.z * n.z);
+        *t = v3(-n.z, 0.0f, n.x) / denom;
+    } else {
+        float denom = std::sqrt(n.y * n.y + n.z * n.z);
+        *t = v3(0.0f, n.z, -n.y) / denom;
+    }
+    *b = cross(n, *t);
+}
+struct Frame {
+    V3 n, t, b;
+    // ShadingSpace.WorldToShading (ShadingSpace.cs:40-49)
+    V3 to_local(V3 d) const { return {dot(t, d), dot(b, d), dot(n, d)}; }
+    // ShadingSpace.ShadingToWorld (ShadingSpace.cs:62-71)
+    V3 to_world(V3 d) const { return d.z * n + d.x * t + d.y * b; }
+};
+inline float cos_theta(V3 d) { return d.z; }
+inline float cos2_theta(V3 d) { return d.z * d.z; }
+inline float abs_cos_theta(V3 d) { return std::fabs(d.z); }
+inline float sin2_theta(V3 d) { return std::max(0.0f, 1.0f - cos2_theta(d)); }
+inline float sin_theta(V3 d) { return std::sqrt(sin2_theta(d)); }
+inline float tan_theta(V3 d) { return sin_theta(d) / cos_theta(d); }
+inline float tan2_theta(V3 d) { return sin2_theta(d) / cos2_theta(d); }
+inline float cos_phi(V3 d) {
+    float s = sin_theta(d);
+    return s == 0.0f ? 1.0f : std::min(std::max(d.x / s, -1.0f), 1.0f);
+}
+inline float sin_phi(V3 d) {
+    float s = sin_theta(d);
+    return s == 0.0f ? 0.0f : std::min(std::max(d.y / s, -1.0f), 1.0f);
+}
+inline float cos2_phi(V3 d) {
+    float c = cos_phi(d);
+    return c * c;
+}
+inline float sin2_phi(V3 d) {
+    float s = sin_phi(d);
+    return s * s;
+}
+// ShadingSpace.Reflect / Refract / SameHemisphere (ShadingSpace.cs:109-140)
+inline V3 reflect(V3 o, V3 n) { return -o + 2.0f * dot(o, n) * n; }
+inline bool refract(V3 i, V3 n, float eta, V3* out) {
+    float cos_i = dot(n, i);
+    float sin2_i = std::max(0.0f, 1.0f - cos_i * cos_i);
+    float sin2_t = eta * eta * sin2_i;
+    if (sin2_t >= 1.0f) return false;
+    float cos_t = std::sqrt(1.0f - sin2_t);
+    *out = eta * -i + (eta * cos_i - cos_t) * n;
+    return true;
+}
+inline bool same_hemisphere(V3 a, V3 b) { return a.z * b.z > 0.0f; }
+
+// ------------------------------------------------------------- microfacet --
+// TrowbridgeReitzDistribution (SeeSharp/Shading/TrowbridgeReitzDistribution.cs)
+struct Ggx {
+    float ax, ay;
+    float ndf(V3 n) const {  // :22-34
+        float tan2 = tan2_theta(n);
+        if (std::isinf(tan2)) return 0.0f;
+        float cos4 = cos2_theta(n) * cos2_theta(n);
+        float e = tan2 * (cos2_phi(n) / (ax * ax) + sin2_phi(n) / (ay * ay));
+        return 1.0f / (kPi * ax * ay * cos4 * (1.0f + e) * (1.0f + e));
+    }
+    float masking_ratio(V3 o) const {  // :104-113
+        float abs_tan = std::fabs(tan_theta(o));
+        if (std::isinf(abs_tan)) return 0.0f;
+        float alpha = std::sqrt(cos2_phi(o) * ax * ax + sin2_phi(o) * ay * ay);
+        float a2t2 = alpha * abs_tan * (alpha * abs_tan);
+        return (-1.0f + std::sqrt(1.0f + a2t2)) / 2.0f;
+    }
+    float g1(V3 o) const { return 1.0f / (1.0f + masking_ratio(o)); }
+    float g(V3 o, V3 i) const { return 1.0f / (1.0f + masking_ratio(o) + masking_ratio(i)); }
+    float pdf(V3 o, V3 n) const {  // :59-63
+        if (!same_hemisphere(o, n)) return 0.0f;
+        return g1(o) * std::max(0.0f, dot(o, n)) * ndf(n) / std::fabs(o.z);
+    }
+    V3 sample(V3 o, V2 p) const {  // :70-100, Heitz 2018 VNDF
+        bool flip = false;
+        if (o.z < 0.0f) {
+            flip = true;
+            o = -o;
+        }
+        V3 vh = normalize(v3(ax * o.x, ay * o.y, o.z));
+        float lensq = vh.x * vh.x + vh.y * vh.y;
+        V3 t1 = lensq > 0.0f ? v3(-vh.y, vh.x, 0.0f) * (1.0f / std::sqrt(lensq)) : v3(1, 0, 0);
+        V3 t2 = cross(vh, t1);
+        float r = std::sqrt(p.x);
+        float phi = 2.0f * kPi * p.y;
+        float a1 = r * std::cos(phi);
+        float a2 = r * std::sin(phi);
+        float s = 0.5f * (1.0f + vh.z);
+        a2 = (1.0f - s) * std::sqrt(1.0f - a1 * a1) + s * a2;
+        V3 nh = a1 * t1 + a2 * t2 + std::sqrt(std::max(0.0f, 1.0f - a1 * a1 - a2 * a2)) * vh;
+        V3 ne = normalize(v3(ax * nh.x, ay * nh.y, std::max(0.0f, nh.z)));
+        return flip ? -ne : ne;
+    }
+};
+
+// Fresnel (SeeSharp/Shading/Fresnel.cs)
+inline float fresnel_dielectric(float cos_i, float eta_i, float eta_t) {  // :4-25
+    cos_i = std::min(std::max(cos_i, -1.0f), 1.0f);
+    bool entering = cos_i > 0.0f;
+    if (!entering) {
+        std::swap(eta_i, eta_t);
+        cos_i = std::fabs(cos_i);
+    }
+    float sin_i = std::sqrt(std::max(0.0f, 1.0f - cos_i * cos_i));
+    float sin_t = eta_i / eta_t * sin_i;
+    if (sin_t >= 1.0f) return 1.0f;
+    float cos_t = std::sqrt(std::max(0.0f, 1.0f - sin_t * sin_t));
+    float rparl = (eta_t * cos_i - eta_i * cos_t) / (eta_t * cos_i + eta_i * cos_t);
+    float rperp = (eta_i * cos_i - eta_t * cos_t) / (eta_i * cos_i + eta_t * cos_t);
+    return (rparl * rparl + rperp * rperp) / 2.0f;
+}
+inline float schlick_weight(float c) {  // :35-38
+    float m = std::min(std::max(1.0f - c, 0.0f), 1.0f);
+    return (m * m) * (m * m) * m;
+}
+inline Rgb schlick_fresnel(Rgb r0, float c) { return lerp(schlick_weight(c), r0, rgb(1.0f)); }
+inline float schlick_r0(float eta) {
+    float r = (eta - 1.0f) / (eta + 1.0f);
+    return r * r;
+}
+
+// ---------------------------------------------------------------- scene ----
+struct Texture {
+    int w = 0, h = 0, ch = 0, border = 0;
+    std::vector<float> texels;
+    // ImageTexture.ComputeTexel (ImageTexture.cs:27-44)
+    int texel(V2 uv) const {
+        int col = to_int(uv.x * (float)w);
+        int row = to_int(uv.y * (float)h);
+        if (border == 0) {
+            row = (row % h + h) % h;
+            col = (col % w + w) % w;
+        } else {
+            row = std::min(std::max(row, 0), h - 1);
+            col = std::min(std::max(col, 0), w - 1);
+        }
+        return row * w + col;
+    }
+};
+
+struct Material {
+    int kind;
+    Rgb base_color;
+    int base_tex;
+    float roughness;
+    int rough_tex;
+    float metallic, spec_tint, anisotropic, spec_trans, ior;
+    bool transmitter;
+};
+
+struct Tri {
+    V3 p0, p1, p2;
+    V3 n0, n1, n2;
+    V2 t0, t1, t2;
+    V3 face_normal;
+    int mesh, prim, material;
+    bool has_normals, has_uvs;
+};
+
+struct SurfacePoint {
+    V3 position, normal, shading_normal;
+    V2 bary, uv;
+    int tri = -1;  // global triangle id
+    float error_offset = 0.0f, distance = 0.0f;
+};
+
+struct Emitter {
+    int tri;
+    Rgb radiance;
+    float inv_area;
+};
+
+struct EnvMap {
+    int w = 0, h = 0;
+    std::vector<float> rgb;
+    Grid2d grid;
+    Rgb pixel(int col, int row) const {
+        col = std::min(std::max(col, 0), w - 1);
+        row = std::min(std::max(row, 0), h - 1);
+        const float* p = &rgb[((size_t)row * w + col) * 3];
+        return v3(p[0], p[1], p[2]);
+    }
+};
+
+struct Bvh {
+    struct Node {
+        float lo[3], hi[3];
+        int left, count;  // count > 0: leaf, left = first index into order
+    };
+    std::vector<Node> nodes;
+    std::vector<int> order;
+};
+
+}  // namespace
+
+struct or_scene {
+    std::vector<Tri> tris;
+    std::vector<Material> materials;
+    std::vector<Texture> textures;
+    std::vector<Emitter> emitters;
+    std::vector<int> tri_emitter;  // global tri -> emitter index or -1
+    bool has_env = false;
+    EnvMap env;
+    Camera camera;
+    int width = 0, height = 0;
+    Bvh bvh;
+};
+
+namespace {
+
+// ------------------------------------------------------- surface queries ---
+// Mesh.ComputeErrorOffset (Mesh.cs:50-60)
+inline float error_offset(const Tri& t, V2 b) {
+    V3 diag = vabs(b.x * t.p1) + vabs(b.y * t.p2) + vabs((1.0f - b.x - b.y) * t.p0);
+    return length(diag) * 32.0f * 1.19209e-07f;
+}
+
+SurfacePoint make_point(const or_scene& s, int tri, V2 b, float dist) {
+    const Tri& t = s.tris[tri];
+    SurfacePoint p;
+    p.tri = tri;
+    p.bary = b;
+    p.distance = dist;
+    float w0 = 1.0f - b.x - b.y;
+    p.position = b.x * t.p1 + b.y * t.p2 + w0 * t.p0;
+    p.normal = t.face_normal;
+    p.shading_normal = t.has_normals ? normalize(b.x * t.n1 + b.y * t.n2 + w0 * t.n0) : t.face_normal;
+    if (t.has_uvs) {
+        p.uv = {b.x * t.t1.x + b.y * t.t2.x + w0 * t.t0.x, b.x * t.t1.y + b.y * t.t2.y + w0 * t.t0.y};
+    } else {
+        p.uv = {0.0f, 0.0f};
+    }
+    p.error_offset = error_offset(t, b);
+    return p;
+}
+
+inline Rgb lookup_rgb(const or_scene& s, int tex, Rgb constant, V2 uv) {
+    if (tex < 0) return constant;
+    const Texture& t = s.textures[tex];
+    const float* p = &t.texels[(size_t)t.texel(uv) * 3];
+    return v3(p[0], p[1], p[2]);
+}
+inline float lookup_mono(const or_scene& s, int tex, float constant, V2 uv) {
+    if (tex < 0) return constant;
+    const Texture& t = s.textures[tex];
+    return t.texels[t.texel(uv)];
+}
+
+// ------------------------------------------------------------ intersection -
+struct Ray {
+    V3 o, d;
+    float tmin, tmax;
+};
+
+// Watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2(1) 2013).
+// Returns t and the barycentrics (u, v) of vertices 1 and 2.
+bool intersect_tri(const Ray& r, int kx, int ky, int kz, float sx, float sy, float sz, V3 p0, V3 p1,
+                   V3 p2, float* t_out, float* u_out, float* v_out) {
+    V3 a = p0 - r.o, b = p1 - r.o, c = p2 - r.o;
+    float az_ = get(a, kz), bz_ = get(b, kz), cz_ = get(c, kz);
+    float ax = get(a, kx) - sx * az_;
+    float ay = get(a, ky) - sy * az_;
+    float bx = get(b, kx) - sx * bz_;
+    float by = get(b, ky) - sy * bz_;
+    float cx = get(c, kx) - sx * cz_;
+    float cy = get(c, ky) - sy * cz_;
+    float U = cx * by - cy * bx;
+    float V = ax * cy - ay * cx;
+    float W = bx * ay - by * ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        double cxby = (double)cx * (double)by, cybx = (double)cy * (double)bx;
+        U = (float)(cxby - cybx);
+        double axcy = (double)ax * (double)cy, aycx = (double)ay * (double)cx;
+        V = (float)(axcy - aycx);
+        double bxay = (double)bx * (double)ay, byax = (double)by * (double)ax;
+        W = (float)(bxay - byax);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    float det = U + V + W;
+    if (det == 0.0f) return false;
+    float az = sz * az_, bz = sz * bz_, cz = sz * cz_;
+    float T = U * az + V * bz + W * cz;
+    float rcp = 1.0f / det;
+    float t = T * rcp;
+    if (!(t >= r.tmin && t <= r.tmax)) return false;
+    *t_out = t;
+    *u_out = V * rcp;
+    *v_out = W * rcp;
+    return true;
+}
+
+struct RayPre {
+    int kx, ky, kz;
+    float sx, sy, sz;
+    V3 inv;
+};
+inline RayPre precompute(const Ray& r) {
+    RayPre p;
+    float ax = std::fabs(r.d.x), ay = std::fabs(r.d.y), az = std::fabs(r.d.z);
+    p.kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+    p.kx = p.kz + 1 == 3 ? 0 : p.kz + 1;
+    p.ky = p.kx + 1 == 3 ? 0 : p.kx + 1;
+    if (get(r.d, p.kz) < 0.0f) std::swap(p.kx, p.ky);
+    float dz = get(r.d, p.kz);
+    p.sx = get(r.d, p.kx) / dz;
+    p.sy = get(r.d, p.ky) / dz;
+    p.sz = 1.0f / dz;
+    p.inv = v3(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
+    return p;
+}
+
+struct HitRec {
+    float t;
+    float u, v;
+    int tri;
+};
+
+inline void consider(const or_scene& s, const Ray& r, const RayPre& p, int tri, HitRec* best,
+                     float* tmax) {
+    const Tri& T = s.tris[tri];
+    float t, u, v;
+    Ray rr = r;
+    rr.tmax = *tmax;
+    if (!intersect_tri(rr, p.kx, p.ky, p.kz, p.sx, p.sy, p.sz, T.p0, T.p1, T.p2, &t, &u, &v)) return;
+    if (t < best->t || (t == best->t && tri < best->tri)) {
+        best->t = t;
+        best->u = u;
+        best->v = v;
+        best->tri = tri;
+        *tmax = t;
+    }
+}
+
+// Robust slab test (Ize, JCGT 2013): the far distance is widened by 2 gamma(3).
+inline bool box_hit(const Bvh::Node& n, const Ray& r, const RayPre& p, float tmax, float* tnear) {
+    float t0 = r.tmin, t1 = tmax;
+    for (int k = 0; k < 3; ++k) {
+        float o = get(r.o, k), inv = get(p.inv, k);
+        float a = (n.lo[k] - o) * inv, b = (n.hi[k] - o) * inv;
+        float lo = std::fmin(a, b), hi = std::fmax(a, b);
+        hi *= 1.0000004f;
+        t0 = std::fmax(t0, lo);
+        t1 = std::fmin(t1, hi);
+    }
+    *tnear = t0;
+    return t0 <= t1;
+}
+
+HitRec trace_closest(const or_scene& s, const Ray& r, bool brute) {
+    HitRec best{INFINITY, 0.0f, 0.0f, -1};
+    RayPre p = precompute(r);
+    float tmax = r.tmax;
+    if (brute || s.bvh.nodes.empty()) {
+        for (int i = 0; i < (int)s.tris.size(); ++i) consider(s, r, p, i, &best, &tmax);
+        return best;
+    }
+    int stack[128];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp) {
+        const Bvh::Node& n = s.bvh.nodes[stack[--sp]];
+        float tn;
+        if (!box_hit(n, r, p, tmax, &tn)) continue;
+        if (n.count > 0) {
+            for (int i = 0; i < n.count; ++i) consider(s, r, p, s.bvh.order[n.left + i], &best, &tmax);
+        } else {
+            float ta, tb;
+            bool ha = box_hit(s.bvh.nodes[n.left], r, p, tmax, &ta);
+            bool hb = box_hit(s.bvh.nodes[n.left + 1], r, p, tmax, &tb);
+            if (ha && hb) {
+                if (ta <= tb) {
+                    stack[sp++] = n.left + 1;
+                    stack[sp++] = n.left;
+                } else {
+                    stack[sp++] = n.left;
+                    stack[sp++] = n.left + 1;
+                }
+            } else if (ha) {
+                stack[sp++] = n.left;
+            } else if (hb) {
+                stack[sp++] = n.left + 1;
+            }
+        }
+    }
+    return best;
+}
+
+bool trace_any(const or_scene& s, const Ray& r, bool brute) {
+    RayPre p = precompute(r);
+    float t, u, v;
+    if (brute || s.bvh.nodes.empty()) {
+        for (const Tri& T : s.tris)
+            if (intersect_tri(r, p.kx, p.ky, p.kz, p.sx, p.sy, p.sz, T.p0, T.p1, T.p2, &t, &u, &v))
+                return true;
+        return false;
+    }
+    int stack[128];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp) {
+        const Bvh::Node& n = s.bvh.nodes[stack[--sp]];
+        float tn;
+        if (!box_hit(n, r, p, r.tmax, &tn)) continue;
+        if (n.count > 0) {
+            for (int i = 0; i < n.count; ++i) {
+                const Tri& T = s.tris[s.bvh.order[n.left + i]];
+                if (intersect_tri(r, p.kx, p.ky, p.kz, p.sx, p.sy, p.sz, T.p0, T.p1, T.p2, &t, &u, &v))
+                    return true;
+            }
+        } else {
+            stack[sp++] = n.left;
+            stack[sp++] = n.left + 1;
+        }
+    }
+    return false;
+}
+
+// Binned-SAH BVH over triangle centroids, 16 bins, leaves of <= 4 triangles.
+void build_bvh(or_scene& s) {
+    int n = (int)s.tris.size();
+    Bvh& b = s.bvh;
+    b.nodes.clear();
+    b.order.resize(n);
+    if (n == 0) return;
+    std::vector<float> lo((size_t)n * 3), hi((size_t)n * 3), ce((size_t)n * 3);
+    for (int i = 0; i < n; ++i) {
+        b.order[i] = i;
+        const Tri& t = s.tris[i];
+        for (int k = 0; k < 3; ++k) {
+            float a = get(t.p0, k), c = get(t.p1, k), d = get(t.p2, k);
+            lo[i * 3 + k] = std::min(a, std::min(c, d));
+            hi[i * 3 + k] = std::max(a, std::max(c, d));
+            ce[i * 3 + k] = 0.5f * (lo[i * 3 + k] + hi[i * 3 + k]);
+        }
+    }
+    struct Task {
+        int node, begin, end;
+    };
+    b.nodes.reserve(2 * (size_t)n);
+    b.nodes.push_back({});
+    std::vector<Task> stack{{0, 0, n}};
+    auto area = [](const float* l, const float* h) {
+        float dx = h[0] - l[0], dy = h[1] - l[1], dz = h[2] - l[2];
+        if (dx < 0) return 0.0f;
+        return dx * dy + dy * dz + dz * dx;
+    };
+    while (!stack.empty()) {
+        Task tk = stack.back();
+        stack.pop_back();
+        float nl[3] = {INFINITY, INFINITY, INFINITY}, nh[3] = {-INFINITY, -INFINITY, -INFINITY};
+        float cl[3] = {INFINITY, INFINITY, INFINITY}, chh[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int i = tk.begin; i < tk.end; ++i) {
+            int t = b.order[i];
+            for (int k = 0; k < 3; ++k) {
+                nl[k] = std::min(nl[k], lo[t * 3 + k]);
+                nh[k] = std::max(nh[k], hi[t * 3 + k]);
+                cl[k] = std::min(cl[k], ce[t * 3 + k]);
+                chh[k] = std::max(chh[k], ce[t * 3 + k]);
+            }
+        }
+        Bvh::Node& node = b.nodes[tk.node];
+        for (int k = 0; k < 3; ++k) {
+            node.lo[k] = nl[k];
+            node.hi[k] = nh[k];
+        }
+        int count = tk.end - tk.begin;
+        if (count <= 4) {
+            node.left = tk.begin;
+            node.count = count;
+            continue;
+        }
+        const int B = 16;
+        float best_cost = INFINITY;
+        int best_axis = -1, best_split = -1;
+        for (int k = 0; k < 3; ++k) {
+            float extent = chh[k] - cl[k];
+            if (extent <= 0.0f) continue;
+            int cnt[B] = {0};
+            float bl[B][3], bh[B][3];
+            for (int j = 0; j < B; ++j)
+                for (int q = 0; q < 3; ++q) bl[j][q] = INFINITY, bh[j][q] = -INFINITY;
+            for (int i = tk.begin; i < tk.end; ++i) {
+                int t = b.order[i];
+                int bin = std::min(B - 1, (int)((ce[t * 3 + k] - cl[k]) / extent * B));
+                cnt[bin]++;
+                for (int q = 0; q < 3; ++q) {
+                    bl[bin][q] = std::min(bl[bin][q], lo[t * 3 + q]);
+                    bh[bin][q] = std::max(bh[bin][q], hi[t * 3 + q]);
+                }
+            }
+            float ra[B], rl[3] = {INFINITY, INFINITY, INFINITY}, rh[3] = {-INFINITY, -INFINITY, -INFINITY};
+            int rc[B], rcount = 0;
+            for (int j = B - 1; j > 0; --j) {
+                rcount += cnt[j];
+                for (int q = 0; q < 3; ++q) rl[q] = std::min(rl[q], bl[j][q]), rh[q] = std::max(rh[q], bh[j][q]);
+                ra[j] = area(rl, rh);
+                rc[j] = rcount;
+            }
+            float ll[3] = {INFINITY, INFINITY, INFINITY}, lh[3] = {-INFINITY, -INFINITY, -INFINITY};
+            int lcount = 0;
+            for (int j = 0; j < B - 1; ++j) {
+                lcount += cnt[j];
+                for (int q = 0; q < 3; ++q) ll[q] = std::min(ll[q], bl[j][q]), lh[q] = std::max(lh[q], bh[j][q]);
+                if (lcount == 0 || rc[j + 1] == 0) continue;
+                float cost = area(ll, lh) * lcount + ra[j + 1] * rc[j + 1];
+                if (cost < best_cost) {
+                    best_cost = cost;
+                    best_axis = k;
+                    best_split = j;
+                }
+            }
+        }
+        int mid;
+        if (best_axis < 0) {
+            mid = tk.begin + count / 2;
+        } else {
+            int k = best_axis;
+            float extent = chh[k] - cl[k];
+            auto it = std::partition(b.order.begin() + tk.begin, b.order.begin() + tk.end, [&](int t) {
+                int bin = std::min(B - 1, (int)((ce[t * 3 + k] - cl[k]) / extent * B));
+                return bin <= best_split;
+            });
+            mid = (int)(it - b.order.begin());
+            if (mid == tk.begin || mid == tk.end) mid = tk.begin + count / 2;
+        }
+        int left = (int)b.nodes.size();
+        b.nodes.push_back({});
+        b.nodes.push_back({});
+        b.nodes[tk.node].left = left;
+        b.nodes[tk.node].count = 0;
+        stack.push_back({left + 1, mid, tk.end});
+        stack.push_back({left, tk.begin, mid});
+    }
+}
+
+// TinyEmbree Raytracer.SpawnRay
+inline Ray spawn_ray(const SurfacePoint& from, V3 dir) {
+    float sign = dot(dir, from.normal) < 0.0f ? -1.0f : 1.0f;
+    Ray r;
+    r.o = from.position + sign * from.error_offset * from.normal;
+    r.d = dir;
+    r.tmin = from.error_offset;
+    r.tmax = INFINITY;
+    return r;
+}
+inline V3 offset_point(const SurfacePoint& p, V3 dir) {
+    float sign = dot(dir, p.normal) < 0.0f ? -1.0f : 1.0f;
+    return p.position + sign * p.error_offset * p.normal;
+}
+// TinyEmbree Raytracer.IsOccluded(from, to)
+inline Ray shadow_ray(const SurfacePoint& from, const SurfacePoint& to) {
+    V3 d = to.position - from.position;
+    V3 p0 = offset_point(from, d);
+    V3 p1 = offset_point(to, -d);
+    Ray r;
+    r.o = p0;
+    r.d = p1 - p0;
+    r.tmin = 1e-5f;
+    r.tmax = 1.0f - 1e-5f;
+    return r;
+}
+
+// ----------------------------------------------------------- environment ---
+// EnvironmentMap (SeeSharp/Shading/Background/EnvironmentMap.cs)
+inline V2 world_to_spherical(V3 dir) {  // :158-166
+    dir = normalize(dir);
+    V2 sp{std::atan2(dir.z, dir.x), std::atan2(std::sqrt(dir.x * dir.x + dir.z * dir.z), dir.y)};
+    if (sp.x < 0.0f) sp.x += kPi * 2.0f;
+    return sp;
+}
+inline V3 spherical_to_world(V2 s) {  // :168-175
+    float sin_t = std::sin(s.y);
+    return v3(sin_t * std::cos(s.x), std::cos(s.y), sin_t * std::sin(s.x));
+}
+inline V2 spherical_to_pixel(V2 s) { return {s.x / (2.0f * kPi), s.y / kPi}; }
+inline V2 pixel_to_spherical(V2 p) { return {p.x * 2.0f * kPi, p.y * kPi}; }
+
+Rgb env_radiance(const EnvMap& e, V3 dir) {  // :19-26
+    V2 px = spherical_to_pixel(world_to_spherical(dir));
+    float col = px.x * (float)e.w, row = px.y * (float)e.h;
+    return e.pixel(to_int(col), to_int(row));
+}
+float env_pdf(const EnvMap& e, V3 dir) {  // :81-85
+    V2 sph = world_to_spherical(dir);
+    V2 px = spherical_to_pixel(sph);
+    return e.grid.pdf(px) / (std::sin(sph.y) * kPi * kPi * 2.0f);
+}
+struct EnvSample {
+    V3 dir;
+    float pdf;
+    Rgb weight;
+};
+EnvSample env_sample(const EnvMap& e, V2 primary) {  // :40-73
+    V2 pp = e.grid.sample(primary);
+    float pdf = e.grid.pdf(pp);
+    V2 sph = pixel_to_spherical(pp);
+    V3 dir = spherical_to_world(sph);
+    float jac = std::sin(sph.y) * kPi * kPi * 2.0f;
+    if (jac == 0.0f) return {dir, 0.0f, rgb(0.0f)};
+    pdf /= jac;
+    Rgb w = e.pixel(to_int(pp.x * (float)e.w), to_int(pp.y * (float)e.h)) / pdf;
+    return {dir, pdf, w};
+}
+
+// ------------------------------------------------------------- materials ---
+struct Shader {
+    const or_scene* s;
+    const Material* m;
+    SurfacePoint p;
+    Frame f;
+    V3 out_local, out_world;
+};
+
+inline Shader make_shader(const or_scene& s, const SurfacePoint& p, V3 out_dir) {
+    Shader sh;
+    sh.s = &s;
+    sh.m = &s.materials[s.tris[p.tri].material];
+    sh.p = p;
+    sh.f.n = p.shading_normal;
+    basis(sh.f.n, &sh.f.t, &sh.f.b);
+    sh.out_local = sh.f.to_local(out_dir);
+    sh.out_world = out_dir;
+    return sh;
+}
+
+// Material.ShouldReflect (Material.cs:127-134)
+inline bool should_reflect(const SurfacePoint& p, V3 out_dir, V3 in_dir) {
+    float a = dot(out_dir, p.normal), b = dot(in_dir, p.normal);
+    return a * b >= 0.0f;
+}
+
+struct GenericLocal {  // GenericMaterial.LocalParams (GenericMaterial.cs:340-381)
+    Rgb color_tint, spec_tint;
+    float roughness, ax, ay;
+    Rgb diffuse_refl, retro_refl, spec_trans, spec_r0;
+};
+GenericLocal generic_local(const Shader& sh) {
+    const Material& m = *sh.m;
+    GenericLocal r;
+    Rgb base = lookup_rgb(*sh.s, m.base_tex, m.base_color, sh.p.uv);
+    float lum = luminance(base);
+    r.color_tint = lum > 0.0f ? base / lum : rgb(1.0f);
+    r.spec_tint = lerp(m.spec_tint, rgb(1.0f), r.color_tint);
+    r.roughness = lookup_mono(*sh.s, m.rough_tex, m.roughness, sh.p.uv);
+    float aspect = std::sqrt(1.0f - m.anisotropic * 0.9f);
+    r.ax = std::max(0.001f, r.roughness * r.roughness / aspect);
+    r.ay = std::max(0.001f, r.roughness * r.roughness * aspect);
+    r.spec_r0 = lerp(m.metallic, schlick_r0(m.ior) * r.spec_tint, base);
+    float dw = (1.0f - m.metallic) * (1.0f - m.spec_trans);
+    r.diffuse_refl = base * dw;
+    r.spec_trans = m.spec_trans * base;
+    r.retro_refl = base * dw;
+    return r;
+}
+
+// GenericMaterial.ComputeValueAndPdf (GenericMaterial.cs:152-338), forward pdf only.
+void generic_value_pdf(const Shader& sh, V3 in, const GenericLocal& lp, Rgb* value, float* pdf_out) {
+    const Material& m = *sh.m;
+    Ggx ggx{lp.ax, lp.ay};
+    V3 out = sh.out_local;
+    float cos_o = std::fabs(out.z), cos_i = std::fabs(in.z);
+    bool same_geo = should_reflect(sh.p, sh.out_world, sh.f.to_world(in));
+    Rgb val = rgb(0.0f);
+    float pdf0 = 0.0f, pdf1 = 0.0f, pdf2 = 0.0f;
+
+    float f_out = schlick_weight(cos_o), f_in = schlick_weight(cos_i);
+    if (same_hemisphere(out, in)) {
+        if (same_geo) {
+            Rgb diffuse = lp.diffuse_refl / kPi * (1.0f - f_out * 0.5f) * (1.0f - f_in * 0.5f);
+            val = val + diffuse;
+        }
+        pdf0 = cos_i / kPi;
+    }
+
+    V3 h = out + in;
+    if (h != v3(0, 0, 0)) {
+        h = normalize(h);
+        float c_in = dot(in, h);
+        float rr = 2.0f * lp.roughness * c_in * c_in;
+        if (same_hemisphere(out, in) && same_geo) {
+            Rgb retro = lp.retro_refl / kPi * rr * (f_out + f_in + f_out * f_in * (rr - 1.0f));
+            val = val + retro;
+        }
+        if (cos_i != 0.0f && cos_o != 0.0f) {
+            float c_out = dot(out, h);
+            if (same_hemisphere(out, in) && same_geo && c_in * c_out > 0.0f) {
+                float c_tir = dot(in, h.z < 0.0f ? -h : h);
+                Rgb diel = rgb(fresnel_dielectric(c_tir, 1.0f, m.ior));
+                Rgb schl = schlick_fresnel(lp.spec_r0, c_tir);
+                Rgb fres = lerp(m.metallic, diel, schl);
+                Rgb refl = lp.spec_tint * ggx.ndf(h) * ggx.g(out, in) * fres / (4.0f * cos_i * cos_o);
+                val = val + refl;
+            }
+            float jf = std::fabs(4.0f * c_out), jr = std::fabs(4.0f * c_in);
+            if (jf != 0.0f && jr != 0.0f) pdf1 = ggx.pdf(out, h) / jf;
+        }
+    }
+
+    float eta = out.z > 0.0f ? m.ior : (1.0f / m.ior);
+    V3 ht = out + in * eta;
+    if (cos_o != 0.0f && cos_i != 0.0f && ht != v3(0, 0, 0) && m.spec_trans > 0.0f) {
+        ht = normalize(ht);
+        ht = ht.z < 0.0f ? -ht : ht;
+        float c_out = dot(out, ht), c_in = dot(in, ht);
+        float sqrt_denom = c_out + eta * c_in;
+        if (!same_hemisphere(out, in) && c_out * c_in < 0.0f) {
+            Rgb F = rgb(fresnel_dielectric(c_out, 1.0f, m.ior));
+            float factor = 1.0f;  // camera subpath
+            V3 wh = !same_hemisphere(out, ht) ? -ht : ht;
+            float num = ggx.ndf(wh) * ggx.g(out, in);
+            num *= eta * eta * std::max(0.0f, dot(in, -wh)) * std::max(0.0f, dot(out, wh));
+            num *= factor * factor;
+            float denom = in.z * out.z * sqrt_denom * sqrt_denom;
+            Rgb tr = (rgb(1.0f) - F) * lp.spec_trans * std::fabs(num / denom);
+            val = val + tr;
+        }
+        if (sqrt_denom != 0.0f) {
+            V3 wh = !same_hemisphere(out, ht) ? -ht : ht;
+            float jac = eta * eta * std::max(0.0f, dot(in, -wh)) / (sqrt_denom * sqrt_denom);
+            pdf2 += ggx.pdf(out, wh) * jac;
+        }
+    }
+
+    float schl = average(schlick_fresnel(lp.spec_r0, std::fabs(out.z)));
+    float bias = std::max(1.0f - schl, 0.75f);
+    float dw = bias * (1.0f - m.metallic) * (1.0f - m.spec_trans);
+    float fr = fresnel_dielectric(dot(out, h.z < 0.0f ? -h : h), 1.0f, m.ior);
+    float sel_refl = (1.0f - dw) * (1.0f - (1.0f - fr) * m.spec_trans);
+    float ft = fresnel_dielectric(dot(out, ht), 1.0f, m.ior);
+    float sel_trans = (1.0f - dw) * (1.0f - ft) * m.spec_trans;
+    *value = val;
+    *pdf_out = pdf0 * dw + pdf1 * sel_refl + pdf2 * sel_trans;
+}
+
+struct BsdfSample {
+    V3 dir;
+    float pdf;
+    Rgb weight;
+};
+
+// GenericMaterial.Sample (GenericMaterial.cs:92-150)
+BsdfSample generic_sample(const Shader& sh, float pc, V2 pd) {
+    const Material& m = *sh.m;
+    V3 out = sh.out_local;
+    if (out.z == 0.0f) return {v3(0, 0, 0), 0.0f, rgb(0.0f)};
+    GenericLocal lp = generic_local(sh);
+    Ggx ggx{lp.ax, lp.ay};
+    float eta = out.z > 0.0f ? (1.0f / m.ior) : m.ior;
+    float schl = average(schlick_fresnel(lp.spec_r0, std::fabs(out.z)));
+    float bias = std::max(1.0f - schl, 0.75f);
+    float dw = bias * (1.0f - m.metallic) * (1.0f - m.spec_trans);
+    V3 in;
+    if (pc <= dw) {
+        in = to_cos_hemisphere(pd).dir;
+        if (out.z < 0.0f) in.z *= -1.0f;
+    } else {
+        V3 h = ggx.sample(out, pd);
+        float c_out = dot(out, h.z < 0.0f ? -h : h);
+        float fres = fresnel_dielectric(c_out, 1.0f, m.ior);
+        float sel_t = (1.0f - fres) * m.spec_trans;
+        if ((pc - dw) / (1.0f - dw) < sel_t) {
+            if (dot(out, h) < 0.0f) return {v3(0, 0, 0), 0.0f, rgb(0.0f)};
+            V3 r;
+            if (!refract(out, h, eta, &r)) return {v3(0, 0, 0), 0.0f, rgb(0.0f)};
+            in = r;
+        } else {
+            in = reflect(out, h);
+        }
+    }
+    Rgb val;
+    float pdf;
+    generic_value_pdf(sh, in, lp, &val, &pdf);
+    BsdfSample bs;
+    bs.pdf = pdf;
+    bs.weight = val * std::fabs(in.z) / pdf;
+    bs.dir = sh.f.to_world(in);
+    return bs;
+}
+
+// DiffuseMaterial.Evaluate (DiffuseMaterial.cs:127-139)
+Rgb diffuse_eval(const Shader& sh, V3 in_world) {
+    bool refl = should_reflect(sh.p, sh.out_world, in_world);
+    V3 in = sh.f.to_local(in_world);
+    Rgb base = lookup_rgb(*sh.s, sh.m->base_tex, sh.m->base_color, sh.p.uv);
+    if (sh.m->transmitter && !refl) {
+        if (same_hemisphere(sh.out_local, in)) return rgb(0.0f);
+        return base / kPi;
+    } else if (refl) {
+        if (!same_hemisphere(sh.out_local, in)) return rgb(0.0f);
+        return base / kPi;
+    }
+    return rgb(0.0f);
+}
+// DiffuseMaterial.Pdf (DiffuseMaterial.cs:167-188), forward pdf.
+float diffuse_pdf(const Shader& sh, V3 in_world) {
+    V3 in = sh.f.to_local(in_world);
+    float refl = same_hemisphere(sh.out_local, in) ? abs_cos_theta(in) / kPi : 0.0f;
+    if (sh.m->transmitter) {
+        float trans = !same_hemisphere(sh.out_local, in) ? abs_cos_theta(in) / kPi : 0.0f;
+        return (refl + trans) * 0.5f;
+    }
+    return refl;
+}
+
+// SurfaceShader.EvaluateWithCosine (SurfaceShader.cs, Material.cs:41-44, GenericMaterial.cs:72-77)
+Rgb eval_cos(const Shader& sh, V3 in_world) {
+    if (sh.m->kind == SSPT_MATERIAL_GENERIC) {
+        V3 in = sh.f.to_local(in_world);
+        Rgb v;
+        float pdf;
+        generic_value_pdf(sh, in, generic_local(sh), &v, &pdf);
+        return v * std::fabs(in.z);
+    }
+    Rgb v = diffuse_eval(sh, in_world);
+    return v * abs_cos_theta(sh.f.to_local(in_world));
+}
+Rgb eval_nocos(const Shader& sh, V3 in_world) {
+    if (sh.m->kind == SSPT_MATERIAL_GENERIC) {
+        V3 in = sh.f.to_local(in_world);
+        Rgb v;
+        float pdf;
+        generic_value_pdf(sh, in, generic_local(sh), &v, &pdf);
+        return v;
+    }
+    return diffuse_eval(sh, in_world);
+}
+// SurfaceShader.Pdf(inDir).Pdf
+float pdf_fwd(const Shader& sh, V3 in_world) {
+    if (sh.m->kind == SSPT_MATERIAL_GENERIC) {
+        V3 in = sh.f.to_local(in_world);
+        Rgb v;
+        float pdf;
+        generic_value_pdf(sh, in, generic_local(sh), &v, &pdf);
+        return pdf;
+    }
+    return diffuse_pdf(sh, in_world);
+}
+// SurfaceShader.Sample (DiffuseMaterial.cs:141-165, GenericMaterial.cs:92-150)
+BsdfSample sample_bsdf(const Shader& sh, float pc, V2 pd) {
+    if (sh.m->kind == SSPT_MATERIAL_GENERIC) return generic_sample(sh, pc, pd);
+    DirSample loc = to_cos_hemisphere(pd);
+    V3 d = loc.dir;
+    if (sh.m->transmitter && pc < 0.5f) {
+        if (cos_theta(sh.out_local) > 0.0f) d.z *= -1.0f;
+    } else {
+        if (cos_theta(sh.out_local) < 0.0f) d.z *= -1.0f;
+    }
+    // ShadingToWorld(context.Normal, sample) recomputes the same basis.
+    V3 dir = sh.f.to_world(d);
+    Rgb value = eval_cos(sh, dir);
+    float pdf = diffuse_pdf(sh, dir);
+    if (pdf == 0.0f) return {v3(0, 0, 0), 0.0f, rgb(0.0f)};
+    return {dir, pdf, value / pdf};
+}
+
+// ----------------------------------------------------------- path tracer ---
+struct Counters {
+    uint64_t rays = 0, shadow = 0, occluded = 0, hits = 0;
+};
+
+// SampleWarp.SurfaceAreaToSolidAngle (SampleWarp.cs:290-295)
+inline float area_to_solid_angle(const SurfacePoint& from, const SurfacePoint& to) {
+    V3 d = to.position - from.position;
+    float d2 = dot(d, d);
+    return std::fabs(dot(to.normal, -d)) / (d2 * std::sqrt(d2));
+}
+
+// DiffuseEmitter.EmittedRadiance (DiffuseEmitter.cs:33-37)
+inline Rgb emitted(const Emitter& e, const SurfacePoint& p, V3 dir) {
+    if (dot(p.shading_normal, dir) < 0.0f) return rgb(0.0f);
+    return e.radiance;
+}
+
+struct PathCtx {
+    const or_scene* s;
+    const sspt_pt_settings* cfg;
+    bool brute;
+    Counters* c;
+};
+
+// PathTracerBase.PerformBackgroundNextEvent (PathTracer.cs:423-452)
+Rgb background_nee(const PathCtx& pc, const Shader& sh, Rng& rng) {
+    const or_scene& s = *pc.s;
+    if (!s.has_env) return rgb(0.0f);
+    EnvSample smp = env_sample(s.env, rng.next_float2());
+    Ray r = spawn_ray(sh.p, smp.dir);
+    pc.c->shadow++;
+    if (trace_any(s, r, pc.brute)) {
+        pc.c->occluded++;
+        return rgb(0.0f);
+    }
+    Rgb bsdf_cos = eval_cos(sh, smp.dir);
+    float pdf_bsdf = pdf_fwd(sh, smp.dir);
+    if (pdf_bsdf == 0.0f || smp.pdf == 0.0f) return rgb(0.0f);
+    float nsr = (float)pc.cfg->num_shadow_rays;
+    float mis = pc.cfg->enable_bsdf_di ? 1.0f / (1.0f + pdf_bsdf / (smp.pdf * nsr)) : 1.0f;
+    Rgb contrib = smp.weight * bsdf_cos / nsr;
+    return mis * contrib;
+}
+
+// PathTracerBase.PerformNextEventEstimation (PathTracer.cs:454-501)
+Rgb light_nee(const PathCtx& pc, const Shader& sh, Rng& rng) {
+    const or_scene& s = *pc.s;
+    if (s.emitters.empty()) return rgb(0.0f);
+    int count = (int)s.emitters.size();
+    int idx = (int)rng.next_int((uint32_t)count);
+    const Emitter& e = s.emitters[idx];
+    float sel = 1.0f / (float)count;
+    // Triangle.SampleUniformArea (Triangle.cs:42-56)
+    V2 b = to_uniform_triangle(rng.next_float2());
+    SurfacePoint lp = make_point(s, e.tri, b, 0.0f);
+    float lpdf = e.inv_area;
+    V3 l2s = normalize(sh.p.position - lp.position);
+    Ray r = shadow_ray(sh.p, lp);
+    pc.c->shadow++;
+    if (trace_any(s, r, pc.brute)) {
+        pc.c->occluded++;
+        return rgb(0.0f);
+    }
+    Rgb em = emitted(e, lp, l2s);
+    float jac = area_to_solid_angle(sh.p, lp);
+    Rgb bsdf_cos = eval_cos(sh, -l2s);
+    float nsr = (float)pc.cfg->num_shadow_rays;
+    float pdf_ne = lpdf * sel * nsr;
+    float pdf_bsdf_sa = pdf_fwd(sh, -l2s);
+    float pdf_bsdf = pdf_bsdf_sa * jac;
+    if (jac == 0.0f) return rgb(0.0f);
+    float ratio = pdf_bsdf / pdf_ne;
+    float mis = pc.cfg->enable_bsdf_di ? 1.0f / (ratio + 1.0f) : 1.0f;
+    float pdf = lpdf / jac * sel * nsr;
+    Rgb contrib = em / pdf * bsdf_cos;
+    return mis * contrib;
+}
+
+// PathTracerBase.RenderPixel + EstimateIncidentRadiance (PathTracer.cs:287-382)
+Rgb render_pixel(const PathCtx& pc, uint32_t row, uint32_t col, uint32_t iteration) {
+    const or_scene& s = *pc.s;
+    const sspt_pt_settings& cfg = *pc.cfg;
+    Rng rng{hash_seed(cfg.base_seed, row * (uint32_t)s.width + col, iteration)};
+    V2 off = rng.next_float2();
+    Ray ray;
+    s.camera.generate_ray((float)col + off.x, (float)row + off.y, &ray.o, &ray.d);
+    ray.tmin = 0.0f;
+    ray.tmax = INFINITY;
+
+    Rgb radiance = rgb(0.0f);
+    Rgb prefix = rgb(1.0f), approx = rgb(1.0f);
+    int depth = 1;
+    SurfacePoint prev;
+    float prev_pdf = 0.0f;
+    int count = (int)s.emitters.size();
+    float nsr = (float)cfg.num_shadow_rays;
+
+    while (depth <= cfg.max_depth) {
+        HitRec h = trace_closest(s, ray, pc.brute);
+        pc.c->rays++;
+        if (h.tri < 0) {
+            if (depth >= cfg.min_depth) {
+                // OnBackgroundHit (PathTracer.cs:384-400)
+                float mis = 0.0f;
+                Rgb contrib = rgb(0.0f);
+                if (s.has_env && cfg.enable_bsdf_di) {
+                    mis = 1.0f;
+                    if (depth > 1) {
+                        float pdf_ne = env_pdf(s.env, ray.d) * nsr;
+                        mis = 1.0f / (1.0f + pdf_ne / prev_pdf);
+                    }
+                    contrib = env_radiance(s.env, ray.d);
+                }
+                radiance = radiance + prefix * mis * contrib;
+            }
+            break;
+        }
+        pc.c->hits++;
+        SurfacePoint hit = make_point(s, h.tri, {h.u, h.v}, h.t);
+        Shader sh = make_shader(s, hit, -ray.d);
+
+        int ei = s.tri_emitter[h.tri];
+        if (ei >= 0 && depth >= cfg.min_depth) {
+            // OnLightHit (PathTracer.cs:402-421)
+            const Emitter& e = s.emitters[ei];
+            float mis = 1.0f;
+            if (depth > 1) {
+                float jac = area_to_solid_angle(prev, hit);
+                float pdf_ne = e.inv_area / (float)count * nsr / jac;
+                float ratio = pdf_ne / prev_pdf;
+                mis = 1.0f / (ratio + 1.0f);
+                if (!cfg.enable_bsdf_di) mis = 0.0f;
+            }
+            Rgb em = emitted(e, hit, -ray.d);
+            radiance = radiance + prefix * mis * em;
+        }
+
+        // ComputeSurvivalProbability (PathTracer.cs:277-282)
+        float q = depth > 4 ? std::min(std::max(average(approx), 0.05f), 0.95f) : 1.0f;
+        if (rng.next_float() > q || depth == cfg.max_depth) break;
+
+        if (depth + 1 >= cfg.min_depth) {
+            Rgb nee = rgb(0.0f);
+            for (int i = 0; i < cfg.num_shadow_rays; ++i) {
+                nee = nee + background_nee(pc, sh, rng);
+                nee = nee + light_nee(pc, sh, rng);
+            }
+            radiance = radiance + prefix * nee / q;
+        }
+
+        // SampleDirection (PathTracer.cs:526-530)
+        float u_comp = rng.next_float();
+        V2 u_dir = rng.next_float2();
+        BsdfSample bs = sample_bsdf(sh, u_comp, u_dir);
+        if (bs.pdf == 0.0f || is_black(bs.weight)) break;
+        ray = spawn_ray(hit, bs.dir);
+
+        prefix = prefix * (bs.weight / q);
+        approx = approx * (bs.weight / q);
+        depth++;
+        prev = hit;
+        prev_pdf = bs.pdf * q;
+    }
+    return radiance;
+}
+
+template <class F>
+void parallel_for(int64_t n, int threads, F&& f) {
+    if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    std::atomic<int64_t> next{0};
+    auto work = [&]() {
+        for (;;) {
+            int64_t i = next.fetch_add(1);
+            if (i >= n) break;
+            f(i);
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < threads; ++t) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
+}
+
+void set_err(char* err, const char* msg) {
+    if (err) std::snprintf(err, 256, "%s", msg);
+}
+
+}  // namespace
+
+// ======================================================================= C API
+
+extern "C" {
+
+uint32_t or_hash_seed(uint32_t b, uint32_t c, uint32_t s) { return hash_seed(b, c, s); }
+uint32_t or_pcg_hash(uint32_t x) { return pcg_hash(x); }
+uint32_t or_fnv_hash(uint32_t h, uint32_t d) { return fnv_hash(h, d); }
+
+void or_rng_stream(uint32_t seed, uint32_t n, uint32_t* out_u32, float* out_float) {
+    Rng a{seed}, b{seed};
+    for (uint32_t i = 0; i < n; ++i) {
+        if (out_u32) out_u32[i] = a.next();
+        if (out_float) out_float[i] = b.next_float();
+    }
+}
+
+void or_rng_next_int(uint32_t seed, uint32_t max, uint32_t n, uint32_t* out) {
+    Rng r{seed};
+    for (uint32_t i = 0; i < n; ++i) out[i] = r.next_int(max);
+}
+
+void or_warp_uniform_triangle(const float* p, float* out) {
+    V2 r = to_uniform_triangle({p[0], p[1]});
+    out[0] = r.x;
+    out[1] = r.y;
+}
+
+void or_warp_cos_hemisphere(const float* p, float* out, float* pdf) {
+    DirSample d = to_cos_hemisphere({p[0], p[1]});
+    out[0] = d.dir.x;
+    out[1] = d.dir.y;
+    out[2] = d.dir.z;
+    *pdf = d.pdf;
+}
+
+void or_pwc_sample(const float* w, int32_t n, float u, int32_t* idx, float* rel) {
+    Pwc p;
+    p.build(w, n);
+    int i;
+    p.sample(u, &i, rel);
+    *idx = i;
+}
+
+float or_pwc_probability(const float* w, int32_t n, int32_t idx) {
+    Pwc p;
+    p.build(w, n);
+    return p.probability(idx);
+}
+
+void or_camera_ray(const float* w2c, float fov, int32_t w, int32_t h, float fx, float fy, float* out) {
+    Camera c;
+    c.init(w2c, fov, w, h);
+    V3 o, d;
+    c.generate_ray(fx, fy, &o, &d);
+    out[0] = o.x, out[1] = o.y, out[2] = o.z;
+    out[3] = d.x, out[4] = d.y, out[5] = d.z;
+}
+
+// Matrix4x4.CreateLookAt (right-handed), fp32.
+void or_look_at(const float* pos, const float* target, const float* up, float* out) {
+    V3 p = v3(pos[0], pos[1], pos[2]), t = v3(target[0], target[1], target[2]);
+    V3 u = v3(up[0], up[1], up[2]);
+    V3 z = normalize(p - t);
+    V3 x = normalize(cross(u, z));
+    V3 y = cross(z, x);
+    float m[16] = {x.x, y.x, z.x, 0, x.y, y.y, z.y, 0, x.z, y.z, z.z, 0,
+                   -dot(x, p), -dot(y, p), -dot(z, p), 1};
+    std::memcpy(out, m, sizeof(m));
+}
+
+float or_ggx_ndf(float ax, float ay, const float* n) { return Ggx{ax, ay}.ndf(v3(n[0], n[1], n[2])); }
+float or_ggx_pdf(float ax, float ay, const float* o, const float* n) {
+    return Ggx{ax, ay}.pdf(v3(o[0], o[1], o[2]), v3(n[0], n[1], n[2]));
+}
+void or_ggx_sample(float ax, float ay, const float* o, const float* p, float* out) {
+    V3 r = Ggx{ax, ay}.sample(v3(o[0], o[1], o[2]), {p[0], p[1]});
+    out[0] = r.x, out[1] = r.y, out[2] = r.z;
+}
+float or_fresnel_dielectric(float c, float a, float b) { return fresnel_dielectric(c, a, b); }
+
+static or_scene* flat_surface(const sspt_material_desc* mat, const sspt_texture_desc* textures,
+                              const float* normal, const float* uv, SurfacePoint* p) {
+    or_scene* s = new or_scene();
+    Material m{mat->kind,
+               v3(mat->base_color[0], mat->base_color[1], mat->base_color[2]),
+               mat->base_color_texture,
+               mat->roughness,
+               mat->roughness_texture,
+               mat->metallic,
+               mat->specular_tint_strength,
+               mat->anisotropic,
+               mat->specular_transmittance,
+               mat->index_of_refraction,
+               mat->transmitter != 0};
+    s->materials.push_back(m);
+    int max_tex = std::max(mat->base_color_texture, mat->roughness_texture);
+    for (int i = 0; i <= max_tex; ++i) {
+        Texture t;
+        t.w = textures[i].width, t.h = textures[i].height, t.ch = textures[i].channels;
+        t.border = textures[i].border;
+        t.texels.assign(textures[i].texels, textures[i].texels + (size_t)t.w * t.h * t.ch);
+        s->textures.push_back(t);
+    }
+    Tri t{};
+    t.material = 0;
+    t.face_normal = v3(normal[0], normal[1], normal[2]);
+    s->tris.push_back(t);
+    p->tri = 0;
+    p->position = v3(0, 0, 0);
+    p->normal = t.face_normal;
+    p->shading_normal = t.face_normal;
+    p->uv = {uv[0], uv[1]};
+    return s;
+}
+
+void or_material_eval(const sspt_material_desc* mat, const sspt_texture_desc* textures,
+                      const float* normal, const float* uv, const float* out_dir, const float* in_dir,
+                      float* value, float* value_cos, float* pdf) {
+    SurfacePoint p;
+    or_scene* s = flat_surface(mat, textures, normal, uv, &p);
+    Shader sh = make_shader(*s, p, v3(out_dir[0], out_dir[1], out_dir[2]));
+    V3 in = v3(in_dir[0], in_dir[1], in_dir[2]);
+    Rgb v = eval_nocos(sh, in), vc = eval_cos(sh, in);
+    value[0] = v.x, value[1] = v.y, value[2] = v.z;
+    value_cos[0] = vc.x, value_cos[1] = vc.y, value_cos[2] = vc.z;
+    *pdf = pdf_fwd(sh, in);
+    delete s;
+}
+
+void or_material_sample(const sspt_material_desc* mat, const sspt_texture_desc* textures,
+                        const float* normal, const float* uv, const float* out_dir, float pc,
+                        const float* p2, float* out) {
+    SurfacePoint p;
+    or_scene* s = flat_surface(mat, textures, normal, uv, &p);
+    Shader sh = make_shader(*s, p, v3(out_dir[0], out_dir[1], out_dir[2]));
+    BsdfSample bs = sample_bsdf(sh, pc, {p2[0], p2[1]});
+    out[0] = bs.dir.x, out[1] = bs.dir.y, out[2] = bs.dir.z;
+    out[3] = bs.pdf;
+    out[4] = bs.weight.x, out[5] = bs.weight.y, out[6] = bs.weight.z;
+    delete s;
+}
+
+// Scene.Prepare (SeeSharp/Scene.cs:103-164) + IsValid (:170-197)
+or_scene* or_scene_create(const sspt_scene_desc* d, char* err) {
+    if (!d) {
+        set_err(err, "null scene description");
+        return nullptr;
+    }
+    if (d->width == 0 || d->height == 0) {
+        set_err(err, "Framebuffer not set.");
+        return nullptr;
+    }
+    if (d->num_meshes == 0) {
+        set_err(err, "No meshes in the scene.");
+        return nullptr;
+    }
+    if (d->num_emitters == 0 && !d->background) {
+        set_err(err, "No emitters and no background in the scene.");
+        return nullptr;
+    }
+    or_scene* s = new or_scene();
+    s->width = (int)d->width;
+    s->height = (int)d->height;
+    for (uint32_t i = 0; i < d->num_textures; ++i) {
+        const sspt_texture_desc& td = d->textures[i];
+        Texture t;
+        t.w = (int)td.width, t.h = (int)td.height, t.ch = (int)td.channels, t.border = td.border;
+        t.texels.assign(td.texels, td.texels + (size_t)t.w * t.h * t.ch);
+        s->textures.push_back(std::move(t));
+    }
+    for (uint32_t i = 0; i < d->num_materials; ++i) {
+        const sspt_material_desc& m = d->materials[i];
+        s->materials.push_back(Material{m.kind, v3(m.base_color[0], m.base_color[1], m.base_color[2]),
+                                        m.base_color_texture, m.roughness, m.roughness_texture,
+                                        m.metallic, m.specular_tint_strength, m.anisotropic,
+                                        m.specular_transmittance, m.index_of_refraction,
+                                        m.transmitter != 0});
+    }
+    std::vector<int> mesh_offset(d->num_meshes);
+    for (uint32_t mi = 0; mi < d->num_meshes; ++mi) {
+        const sspt_mesh_desc& m = d->meshes[mi];
+        if (m.material < 0 || (uint32_t)m.material >= d->num_materials) {
+            char buf[128];
+            std::snprintf(buf, sizeof(buf), "Mesh[%u] does not have a material.", mi);
+            set_err(err, buf);
+            delete s;
+            return nullptr;
+        }
+        mesh_offset[mi] = (int)s->tris.size();
+        auto P = [&](int v) { return v3(m.positions[v * 3], m.positions[v * 3 + 1], m.positions[v * 3 + 2]); };
+        for (uint32_t f = 0; f < m.num_triangles; ++f) {
+            Tri t{};
+            int i0 = m.indices[f * 3], i1 = m.indices[f * 3 + 1], i2 = m.indices[f * 3 + 2];
+            t.p0 = P(i0), t.p1 = P(i1), t.p2 = P(i2);
+            t.face_normal = normalize(cross(t.p1 - t.p0, t.p2 - t.p0));
+            t.has_normals = m.normals != nullptr;
+            if (t.has_normals) {
+                auto N = [&](int v) { return v3(m.normals[v * 3], m.normals[v * 3 + 1], m.normals[v * 3 + 2]); };
+                t.n0 = N(i0), t.n1 = N(i1), t.n2 = N(i2);
+            }
+            t.has_uvs = m.uvs != nullptr;
+            if (t.has_uvs) {
+                t.t0 = {m.uvs[i0 * 2], m.uvs[i0 * 2 + 1]};
+                t.t1 = {m.uvs[i1 * 2], m.uvs[i1 * 2 + 1]};
+                t.t2 = {m.uvs[i2 * 2], m.uvs[i2 * 2 + 1]};
+            }
+            t.mesh = (int)mi;
+            t.prim = (int)f;
+            t.material = m.material;
+            s->tris.push_back(t);
+        }
+    }
+    s->tri_emitter.assign(s->tris.size(), -1);
+    for (uint32_t i = 0; i < d->num_emitters; ++i) {
+        const sspt_emitter_desc& e = d->emitters[i];
+        int tri = mesh_offset[e.mesh] + (int)e.face;
+        const Tri& t = s->tris[tri];
+        V3 n = cross(t.p1 - t.p0, t.p2 - t.p0);
+        float area = length(n) * 0.5f;
+        if (area == 0.0f) {
+            set_err(err, "Zero-area triangles cannot be emitters");
+            delete s;
+            return nullptr;
+        }
+        if (s->tri_emitter[tri] >= 0) {
+            set_err(err, "duplicate emitter for one triangle");
+            delete s;
+            return nullptr;
+        }
+        s->tri_emitter[tri] = (int)i;
+        s->emitters.push_back({tri, v3(e.radiance[0], e.radiance[1], e.radiance[2]), 1.0f / area});
+    }
+    if (d->background) {
+        s->has_env = true;
+        EnvMap& e = s->env;
+        e.w = (int)d->background->width;
+        e.h = (int)d->background->height;
+        e.rgb.assign(d->background->rgb, d->background->rgb + (size_t)e.w * e.h * 3);
+        // EnvironmentMap.BuildSamplingGrid (EnvironmentMap.cs:139-156)
+        e.grid.init(e.w, e.h);
+        for (int r = 0; r < e.h; ++r)
+            for (int c = 0; c < e.w; ++c) e.grid.splat(c, r, luminance(e.pixel(c, r)));
+        e.grid.normalize();
+    }
+    if (!s->camera.init(d->camera.world_to_camera, d->camera.vertical_fov_degrees, s->width, s->height)) {
+        set_err(err, "World to camera transform must be invertible.");
+        delete s;
+        return nullptr;
+    }
+    build_bvh(*s);
+    return s;
+}
+
+void or_scene_destroy(or_scene* s) { delete s; }
+
+void or_env_eval(const or_scene* s, const float* d, float* out) {
+    Rgb c = env_radiance(s->env, v3(d[0], d[1], d[2]));
+    out[0] = c.x, out[1] = c.y, out[2] = c.z;
+}
+float or_env_pdf(const or_scene* s, const float* d) { return env_pdf(s->env, v3(d[0], d[1], d[2])); }
+void or_env_sample(const or_scene* s, const float* p, float* out) {
+    EnvSample e = env_sample(s->env, {p[0], p[1]});
+    out[0] = e.dir.x, out[1] = e.dir.y, out[2] = e.dir.z, out[3] = e.pdf;
+    out[4] = e.weight.x, out[5] = e.weight.y, out[6] = e.weight.z;
+}
+
+static Ray read_ray(const float* r) {
+    Ray ray;
+    ray.o = v3(r[0], r[1], r[2]);
+    ray.tmin = r[3];
+    ray.d = v3(r[4], r[5], r[6]);
+    ray.tmax = r[7];
+    return ray;
+}
+
+void or_trace_closest(const or_scene* s, const float* rays, uint64_t n, sspt_hit* hits, int32_t brute) {
+    parallel_for((int64_t)((n + 255) / 256), 0, [&](int64_t blk) {
+        uint64_t end = std::min<uint64_t>(n, (uint64_t)(blk + 1) * 256);
+        for (uint64_t i = (uint64_t)blk * 256; i < end; ++i) {
+            HitRec h = trace_closest(*s, read_ray(rays + i * 8), brute != 0);
+            hits[i] = sspt_hit{h.t, h.u, h.v, h.tri};
+        }
+    });
+}
+
+void or_trace_occluded(const or_scene* s, const float* rays, uint64_t n, uint8_t* occ, int32_t brute) {
+    parallel_for((int64_t)((n + 255) / 256), 0, [&](int64_t blk) {
+        uint64_t end = std::min<uint64_t>(n, (uint64_t)(blk + 1) * 256);
+        for (uint64_t i = (uint64_t)blk * 256; i < end; ++i)
+            occ[i] = trace_any(*s, read_ray(rays + i * 8), brute != 0) ? 1 : 0;
+    });
+}
+
+void or_primary_hits(const or_scene* s, uint32_t base_seed, uint32_t iteration, float* rays_out,
+                     sspt_hit* hits_out, int32_t brute) {
+    int W = s->width, H = s->height;
+    parallel_for(H, 0, [&](int64_t row) {
+        for (int col = 0; col < W; ++col) {
+            size_t pix = (size_t)row * W + col;
+            Rng rng{hash_seed(base_seed, (uint32_t)pix, iteration)};
+            V2 off = rng.next_float2();
+            Ray r;
+            s->camera.generate_ray((float)col + off.x, (float)row + off.y, &r.o, &r.d);
+            r.tmin = 0.0f;
+            r.tmax = INFINITY;
+            if (rays_out) {
+                float* o = rays_out + pix * 8;
+                o[0] = r.o.x, o[1] = r.o.y, o[2] = r.o.z, o[3] = r.tmin;
+                o[4] = r.d.x, o[5] = r.d.y, o[6] = r.d.z, o[7] = r.tmax;
+            }
+            if (hits_out) {
+                HitRec h = trace_closest(*s, r, brute != 0);
+                hits_out[pix] = sspt_hit{h.t, h.u, h.v, h.tri};
+            }
+        }
+    });
+}
+
+uint64_t or_render(const or_scene* s, const sspt_pt_settings* cfg, uint32_t first, uint32_t num,
+                   float* image, sspt_ray_stats* stats, const int32_t* pixels, int64_t num_pixels,
+                   int32_t threads) {
+    int W = s->width, H = s->height;
+    int64_t total = pixels ? num_pixels : (int64_t)W * H;
+    std::atomic<uint64_t> rays{0}, shadow{0}, occl{0}, hits{0}, nans{0};
+    for (uint32_t it = first; it < first + num; ++it) {
+        // FrameBuffer.StartIteration / Normalize (FrameBuffer.cs:146-160)
+        float n = (float)(it + 1);
+        if (it + 1 > 1) {
+            float scale = (n - 1.0f) / n;
+            for (int64_t k = 0; k < total; ++k) {
+                int64_t pix = pixels ? pixels[k] : k;
+                for (int c = 0; c < 3; ++c) image[pix * 3 + c] *= scale;
+            }
+        }
+        int64_t chunks = (total + 63) / 64;
+        parallel_for(chunks, threads, [&](int64_t ch) {
+            Counters c;
+            PathCtx pc{s, cfg, false, &c};
+            int64_t end = std::min<int64_t>(total, (ch + 1) * 64);
+            for (int64_t k = ch * 64; k < end; ++k) {
+                int64_t pix = pixels ? pixels[k] : k;
+                uint32_t row = (uint32_t)(pix / W), col = (uint32_t)(pix % W);
+                Rgb L = render_pixel(pc, row, col, it);
+                // FrameBuffer.Splat (FrameBuffer.cs:89-114), IgnoreNanAndInf.
+                if (!std::isfinite(average(L))) {
+                    nans++;
+                    continue;
+                }
+                Rgb add = L / n;
+                image[pix * 3 + 0] += add.x;
+                image[pix * 3 + 1] += add.y;
+                image[pix * 3 + 2] += add.z;
+            }
+            rays += c.rays, shadow += c.shadow, occl += c.occluded, hits += c.hits;
+        });
+    }
+    if (stats) {
+        stats->num_rays = rays;
+        stats->num_shadow_rays = shadow;
+        stats->num_occluded = occl;
+        stats->num_ray_hits = hits;
+    }
+    return nans;
+}
+
+void or_render_pixel(const or_scene* s, const sspt_pt_settings* cfg, uint32_t row, uint32_t col,
+                     uint32_t iteration, float* out) {
+    Counters c;
+    PathCtx pc{s, cfg, false, &c};
+    Rgb L = render_pixel(pc, row, col, iteration);
+    out[0] = L.x, out[1] = L.y, out[2] = L.z;
+}
+
+}  // extern "C"

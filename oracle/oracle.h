@@ -1,0 +1,95 @@
+/*
+ * oracle.h -- CPU restatement of SeeSharp's unidirectional PathTracer.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker the GPU path is
+ * compared against and the CPU baseline that bench.py times beside it. The
+ * product (libsspt.so and the seesharp_amd package) never links, loads or
+ * calls it. Each function cites the reference file:line it restates.
+ */
+#ifndef SSPT_ORACLE_H
+#define SSPT_ORACLE_H
+
+#include <stdint.h>
+
+#include "../include/sspt.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- RNG (SeeSharp/Sampling/RNG.cs) ---- */
+uint32_t or_hash_seed(uint32_t base_seed, uint32_t chain_index, uint32_t sample_index);
+uint32_t or_pcg_hash(uint32_t x);
+uint32_t or_fnv_hash(uint32_t hash, uint32_t data);
+/* n successive Next() values and NextFloat() values of one generator. */
+void or_rng_stream(uint32_t seed, uint32_t n, uint32_t* out_u32, float* out_float);
+/* NextInt(max) n times from one generator. */
+void or_rng_next_int(uint32_t seed, uint32_t max, uint32_t n, uint32_t* out);
+
+/* ---- SampleWarp (SeeSharp/Sampling/SampleWarp.cs) ---- */
+void or_warp_uniform_triangle(const float* primary2, float* out_bary2);
+void or_warp_cos_hemisphere(const float* primary2, float* out_dir3, float* out_pdf);
+
+/* ---- PiecewiseConstantPDF (SeeSharp/Sampling/PiecewiseConstant.cs) ---- */
+void or_pwc_sample(const float* weights, int32_t n, float u, int32_t* out_idx, float* out_rel);
+float or_pwc_probability(const float* weights, int32_t n, int32_t idx);
+
+/* ---- camera (SeeSharp/Cameras/PerspectiveCamera.cs) ---- */
+/* ray_out: ox,oy,oz,dx,dy,dz */
+void or_camera_ray(const float* world_to_camera16, float fov_deg, int32_t width, int32_t height,
+                   float film_x, float film_y, float* ray_out);
+void or_look_at(const float* pos3, const float* target3, const float* up3, float* out16);
+
+/* ---- microfacet + fresnel (SeeSharp/Shading/TrowbridgeReitzDistribution.cs, Fresnel.cs) ---- */
+float or_ggx_ndf(float ax, float ay, const float* n3);
+float or_ggx_pdf(float ax, float ay, const float* out3, const float* n3);
+void or_ggx_sample(float ax, float ay, const float* out3, const float* primary2, float* out_n3);
+float or_fresnel_dielectric(float cos_i, float eta_i, float eta_t);
+
+/* ---- materials on a flat test surface (SeeSharp/Shading/Materials/) ----
+ * The surface point has face normal = shading normal = normal3, texture
+ * coordinates uv2, and the material's textures taken from `textures`. */
+void or_material_eval(const sspt_material_desc* mat, const sspt_texture_desc* textures,
+                      const float* normal3, const float* uv2, const float* out_dir3,
+                      const float* in_dir3, float* value3, float* value_cos3, float* pdf);
+/* sample_out: dir3, pdf, weight3 -> 7 floats */
+void or_material_sample(const sspt_material_desc* mat, const sspt_texture_desc* textures,
+                        const float* normal3, const float* uv2, const float* out_dir3,
+                        float primary_component, const float* primary2, float* sample_out);
+
+/* ---- scene (SeeSharp/Scene.cs Prepare) ---- */
+typedef struct or_scene or_scene;
+/* Returns NULL and writes a message to err (if non-NULL, 256 bytes) on failure. */
+or_scene* or_scene_create(const sspt_scene_desc* desc, char* err);
+void or_scene_destroy(or_scene* scene);
+
+/* EnvironmentMap of the scene (SeeSharp/Shading/Background/EnvironmentMap.cs) */
+void or_env_eval(const or_scene* s, const float* dir3, float* rgb3);
+float or_env_pdf(const or_scene* s, const float* dir3);
+/* out: dir3, pdf, weight3 */
+void or_env_sample(const or_scene* s, const float* primary2, float* out7);
+
+/* ---- ray queries: rays are 8 floats (ox,oy,oz,tmin,dx,dy,dz,tmax) ---- */
+void or_trace_closest(const or_scene* s, const float* rays, uint64_t n, sspt_hit* hits,
+                      int32_t brute_force);
+void or_trace_occluded(const or_scene* s, const float* rays, uint64_t n, uint8_t* occluded,
+                       int32_t brute_force);
+void or_primary_hits(const or_scene* s, uint32_t base_seed, uint32_t iteration,
+                     float* rays_out, sspt_hit* hits_out, int32_t brute_force);
+
+/* ---- PathTracerBase.Render (SeeSharp/Integrators/PathTracer.cs:199-382) ----
+ * Renders iterations first..first+num-1 into image (W*H*3 running mean) for
+ * all pixels, or only the `num_pixels` pixel indices listed in `pixels`.
+ * num_threads <= 0 uses all hardware threads. Returns the NaN/Inf splat count. */
+uint64_t or_render(const or_scene* s, const sspt_pt_settings* settings, uint32_t first_iteration,
+                   uint32_t num_iterations, float* image, sspt_ray_stats* stats,
+                   const int32_t* pixels, int64_t num_pixels, int32_t num_threads);
+/* The path estimate of one (pixel, iteration), PathTracer.RenderPixel without the splat. */
+void or_render_pixel(const or_scene* s, const sspt_pt_settings* settings, uint32_t row,
+                     uint32_t col, uint32_t iteration, float* rgb3);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,143 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so) for tests and bench.py's CPU baseline.
+
+The oracle is the checker: tests compare the GPU library against it; it is never
+the thing measured as the product or shipped.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from seesharp_amd import abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
+
+_fp = C.POINTER(C.c_float)
+_u32p = C.POINTER(C.c_uint32)
+_vp = C.c_void_p
+
+_SIGS = {
+    "or_hash_seed": (C.c_uint32, [C.c_uint32] * 3),
+    "or_pcg_hash": (C.c_uint32, [C.c_uint32]),
+    "or_fnv_hash": (C.c_uint32, [C.c_uint32, C.c_uint32]),
+    "or_rng_stream": (None, [C.c_uint32, C.c_uint32, _u32p, _fp]),
+    "or_rng_next_int": (None, [C.c_uint32, C.c_uint32, C.c_uint32, _u32p]),
+    "or_warp_uniform_triangle": (None, [_fp, _fp]),
+    "or_warp_cos_hemisphere": (None, [_fp, _fp, _fp]),
+    "or_pwc_sample": (None, [_fp, C.c_int32, C.c_float, C.POINTER(C.c_int32), _fp]),
+    "or_pwc_probability": (C.c_float, [_fp, C.c_int32, C.c_int32]),
+    "or_camera_ray": (None, [_fp, C.c_float, C.c_int32, C.c_int32, C.c_float, C.c_float, _fp]),
+    "or_look_at": (None, [_fp, _fp, _fp, _fp]),
+    "or_ggx_ndf": (C.c_float, [C.c_float, C.c_float, _fp]),
+    "or_ggx_pdf": (C.c_float, [C.c_float, C.c_float, _fp, _fp]),
+    "or_ggx_sample": (None, [C.c_float, C.c_float, _fp, _fp, _fp]),
+    "or_fresnel_dielectric": (C.c_float, [C.c_float] * 3),
+    "or_material_eval": (None, [C.POINTER(abi.MaterialDesc), C.POINTER(abi.TextureDesc), _fp, _fp, _fp,
+                                _fp, _fp, _fp, _fp]),
+    "or_material_sample": (None, [C.POINTER(abi.MaterialDesc), C.POINTER(abi.TextureDesc), _fp, _fp, _fp,
+                                  C.c_float, _fp, _fp]),
+    "or_scene_create": (_vp, [C.POINTER(abi.SceneDesc), C.c_char_p]),
+    "or_scene_destroy": (None, [_vp]),
+    "or_env_eval": (None, [_vp, _fp, _fp]),
+    "or_env_pdf": (C.c_float, [_vp, _fp]),
+    "or_env_sample": (None, [_vp, _fp, _fp]),
+    "or_trace_closest": (None, [_vp, _fp, C.c_uint64, C.POINTER(abi.Hit), C.c_int32]),
+    "or_trace_occluded": (None, [_vp, _fp, C.c_uint64, C.POINTER(C.c_uint8), C.c_int32]),
+    "or_primary_hits": (None, [_vp, C.c_uint32, C.c_uint32, _fp, C.POINTER(abi.Hit), C.c_int32]),
+    "or_render": (C.c_uint64, [_vp, C.POINTER(abi.PtSettings), C.c_uint32, C.c_uint32, _fp,
+                               C.POINTER(abi.RayStats), C.POINTER(C.c_int32), C.c_int64, C.c_int32]),
+    "or_render_pixel": (None, [_vp, C.POINTER(abi.PtSettings), C.c_uint32, C.c_uint32, C.c_uint32, _fp]),
+}
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            build()
+        L = C.CDLL(ORACLE_SO)
+        for k, (r, a) in _SIGS.items():
+            f = getattr(L, k)
+            f.restype = r
+            f.argtypes = a
+        _lib = L
+    return _lib
+
+
+def fp(a):
+    return a.ctypes.data_as(_fp)
+
+
+def settings(base_seed=0xC030114, total_spp=1, max_depth=5, min_depth=1, num_shadow_rays=1,
+             enable_bsdf_di=True, max_render_time_ms=-1):
+    return abi.PtSettings(base_seed, total_spp, max_depth, min_depth, num_shadow_rays,
+                          1 if enable_bsdf_di else 0, max_render_time_ms)
+
+
+class OracleScene:
+    def __init__(self, scene):
+        self.scene = scene
+        self.desc = scene.to_desc()
+        err = C.create_string_buffer(256)
+        self.ptr = lib().or_scene_create(C.byref(self.desc), err)
+        if not self.ptr:
+            raise ValueError(err.value.decode())
+        self.width = scene.FrameBuffer.Width
+        self.height = scene.FrameBuffer.Height
+
+    def close(self):
+        if self.ptr:
+            lib().or_scene_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        self.close()
+
+    def render(self, cfg, first=0, num=1, image=None, pixels=None, threads=0):
+        if image is None:
+            image = np.zeros((self.height, self.width, 3), np.float32)
+        stats = abi.RayStats()
+        px = None
+        n = 0
+        if pixels is not None:
+            pixels = np.ascontiguousarray(pixels, np.int32)
+            px = pixels.ctypes.data_as(C.POINTER(C.c_int32))
+            n = pixels.size
+        nans = lib().or_render(self.ptr, C.byref(cfg), first, num, fp(image), C.byref(stats), px, n, threads)
+        return image, stats, int(nans)
+
+    def trace_closest(self, rays, brute=False):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        hits = (abi.Hit * rays.shape[0])()
+        lib().or_trace_closest(self.ptr, fp(rays), rays.shape[0], hits, 1 if brute else 0)
+        return hits_to_numpy(hits)
+
+    def trace_occluded(self, rays, brute=False):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        out = np.zeros(rays.shape[0], np.uint8)
+        lib().or_trace_occluded(self.ptr, fp(rays), rays.shape[0],
+                                out.ctypes.data_as(C.POINTER(C.c_uint8)), 1 if brute else 0)
+        return out
+
+    def primary_hits(self, base_seed, iteration, brute=False):
+        n = self.width * self.height
+        rays = np.zeros((n, 8), np.float32)
+        hits = (abi.Hit * n)()
+        lib().or_primary_hits(self.ptr, base_seed, iteration, fp(rays), hits, 1 if brute else 0)
+        return rays, hits_to_numpy(hits)
+
+
+def hits_to_numpy(hits):
+    arr = np.frombuffer(hits, dtype=np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("prim", "<i4")]))
+    return arr.copy()
