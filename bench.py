@@ -1,0 +1,189 @@
+"""Benchmark: Mpaths/s of the GPU path tracer on the 1M-triangle 1024^2 depth-8 scene.
+
+One step = one full render of the large scene (1 036 296 triangles, 1024 x 1024,
+64 spp, MaxDepth 8 = 67 108 864 paths) into the device framebuffer, with the
+scene already resident in HBM (BVH build and upload happen before timing).
+
+Prints one JSON line with the metric, the roofline of the dominant kernel
+(from HIP events on the library's own stream) and the CPU baseline (the
+oracle restatement of the reference, timed on a bounded pixel sample).
+
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("Mpaths/s at 1M-tri 1024² depth-8 PT (relMSE≤1e-3); adjoint/forward cost ratio")
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes per unit (per ray / hit / path), split by the kernel that
+# moves them: extension ray 32 B ray record + 16 B hit record (extend) and
+# 64 B path state (shade); shadow ray 32 B + 4 B visibility (shadow); hit
+# 3x12 B positions + 3x12 B normals + 3x8 B uvs + 4 B material (shade);
+# path 12 B pixel read + 12 B pixel write (accumulate).
+BYTES_EXTEND_PER_RAY = 48
+BYTES_SHADOW_PER_RAY = 36
+BYTES_SHADE_PER_RAY = 64
+BYTES_SHADE_PER_HIT = 100
+BYTES_PER_PATH = 24
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(scene, spp, max_depth, budget_s):
+    """Times the oracle (CPU restatement) on a seeded pixel subset of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+
+    threads = min(16, len(os.sched_getaffinity(0)))
+    t0 = time.perf_counter()
+    o = ol.OracleScene(scene)
+    log(f"[bench] oracle scene ready in {time.perf_counter() - t0:.1f} s")
+    npx = scene.FrameBuffer.Width * scene.FrameBuffer.Height
+    rng = np.random.default_rng(12345)
+    cfg = ol.settings(total_spp=spp, max_depth=max_depth)
+    # calibrate on a small sample, then size the timed sample to the budget
+    pix = np.sort(rng.choice(npx, 1024, replace=False)).astype(np.int32)
+    t0 = time.perf_counter()
+    o.render(cfg, 0, spp, pixels=pix, threads=threads)
+    rate = 1024 * spp / (time.perf_counter() - t0)
+    n = int(min(npx, max(1024, rate * budget_s / spp)))
+    pix = np.sort(rng.choice(npx, n, replace=False)).astype(np.int32)
+    t0 = time.perf_counter()
+    o.render(cfg, 0, spp, pixels=pix, threads=threads)
+    dt = time.perf_counter() - t0
+    o.close()
+    return {"value": n * spp / dt / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port",
+            "sample": f"{n} seeded pixels x {spp} spp of the same scene ({n * spp} paths, {dt:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--spp", type=int, default=64)
+    ap.add_argument("--res", type=int, default=1024)
+    ap.add_argument("--max-depth", type=int, default=8)
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(the multi-GPU design is described in DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    from seesharp_amd import Device, GpuScene, scenes
+    from seesharp_amd import abi
+
+    t0 = time.perf_counter()
+    scene = scenes.large_scene(args.res, args.res)
+    log(f"[bench] scene generated: {scene.NumTriangles} triangles in {time.perf_counter() - t0:.1f} s")
+    dev = Device(0)
+    t0 = time.perf_counter()
+    gs = GpuScene(dev, scene)
+    info = gs.info()
+    log(f"[bench] device scene ready in {time.perf_counter() - t0:.1f} s: {info}")
+    cfg = abi.PtSettings(0xC030114, args.spp, args.max_depth, 1, 1, 1, -1)
+    npaths = args.res * args.res * args.spp
+
+    for i in range(args.warmup):
+        gs.clear()
+        gs.render(cfg, 0, args.spp)
+        log(f"[bench] warmup {i + 1}/{args.warmup} done")
+    dev.set_kernel_timing(True)
+    dev.reset_kernel_timing()
+    totals = {"num_rays": 0, "num_shadow_rays": 0, "num_occluded": 0, "num_ray_hits": 0}
+    step_ms = []
+    for i in range(args.steps):
+        gs.clear()
+        t0 = time.perf_counter()
+        st = gs.render(cfg, 0, args.spp)
+        step_ms.append((time.perf_counter() - t0) * 1000.0)
+        for k in totals:
+            totals[k] += st[k]
+        log(f"[bench] step {i + 1}/{args.steps}: {step_ms[-1]:.1f} ms")
+    kernels = {k: dev.kernel_timing(k) for k in ("generate", "extend", "shade", "shadow", "accumulate")}
+    dev.set_kernel_timing(False)
+    img = gs.read_image()
+    ms = float(np.mean(step_ms))
+    value = npaths / (ms / 1000.0) / 1e6
+
+    n_ext = totals["num_rays"]
+    n_sh = totals["num_shadow_rays"]
+    n_hit = totals["num_ray_hits"]
+    alg = {
+        "extend": BYTES_EXTEND_PER_RAY * n_ext,
+        "shade": BYTES_SHADE_PER_RAY * n_ext + BYTES_SHADE_PER_HIT * n_hit,
+        "shadow": BYTES_SHADOW_PER_RAY * n_sh,
+        "accumulate": BYTES_PER_PATH * npaths * args.steps,
+        "generate": 32 * npaths * args.steps,
+    }
+    dom = max(kernels, key=lambda k: kernels[k][1])
+    launches, tot_ms = kernels[dom]
+    avg_ms = tot_ms / max(1, launches)
+    bytes_per_launch = alg[dom] / max(1, launches)
+    achieved = bytes_per_launch / (avg_ms / 1000.0) / 1e9
+    kernel_ms = sum(v[1] for v in kernels.values()) / args.steps
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Mpaths/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "large scene: 400 icospheres + 512x512-quad fBm ground + 4 quad lights + 512x256 sky, "
+                        f"{info['num_triangles']} triangles, {args.res}x{args.res}, {args.spp} spp, "
+                        f"MaxDepth {args.max_depth}, NEE + balance MIS, Russian roulette",
+            "triangles": info["num_triangles"],
+            "resolution": [args.res, args.res],
+            "spp": args.spp,
+            "max_depth": args.max_depth,
+            "paths_per_step": npaths,
+        },
+        "mrays_per_s": round((n_ext + n_sh) / args.steps / (ms / 1000.0) / 1e6, 2),
+        "ray_stats_per_step": {k: v // args.steps for k, v in totals.items()},
+        "kernel_ms_per_step": {k: round(v[1] / args.steps, 3) for k, v in kernels.items()},
+        "kernel_sum_ms_per_step": round(kernel_ms, 3),
+        "roofline": {
+            "kernel": dom,
+            "bound": "hbm",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": None,
+            "avg_launch_ms": round(avg_ms, 4),
+            "launches": launches,
+            "alg_bytes_per_launch": int(bytes_per_launch),
+        },
+        "image_mean": float(img.mean()),
+        "nan_paths": gs.nan_count(),
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(scene, args.spp, args.max_depth, args.cpu_budget)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

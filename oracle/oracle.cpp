@@ -539,8 +539,10 @@ struct or_scene {
     EnvMap env;
     Camera camera;
     int width = 0, height = 0;
+    float abs_max = 0.0f;  // largest vertex coordinate magnitude
     Bvh bvh;
 };
+
 
 namespace {
 
@@ -668,8 +670,8 @@ inline void consider(const or_scene& s, const Ray& r, const RayPre& p, int tri, 
 }
 
 // Robust slab test (Ize, JCGT 2013): the far distance is widened by 2 gamma(3).
-inline bool box_hit(const Bvh::Node& n, const Ray& r, const RayPre& p, float tmax, float* tnear) {
-    float t0 = r.tmin, t1 = tmax;
+// [t0, t1] is the query interval, already widened by the ray slack.
+inline bool box_hit(const Bvh::Node& n, const Ray& r, const RayPre& p, float t0, float t1, float* tnear) {
     for (int k = 0; k < 3; ++k) {
         float o = get(r.o, k), inv = get(p.inv, k);
         float a = (n.lo[k] - o) * inv, b = (n.hi[k] - o) * inv;
@@ -682,6 +684,15 @@ inline bool box_hit(const Bvh::Node& n, const Ray& r, const RayPre& p, float tma
     return t0 <= t1;
 }
 
+// The watertight t carries an absolute error of a few ulps of the coordinate
+// magnitudes divided by |d|; box culling uses [tmin - slack, tmax + slack] with
+// slack = 2^-16 * max(|o|, scene extent) / |d| so that it never changes which
+// triangle is closest.
+inline float ray_slack(const or_scene& s, const Ray& r) {
+    float m = std::max(s.abs_max, std::max(std::max(std::fabs(r.o.x), std::fabs(r.o.y)), std::fabs(r.o.z)));
+    return m * 1.52587890625e-05f / length(r.d);
+}
+
 HitRec trace_closest(const or_scene& s, const Ray& r, bool brute) {
     HitRec best{INFINITY, 0.0f, 0.0f, -1};
     RayPre p = precompute(r);
@@ -690,19 +701,20 @@ HitRec trace_closest(const or_scene& s, const Ray& r, bool brute) {
         for (int i = 0; i < (int)s.tris.size(); ++i) consider(s, r, p, i, &best, &tmax);
         return best;
     }
+    const float slack = ray_slack(s, r);
     int stack[128];
     int sp = 0;
     stack[sp++] = 0;
     while (sp) {
         const Bvh::Node& n = s.bvh.nodes[stack[--sp]];
         float tn;
-        if (!box_hit(n, r, p, tmax, &tn)) continue;
+        if (!box_hit(n, r, p, r.tmin - slack, tmax + slack, &tn)) continue;
         if (n.count > 0) {
             for (int i = 0; i < n.count; ++i) consider(s, r, p, s.bvh.order[n.left + i], &best, &tmax);
         } else {
             float ta, tb;
-            bool ha = box_hit(s.bvh.nodes[n.left], r, p, tmax, &ta);
-            bool hb = box_hit(s.bvh.nodes[n.left + 1], r, p, tmax, &tb);
+            bool ha = box_hit(s.bvh.nodes[n.left], r, p, r.tmin - slack, tmax + slack, &ta);
+            bool hb = box_hit(s.bvh.nodes[n.left + 1], r, p, r.tmin - slack, tmax + slack, &tb);
             if (ha && hb) {
                 if (ta <= tb) {
                     stack[sp++] = n.left + 1;
@@ -723,6 +735,7 @@ HitRec trace_closest(const or_scene& s, const Ray& r, bool brute) {
 
 bool trace_any(const or_scene& s, const Ray& r, bool brute) {
     RayPre p = precompute(r);
+    const float slack = ray_slack(s, r);
     float t, u, v;
     if (brute || s.bvh.nodes.empty()) {
         for (const Tri& T : s.tris)
@@ -736,7 +749,7 @@ bool trace_any(const or_scene& s, const Ray& r, bool brute) {
     while (sp) {
         const Bvh::Node& n = s.bvh.nodes[stack[--sp]];
         float tn;
-        if (!box_hit(n, r, p, r.tmax, &tn)) continue;
+        if (!box_hit(n, r, p, r.tmin - slack, r.tmax + slack, &tn)) continue;
         if (n.count > 0) {
             for (int i = 0; i < n.count; ++i) {
                 const Tri& T = s.tris[s.bvh.order[n.left + i]];
@@ -1628,6 +1641,9 @@ or_scene* or_scene_create(const sspt_scene_desc* d, char* err) {
         delete s;
         return nullptr;
     }
+    for (const Tri& t : s->tris)
+        for (V3 v : {t.p0, t.p1, t.p2})
+            s->abs_max = std::max(s->abs_max, std::max(std::fabs(v.x), std::max(std::fabs(v.y), std::fabs(v.z))));
     build_bvh(*s);
     return s;
 }

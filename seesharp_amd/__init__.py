@@ -1,0 +1,20 @@
+"""seesharp_amd: an MI355X-native (gfx950) GPU path tracer that is a drop-in
+`Integrator` for SeeSharp's scene model.
+
+The product is libsspt.so (HIP kernels + C ABI, include/sspt.h); this package
+is its Python host mirror of SeeSharp's object model.
+"""
+from .scene import (  # noqa: F401
+    DiffuseEmitter,
+    DiffuseMaterial,
+    EnvironmentMap,
+    FrameBuffer,
+    GenericMaterial,
+    Mesh,
+    PerspectiveCamera,
+    Scene,
+    TextureMono,
+    TextureRgb,
+    create_look_at,
+)
+from .integrator import Device, GpuPathTracer, GpuScene, Integrator  # noqa: F401

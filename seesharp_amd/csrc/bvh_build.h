@@ -1,0 +1,27 @@
+// bvh_build.h -- host binned-SAH BVH builder producing the device layout of scene_types.h.
+#pragma once
+
+#include <stdint.h>
+
+#include <vector>
+
+namespace sspt {
+
+struct BvhInput {
+    // triangle vertex positions, 9 floats per triangle, in global triangle order
+    const float* tri_vertices;
+    int num_tris;
+};
+
+struct BvhOutput {
+    std::vector<float> nodes;     // 16 floats per node (4 x float4)
+    std::vector<int> leaf_order;  // leaf_order[i] = global triangle id stored at leaf index i
+    int depth = 0;
+    float scene_lo[3], scene_hi[3];
+};
+
+// Binned SAH (16 bins on centroid bounds), leaves of at most `max_leaf`
+// triangles, depth bounded by kMaxBvhDepth. Uses up to `threads` host threads.
+void build_bvh(const BvhInput& in, int max_leaf, int threads, BvhOutput* out);
+
+}  // namespace sspt

@@ -1,0 +1,197 @@
+"""GpuPathTracer: the drop-in `Integrator` over libsspt.so.
+
+Mirrors ``Integrator`` (SeeSharp/Integrators/Integrator.cs:6-30) and the public
+settings of ``PathTracerBase`` (SeeSharp/Integrators/PathTracer.cs:16-49):
+BaseSeed, TotalSpp, MaximumRenderTimeMs, NumShadowRays, EnableBsdfDI,
+MaxDepth, MinDepth. ``Render(scene)`` runs every iteration on the GPU and
+leaves the running mean in ``scene.FrameBuffer.Image`` with the same
+metadata keys the reference writes (RenderTime, NumIterations,
+RayTracerStats; PathTracer.cs:254-257, FrameBuffer.cs:169-180).
+
+There is no CPU fallback: without libsspt.so or a GPU every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+
+import numpy as np
+
+from . import abi
+
+
+class Device:
+    """One HIP device context (sspt_ctx) with its own stream."""
+
+    def __init__(self, device: int = 0):
+        self.lib = abi.load_library()
+        h = C.c_void_p()
+        st = self.lib.sspt_ctx_create(device, C.byref(h))
+        if st != abi.SSPT_OK:
+            raise abi.SsptError(st, f"sspt_ctx_create(device={device}) failed (no usable HIP device?)")
+        self.ptr = h
+
+    def check(self, st: int):
+        if st != abi.SSPT_OK:
+            raise abi.SsptError(st, self.lib.sspt_last_error(self.ptr).decode())
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            self.lib.sspt_ctx_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        self.close()
+
+    # kernel timing with HIP events on the context stream
+    def set_kernel_timing(self, on: bool):
+        self.check(self.lib.sspt_set_kernel_timing(self.ptr, 1 if on else 0))
+
+    def reset_kernel_timing(self):
+        self.check(self.lib.sspt_reset_kernel_timing(self.ptr))
+
+    def kernel_timing(self, name: str):
+        n = C.c_uint64()
+        ms = C.c_double()
+        self.check(self.lib.sspt_get_kernel_timing(self.ptr, name.encode(), C.byref(n), C.byref(ms)))
+        return int(n.value), float(ms.value)
+
+
+class GpuScene:
+    """The device copy of a prepared Scene: BVH, triangles, materials, emitters,
+    environment sampling grid and the framebuffer (sspt_scene)."""
+
+    def __init__(self, device: Device, scene):
+        self.device = device
+        self.scene = scene
+        self.desc = scene.to_desc()
+        h = C.c_void_p()
+        device.check(device.lib.sspt_scene_create(device.ptr, C.byref(self.desc), C.byref(h)))
+        self.ptr = h
+        self.width = scene.FrameBuffer.Width
+        self.height = scene.FrameBuffer.Height
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            self.device.lib.sspt_scene_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        self.close()
+
+    def info(self):
+        v = [C.c_uint64() for _ in range(4)]
+        self.device.check(self.device.lib.sspt_scene_info(self.ptr, *[C.byref(x) for x in v]))
+        return dict(zip(("num_triangles", "num_emitters", "num_bvh_nodes", "device_bytes"), (x.value for x in v)))
+
+    def clear(self):
+        self.device.check(self.device.lib.sspt_framebuffer_clear(self.device.ptr, self.ptr))
+
+    def read_image(self) -> np.ndarray:
+        img = np.zeros((self.height, self.width, 3), np.float32)
+        self.device.check(self.device.lib.sspt_framebuffer_read(self.device.ptr, self.ptr, abi.fptr(img)))
+        return img
+
+    def nan_count(self) -> int:
+        n = C.c_uint64()
+        self.device.check(self.device.lib.sspt_framebuffer_nan_count(self.device.ptr, self.ptr, C.byref(n)))
+        return int(n.value)
+
+    def render(self, settings: abi.PtSettings, first_iteration: int, num_iterations: int) -> dict:
+        stats = abi.RayStats()
+        self.device.check(self.device.lib.sspt_render(self.device.ptr, self.ptr, C.byref(settings),
+                                                      first_iteration, num_iterations, C.byref(stats)))
+        return stats.as_dict()
+
+    def trace_closest(self, rays: np.ndarray) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        hits = (abi.Hit * rays.shape[0])()
+        self.device.check(self.device.lib.sspt_trace_closest(self.device.ptr, self.ptr, abi.fptr(rays),
+                                                             rays.shape[0], hits))
+        return _hits(hits)
+
+    def trace_occluded(self, rays: np.ndarray) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        out = np.zeros(rays.shape[0], np.uint8)
+        self.device.check(self.device.lib.sspt_trace_occluded(
+            self.device.ptr, self.ptr, abi.fptr(rays), rays.shape[0], out.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return out
+
+    def primary_hits(self, base_seed: int, iteration: int):
+        n = self.width * self.height
+        rays = np.zeros((n, 8), np.float32)
+        hits = (abi.Hit * n)()
+        self.device.check(self.device.lib.sspt_primary_hits(self.device.ptr, self.ptr, base_seed, iteration,
+                                                            abi.fptr(rays), hits))
+        return rays, _hits(hits)
+
+
+def _hits(hits):
+    dt = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("prim", "<i4")])
+    return np.frombuffer(hits, dtype=dt).copy()
+
+
+class Integrator:
+    """SeeSharp/Integrators/Integrator.cs:6-30."""
+
+    def __init__(self):
+        self.MaxDepth = 100
+        self.MinDepth = 1
+
+    def Render(self, scene):  # pragma: no cover - abstract
+        raise NotImplementedError
+
+    def ReplayPixel(self, scene, pixel, iteration):
+        raise NotImplementedError("Path replay is not supported by this integrator")
+
+
+class GpuPathTracer(Integrator):
+    """The unidirectional path tracer with NEE, balance-heuristic MIS and
+    Russian roulette (SeeSharp/Integrators/PathTracer.cs) on one GPU."""
+
+    def __init__(self, device: Device | None = None):
+        super().__init__()
+        self.BaseSeed = 0xC030114
+        self.TotalSpp = 20
+        self.MaximumRenderTimeMs = None
+        self.NumShadowRays = 1
+        self.EnableBsdfDI = True
+        # The OIDN denoiser and the technique pyramid are separate products; the
+        # image the path tracer writes does not depend on them.
+        self.RenderTechniquePyramid = False
+        self.EnableDenoiser = False
+        self.device = device
+        self._gpu_scene = None
+
+    def settings(self) -> abi.PtSettings:
+        return abi.PtSettings(
+            self.BaseSeed & 0xFFFFFFFF, self.TotalSpp, self.MaxDepth, self.MinDepth, self.NumShadowRays,
+            1 if self.EnableBsdfDI else 0,
+            -1 if self.MaximumRenderTimeMs is None else int(self.MaximumRenderTimeMs))
+
+    def prepare(self, scene) -> GpuScene:
+        """Scene.Prepare on the device (BVH build, emitter maps, env grid)."""
+        if self.device is None:
+            self.device = Device(0)
+        if self._gpu_scene is None or self._gpu_scene.scene is not scene:
+            self._gpu_scene = GpuScene(self.device, scene)
+        return self._gpu_scene
+
+    def Render(self, scene):
+        if self.RenderTechniquePyramid:
+            raise NotImplementedError("RenderTechniquePyramid is not supported by the GPU path tracer")
+        gs = self.prepare(scene)
+        fb = scene.FrameBuffer
+        gs.clear()
+        t0 = time.perf_counter()
+        stats = gs.render(self.settings(), 0, self.TotalSpp)
+        render_ms = (time.perf_counter() - t0) * 1000.0
+        fb.Image = gs.read_image()
+        fb.CurIteration = self.TotalSpp
+        fb.NaNWarnings = gs.nan_count()
+        fb.MetaData["RenderTime"] = render_ms
+        fb.MetaData["NumIterations"] = self.TotalSpp
+        fb.MetaData["RayTracerStats"] = {
+            "NumRays": stats["num_rays"], "NumShadowRays": stats["num_shadow_rays"],
+            "NumOccluded": stats["num_occluded"], "NumRayHits": stats["num_ray_hits"]}
+        return fb
