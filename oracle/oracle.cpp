@@ -1534,6 +1534,41 @@ void or_material_sample(const sspt_material_desc* mat, const sspt_texture_desc* 
     delete s;
 }
 
+// Monte Carlo integrals of Material_SampleTransform.cs:26-117 over n samples:
+// mode 0: pdf with uniform sphere directions, 1: average of f*|cos| with uniform
+// sphere directions, 2: average BSDF-sample weight. Sample i draws from
+// RNG(HashSeed(seed, i, 0)); uniform sphere = SampleWarp.ToUniformSphere (SampleWarp.cs:230-237).
+void or_material_integrate(const sspt_material_desc* mat, const sspt_texture_desc* textures,
+                           const float* normal, const float* uv, const float* out_dir, int64_t n,
+                           uint32_t seed, int32_t mode, double* mean, double* var) {
+    SurfacePoint p;
+    or_scene* s = flat_surface(mat, textures, normal, uv, &p);
+    Shader sh = make_shader(*s, p, v3(out_dir[0], out_dir[1], out_dir[2]));
+    double sum = 0.0, sq = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        Rng rng{hash_seed(seed, (uint32_t)i, 0)};
+        float val = 0.0f;
+        if (mode == 2) {
+            float pc = rng.next_float();
+            V2 pd = rng.next_float2();
+            BsdfSample bs = sample_bsdf(sh, pc, pd);
+            if (bs.pdf > 0.0f) val = average(bs.weight);
+        } else {
+            V2 u = rng.next_float2();
+            float a = 2.0f * kPi * u.y;
+            float b = 2.0f * std::sqrt(u.x - u.x * u.x);
+            V3 d = spherical_to_cartesian(b, 1.0f - 2.0f * u.x, a);
+            float pdf_u = 1.0f / (4.0f * kPi);
+            val = (mode == 0 ? pdf_fwd(sh, d) : average(eval_cos(sh, d))) / pdf_u;
+        }
+        sum += val;
+        sq += (double)val * val;
+    }
+    *mean = sum / (double)n;
+    *var = sq / (double)n - (*mean) * (*mean);
+    delete s;
+}
+
 // Scene.Prepare (SeeSharp/Scene.cs:103-164) + IsValid (:170-197)
 or_scene* or_scene_create(const sspt_scene_desc* d, char* err) {
     if (!d) {

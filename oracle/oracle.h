@@ -57,6 +57,12 @@ void or_material_sample(const sspt_material_desc* mat, const sspt_texture_desc* 
                         const float* normal3, const float* uv2, const float* out_dir3,
                         float primary_component, const float* primary2, float* sample_out);
 
+/* Monte Carlo integral over n samples: mode 0 pdf (uniform sphere), 1 f*|cos|
+ * (uniform sphere), 2 BSDF-sample weight. Returns the mean and the variance. */
+void or_material_integrate(const sspt_material_desc* mat, const sspt_texture_desc* textures,
+                           const float* normal3, const float* uv2, const float* out_dir3, int64_t n,
+                           uint32_t seed, int32_t mode, double* mean, double* var);
+
 /* ---- scene (SeeSharp/Scene.cs Prepare) ---- */
 typedef struct or_scene or_scene;
 /* Returns NULL and writes a message to err (if non-NULL, 256 bytes) on failure. */

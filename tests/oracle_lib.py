@@ -41,6 +41,9 @@ _SIGS = {
                                 _fp, _fp, _fp, _fp]),
     "or_material_sample": (None, [C.POINTER(abi.MaterialDesc), C.POINTER(abi.TextureDesc), _fp, _fp, _fp,
                                   C.c_float, _fp, _fp]),
+    "or_material_integrate": (None, [C.POINTER(abi.MaterialDesc), C.POINTER(abi.TextureDesc), _fp, _fp, _fp,
+                                     C.c_int64, C.c_uint32, C.c_int32, C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double)]),
     "or_scene_create": (_vp, [C.POINTER(abi.SceneDesc), C.c_char_p]),
     "or_scene_destroy": (None, [_vp]),
     "or_env_eval": (None, [_vp, _fp, _fp]),

@@ -1,0 +1,569 @@
+"""CPU tests that pin the oracle (the checker of the GPU path).
+
+Each test restates one of the reference's own unit tests with its inputs and
+tolerances (SeeSharp.Tests/...), or checks a closed form, an invariant or a
+published test vector that a plausible mistake in the restatement would break.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from seesharp_amd import abi, scenes
+from seesharp_amd.scene import (DiffuseEmitter, DiffuseMaterial, EnvironmentMap, FrameBuffer, GenericMaterial,
+                                Mesh, PerspectiveCamera, Scene, create_look_at)
+
+L = ol.lib()
+
+
+def f3(*v):
+    return (C.c_float * len(v))(*v)
+
+
+def arr(n):
+    return (C.c_float * n)()
+
+
+# ------------------------------------------------------------------------- RNG
+
+def test_fnv_hash_matches_published_fnv1_vector():
+    # FNV-1 32-bit of "a" is 0x050c5d7e (FNV reference test vectors). RNG.FnvHash
+    # consumes the 4 bytes of a u32; for bytes (0x61, 0, 0, 0) the last three
+    # steps are plain multiplications by the prime.
+    prime = 16777619
+    expected = (0x050C5D7E * prime * prime * prime) & 0xFFFFFFFF
+    assert L.or_fnv_hash(2166136261, 0x61) == expected
+
+
+def test_rng_state_is_the_published_pcg32_lcg():
+    # PCG's 32-bit LCG: multiplier 747796405, increment 2891336453 (pcg_variants.h
+    # PCG_DEFAULT_MULTIPLIER_32 / PCG_DEFAULT_INCREMENT_32); output RXS-M-XS.
+    seed = 0x12345678
+    u = (C.c_uint32 * 4)()
+    L.or_rng_stream(seed, 4, u, None)
+    state = seed
+    for k in range(4):
+        word = (((state >> ((state >> 28) + 4)) ^ state) * 277803737) & 0xFFFFFFFF
+        assert u[k] == ((word >> 22) ^ word) & 0xFFFFFFFF
+        state = (state * 747796405 + 2891336453) & 0xFFFFFFFF
+
+
+def test_hash_seed_agrees_with_independent_python_restatement():
+    rng = np.random.default_rng(0)
+    for b, c, s in rng.integers(0, 2**32, size=(200, 3), dtype=np.uint64):
+        assert L.or_hash_seed(int(b), int(c), int(s)) == scenes.hash_seed(int(b), int(c), int(s))
+    r = scenes.Rng(L.or_hash_seed(0xC030114, 5, 7))
+    f = arr(8)
+    L.or_rng_stream(L.or_hash_seed(0xC030114, 5, 7), 8, None, f)
+    for k in range(8):
+        assert np.float32(r.next_float()) == np.float32(f[k])
+
+
+def test_pcg_hash_is_a_bijection_on_a_large_range():
+    xs = np.arange(1 << 18, dtype=np.uint64)
+    out = {L.or_pcg_hash(int(x)) for x in xs[::7]}
+    assert len(out) == len(xs[::7])
+
+
+def test_next_float_is_uniform_in_closed_unit_interval():
+    n = 200000
+    f = np.zeros(n, np.float32)
+    L.or_rng_stream(0xABCDEF, n, None, ol.fp(f))
+    assert f.min() >= 0.0 and f.max() <= 1.0
+    hist, _ = np.histogram(f, bins=20, range=(0, 1))
+    chi2 = ((hist - n / 20) ** 2 / (n / 20)).sum()
+    assert chi2 < 45  # 19 dof, p ~ 1e-3
+
+
+def test_next_int_range_and_uniformity():
+    for m in (2, 3, 7, 8, 1000):
+        n = 60000
+        out = (C.c_uint32 * n)()
+        L.or_rng_next_int(0x777 + m, m, n, out)
+        a = np.frombuffer(out, np.uint32)
+        assert a.max() < m
+        if m <= 8:
+            cnt = np.bincount(a, minlength=m)
+            assert ((cnt - n / m) ** 2 / (n / m)).sum() < 30
+
+
+# ----------------------------------------------------------------------- warps
+
+def test_cos_hemisphere_closed_forms():
+    rng = np.random.default_rng(3)
+    zs = []
+    d = arr(3)
+    pdf = C.c_float()
+    for u in rng.uniform(size=(20000, 2)).astype(np.float32):
+        L.or_warp_cos_hemisphere(f3(*u), d, C.byref(pdf))
+        assert abs(d[0] ** 2 + d[1] ** 2 + d[2] ** 2 - 1) < 1e-5
+        assert abs(pdf.value - d[2] / math.pi) < 1e-6  # SampleWarp.cs:136
+        zs.append(d[2])
+    assert abs(np.mean(zs) - 2 / 3) < 0.01  # E[cos] under cos/pi
+
+
+def test_uniform_triangle_is_uniform():
+    rng = np.random.default_rng(4)
+    b = arr(2)
+    us, vs = [], []
+    for u in rng.uniform(size=(20000, 2)).astype(np.float32):
+        L.or_warp_uniform_triangle(f3(*u), b)
+        assert b[0] >= 0 and b[1] >= 0 and b[0] + b[1] <= 1 + 1e-6
+        us.append(b[0])
+        vs.append(b[1])
+    assert abs(np.mean(us) - 1 / 3) < 0.01 and abs(np.mean(vs) - 1 / 3) < 0.01
+
+
+# ------------------------------------------------------------ piecewise const
+
+def pwc_sample(w, u):
+    w = np.asarray(w, np.float32)
+    idx = C.c_int32()
+    rel = C.c_float()
+    L.or_pwc_sample(ol.fp(w), w.size, u, C.byref(idx), C.byref(rel))
+    return idx.value, rel.value
+
+
+def test_piecewise_constant_reference_cases():
+    # SeeSharp.Tests/Core/Sampling/PiecewiseConstant_DistributionTest.cs:8-98
+    w = np.array([1, 1, 2, 2], np.float32)
+    for i, p in enumerate([1 / 6, 1 / 6, 2 / 6, 2 / 6]):
+        assert round(L.or_pwc_probability(ol.fp(w), 4, i), 3) == round(p, 3)
+    assert pwc_sample(w, 0.0) == (0, 0.0)
+    i, r = pwc_sample(w, 1.0)
+    assert i == 3 and round(r, 6) == 1.0
+    i, r = pwc_sample(w, 0.999)
+    assert i == 3 and round(r, 2) == round(0.999, 2)
+    i, r = pwc_sample(w, 0.0001)
+    assert i == 0 and round(r, 6) == round(0.0001 * 6, 6)
+    counters = np.zeros(4)
+    for u in np.arange(0, 1, 0.01, dtype=np.float32):
+        counters[pwc_sample(w, float(u))[0]] += 6 / 100
+    assert np.allclose(counters, [1, 1, 2, 2], atol=0.05)
+    single = [283, 0, 0, 0, 0]
+    for u in (0.0, 1.0, 0.999, 0.0001, 0.1, 0.8):
+        assert pwc_sample(single, u)[0] == 0
+
+
+# -------------------------------------------------------------------- camera
+
+def camera_ray(fx, fy):
+    m = create_look_at((0, 0, 0), (0, 0, 10), (0, 1, 0))
+    out = arr(6)
+    L.or_camera_ray(ol.fp(np.ascontiguousarray(m.reshape(16))), 90.0, 3, 3, fx, fy, out)
+    return np.array(out[:])
+
+
+def test_perspective_rays_reference_cases():
+    # SeeSharp.Tests/Core/Camera/Perspective_Rays.cs: 3x3, 90 deg, at the origin looking at +z
+    c = math.cos(math.pi / 4)
+    e = c / math.sqrt(c * c * 3)
+    r = camera_ray(0, 0)
+    assert np.allclose(r[:3], 0) and np.allclose(r[3:], [e, e, e], atol=5e-4)
+    r = camera_ray(1.5, 1.5)
+    assert np.allclose(r[3:], [0, 0, 1], atol=5e-4)
+    r = camera_ray(3, 3)
+    assert np.allclose(r[3:], [-e, -e, e], atol=5e-4)
+    r = camera_ray(3, 0)
+    assert np.allclose(r[3:], [-e, e, e], atol=5e-4)
+
+
+def test_look_at_matches_oracle_restatement():
+    pos, tgt, up = (0.5, 0.5, 2.4), (0.5, 0.5, 0.0), (0.0, 1.0, 0.0)
+    out = arr(16)
+    L.or_look_at(f3(*pos), f3(*tgt), f3(*up), out)
+    assert np.allclose(np.array(out[:]).reshape(4, 4), create_look_at(pos, tgt, up), atol=1e-6)
+
+
+# ----------------------------------------------------------- GGX / fresnel
+
+def test_trowbridge_reitz_reference_cases():
+    # SeeSharp.Tests/Core/Sampling/TrowbridgeReitz_CorrectValues.cs
+    ax, ay = 0.5, 0.3
+    assert round(L.or_ggx_ndf(ax, ay, f3(0, 0, 1)), 4) == round(1 / (math.pi * ax * ay), 4)
+    assert round(L.or_ggx_ndf(ax, ay, f3(1, 0, 0)), 4) == 0
+    out = arr(3)
+    for o, p in (((0, 0, 1), (0, 1)), ((math.sqrt(2), 0, math.sqrt(2)), (1, 0)), ((0, 0, 1), (1, 0))):
+        L.or_ggx_sample(ax, ay, f3(*o), f3(*p), out)
+        assert all(math.isfinite(x) for x in out)
+
+
+def test_ggx_visible_normal_pdf_integrates_to_one():
+    # the VNDF pdf over the hemisphere of half vectors integrates to 1 (closed form)
+    ax, ay = 0.4, 0.25
+    o = np.array([0.3, -0.2, 0.93], np.float32)
+    o /= np.linalg.norm(o)
+    rng = np.random.default_rng(5)
+    n = 40000
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    d[:, 2] = np.abs(d[:, 2])
+    vals = [L.or_ggx_pdf(ax, ay, f3(*o), f3(*x)) for x in d.astype(np.float32)]
+    est = np.mean(vals) * 2 * math.pi
+    assert abs(est - 1) < 0.05
+
+
+def test_fresnel_dielectric_closed_forms():
+    # normal incidence ((eta-1)/(eta+1))^2, total internal reflection beyond the critical angle
+    assert abs(L.or_fresnel_dielectric(1.0, 1.0, 1.5) - 0.04) < 1e-6
+    assert L.or_fresnel_dielectric(-0.1, 1.0, 1.5) == 1.0
+    assert abs(L.or_fresnel_dielectric(-1.0, 1.0, 1.5) - 0.04) < 1e-6
+
+
+# ---------------------------------------------------------------- materials
+
+def diffuse_desc(color=(0.5, 0.5, 0.5), transmitter=False):
+    m = abi.MaterialDesc()
+    m.kind = abi.MATERIAL_DIFFUSE
+    m.base_color[:] = color
+    m.base_color_texture = -1
+    m.roughness_texture = -1
+    m.transmitter = 1 if transmitter else 0
+    return m
+
+
+def generic_desc(color=(0.6, 0.5, 0.4), rough=0.4, metallic=0.0, trans=0.0, ior=1.5, tint=1.0):
+    m = abi.MaterialDesc()
+    m.kind = abi.MATERIAL_GENERIC
+    m.base_color[:] = color
+    m.base_color_texture = -1
+    m.roughness = rough
+    m.roughness_texture = -1
+    m.metallic = metallic
+    m.specular_tint_strength = tint
+    m.specular_transmittance = trans
+    m.index_of_refraction = ior
+    return m
+
+
+NORMAL = (0.0, 1.0, 0.0)
+UV = (0.0, 0.0)
+
+
+def evaluate(m, out_dir, in_dir):
+    v, vc, pdf = arr(3), arr(3), C.c_float()
+    L.or_material_eval(C.byref(m), None, f3(*NORMAL), f3(*UV), f3(*out_dir), f3(*in_dir), v, vc, C.byref(pdf))
+    return np.array(v[:]), np.array(vc[:]), pdf.value
+
+
+def sample(m, out_dir, pc, p2):
+    out = arr(7)
+    L.or_material_sample(C.byref(m), None, f3(*NORMAL), f3(*UV), f3(*out_dir), pc, f3(*p2), out)
+    return np.array(out[:3]), out[3], np.array(out[4:7])
+
+
+def unit(v):
+    v = np.asarray(v, np.float64)
+    return tuple((v / np.linalg.norm(v)).astype(np.float32))
+
+
+def test_diffuse_reference_cases():
+    # SeeSharp.Tests/Core/Shading/Material_Diffuse.cs: no light leaks, value albedo/pi,
+    # pdf symmetric in the cosines, sample weight = albedo
+    m = diffuse_desc((1, 1, 1))
+    o, i = unit((0.2, 1, 0.1)), unit((-0.4, 1, 0.3))
+    v, vc, pdf = evaluate(m, o, i)
+    assert np.allclose(v, 1 / math.pi, rtol=1e-6)
+    assert abs(pdf - i[1] / math.pi) < 1e-6
+    v, _, pdf = evaluate(m, o, unit((0.3, -1, 0)))
+    assert np.all(v == 0) and pdf == 0
+    d, pdf, w = sample(m, o, 0.3, (0.3, 0.7))
+    assert pdf > 0 and np.allclose(w, 1.0, rtol=1e-4) and d[1] > 0
+    d, pdf, w = sample(m, unit((0.2, -1, 0)), 0.3, (0.3, 0.7))
+    assert d[1] < 0  # sampled on the side of the outgoing direction
+
+
+def test_diffuse_transmitter_splits_pdf():
+    m = diffuse_desc((1, 1, 1), transmitter=True)
+    o = unit((0.2, 1, 0.1))
+    _, _, pdf_r = evaluate(m, o, unit((0.1, 1, 0.1)))
+    v, _, pdf_t = evaluate(m, o, unit((0.1, -1, 0.1)))
+    assert abs(pdf_r - 0.5 * unit((0.1, 1, 0.1))[1] / math.pi) < 1e-6
+    assert np.allclose(v, 1 / math.pi, rtol=1e-6) and pdf_t > 0
+
+
+@pytest.mark.parametrize("params", [
+    dict(rough=1.0), dict(rough=0.3), dict(rough=0.1, metallic=1.0), dict(rough=0.2, trans=0.9),
+    dict(rough=0.6, metallic=0.5, tint=0.0)])
+def test_generic_sample_pdf_consistency(params):
+    # SeeSharp.Tests/Core/Shading/Material_Generic.cs:11-54: the pdf of a sampled
+    # direction equals the pdf the sample reports; weight = f * |cos| / pdf.
+    m = generic_desc(**params)
+    o = unit((0.3, 0.8, -0.2))
+    rng = np.random.default_rng(6)
+    for pc, a, b in rng.uniform(size=(200, 3)).astype(np.float32):
+        d, pdf, w = sample(m, o, float(pc), (float(a), float(b)))
+        if pdf == 0:
+            continue
+        v, vc, pdf2 = evaluate(m, o, tuple(d))
+        assert abs(pdf2 - pdf) <= 1e-3 * max(1.0, pdf)
+        assert np.allclose(w, vc / pdf, rtol=2e-3, atol=1e-6)
+
+
+# SeeSharp.Tests/Core/Shading/Material_SampleTransform.cs:119-303: the dummy hit
+# is a z = -2 quad hit from the origin, so the normal is +z. (name, material,
+# outgoing direction) exactly as the reference's cases.
+def _white(**k):
+    return generic_desc(color=(1, 1, 1), **k)
+
+
+SAMPLE_TRANSFORM_CASES = [
+    ("diffuse_transmitter", lambda: diffuse_desc((1, 1, 1), transmitter=True), (1, 0, 0.1)),
+    ("generic_rough", lambda: _white(rough=0.9, ior=1.1), (0, 0, 1)),
+    ("generic_glossy_normal", lambda: _white(rough=0.2, metallic=0.6, ior=1.45), (0, 0, 1)),
+    ("generic_glossy_underneath", lambda: _white(rough=0.2, metallic=0.6, ior=1.45), (0, 0, -1)),
+    ("generic_glossy_grazing", lambda: _white(rough=0.2, metallic=0.6, ior=1.45), (1, 0, 0.01)),
+    ("generic_glass_normal", lambda: _white(rough=0.1, ior=1.45, tint=0.0, trans=1.0), (0, 0, 1)),
+    ("generic_glass_grazing", lambda: _white(rough=0.1, ior=1.45, tint=0.0, trans=1.0), (1, 0, 0.01)),
+    ("generic_glass_underneath", lambda: _white(rough=0.1, ior=1.45, tint=0.0, trans=1.0), (1, 0, -1.1)),
+]
+
+
+def _uniform_dirs(n, seed):
+    d = np.random.default_rng(seed).normal(size=(n, 3))
+    return (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+
+
+def _ci(vals):
+    # mean +- 3.3 standard errors (the reference uses 99% intervals)
+    m = float(np.mean(vals))
+    h = 3.3 * float(np.std(vals)) / math.sqrt(len(vals))
+    return m - h, m + h
+
+
+def _overlap(a, b):
+    return b[0] < a[1] and b[1] > a[0]
+
+
+def integrate(m, out_dir, mode, n=1_000_000, seed=7):
+    mean, var = C.c_double(), C.c_double()
+    L.or_material_integrate(C.byref(m), None, f3(0, 0, 1), f3(0, 0), f3(*unit(out_dir)), n, seed, mode,
+                            C.byref(mean), C.byref(var))
+    h = 3.3 * math.sqrt(max(var.value, 0.0) / n)
+    return mean.value - h, mean.value + h
+
+
+# the reference asserts pdf densities for every case but glass along the normal
+PDF_CASES = [c for c in SAMPLE_TRANSFORM_CASES if c[0] != "generic_glass_normal"]
+
+
+@pytest.mark.parametrize("name,make,out_dir", PDF_CASES, ids=[c[0] for c in PDF_CASES])
+def test_sample_transform_pdf_is_a_density(name, make, out_dir):
+    ci = integrate(make(), out_dir, 0)
+    assert _overlap(ci, (1.0, 1.0)), ci
+
+
+@pytest.mark.parametrize("name,make,out_dir", SAMPLE_TRANSFORM_CASES, ids=[c[0] for c in SAMPLE_TRANSFORM_CASES])
+def test_sample_transform_albedo_matches(name, make, out_dir):
+    a, b = integrate(make(), out_dir, 1, seed=8), integrate(make(), out_dir, 2, seed=9)
+    assert _overlap(a, b), (a, b)
+
+
+@pytest.mark.parametrize("rough,out_dir", [(0.9, (1, 0, 0.1)), (0.1, (1, 0, 0.1)), (0.9, (1, 0, 1)), (0.1, (1, 0, 1))])
+def test_trowbridge_reitz_pdf_is_a_density(rough, out_dir):
+    # Material_SampleTransform.cs:282-303
+    a = max(0.001, rough * rough)
+    o = unit(out_dir)
+    vals = [L.or_ggx_pdf(a, a, f3(*o), f3(*x)) * 4 * math.pi for x in _uniform_dirs(100000, 14)]
+    assert _overlap(_ci(vals), (1.0, 1.0)), _ci(vals)
+
+
+# ------------------------------------------------------------ environment map
+
+def env_scene(image):
+    s = Scene()
+    mesh = Mesh([[100, 100, 100], [101, 100, 100], [100, 101, 100]], [0, 1, 2])
+    mesh.Material = DiffuseMaterial(DiffuseMaterial.Parameters())
+    s.Meshes.append(mesh)
+    s.Background = EnvironmentMap(image)
+    s.Camera = PerspectiveCamera(create_look_at((0, 0, 0), (0, 0, 1), (0, 1, 0)), 40)
+    s.FrameBuffer = FrameBuffer(8, 8)
+    return ol.OracleScene(s)
+
+
+def env_eval(o, d):
+    out = arr(3)
+    L.or_env_eval(o.ptr, f3(*d), out)
+    return np.array(out[:])
+
+
+def test_environment_map_reference_cases():
+    # SeeSharp.Tests/Core/Shading/Background_EnvironmentMap.cs
+    img = np.zeros((256, 512, 3), np.float32)
+    img[64, 128] = 10
+    o = env_scene(img)
+    assert np.all(env_eval(o, (0, 1, 1)) == 10.0)
+    assert np.all(env_eval(o, (0, -1, 0)) == 0) and np.all(env_eval(o, (-1, 0, 0)) == 0)
+    out = arr(7)
+    L.or_env_sample(o.ptr, f3(0.2, 0.2), out)
+    pdf = L.or_env_pdf(o.ptr, f3(*out[:3]))
+    assert round(pdf, 3) == round(out[3], 3)
+    rad = env_eval(o, tuple(out[:3]))
+    assert np.all(np.float32(rad / np.float32(pdf)) == np.array(out[4:7], np.float32))
+    img2 = np.zeros((256, 512, 3), np.float32)
+    img2[128, 0] = 10
+    img2[128, 128] = 5
+    o2 = env_scene(img2)
+    assert np.all(env_eval(o2, (1, 0, 0)) == 10) and np.all(env_eval(o2, (0, 0, 1)) == 5)
+
+
+def test_environment_sampling_pdf_matches_luminance_density():
+    rng = np.random.default_rng(9)
+    img = rng.uniform(0.1, 2.0, size=(32, 64, 3)).astype(np.float32)
+    o = env_scene(img)
+    out = arr(7)
+    for u in rng.uniform(0.01, 0.99, size=(300, 2)).astype(np.float32):
+        L.or_env_sample(o.ptr, f3(*u), out)
+        pdf = L.or_env_pdf(o.ptr, f3(*out[:3]))
+        assert abs(pdf - out[3]) <= 2e-3 * out[3]
+    # the solid-angle pdf integrates to 1 over the sphere
+    d = rng.normal(size=(40000, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    est = np.mean([L.or_env_pdf(o.ptr, f3(*x)) for x in d.astype(np.float32)]) * 4 * math.pi
+    assert abs(est - 1) < 0.03
+
+
+# ---------------------------------------------------------- mesh + scene
+
+def test_face_normal_convention_and_hit_distance():
+    # Mesh_Attributes.cs:50-75: quad in the xz plane has face normal (0,-1,0);
+    # Scene_Assemble.cs:50-63: a ray from 10 units away hits at distance 10.
+    s = Scene()
+    m = Mesh([[-1, 0, -1], [1, 0, -1], [1, 0, 1], [-1, 0, 1]], [0, 1, 2, 0, 2, 3])
+    assert np.allclose(m.FaceNormals, [[0, -1, 0], [0, -1, 0]], atol=1e-6)
+    m.Material = DiffuseMaterial(DiffuseMaterial.Parameters())
+    s.Meshes.append(m)
+    s.Emitters.extend(DiffuseEmitter.MakeFromMesh(m, (1, 1, 1)))
+    s.Camera = PerspectiveCamera(create_look_at((0, 5, 0), (0, 0, 0), (0, 0, 1)), 40)
+    s.FrameBuffer = FrameBuffer(4, 4)
+    o = ol.OracleScene(s)
+    rays = np.array([[0.1, 10, 0.2, 0, 0, -1, 0, np.inf]], np.float32)
+    h = o.trace_closest(rays, brute=True)
+    assert abs(h["t"][0] - 10) < 1e-4 and h["prim"][0] in (0, 1)
+
+
+def test_scene_validation_messages():
+    s = Scene()
+    m = Mesh([[0, 0, 0], [1, 0, 0], [0, 1, 0]], [0, 1, 2])
+    s.Meshes.append(m)
+    s.Camera = PerspectiveCamera(np.eye(4), 40)
+    s.FrameBuffer = FrameBuffer(4, 4)
+    assert not s.IsValid
+    assert "Mesh[0] does not have a material." in s.ValidationErrorMessages
+    assert "No emitters and no background in the scene." in s.ValidationErrorMessages
+
+
+def test_bvh_matches_brute_force_on_the_box():
+    s = scenes.box_scene(32, 32)
+    o = ol.OracleScene(s)
+    rng = np.random.default_rng(10)
+    n = 5000
+    r = np.zeros((n, 8), np.float32)
+    r[:, :3] = rng.uniform(-0.2, 1.2, (n, 3))
+    d = rng.normal(size=(n, 3))
+    r[:, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    r[:, 7] = np.inf
+    a, b = o.trace_closest(r), o.trace_closest(r, brute=True)
+    assert np.array_equal(a["prim"], b["prim"]) and np.array_equal(a["t"], b["t"])
+    r[:, 7] = 0.25
+    assert np.array_equal(o.trace_occluded(r), o.trace_occluded(r, brute=True))
+
+
+def test_generated_meshes_wind_outward():
+    v, idx = scenes.box_mesh((0.5, 0.5, 0.5), (0.3, 0.6, 0.3), 18.0)
+    t = idx.reshape(-1, 3)
+    n = np.cross(v[t[:, 1]] - v[t[:, 0]], v[t[:, 2]] - v[t[:, 0]])
+    c = v[t].mean(axis=1) - v.mean(axis=0)
+    assert np.all((n * c).sum(axis=1) > 0)
+    sv, sf = scenes.icosphere(3)
+    assert sf.shape[0] == 1280
+    n = np.cross(sv[sf[:, 1]] - sv[sf[:, 0]], sv[sf[:, 2]] - sv[sf[:, 0]])
+    assert np.all((n * sv[sf].mean(axis=1)).sum(axis=1) > 0)
+    assert scenes.large_scene(8, 8).NumTriangles == 1036296
+
+
+# ------------------------------------------------------- whole integrator
+
+def furnace_scene(albedo, le, width=16):
+    """Closed cube, inward faces, every triangle a diffuse emitter of radiance le."""
+    s = Scene()
+    v = np.array([[x, y, z] for x in (0, 1) for y in (0, 1) for z in (0, 1)], np.float32)
+    faces = [(0, 2, 3, 1), (4, 5, 7, 6), (0, 1, 5, 4), (2, 6, 7, 3), (0, 4, 6, 2), (1, 3, 7, 5)]
+    idx = []
+    for a, b, c, d in faces:  # listed with inward-facing winding
+        idx += [a, b, c, a, c, d]
+    m = Mesh(v, idx)
+    m.Material = DiffuseMaterial(DiffuseMaterial.Parameters(BaseColor=(albedo,) * 3))
+    s.Meshes.append(m)
+    s.Emitters.extend(DiffuseEmitter.MakeFromMesh(m, (le,) * 3))
+    s.Camera = PerspectiveCamera(create_look_at((0.5, 0.5, 0.5), (0.5, 0.5, 0.0), (0, 1, 0)), 60)
+    s.FrameBuffer = FrameBuffer(width, width)
+    return s
+
+
+@pytest.mark.parametrize("depth", [1, 2, 3, 5])
+def test_furnace_closed_form(depth):
+    # radiance = Le * sum_{k < D} rho^k when Russian roulette is off (D <= 5)
+    rho, le = 0.5, 1.0
+    o = ol.OracleScene(furnace_scene(rho, le))
+    img, _, nans = o.render(ol.settings(max_depth=depth), 0, 16)
+    expected = le * sum(rho ** k for k in range(depth))
+    assert nans == 0
+    assert abs(img.mean() - expected) < 0.01 * expected
+
+
+def lambert_polygon_irradiance(p, n, poly):
+    """Irradiance at p (normal n) from a unit-radiance polygon: E = 1/2 sum beta_i (gamma_i . n)."""
+    e = 0.0
+    k = len(poly)
+    for i in range(k):
+        a = poly[i] - p
+        b = poly[(i + 1) % k] - p
+        a /= np.linalg.norm(a)
+        b /= np.linalg.norm(b)
+        beta = math.acos(np.clip(a @ b, -1, 1))
+        g = np.cross(a, b)
+        g /= np.linalg.norm(g)
+        e += beta * (g @ n)
+    return abs(0.5 * e)
+
+
+def test_direct_illumination_closed_form():
+    # simpledi-style: a 0.2 x 0.2 light of radiance 20 at y = 2 over an albedo-1
+    # diffuse plane; MaxDepth 2. L_floor = (rho/pi) * Le * E_unit (Lambert's formula).
+    s = Scene()
+    ground = Mesh([[-5, 0, -5], [-5, 0, 5], [5, 0, 5], [5, 0, -5]], [0, 1, 2, 0, 2, 3])
+    ground.Material = DiffuseMaterial(DiffuseMaterial.Parameters(BaseColor=(1, 1, 1)))
+    corners = np.array([[-0.1, 2, -0.1], [0.1, 2, -0.1], [0.1, 2, 0.1], [-0.1, 2, 0.1]], np.float32)
+    light = Mesh(corners, [0, 1, 2, 0, 2, 3])
+    light.Material = DiffuseMaterial(DiffuseMaterial.Parameters(BaseColor=(0, 0, 0)))
+    s.Meshes += [ground, light]
+    s.Emitters.extend(DiffuseEmitter.MakeFromMesh(light, (20, 20, 20)))
+    s.Camera = PerspectiveCamera(create_look_at((0, 1.0, 0), (0, 0, 0), (0, 0, 1)), 30)
+    s.FrameBuffer = FrameBuffer(8, 8)
+    o = ol.OracleScene(s)
+    n_iter = 256
+    img, _, _ = o.render(ol.settings(max_depth=2), 0, n_iter)
+    # centre pixels look straight down at points near the origin
+    rays, hits = o.primary_hits(0xC030114, 0)
+    for pix in (27, 28, 35, 36):
+        row, col = divmod(pix, 8)
+        ray = rays[pix]
+        pt = ray[:3] + hits["t"][pix] * ray[4:7]
+        e = lambert_polygon_irradiance(pt.astype(np.float64), np.array([0, 1.0, 0]), corners.astype(np.float64))
+        expected = 20 * e / math.pi
+        # jitter moves the point by < 0.05: the irradiance changes by < 0.5 % there
+        assert abs(img[row, col].mean() - expected) < 0.03 * expected
+
+
+def test_nan_estimates_are_skipped_and_counted():
+    # FrameBuffer.Splat with IgnoreNanAndInf (FrameBuffer.cs:89-107): a NaN albedo makes
+    # every path that scatters on it non-finite; those splats are skipped and counted.
+    s = furnace_scene(0.5, 1.0, width=8)
+    s.Meshes[0].Material = DiffuseMaterial(DiffuseMaterial.Parameters(BaseColor=(float("nan"),) * 3))
+    o = ol.OracleScene(s)
+    img, _, nans = o.render(ol.settings(max_depth=3), 0, 2)
+    assert nans == 2 * 64
+    assert np.all(np.isfinite(img))
