@@ -9,6 +9,7 @@ Prints one JSON line with the metric, the roofline of the dominant kernel
 oracle restatement of the reference, timed on a bounded pixel sample).
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1]
+    python bench.py --workload batch     # 64 scenes in shared waves vs per-scene launches
 """
 from __future__ import annotations
 
@@ -69,6 +70,68 @@ def cpu_baseline(scene, spp, max_depth, budget_s):
             "sample": f"{n} seeded pixels x {spp} spp of the same scene ({n * spp} paths, {dt:.1f} s)"}
 
 
+def batch_main(args):
+    """The batching workload: 64 scenes (seeds 1000..1063, 1.8k-194k triangles,
+    128^2-512^2, 4/16/64 spp, MaxDepth 8) rendered in shared waves, against the
+    same 64 scenes rendered one sspt_render call after the other."""
+    from seesharp_amd import Device, GpuBatch, GpuScene, abi, autotune, scenes
+
+    t0 = time.perf_counter()
+    sc, spp = scenes.batch_scenes(64, 1000)
+    log(f"[bench] 64 scenes generated in {time.perf_counter() - t0:.1f} s")
+    dev = Device(0)
+    cfgs = [abi.PtSettings(0xC030114, n, 8, 1, 1, 1, -1) for n in spp]
+    npaths = sum(s.FrameBuffer.Width * s.FrameBuffer.Height * n for s, n in zip(sc, spp))
+    t0 = time.perf_counter()
+    batch = GpuBatch(dev, sc)
+    info = batch.info()
+    log(f"[bench] batch world ready in {time.perf_counter() - t0:.1f} s: {info}")
+    tuned = None
+    if args.autotune:
+        tuned, _ = autotune.tune(dev, lambda: batch.render(cfgs), capacities=(1 << 20, 1 << 21, 1 << 22),
+                                 trace_per_cu=(8, 16), shade_per_cu=(4, 8), log=log)
+        log(f"[bench] autotuned: {tuned}")
+    batch.render(cfgs)  # warm-up
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.render(cfgs)
+    batch_ms = (time.perf_counter() - t0) * 1000.0 / args.steps
+    log(f"[bench] batched: {batch_ms:.1f} ms per pass")
+    # per-scene launches with the same execution settings
+    singles = [GpuScene(dev, s) for s in sc]
+    for g, c, n in zip(singles, cfgs, spp):
+        g.clear()
+        g.render(c, 0, n)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for g, c, n in zip(singles, cfgs, spp):
+            g.clear()
+            g.render(c, 0, n)
+    single_ms = (time.perf_counter() - t0) * 1000.0 / args.steps
+    log(f"[bench] per-scene launches: {single_ms:.1f} ms per pass")
+    out = {
+        "metric": "Mpaths/s of the batched wavefront (64 scenes, one world) vs per-scene launches",
+        "value": round(npaths / (batch_ms / 1000.0) / 1e6, 3),
+        "unit": "Mpaths/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": 1,
+        "ms_per_step": round(batch_ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "64 scenes, seeds 1000..1063, triangle budget log-uniform [1k, 200k], "
+                               "128^2..512^2, spp in {4,16,64}, MaxDepth 8",
+                   "triangles": info["num_triangles"], "paths_per_step": npaths},
+        "per_scene_launches_mpaths_per_s": round(npaths / (single_ms / 1000.0) / 1e6, 3),
+        "speedup_vs_per_scene": round(single_ms / batch_ms, 3),
+        "autotuned": tuned,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,11 +142,17 @@ def main():
     ap.add_argument("--max-depth", type=int, default=8)
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=("large", "batch"), default="large")
+    ap.add_argument("--autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
               "(the multi-GPU design is described in DESIGN.md).", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "batch":
+        batch_main(args)
+        return
 
     from seesharp_amd import Device, GpuScene, scenes
     from seesharp_amd import abi
@@ -103,9 +172,9 @@ def main():
         gs.clear()
         gs.render(cfg, 0, args.spp)
         log(f"[bench] warmup {i + 1}/{args.warmup} done")
-    dev.set_kernel_timing(True)
-    dev.reset_kernel_timing()
-    totals = {"num_rays": 0, "num_shadow_rays": 0, "num_occluded": 0, "num_ray_hits": 0}
+    # timed steps: waves replayed from a captured hipGraph, no per-kernel events
+    totals = {"num_rays": 0, "num_shadow_rays": 0, "num_occluded": 0, "num_ray_hits": 0,
+              "num_shadow_rays_traced": 0}
     step_ms = []
     for i in range(args.steps):
         gs.clear()
@@ -115,28 +184,38 @@ def main():
         for k in totals:
             totals[k] += st[k]
         log(f"[bench] step {i + 1}/{args.steps}: {step_ms[-1]:.1f} ms")
-    kernels = {k: dev.kernel_timing(k) for k in ("generate", "extend", "shade", "shadow", "accumulate")}
-    dev.set_kernel_timing(False)
     img = gs.read_image()
     ms = float(np.mean(step_ms))
     value = npaths / (ms / 1000.0) / 1e6
+    # one more render with HIP events around every kernel launch (direct launches)
+    dev.set_kernel_timing(True)
+    dev.reset_kernel_timing()
+    gs.clear()
+    t0 = time.perf_counter()
+    gs.render(cfg, 0, args.spp)
+    timed_ms = (time.perf_counter() - t0) * 1000.0
+    kernels = {k: dev.kernel_timing(k) for k in ("generate", "extend", "shade", "shadow", "accumulate")}
+    dev.set_kernel_timing(False)
+    log(f"[bench] event-timed render: {timed_ms:.1f} ms")
 
     n_ext = totals["num_rays"]
-    n_sh = totals["num_shadow_rays"]
+    n_sh = totals["num_shadow_rays_traced"]
     n_hit = totals["num_ray_hits"]
+    # per render (the event-timed render is one render)
+    per = 1.0 / args.steps
     alg = {
-        "extend": BYTES_EXTEND_PER_RAY * n_ext,
-        "shade": BYTES_SHADE_PER_RAY * n_ext + BYTES_SHADE_PER_HIT * n_hit,
-        "shadow": BYTES_SHADOW_PER_RAY * n_sh,
-        "accumulate": BYTES_PER_PATH * npaths * args.steps,
-        "generate": 32 * npaths * args.steps,
+        "extend": BYTES_EXTEND_PER_RAY * n_ext * per,
+        "shade": (BYTES_SHADE_PER_RAY * n_ext + BYTES_SHADE_PER_HIT * n_hit) * per,
+        "shadow": BYTES_SHADOW_PER_RAY * n_sh * per,
+        "accumulate": BYTES_PER_PATH * npaths,
+        "generate": 32 * npaths,
     }
     dom = max(kernels, key=lambda k: kernels[k][1])
     launches, tot_ms = kernels[dom]
     avg_ms = tot_ms / max(1, launches)
     bytes_per_launch = alg[dom] / max(1, launches)
     achieved = bytes_per_launch / (avg_ms / 1000.0) / 1e9
-    kernel_ms = sum(v[1] for v in kernels.values()) / args.steps
+    kernel_ms = sum(v[1] for v in kernels.values())
 
     out = {
         "metric": METRIC,
@@ -163,7 +242,8 @@ def main():
         },
         "mrays_per_s": round((n_ext + n_sh) / args.steps / (ms / 1000.0) / 1e6, 2),
         "ray_stats_per_step": {k: v // args.steps for k, v in totals.items()},
-        "kernel_ms_per_step": {k: round(v[1] / args.steps, 3) for k, v in kernels.items()},
+        "kernel_ms_per_step": {k: round(v[1], 3) for k, v in kernels.items()},
+        "event_timed_render_ms": round(timed_ms, 3),
         "kernel_sum_ms_per_step": round(kernel_ms, 3),
         "roofline": {
             "kernel": dom,

@@ -131,12 +131,17 @@ typedef struct sspt_pt_settings {
     int64_t max_render_time_ms;  /* MaximumRenderTimeMs, < 0 = unlimited */
 } sspt_pt_settings;
 
-/* TinyEmbree RayTracerStats as stored by PathTracer.cs:257. */
+/* TinyEmbree RayTracerStats as stored by PathTracer.cs:257: num_rays counts
+ * closest-hit traces, num_shadow_rays every IsOccluded / LeavesScene call,
+ * num_ray_hits the traces that hit. Shadow rays whose contribution is exactly
+ * zero are not traced: num_shadow_rays_traced counts the traced ones and
+ * num_occluded the traced ones that were blocked. */
 typedef struct sspt_ray_stats {
     uint64_t num_rays;
     uint64_t num_shadow_rays;
     uint64_t num_occluded;
     uint64_t num_ray_hits;
+    uint64_t num_shadow_rays_traced;
 } sspt_ray_stats;
 
 /* Closest-hit record of TinyEmbree `Hit` for the trace entry points. */
@@ -168,6 +173,32 @@ sspt_status sspt_scene_info(const sspt_scene* scene, uint64_t* num_triangles,
                             uint64_t* num_emitters, uint64_t* num_bvh_nodes,
                             uint64_t* device_bytes);
 
+/* ---- batching: many scenes (cameras, resolutions, triangle counts) in one
+ *      device world, rendered in shared waves. No counterpart in the reference,
+ *      which renders one Scene per Integrator.Render call; each scene of a batch
+ *      keeps the semantics of its own PathTracer render. --------------------- */
+/* Builds n (<= 128) scenes into one world (Scene.Prepare for each). */
+sspt_status sspt_batch_create(sspt_ctx* ctx, const sspt_scene_desc* descs, uint32_t n,
+                              sspt_scene** out_scene);
+sspt_status sspt_scene_count(const sspt_scene* scene, uint32_t* n);
+/* Renders settings[i].total_spp iterations (0 .. total_spp-1) of every scene i
+ * from a cleared framebuffer; each framebuffer ends as the mean of its
+ * samples (fp32 sums in arbitrary order, divided by total_spp). */
+sspt_status sspt_render_batch(sspt_ctx* ctx, sspt_scene* scene, const sspt_pt_settings* settings,
+                              sspt_ray_stats* stats);
+/* Copies scene `index`'s W*H*3 framebuffer to host memory. */
+sspt_status sspt_framebuffer_read_scene(sspt_ctx* ctx, sspt_scene* scene, uint32_t index,
+                                        float* host_rgb);
+
+/* ---- execution knobs (the autotuner's search space) ---------------------
+ * Wave capacity in path slots (0 = default 4 Mi); grid sizes of the persistent
+ * traversal and shading kernels in blocks (0 = 16 resp. 8 per CU); whether a
+ * wave is captured once as a hipGraph and replayed (default on; off while
+ * kernel timing is enabled). */
+sspt_status sspt_set_wave_capacity(sspt_ctx* ctx, uint32_t slots);
+sspt_status sspt_set_launch_config(sspt_ctx* ctx, uint32_t trace_blocks, uint32_t shade_blocks,
+                                   int32_t use_graphs);
+
 /* ---- framebuffer: the scene's FrameBuffer.Image running mean
  *      (SeeSharp/Images/FrameBuffer.cs:89-160). ------------------------- */
 /* Zeroes the device image (FrameBuffer with CurIteration = 0). */
@@ -189,6 +220,23 @@ sspt_status sspt_framebuffer_nan_count(sspt_ctx* ctx, sspt_scene* scene, uint64_
 sspt_status sspt_render(sspt_ctx* ctx, sspt_scene* scene, const sspt_pt_settings* settings,
                         uint32_t first_iteration, uint32_t num_iterations,
                         sspt_ray_stats* stats);
+
+/* ---- differentiation: path-replay adjoint of sspt_render. No counterpart in
+ *      the reference. Given dLoss/dImage for the image of iterations
+ *      first_iteration .. first_iteration+num_iterations-1, replays those paths
+ *      from their seeds and adds dLoss/dtexel of every textured base colour and
+ *      roughness to the scene's gradient buffers (BSDF values differentiated;
+ *      sampling pdfs, MIS weights and Russian roulette held constant). --- */
+sspt_status sspt_render_adjoint(sspt_ctx* ctx, sspt_scene* scene, const sspt_pt_settings* settings,
+                                const float* host_dloss_dimage, uint32_t first_iteration,
+                                uint32_t num_iterations, sspt_ray_stats* stats);
+/* Zeroes every texture gradient of the scene. */
+sspt_status sspt_gradient_clear(sspt_ctx* ctx, sspt_scene* scene);
+/* Copies the gradient of texture `texture_index` (width*height*channels floats). */
+sspt_status sspt_gradient_read(sspt_ctx* ctx, sspt_scene* scene, uint32_t texture_index, float* host);
+/* Replaces the texels of texture `texture_index` (the differentiable parameters). */
+sspt_status sspt_texture_write(sspt_ctx* ctx, sspt_scene* scene, uint32_t texture_index,
+                               const float* host_texels);
 
 /* ---- ray queries: TinyEmbree Raytracer.Trace (PathTracer.cs:319) and
  *      IsOccluded (:467) over the scene BVH, for callers and parity tests.

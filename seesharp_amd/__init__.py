@@ -17,4 +17,4 @@ from .scene import (  # noqa: F401
     TextureRgb,
     create_look_at,
 )
-from .integrator import Device, GpuPathTracer, GpuScene, Integrator  # noqa: F401
+from .integrator import Device, GpuBatch, GpuPathTracer, GpuScene, Integrator  # noqa: F401

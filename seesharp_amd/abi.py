@@ -113,6 +113,7 @@ class RayStats(C.Structure):
         ("num_shadow_rays", C.c_uint64),
         ("num_occluded", C.c_uint64),
         ("num_ray_hits", C.c_uint64),
+        ("num_shadow_rays_traced", C.c_uint64),
     ]
 
     def as_dict(self):
@@ -141,6 +142,17 @@ SYMBOLS = {
     "sspt_trace_closest": (C.c_int, [_vp, _vp, _fp, C.c_uint64, C.POINTER(Hit)]),
     "sspt_trace_occluded": (C.c_int, [_vp, _vp, _fp, C.c_uint64, C.POINTER(C.c_uint8)]),
     "sspt_primary_hits": (C.c_int, [_vp, _vp, C.c_uint32, C.c_uint32, _fp, C.POINTER(Hit)]),
+    "sspt_batch_create": (C.c_int, [_vp, C.POINTER(SceneDesc), C.c_uint32, C.POINTER(_vp)]),
+    "sspt_scene_count": (C.c_int, [_vp, C.POINTER(C.c_uint32)]),
+    "sspt_render_batch": (C.c_int, [_vp, _vp, C.POINTER(PtSettings), C.POINTER(RayStats)]),
+    "sspt_framebuffer_read_scene": (C.c_int, [_vp, _vp, C.c_uint32, _fp]),
+    "sspt_render_adjoint": (C.c_int, [_vp, _vp, C.POINTER(PtSettings), _fp, C.c_uint32, C.c_uint32,
+                                      C.POINTER(RayStats)]),
+    "sspt_gradient_clear": (C.c_int, [_vp, _vp]),
+    "sspt_gradient_read": (C.c_int, [_vp, _vp, C.c_uint32, _fp]),
+    "sspt_texture_write": (C.c_int, [_vp, _vp, C.c_uint32, _fp]),
+    "sspt_set_wave_capacity": (C.c_int, [_vp, C.c_uint32]),
+    "sspt_set_launch_config": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_int32]),
     "sspt_set_kernel_timing": (C.c_int, [_vp, C.c_int32]),
     "sspt_get_kernel_timing": (C.c_int, [_vp, C.c_char_p, C.POINTER(C.c_uint64),
                                          C.POINTER(C.c_double)]),

@@ -1,6 +1,6 @@
-// api.hip -- the C ABI of include/sspt.h: contexts, scenes (Scene.Prepare),
-// framebuffers and the render loop of PathTracerBase.Render
-// (SeeSharp/Integrators/PathTracer.cs:199-267).
+// api.hip -- the C ABI of include/sspt.h: contexts, scenes and batches of
+// scenes (Scene.Prepare), framebuffers and the render loop of
+// PathTracerBase.Render (SeeSharp/Integrators/PathTracer.cs:199-267).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -71,25 +71,44 @@ struct Timing {
 
 struct sspt_ctx {
     int device = 0;
+    int num_cus = 256;
     hipStream_t stream = nullptr;
     std::string error;
     Timing timing;
     // wavefront buffers, grown on demand
     Wave wave{};
     uint32_t wave_capacity = 0;
+    size_t shadow_capacity = 0;
     std::vector<void*> wave_allocs;
-    uint32_t* host_counters = nullptr;  // pinned
-    size_t shadow_capacity = 0;         // shadow queue entries allocated
+    WaveDesc* wave_descs = nullptr;  // device
+    size_t wave_descs_cap = 0;
+    // execution knobs (0 = default)
+    uint32_t want_capacity = 0;
+    uint32_t trace_blocks = 0, shade_blocks = 0;
+    bool use_graphs = true;
+    // adjoint buffers, grown on demand
+    std::vector<void*> adj_allocs;
+    uint64_t adj_records = 0;   // capacity * depth
+    size_t adj_shadow = 0;
 };
 
 struct sspt_scene {
     sspt_ctx* ctx = nullptr;
     DevScene dev{};
+    std::vector<SceneRec> recs;  // host copy
+    SceneRec* d_recs = nullptr;
+    uint64_t* d_path_offsets = nullptr;
     std::vector<void*> allocs;
-    float* image = nullptr;  // W*H*3 running mean
+    float* image = nullptr;  // pooled framebuffers
+    uint64_t image_floats = 0;
     uint64_t num_tris = 0, num_emitters = 0, num_nodes = 0, bytes = 0;
     uint64_t nan_count = 0;
-    int bvh_depth = 0;
+    int stack_need = 0;
+    // adjoint: d loss / d texel (texel-pool layout) and d loss / d image (framebuffer layout)
+    std::vector<DevTexture> textures;  // host copy of the texture table
+    uint64_t texel_floats = 0;
+    float* grad = nullptr;
+    float* adj_image = nullptr;
 };
 
 namespace {
@@ -104,12 +123,18 @@ sspt_status fail(sspt_ctx* ctx, sspt_status st, const char* fmt, ...) {
     return st;
 }
 
-#define HIP_TRY(ctx, expr)                                                                       \
-    do {                                                                                         \
-        hipError_t _e = (expr);                                                                  \
-        if (_e != hipSuccess)                                                                    \
+#define HIP_TRY(ctx, expr)                                                                            \
+    do {                                                                                              \
+        hipError_t _e = (expr);                                                                       \
+        if (_e != hipSuccess)                                                                         \
             return fail((ctx), _e == hipErrorOutOfMemory ? SSPT_ERR_OUT_OF_MEMORY : SSPT_ERR_DEVICE, \
-                        "%s failed: %s", #expr, hipGetErrorString(_e));                          \
+                        "%s failed: %s", #expr, hipGetErrorString(_e));                               \
+    } while (0)
+
+#define TRY(expr)                     \
+    do {                              \
+        sspt_status _s = (expr);      \
+        if (_s != SSPT_OK) return _s; \
     } while (0)
 
 template <class T>
@@ -126,12 +151,6 @@ sspt_status upload(sspt_ctx* ctx, sspt_scene* sc, const T* host, size_t count, c
     *out = static_cast<const T*>(p);
     return SSPT_OK;
 }
-
-#define TRY(expr)                        \
-    do {                                 \
-        sspt_status _s = (expr);         \
-        if (_s != SSPT_OK) return _s;    \
-    } while (0)
 
 // General 4x4 inverse by cofactors in double, rounded to float (the rule the
 // test oracle follows too). Returns false for a singular matrix.
@@ -203,120 +222,28 @@ bool setup_camera(const sspt_camera_desc& cd, int w, int h, DevCamera* cam) {
     return true;
 }
 
-sspt_status ensure_wave(sspt_ctx* ctx, uint32_t capacity) {
-    if (ctx->wave_capacity >= capacity) return SSPT_OK;
-    for (void* p : ctx->wave_allocs) hipFree(p);
-    ctx->wave_allocs.clear();
-    ctx->wave_capacity = 0;
-    ctx->shadow_capacity = 0;
-    ctx->wave.s_o = ctx->wave.s_d = ctx->wave.s_c = nullptr;
-    auto alloc = [&](void** p, size_t bytes) -> sspt_status {
-        HIP_TRY(ctx, hipMalloc(p, bytes));
-        ctx->wave_allocs.push_back(*p);
-        return SSPT_OK;
-    };
-    Wave& w = ctx->wave;
-    size_t P = capacity;
-    // every queue holds at most one entry per slot; the shadow queue two per
-    // slot and shadow ray (environment + area light)
-    for (int b = 0; b < 2; ++b) {
-        TRY(alloc((void**)&w.q_o[b], P * 16));
-        TRY(alloc((void**)&w.q_d[b], P * 16));
-        TRY(alloc((void**)&w.q_slot[b], P * 4));
-    }
-    TRY(alloc((void**)&w.hits, P * 16));
-    TRY(alloc((void**)&w.st_rng, P * 4));
-    TRY(alloc((void**)&w.st_thr, P * 16));
-    TRY(alloc((void**)&w.st_prev, P * 16));
-    TRY(alloc((void**)&w.st_L, P * 16));
-    TRY(alloc((void**)&w.st_Le, P * 16));
-    TRY(alloc((void**)&w.st_La, P * 16));
-    TRY(alloc((void**)&w.counters, 64));
-    TRY(alloc((void**)&w.stats, 64));
-    w.capacity = capacity;
-    w.s_o = w.s_d = w.s_c = nullptr;
-    ctx->wave_capacity = capacity;
-    return SSPT_OK;
-}
 
-sspt_status ensure_shadow(sspt_ctx* ctx, size_t entries, size_t* have) {
-    if (*have >= entries) return SSPT_OK;
-    Wave& w = ctx->wave;
-    for (void* p : {(void*)w.s_o, (void*)w.s_d, (void*)w.s_c}) {
-        if (!p) continue;
-        hipFree(p);
-        ctx->wave_allocs.erase(std::remove(ctx->wave_allocs.begin(), ctx->wave_allocs.end(), p),
-                               ctx->wave_allocs.end());
-    }
-    for (float4** p : {&w.s_o, &w.s_d, &w.s_c}) {
-        HIP_TRY(ctx, hipMalloc((void**)p, entries * 16));
-        ctx->wave_allocs.push_back(*p);
-    }
-    *have = entries;
-    return SSPT_OK;
-}
+// Host-side pools of a world under construction.
+struct WorldBuilder {
+    std::vector<float> nodes;   // 32 floats per node
+    std::vector<float4> tri_pos, tri_aux;
+    std::vector<int> tri_emitter;
+    std::vector<DevMaterial> materials;
+    std::vector<DevTexture> textures;
+    std::vector<float> texels;
+    std::vector<DevEmitter> emitters;
+    std::vector<float4> env_rgb;
+    std::vector<float> env_cdf;
+    std::vector<SceneRec> recs;
+    uint64_t fb_floats = 0;
+    int stack_need = 0;
+};
 
-template <class F>
-void timed(sspt_ctx* ctx, const char* name, F&& launch) {
-    if (!ctx->timing.enabled) {
-        launch();
-        return;
-    }
-    hipEvent_t a = ctx->timing.get(), b = ctx->timing.get();
-    hipEventRecord(a, ctx->stream);
-    launch();
-    hipEventRecord(b, ctx->stream);
-    ctx->timing.pending.push_back({name, a, b});
-}
-
-}  // namespace
-
-extern "C" {
-
-int32_t sspt_abi_version(void) { return SSPT_ABI_VERSION; }
-
-sspt_status sspt_ctx_create(int32_t device, sspt_ctx** out) {
-    if (!out) return SSPT_ERR_INVALID_ARGUMENT;
-    *out = nullptr;
-    int n = 0;
-    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return SSPT_ERR_DEVICE;
-    if (device < 0 || device >= n) return SSPT_ERR_INVALID_ARGUMENT;
-    if (hipSetDevice(device) != hipSuccess) return SSPT_ERR_DEVICE;
-    sspt_ctx* ctx = new (std::nothrow) sspt_ctx();
-    if (!ctx) return SSPT_ERR_OUT_OF_MEMORY;
-    ctx->device = device;
-    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
-        delete ctx;
-        return SSPT_ERR_DEVICE;
-    }
-    if (hipHostMalloc((void**)&ctx->host_counters, 64, hipHostMallocDefault) != hipSuccess) {
-        hipStreamDestroy(ctx->stream);
-        delete ctx;
-        return SSPT_ERR_DEVICE;
-    }
-    *out = ctx;
-    return SSPT_OK;
-}
-
-void sspt_ctx_destroy(sspt_ctx* ctx) {
-    if (!ctx) return;
-    hipSetDevice(ctx->device);
-    hipStreamSynchronize(ctx->stream);
-    ctx->timing.collect();
-    for (void* p : ctx->wave_allocs) hipFree(p);
-    if (ctx->host_counters) hipHostFree(ctx->host_counters);
-    hipStreamDestroy(ctx->stream);
-    delete ctx;
-}
-
-const char* sspt_last_error(const sspt_ctx* ctx) { return ctx ? ctx->error.c_str() : "null context"; }
-
-sspt_status sspt_scene_create(sspt_ctx* ctx, const sspt_scene_desc* d, sspt_scene** out) {
-    if (!ctx || !d || !out) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "null argument");
-    *out = nullptr;
-    hipSetDevice(ctx->device);
-    // Scene.IsValid (SeeSharp/Scene.cs:170-197)
+// Scene.IsValid (SeeSharp/Scene.cs:170-197) and structural checks of a description.
+sspt_status validate(sspt_ctx* ctx, const sspt_scene_desc* d) {
     if (d->width == 0 || d->height == 0) return fail(ctx, SSPT_ERR_INVALID_SCENE, "Framebuffer not set.");
+    if ((uint64_t)d->width * d->height > kSlotMask)
+        return fail(ctx, SSPT_ERR_UNSUPPORTED, "framebuffer of %ux%u is too large", d->width, d->height);
     if (d->num_meshes == 0 || !d->meshes) return fail(ctx, SSPT_ERR_INVALID_SCENE, "No meshes in the scene.");
     for (uint32_t i = 0; i < d->num_meshes; ++i) {
         const sspt_mesh_desc& m = d->meshes[i];
@@ -339,23 +266,40 @@ sspt_status sspt_scene_create(sspt_ctx* ctx, const sspt_scene_desc* d, sspt_scen
             if (t >= (int)d->num_textures) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "material %u: bad texture", i);
         if (m.base_color_texture >= 0 && d->textures[m.base_color_texture].channels != 3)
             return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "material %u: base color texture must be RGB", i);
-        if (m.roughness_texture >= 0 && d->textures[m.roughness_texture].channels != 1)
+        if (m.kind == SSPT_MATERIAL_GENERIC && m.roughness_texture >= 0 &&
+            d->textures[m.roughness_texture].channels != 1)
             return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "material %u: roughness texture must be mono", i);
     }
+    for (uint32_t i = 0; i < d->num_textures; ++i) {
+        const sspt_texture_desc& t = d->textures[i];
+        if (!t.texels || t.width == 0 || t.height == 0 || (t.channels != 1 && t.channels != 3))
+            return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "texture %u is malformed", i);
+    }
+    if (d->background && (d->background->width == 0 || d->background->height == 0 || !d->background->rgb))
+        return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "environment map is malformed");
+    return SSPT_OK;
+}
 
-    std::unique_ptr<sspt_scene> sc(new sspt_scene());
-    sc->ctx = ctx;
+// Scene.Prepare for one scene appended to the world: BVH build (TinyEmbree
+// CommitScene, Scene.cs:108-113), emitter map (:147-163), environment grid
+// (EnvironmentMap.cs:139-156), camera (PerspectiveCamera.UpdateResolution).
+sspt_status add_scene(sspt_ctx* ctx, WorldBuilder& W, const sspt_scene_desc* d) {
+    TRY(validate(ctx, d));
+    const uint32_t tri_base = (uint32_t)(W.tri_pos.size() / 3);
+    const int node_base = (int)(W.nodes.size() / 32);
+    const int mat_base = (int)W.materials.size();
+    const int tex_base = (int)W.textures.size();
+    const int em_base = (int)W.emitters.size();
 
-    // flatten triangles in global order (mesh offset + PrimId)
     std::vector<uint32_t> mesh_offset(d->num_meshes);
     uint64_t ntri = 0;
     for (uint32_t i = 0; i < d->num_meshes; ++i) {
         mesh_offset[i] = (uint32_t)ntri;
         ntri += d->meshes[i].num_triangles;
     }
-    if (ntri >= (1u << 30)) return fail(ctx, SSPT_ERR_UNSUPPORTED, "too many triangles");
+    if (ntri + tri_base >= (1u << 30)) return fail(ctx, SSPT_ERR_UNSUPPORTED, "too many triangles");
     std::vector<float> verts(ntri * 9);
-    std::vector<uint32_t> tri_mesh(ntri);
+    std::vector<uint32_t> tri_mesh(ntri), local_face(ntri);
     for (uint32_t i = 0, t = 0; i < d->num_meshes; ++i) {
         const sspt_mesh_desc& m = d->meshes[i];
         for (uint32_t f = 0; f < m.num_triangles; ++f, ++t) {
@@ -364,10 +308,10 @@ sspt_status sspt_scene_create(sspt_ctx* ctx, const sspt_scene_desc* d, sspt_scen
                 for (int c = 0; c < 3; ++c) verts[(size_t)t * 9 + k * 3 + c] = m.positions[(size_t)v * 3 + c];
             }
             tri_mesh[t] = i;
+            local_face[t] = f;
         }
     }
 
-    // emitters: Scene.Prepare's mesh -> face -> emitter map (Scene.cs:147-163)
     std::vector<int> tri_emitter_global(ntri, -1);
     std::vector<DevEmitter> emitters(d->num_emitters);
     for (uint32_t i = 0; i < d->num_emitters; ++i) {
@@ -384,36 +328,46 @@ sspt_status sspt_scene_create(sspt_ctx* ctx, const sspt_scene_desc* d, sspt_scen
         float e2[3] = {v[6] - v[0], v[7] - v[1], v[8] - v[2]};
         float n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
         float area = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]) * 0.5f;
-        if (area == 0.0f)
-            return fail(ctx, SSPT_ERR_ZERO_AREA_EMITTER, "Zero-area triangles cannot be emitters");
+        if (area == 0.0f) return fail(ctx, SSPT_ERR_ZERO_AREA_EMITTER, "Zero-area triangles cannot be emitters");
         emitters[i] = DevEmitter{(int)t, e.radiance[0], e.radiance[1], e.radiance[2], 1.0f / area, {0, 0, 0}};
     }
 
-    // BVH over all triangles
     BvhOutput bvh;
     unsigned hw = std::max(1u, std::thread::hardware_concurrency());
     build_bvh(BvhInput{verts.data(), (int)ntri}, 4, (int)std::min(hw, 16u), &bvh);
-    sc->bvh_depth = bvh.depth;
+    if (bvh.stack_need > kStackTotal)
+        return fail(ctx, SSPT_ERR_UNSUPPORTED, "BVH needs a traversal stack of %d entries (limit %d)", bvh.stack_need,
+                    kStackTotal);
+    W.stack_need = std::max(W.stack_need, bvh.stack_need);
     std::vector<int> global_to_leaf(ntri);
     for (uint64_t li = 0; li < ntri; ++li) global_to_leaf[bvh.leaf_order[li]] = (int)li;
-    for (auto& e : emitters) e.tri = global_to_leaf[e.tri];
+    for (auto& e : emitters) e.tri = (int)tri_base + global_to_leaf[e.tri];
+
+    // nodes: shift inner-child indices by the node base and leaf ranges by the triangle base
+    size_t nn = bvh.nodes.size() / 32;
+    for (size_t n = 0; n < nn; ++n) {
+        float* node = &bvh.nodes[n * 32];
+        int* idx = reinterpret_cast<int*>(node + 24);
+        int* cnt = reinterpret_cast<int*>(node + 28);
+        for (int k = 0; k < 4; ++k) {
+            if (cnt[k] > 0) idx[k] += (int)tri_base;
+            else if (cnt[k] == 0) idx[k] += node_base;
+        }
+    }
+    W.nodes.insert(W.nodes.end(), bvh.nodes.begin(), bvh.nodes.end());
 
     // leaf-order triangle data
-    std::vector<float4> tri_pos(ntri * 3), tri_aux(ntri * 4);
-    std::vector<int> tri_emitter(ntri);
-    std::vector<uint32_t> local_face(ntri);
-    for (uint32_t i = 0, t = 0; i < d->num_meshes; ++i)
-        for (uint32_t f = 0; f < d->meshes[i].num_triangles; ++f, ++t) local_face[t] = f;
     for (uint64_t li = 0; li < ntri; ++li) {
         int g = bvh.leaf_order[li];
         const sspt_mesh_desc& m = d->meshes[tri_mesh[g]];
         uint32_t f = local_face[g];
         const float* v = &verts[(size_t)g * 9];
+        int wid = (int)tri_base + g;
         float gbits;
-        std::memcpy(&gbits, &g, 4);
-        tri_pos[li * 3 + 0] = make_float4(v[0], v[1], v[2], gbits);
-        tri_pos[li * 3 + 1] = make_float4(v[3], v[4], v[5], 0.0f);
-        tri_pos[li * 3 + 2] = make_float4(v[6], v[7], v[8], 0.0f);
+        std::memcpy(&gbits, &wid, 4);
+        W.tri_pos.push_back(make_float4(v[0], v[1], v[2], gbits));
+        W.tri_pos.push_back(make_float4(v[3], v[4], v[5], 0.0f));
+        W.tri_pos.push_back(make_float4(v[6], v[7], v[8], 0.0f));
         int idx[3] = {m.indices[f * 3], m.indices[f * 3 + 1], m.indices[f * 3 + 2]};
         float n[3][3] = {{0}}, uv[3][2] = {{0}};
         if (m.normals)
@@ -422,60 +376,69 @@ sspt_status sspt_scene_create(sspt_ctx* ctx, const sspt_scene_desc* d, sspt_scen
         if (m.uvs)
             for (int k = 0; k < 3; ++k)
                 for (int c = 0; c < 2; ++c) uv[k][c] = m.uvs[(size_t)idx[k] * 2 + c];
-        int info = (m.material & kInfoMaterialMask) | (m.normals ? kInfoHasNormals : 0) | (m.uvs ? kInfoHasUvs : 0);
+        int info = ((mat_base + m.material) & kInfoMaterialMask) | (m.normals ? kInfoHasNormals : 0) |
+                   (m.uvs ? kInfoHasUvs : 0);
         float ibits;
         std::memcpy(&ibits, &info, 4);
-        tri_aux[li * 4 + 0] = make_float4(n[0][0], n[0][1], n[0][2], uv[0][0]);
-        tri_aux[li * 4 + 1] = make_float4(n[1][0], n[1][1], n[1][2], uv[0][1]);
-        tri_aux[li * 4 + 2] = make_float4(n[2][0], n[2][1], n[2][2], uv[1][0]);
-        tri_aux[li * 4 + 3] = make_float4(uv[1][1], uv[2][0], uv[2][1], ibits);
-        tri_emitter[li] = tri_emitter_global[g];
+        W.tri_aux.push_back(make_float4(n[0][0], n[0][1], n[0][2], uv[0][0]));
+        W.tri_aux.push_back(make_float4(n[1][0], n[1][1], n[1][2], uv[0][1]));
+        W.tri_aux.push_back(make_float4(n[2][0], n[2][1], n[2][2], uv[1][0]));
+        W.tri_aux.push_back(make_float4(uv[1][1], uv[2][0], uv[2][1], ibits));
+        W.tri_emitter.push_back(tri_emitter_global[g]);
     }
 
-    // materials and textures
-    std::vector<DevMaterial> mats(d->num_materials);
     for (uint32_t i = 0; i < d->num_materials; ++i) {
         const sspt_material_desc& m = d->materials[i];
-        DevMaterial& o = mats[i];
+        DevMaterial o;
         std::memset(&o, 0, sizeof(o));
         o.kind = m.kind;
         for (int c = 0; c < 3; ++c) o.base[c] = m.base_color[c];
-        o.base_tex = m.base_color_texture;
+        o.base_tex = m.base_color_texture >= 0 ? tex_base + m.base_color_texture : -1;
         o.roughness = m.roughness;
-        o.rough_tex = m.kind == SSPT_MATERIAL_GENERIC ? m.roughness_texture : -1;
+        o.rough_tex = (m.kind == SSPT_MATERIAL_GENERIC && m.roughness_texture >= 0) ? tex_base + m.roughness_texture : -1;
         o.metallic = m.metallic;
         o.spec_tint = m.specular_tint_strength;
         o.anisotropic = m.anisotropic;
         o.spec_trans = m.specular_transmittance;
         o.ior = m.index_of_refraction;
         o.transmitter = m.transmitter;
+        W.materials.push_back(o);
     }
-    std::vector<DevTexture> texs(d->num_textures);
-    std::vector<float> texels;
     for (uint32_t i = 0; i < d->num_textures; ++i) {
         const sspt_texture_desc& t = d->textures[i];
-        if (!t.texels || t.width == 0 || t.height == 0 || (t.channels != 1 && t.channels != 3))
-            return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "texture %u is malformed", i);
-        texs[i] = DevTexture{texels.size(), (int)t.width, (int)t.height, (int)t.channels, t.border};
-        texels.insert(texels.end(), t.texels, t.texels + (size_t)t.width * t.height * t.channels);
+        W.textures.push_back(DevTexture{W.texels.size(), (int)t.width, (int)t.height, (int)t.channels, t.border});
+        W.texels.insert(W.texels.end(), t.texels, t.texels + (size_t)t.width * t.height * t.channels);
     }
+    W.emitters.insert(W.emitters.end(), emitters.begin(), emitters.end());
 
-    // environment map: EnvironmentMap.BuildSamplingGrid (EnvironmentMap.cs:139-156),
-    // RegularGrid2d.Normalize + PiecewiseConstantPDF ctor (RegularGrid2d.cs:56-71, PiecewiseConstant.cs:12-31)
-    std::vector<float4> env_rgb;
-    std::vector<float> row_cdf, col_cdf;
-    int ew = 0, eh = 0;
+    SceneRec R;
+    std::memset(&R, 0, sizeof(R));
+    if (!setup_camera(d->camera, (int)d->width, (int)d->height, &R.cam))
+        return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "World to camera transform must be invertible.");
+    R.root = node_base;
+    R.emitter_off = em_base;
+    R.num_emitters = (int)d->num_emitters;
+    float abs_max = 0.0f;
+    for (float v : verts) abs_max = std::max(abs_max, std::fabs(v));
+    R.abs_max = abs_max;
+    R.num_pixels = d->width * d->height;
+    R.fb_off = W.fb_floats;
+    W.fb_floats += (uint64_t)R.num_pixels * 3;
+
+    // EnvironmentMap.BuildSamplingGrid (EnvironmentMap.cs:139-156): luminance grid,
+    // RegularGrid2d.Normalize + PiecewiseConstantPDF ctor (RegularGrid2d.cs:56-71,
+    // PiecewiseConstant.cs:12-31)
     if (d->background) {
-        ew = (int)d->background->width;
-        eh = (int)d->background->height;
-        if (ew <= 0 || eh <= 0 || !d->background->rgb)
-            return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "environment map is malformed");
-        env_rgb.resize((size_t)ew * eh);
+        int ew = (int)d->background->width, eh = (int)d->background->height;
+        R.has_env = 1;
+        R.env_w = ew;
+        R.env_h = eh;
+        R.env_rgb_off = (uint32_t)W.env_rgb.size();
         std::vector<float> density((size_t)ew * eh, 0.0f), marg(eh, 0.0f);
         for (int r = 0; r < eh; ++r)
             for (int c = 0; c < ew; ++c) {
                 const float* p = d->background->rgb + ((size_t)r * ew + c) * 3;
-                env_rgb[(size_t)r * ew + c] = make_float4(p[0], p[1], p[2], 0.0f);
+                W.env_rgb.push_back(make_float4(p[0], p[1], p[2], 0.0f));
                 float lum = 0.2126f * p[0] + 0.7152f * p[1] + 0.0722f * p[2];
                 density[(size_t)r * ew + c] += lum;
                 marg[r] += lum;
@@ -487,68 +450,384 @@ sspt_status sspt_scene_create(sspt_ctx* ctx, const sspt_scene_desc* d, sspt_scen
             for (int i = 0; i < n && total > 0.0f; ++i) cdf[i] /= total;
             cdf[n - 1] = 1.0f;
         };
-        row_cdf.resize(eh);
-        build_cdf(marg.data(), eh, row_cdf.data());
-        col_cdf.resize((size_t)ew * eh);
+        R.env_row_off = (uint32_t)W.env_cdf.size();
+        W.env_cdf.resize(W.env_cdf.size() + eh);
+        build_cdf(marg.data(), eh, &W.env_cdf[R.env_row_off]);
+        R.env_col_off = (uint32_t)W.env_cdf.size();
+        W.env_cdf.resize(W.env_cdf.size() + (size_t)ew * eh);
         for (int r = 0; r < eh; ++r) {
+            float* cdf = &W.env_cdf[R.env_col_off + (size_t)r * ew];
             if (marg[r] == 0.0f) {
-                // zero-probability row, never selected: keep a uniform cdf so sampling stays finite
-                for (int c = 0; c < ew; ++c) col_cdf[(size_t)r * ew + c] = (float)(c + 1) / (float)ew;
+                // zero-probability row, never selected: a uniform cdf keeps sampling finite
+                for (int c = 0; c < ew; ++c) cdf[c] = (float)(c + 1) / (float)ew;
                 continue;
             }
-            build_cdf(&density[(size_t)r * ew], ew, &col_cdf[(size_t)r * ew]);
+            build_cdf(&density[(size_t)r * ew], ew, cdf);
         }
     }
+    W.recs.push_back(R);
+    return SSPT_OK;
+}
 
+sspt_status create_world(sspt_ctx* ctx, const sspt_scene_desc* descs, uint32_t n, sspt_scene** out) {
+    if (!ctx || !descs || !out) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (n == 0 || n > (uint32_t)kMaxScenes)
+        return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "a batch holds 1..%d scenes", kMaxScenes);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    WorldBuilder W;
+    for (uint32_t i = 0; i < n; ++i) TRY(add_scene(ctx, W, &descs[i]));
+    if (W.materials.size() > (size_t)kInfoMaterialMask)
+        return fail(ctx, SSPT_ERR_UNSUPPORTED, "too many materials");
+
+    std::unique_ptr<sspt_scene> sc(new sspt_scene());
+    sc->ctx = ctx;
     DevScene& S = sc->dev;
-    const float4* p4;
-    TRY(upload(ctx, sc.get(), reinterpret_cast<const float4*>(bvh.nodes.data()), bvh.nodes.size() / 4, &p4));
-    S.nodes = p4;
-    TRY(upload(ctx, sc.get(), tri_pos.data(), tri_pos.size(), &S.tri_pos));
-    TRY(upload(ctx, sc.get(), tri_aux.data(), tri_aux.size(), &S.tri_aux));
-    TRY(upload(ctx, sc.get(), tri_emitter.data(), tri_emitter.size(), &S.tri_emitter));
-    TRY(upload(ctx, sc.get(), mats.data(), mats.size(), &S.materials));
-    TRY(upload(ctx, sc.get(), texs.data(), texs.size(), &S.textures));
-    TRY(upload(ctx, sc.get(), texels.data(), texels.size(), &S.texels));
-    TRY(upload(ctx, sc.get(), emitters.data(), emitters.size(), &S.emitters));
-    S.num_emitters = (int)emitters.size();
-    S.num_tris = (int)ntri;
-    float abs_max = 0.0f;
-    for (float v : verts) abs_max = std::max(abs_max, std::fabs(v));
-    S.abs_max = abs_max;
-    S.has_env = d->background ? 1 : 0;
-    S.env_w = ew;
-    S.env_h = eh;
-    TRY(upload(ctx, sc.get(), env_rgb.data(), env_rgb.size(), &S.env_rgb));
-    TRY(upload(ctx, sc.get(), row_cdf.data(), row_cdf.size(), &S.env_row_cdf));
-    TRY(upload(ctx, sc.get(), col_cdf.data(), col_cdf.size(), &S.env_col_cdf));
-    if (!setup_camera(d->camera, (int)d->width, (int)d->height, &S.cam))
-        return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "World to camera transform must be invertible.");
-    S.width = (int)d->width;
-    S.height = (int)d->height;
-
-    size_t img_bytes = (size_t)d->width * d->height * 3 * sizeof(float);
-    HIP_TRY(ctx, hipMalloc((void**)&sc->image, img_bytes));
+    TRY(upload(ctx, sc.get(), reinterpret_cast<const float4*>(W.nodes.data()), W.nodes.size() / 4, &S.nodes));
+    TRY(upload(ctx, sc.get(), W.tri_pos.data(), W.tri_pos.size(), &S.tri_pos));
+    TRY(upload(ctx, sc.get(), W.tri_aux.data(), W.tri_aux.size(), &S.tri_aux));
+    TRY(upload(ctx, sc.get(), W.tri_emitter.data(), W.tri_emitter.size(), &S.tri_emitter));
+    TRY(upload(ctx, sc.get(), W.materials.data(), W.materials.size(), &S.materials));
+    TRY(upload(ctx, sc.get(), W.textures.data(), W.textures.size(), &S.textures));
+    TRY(upload(ctx, sc.get(), W.texels.data(), W.texels.size(), &S.texels));
+    TRY(upload(ctx, sc.get(), W.emitters.data(), W.emitters.size(), &S.emitters));
+    TRY(upload(ctx, sc.get(), W.env_rgb.data(), W.env_rgb.size(), &S.env_rgb));
+    TRY(upload(ctx, sc.get(), W.env_cdf.data(), W.env_cdf.size(), &S.env_cdf));
+    HIP_TRY(ctx, hipMalloc((void**)&sc->d_recs, W.recs.size() * sizeof(SceneRec)));
+    sc->allocs.push_back(sc->d_recs);
+    HIP_TRY(ctx, hipMalloc((void**)&sc->d_path_offsets, (W.recs.size() + 1) * sizeof(uint64_t)));
+    sc->allocs.push_back(sc->d_path_offsets);
+    HIP_TRY(ctx, hipMemcpy(sc->d_recs, W.recs.data(), W.recs.size() * sizeof(SceneRec), hipMemcpyHostToDevice));
+    S.recs = sc->d_recs;
+    S.path_offsets = sc->d_path_offsets;
+    S.num_scenes = (int)W.recs.size();
+    S.num_nodes = (int)(W.nodes.size() / 32);
+    sc->recs = W.recs;
+    sc->image_floats = W.fb_floats;
+    HIP_TRY(ctx, hipMalloc((void**)&sc->image, W.fb_floats * sizeof(float)));
     sc->allocs.push_back(sc->image);
-    sc->bytes += img_bytes;
-    HIP_TRY(ctx, hipMemset(sc->image, 0, img_bytes));
-    sc->num_tris = ntri;
-    sc->num_emitters = emitters.size();
-    sc->num_nodes = bvh.nodes.size() / 16;
+    sc->bytes += W.fb_floats * sizeof(float);
+    HIP_TRY(ctx, hipMemset(sc->image, 0, W.fb_floats * sizeof(float)));
+    sc->textures = W.textures;
+    sc->texel_floats = W.texels.size();
+    if (sc->texel_floats) {
+        HIP_TRY(ctx, hipMalloc((void**)&sc->grad, sc->texel_floats * sizeof(float)));
+        sc->allocs.push_back(sc->grad);
+        HIP_TRY(ctx, hipMemset(sc->grad, 0, sc->texel_floats * sizeof(float)));
+    }
+    HIP_TRY(ctx, hipMalloc((void**)&sc->adj_image, W.fb_floats * sizeof(float)));
+    sc->allocs.push_back(sc->adj_image);
+    S.grad = sc->grad;
+    S.adj_image = sc->adj_image;
+    sc->num_tris = W.tri_pos.size() / 3;
+    sc->num_emitters = W.emitters.size();
+    sc->num_nodes = W.nodes.size() / 32;
+    sc->stack_need = W.stack_need;
     *out = sc.release();
     return SSPT_OK;
 }
 
+sspt_status ensure_wave(sspt_ctx* ctx, uint32_t capacity) {
+    if (ctx->wave_capacity >= capacity) return SSPT_OK;
+    for (void* p : ctx->wave_allocs) (void)hipFree(p);
+    ctx->wave_allocs.clear();
+    for (void* p : ctx->adj_allocs) (void)hipFree(p);
+    ctx->adj_allocs.clear();
+    ctx->adj_records = 0;
+    ctx->adj_shadow = 0;
+    ctx->wave.ad_dlog = nullptr;
+    ctx->wave.ad_A = nullptr;
+    ctx->wave.ad_tex = nullptr;
+    ctx->wave.s_g = nullptr;
+    ctx->wave.s_t = nullptr;
+    ctx->wave.ad_depth = 0;
+    ctx->wave_capacity = 0;
+    ctx->shadow_capacity = 0;
+    auto alloc = [&](void** p, size_t bytes) -> sspt_status {
+        HIP_TRY(ctx, hipMalloc(p, bytes));
+        ctx->wave_allocs.push_back(*p);
+        return SSPT_OK;
+    };
+    Wave& w = ctx->wave;
+    size_t P = capacity;
+    for (int b = 0; b < 2; ++b) {
+        TRY(alloc((void**)&w.q_o[b], P * 16));
+        TRY(alloc((void**)&w.q_d[b], P * 16));
+        TRY(alloc((void**)&w.q_slot[b], P * 4));
+    }
+    TRY(alloc((void**)&w.hits, P * 16));
+    TRY(alloc((void**)&w.st_rng, P * 4));
+    TRY(alloc((void**)&w.st_pix, P * 4));
+    TRY(alloc((void**)&w.st_thr, P * 16));
+    TRY(alloc((void**)&w.st_prev, P * 16));
+    TRY(alloc((void**)&w.st_L, P * 16));
+    TRY(alloc((void**)&w.st_Le, P * 16));
+    TRY(alloc((void**)&w.st_La, P * 16));
+    TRY(alloc((void**)&w.counters, 64));
+    TRY(alloc((void**)&w.stats, 64));
+    w.capacity = capacity;
+    w.s_o = w.s_d = w.s_c = nullptr;
+    ctx->wave_capacity = capacity;
+    return SSPT_OK;
+}
+
+sspt_status ensure_shadow(sspt_ctx* ctx, size_t entries) {
+    if (ctx->shadow_capacity >= entries) return SSPT_OK;
+    Wave& w = ctx->wave;
+    for (void* p : {(void*)w.s_o, (void*)w.s_d, (void*)w.s_c}) {
+        if (!p) continue;
+        (void)hipFree(p);
+        ctx->wave_allocs.erase(std::remove(ctx->wave_allocs.begin(), ctx->wave_allocs.end(), p),
+                               ctx->wave_allocs.end());
+    }
+    w.s_o = w.s_d = w.s_c = nullptr;
+    ctx->shadow_capacity = 0;
+    for (float4** p : {&w.s_o, &w.s_d, &w.s_c}) {
+        HIP_TRY(ctx, hipMalloc((void**)p, entries * 16));
+        ctx->wave_allocs.push_back(*p);
+    }
+    ctx->shadow_capacity = entries;
+    return SSPT_OK;
+}
+
+sspt_status ensure_wave_descs(sspt_ctx* ctx, size_t n) {
+    if (ctx->wave_descs_cap >= n) return SSPT_OK;
+    if (ctx->wave_descs) (void)hipFree(ctx->wave_descs);
+    ctx->wave_descs = nullptr;
+    ctx->wave_descs_cap = 0;
+    HIP_TRY(ctx, hipMalloc((void**)&ctx->wave_descs, n * sizeof(WaveDesc)));
+    ctx->wave_descs_cap = n;
+    return SSPT_OK;
+}
+
+// Per-vertex adjoint records (capacity x depth) and per-shadow-ray gradients.
+sspt_status ensure_adjoint(sspt_ctx* ctx, uint32_t depth) {
+    Wave& w = ctx->wave;
+    uint64_t records = (uint64_t)ctx->wave_capacity * depth;
+    if (ctx->adj_records < records || ctx->adj_shadow < ctx->shadow_capacity) {
+        for (void* p : ctx->adj_allocs) (void)hipFree(p);
+        ctx->adj_allocs.clear();
+        ctx->adj_records = 0;
+        ctx->adj_shadow = 0;
+        auto alloc = [&](void** p, size_t bytes) -> sspt_status {
+            HIP_TRY(ctx, hipMalloc(p, bytes));
+            ctx->adj_allocs.push_back(*p);
+            return SSPT_OK;
+        };
+        TRY(alloc((void**)&w.ad_dlog, records * 48));
+        TRY(alloc((void**)&w.ad_A, records * 16));
+        TRY(alloc((void**)&w.ad_tex, records * 8));
+        TRY(alloc((void**)&w.s_g, ctx->shadow_capacity * 16));
+        TRY(alloc((void**)&w.s_t, ctx->shadow_capacity * 8));
+        ctx->adj_records = records;
+        ctx->adj_shadow = ctx->shadow_capacity;
+    }
+    w.ad_depth = (uint32_t)(ctx->adj_records / std::max(1u, ctx->wave_capacity));
+    return SSPT_OK;
+}
+
+template <class F>
+void timed(sspt_ctx* ctx, const char* name, F&& launch) {
+    if (!ctx->timing.enabled) {
+        launch();
+        return;
+    }
+    hipEvent_t a = ctx->timing.get(), b = ctx->timing.get();
+    (void)hipEventRecord(a, ctx->stream);
+    launch();
+    (void)hipEventRecord(b, ctx->stream);
+    ctx->timing.pending.push_back({name, a, b});
+}
+
+LaunchConfig launch_config(const sspt_ctx* ctx) {
+    LaunchConfig lc;
+    lc.trace_blocks = ctx->trace_blocks ? ctx->trace_blocks : (uint32_t)ctx->num_cus * 16;
+    lc.shade_blocks = ctx->shade_blocks ? ctx->shade_blocks : (uint32_t)ctx->num_cus * 8;
+    return lc;
+}
+
+// Enqueues the kernels of one wave; the wave's work is read on the device from
+// the descriptor the wave counter selects.
+enum class Mode { Mean, Sum, Adjoint };
+
+void enqueue_wave(sspt_ctx* ctx, sspt_scene* sc, int max_depth, Mode mode, float* mean_image,
+                  uint32_t mean_pixels) {
+    const DevScene& S = sc->dev;
+    const Wave& w = ctx->wave;
+    const LaunchConfig lc = launch_config(ctx);
+    hipStream_t st = ctx->stream;
+    const bool adj = mode == Mode::Adjoint;
+    launch_wave_begin(st, w);
+    timed(ctx, "generate", [&] { launch_generate(st, S, w, lc); });
+    int buf = 0;
+    for (int depth = 1; depth <= max_depth; ++depth) {
+        launch_bounce_reset(st, w, buf);
+        timed(ctx, "extend", [&] { launch_extend(st, S, w, lc, buf); });
+        if (adj) {
+            timed(ctx, "shade_adjoint", [&] { launch_shade_adjoint(st, S, w, lc, buf); });
+            timed(ctx, "shadow_adjoint", [&] { launch_shadow_adjoint(st, S, w, lc); });
+        } else {
+            timed(ctx, "shade", [&] { launch_shade(st, S, w, lc, buf); });
+            timed(ctx, "shadow", [&] { launch_shadow(st, S, w, lc); });
+        }
+        buf ^= 1;
+    }
+    if (mode == Mode::Mean)
+        timed(ctx, "accumulate", [&] { launch_accumulate_mean(st, w, mean_image, mean_pixels); });
+    else if (mode == Mode::Sum)
+        timed(ctx, "accumulate", [&] { launch_accumulate_sum(st, S, w, lc, sc->image); });
+    else
+        timed(ctx, "adjoint_finalize", [&] { launch_adjoint_finalize(st, S, w, lc); });
+    launch_wave_end(st, w);
+}
+
+// Runs the given waves: either every wave enqueued directly (kernel timing on,
+// or graphs off), or one wave captured as a graph and replayed.
+sspt_status run_waves(sspt_ctx* ctx, sspt_scene* sc, const std::vector<WaveDesc>& waves, int max_depth, Mode mode,
+                      float* mean_image, uint32_t mean_pixels, int64_t budget_ms) {
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->wave_descs, waves.data(), waves.size() * sizeof(WaveDesc),
+                                hipMemcpyHostToDevice, ctx->stream));
+    // counters: queues 0, wave index 0 (a fill kernel, not a memset, so that it
+    // could itself live in a graph)
+    HIP_TRY(ctx, hipMemsetAsync(ctx->wave.counters, 0, 64, ctx->stream));
+    ctx->wave.waves = ctx->wave_descs;
+    const bool graphs = ctx->use_graphs && !ctx->timing.enabled && budget_ms < 0;
+    hipGraphExec_t exec = nullptr;
+    if (graphs) {
+        hipGraph_t graph = nullptr;
+        HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        enqueue_wave(ctx, sc, max_depth, mode, mean_image, mean_pixels);
+        HIP_TRY(ctx, hipStreamEndCapture(ctx->stream, &graph));
+        hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return fail(ctx, SSPT_ERR_DEVICE, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    sspt_status st = SSPT_OK;
+    for (size_t i = 0; i < waves.size(); ++i) {
+        if (budget_ms >= 0 && i > 0) {
+            // MaximumRenderTimeMs (PathTracer.cs:221-227), checked per wave
+            hipError_t e = hipStreamSynchronize(ctx->stream);
+            if (e != hipSuccess) {
+                st = fail(ctx, SSPT_ERR_DEVICE, "render failed: %s", hipGetErrorString(e));
+                break;
+            }
+            double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            if (ms + ms / (double)i > (double)budget_ms) break;
+        }
+        if (graphs) {
+            hipError_t e = hipGraphLaunch(exec, ctx->stream);
+            if (e != hipSuccess) {
+                st = fail(ctx, SSPT_ERR_DEVICE, "hipGraphLaunch failed: %s", hipGetErrorString(e));
+                break;
+            }
+        } else {
+            enqueue_wave(ctx, sc, max_depth, mode, mean_image, mean_pixels);
+        }
+        if (ctx->timing.enabled && (i % 4 == 3)) {
+            (void)hipStreamSynchronize(ctx->stream);
+            ctx->timing.collect();
+        }
+    }
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (st != SSPT_OK) return st;
+    if (e != hipSuccess) return fail(ctx, SSPT_ERR_DEVICE, "render failed: %s", hipGetErrorString(e));
+    HIP_TRY(ctx, hipGetLastError());
+    ctx->timing.collect();
+    return SSPT_OK;
+}
+
+sspt_status read_stats(sspt_ctx* ctx, sspt_scene* sc, sspt_ray_stats* stats) {
+    unsigned long long hs[kStatNum];
+    HIP_TRY(ctx, hipMemcpyAsync(hs, ctx->wave.stats, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    sc->nan_count += hs[kStatNan];
+    if (stats) {
+        stats->num_rays = hs[kStatRays];
+        stats->num_shadow_rays = hs[kStatShadow];
+        stats->num_occluded = hs[kStatOccluded];
+        stats->num_ray_hits = hs[kStatHits];
+        stats->num_shadow_rays_traced = hs[kStatShadowTraced];
+    }
+    return SSPT_OK;
+}
+
+bool valid_settings(const sspt_pt_settings& s) {
+    return s.max_depth >= 1 && s.num_shadow_rays >= 0 && s.min_depth >= 0 && s.total_spp >= 0;
+}
+
+void apply_settings(SceneRec& R, const sspt_pt_settings& s) {
+    R.base_seed = s.base_seed;
+    R.max_depth = s.max_depth;
+    R.min_depth = s.min_depth;
+    R.num_shadow_rays = s.num_shadow_rays;
+    R.enable_bsdf_di = s.enable_bsdf_di;
+}
+
+uint32_t wave_capacity(const sspt_ctx* ctx) {
+    uint32_t c = ctx->want_capacity ? ctx->want_capacity : (1u << 22);
+    return std::min(c, kMaxSlots);
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t sspt_abi_version(void) { return SSPT_ABI_VERSION; }
+
+sspt_status sspt_ctx_create(int32_t device, sspt_ctx** out) {
+    if (!out) return SSPT_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return SSPT_ERR_DEVICE;
+    if (device < 0 || device >= n) return SSPT_ERR_INVALID_ARGUMENT;
+    if (hipSetDevice(device) != hipSuccess) return SSPT_ERR_DEVICE;
+    sspt_ctx* ctx = new (std::nothrow) sspt_ctx();
+    if (!ctx) return SSPT_ERR_OUT_OF_MEMORY;
+    ctx->device = device;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+        ctx->num_cus = cus;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return SSPT_ERR_DEVICE;
+    }
+    *out = ctx;
+    return SSPT_OK;
+}
+
+void sspt_ctx_destroy(sspt_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->timing.collect();
+    for (void* p : ctx->wave_allocs) (void)hipFree(p);
+    for (void* p : ctx->adj_allocs) (void)hipFree(p);
+    if (ctx->wave_descs) (void)hipFree(ctx->wave_descs);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* sspt_last_error(const sspt_ctx* ctx) { return ctx ? ctx->error.c_str() : "null context"; }
+
+sspt_status sspt_scene_create(sspt_ctx* ctx, const sspt_scene_desc* d, sspt_scene** out) {
+    return create_world(ctx, d, 1, out);
+}
+
+sspt_status sspt_batch_create(sspt_ctx* ctx, const sspt_scene_desc* descs, uint32_t n, sspt_scene** out) {
+    return create_world(ctx, descs, n, out);
+}
+
 void sspt_scene_destroy(sspt_scene* sc) {
     if (!sc) return;
-    hipSetDevice(sc->ctx->device);
-    hipStreamSynchronize(sc->ctx->stream);
-    for (void* p : sc->allocs) hipFree(p);
+    (void)hipSetDevice(sc->ctx->device);
+    (void)hipStreamSynchronize(sc->ctx->stream);
+    for (void* p : sc->allocs) (void)hipFree(p);
     delete sc;
 }
 
-sspt_status sspt_scene_info(const sspt_scene* sc, uint64_t* ntri, uint64_t* nem, uint64_t* nnodes,
-                            uint64_t* bytes) {
+sspt_status sspt_scene_info(const sspt_scene* sc, uint64_t* ntri, uint64_t* nem, uint64_t* nnodes, uint64_t* bytes) {
     if (!sc) return SSPT_ERR_INVALID_ARGUMENT;
     if (ntri) *ntri = sc->num_tris;
     if (nem) *nem = sc->num_emitters;
@@ -557,22 +836,34 @@ sspt_status sspt_scene_info(const sspt_scene* sc, uint64_t* ntri, uint64_t* nem,
     return SSPT_OK;
 }
 
+sspt_status sspt_scene_count(const sspt_scene* sc, uint32_t* n) {
+    if (!sc || !n) return SSPT_ERR_INVALID_ARGUMENT;
+    *n = (uint32_t)sc->recs.size();
+    return SSPT_OK;
+}
+
 sspt_status sspt_framebuffer_clear(sspt_ctx* ctx, sspt_scene* sc) {
     if (!ctx || !sc) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "null argument");
-    hipSetDevice(ctx->device);
-    HIP_TRY(ctx, hipMemsetAsync(sc->image, 0, (size_t)sc->dev.width * sc->dev.height * 12, ctx->stream));
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipMemsetAsync(sc->image, 0, sc->image_floats * sizeof(float), ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     sc->nan_count = 0;
     return SSPT_OK;
 }
 
-sspt_status sspt_framebuffer_read(sspt_ctx* ctx, sspt_scene* sc, float* host) {
+sspt_status sspt_framebuffer_read_scene(sspt_ctx* ctx, sspt_scene* sc, uint32_t index, float* host) {
     if (!ctx || !sc || !host) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "null argument");
-    hipSetDevice(ctx->device);
-    HIP_TRY(ctx, hipMemcpyAsync(host, sc->image, (size_t)sc->dev.width * sc->dev.height * 12,
-                                hipMemcpyDeviceToHost, ctx->stream));
+    if (index >= sc->recs.size()) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "scene index out of range");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const SceneRec& R = sc->recs[index];
+    HIP_TRY(ctx, hipMemcpyAsync(host, sc->image + R.fb_off, (size_t)R.num_pixels * 12, hipMemcpyDeviceToHost,
+                                ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return SSPT_OK;
+}
+
+sspt_status sspt_framebuffer_read(sspt_ctx* ctx, sspt_scene* sc, float* host) {
+    return sspt_framebuffer_read_scene(ctx, sc, 0, host);
 }
 
 sspt_status sspt_framebuffer_nan_count(sspt_ctx* ctx, sspt_scene* sc, uint64_t* count) {
@@ -584,86 +875,173 @@ sspt_status sspt_framebuffer_nan_count(sspt_ctx* ctx, sspt_scene* sc, uint64_t* 
 sspt_status sspt_render(sspt_ctx* ctx, sspt_scene* sc, const sspt_pt_settings* cfg, uint32_t first_iteration,
                         uint32_t num_iterations, sspt_ray_stats* stats) {
     if (!ctx || !sc || !cfg) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "null argument");
-    if (cfg->max_depth < 1 || cfg->num_shadow_rays < 0 || cfg->min_depth < 0)
-        return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "invalid settings");
-    hipSetDevice(ctx->device);
-    const DevScene& S = sc->dev;
-    uint32_t np = (uint32_t)(S.width * S.height);
-    // wave: as many whole iterations as fit in ~4M path slots
-    const uint32_t kTargetSlots = 1u << 22;
-    uint32_t per_wave = std::max(1u, std::min(num_iterations, kTargetSlots / std::max(1u, np)));
-    if ((uint64_t)np * per_wave >= (1ull << 30)) per_wave = 1;
-    TRY(ensure_wave(ctx, np * per_wave));
-    size_t shadow_need = (size_t)np * per_wave * 2 * std::max(1, cfg->num_shadow_rays);
-    TRY(ensure_shadow(ctx, shadow_need, &ctx->shadow_capacity));
-    Wave& w = ctx->wave;
-    HIP_TRY(ctx, hipMemsetAsync(w.stats, 0, 64, ctx->stream));
-    DevSettings ds{cfg->base_seed, cfg->max_depth, cfg->min_depth, cfg->num_shadow_rays, cfg->enable_bsdf_di};
+    if (!valid_settings(*cfg)) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "invalid settings");
+    if (sc->recs.size() != 1) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "sspt_render renders a single scene");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    SceneRec& R = sc->recs[0];
+    const uint32_t np = R.num_pixels;
+    // waves of whole iterations (running mean), as many as fit the wave capacity
+    uint32_t per_wave = std::max(1u, std::min(num_iterations, wave_capacity(ctx) / std::max(1u, np)));
+    uint32_t cap = np * per_wave;
+    TRY(ensure_wave(ctx, cap));
+    TRY(ensure_shadow(ctx, (size_t)cap * 2 * std::max(1, cfg->num_shadow_rays)));
+    std::vector<WaveDesc> waves;
+    for (uint32_t done = 0; done < num_iterations;) {
+        uint32_t it = std::min(per_wave, num_iterations - done);
+        waves.push_back(WaveDesc{(uint64_t)done * np, it * np, first_iteration + done, it, 0});
+        done += it;
+    }
+    HIP_TRY(ctx, hipMemsetAsync(ctx->wave.stats, 0, 64, ctx->stream));
+    if (!waves.empty()) {
+        TRY(ensure_wave_descs(ctx, waves.size()));
+        apply_settings(R, *cfg);
+        R.first_iter = first_iteration;
+        R.path_off = 0;
+        uint64_t offs[2] = {0, (uint64_t)num_iterations * np};
+        HIP_TRY(ctx, hipMemcpyAsync(sc->d_recs, &R, sizeof(SceneRec), hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(ctx, hipMemcpyAsync(sc->d_path_offsets, offs, sizeof(offs), hipMemcpyHostToDevice, ctx->stream));
+        TRY(run_waves(ctx, sc, waves, cfg->max_depth, Mode::Mean, sc->image + R.fb_off, np, cfg->max_render_time_ms));
+    }
+    return read_stats(ctx, sc, stats);
+}
 
-    auto t0 = std::chrono::steady_clock::now();
-    uint32_t done = 0;
-    while (done < num_iterations) {
-        // MaximumRenderTimeMs (PathTracer.cs:221-227), checked per wave
-        if (cfg->max_render_time_ms >= 0 && done > 0) {
-            double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-            double per = ms / done;
-            if (ms + per * std::min(per_wave, num_iterations - done) > (double)cfg->max_render_time_ms) break;
-        }
-        uint32_t iters = std::min(per_wave, num_iterations - done);
-        uint32_t first = first_iteration + done;
-        timed(ctx, "generate", [&] { launch_generate(ctx->stream, S, w, cfg->base_seed, first, iters); });
-        uint32_t count = np * iters;
-        int buf = 0;
-        for (int depth = 1; depth <= cfg->max_depth && count > 0; ++depth) {
-            HIP_TRY(ctx, hipMemsetAsync(w.counters, 0, 8, ctx->stream));
-            timed(ctx, "extend", [&] { launch_extend(ctx->stream, S, w, buf, count); });
-            timed(ctx, "shade", [&] { launch_shade(ctx->stream, S, w, ds, buf, count); });
-            HIP_TRY(ctx, hipMemcpyAsync(ctx->host_counters, w.counters, 8, hipMemcpyDeviceToHost, ctx->stream));
-            HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-            uint32_t next = ctx->host_counters[kCntNext];
-            uint32_t nshadow = ctx->host_counters[kCntShadow];
-            timed(ctx, "shadow", [&] { launch_shadow(ctx->stream, S, w, nshadow); });
-            count = next;
-            buf ^= 1;
-        }
-        timed(ctx, "accumulate", [&] { launch_accumulate(ctx->stream, w, sc->image, np, first, iters); });
-        done += iters;
+sspt_status sspt_render_batch(sspt_ctx* ctx, sspt_scene* sc, const sspt_pt_settings* cfgs, sspt_ray_stats* stats) {
+    if (!ctx || !sc || !cfgs) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "null argument");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const size_t n = sc->recs.size();
+    std::vector<uint64_t> offs(n + 1, 0);
+    int max_depth = 1, max_nsr = 1;
+    for (size_t i = 0; i < n; ++i) {
+        if (!valid_settings(cfgs[i])) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "invalid settings for scene %zu", i);
+        SceneRec& R = sc->recs[i];
+        apply_settings(R, cfgs[i]);
+        R.first_iter = 0;
+        R.path_off = offs[i];
+        offs[i + 1] = offs[i] + (uint64_t)R.num_pixels * (uint64_t)cfgs[i].total_spp;
+        max_depth = std::max(max_depth, cfgs[i].max_depth);
+        max_nsr = std::max(max_nsr, cfgs[i].num_shadow_rays);
+    }
+    const uint64_t total = offs[n];
+    uint32_t cap = wave_capacity(ctx);
+    if (total < cap) cap = (uint32_t)std::max<uint64_t>(total, 1);
+    TRY(ensure_wave(ctx, cap));
+    TRY(ensure_shadow(ctx, (size_t)cap * 2 * max_nsr));
+    std::vector<WaveDesc> waves;
+    for (uint64_t start = 0; start < total; start += cap)
+        waves.push_back(WaveDesc{start, (uint32_t)std::min<uint64_t>(cap, total - start), 0, 0, 0});
+    HIP_TRY(ctx, hipMemsetAsync(ctx->wave.stats, 0, 64, ctx->stream));
+    HIP_TRY(ctx, hipMemsetAsync(sc->image, 0, sc->image_floats * sizeof(float), ctx->stream));
+    sc->nan_count = 0;
+    if (!waves.empty()) {
+        TRY(ensure_wave_descs(ctx, waves.size()));
+        HIP_TRY(ctx, hipMemcpyAsync(sc->d_recs, sc->recs.data(), n * sizeof(SceneRec), hipMemcpyHostToDevice,
+                                    ctx->stream));
+        HIP_TRY(ctx, hipMemcpyAsync(sc->d_path_offsets, offs.data(), offs.size() * sizeof(uint64_t),
+                                    hipMemcpyHostToDevice, ctx->stream));
+        TRY(run_waves(ctx, sc, waves, max_depth, Mode::Sum, nullptr, 0, -1));
+        // sums -> means
+        for (size_t i = 0; i < n; ++i)
+            if (cfgs[i].total_spp > 0)
+                launch_scale(ctx->stream, sc->image + sc->recs[i].fb_off, (uint64_t)sc->recs[i].num_pixels * 3,
+                             1.0f / (float)cfgs[i].total_spp);
         HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-        ctx->timing.collect();
     }
-    HIP_TRY(ctx, hipGetLastError());
-    unsigned long long hs[kStatNum];
-    HIP_TRY(ctx, hipMemcpyAsync(hs, w.stats, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
+    return read_stats(ctx, sc, stats);
+}
+
+sspt_status sspt_render_adjoint(sspt_ctx* ctx, sspt_scene* sc, const sspt_pt_settings* cfg,
+                                const float* host_dloss_dimage, uint32_t first_iteration, uint32_t num_iterations,
+                                sspt_ray_stats* stats) {
+    if (!ctx || !sc || !cfg || !host_dloss_dimage) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "null argument");
+    if (!valid_settings(*cfg)) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "invalid settings");
+    if (sc->recs.size() != 1) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "sspt_render_adjoint renders a single scene");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    SceneRec& R = sc->recs[0];
+    const uint32_t np = R.num_pixels;
+    uint32_t per_wave = std::max(1u, std::min(num_iterations, wave_capacity(ctx) / std::max(1u, np)));
+    uint32_t cap = np * per_wave;
+    TRY(ensure_wave(ctx, cap));
+    TRY(ensure_shadow(ctx, (size_t)cap * 2 * std::max(1, cfg->num_shadow_rays)));
+    TRY(ensure_adjoint(ctx, (uint32_t)cfg->max_depth));
+    std::vector<WaveDesc> waves;
+    for (uint32_t done = 0; done < num_iterations;) {
+        uint32_t it = std::min(per_wave, num_iterations - done);
+        waves.push_back(WaveDesc{(uint64_t)done * np, it * np, first_iteration + done, it, 0});
+        done += it;
+    }
+    HIP_TRY(ctx, hipMemsetAsync(ctx->wave.stats, 0, 64, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(sc->adj_image + R.fb_off, host_dloss_dimage, (size_t)np * 12, hipMemcpyHostToDevice,
+                                ctx->stream));
+    if (!waves.empty()) {
+        TRY(ensure_wave_descs(ctx, waves.size()));
+        apply_settings(R, *cfg);
+        R.first_iter = first_iteration;
+        R.path_off = 0;
+        R.adj_scale = 1.0f / (float)num_iterations;
+        uint64_t offs[2] = {0, (uint64_t)num_iterations * np};
+        HIP_TRY(ctx, hipMemcpyAsync(sc->d_recs, &R, sizeof(SceneRec), hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(ctx, hipMemcpyAsync(sc->d_path_offsets, offs, sizeof(offs), hipMemcpyHostToDevice, ctx->stream));
+        TRY(run_waves(ctx, sc, waves, cfg->max_depth, Mode::Adjoint, nullptr, 0, -1));
+    }
+    return read_stats(ctx, sc, stats);
+}
+
+sspt_status sspt_gradient_clear(sspt_ctx* ctx, sspt_scene* sc) {
+    if (!ctx || !sc) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "null argument");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    if (sc->grad) HIP_TRY(ctx, hipMemsetAsync(sc->grad, 0, sc->texel_floats * sizeof(float), ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    sc->nan_count += hs[kStatNan];
-    if (stats) {
-        stats->num_rays = hs[kStatRays];
-        stats->num_shadow_rays = hs[kStatShadow];
-        stats->num_occluded = hs[kStatOccluded];
-        stats->num_ray_hits = hs[kStatHits];
-    }
+    return SSPT_OK;
+}
+
+static sspt_status texture_range(sspt_ctx* ctx, sspt_scene* sc, uint32_t index, uint64_t* off, uint64_t* n) {
+    if (index >= sc->textures.size()) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "texture index out of range");
+    const DevTexture& t = sc->textures[index];
+    *off = t.offset;
+    *n = (uint64_t)t.w * t.h * t.ch;
+    return SSPT_OK;
+}
+
+sspt_status sspt_gradient_read(sspt_ctx* ctx, sspt_scene* sc, uint32_t texture_index, float* host) {
+    if (!ctx || !sc || !host) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "null argument");
+    uint64_t off, n;
+    TRY(texture_range(ctx, sc, texture_index, &off, &n));
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipMemcpyAsync(host, sc->grad + off, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return SSPT_OK;
+}
+
+sspt_status sspt_texture_write(sspt_ctx* ctx, sspt_scene* sc, uint32_t texture_index, const float* host) {
+    if (!ctx || !sc || !host) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "null argument");
+    uint64_t off, n;
+    TRY(texture_range(ctx, sc, texture_index, &off, &n));
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipMemcpyAsync(const_cast<float*>(sc->dev.texels) + off, host, n * sizeof(float),
+                                hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return SSPT_OK;
 }
 
 sspt_status sspt_trace_closest(sspt_ctx* ctx, sspt_scene* sc, const float* rays, uint64_t n, sspt_hit* hits) {
     if (!ctx || !sc || (n && (!rays || !hits))) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "null argument");
     if (!n) return SSPT_OK;
-    hipSetDevice(ctx->device);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
     float* dr = nullptr;
     float4* dh = nullptr;
     HIP_TRY(ctx, hipMalloc(&dr, n * 32));
     if (hipMalloc(&dh, n * 16) != hipSuccess) {
-        hipFree(dr);
+        (void)hipFree(dr);
         return fail(ctx, SSPT_ERR_OUT_OF_MEMORY, "hipMalloc failed");
     }
     hipError_t e = hipMemcpyAsync(dr, rays, n * 32, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) {
-        launch_trace_closest(ctx->stream, sc->dev, dr, n, dh);
+        launch_trace_closest(ctx->stream, sc->dev, 0, dr, n, dh);
         e = hipMemcpyAsync(hits, dh, n * 16, hipMemcpyDeviceToHost, ctx->stream);
     }
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    hipFree(dr);
-    hipFree(dh);
+    (void)hipFree(dr);
+    (void)hipFree(dh);
     if (e != hipSuccess) return fail(ctx, SSPT_ERR_DEVICE, "trace failed: %s", hipGetErrorString(e));
     return SSPT_OK;
 }
@@ -671,22 +1049,22 @@ sspt_status sspt_trace_closest(sspt_ctx* ctx, sspt_scene* sc, const float* rays,
 sspt_status sspt_trace_occluded(sspt_ctx* ctx, sspt_scene* sc, const float* rays, uint64_t n, uint8_t* occ) {
     if (!ctx || !sc || (n && (!rays || !occ))) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "null argument");
     if (!n) return SSPT_OK;
-    hipSetDevice(ctx->device);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
     float* dr = nullptr;
     uint8_t* dh = nullptr;
     HIP_TRY(ctx, hipMalloc(&dr, n * 32));
     if (hipMalloc(&dh, n) != hipSuccess) {
-        hipFree(dr);
+        (void)hipFree(dr);
         return fail(ctx, SSPT_ERR_OUT_OF_MEMORY, "hipMalloc failed");
     }
     hipError_t e = hipMemcpyAsync(dr, rays, n * 32, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) {
-        launch_trace_occluded(ctx->stream, sc->dev, dr, n, dh);
+        launch_trace_occluded(ctx->stream, sc->dev, 0, dr, n, dh);
         e = hipMemcpyAsync(occ, dh, n, hipMemcpyDeviceToHost, ctx->stream);
     }
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    hipFree(dr);
-    hipFree(dh);
+    (void)hipFree(dr);
+    (void)hipFree(dh);
     if (e != hipSuccess) return fail(ctx, SSPT_ERR_DEVICE, "trace failed: %s", hipGetErrorString(e));
     return SSPT_OK;
 }
@@ -694,24 +1072,42 @@ sspt_status sspt_trace_occluded(sspt_ctx* ctx, sspt_scene* sc, const float* rays
 sspt_status sspt_primary_hits(sspt_ctx* ctx, sspt_scene* sc, uint32_t base_seed, uint32_t iteration,
                               float* rays_out, sspt_hit* hits_out) {
     if (!ctx || !sc) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "null argument");
-    hipSetDevice(ctx->device);
-    uint64_t n = (uint64_t)sc->dev.width * sc->dev.height;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    uint64_t n = sc->recs[0].num_pixels;
     float* dr = nullptr;
     float4* dh = nullptr;
     HIP_TRY(ctx, hipMalloc(&dr, n * 32));
     if (hipMalloc(&dh, n * 16) != hipSuccess) {
-        hipFree(dr);
+        (void)hipFree(dr);
         return fail(ctx, SSPT_ERR_OUT_OF_MEMORY, "hipMalloc failed");
     }
-    launch_primary_rays(ctx->stream, sc->dev, base_seed, iteration, dr);
-    launch_trace_closest(ctx->stream, sc->dev, dr, n, dh);
-    hipError_t e = hipSuccess;
-    if (rays_out) e = hipMemcpyAsync(rays_out, dr, n * 32, hipMemcpyDeviceToHost, ctx->stream);
+    // the camera of the scene record; settings do not matter for primary rays
+    hipError_t e = hipMemcpyAsync(sc->d_recs, sc->recs.data(), sizeof(SceneRec), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) {
+        launch_primary_rays(ctx->stream, sc->dev, 0, (uint32_t)n, base_seed, iteration, dr);
+        launch_trace_closest(ctx->stream, sc->dev, 0, dr, n, dh);
+        if (rays_out) e = hipMemcpyAsync(rays_out, dr, n * 32, hipMemcpyDeviceToHost, ctx->stream);
+    }
     if (e == hipSuccess && hits_out) e = hipMemcpyAsync(hits_out, dh, n * 16, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    hipFree(dr);
-    hipFree(dh);
+    (void)hipFree(dr);
+    (void)hipFree(dh);
     if (e != hipSuccess) return fail(ctx, SSPT_ERR_DEVICE, "primary hits failed: %s", hipGetErrorString(e));
+    return SSPT_OK;
+}
+
+sspt_status sspt_set_wave_capacity(sspt_ctx* ctx, uint32_t slots) {
+    if (!ctx) return SSPT_ERR_INVALID_ARGUMENT;
+    if (slots > kMaxSlots) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "wave capacity above %u", kMaxSlots);
+    ctx->want_capacity = slots;
+    return SSPT_OK;
+}
+
+sspt_status sspt_set_launch_config(sspt_ctx* ctx, uint32_t trace_blocks, uint32_t shade_blocks, int32_t use_graphs) {
+    if (!ctx) return SSPT_ERR_INVALID_ARGUMENT;
+    ctx->trace_blocks = trace_blocks;
+    ctx->shade_blocks = shade_blocks;
+    ctx->use_graphs = use_graphs != 0;
     return SSPT_OK;
 }
 
@@ -723,7 +1119,7 @@ sspt_status sspt_set_kernel_timing(sspt_ctx* ctx, int32_t enable) {
 
 sspt_status sspt_get_kernel_timing(sspt_ctx* ctx, const char* name, uint64_t* launches, double* total_ms) {
     if (!ctx || !name) return SSPT_ERR_INVALID_ARGUMENT;
-    hipStreamSynchronize(ctx->stream);
+    (void)hipStreamSynchronize(ctx->stream);
     ctx->timing.collect();
     auto it = ctx->timing.totals.find(name);
     if (launches) *launches = it == ctx->timing.totals.end() ? 0 : it->second.first;
@@ -733,7 +1129,7 @@ sspt_status sspt_get_kernel_timing(sspt_ctx* ctx, const char* name, uint64_t* la
 
 sspt_status sspt_reset_kernel_timing(sspt_ctx* ctx) {
     if (!ctx) return SSPT_ERR_INVALID_ARGUMENT;
-    hipStreamSynchronize(ctx->stream);
+    (void)hipStreamSynchronize(ctx->stream);
     ctx->timing.collect();
     ctx->timing.totals.clear();
     return SSPT_OK;

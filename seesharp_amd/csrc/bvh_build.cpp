@@ -3,6 +3,7 @@
 #include "bvh_build.h"
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <cstring>
@@ -171,61 +172,98 @@ void build_bvh(const BvhInput& in, int max_leaf, int threads, BvhOutput* out) {
     if (n > 0) b.build(0, 0, n, 0);
     int used = b.next_node.load();
 
-    // Convert: one device node per inner BNode holding both child boxes.
-    std::vector<int> dev_index(used, -1);
-    int dev_count = 0;
-    // Breadth-first order keeps the top of the tree compact.
+    // Collapse into a 4-wide BVH: each device node adopts the children of its
+    // largest-area inner children until it holds four, breadth-first so the top
+    // of the tree is contiguous in memory.
     const bool root_is_leaf = n == 0 || b.nodes[0].count > 0;
-    std::vector<int> queue;
-    if (!root_is_leaf) queue.push_back(0);
-    for (size_t q = 0; q < queue.size(); ++q) {
-        int bn = queue[q];
-        if (b.nodes[bn].count > 0) continue;
-        dev_index[bn] = dev_count++;
-        queue.push_back(b.nodes[bn].left);
-        queue.push_back(b.nodes[bn].right);
+    auto collapse = [&](int bn, int* kids) {
+        int k = 2;
+        kids[0] = b.nodes[bn].left;
+        kids[1] = b.nodes[bn].right;
+        while (k < 4) {
+            int best = -1;
+            float best_area = -1.0f;
+            for (int i = 0; i < k; ++i) {
+                const BNode& c = b.nodes[kids[i]];
+                if (c.count > 0) continue;
+                float a = Builder::half_area(c.lo, c.hi);
+                if (a > best_area) best_area = a, best = i;
+            }
+            if (best < 0) break;
+            int c = kids[best];
+            kids[best] = b.nodes[c].left;
+            kids[k++] = b.nodes[c].right;
+        }
+        return k;
+    };
+    std::vector<int> dev_of(used, -1);   // BNode -> device node
+    std::vector<int> order4;             // device node -> BNode
+    std::vector<int> depth4;             // device node depth
+    std::vector<std::array<int, 4>> kids4;
+    std::vector<int> nkids4;
+    if (!root_is_leaf) {
+        dev_of[0] = 0;
+        order4.push_back(0);
+        depth4.push_back(0);
+    }
+    for (size_t q = 0; q < order4.size(); ++q) {
+        std::array<int, 4> kids{-1, -1, -1, -1};
+        int k = collapse(order4[q], kids.data());
+        for (int i = 0; i < k; ++i) {
+            if (b.nodes[kids[i]].count > 0) continue;
+            dev_of[kids[i]] = (int)order4.size();
+            order4.push_back(kids[i]);
+            depth4.push_back(depth4[q] + 1);
+        }
+        kids4.push_back(kids);
+        nkids4.push_back(k);
     }
     const BNode& root = b.nodes[0];
-    bool root_leaf = root_is_leaf;
-    if (root_leaf) dev_count = 1;
-    out->nodes.assign((size_t)dev_count * 16, 0.0f);
-    auto put_child = [&](float* node, int side, const BNode* c) {
-        // side 0 = left, 1 = right
-        // An absent child gets the box [+inf, +inf]^3: both slab distances are
-        // +inf (or -inf) on every axis, so the slab test never accepts it.
-        float lox = c ? c->lo[0] : INFINITY, hix = c ? c->hi[0] : INFINITY;
-        float loy = c ? c->lo[1] : INFINITY, hiy = c ? c->hi[1] : INFINITY;
-        float loz = c ? c->lo[2] : INFINITY, hiz = c ? c->hi[2] : INFINITY;
-        float* row = node + side * 4;
-        row[0] = lox, row[1] = hix, row[2] = loy, row[3] = hiy;
-        node[8 + side * 2 + 0] = loz;
-        node[8 + side * 2 + 1] = hiz;
-        int idx, cnt;
-        if (!c) {
-            idx = 0, cnt = 0;
-            // an empty box is never entered, so the index is never followed
-        } else if (c->count > 0) {
+    int dev_count = root_is_leaf ? 1 : (int)order4.size();
+    out->nodes.assign((size_t)dev_count * 32, 0.0f);
+    auto put_child = [&](float* node, int slot, const BNode* c) {
+        // An absent child has count -1 (the traversal skips it) and the box
+        // [+inf, +inf]^3. The box alone is not enough: with tmax = +inf and a
+        // direction whose components are all positive every slab gives +inf.
+        for (int a = 0; a < 3; ++a) {
+            node[(2 * a) * 4 + slot] = c ? c->lo[a] : INFINITY;
+            node[(2 * a + 1) * 4 + slot] = c ? c->hi[a] : INFINITY;
+        }
+        int idx = 0, cnt = c ? 0 : -1;
+        if (c && c->count > 0) {
             idx = c->first, cnt = c->count;
-        } else {
-            idx = dev_index[(int)(c - b.nodes.data())], cnt = 0;
+        } else if (c) {
+            idx = dev_of[(int)(c - b.nodes.data())];
         }
-        int* ints = reinterpret_cast<int*>(node + 12);
-        ints[side] = idx;
-        ints[2 + side] = cnt;
+        int* ints = reinterpret_cast<int*>(node + 24);
+        ints[slot] = idx;
+        ints[4 + slot] = cnt;
     };
-    if (root_leaf) {
+    if (root_is_leaf) {
         float* node = out->nodes.data();
-        if (n > 0) put_child(node, 0, &root);
-        else put_child(node, 0, nullptr);
-        put_child(node, 1, nullptr);
+        put_child(node, 0, n > 0 ? &root : nullptr);
+        for (int i = 1; i < 4; ++i) put_child(node, i, nullptr);
+        out->stack_need = 1;
+        out->depth4 = 0;
     } else {
-        for (int bn = 0; bn < used; ++bn) {
-            int d = dev_index[bn];
-            if (d < 0) continue;
-            float* node = out->nodes.data() + (size_t)d * 16;
-            put_child(node, 0, &b.nodes[b.nodes[bn].left]);
-            put_child(node, 1, &b.nodes[b.nodes[bn].right]);
+        for (size_t d = 0; d < order4.size(); ++d) {
+            float* node = out->nodes.data() + d * 32;
+            for (int i = 0; i < 4; ++i) put_child(node, i, i < nkids4[d] ? &b.nodes[kids4[d][i]] : nullptr);
         }
+        // exact traversal stack bound: a node pushes all but one of its inner children
+        std::vector<int> need(order4.size(), 0);
+        for (size_t d = order4.size(); d-- > 0;) {
+            int inner = 0, deepest = 0;
+            for (int i = 0; i < nkids4[d]; ++i) {
+                int c = dev_of[kids4[d][i]];
+                if (b.nodes[kids4[d][i]].count > 0 || c < 0) continue;
+                ++inner;
+                deepest = std::max(deepest, need[c]);
+            }
+            need[d] = std::max(0, inner - 1) + deepest;
+        }
+        out->stack_need = need[0] + 1;
+        out->depth4 = *std::max_element(depth4.begin(), depth4.end());
     }
     out->leaf_order = std::move(b.order);
     out->depth = b.max_depth.load();

@@ -14,7 +14,9 @@ struct BvhInput {
 };
 
 struct BvhOutput {
-    std::vector<float> nodes;     // 16 floats per node (4 x float4)
+    std::vector<float> nodes;     // 32 floats per node (8 x float4), 4-wide
+    int stack_need = 0;           // largest traversal stack the tree can require
+    int depth4 = 0;               // depth of the 4-wide tree
     std::vector<int> leaf_order;  // leaf_order[i] = global triangle id stored at leaf index i
     int depth = 0;
     float scene_lo[3], scene_hi[3];

@@ -8,41 +8,69 @@
 
 namespace sspt {
 
-// Wavefront buffers, all in HBM. P = path slots of one wave (iterations x pixels).
+// Wavefront buffers, all in HBM. P = path slots of one wave.
 struct Wave {
     uint32_t capacity;     // P
-    // extension queues (double buffered): origin+tmin, direction+tmax, path slot
+    // extension queues (double buffered): origin+tmin, direction+tmax, (scene << 24 | slot)
     float4* q_o[2];
     float4* q_d[2];
-    int* q_slot[2];
+    uint32_t* q_slot[2];
     float4* hits;          // (t, u, v, leaf triangle index bits), per queue entry
-    // shadow queue: origin+tmin, direction+tmax, contribution rgb + (slot<<1 | kind)
+    // shadow queue: origin+tmin, direction+tmax, contribution rgb + (scene << 25 | slot << 1 | kind)
     float4* s_o;
     float4* s_d;
     float4* s_c;
     // path state per slot
     uint32_t* st_rng;
-    float4* st_thr;   // prefix weight rgb, previous pdf
-    float4* st_prev;  // previous hit position, depth (int bits)
-    float4* st_L;     // radiance from emitter / background hits
-    float4* st_Le;    // environment next-event contributions
-    float4* st_La;    // area-light next-event contributions
-    uint32_t* counters;          // kCntNum
-    unsigned long long* stats;   // kStatNum
+    uint32_t* st_pix;  // (scene << 24 | pixel)
+    float4* st_thr;    // prefix weight rgb, previous pdf
+    float4* st_prev;   // previous hit position, depth (int bits)
+    float4* st_L;      // radiance from emitter / background hits
+    float4* st_Le;     // environment next-event contributions
+    float4* st_La;     // area-light next-event contributions
+    // adjoint renders: per vertex (index depth-1) and slot, [depth * capacity + slot]
+    float4* ad_dlog;   // 3 float4 per record: d log f / d(base r,g,b, roughness) per colour channel
+    float4* ad_A;      // radiance accumulated after the vertex's next-event rays resolved
+    int2* ad_tex;      // float offsets of the vertex's albedo / roughness texels (-1 = none)
+    uint32_t ad_depth; // records allocated per slot
+    float4* s_g;       // per shadow ray: gradient of its contribution w.r.t. the vertex's 4 parameters
+    int2* s_t;         // per shadow ray: texel offsets of the vertex
+    uint32_t* counters;            // kCntNum
+    unsigned long long* stats;     // kStatNum
+    const WaveDesc* waves;         // per-wave work of the current render
 };
 
-void launch_generate(hipStream_t s, const DevScene& sc, const Wave& w, uint32_t base_seed,
-                     uint32_t first_iter, uint32_t num_iters);
-void launch_extend(hipStream_t s, const DevScene& sc, const Wave& w, int buf, uint32_t count);
-void launch_shade(hipStream_t s, const DevScene& sc, const Wave& w, const DevSettings& cfg, int buf,
-                  uint32_t count);
-void launch_shadow(hipStream_t s, const DevScene& sc, const Wave& w, uint32_t count);
-void launch_accumulate(hipStream_t s, const Wave& w, float* image, uint32_t num_pixels,
-                       uint32_t first_iter, uint32_t num_iters);
+// Grid sizes of the persistent kernels (blocks); every kernel loops over the
+// device-side count of its queue.
+struct LaunchConfig {
+    uint32_t trace_blocks;   // blocks of kTraceBlock threads for extend / shadow
+    uint32_t shade_blocks;   // blocks of 256 threads for generate / shade / accumulate
+};
 
-// Ray queries for the API: rays are 8 floats (ox,oy,oz,tmin,dx,dy,dz,tmax).
-void launch_trace_closest(hipStream_t s, const DevScene& sc, const float* rays, uint64_t n, float4* hits_out);
-void launch_trace_occluded(hipStream_t s, const DevScene& sc, const float* rays, uint64_t n, uint8_t* occ);
-void launch_primary_rays(hipStream_t s, const DevScene& sc, uint32_t base_seed, uint32_t iteration, float* rays);
+void launch_wave_begin(hipStream_t s, const Wave& w);
+void launch_generate(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc);
+void launch_bounce_reset(hipStream_t s, const Wave& w, int buf);
+void launch_extend(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int buf);
+void launch_shade(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int buf);
+void launch_shadow(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc);
+// running mean of one scene (waves of whole iterations)
+void launch_accumulate_mean(hipStream_t s, const Wave& w, float* image, uint32_t num_pixels);
+// per-scene sums of a batch (any wave layout)
+void launch_accumulate_sum(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc,
+                           float* image_pool);
+void launch_wave_end(hipStream_t s, const Wave& w);
+// adjoint (path replay) variants
+void launch_shade_adjoint(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int buf);
+void launch_shadow_adjoint(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc);
+void launch_adjoint_finalize(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc);
+void launch_scale(hipStream_t s, float* data, uint64_t n, float factor);
+
+// Ray queries on scene `scene` of the world: rays are 8 floats (ox,oy,oz,tmin,dx,dy,dz,tmax).
+void launch_trace_closest(hipStream_t s, const DevScene& sc, int scene, const float* rays, uint64_t n,
+                          float4* hits_out);
+void launch_trace_occluded(hipStream_t s, const DevScene& sc, int scene, const float* rays, uint64_t n,
+                           uint8_t* occ);
+void launch_primary_rays(hipStream_t s, const DevScene& sc, int scene, uint32_t num_pixels, uint32_t base_seed,
+                         uint32_t iteration, float* rays);
 
 }  // namespace sspt

@@ -10,6 +10,9 @@
 // and the batch ends with `accumulate`, the FrameBuffer running mean.
 // This is the GPU form of PathTracerBase.Render / RenderPixel /
 // EstimateIncidentRadiance (SeeSharp/Integrators/PathTracer.cs:199-530).
+#include <algorithm>
+
+#include "adjoint.h"
 #include "kernels.h"
 #include "shading.h"
 
@@ -132,8 +135,8 @@ __device__ __forceinline__ bool box_test(float lox, float hix, float loy, float 
 // hit so far uses [tmin - slack, tmax + slack] with slack = 2^-16 * max(|o|,
 // scene extent) / |d|, so a box is never culled by rounding against a triangle
 // distance that is equal in exact arithmetic.
-__device__ __forceinline__ float ray_slack(const DevScene& S, f3 o, f3 d) {
-    float m = fmaxf(S.abs_max, fmaxf(fmaxf(fabsf(o.x), fabsf(o.y)), fabsf(o.z)));
+__device__ __forceinline__ float ray_slack(float abs_max, f3 o, f3 d) {
+    float m = fmaxf(abs_max, fmaxf(fmaxf(fabsf(o.x), fabsf(o.y)), fabsf(o.z)));
     return m * 1.52587890625e-05f / length(d);
 }
 
@@ -143,136 +146,262 @@ struct HitOut {
     int gid;  // global triangle id
 };
 
-// BVH traversal with a per-lane stack in LDS (stack[depth][lane] layout: no
-// bank conflicts). ANY = true stops at the first hit (shadow rays).
+// Per-lane traversal stack of (node, entry distance): the first kStackLds
+// entries in LDS ([depth][lane] layout, conflict free), the rest in a private
+// array that only deep trees reach.
+struct TraceStack {
+    int* lds_node;    // + threadIdx.x, stride kTraceBlock
+    float* lds_t;
+    int spill_node[kStackTotal - kStackLds];
+    float spill_t[kStackTotal - kStackLds];
+    int sp;
+    __device__ __forceinline__ void push(int node, float t) {
+        // The builder rejects trees needing more than kStackTotal entries; the
+        // guard keeps memory safe regardless.
+        if (sp >= kStackTotal) return;
+        if (sp < kStackLds) {
+            lds_node[sp * kTraceBlock] = node;
+            lds_t[sp * kTraceBlock] = t;
+        } else {
+            spill_node[sp - kStackLds] = node;
+            spill_t[sp - kStackLds] = t;
+        }
+        ++sp;
+    }
+    __device__ __forceinline__ void pop(int* node, float* t) {
+        --sp;
+        if (sp < kStackLds) {
+            *node = lds_node[sp * kTraceBlock];
+            *t = lds_t[sp * kTraceBlock];
+        } else {
+            *node = spill_node[sp - kStackLds];
+            *t = spill_t[sp - kStackLds];
+        }
+    }
+};
+
+__device__ __forceinline__ void cswap(float& ta, int& na, float& tb, int& nb) {
+    bool sw = tb < ta;
+    float t = sw ? tb : ta;
+    int n = sw ? nb : na;
+    tb = sw ? ta : tb;
+    nb = sw ? na : nb;
+    ta = t;
+    na = n;
+}
+
+// 4-wide BVH traversal. Closest hit visits hit children nearest first and
+// skips popped nodes whose entry distance is beyond the best hit; ANY = true
+// stops at the first hit (shadow rays).
 template <bool ANY>
-__device__ __forceinline__ HitOut traverse(const DevScene& S, f3 o, f3 d, float tmin, float tmax, int* stk) {
+__device__ __forceinline__ HitOut traverse(const DevScene& S, int root, float abs_max, f3 o, f3 d, float tmin,
+                                           float tmax, int* lds_node, float* lds_t) {
     RayPre p = precompute(d);
-    const float slack = ray_slack(S, o, d);
+    const float slack = ray_slack(abs_max, o, d);
     const float tmin_box = tmin - slack;
     HitOut best{tmax, 0.0f, 0.0f, -1, 0x7fffffff};
-    int sp = 0;
-    int node = 0;
-    const int stride = kTraceBlock;
+    TraceStack st;
+    st.lds_node = lds_node;
+    st.lds_t = lds_t;
+    st.sp = 0;
+    int node = root;
     for (;;) {
-        const float4* N = S.nodes + 4 * node;
-        float4 n0 = N[0], n1 = N[1], n2 = N[2];
-        int4 c = *reinterpret_cast<const int4*>(N + 3);
-        float tl, tr;
+        if ((unsigned)node >= (unsigned)S.num_nodes) break;  // never taken for a valid tree
+        const float4* N = S.nodes + 8 * node;
+        float4 lox = N[0], hix = N[1], loy = N[2], hiy = N[3], loz = N[4], hiz = N[5];
+        int4 child = *reinterpret_cast<const int4*>(N + 6);
+        int4 count = *reinterpret_cast<const int4*>(N + 7);
         const float tmax_box = best.t + slack;
-        bool hl = box_test(n0.x, n0.y, n0.z, n0.w, n2.x, n2.y, o, p.inv, tmin_box, tmax_box, &tl);
-        bool hr = box_test(n1.x, n1.y, n1.z, n1.w, n2.z, n2.w, o, p.inv, tmin_box, tmax_box, &tr);
-        // leaf children are intersected right away
-        for (int side = 0; side < 2; ++side) {
-            bool h = side == 0 ? hl : hr;
-            int cnt = side == 0 ? c.z : c.w;
-            if (!(h && cnt > 0)) continue;
-            int first = side == 0 ? c.x : c.y;
-            for (int k = first; k < first + cnt; ++k) {
-                float4 a = S.tri_pos[3 * k], b = S.tri_pos[3 * k + 1], cc = S.tri_pos[3 * k + 2];
+        float tc[4];
+        bool hc[4];
+        hc[0] = box_test(lox.x, hix.x, loy.x, hiy.x, loz.x, hiz.x, o, p.inv, tmin_box, tmax_box, &tc[0]);
+        hc[1] = box_test(lox.y, hix.y, loy.y, hiy.y, loz.y, hiz.y, o, p.inv, tmin_box, tmax_box, &tc[1]);
+        hc[2] = box_test(lox.z, hix.z, loy.z, hiy.z, loz.z, hiz.z, o, p.inv, tmin_box, tmax_box, &tc[2]);
+        hc[3] = box_test(lox.w, hix.w, loy.w, hiy.w, loz.w, hiz.w, o, p.inv, tmin_box, tmax_box, &tc[3]);
+        const int cidx[4] = {child.x, child.y, child.z, child.w};
+        const int ccnt[4] = {count.x, count.y, count.z, count.w};
+        for (int k = 0; k < 4; ++k) hc[k] = hc[k] && ccnt[k] >= 0;  // absent slots
+        // leaf children: intersect their triangles right away
+        for (int k = 0; k < 4; ++k) {
+            if (!(hc[k] && ccnt[k] > 0)) continue;
+            hc[k] = false;
+            for (int tri = cidx[k]; tri < cidx[k] + ccnt[k]; ++tri) {
+                float4 a = S.tri_pos[3 * tri], b = S.tri_pos[3 * tri + 1], c = S.tri_pos[3 * tri + 2];
                 float t, u, v;
-                if (!intersect_tri(o, p, tmin, best.t, f3_of(a), f3_of(b), f3_of(cc), &t, &u, &v)) continue;
+                if (!intersect_tri(o, p, tmin, best.t, f3_of(a), f3_of(b), f3_of(c), &t, &u, &v)) continue;
                 int gid = __float_as_int(a.w);
-                if (ANY) {
+                if (ANY || t < best.t || gid < best.gid) {
                     best.t = t;
                     best.u = u;
                     best.v = v;
-                    best.li = k;
+                    best.li = tri;
                     best.gid = gid;
-                    return best;
-                }
-                if (t < best.t || gid < best.gid) {
-                    best.t = t;
-                    best.u = u;
-                    best.v = v;
-                    best.li = k;
-                    best.gid = gid;
+                    if (ANY) return best;
                 }
             }
-            if (side == 0) hl = false;
-            else hr = false;
         }
-        if (hl && hr) {
-            int near = tl <= tr ? c.x : c.y;
-            int far = tl <= tr ? c.y : c.x;
-            stk[sp * stride] = far;
-            ++sp;
-            node = near;
-        } else if (hl) {
-            node = c.x;
-        } else if (hr) {
-            node = c.y;
-        } else {
-            if (sp == 0) break;
-            --sp;
-            node = stk[sp * stride];
+        // inner children, nearest first. A hit child's key is clamped to FLT_MAX
+        // (its entry distance is +inf when the ray runs parallel to a slab and no
+        // hit bounds tmax yet) so every hit child sorts before every other slot.
+        float t0 = hc[0] ? fminf(tc[0], 3.402823466e38f) : INFINITY;
+        float t1 = hc[1] ? fminf(tc[1], 3.402823466e38f) : INFINITY;
+        float t2 = hc[2] ? fminf(tc[2], 3.402823466e38f) : INFINITY;
+        float t3 = hc[3] ? fminf(tc[3], 3.402823466e38f) : INFINITY;
+        int n0 = cidx[0], n1 = cidx[1], n2 = cidx[2], n3 = cidx[3];
+        int nhit = (int)hc[0] + (int)hc[1] + (int)hc[2] + (int)hc[3];
+        if (nhit > 0) {
+            cswap(t0, n0, t1, n1);
+            cswap(t2, n2, t3, n3);
+            cswap(t0, n0, t2, n2);
+            cswap(t1, n1, t3, n3);
+            cswap(t1, n1, t2, n2);
+            if (nhit > 3) st.push(n3, t3);
+            if (nhit > 2) st.push(n2, t2);
+            if (nhit > 1) st.push(n1, t1);
+            node = n0;
+            continue;
         }
+        // pop the next node still in front of the best hit
+        bool found = false;
+        while (st.sp > 0) {
+            float tn;
+            st.pop(&node, &tn);
+            if (ANY || tn <= best.t + slack) {
+                found = true;
+                break;
+            }
+        }
+        if (!found) break;
     }
     return best;
 }
 
 // ---------------------------------------------------------------- kernels --
-__global__ void __launch_bounds__(256) k_generate(DevScene S, Wave w, uint32_t base_seed,
-                                                  uint32_t first_iter, uint32_t num_pixels,
-                                                  uint32_t total) {
-    uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-    if (slot >= total) return;
-    uint32_t it = slot / num_pixels;
-    uint32_t pix = slot - it * num_pixels;
-    uint32_t row = pix / (uint32_t)S.width;
-    uint32_t col = pix - row * (uint32_t)S.width;
-    // RNG(BaseSeed, row * W + col, sampleIndex) (PathTracer.cs:236-237); jitter (:289-290)
-    Rng rng{hash_seed(base_seed, pix, first_iter + it)};
-    f2 off = rng.next_float2();
-    f3 o, d;
-    camera_ray(S.cam, (float)col + off.x, (float)row + off.y, &o, &d);
-    w.q_o[0][slot] = f4_of(o, 0.0f);
-    w.q_d[0][slot] = f4_of(d, INFINITY);
-    w.q_slot[0][slot] = (int)slot;
-    w.st_rng[slot] = rng.state;
-    w.st_thr[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-    w.st_prev[slot] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(1));
-    w.st_L[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    w.st_Le[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    w.st_La[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+// Every wave kernel is persistent: a fixed grid loops over the device-side
+// length of its queue, so a whole wave runs without host synchronisation and
+// can be captured in a graph.
+
+__device__ __forceinline__ const WaveDesc& cur_wave(const Wave& w) { return w.waves[w.counters[kCntWave]]; }
+
+__global__ void k_wave_begin(Wave w) {
+    const WaveDesc& d = cur_wave(w);
+    w.counters[kCntQueue0] = d.count;
+    w.counters[kCntQueue1] = 0;
+    w.counters[kCntShadow] = 0;
 }
 
-__global__ void __launch_bounds__(kTraceBlock) k_extend(DevScene S, const float4* __restrict__ q_o,
-                                                        const float4* __restrict__ q_d, float4* hits,
-                                                        uint32_t count, unsigned long long* stats) {
-    __shared__ int stack[kMaxBvhDepth * kTraceBlock];
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
-    float4 o4 = q_o[i], d4 = q_d[i];
-    HitOut h = traverse<false>(S, f3_of(o4), f3_of(d4), o4.w, d4.w, stack + threadIdx.x);
-    hits[i] = h.li >= 0 ? make_float4(h.t, h.u, h.v, __int_as_float(h.li))
-                        : make_float4(INFINITY, 0.0f, 0.0f, __int_as_float(-1));
-    wave_count(&stats[kStatRays], true);
-    wave_count(&stats[kStatHits], h.li >= 0);
+__global__ void k_wave_end(Wave w) { w.counters[kCntWave] += 1; }
+
+__global__ void k_bounce_reset(Wave w, int next) {
+    w.counters[next] = 0;
+    w.counters[kCntShadow] = 0;
 }
 
-__global__ void __launch_bounds__(kTraceBlock) k_shadow(DevScene S, const float4* __restrict__ s_o,
-                                                        const float4* __restrict__ s_d,
-                                                        const float4* __restrict__ s_c, float4* L_env,
-                                                        float4* L_area, uint32_t count,
-                                                        unsigned long long* stats) {
-    __shared__ int stack[kMaxBvhDepth * kTraceBlock];
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
-    float4 o4 = s_o[i], d4 = s_d[i];
-    HitOut h = traverse<true>(S, f3_of(o4), f3_of(d4), o4.w, d4.w, stack + threadIdx.x);
-    bool occluded = h.li >= 0;
-    if (!occluded) {
-        float4 c = s_c[i];
-        int tag = __float_as_int(c.w);
-        float4* dst = (tag & 1) ? L_area : L_env;
-        int slot = tag >> 1;
-        atomicAdd(&dst[slot].x, c.x);
-        atomicAdd(&dst[slot].y, c.y);
-        atomicAdd(&dst[slot].z, c.z);
+// Scene of world-wide path g: last scene whose path offset is <= g.
+__device__ __forceinline__ int scene_of(const DevScene& S, uint64_t g) {
+    int lo = 0, hi = S.num_scenes - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (S.path_offsets[mid] <= g) lo = mid;
+        else hi = mid - 1;
     }
-    wave_count(&stats[kStatShadow], true);
-    wave_count(&stats[kStatOccluded], occluded);
+    return lo;
+}
+
+// RNG(BaseSeed, row * W + col, iteration) (PathTracer.cs:236-237), pixel jitter
+// (:289-290), camera ray (:291) and the initial PathState (:293-302).
+__global__ void __launch_bounds__(256) k_generate(DevScene S, Wave w) {
+    const WaveDesc d = cur_wave(w);
+    for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < d.count; slot += gridDim.x * blockDim.x) {
+        uint64_t g = d.start + slot;
+        int sc = scene_of(S, g);
+        const SceneRec& R = S.recs[sc];
+        uint64_t k = g - R.path_off;
+        uint32_t it = (uint32_t)(k / R.num_pixels);
+        uint32_t pix = (uint32_t)(k - (uint64_t)it * R.num_pixels);
+        uint32_t row = pix / (uint32_t)R.cam.width;
+        uint32_t col = pix - row * (uint32_t)R.cam.width;
+        Rng rng{hash_seed(R.base_seed, pix, R.first_iter + it)};
+        f2 off = rng.next_float2();
+        f3 o, dir;
+        camera_ray(R.cam, (float)col + off.x, (float)row + off.y, &o, &dir);
+        w.q_o[0][slot] = f4_of(o, 0.0f);
+        w.q_d[0][slot] = f4_of(dir, INFINITY);
+        w.q_slot[0][slot] = slot | ((uint32_t)sc << kSlotBits);
+        w.st_pix[slot] = pix | ((uint32_t)sc << kSlotBits);
+        w.st_rng[slot] = rng.state;
+        w.st_thr[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+        w.st_prev[slot] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(1));
+        w.st_L[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        w.st_Le[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        w.st_La[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+}
+
+__global__ void __launch_bounds__(kTraceBlock) k_extend(DevScene S, Wave w, int buf) {
+    __shared__ int stack_node[kStackLds * kTraceBlock];
+    __shared__ float stack_t[kStackLds * kTraceBlock];
+    const uint32_t count = w.counters[buf];
+    const float4* __restrict__ q_o = w.q_o[buf];
+    const float4* __restrict__ q_d = w.q_d[buf];
+    const uint32_t* __restrict__ q_slot = w.q_slot[buf];
+    for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += gridDim.x * blockDim.x) {
+        uint32_t i = base + threadIdx.x;
+        bool active = i < count;
+        HitOut h{0.0f, 0.0f, 0.0f, -1, 0};
+        if (active) {
+            float4 o4 = q_o[i], d4 = q_d[i];
+            const SceneRec& R = S.recs[q_slot[i] >> kSlotBits];
+            h = traverse<false>(S, R.root, R.abs_max, f3_of(o4), f3_of(d4), o4.w, d4.w, stack_node + threadIdx.x,
+                                stack_t + threadIdx.x);
+            w.hits[i] = h.li >= 0 ? make_float4(h.t, h.u, h.v, __int_as_float(h.li))
+                                  : make_float4(INFINITY, 0.0f, 0.0f, __int_as_float(-1));
+        }
+        wave_count(&w.stats[kStatRays], active);
+        wave_count(&w.stats[kStatHits], active && h.li >= 0);
+    }
+}
+
+template <bool ADJ>
+__global__ void __launch_bounds__(kTraceBlock) k_shadow(DevScene S, Wave w) {
+    __shared__ int stack_node[kStackLds * kTraceBlock];
+    __shared__ float stack_t[kStackLds * kTraceBlock];
+    const uint32_t count = w.counters[kCntShadow];
+    for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += gridDim.x * blockDim.x) {
+        uint32_t i = base + threadIdx.x;
+        bool active = i < count;
+        bool occluded = false;
+        if (active) {
+            float4 o4 = w.s_o[i], d4 = w.s_d[i];
+            float4 c = w.s_c[i];
+            uint32_t tag = (uint32_t)__float_as_int(c.w);
+            const SceneRec& R = S.recs[tag >> (kSlotBits + 1)];
+            HitOut h = traverse<true>(S, R.root, R.abs_max, f3_of(o4), f3_of(d4), o4.w, d4.w,
+                                      stack_node + threadIdx.x, stack_t + threadIdx.x);
+            occluded = h.li >= 0;
+            if (!occluded) {
+                // the contribution is non-zero by construction
+                float4* dst = (tag & 1u) ? w.st_La : w.st_Le;
+                uint32_t slot = (tag >> 1) & kSlotMask;
+                atomicAdd(&dst[slot].x, c.x);
+                atomicAdd(&dst[slot].y, c.y);
+                atomicAdd(&dst[slot].z, c.z);
+                if (ADJ) {
+                    // the vertex's own term: d(contribution)/d(parameters), final once visible
+                    int2 t = w.s_t[i];
+                    float4 g = w.s_g[i];
+                    if (t.x >= 0) {
+                        atomicAdd(&S.grad[t.x], g.x);
+                        atomicAdd(&S.grad[t.x + 1], g.y);
+                        atomicAdd(&S.grad[t.x + 2], g.z);
+                    }
+                    if (t.y >= 0) atomicAdd(&S.grad[t.y], g.w);
+                }
+            }
+        }
+        wave_count(&w.stats[kStatShadowTraced], active);
+        wave_count(&w.stats[kStatOccluded], active && occluded);
+    }
 }
 
 // TinyEmbree SpawnRay / IsOccluded offset rule: move along the face normal by
@@ -282,225 +411,375 @@ __device__ __forceinline__ f3 offset_point(const Surf& s, f3 dir) {
     return s.pos + sign * s.err * s.ng;
 }
 
-__global__ void __launch_bounds__(256) k_shade(DevScene S, Wave w, DevSettings cfg, int buf, uint32_t count) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
+__device__ __forceinline__ uint32_t shadow_tag(uint32_t slot, uint32_t scene, uint32_t kind) {
+    return (scene << (kSlotBits + 1)) | (slot << 1) | kind;
+}
+
+// One bounce of EstimateIncidentRadiance (PathTracer.cs:315-382) for every path
+// of the queue: background / emitter hits with MIS, Russian roulette, next-event
+// estimation (environment, then area light) and BSDF sampling.
+// Texel float offsets of the shaded vertex's textured parameters (-1 = constant).
+__device__ __forceinline__ int2 vertex_texels(const DevScene& S, const Shader& sh, f2 uv) {
+    int2 t = make_int2(-1, -1);
+    if (sh.m.base_tex >= 0) {
+        DevTexture tx = S.textures[sh.m.base_tex];
+        t.x = (int)(tx.offset + 3ull * (uint64_t)texel_index(tx, uv));
+    }
+    if (sh.m.kind == 1 && sh.m.rough_tex >= 0) {
+        DevTexture tx = S.textures[sh.m.rough_tex];
+        t.y = (int)(tx.offset + (uint64_t)texel_index(tx, uv));
+    }
+    return t;
+}
+
+// Gradient of a contribution `val` reached through the vertex's material value
+// for direction `in` (local): g_p = sum_c delta_c * val_c * dlog f_c / dp.
+__device__ __forceinline__ float4 contribution_gradient(const DevScene& S, const Shader& sh, f2 uv, int2 tex, f3 in,
+                                                        f3 val, f3 delta) {
+    D3 f = material_value_dual(S, sh, uv, in, tex.x >= 0, tex.y >= 0);
+    float dl[12];
+    dlog_of(f, dl);
+    float g[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) g[k] = delta.x * val.x * dl[k] + delta.y * val.y * dl[4 + k] + delta.z * val.z * dl[8 + k];
+    return make_float4(g[0], g[1], g[2], g[3]);
+}
+
+template <bool ADJ>
+__global__ void __launch_bounds__(256) k_shade(DevScene S, Wave w, int buf) {
+    const uint32_t count = w.counters[buf];
     const int nb = buf ^ 1;
-    int slot = w.q_slot[buf][i];
-    f3 dir = f3_of(w.q_d[buf][i]);
-    float4 h = w.hits[i];
-    Rng rng{w.st_rng[slot]};
-    float4 thr = w.st_thr[slot];
-    f3 prefix = f3_of(thr);
-    float prev_pdf = thr.w;
-    float4 pv = w.st_prev[slot];
-    int depth = __float_as_int(pv.w);
-    f3 prev_pos = f3_of(pv);
-    float4 L4 = w.st_L[slot];
-    f3 L = f3_of(L4);
-    const float nsr = (float)cfg.num_shadow_rays;
-    const int li = __float_as_int(h.w);
-
-    bool cont = false;
-    bool do_nee = false;
-    f3 new_dir = gray(0.0f);
-    Surf s;
-    Shader sh;
-    float q = 1.0f;
-    if (li < 0) {
-        // OnBackgroundHit (PathTracer.cs:384-400)
-        if (depth >= cfg.min_depth) {
-            float mis = 0.0f;
-            f3 contrib = gray(0.0f);
-            if (S.has_env && cfg.enable_bsdf_di) {
-                mis = 1.0f;
-                if (depth > 1) {
-                    float pdf_ne = env_pdf(S, dir) * nsr;
-                    mis = 1.0f / (1.0f + pdf_ne / prev_pdf);
+    for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += gridDim.x * blockDim.x) {
+        uint32_t i = base + threadIdx.x;
+        bool active = i < count;
+        uint32_t qs = active ? w.q_slot[buf][i] : 0u;
+        uint32_t slot = qs & kSlotMask, sc = qs >> kSlotBits;
+        const SceneRec& R = S.recs[sc];
+        const int nsr_i = R.num_shadow_rays;
+        const float nsr = (float)nsr_i;
+        bool cont = false, do_nee = false;
+        Rng rng{0};
+        f3 prefix = gray(0.0f), L = gray(0.0f), dir = gray(0.0f);
+        Surf s;
+        Shader sh;
+        float q = 1.0f;
+        int depth = 0;
+        f3 delta = gray(0.0f);
+        int2 vtex = make_int2(-1, -1);
+        Env E = env_of(S, R);
+        if (active) {
+            dir = f3_of(w.q_d[buf][i]);
+            float4 h = w.hits[i];
+            rng.state = w.st_rng[slot];
+            float4 thr = w.st_thr[slot];
+            prefix = f3_of(thr);
+            float prev_pdf = thr.w;
+            float4 pv = w.st_prev[slot];
+            depth = __float_as_int(pv.w);
+            f3 prev_pos = f3_of(pv);
+            L = f3_of(w.st_L[slot]);
+            if (ADJ) {
+                // the previous vertex's next-event rays have resolved: snapshot A_{depth-1}
+                uint32_t pix = w.st_pix[slot] & kSlotMask;
+                const float* dl = S.adj_image + R.fb_off + 3ull * pix;
+                delta = mk3(dl[0], dl[1], dl[2]) * R.adj_scale;
+                if (depth >= 2) {
+                    f3 A = L + f3_of(w.st_Le[slot]) + f3_of(w.st_La[slot]);
+                    w.ad_A[(size_t)(depth - 2) * w.capacity + slot] = f4_of(A, 1.0f);
                 }
-                contrib = env_radiance(S, dir);
             }
-            L = L + prefix * mis * contrib;
-        }
-    } else {
-        s = surface(S, li, h.y, h.z);
-        make_shader(S, s, -dir, &sh);
-        int ei = S.tri_emitter[li];
-        if (ei >= 0 && depth >= cfg.min_depth) {
-            // OnLightHit (PathTracer.cs:402-421)
-            DevEmitter e = S.emitters[ei];
-            float mis = 1.0f;
-            if (depth > 1) {
-                float jac = area_to_solid_angle(prev_pos, s.pos, s.ng);
-                float pdf_ne = e.inv_area / (float)S.num_emitters * nsr / jac;
-                float ratio = pdf_ne / prev_pdf;
-                mis = 1.0f / (ratio + 1.0f);
-                if (!cfg.enable_bsdf_di) mis = 0.0f;
-            }
-            // DiffuseEmitter.EmittedRadiance: one-sided on the shading normal
-            f3 em = dot(s.ns, -dir) < 0.0f ? gray(0.0f) : mk3(e.r, e.g, e.b);
-            L = L + prefix * mis * em;
-        }
-        // ComputeSurvivalProbability (PathTracer.cs:277-282); ApproxThroughput == PrefixWeight here
-        q = depth > 4 ? clampf(average(prefix), 0.05f, 0.95f) : 1.0f;
-        bool stop = rng.next_float() > q || depth == cfg.max_depth;
-        do_nee = !stop && depth + 1 >= cfg.min_depth;
-        cont = !stop;
-    }
-
-    // Next-event estimation (PathTracer.cs:356-364): per shadow ray, environment then area light.
-    for (int k = 0; k < cfg.num_shadow_rays; ++k) {
-        // PerformBackgroundNextEvent (PathTracer.cs:423-452)
-        bool env_ray = false;
-        float4 eo, ed, ec;
-        if (do_nee && S.has_env) {
-            EnvSample es = env_sample(S, rng.next_float2());
-            f3 bc;
-            float pdf_bsdf;
-            eval_cos_pdf(sh, es.dir, &bc, &pdf_bsdf);
-            if (!(pdf_bsdf == 0.0f || es.pdf == 0.0f)) {
-                float mis = cfg.enable_bsdf_di ? 1.0f / (1.0f + pdf_bsdf / (es.pdf * nsr)) : 1.0f;
-                f3 contrib = es.weight * bc / nsr;
-                f3 val = prefix * (mis * contrib) / q;
-                env_ray = true;
-                eo = f4_of(offset_point(s, es.dir), s.err);
-                ed = f4_of(es.dir, INFINITY);
-                ec = f4_of(val, __int_as_float(slot << 1));
+            const int li = __float_as_int(h.w);
+            if (li < 0) {
+                // OnBackgroundHit (PathTracer.cs:384-400)
+                if (depth >= R.min_depth) {
+                    float mis = 0.0f;
+                    f3 contrib = gray(0.0f);
+                    if (R.has_env && R.enable_bsdf_di) {
+                        mis = 1.0f;
+                        if (depth > 1) {
+                            float pdf_ne = env_pdf(E, dir) * nsr;
+                            mis = 1.0f / (1.0f + pdf_ne / prev_pdf);
+                        }
+                        contrib = env_radiance(E, dir);
+                    }
+                    L = L + prefix * mis * contrib;
+                }
             } else {
-                env_ray = true;  // LeavesScene is still traced, contributes zero
-                eo = f4_of(offset_point(s, es.dir), s.err);
-                ed = f4_of(es.dir, INFINITY);
-                ec = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(slot << 1));
+                s = surface(S, li, h.y, h.z);
+                make_shader(S, s, -dir, &sh);
+                if (ADJ) vtex = vertex_texels(S, sh, s.uv);
+                int ei = S.tri_emitter[li];
+                if (ei >= 0 && depth >= R.min_depth) {
+                    // OnLightHit (PathTracer.cs:402-421)
+                    DevEmitter e = S.emitters[R.emitter_off + ei];
+                    float mis = 1.0f;
+                    if (depth > 1) {
+                        float jac = area_to_solid_angle(prev_pos, s.pos, s.ng);
+                        float pdf_ne = e.inv_area / (float)R.num_emitters * nsr / jac;
+                        float ratio = pdf_ne / prev_pdf;
+                        mis = 1.0f / (ratio + 1.0f);
+                        if (!R.enable_bsdf_di) mis = 0.0f;
+                    }
+                    // DiffuseEmitter.EmittedRadiance: one-sided on the shading normal
+                    f3 em = dot(s.ns, -dir) < 0.0f ? gray(0.0f) : mk3(e.r, e.g, e.b);
+                    L = L + prefix * mis * em;
+                }
+                // ComputeSurvivalProbability (PathTracer.cs:277-282); ApproxThroughput == PrefixWeight
+                q = depth > 4 ? clampf(average(prefix), 0.05f, 0.95f) : 1.0f;
+                bool stop = rng.next_float() > q || depth == R.max_depth;
+                do_nee = !stop && depth + 1 >= R.min_depth;
+                cont = !stop;
             }
         }
-        uint32_t e_idx = wave_append(&w.counters[kCntShadow], env_ray);
-        if (env_ray) {
-            w.s_o[e_idx] = eo;
-            w.s_d[e_idx] = ed;
-            w.s_c[e_idx] = ec;
-        }
-        // PerformNextEventEstimation (PathTracer.cs:454-501)
-        bool area_ray = false;
-        float4 ao, ad, ac;
-        if (do_nee && S.num_emitters > 0) {
-            int idx = (int)rng.next_int((uint32_t)S.num_emitters);
-            DevEmitter e = S.emitters[idx];
-            float sel = 1.0f / (float)S.num_emitters;
-            f2 b = to_uniform_triangle(rng.next_float2());
-            Surf lp = surface(S, e.tri, b.x, b.y);
-            f3 l2s = normalize(s.pos - lp.pos);
-            f3 em = dot(lp.ns, l2s) < 0.0f ? gray(0.0f) : mk3(e.r, e.g, e.b);
-            float jac = area_to_solid_angle(s.pos, lp.pos, lp.ng);
-            f3 bc;
-            float pdf_bsdf_sa;
-            eval_cos_pdf(sh, -l2s, &bc, &pdf_bsdf_sa);
-            float pdf_ne = e.inv_area * sel * nsr;
-            float pdf_bsdf = pdf_bsdf_sa * jac;
-            f3 val = gray(0.0f);
-            if (jac != 0.0f) {
-                float ratio = pdf_bsdf / pdf_ne;
-                float mis = cfg.enable_bsdf_di ? 1.0f / (ratio + 1.0f) : 1.0f;
-                float pdf = e.inv_area / jac * sel * nsr;
-                f3 contrib = em / pdf * bc;
-                val = prefix * (mis * contrib) / q;
-            }
-            f3 dd = lp.pos - s.pos;
-            f3 p0 = offset_point(s, dd);
-            f3 p1 = offset_point(lp, -dd);
-            area_ray = true;
-            ao = f4_of(p0, 1e-5f);
-            ad = f4_of(p1 - p0, 1.0f - 1e-5f);
-            ac = f4_of(val, __int_as_float((slot << 1) | 1));
-        }
-        uint32_t a_idx = wave_append(&w.counters[kCntShadow], area_ray);
-        if (area_ray) {
-            w.s_o[a_idx] = ao;
-            w.s_d[a_idx] = ad;
-            w.s_c[a_idx] = ac;
-        }
-    }
 
-    // SampleDirection (PathTracer.cs:526-530) and the path-state update (:367-378)
-    float4 no, nd;
-    if (cont) {
-        float u_comp = rng.next_float();
-        f2 u_dir = rng.next_float2();
-        BsdfSample bs = sample_bsdf(sh, u_comp, u_dir);
-        if (bs.pdf == 0.0f || is_zero(bs.weight)) {
-            cont = false;
-        } else {
-            new_dir = bs.dir;
-            float sign = dot(bs.dir, s.ng) < 0.0f ? -1.0f : 1.0f;
-            no = f4_of(s.pos + sign * s.err * s.ng, s.err);
-            nd = f4_of(bs.dir, INFINITY);
-            prefix = prefix * (bs.weight / q);
-            w.st_thr[slot] = f4_of(prefix, bs.pdf * q);
-            w.st_prev[slot] = f4_of(s.pos, __int_as_float(depth + 1));
+        // Next-event estimation (PathTracer.cs:356-364): per shadow ray, environment
+        // then area light. A shadow ray whose contribution is exactly zero cannot
+        // change the estimate and is not traced; it still counts as an attempt.
+        // The loop bound is uniform over the wave (lanes of other scenes with fewer
+        // shadow rays simply do not take part).
+        int max_nsr = nsr_i;
+        for (int off = 32; off > 0; off >>= 1) max_nsr = max(max_nsr, __shfl_xor(max_nsr, off));
+        for (int k = 0; k < max_nsr; ++k) {
+            const bool nee_k = do_nee && k < nsr_i;
+            // PerformBackgroundNextEvent (PathTracer.cs:423-452)
+            bool env_ray = false;
+            float4 eo, ed, ec, eg = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (nee_k && R.has_env) {
+                EnvSample es = env_sample(E, rng.next_float2());
+                f3 bc;
+                float pdf_bsdf;
+                eval_cos_pdf(sh, es.dir, &bc, &pdf_bsdf);
+                if (!(pdf_bsdf == 0.0f || es.pdf == 0.0f)) {
+                    float mis = R.enable_bsdf_di ? 1.0f / (1.0f + pdf_bsdf / (es.pdf * nsr)) : 1.0f;
+                    f3 contrib = es.weight * bc / nsr;
+                    f3 val = prefix * (mis * contrib) / q;
+                    env_ray = !is_zero(val);
+                    eo = f4_of(offset_point(s, es.dir), s.err);
+                    ed = f4_of(es.dir, INFINITY);
+                    ec = f4_of(val, __int_as_float((int)shadow_tag(slot, sc, 0)));
+                    if (ADJ && env_ray && (vtex.x >= 0 || vtex.y >= 0))
+                        eg = contribution_gradient(S, sh, s.uv, vtex, sh.f.to_local(es.dir), val, delta);
+                }
+            }
+            wave_count(&w.stats[kStatShadow], nee_k && R.has_env);
+            uint32_t e_idx = wave_append(&w.counters[kCntShadow], env_ray);
+            if (env_ray) {
+                w.s_o[e_idx] = eo;
+                w.s_d[e_idx] = ed;
+                w.s_c[e_idx] = ec;
+                if (ADJ) {
+                    w.s_g[e_idx] = eg;
+                    w.s_t[e_idx] = vtex;
+                }
+            }
+            // PerformNextEventEstimation (PathTracer.cs:454-501)
+            bool area_ray = false;
+            float4 ao, ad, ac, ag = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (nee_k && R.num_emitters > 0) {
+                int idx = (int)rng.next_int((uint32_t)R.num_emitters);
+                DevEmitter e = S.emitters[R.emitter_off + idx];
+                float sel = 1.0f / (float)R.num_emitters;
+                f2 b = to_uniform_triangle(rng.next_float2());
+                Surf lp = surface(S, e.tri, b.x, b.y);
+                f3 l2s = normalize(s.pos - lp.pos);
+                f3 em = dot(lp.ns, l2s) < 0.0f ? gray(0.0f) : mk3(e.r, e.g, e.b);
+                float jac = area_to_solid_angle(s.pos, lp.pos, lp.ng);
+                f3 bc;
+                float pdf_bsdf_sa;
+                eval_cos_pdf(sh, -l2s, &bc, &pdf_bsdf_sa);
+                float pdf_ne = e.inv_area * sel * nsr;
+                float pdf_bsdf = pdf_bsdf_sa * jac;
+                f3 val = gray(0.0f);
+                if (jac != 0.0f) {
+                    float ratio = pdf_bsdf / pdf_ne;
+                    float mis = R.enable_bsdf_di ? 1.0f / (ratio + 1.0f) : 1.0f;
+                    float pdf = e.inv_area / jac * sel * nsr;
+                    f3 contrib = em / pdf * bc;
+                    val = prefix * (mis * contrib) / q;
+                }
+                f3 dd = lp.pos - s.pos;
+                f3 p0 = offset_point(s, dd);
+                f3 p1 = offset_point(lp, -dd);
+                area_ray = !is_zero(val);
+                if (ADJ && area_ray && (vtex.x >= 0 || vtex.y >= 0))
+                    ag = contribution_gradient(S, sh, s.uv, vtex, sh.f.to_local(-l2s), val, delta);
+                ao = f4_of(p0, 1e-5f);
+                ad = f4_of(p1 - p0, 1.0f - 1e-5f);
+                ac = f4_of(val, __int_as_float((int)shadow_tag(slot, sc, 1)));
+            }
+            wave_count(&w.stats[kStatShadow], nee_k && R.num_emitters > 0);
+            uint32_t a_idx = wave_append(&w.counters[kCntShadow], area_ray);
+            if (area_ray) {
+                w.s_o[a_idx] = ao;
+                w.s_d[a_idx] = ad;
+                w.s_c[a_idx] = ac;
+                if (ADJ) {
+                    w.s_g[a_idx] = ag;
+                    w.s_t[a_idx] = vtex;
+                }
+            }
+        }
+
+        // SampleDirection (PathTracer.cs:526-530) and the path-state update (:367-378)
+        float4 no, nd;
+        if (cont) {
+            float u_comp = rng.next_float();
+            f2 u_dir = rng.next_float2();
+            BsdfSample bs = sample_bsdf(sh, u_comp, u_dir);
+            if (bs.pdf == 0.0f || is_zero(bs.weight)) {
+                cont = false;
+            } else {
+                no = f4_of(offset_point(s, bs.dir), s.err);
+                nd = f4_of(bs.dir, INFINITY);
+                prefix = prefix * (bs.weight / q);
+                w.st_thr[slot] = f4_of(prefix, bs.pdf * q);
+                w.st_prev[slot] = f4_of(s.pos, __int_as_float(depth + 1));
+                if (ADJ) {
+                    // d log of the sampled BSDF weight (the sampling pdf is detached)
+                    size_t rec = (size_t)(depth - 1) * w.capacity + slot;
+                    w.ad_tex[rec] = vtex;
+                    if (vtex.x >= 0 || vtex.y >= 0) {
+                        D3 f = material_value_dual(S, sh, s.uv, sh.f.to_local(bs.dir), vtex.x >= 0, vtex.y >= 0);
+                        float dl[12];
+                        dlog_of(f, dl);
+                        w.ad_dlog[rec * 3 + 0] = make_float4(dl[0], dl[1], dl[2], dl[3]);
+                        w.ad_dlog[rec * 3 + 1] = make_float4(dl[4], dl[5], dl[6], dl[7]);
+                        w.ad_dlog[rec * 3 + 2] = make_float4(dl[8], dl[9], dl[10], dl[11]);
+                    }
+                }
+            }
+        }
+        uint32_t n_idx = wave_append(&w.counters[nb], cont);
+        if (cont) {
+            w.q_o[nb][n_idx] = no;
+            w.q_d[nb][n_idx] = nd;
+            w.q_slot[nb][n_idx] = qs;
+        }
+        if (active) {
+            w.st_rng[slot] = rng.state;
+            w.st_L[slot] = f4_of(L, 0.0f);
         }
     }
-    uint32_t n_idx = wave_append(&w.counters[kCntNext], cont);
-    if (cont) {
-        w.q_o[nb][n_idx] = no;
-        w.q_d[nb][n_idx] = nd;
-        w.q_slot[nb][n_idx] = slot;
-    }
-    w.st_rng[slot] = rng.state;
-    w.st_L[slot] = f4_of(L, 0.0f);
-    (void)new_dir;
 }
 
-// FrameBuffer running mean (SeeSharp/Images/FrameBuffer.cs:89-160): at
-// iteration n the image is scaled by (n-1)/n and the finite estimate / n added.
-__global__ void __launch_bounds__(256) k_accumulate(Wave w, float* image, uint32_t num_pixels,
-                                                    uint32_t first_iter, uint32_t num_iters) {
-    uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pix >= num_pixels) return;
-    f3 img = mk3(image[3 * pix], image[3 * pix + 1], image[3 * pix + 2]);
-    uint32_t nans = 0;
-    for (uint32_t s = 0; s < num_iters; ++s) {
-        uint32_t slot = s * num_pixels + pix;
+// Path-replay adjoint, per path after its last bounce: every vertex k that
+// continued passes the contributions collected after it, A_final - A_k, through
+// its sampled BSDF weight; d/dp of that sum is (A_final - A_k) * dlog f_k / dp.
+// Paths whose estimate is not finite are skipped, as the framebuffer skips them.
+__global__ void __launch_bounds__(256) k_adjoint_finalize(DevScene S, Wave w) {
+    const WaveDesc d = cur_wave(w);
+    for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < d.count; slot += gridDim.x * blockDim.x) {
+        f3 Af = f3_of(w.st_L[slot]) + f3_of(w.st_Le[slot]) + f3_of(w.st_La[slot]);
+        if (!isfinite(average(Af))) continue;
+        uint32_t tag = w.st_pix[slot];
+        const SceneRec& R = S.recs[tag >> kSlotBits];
+        const float* dl = S.adj_image + R.fb_off + 3ull * (tag & kSlotMask);
+        f3 delta = mk3(dl[0], dl[1], dl[2]) * R.adj_scale;
+        int depth = __float_as_int(w.st_prev[slot].w);
+        for (int k = 1; k < depth && k <= (int)w.ad_depth; ++k) {
+            size_t rec = (size_t)(k - 1) * w.capacity + slot;
+            int2 t = w.ad_tex[rec];
+            if (t.x < 0 && t.y < 0) continue;
+            f3 Sk = Af - f3_of(w.ad_A[rec]);
+            float4 a = w.ad_dlog[rec * 3], b = w.ad_dlog[rec * 3 + 1], c = w.ad_dlog[rec * 3 + 2];
+            float wx = delta.x * Sk.x, wy = delta.y * Sk.y, wz = delta.z * Sk.z;
+            float g0 = wx * a.x + wy * b.x + wz * c.x;
+            float g1 = wx * a.y + wy * b.y + wz * c.y;
+            float g2 = wx * a.z + wy * b.z + wz * c.z;
+            float g3 = wx * a.w + wy * b.w + wz * c.w;
+            if (t.x >= 0) {
+                atomicAdd(&S.grad[t.x], g0);
+                atomicAdd(&S.grad[t.x + 1], g1);
+                atomicAdd(&S.grad[t.x + 2], g2);
+            }
+            if (t.y >= 0) atomicAdd(&S.grad[t.y], g3);
+        }
+    }
+}
+
+// FrameBuffer running mean of one scene (SeeSharp/Images/FrameBuffer.cs:89-160):
+// at iteration n the image is scaled by (n-1)/n and the finite estimate / n added.
+// The wave holds whole iterations, slot = iteration * W*H + pixel.
+__global__ void __launch_bounds__(256) k_accumulate_mean(Wave w, float* image, uint32_t num_pixels) {
+    const WaveDesc d = cur_wave(w);
+    for (uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x; pix < num_pixels; pix += gridDim.x * blockDim.x) {
+        f3 img = mk3(image[3 * pix], image[3 * pix + 1], image[3 * pix + 2]);
+        uint32_t nans = 0;
+        for (uint32_t s = 0; s < d.iters; ++s) {
+            uint32_t slot = s * num_pixels + pix;
+            f3 L = f3_of(w.st_L[slot]) + f3_of(w.st_Le[slot]) + f3_of(w.st_La[slot]);
+            float n = (float)(d.first_iter + s + 1);
+            if (d.first_iter + s + 1 > 1) img = img * ((n - 1.0f) / n);
+            if (isfinite(average(L))) img = img + L / n;
+            else ++nans;
+        }
+        image[3 * pix] = img.x;
+        image[3 * pix + 1] = img.y;
+        image[3 * pix + 2] = img.z;
+        if (nans) atomicAdd(&w.stats[kStatNan], (unsigned long long)nans);
+    }
+}
+
+// Batch accumulation: every finite path estimate is added to its scene's sum
+// image (divided by the scene's sample count after the last wave).
+__global__ void __launch_bounds__(256) k_accumulate_sum(DevScene S, Wave w, float* pool) {
+    const WaveDesc d = cur_wave(w);
+    for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < d.count; slot += gridDim.x * blockDim.x) {
         f3 L = f3_of(w.st_L[slot]) + f3_of(w.st_Le[slot]) + f3_of(w.st_La[slot]);
-        float n = (float)(first_iter + s + 1);
-        if (first_iter + s + 1 > 1) img = img * ((n - 1.0f) / n);
-        if (isfinite(average(L))) img = img + L / n;
-        else ++nans;
+        uint32_t tag = w.st_pix[slot];
+        if (!isfinite(average(L))) {
+            atomicAdd(&w.stats[kStatNan], 1ull);
+            continue;
+        }
+        float* px = pool + S.recs[tag >> kSlotBits].fb_off + 3ull * (tag & kSlotMask);
+        atomicAdd(px, L.x);
+        atomicAdd(px + 1, L.y);
+        atomicAdd(px + 2, L.z);
     }
-    image[3 * pix] = img.x;
-    image[3 * pix + 1] = img.y;
-    image[3 * pix + 2] = img.z;
-    if (nans) atomicAdd(&w.stats[kStatNan], (unsigned long long)nans);
 }
 
-__global__ void __launch_bounds__(kTraceBlock) k_trace_closest(DevScene S, const float* rays, uint64_t n,
+__global__ void k_scale(float* data, uint64_t n, float factor) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        data[i] *= factor;
+}
+
+__global__ void __launch_bounds__(kTraceBlock) k_trace_closest(DevScene S, int scene, const float* rays, uint64_t n,
                                                                float4* out) {
-    __shared__ int stack[kMaxBvhDepth * kTraceBlock];
+    __shared__ int stack_node[kStackLds * kTraceBlock];
+    __shared__ float stack_t[kStackLds * kTraceBlock];
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float* r = rays + i * 8;
-    HitOut h = traverse<false>(S, mk3(r[0], r[1], r[2]), mk3(r[4], r[5], r[6]), r[3], r[7], stack + threadIdx.x);
+    const SceneRec& R = S.recs[scene];
+    HitOut h = traverse<false>(S, R.root, R.abs_max, mk3(r[0], r[1], r[2]), mk3(r[4], r[5], r[6]), r[3], r[7],
+                               stack_node + threadIdx.x, stack_t + threadIdx.x);
     out[i] = h.li >= 0 ? make_float4(h.t, h.u, h.v, __int_as_float(h.gid))
                        : make_float4(INFINITY, 0.0f, 0.0f, __int_as_float(-1));
 }
 
-__global__ void __launch_bounds__(kTraceBlock) k_trace_occluded(DevScene S, const float* rays, uint64_t n,
-                                                                uint8_t* out) {
-    __shared__ int stack[kMaxBvhDepth * kTraceBlock];
+__global__ void __launch_bounds__(kTraceBlock) k_trace_occluded(DevScene S, int scene, const float* rays,
+                                                                uint64_t n, uint8_t* out) {
+    __shared__ int stack_node[kStackLds * kTraceBlock];
+    __shared__ float stack_t[kStackLds * kTraceBlock];
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float* r = rays + i * 8;
-    HitOut h = traverse<true>(S, mk3(r[0], r[1], r[2]), mk3(r[4], r[5], r[6]), r[3], r[7], stack + threadIdx.x);
+    const SceneRec& R = S.recs[scene];
+    HitOut h = traverse<true>(S, R.root, R.abs_max, mk3(r[0], r[1], r[2]), mk3(r[4], r[5], r[6]), r[3], r[7],
+                              stack_node + threadIdx.x, stack_t + threadIdx.x);
     out[i] = h.li >= 0 ? 1 : 0;
 }
 
-__global__ void __launch_bounds__(256) k_primary_rays(DevScene S, uint32_t base_seed, uint32_t iteration,
-                                                      float* rays) {
+__global__ void __launch_bounds__(256) k_primary_rays(DevScene S, int scene, uint32_t num_pixels, uint32_t base_seed,
+                                                      uint32_t iteration, float* rays) {
     uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pix >= (uint32_t)(S.width * S.height)) return;
-    uint32_t row = pix / (uint32_t)S.width, col = pix - row * (uint32_t)S.width;
+    if (pix >= num_pixels) return;
+    const DevCamera& c = S.recs[scene].cam;
+    uint32_t row = pix / (uint32_t)c.width, col = pix - row * (uint32_t)c.width;
     Rng rng{hash_seed(base_seed, pix, iteration)};
     f2 off = rng.next_float2();
     f3 o, d;
-    camera_ray(S.cam, (float)col + off.x, (float)row + off.y, &o, &d);
+    camera_ray(c, (float)col + off.x, (float)row + off.y, &o, &d);
     float* r = rays + (uint64_t)pix * 8;
     r[0] = o.x, r[1] = o.y, r[2] = o.z, r[3] = 0.0f;
     r[4] = d.x, r[5] = d.y, r[6] = d.z, r[7] = INFINITY;
@@ -509,44 +788,59 @@ __global__ void __launch_bounds__(256) k_primary_rays(DevScene S, uint32_t base_
 // ------------------------------------------------------------------ launch --
 static inline uint32_t blocks(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
 
-void launch_generate(hipStream_t s, const DevScene& sc, const Wave& w, uint32_t base_seed, uint32_t first_iter,
-                     uint32_t num_iters) {
-    uint32_t np = (uint32_t)(sc.width * sc.height);
-    uint32_t total = np * num_iters;
-    hipLaunchKernelGGL(k_generate, dim3(blocks(total, 256)), dim3(256), 0, s, sc, w, base_seed, first_iter, np,
-                       total);
+void launch_wave_begin(hipStream_t s, const Wave& w) { hipLaunchKernelGGL(k_wave_begin, dim3(1), dim3(1), 0, s, w); }
+void launch_wave_end(hipStream_t s, const Wave& w) { hipLaunchKernelGGL(k_wave_end, dim3(1), dim3(1), 0, s, w); }
+void launch_bounce_reset(hipStream_t s, const Wave& w, int buf) {
+    hipLaunchKernelGGL(k_bounce_reset, dim3(1), dim3(1), 0, s, w, buf ^ 1);
 }
-void launch_extend(hipStream_t s, const DevScene& sc, const Wave& w, int buf, uint32_t count) {
-    if (!count) return;
-    hipLaunchKernelGGL(k_extend, dim3(blocks(count, kTraceBlock)), dim3(kTraceBlock), 0, s, sc, w.q_o[buf],
-                       w.q_d[buf], w.hits, count, w.stats);
+void launch_generate(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc) {
+    hipLaunchKernelGGL(k_generate, dim3(lc.shade_blocks), dim3(256), 0, s, sc, w);
 }
-void launch_shade(hipStream_t s, const DevScene& sc, const Wave& w, const DevSettings& cfg, int buf,
-                  uint32_t count) {
-    if (!count) return;
-    hipLaunchKernelGGL(k_shade, dim3(blocks(count, 256)), dim3(256), 0, s, sc, w, cfg, buf, count);
+void launch_extend(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int buf) {
+    hipLaunchKernelGGL(k_extend, dim3(lc.trace_blocks), dim3(kTraceBlock), 0, s, sc, w, buf);
 }
-void launch_shadow(hipStream_t s, const DevScene& sc, const Wave& w, uint32_t count) {
-    if (!count) return;
-    hipLaunchKernelGGL(k_shadow, dim3(blocks(count, kTraceBlock)), dim3(kTraceBlock), 0, s, sc, w.s_o, w.s_d,
-                       w.s_c, w.st_Le, w.st_La, count, w.stats);
+void launch_shade(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int buf) {
+    hipLaunchKernelGGL(k_shade<false>, dim3(lc.shade_blocks), dim3(256), 0, s, sc, w, buf);
 }
-void launch_accumulate(hipStream_t s, const Wave& w, float* image, uint32_t num_pixels, uint32_t first_iter,
-                       uint32_t num_iters) {
-    hipLaunchKernelGGL(k_accumulate, dim3(blocks(num_pixels, 256)), dim3(256), 0, s, w, image, num_pixels,
-                       first_iter, num_iters);
+void launch_shadow(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc) {
+    hipLaunchKernelGGL(k_shadow<false>, dim3(lc.trace_blocks), dim3(kTraceBlock), 0, s, sc, w);
 }
-void launch_trace_closest(hipStream_t s, const DevScene& sc, const float* rays, uint64_t n, float4* out) {
+void launch_shade_adjoint(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int buf) {
+    hipLaunchKernelGGL(k_shade<true>, dim3(lc.shade_blocks), dim3(256), 0, s, sc, w, buf);
+}
+void launch_shadow_adjoint(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc) {
+    hipLaunchKernelGGL(k_shadow<true>, dim3(lc.trace_blocks), dim3(kTraceBlock), 0, s, sc, w);
+}
+void launch_adjoint_finalize(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc) {
+    hipLaunchKernelGGL(k_adjoint_finalize, dim3(lc.shade_blocks), dim3(256), 0, s, sc, w);
+}
+void launch_accumulate_mean(hipStream_t s, const Wave& w, float* image, uint32_t num_pixels) {
+    hipLaunchKernelGGL(k_accumulate_mean, dim3(std::min(blocks(num_pixels, 256), 8192u)), dim3(256), 0, s, w, image,
+                       num_pixels);
+}
+void launch_accumulate_sum(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, float* pool) {
+    hipLaunchKernelGGL(k_accumulate_sum, dim3(lc.shade_blocks), dim3(256), 0, s, sc, w, pool);
+}
+void launch_scale(hipStream_t s, float* data, uint64_t n, float factor) {
     if (!n) return;
-    hipLaunchKernelGGL(k_trace_closest, dim3(blocks(n, kTraceBlock)), dim3(kTraceBlock), 0, s, sc, rays, n, out);
+    hipLaunchKernelGGL(k_scale, dim3((uint32_t)std::min<uint64_t>(blocks(n, 256), 4096)), dim3(256), 0, s, data, n,
+                       factor);
 }
-void launch_trace_occluded(hipStream_t s, const DevScene& sc, const float* rays, uint64_t n, uint8_t* out) {
+void launch_trace_closest(hipStream_t s, const DevScene& sc, int scene, const float* rays, uint64_t n, float4* out) {
     if (!n) return;
-    hipLaunchKernelGGL(k_trace_occluded, dim3(blocks(n, kTraceBlock)), dim3(kTraceBlock), 0, s, sc, rays, n, out);
+    hipLaunchKernelGGL(k_trace_closest, dim3(blocks(n, kTraceBlock)), dim3(kTraceBlock), 0, s, sc, scene, rays, n,
+                       out);
 }
-void launch_primary_rays(hipStream_t s, const DevScene& sc, uint32_t base_seed, uint32_t iteration, float* rays) {
-    uint32_t np = (uint32_t)(sc.width * sc.height);
-    hipLaunchKernelGGL(k_primary_rays, dim3(blocks(np, 256)), dim3(256), 0, s, sc, base_seed, iteration, rays);
+void launch_trace_occluded(hipStream_t s, const DevScene& sc, int scene, const float* rays, uint64_t n,
+                           uint8_t* out) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_trace_occluded, dim3(blocks(n, kTraceBlock)), dim3(kTraceBlock), 0, s, sc, scene, rays, n,
+                       out);
+}
+void launch_primary_rays(hipStream_t s, const DevScene& sc, int scene, uint32_t num_pixels, uint32_t base_seed,
+                         uint32_t iteration, float* rays) {
+    hipLaunchKernelGGL(k_primary_rays, dim3(blocks(num_pixels, 256)), dim3(256), 0, s, sc, scene, num_pixels,
+                       base_seed, iteration, rays);
 }
 
 }  // namespace sspt

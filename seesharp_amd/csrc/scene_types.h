@@ -1,18 +1,23 @@
 // scene_types.h -- device-resident scene layout (HBM), shared by host and device code.
 //
+// One device "world" holds one or more scenes (a batch); all scenes share
+// pooled arrays, and a per-scene record gives each its root node, emitter
+// range, environment map, camera, settings and framebuffer.
+//
 // Triangles are stored once, in BVH leaf order, so the index the traversal
 // returns addresses the shading data directly:
-//   tri_pos[3*i+k]  float4  vertex k position; tri_pos[3*i].w = global triangle id (bits)
+//   tri_pos[3*i+k]  float4  vertex k position; tri_pos[3*i].w = triangle id within the
+//                           world (scene offset + mesh offset + PrimId), int bits
 //   tri_aux[4*i+k]  float4  k<3: shading normal of vertex k (xyz), w = uv component
 //                           (w0 = u0, w1 = v0, w2 = u1); tri_aux[4*i+3] = (v1, u2, v2, info bits)
-//   info bits: 0..23 material id, bit 24 has normals, bit 25 has uvs
-//   tri_emitter[i]  int     emitter index in Scene.Emitters order, -1 if none
-// BVH: binary tree whose nodes hold both child boxes (64 B per node):
-//   node[4*n+0] = (L.lo.x, L.hi.x, L.lo.y, L.hi.y)
-//   node[4*n+1] = (R.lo.x, R.hi.x, R.lo.y, R.hi.y)
-//   node[4*n+2] = (L.lo.z, L.hi.z, R.lo.z, R.hi.z)
-//   node[4*n+3] = (left, right, left_count, right_count) as int bits;
-//                 count > 0: the child is a leaf of `count` triangles starting at index `left`/`right`.
+//   info bits: 0..23 material id (world-wide), bit 24 has normals, bit 25 has uvs
+//   tri_emitter[i]  int     emitter index within the scene's emitter range, -1 if none
+// BVH: 4-wide trees (one per scene, node indices world-wide), 128 B per node,
+// children in SoA float4s:
+//   node[8*n+0..5] = lo.x, hi.x, lo.y, hi.y, lo.z, hi.z of children 0..3
+//   node[8*n+6]    = child index per slot (int bits): device node, or first leaf triangle
+//   node[8*n+7]    = triangle count per slot (int bits): 0 = inner node, > 0 = leaf,
+//                    -1 = absent slot (box [+inf, +inf]^3, skipped by the traversal)
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -24,9 +29,20 @@ constexpr int kInfoMaterialMask = 0xFFFFFF;
 constexpr int kInfoHasNormals = 1 << 24;
 constexpr int kInfoHasUvs = 1 << 25;
 
-// BVH depth bound = traversal stack entries per lane (LDS).
+// Depth bound of the binary SAH build.
 constexpr int kMaxBvhDepth = 40;
+// Traversal stack per lane: the top kStackLds entries live in LDS, the rest in
+// a private (scratch) array; scene creation rejects trees that need more than
+// kStackTotal entries.
+constexpr int kStackLds = 12;
+constexpr int kStackTotal = 64;
 constexpr int kTraceBlock = 128;
+
+// Queue entries pack (scene, slot): slot in the low 24 bits, scene above.
+constexpr int kSlotBits = 24;
+constexpr uint32_t kSlotMask = (1u << kSlotBits) - 1u;
+constexpr int kMaxScenes = 128;
+constexpr uint32_t kMaxSlots = 1u << kSlotBits;
 
 struct DevMaterial {  // 64 B
     int kind;
@@ -45,7 +61,7 @@ struct DevTexture {
 };
 
 struct DevEmitter {  // 32 B
-    int tri;  // leaf-order triangle index
+    int tri;  // leaf-order triangle index (world-wide)
     float r, g, b;
     float inv_area;
     int pad[3];
@@ -58,6 +74,30 @@ struct DevCamera {
     int width, height;
 };
 
+// Per-scene record. The render settings and the path range are refreshed by
+// every render call.
+struct SceneRec {
+    DevCamera cam;
+    int root;            // root node of the scene's BVH
+    int emitter_off;     // first emitter of the scene in the emitter pool
+    int num_emitters;
+    int has_env;
+    int env_w, env_h;
+    uint32_t env_rgb_off;  // float4 index into the environment texel pool
+    uint32_t env_row_off;  // float index into the CDF pool (env_h entries)
+    uint32_t env_col_off;  // float index into the CDF pool (env_h * env_w entries)
+    float abs_max;         // largest vertex coordinate magnitude (ray slack scale)
+    uint32_t num_pixels;
+    // settings (PathTracerBase fields)
+    uint32_t base_seed;
+    int max_depth, min_depth, num_shadow_rays, enable_bsdf_di;
+    // current render
+    uint32_t first_iter;   // iteration index of the scene's first path in this render
+    float adj_scale;       // adjoint renders: 1 / number of iterations (d image / d sample)
+    uint64_t path_off;     // first world-wide path index of the scene in this render
+    uint64_t fb_off;       // float offset of the scene's W*H*3 framebuffer
+};
+
 struct DevScene {
     const float4* nodes;
     const float4* tri_pos;
@@ -67,30 +107,43 @@ struct DevScene {
     const DevTexture* textures;
     const float* texels;
     const DevEmitter* emitters;
-    int num_emitters;
-    int num_tris;
-    float abs_max;  // largest vertex coordinate magnitude (ray slack scale)
-    int has_env;
-    int env_w, env_h;
-    const float4* env_rgb;     // env_w * env_h, w unused
-    const float* env_row_cdf;  // env_h
-    const float* env_col_cdf;  // env_h * env_w
-    DevCamera cam;
-    int width, height;
+    const float4* env_rgb;  // pooled environment texels (w unused)
+    const float* env_cdf;   // pooled row and column CDFs
+    const SceneRec* recs;
+    const uint64_t* path_offsets;  // num_scenes + 1 world-wide path offsets of this render
+    const float* adj_image;        // adjoint renders: d loss / d image, framebuffer layout
+    float* grad;                   // adjoint renders: d loss / d texel, texel-pool layout
+    int num_scenes;
+    int num_nodes;
 };
 
-// Path-tracer settings as seen by the kernels.
-struct DevSettings {
-    uint32_t base_seed;
-    int max_depth, min_depth, num_shadow_rays, enable_bsdf_di;
+// Device-side per-wave work: the wave covers world-wide paths
+// [start, start + count); in running-mean mode it covers whole iterations
+// first_iter .. first_iter + iters - 1 of a single scene.
+struct WaveDesc {
+    uint64_t start;
+    uint32_t count;
+    uint32_t first_iter;
+    uint32_t iters;
+    uint32_t pad;
 };
 
-// Device-side counters. Index names.
+// Device-side counters.
 enum Counter {
-    kCntNext = 0,    // next extension queue length
-    kCntShadow = 1,  // shadow queue length
-    kCntNum = 2
+    kCntQueue0 = 0,  // extension queue 0 length
+    kCntQueue1 = 1,  // extension queue 1 length
+    kCntShadow = 2,  // shadow queue length
+    kCntWave = 3,    // index of the wave being rendered
+    kCntNum = 4
 };
-enum Stat { kStatRays = 0, kStatShadow = 1, kStatOccluded = 2, kStatHits = 3, kStatNan = 4, kStatNum = 5 };
+enum Stat {
+    kStatRays = 0,
+    kStatShadow = 1,        // next-event attempts (IsOccluded / LeavesScene calls)
+    kStatOccluded = 2,      // traced shadow rays that were blocked
+    kStatHits = 3,
+    kStatNan = 4,
+    kStatShadowTraced = 5,  // shadow rays traced (non-zero contribution)
+    kStatNum = 6
+};
 
 }  // namespace sspt

@@ -271,37 +271,48 @@ __device__ BsdfSample sample_bsdf(const Shader& sh, float pc, f2 pd) {
 }
 
 // ------------------------------------------------------------ environment --
+// The scene's environment map: texels and the RegularGrid2d CDFs live in the
+// world's pools at the offsets of its SceneRec.
+struct Env {
+    const float4* rgb;
+    const float* row_cdf;
+    const float* col_cdf;
+    int w, h;
+};
+__device__ __forceinline__ Env env_of(const DevScene& S, const SceneRec& R) {
+    return Env{S.env_rgb + R.env_rgb_off, S.env_cdf + R.env_row_off, S.env_cdf + R.env_col_off, R.env_w, R.env_h};
+}
 __device__ __forceinline__ f2 world_to_spherical(f3 dir) {
     dir = normalize(dir);
     f2 sp{atan2f(dir.z, dir.x), atan2f(sqrtf(dir.x * dir.x + dir.z * dir.z), dir.y)};
     if (sp.x < 0.0f) sp.x += kPi * 2.0f;
     return sp;
 }
-__device__ __forceinline__ f3 env_pixel(const DevScene& S, int col, int row) {
-    col = min(max(col, 0), S.env_w - 1);
-    row = min(max(row, 0), S.env_h - 1);
-    return f3_of(S.env_rgb[row * S.env_w + col]);
+__device__ __forceinline__ f3 env_pixel(const Env& E, int col, int row) {
+    col = min(max(col, 0), E.w - 1);
+    row = min(max(row, 0), E.h - 1);
+    return f3_of(E.rgb[row * E.w + col]);
 }
-__device__ __forceinline__ f3 env_radiance(const DevScene& S, f3 dir) {
+__device__ __forceinline__ f3 env_radiance(const Env& E, f3 dir) {
     f2 sp = world_to_spherical(dir);
     float px = sp.x / (2.0f * kPi), py = sp.y / kPi;
-    return env_pixel(S, to_int(px * (float)S.env_w), to_int(py * (float)S.env_h));
+    return env_pixel(E, to_int(px * (float)E.w), to_int(py * (float)E.h));
 }
 __device__ __forceinline__ float cdf_prob(const float* cdf, int i) {
     return i > 0 ? cdf[i] - cdf[i - 1] : cdf[i];
 }
 // RegularGrid2d.Pdf (SeeSharp/Sampling/RegularGrid2d.cs:39-47)
-__device__ __forceinline__ float grid_pdf(const DevScene& S, float x, float y) {
-    int row = min(max(to_int(y * (float)S.env_h), 0), S.env_h - 1);
-    int col = min(max(to_int(x * (float)S.env_w), 0), S.env_w - 1);
-    float p = cdf_prob(S.env_row_cdf, row);
+__device__ __forceinline__ float grid_pdf(const Env& E, float x, float y) {
+    int row = min(max(to_int(y * (float)E.h), 0), E.h - 1);
+    int col = min(max(to_int(x * (float)E.w), 0), E.w - 1);
+    float p = cdf_prob(E.row_cdf, row);
     if (p == 0.0f) return 0.0f;
-    p *= cdf_prob(S.env_col_cdf + (size_t)row * S.env_w, col);
-    return p * (float)(S.env_h * S.env_w);
+    p *= cdf_prob(E.col_cdf + (size_t)row * E.w, col);
+    return p * (float)(E.h * E.w);
 }
-__device__ __forceinline__ float env_pdf(const DevScene& S, f3 dir) {
+__device__ __forceinline__ float env_pdf(const Env& E, f3 dir) {
     f2 sp = world_to_spherical(dir);
-    return grid_pdf(S, sp.x / (2.0f * kPi), sp.y / kPi) / (sinf(sp.y) * kPi * kPi * 2.0f);
+    return grid_pdf(E, sp.x / (2.0f * kPi), sp.y / kPi) / (sinf(sp.y) * kPi * kPi * 2.0f);
 }
 // PiecewiseConstantPDF.Sample (SeeSharp/Sampling/PiecewiseConstant.cs:39-51): lower bound.
 __device__ __forceinline__ int lower_bound(const float* cdf, int n, float u) {
@@ -323,17 +334,17 @@ struct EnvSample {
     f3 weight;
 };
 // EnvironmentMap.SampleDirection (EnvironmentMap.cs:40-73)
-__device__ EnvSample env_sample(const DevScene& S, f2 u) {
-    int ri = min(lower_bound(S.env_row_cdf, S.env_h, u.y), S.env_h - 1);
-    float rlo = ri == 0 ? 0.0f : S.env_row_cdf[ri - 1];
-    float rrel = (u.y - rlo) / (S.env_row_cdf[ri] - rlo);
-    const float* ccdf = S.env_col_cdf + (size_t)ri * S.env_w;
-    int ci = min(lower_bound(ccdf, S.env_w, u.x), S.env_w - 1);
+__device__ EnvSample env_sample(const Env& E, f2 u) {
+    int ri = min(lower_bound(E.row_cdf, E.h, u.y), E.h - 1);
+    float rlo = ri == 0 ? 0.0f : E.row_cdf[ri - 1];
+    float rrel = (u.y - rlo) / (E.row_cdf[ri] - rlo);
+    const float* ccdf = E.col_cdf + (size_t)ri * E.w;
+    int ci = min(lower_bound(ccdf, E.w, u.x), E.w - 1);
     float clo = ci == 0 ? 0.0f : ccdf[ci - 1];
     float crel = (u.x - clo) / (ccdf[ci] - clo);
-    float px = ((float)ci + crel) / (float)S.env_w;
-    float py = ((float)ri + rrel) / (float)S.env_h;
-    float pdf = grid_pdf(S, px, py);
+    float px = ((float)ci + crel) / (float)E.w;
+    float py = ((float)ri + rrel) / (float)E.h;
+    float pdf = grid_pdf(E, px, py);
     float phi = px * 2.0f * kPi, theta = py * kPi;
     float sin_t = sinf(theta);
     EnvSample r;
@@ -346,7 +357,7 @@ __device__ EnvSample env_sample(const DevScene& S, f2 u) {
     }
     pdf /= jac;
     r.pdf = pdf;
-    r.weight = env_pixel(S, to_int(px * (float)S.env_w), to_int(py * (float)S.env_h)) / pdf;
+    r.weight = env_pixel(E, to_int(px * (float)E.w), to_int(py * (float)E.h)) / pdf;
     return r;
 }
 

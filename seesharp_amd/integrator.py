@@ -43,6 +43,13 @@ class Device:
     def __del__(self):
         self.close()
 
+    def set_wave_capacity(self, slots: int):
+        self.check(self.lib.sspt_set_wave_capacity(self.ptr, int(slots)))
+
+    def set_launch_config(self, trace_blocks: int = 0, shade_blocks: int = 0, use_graphs: bool = True):
+        self.check(self.lib.sspt_set_launch_config(self.ptr, int(trace_blocks), int(shade_blocks),
+                                                   1 if use_graphs else 0))
+
     # kernel timing with HIP events on the context stream
     def set_kernel_timing(self, on: bool):
         self.check(self.lib.sspt_set_kernel_timing(self.ptr, 1 if on else 0))
@@ -103,6 +110,37 @@ class GpuScene:
                                                       first_iteration, num_iterations, C.byref(stats)))
         return stats.as_dict()
 
+    # ---- differentiation (path-replay adjoint)
+    def texture_shapes(self):
+        """(height, width[, channels]) of each texture, in the order of the scene description."""
+        shapes = []
+        for k in range(self.desc.num_textures):
+            t = self.desc.textures[k]
+            shapes.append((t.height, t.width, t.channels) if t.channels > 1 else (t.height, t.width))
+        return shapes
+
+    def write_texture(self, index: int, texels: np.ndarray):
+        texels = np.ascontiguousarray(texels, np.float32)
+        self.device.check(self.device.lib.sspt_texture_write(self.device.ptr, self.ptr, index, abi.fptr(texels)))
+
+    def clear_gradients(self):
+        self.device.check(self.device.lib.sspt_gradient_clear(self.device.ptr, self.ptr))
+
+    def gradient(self, index: int) -> np.ndarray:
+        out = np.zeros(self.texture_shapes()[index], np.float32)
+        self.device.check(self.device.lib.sspt_gradient_read(self.device.ptr, self.ptr, index, abi.fptr(out)))
+        return out
+
+    def render_adjoint(self, settings: abi.PtSettings, dloss_dimage: np.ndarray, first_iteration: int,
+                       num_iterations: int) -> dict:
+        """Adds dLoss/dtexel for the image of iterations first..first+num-1 to the gradients."""
+        g = np.ascontiguousarray(dloss_dimage, np.float32).reshape(self.height, self.width, 3)
+        stats = abi.RayStats()
+        self.device.check(self.device.lib.sspt_render_adjoint(self.device.ptr, self.ptr, C.byref(settings),
+                                                              abi.fptr(g), first_iteration, num_iterations,
+                                                              C.byref(stats)))
+        return stats.as_dict()
+
     def trace_closest(self, rays: np.ndarray) -> np.ndarray:
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
         hits = (abi.Hit * rays.shape[0])()
@@ -124,6 +162,48 @@ class GpuScene:
         self.device.check(self.device.lib.sspt_primary_hits(self.device.ptr, self.ptr, base_seed, iteration,
                                                             abi.fptr(rays), hits))
         return rays, _hits(hits)
+
+
+class GpuBatch:
+    """Many scenes in one device world, rendered together (sspt_batch_create /
+    sspt_render_batch): waves are filled with paths of every scene, so small
+    scenes keep the GPU busy."""
+
+    def __init__(self, device: Device, scenes):
+        self.device = device
+        self.scenes = list(scenes)
+        descs = [s.to_desc() for s in self.scenes]
+        self._descs = (abi.SceneDesc * len(descs))(*descs)
+        h = C.c_void_p()
+        device.check(device.lib.sspt_batch_create(device.ptr, self._descs, len(descs), C.byref(h)))
+        self.ptr = h
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            self.device.lib.sspt_scene_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        self.close()
+
+    def info(self):
+        v = [C.c_uint64() for _ in range(4)]
+        self.device.check(self.device.lib.sspt_scene_info(self.ptr, *[C.byref(x) for x in v]))
+        return dict(zip(("num_triangles", "num_emitters", "num_bvh_nodes", "device_bytes"), (x.value for x in v)))
+
+    def render(self, settings) -> dict:
+        """settings: one abi.PtSettings per scene (total_spp iterations each)."""
+        arr = (abi.PtSettings * len(self.scenes))(*settings)
+        stats = abi.RayStats()
+        self.device.check(self.device.lib.sspt_render_batch(self.device.ptr, self.ptr, arr, C.byref(stats)))
+        return stats.as_dict()
+
+    def read_image(self, index: int) -> np.ndarray:
+        fb = self.scenes[index].FrameBuffer
+        img = np.zeros((fb.Height, fb.Width, 3), np.float32)
+        self.device.check(self.device.lib.sspt_framebuffer_read_scene(self.device.ptr, self.ptr, index,
+                                                                      abi.fptr(img)))
+        return img
 
 
 def _hits(hits):

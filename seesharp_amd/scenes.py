@@ -298,8 +298,9 @@ def textured_materials(res: int, seed: int, uniform: float | None):
 
 
 def large_scene(width: int = 1024, height: int = 1024, num_spheres: int = 400, grid: int = 512,
-                scene_seed: int = 1, with_env: bool = True) -> Scene:
-    """The 1 036 296-triangle scene (sceneSeed 1)."""
+                scene_seed: int = 1, with_env: bool = True, camera_pos=(0.0, 3.0, 9.0),
+                camera_target=(0.0, 0.5, 0.0)) -> Scene:
+    """The 1 036 296-triangle scene (sceneSeed 1) with the default arguments."""
     s = Scene()
     s.Name = "large"
     sv, sf = icosphere(3)  # 1280 faces
@@ -334,6 +335,36 @@ def large_scene(width: int = 1024, height: int = 1024, num_spheres: int = 400, g
         s.Emitters.extend(DiffuseEmitter.MakeFromMesh(lm, (20.0, 20.0, 20.0)))
     if with_env:
         s.Background = EnvironmentMap(sky_image())
-    s.Camera = PerspectiveCamera(create_look_at((0.0, 3.0, 9.0), (0.0, 0.5, 0.0), (0, 1, 0)), 40.0)
+    s.Camera = PerspectiveCamera(create_look_at(camera_pos, camera_target, (0, 1, 0)), 40.0)
     s.FrameBuffer = FrameBuffer(width, height)
     return s
+
+
+BATCH_RESOLUTIONS = (128, 192, 256, 384, 512)
+BATCH_SPP = (4, 16, 64)
+
+
+def batch_scenes(count: int = 64, seed: int = 1000):
+    """The batching workload: `count` independent scenes built like the large
+    scene (spheres + fBm ground + 4 lights + sky), scene i seeded 1000 + i, with
+    a triangle budget drawn log-uniformly from [1k, 200k] (half spheres, half
+    ground grid), a resolution from {128..512}^2, a sample count from {4, 16, 64}
+    and a random look-at camera. Returns (scenes, spp list)."""
+    out, spps = [], []
+    for i in range(count):
+        r = Rng.seeded(seed + i, 0, 0)
+        budget = math.exp(r.next_float(math.log(1000.0), math.log(200000.0)))
+        num_spheres = max(1, int(budget / 2 / 1280))
+        grid = max(8, int(math.sqrt(budget / 2 / 2)))
+        res = BATCH_RESOLUTIONS[min(4, int(r.next_float() * 5))]
+        spp = BATCH_SPP[min(2, int(r.next_float() * 3))]
+        ang = r.next_float(0.0, 2 * math.pi)
+        rad = r.next_float(6.0, 10.0)
+        height = r.next_float(1.0, 4.0)
+        target = (r.next_float(-1.0, 1.0), r.next_float(0.0, 1.0), r.next_float(-1.0, 1.0))
+        sc = large_scene(res, res, num_spheres=num_spheres, grid=grid, scene_seed=seed + i,
+                         camera_pos=(rad * math.cos(ang), height, rad * math.sin(ang)), camera_target=target)
+        sc.Name = f"batch{i}"
+        out.append(sc)
+        spps.append(spp)
+    return out, spps

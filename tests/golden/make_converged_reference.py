@@ -1,7 +1,10 @@
 """Writes tests/golden/large_scene_ref_4096spp.npz: the oracle's converged render of a
 seeded 8192-pixel subset of the large scene (1 036 296 triangles, 1024x1024, MaxDepth 8)
 at 4096 spp with BaseSeed 571298512 (the reference's own reference-render seed,
-SeeSharp/Experiments/SceneFromFile.cs:191), independent of the measured runs' seed.
+SeeSharp/Experiments/SceneFromFile.cs:191), independent of the measured runs' seed,
+and the same pixels rendered with the default BaseSeed 0xC030114 (the GPU runs' seed):
+the GPU must match the latter path for path, and both CPU renders together give the
+Monte Carlo noise floor of the relMSE between two independent 4096-spp images.
 
     python tests/golden/make_converged_reference.py
 """
@@ -23,13 +26,8 @@ NUM_PIXELS = 8192
 SEED = 571298512
 
 
-def main():
-    scene = scenes.large_scene(1024, 1024)
-    o = ol.OracleScene(scene)
-    pix = np.sort(np.random.default_rng(20261203).choice(1024 * 1024, NUM_PIXELS, replace=False)).astype(np.int32)
-    cfg = ol.settings(base_seed=SEED, total_spp=SPP, max_depth=8)
-    img = np.zeros((1024, 1024, 3), np.float32)
-    sq = np.zeros((NUM_PIXELS, 3), np.float64)
+def render(o, pix, seed):
+    cfg = ol.settings(base_seed=seed, total_spp=SPP, max_depth=8)
     t0 = time.time()
     # 64 chunks of 64 iterations; per-chunk means give a standard error per pixel
     chunk_means = []
@@ -43,11 +41,18 @@ def main():
         chunk_means.append(chunk.astype(np.float64))
         print(f"chunk {c + 1}/{SPP // 64} {time.time() - t0:.0f}s", flush=True)
     cm = np.stack(chunk_means)
-    mean = cm.mean(0)
-    se = cm.std(0, ddof=1) / np.sqrt(cm.shape[0])
+    return cm.mean(0), cm.std(0, ddof=1) / np.sqrt(cm.shape[0])
+
+
+def main():
+    scene = scenes.large_scene(1024, 1024)
+    o = ol.OracleScene(scene)
+    pix = np.sort(np.random.default_rng(20261203).choice(1024 * 1024, NUM_PIXELS, replace=False)).astype(np.int32)
+    mean, se = render(o, pix, SEED)
+    same, _ = render(o, pix, 0xC030114)
     np.savez_compressed(os.path.join(HERE, "large_scene_ref_4096spp.npz"), pixels=pix,
-                        mean=mean.astype(np.float32), stderr=se.astype(np.float32), spp=SPP, base_seed=SEED)
-    print("done", time.time() - t0)
+                        mean=mean.astype(np.float32), stderr=se.astype(np.float32), spp=SPP, base_seed=SEED,
+                        mean_same_seed=same.astype(np.float32), same_seed=0xC030114)
 
 
 if __name__ == "__main__":
