@@ -10,6 +10,7 @@ oracle restatement of the reference, timed on a bounded pixel sample).
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1]
     python bench.py --workload batch     # 64 scenes in shared waves vs per-scene launches
+    python bench.py --workload adjoint   # path-replay adjoint vs forward cost (512^2 textured box)
 """
 from __future__ import annotations
 
@@ -132,6 +133,84 @@ def batch_main(args):
     print(json.dumps(out), flush=True)
 
 
+def adjoint_main(args):
+    """The differentiation workload: the box at 512^2 with 256^2 RGB albedo and
+    mono roughness textures on the back wall and floor, forward 32 spp and
+    path-replay adjoint 32 spp of an L2 loss, MaxDepth 6. Reports the adjoint /
+    forward cost ratio (device time of one render each, graphs on)."""
+    from seesharp_amd import Device, GpuScene, abi, scenes
+
+    res, tex_res, spp, depth = 512, 256, 32, 6
+    dev = Device(0)
+    target_scene = scenes.box_scene(res, res, textured=True, tex_res=tex_res)
+    tg = GpuScene(dev, target_scene)
+    cfg = abi.PtSettings(0xC030114, spp, depth, 1, 1, 1, -1)
+    tg.clear()
+    tg.render(cfg, 0, 128)
+    target = tg.read_image()
+    tg.close()
+    s = scenes.box_scene(res, res, textured=True, tex_res=tex_res, uniform_textures=0.5)
+    g = GpuScene(dev, s)
+    npaths = res * res * spp
+
+    def forward():
+        g.clear()
+        return g.render(cfg, 0, spp)
+
+    def adjoint(dimg):
+        g.clear_gradients()
+        return g.render_adjoint(cfg, dimg, spp, spp)
+
+    forward()
+    dimg = (2.0 * (g.read_image() - target) / target.size).astype(np.float32)
+    adjoint(dimg)
+    fwd_ms, adj_ms = [], []
+    for _ in range(max(1, args.steps)):
+        t0 = time.perf_counter()
+        forward()
+        fwd_ms.append((time.perf_counter() - t0) * 1000.0)
+        t0 = time.perf_counter()
+        stats = adjoint(dimg)
+        adj_ms.append((time.perf_counter() - t0) * 1000.0)
+    f_ms, a_ms = float(np.median(fwd_ms)), float(np.median(adj_ms))
+    log(f"[bench] forward {f_ms:.2f} ms, adjoint {a_ms:.2f} ms per {npaths} paths")
+    # per-kernel device time, one event-timed render of each
+    dev.set_kernel_timing(True)
+    dev.reset_kernel_timing()
+    forward()
+    kf = {k: round(dev.kernel_timing(k)[1], 3) for k in ("generate", "extend", "shade", "shadow", "accumulate")}
+    dev.reset_kernel_timing()
+    adjoint(dimg)
+    ka = {k: round(dev.kernel_timing(k)[1], 3)
+          for k in ("generate", "extend", "shade_adjoint", "shadow_adjoint", "adjoint_finalize")}
+    dev.set_kernel_timing(False)
+    out = {
+        "metric": "adjoint/forward cost ratio (path-replay adjoint of an L2 image loss)",
+        "value": round(a_ms / f_ms, 3),
+        "unit": "x",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": 1,
+        "ms_per_step": round(f_ms + a_ms, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "box scene 512x512, 256x256 RGB albedo + mono roughness textures on back wall "
+                               "and floor, forward 32 spp + adjoint 32 spp, MaxDepth 6",
+                   "paths_per_pass": npaths},
+        "forward_ms": round(f_ms, 3),
+        "adjoint_ms": round(a_ms, 3),
+        "forward_mpaths_per_s": round(npaths / f_ms / 1e3, 3),
+        "adjoint_mpaths_per_s": round(npaths / a_ms / 1e3, 3),
+        "kernel_ms_forward": kf,
+        "kernel_ms_adjoint": ka,
+        "adjoint_ray_stats": stats,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -142,7 +221,7 @@ def main():
     ap.add_argument("--max-depth", type=int, default=8)
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("large", "batch"), default="large")
+    ap.add_argument("--workload", choices=("large", "batch", "adjoint"), default="large")
     ap.add_argument("--autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -152,6 +231,9 @@ def main():
 
     if args.workload == "batch":
         batch_main(args)
+        return
+    if args.workload == "adjoint":
+        adjoint_main(args)
         return
 
     from seesharp_amd import Device, GpuScene, scenes

@@ -572,15 +572,17 @@ SurfacePoint make_point(const or_scene& s, int tri, V2 b, float dist) {
     return p;
 }
 
-inline Rgb lookup_rgb(const or_scene& s, int tex, Rgb constant, V2 uv) {
+using TextureSet = std::vector<Texture>;
+
+inline Rgb lookup_rgb(const TextureSet& texs, int tex, Rgb constant, V2 uv) {
     if (tex < 0) return constant;
-    const Texture& t = s.textures[tex];
+    const Texture& t = texs[tex];
     const float* p = &t.texels[(size_t)t.texel(uv) * 3];
     return v3(p[0], p[1], p[2]);
 }
-inline float lookup_mono(const or_scene& s, int tex, float constant, V2 uv) {
+inline float lookup_mono(const TextureSet& texs, int tex, float constant, V2 uv) {
     if (tex < 0) return constant;
-    const Texture& t = s.textures[tex];
+    const Texture& t = texs[tex];
     return t.texels[t.texel(uv)];
 }
 
@@ -954,6 +956,7 @@ EnvSample env_sample(const EnvMap& e, V2 primary) {  // :40-73
 // ------------------------------------------------------------- materials ---
 struct Shader {
     const or_scene* s;
+    const TextureSet* tex;  // the texels the material reads (the scene's, or a perturbed copy)
     const Material* m;
     SurfacePoint p;
     Frame f;
@@ -963,6 +966,7 @@ struct Shader {
 inline Shader make_shader(const or_scene& s, const SurfacePoint& p, V3 out_dir) {
     Shader sh;
     sh.s = &s;
+    sh.tex = &s.textures;
     sh.m = &s.materials[s.tris[p.tri].material];
     sh.p = p;
     sh.f.n = p.shading_normal;
@@ -986,11 +990,11 @@ struct GenericLocal {  // GenericMaterial.LocalParams (GenericMaterial.cs:340-38
 GenericLocal generic_local(const Shader& sh) {
     const Material& m = *sh.m;
     GenericLocal r;
-    Rgb base = lookup_rgb(*sh.s, m.base_tex, m.base_color, sh.p.uv);
+    Rgb base = lookup_rgb(*sh.tex, m.base_tex, m.base_color, sh.p.uv);
     float lum = luminance(base);
     r.color_tint = lum > 0.0f ? base / lum : rgb(1.0f);
     r.spec_tint = lerp(m.spec_tint, rgb(1.0f), r.color_tint);
-    r.roughness = lookup_mono(*sh.s, m.rough_tex, m.roughness, sh.p.uv);
+    r.roughness = lookup_mono(*sh.tex, m.rough_tex, m.roughness, sh.p.uv);
     float aspect = std::sqrt(1.0f - m.anisotropic * 0.9f);
     r.ax = std::max(0.001f, r.roughness * r.roughness / aspect);
     r.ay = std::max(0.001f, r.roughness * r.roughness * aspect);
@@ -1130,7 +1134,7 @@ BsdfSample generic_sample(const Shader& sh, float pc, V2 pd) {
 Rgb diffuse_eval(const Shader& sh, V3 in_world) {
     bool refl = should_reflect(sh.p, sh.out_world, in_world);
     V3 in = sh.f.to_local(in_world);
-    Rgb base = lookup_rgb(*sh.s, sh.m->base_tex, sh.m->base_color, sh.p.uv);
+    Rgb base = lookup_rgb(*sh.tex, sh.m->base_tex, sh.m->base_color, sh.p.uv);
     if (sh.m->transmitter && !refl) {
         if (same_hemisphere(sh.out_local, in)) return rgb(0.0f);
         return base / kPi;
@@ -1227,30 +1231,41 @@ struct PathCtx {
     Counters* c;
 };
 
+// The estimators below run K texture sets along one path: every sample, pdf,
+// MIS weight and Russian-roulette decision comes from set 0 (the scene's own
+// texels); sets 1.. only change the BSDF values the estimate multiplies in.
+// K = 1 is the reference path tracer itself.
+
 // PathTracerBase.PerformBackgroundNextEvent (PathTracer.cs:423-452)
-Rgb background_nee(const PathCtx& pc, const Shader& sh, Rng& rng) {
+template <int K>
+void background_nee(const PathCtx& pc, const Shader* sh, Rng& rng, Rgb* out) {
     const or_scene& s = *pc.s;
-    if (!s.has_env) return rgb(0.0f);
+    for (int k = 0; k < K; ++k) out[k] = rgb(0.0f);
+    if (!s.has_env) return;
     EnvSample smp = env_sample(s.env, rng.next_float2());
-    Ray r = spawn_ray(sh.p, smp.dir);
+    Ray r = spawn_ray(sh[0].p, smp.dir);
     pc.c->shadow++;
     if (trace_any(s, r, pc.brute)) {
         pc.c->occluded++;
-        return rgb(0.0f);
+        return;
     }
-    Rgb bsdf_cos = eval_cos(sh, smp.dir);
-    float pdf_bsdf = pdf_fwd(sh, smp.dir);
-    if (pdf_bsdf == 0.0f || smp.pdf == 0.0f) return rgb(0.0f);
+    float pdf_bsdf = pdf_fwd(sh[0], smp.dir);
+    if (pdf_bsdf == 0.0f || smp.pdf == 0.0f) return;
     float nsr = (float)pc.cfg->num_shadow_rays;
     float mis = pc.cfg->enable_bsdf_di ? 1.0f / (1.0f + pdf_bsdf / (smp.pdf * nsr)) : 1.0f;
-    Rgb contrib = smp.weight * bsdf_cos / nsr;
-    return mis * contrib;
+    for (int k = 0; k < K; ++k) {
+        Rgb bsdf_cos = eval_cos(sh[k], smp.dir);
+        Rgb contrib = smp.weight * bsdf_cos / nsr;
+        out[k] = mis * contrib;
+    }
 }
 
 // PathTracerBase.PerformNextEventEstimation (PathTracer.cs:454-501)
-Rgb light_nee(const PathCtx& pc, const Shader& sh, Rng& rng) {
+template <int K>
+void light_nee(const PathCtx& pc, const Shader* sh, Rng& rng, Rgb* out) {
     const or_scene& s = *pc.s;
-    if (s.emitters.empty()) return rgb(0.0f);
+    for (int k = 0; k < K; ++k) out[k] = rgb(0.0f);
+    if (s.emitters.empty()) return;
     int count = (int)s.emitters.size();
     int idx = (int)rng.next_int((uint32_t)count);
     const Emitter& e = s.emitters[idx];
@@ -1259,30 +1274,34 @@ Rgb light_nee(const PathCtx& pc, const Shader& sh, Rng& rng) {
     V2 b = to_uniform_triangle(rng.next_float2());
     SurfacePoint lp = make_point(s, e.tri, b, 0.0f);
     float lpdf = e.inv_area;
-    V3 l2s = normalize(sh.p.position - lp.position);
-    Ray r = shadow_ray(sh.p, lp);
+    V3 l2s = normalize(sh[0].p.position - lp.position);
+    Ray r = shadow_ray(sh[0].p, lp);
     pc.c->shadow++;
     if (trace_any(s, r, pc.brute)) {
         pc.c->occluded++;
-        return rgb(0.0f);
+        return;
     }
     Rgb em = emitted(e, lp, l2s);
-    float jac = area_to_solid_angle(sh.p, lp);
-    Rgb bsdf_cos = eval_cos(sh, -l2s);
+    float jac = area_to_solid_angle(sh[0].p, lp);
     float nsr = (float)pc.cfg->num_shadow_rays;
     float pdf_ne = lpdf * sel * nsr;
-    float pdf_bsdf_sa = pdf_fwd(sh, -l2s);
+    float pdf_bsdf_sa = pdf_fwd(sh[0], -l2s);
     float pdf_bsdf = pdf_bsdf_sa * jac;
-    if (jac == 0.0f) return rgb(0.0f);
+    if (jac == 0.0f) return;
     float ratio = pdf_bsdf / pdf_ne;
     float mis = pc.cfg->enable_bsdf_di ? 1.0f / (ratio + 1.0f) : 1.0f;
     float pdf = lpdf / jac * sel * nsr;
-    Rgb contrib = em / pdf * bsdf_cos;
-    return mis * contrib;
+    for (int k = 0; k < K; ++k) {
+        Rgb bsdf_cos = eval_cos(sh[k], -l2s);
+        Rgb contrib = em / pdf * bsdf_cos;
+        out[k] = mis * contrib;
+    }
 }
 
 // PathTracerBase.RenderPixel + EstimateIncidentRadiance (PathTracer.cs:287-382)
-Rgb render_pixel(const PathCtx& pc, uint32_t row, uint32_t col, uint32_t iteration) {
+template <int K>
+void render_pixel_k(const PathCtx& pc, const TextureSet* const* sets, uint32_t row, uint32_t col,
+                    uint32_t iteration, Rgb* radiance) {
     const or_scene& s = *pc.s;
     const sspt_pt_settings& cfg = *pc.cfg;
     Rng rng{hash_seed(cfg.base_seed, row * (uint32_t)s.width + col, iteration)};
@@ -1292,8 +1311,12 @@ Rgb render_pixel(const PathCtx& pc, uint32_t row, uint32_t col, uint32_t iterati
     ray.tmin = 0.0f;
     ray.tmax = INFINITY;
 
-    Rgb radiance = rgb(0.0f);
-    Rgb prefix = rgb(1.0f), approx = rgb(1.0f);
+    Rgb prefix[K];
+    for (int k = 0; k < K; ++k) {
+        radiance[k] = rgb(0.0f);
+        prefix[k] = rgb(1.0f);
+    }
+    Rgb approx = rgb(1.0f);
     int depth = 1;
     SurfacePoint prev;
     float prev_pdf = 0.0f;
@@ -1316,13 +1339,18 @@ Rgb render_pixel(const PathCtx& pc, uint32_t row, uint32_t col, uint32_t iterati
                     }
                     contrib = env_radiance(s.env, ray.d);
                 }
-                radiance = radiance + prefix * mis * contrib;
+                for (int k = 0; k < K; ++k) radiance[k] = radiance[k] + prefix[k] * mis * contrib;
             }
             break;
         }
         pc.c->hits++;
         SurfacePoint hit = make_point(s, h.tri, {h.u, h.v}, h.t);
-        Shader sh = make_shader(s, hit, -ray.d);
+        Shader sh[K];
+        sh[0] = make_shader(s, hit, -ray.d);
+        for (int k = 1; k < K; ++k) {
+            sh[k] = sh[0];
+            sh[k].tex = sets[k];
+        }
 
         int ei = s.tri_emitter[h.tri];
         if (ei >= 0 && depth >= cfg.min_depth) {
@@ -1337,7 +1365,7 @@ Rgb render_pixel(const PathCtx& pc, uint32_t row, uint32_t col, uint32_t iterati
                 if (!cfg.enable_bsdf_di) mis = 0.0f;
             }
             Rgb em = emitted(e, hit, -ray.d);
-            radiance = radiance + prefix * mis * em;
+            for (int k = 0; k < K; ++k) radiance[k] = radiance[k] + prefix[k] * mis * em;
         }
 
         // ComputeSurvivalProbability (PathTracer.cs:277-282)
@@ -1345,28 +1373,38 @@ Rgb render_pixel(const PathCtx& pc, uint32_t row, uint32_t col, uint32_t iterati
         if (rng.next_float() > q || depth == cfg.max_depth) break;
 
         if (depth + 1 >= cfg.min_depth) {
-            Rgb nee = rgb(0.0f);
+            Rgb nee[K], part[K];
+            for (int k = 0; k < K; ++k) nee[k] = rgb(0.0f);
             for (int i = 0; i < cfg.num_shadow_rays; ++i) {
-                nee = nee + background_nee(pc, sh, rng);
-                nee = nee + light_nee(pc, sh, rng);
+                background_nee<K>(pc, sh, rng, part);
+                for (int k = 0; k < K; ++k) nee[k] = nee[k] + part[k];
+                light_nee<K>(pc, sh, rng, part);
+                for (int k = 0; k < K; ++k) nee[k] = nee[k] + part[k];
             }
-            radiance = radiance + prefix * nee / q;
+            for (int k = 0; k < K; ++k) radiance[k] = radiance[k] + prefix[k] * nee[k] / q;
         }
 
         // SampleDirection (PathTracer.cs:526-530)
         float u_comp = rng.next_float();
         V2 u_dir = rng.next_float2();
-        BsdfSample bs = sample_bsdf(sh, u_comp, u_dir);
+        BsdfSample bs = sample_bsdf(sh[0], u_comp, u_dir);
         if (bs.pdf == 0.0f || is_black(bs.weight)) break;
         ray = spawn_ray(hit, bs.dir);
 
-        prefix = prefix * (bs.weight / q);
+        prefix[0] = prefix[0] * (bs.weight / q);
+        for (int k = 1; k < K; ++k) prefix[k] = prefix[k] * (eval_cos(sh[k], bs.dir) / bs.pdf / q);
         approx = approx * (bs.weight / q);
         depth++;
         prev = hit;
         prev_pdf = bs.pdf * q;
     }
-    return radiance;
+}
+
+Rgb render_pixel(const PathCtx& pc, uint32_t row, uint32_t col, uint32_t iteration) {
+    const TextureSet* sets[1] = {&pc.s->textures};
+    Rgb L;
+    render_pixel_k<1>(pc, sets, row, col, iteration, &L);
+    return L;
 }
 
 template <class F>
@@ -1801,6 +1839,55 @@ void or_render_pixel(const or_scene* s, const sspt_pt_settings* cfg, uint32_t ro
     PathCtx pc{s, cfg, false, &c};
     Rgb L = render_pixel(pc, row, col, iteration);
     out[0] = L.x, out[1] = L.y, out[2] = L.z;
+}
+
+int32_t or_set_texture(or_scene* s, uint32_t index, const float* texels) {
+    if (index >= s->textures.size()) return 1;
+    Texture& t = s->textures[index];
+    std::copy(texels, texels + t.texels.size(), t.texels.begin());
+    return 0;
+}
+
+uint64_t or_render_tangent(const or_scene* s, const sspt_pt_settings* cfg, uint32_t first, uint32_t num,
+                           uint32_t texture_index, const float* tangent, float eps, float* dimage,
+                           const int32_t* pixels, int64_t num_pixels, int32_t threads) {
+    if (texture_index >= s->textures.size() || num == 0) return 0;
+    // texels + eps * tangent and texels - eps * tangent
+    TextureSet plus = s->textures, minus = s->textures;
+    std::vector<float>& tp = plus[texture_index].texels;
+    std::vector<float>& tm = minus[texture_index].texels;
+    for (size_t k = 0; k < tp.size(); ++k) {
+        tp[k] = tp[k] + eps * tangent[k];
+        tm[k] = tm[k] - eps * tangent[k];
+    }
+    const TextureSet* sets[3] = {&s->textures, &plus, &minus};
+    int W = s->width, H = s->height;
+    int64_t total = pixels ? num_pixels : (int64_t)W * H;
+    std::atomic<uint64_t> nans{0};
+    int64_t chunks = (total + 15) / 16;
+    parallel_for(chunks, threads, [&](int64_t ch) {
+        Counters c;
+        PathCtx pc{s, cfg, false, &c};
+        int64_t end = std::min<int64_t>(total, (ch + 1) * 16);
+        for (int64_t k = ch * 16; k < end; ++k) {
+            int64_t pix = pixels ? pixels[k] : k;
+            uint32_t row = (uint32_t)(pix / W), col = (uint32_t)(pix % W);
+            double acc[3] = {0.0, 0.0, 0.0};
+            for (uint32_t it = first; it < first + num; ++it) {
+                Rgb L[3];
+                render_pixel_k<3>(pc, sets, row, col, it, L);
+                if (!std::isfinite(average(L[0]))) {  // skipped by the FrameBuffer
+                    nans++;
+                    continue;
+                }
+                acc[0] += ((double)L[1].x - (double)L[2].x) / (2.0 * eps);
+                acc[1] += ((double)L[1].y - (double)L[2].y) / (2.0 * eps);
+                acc[2] += ((double)L[1].z - (double)L[2].z) / (2.0 * eps);
+            }
+            for (int c3 = 0; c3 < 3; ++c3) dimage[pix * 3 + c3] = (float)(acc[c3] / (double)num);
+        }
+    });
+    return nans;
 }
 
 }  // extern "C"

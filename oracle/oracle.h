@@ -94,6 +94,24 @@ uint64_t or_render(const or_scene* s, const sspt_pt_settings* settings, uint32_t
 void or_render_pixel(const or_scene* s, const sspt_pt_settings* settings, uint32_t row,
                      uint32_t col, uint32_t iteration, float* rgb3);
 
+/* ---- derivative reference for the path-replay adjoint ----------------------
+ * Replaces the texels of texture `index` (scene-description order). Returns 0,
+ * or 1 for an index out of range. */
+int32_t or_set_texture(or_scene* s, uint32_t index, const float* texels);
+/* Directional derivative of each pixel's estimate (mean over iterations
+ * first..first+num-1) when texture `texture_index` moves along `tangent`:
+ * every path is traced once with the scene's texels, which fixes all samples,
+ * pdfs, MIS weights and Russian-roulette decisions, and its estimate is also
+ * accumulated with the BSDF values of texels +- eps * tangent; the derivative
+ * is their central difference. This is the quantity the adjoint
+ * differentiates (sampling held fixed). Writes W*H*3 floats at the listed
+ * pixels (all pixels if `pixels` is null); returns the skipped non-finite
+ * estimates. */
+uint64_t or_render_tangent(const or_scene* s, const sspt_pt_settings* settings, uint32_t first_iteration,
+                           uint32_t num_iterations, uint32_t texture_index, const float* tangent,
+                           float eps, float* dimage, const int32_t* pixels, int64_t num_pixels,
+                           int32_t num_threads);
+
 #ifdef __cplusplus
 }
 #endif

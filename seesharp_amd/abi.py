@@ -9,9 +9,13 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libsspt.so")
+# SSPT_LIBRARY selects another build of the library (e.g. one compiled with
+# different occupancy targets); the default is the in-tree build.
+LIB_PATH = os.environ.get("SSPT_LIBRARY") or os.path.join(_HERE, "libsspt.so")
 
+ABI_VERSION = 1  # SSPT_ABI_VERSION of include/sspt.h
 SSPT_OK = 0
+INVALID_ARGUMENT = 1
 STATUS_NAMES = {
     0: "SSPT_OK",
     1: "SSPT_ERR_INVALID_ARGUMENT",
@@ -152,6 +156,8 @@ SYMBOLS = {
     "sspt_gradient_read": (C.c_int, [_vp, _vp, C.c_uint32, _fp]),
     "sspt_texture_write": (C.c_int, [_vp, _vp, C.c_uint32, _fp]),
     "sspt_set_wave_capacity": (C.c_int, [_vp, C.c_uint32]),
+    "sspt_get_launch_config": (C.c_int, [_vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                         C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "sspt_set_launch_config": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_int32]),
     "sspt_set_kernel_timing": (C.c_int, [_vp, C.c_int32]),
     "sspt_get_kernel_timing": (C.c_int, [_vp, C.c_char_p, C.POINTER(C.c_uint64),

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libsspt.so")
 SOURCES = ["api.hip", "kernels.hip", "bvh_build.cpp"]
-HEADERS = ["device_math.h", "scene_types.h", "shading.h", "kernels.h", "bvh_build.h"]
+HEADERS = ["device_math.h", "scene_types.h", "shading.h", "adjoint.h", "kernels.h", "bvh_build.h"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -39,9 +39,12 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose: bool = False, extra_flags: list[str] | None = None) -> str:
+def build(verbose: bool = False, extra_flags: list[str] | None = None, out: str | None = None) -> str:
+    """Compiles the library to `out` (default: the in-tree libsspt.so).
+    extra_flags: e.g. ["-DSSPT_TRACE_WAVES=6"] to change the occupancy targets."""
+    out = out or OUT
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    tmp = OUT + ".tmp"
+    tmp = out + ".tmp"
     cmd = [hipcc(), *HIPCC_FLAGS, *(extra_flags or []), "-o", tmp, *srcs]
     if verbose:
         print(" ".join(cmd), flush=True)
@@ -49,8 +52,8 @@ def build(verbose: bool = False, extra_flags: list[str] | None = None) -> str:
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc failed")
-    os.replace(tmp, OUT)
-    return OUT
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

@@ -72,6 +72,7 @@ struct Timing {
 struct sspt_ctx {
     int device = 0;
     int num_cus = 256;
+    int trace_per_cu = 0, shade_per_cu = 0;  // resident blocks per CU (occupancy query)
     hipStream_t stream = nullptr;
     std::string error;
     Timing timing;
@@ -480,6 +481,9 @@ sspt_status create_world(sspt_ctx* ctx, const sspt_scene_desc* descs, uint32_t n
     if (W.materials.size() > (size_t)kInfoMaterialMask)
         return fail(ctx, SSPT_ERR_UNSUPPORTED, "too many materials");
 
+    // texel offsets travel as 32-bit ints through the shading and adjoint kernels
+    if (W.texels.size() > (size_t)INT32_MAX)
+        return fail(ctx, SSPT_ERR_UNSUPPORTED, "texture data exceeds 2^31 floats");
     std::unique_ptr<sspt_scene> sc(new sspt_scene());
     sc->ctx = ctx;
     DevScene& S = sc->dev;
@@ -641,8 +645,12 @@ void timed(sspt_ctx* ctx, const char* name, F&& launch) {
 
 LaunchConfig launch_config(const sspt_ctx* ctx) {
     LaunchConfig lc;
-    lc.trace_blocks = ctx->trace_blocks ? ctx->trace_blocks : (uint32_t)ctx->num_cus * 16;
-    lc.shade_blocks = ctx->shade_blocks ? ctx->shade_blocks : (uint32_t)ctx->num_cus * 8;
+    // default: exactly the blocks that are resident at once (waves fetch their
+    // work dynamically, so more blocks would only wait for a free slot)
+    const uint32_t tpc = ctx->trace_per_cu > 0 ? (uint32_t)ctx->trace_per_cu : 8u;
+    const uint32_t spc = ctx->shade_per_cu > 0 ? (uint32_t)ctx->shade_per_cu : 4u;
+    lc.trace_blocks = ctx->trace_blocks ? ctx->trace_blocks : (uint32_t)ctx->num_cus * tpc;
+    lc.shade_blocks = ctx->shade_blocks ? ctx->shade_blocks : (uint32_t)ctx->num_cus * spc;
     return lc;
 }
 
@@ -687,8 +695,8 @@ sspt_status run_waves(sspt_ctx* ctx, sspt_scene* sc, const std::vector<WaveDesc>
                       float* mean_image, uint32_t mean_pixels, int64_t budget_ms) {
     HIP_TRY(ctx, hipMemcpyAsync(ctx->wave_descs, waves.data(), waves.size() * sizeof(WaveDesc),
                                 hipMemcpyHostToDevice, ctx->stream));
-    // counters: queues 0, wave index 0 (a fill kernel, not a memset, so that it
-    // could itself live in a graph)
+    // counters: queues, cursors and wave index 0 (issued before any capture:
+    // the captured graph holds kernels only)
     HIP_TRY(ctx, hipMemsetAsync(ctx->wave.counters, 0, 64, ctx->stream));
     ctx->wave.waves = ctx->wave_descs;
     const bool graphs = ctx->use_graphs && !ctx->timing.enabled && budget_ms < 0;
@@ -789,6 +797,7 @@ sspt_status sspt_ctx_create(int32_t device, sspt_ctx** out) {
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
         ctx->num_cus = cus;
+    resident_blocks_per_cu(&ctx->trace_per_cu, &ctx->shade_per_cu);
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return SSPT_ERR_DEVICE;
@@ -1108,6 +1117,18 @@ sspt_status sspt_set_launch_config(sspt_ctx* ctx, uint32_t trace_blocks, uint32_
     ctx->trace_blocks = trace_blocks;
     ctx->shade_blocks = shade_blocks;
     ctx->use_graphs = use_graphs != 0;
+    return SSPT_OK;
+}
+
+sspt_status sspt_get_launch_config(sspt_ctx* ctx, uint32_t* trace_blocks, uint32_t* shade_blocks,
+                                   uint32_t* trace_per_cu, uint32_t* shade_per_cu) {
+    if (!ctx || !trace_blocks || !shade_blocks || !trace_per_cu || !shade_per_cu)
+        return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "null argument");
+    LaunchConfig lc = launch_config(ctx);
+    *trace_blocks = lc.trace_blocks;
+    *shade_blocks = lc.shade_blocks;
+    *trace_per_cu = (uint32_t)std::max(0, ctx->trace_per_cu);
+    *shade_per_cu = (uint32_t)std::max(0, ctx->shade_per_cu);
     return SSPT_OK;
 }
 

@@ -64,6 +64,9 @@ void launch_shade_adjoint(hipStream_t s, const DevScene& sc, const Wave& w, cons
 void launch_shadow_adjoint(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc);
 void launch_adjoint_finalize(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc);
 void launch_scale(hipStream_t s, float* data, uint64_t n, float factor);
+// Blocks per CU that fit at once (register / LDS limits) for the trace kernels
+// (extend, shadow) and the shade kernel; 0 if the query fails.
+void resident_blocks_per_cu(int* trace, int* shade);
 
 // Ray queries on scene `scene` of the world: rays are 8 floats (ox,oy,oz,tmin,dx,dy,dz,tmax).
 void launch_trace_closest(hipStream_t s, const DevScene& sc, int scene, const float* rays, uint64_t n,

@@ -16,6 +16,15 @@
 #include "kernels.h"
 #include "shading.h"
 
+// Waves per SIMD the register allocator must leave room for (amdgpu_waves_per_eu):
+// the traversal kernels are latency bound and gain from occupancy.
+#ifndef SSPT_TRACE_WAVES
+#define SSPT_TRACE_WAVES 6
+#endif
+#ifndef SSPT_SHADE_WAVES
+#define SSPT_SHADE_WAVES 2
+#endif
+
 namespace sspt {
 
 // ------------------------------------------------------------ wave helpers -
@@ -29,6 +38,13 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
     if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
     base = __shfl(base, leader);
     return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+}
+// Dynamic work distribution: the wave takes the next 64 queue entries. Waves
+// that finish early take more, so the grid's blocks need not all be resident.
+__device__ __forceinline__ uint32_t wave_fetch(uint32_t* counter) {
+    uint32_t base = 0;
+    if (__lane_id() == 0) base = atomicAdd(counter, 64u);
+    return __shfl(base, 0);
 }
 __device__ __forceinline__ void wave_count(unsigned long long* stat, bool pred) {
     uint64_t mask = __ballot(pred);
@@ -295,6 +311,9 @@ __global__ void k_wave_end(Wave w) { w.counters[kCntWave] += 1; }
 __global__ void k_bounce_reset(Wave w, int next) {
     w.counters[next] = 0;
     w.counters[kCntShadow] = 0;
+    w.counters[kCntFetchExtend] = 0;
+    w.counters[kCntFetchShade] = 0;
+    w.counters[kCntFetchShadow] = 0;
 }
 
 // Scene of world-wide path g: last scene whose path offset is <= g.
@@ -338,15 +357,18 @@ __global__ void __launch_bounds__(256) k_generate(DevScene S, Wave w) {
     }
 }
 
-__global__ void __launch_bounds__(kTraceBlock) k_extend(DevScene S, Wave w, int buf) {
+__global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(SSPT_TRACE_WAVES)))
+k_extend(DevScene S, Wave w, int buf) {
     __shared__ int stack_node[kStackLds * kTraceBlock];
     __shared__ float stack_t[kStackLds * kTraceBlock];
     const uint32_t count = w.counters[buf];
     const float4* __restrict__ q_o = w.q_o[buf];
     const float4* __restrict__ q_d = w.q_d[buf];
     const uint32_t* __restrict__ q_slot = w.q_slot[buf];
-    for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += gridDim.x * blockDim.x) {
-        uint32_t i = base + threadIdx.x;
+    for (;;) {
+        const uint32_t base = wave_fetch(&w.counters[kCntFetchExtend]);
+        if (base >= count) break;
+        uint32_t i = base + __lane_id();
         bool active = i < count;
         HitOut h{0.0f, 0.0f, 0.0f, -1, 0};
         if (active) {
@@ -363,12 +385,15 @@ __global__ void __launch_bounds__(kTraceBlock) k_extend(DevScene S, Wave w, int 
 }
 
 template <bool ADJ>
-__global__ void __launch_bounds__(kTraceBlock) k_shadow(DevScene S, Wave w) {
+__global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(SSPT_TRACE_WAVES)))
+k_shadow(DevScene S, Wave w) {
     __shared__ int stack_node[kStackLds * kTraceBlock];
     __shared__ float stack_t[kStackLds * kTraceBlock];
     const uint32_t count = w.counters[kCntShadow];
-    for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += gridDim.x * blockDim.x) {
-        uint32_t i = base + threadIdx.x;
+    for (;;) {
+        const uint32_t base = wave_fetch(&w.counters[kCntFetchShadow]);
+        if (base >= count) break;
+        uint32_t i = base + __lane_id();
         bool active = i < count;
         bool occluded = false;
         if (active) {
@@ -446,11 +471,14 @@ __device__ __forceinline__ float4 contribution_gradient(const DevScene& S, const
 }
 
 template <bool ADJ>
-__global__ void __launch_bounds__(256) k_shade(DevScene S, Wave w, int buf) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ADJ ? 1 : SSPT_SHADE_WAVES)))
+k_shade(DevScene S, Wave w, int buf) {
     const uint32_t count = w.counters[buf];
     const int nb = buf ^ 1;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += gridDim.x * blockDim.x) {
-        uint32_t i = base + threadIdx.x;
+    for (;;) {
+        const uint32_t base = wave_fetch(&w.counters[kCntFetchShade]);
+        if (base >= count) break;
+        uint32_t i = base + __lane_id();
         bool active = i < count;
         uint32_t qs = active ? w.q_slot[buf][i] : 0u;
         uint32_t slot = qs & kSlotMask, sc = qs >> kSlotBits;
@@ -821,6 +849,15 @@ void launch_accumulate_mean(hipStream_t s, const Wave& w, float* image, uint32_t
 void launch_accumulate_sum(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, float* pool) {
     hipLaunchKernelGGL(k_accumulate_sum, dim3(lc.shade_blocks), dim3(256), 0, s, sc, w, pool);
 }
+void resident_blocks_per_cu(int* trace, int* shade) {
+    int e = 0, sh = 0, sd = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&e, k_extend, kTraceBlock, 0) != hipSuccess) e = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&sh, k_shadow<false>, kTraceBlock, 0) != hipSuccess) sh = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&sd, k_shade<false>, 256, 0) != hipSuccess) sd = 0;
+    *trace = std::max(e, sh);
+    *shade = sd;
+}
+
 void launch_scale(hipStream_t s, float* data, uint64_t n, float factor) {
     if (!n) return;
     hipLaunchKernelGGL(k_scale, dim3((uint32_t)std::min<uint64_t>(blocks(n, 256), 4096)), dim3(256), 0, s, data, n,

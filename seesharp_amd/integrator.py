@@ -50,6 +50,12 @@ class Device:
         self.check(self.lib.sspt_set_launch_config(self.ptr, int(trace_blocks), int(shade_blocks),
                                                    1 if use_graphs else 0))
 
+    def launch_config(self) -> dict:
+        v = [C.c_uint32() for _ in range(4)]
+        self.check(self.lib.sspt_get_launch_config(self.ptr, *[C.byref(x) for x in v]))
+        return {"trace_blocks": v[0].value, "shade_blocks": v[1].value, "trace_per_cu": v[2].value,
+                "shade_per_cu": v[3].value}
+
     # kernel timing with HIP events on the context stream
     def set_kernel_timing(self, on: bool):
         self.check(self.lib.sspt_set_kernel_timing(self.ptr, 1 if on else 0))
@@ -110,7 +116,26 @@ class GpuScene:
                                                       first_iteration, num_iterations, C.byref(stats)))
         return stats.as_dict()
 
+    def render_mean(self, settings: abi.PtSettings, first_iteration: int, num_iterations: int) -> np.ndarray:
+        """Mean of the estimates of iterations first..first+num-1 as an (H, W, 3)
+        image. The framebuffer follows FrameBuffer's running mean, which after a
+        clear and iterations starting at `first` holds their sum / (first + num);
+        this rescales it to the plain mean."""
+        self.clear()
+        self.render(settings, first_iteration, num_iterations)
+        img = self.read_image()
+        if first_iteration:
+            img *= np.float32((first_iteration + num_iterations) / num_iterations)
+        return img
+
     # ---- differentiation (path-replay adjoint)
+    def texture_index(self, texture) -> int:
+        """Device index of a TextureRgb / TextureMono object of the scene."""
+        for k, t in enumerate(self.scene.DeviceTextures):
+            if t is texture:
+                return k
+        raise KeyError("texture is not part of this scene (or has no image)")
+
     def texture_shapes(self):
         """(height, width[, channels]) of each texture, in the order of the scene description."""
         shapes = []

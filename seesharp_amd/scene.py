@@ -262,6 +262,7 @@ class Scene:
         keep = []
         materials, mat_index = [], {}
         textures, tex_index = [], {}
+        tex_objects = []
 
         def tex_id(tex, channels):
             if tex.Image is None:
@@ -273,6 +274,7 @@ class Scene:
                 td = abi.TextureDesc(img.shape[1], img.shape[0], channels, tex.Border, abi.fptr(img))
                 tex_index[key] = len(textures)
                 textures.append(td)
+                tex_objects.append(tex)
             return tex_index[key]
 
         def mat_id(mat):
@@ -327,6 +329,8 @@ class Scene:
         d.materials = mats
         d.num_textures = len(textures)
         d.textures = texs
+        # texture objects in device order: the index the gradient / texture-write calls take
+        self.DeviceTextures = tex_objects
         d.num_emitters = len(self.Emitters)
         d.emitters = emitters
         if self.Background is not None:

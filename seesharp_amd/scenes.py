@@ -234,8 +234,11 @@ def sky_image(width: int = 512, height: int = 256):
 # -------------------------------------------------------------------- scenes --
 
 def box_scene(width: int = 256, height: int = 256, textured: bool = False, tex_res: int = 256,
-              tex_seed: int = 3, uniform_textures: float | None = None) -> Scene:
-    """The 36-triangle box (sceneSeed 0)."""
+              tex_seed: int = 3, uniform_textures: float | None = None, spec_tint: float = 1.0,
+              diffuse_textures: bool = False) -> Scene:
+    """The 36-triangle box (sceneSeed 0). `textured` gives the back wall and floor
+    textured materials: GenericMaterials with albedo + roughness textures, or
+    textured DiffuseMaterials with `diffuse_textures`."""
     s = Scene()
     s.Name = "box"
 
@@ -254,7 +257,7 @@ def box_scene(width: int = 256, height: int = 256, textured: bool = False, tex_r
 
     back_mat, floor_mat = white, white
     if textured:
-        back_mat, floor_mat = textured_materials(tex_res, tex_seed, uniform_textures)
+        back_mat, floor_mat = textured_materials(tex_res, tex_seed, uniform_textures, spec_tint, diffuse_textures)
 
     add(*quad((0, 0, 0), (0, 0, 1), (1, 0, 1), (1, 0, 0)), floor_mat, None, uv_quad)      # floor, +y
     add(*quad((0, 1, 0), (1, 1, 0), (1, 1, 1), (0, 1, 1)), white)                           # ceiling, -y
@@ -281,8 +284,10 @@ def smooth_noise_texture(res: int, channels: int, seed: int, lo: float, hi: floa
     return img if channels > 1 else img[..., 0]
 
 
-def textured_materials(res: int, seed: int, uniform: float | None):
-    """GenericMaterials with res^2 RGB albedo and mono roughness textures (back wall, floor)."""
+def textured_materials(res: int, seed: int, uniform: float | None, spec_tint: float = 1.0,
+                       diffuse: bool = False):
+    """Materials with res^2 RGB albedo and (GenericMaterial) mono roughness textures
+    for the back wall and the floor."""
     mats = []
     for k in range(2):
         if uniform is None:
@@ -291,9 +296,12 @@ def textured_materials(res: int, seed: int, uniform: float | None):
         else:
             alb = np.full((res, res, 3), uniform, np.float32)
             rough = np.full((res, res), uniform, np.float32)
+        if diffuse:
+            mats.append(DiffuseMaterial(DiffuseMaterial.Parameters(BaseColor=TextureRgb(alb))))
+            continue
         mats.append(GenericMaterial(GenericMaterial.Parameters(
             BaseColor=TextureRgb(alb), Roughness=TextureMono(rough), Metallic=0.0,
-            SpecularTintStrength=1.0, SpecularTransmittance=0.0, IndexOfRefraction=1.5)))
+            SpecularTintStrength=spec_tint, SpecularTransmittance=0.0, IndexOfRefraction=1.5)))
     return mats[0], mats[1]
 
 

@@ -55,6 +55,9 @@ _SIGS = {
     "or_render": (C.c_uint64, [_vp, C.POINTER(abi.PtSettings), C.c_uint32, C.c_uint32, _fp,
                                C.POINTER(abi.RayStats), C.POINTER(C.c_int32), C.c_int64, C.c_int32]),
     "or_render_pixel": (None, [_vp, C.POINTER(abi.PtSettings), C.c_uint32, C.c_uint32, C.c_uint32, _fp]),
+    "or_set_texture": (C.c_int32, [_vp, C.c_uint32, _fp]),
+    "or_render_tangent": (C.c_uint64, [_vp, C.POINTER(abi.PtSettings), C.c_uint32, C.c_uint32, C.c_uint32, _fp,
+                                       C.c_float, _fp, C.POINTER(C.c_int32), C.c_int64, C.c_int32]),
 }
 
 _lib = None
@@ -119,6 +122,25 @@ class OracleScene:
             n = pixels.size
         nans = lib().or_render(self.ptr, C.byref(cfg), first, num, fp(image), C.byref(stats), px, n, threads)
         return image, stats, int(nans)
+
+    def set_texture(self, index, texels):
+        texels = np.ascontiguousarray(texels, np.float32)
+        if lib().or_set_texture(self.ptr, index, fp(texels)) != 0:
+            raise IndexError(index)
+
+    def render_tangent(self, cfg, first, num, texture_index, tangent, eps=1e-3, pixels=None, threads=0):
+        """Per-pixel d(estimate)/dt for texels + t * tangent, samples held fixed."""
+        tangent = np.ascontiguousarray(tangent, np.float32)
+        out = np.zeros((self.height, self.width, 3), np.float32)
+        px = None
+        n = 0
+        if pixels is not None:
+            pixels = np.ascontiguousarray(pixels, np.int32)
+            px = pixels.ctypes.data_as(C.POINTER(C.c_int32))
+            n = pixels.size
+        nans = lib().or_render_tangent(self.ptr, C.byref(cfg), first, num, texture_index, fp(tangent), eps,
+                                       fp(out), px, n, threads)
+        return out, int(nans)
 
     def trace_closest(self, rays, brute=False):
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
