@@ -90,7 +90,7 @@ def batch_main(args):
     tuned = None
     if args.autotune:
         tuned, _ = autotune.tune(dev, lambda: batch.render(cfgs), capacities=(1 << 20, 1 << 21, 1 << 22),
-                                 trace_per_cu=(8, 16), shade_per_cu=(4, 8), log=log)
+                                 log=log)
         log(f"[bench] autotuned: {tuned}")
     batch.render(cfgs)  # warm-up
     t0 = time.perf_counter()

@@ -191,15 +191,18 @@ sspt_status sspt_framebuffer_read_scene(sspt_ctx* ctx, sspt_scene* scene, uint32
                                         float* host_rgb);
 
 /* ---- execution knobs (the autotuner's search space) ---------------------
- * Wave capacity in path slots (0 = default 4 Mi); grid sizes of the persistent
- * traversal and shading kernels in blocks (0 = the blocks that fit on the
- * device at once; waves fetch queue entries dynamically, so any size is
- * correct); whether a wave is captured once as a hipGraph and replayed
- * (default on; off while kernel timing is enabled). */
+ * Wave capacity in path slots (0 = default 4 Mi); grid sizes of the
+ * traversal (extend, shadow) and shading kernels in blocks: 0 (the default)
+ * launches one block per chunk of the longest its queue can be and lets the
+ * hardware dispatcher balance the blocks, a non-zero size launches that many
+ * persistent blocks striding over the queue; whether a wave is captured once
+ * as a hipGraph and replayed (default on; off while kernel timing is
+ * enabled). */
 sspt_status sspt_set_wave_capacity(sspt_ctx* ctx, uint32_t slots);
 sspt_status sspt_set_launch_config(sspt_ctx* ctx, uint32_t trace_blocks, uint32_t shade_blocks,
                                    int32_t use_graphs);
-/* Effective grid sizes (blocks) and the blocks per CU resident at once. */
+/* Configured grid sizes (0 = full grid) and the blocks per CU of the
+ * traversal and shading kernels that are resident at once. */
 sspt_status sspt_get_launch_config(sspt_ctx* ctx, uint32_t* trace_blocks, uint32_t* shade_blocks,
                                    uint32_t* trace_per_cu, uint32_t* shade_per_cu);
 

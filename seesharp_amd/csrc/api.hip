@@ -643,14 +643,15 @@ void timed(sspt_ctx* ctx, const char* name, F&& launch) {
     ctx->timing.pending.push_back({name, a, b});
 }
 
-LaunchConfig launch_config(const sspt_ctx* ctx) {
+// Grid sizes for a render whose largest wave has `max_paths` paths and whose
+// scenes trace up to `max_nsr` shadow rays per kind and vertex.
+LaunchConfig launch_config(const sspt_ctx* ctx, uint32_t max_paths, uint32_t max_nsr) {
     LaunchConfig lc;
-    // default: exactly the blocks that are resident at once (waves fetch their
-    // work dynamically, so more blocks would only wait for a free slot)
-    const uint32_t tpc = ctx->trace_per_cu > 0 ? (uint32_t)ctx->trace_per_cu : 8u;
-    const uint32_t spc = ctx->shade_per_cu > 0 ? (uint32_t)ctx->shade_per_cu : 4u;
-    lc.trace_blocks = ctx->trace_blocks ? ctx->trace_blocks : (uint32_t)ctx->num_cus * tpc;
-    lc.shade_blocks = ctx->shade_blocks ? ctx->shade_blocks : (uint32_t)ctx->num_cus * spc;
+    lc.trace_blocks = ctx->trace_blocks;
+    lc.shade_blocks = ctx->shade_blocks;
+    lc.max_paths = max_paths;
+    const uint64_t shadow = (uint64_t)max_paths * 2u * std::max(1u, max_nsr);
+    lc.max_shadow = (uint32_t)std::min<uint64_t>(shadow, ctx->shadow_capacity);
     return lc;
 }
 
@@ -658,11 +659,10 @@ LaunchConfig launch_config(const sspt_ctx* ctx) {
 // the descriptor the wave counter selects.
 enum class Mode { Mean, Sum, Adjoint };
 
-void enqueue_wave(sspt_ctx* ctx, sspt_scene* sc, int max_depth, Mode mode, float* mean_image,
-                  uint32_t mean_pixels) {
+void enqueue_wave(sspt_ctx* ctx, sspt_scene* sc, const LaunchConfig& lc, int max_depth, Mode mode,
+                  float* mean_image, uint32_t mean_pixels) {
     const DevScene& S = sc->dev;
     const Wave& w = ctx->wave;
-    const LaunchConfig lc = launch_config(ctx);
     hipStream_t st = ctx->stream;
     const bool adj = mode == Mode::Adjoint;
     launch_wave_begin(st, w);
@@ -691,12 +691,15 @@ void enqueue_wave(sspt_ctx* ctx, sspt_scene* sc, int max_depth, Mode mode, float
 
 // Runs the given waves: either every wave enqueued directly (kernel timing on,
 // or graphs off), or one wave captured as a graph and replayed.
-sspt_status run_waves(sspt_ctx* ctx, sspt_scene* sc, const std::vector<WaveDesc>& waves, int max_depth, Mode mode,
-                      float* mean_image, uint32_t mean_pixels, int64_t budget_ms) {
+sspt_status run_waves(sspt_ctx* ctx, sspt_scene* sc, const std::vector<WaveDesc>& waves, int max_depth,
+                      int max_nsr, Mode mode, float* mean_image, uint32_t mean_pixels, int64_t budget_ms) {
+    uint32_t max_paths = 0;
+    for (const WaveDesc& d : waves) max_paths = std::max(max_paths, d.count);
+    const LaunchConfig lc = launch_config(ctx, max_paths, (uint32_t)std::max(1, max_nsr));
     HIP_TRY(ctx, hipMemcpyAsync(ctx->wave_descs, waves.data(), waves.size() * sizeof(WaveDesc),
                                 hipMemcpyHostToDevice, ctx->stream));
-    // counters: queues, cursors and wave index 0 (issued before any capture:
-    // the captured graph holds kernels only)
+    // counters: queues and wave index 0 (issued before any capture: the
+    // captured graph holds kernels only)
     HIP_TRY(ctx, hipMemsetAsync(ctx->wave.counters, 0, 64, ctx->stream));
     ctx->wave.waves = ctx->wave_descs;
     const bool graphs = ctx->use_graphs && !ctx->timing.enabled && budget_ms < 0;
@@ -704,7 +707,7 @@ sspt_status run_waves(sspt_ctx* ctx, sspt_scene* sc, const std::vector<WaveDesc>
     if (graphs) {
         hipGraph_t graph = nullptr;
         HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        enqueue_wave(ctx, sc, max_depth, mode, mean_image, mean_pixels);
+        enqueue_wave(ctx, sc, lc, max_depth, mode, mean_image, mean_pixels);
         HIP_TRY(ctx, hipStreamEndCapture(ctx->stream, &graph));
         hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
@@ -730,7 +733,7 @@ sspt_status run_waves(sspt_ctx* ctx, sspt_scene* sc, const std::vector<WaveDesc>
                 break;
             }
         } else {
-            enqueue_wave(ctx, sc, max_depth, mode, mean_image, mean_pixels);
+            enqueue_wave(ctx, sc, lc, max_depth, mode, mean_image, mean_pixels);
         }
         if (ctx->timing.enabled && (i % 4 == 3)) {
             (void)hipStreamSynchronize(ctx->stream);
@@ -909,7 +912,8 @@ sspt_status sspt_render(sspt_ctx* ctx, sspt_scene* sc, const sspt_pt_settings* c
         uint64_t offs[2] = {0, (uint64_t)num_iterations * np};
         HIP_TRY(ctx, hipMemcpyAsync(sc->d_recs, &R, sizeof(SceneRec), hipMemcpyHostToDevice, ctx->stream));
         HIP_TRY(ctx, hipMemcpyAsync(sc->d_path_offsets, offs, sizeof(offs), hipMemcpyHostToDevice, ctx->stream));
-        TRY(run_waves(ctx, sc, waves, cfg->max_depth, Mode::Mean, sc->image + R.fb_off, np, cfg->max_render_time_ms));
+        TRY(run_waves(ctx, sc, waves, cfg->max_depth, cfg->num_shadow_rays, Mode::Mean, sc->image + R.fb_off, np,
+                      cfg->max_render_time_ms));
     }
     return read_stats(ctx, sc, stats);
 }
@@ -947,7 +951,7 @@ sspt_status sspt_render_batch(sspt_ctx* ctx, sspt_scene* sc, const sspt_pt_setti
                                     ctx->stream));
         HIP_TRY(ctx, hipMemcpyAsync(sc->d_path_offsets, offs.data(), offs.size() * sizeof(uint64_t),
                                     hipMemcpyHostToDevice, ctx->stream));
-        TRY(run_waves(ctx, sc, waves, max_depth, Mode::Sum, nullptr, 0, -1));
+        TRY(run_waves(ctx, sc, waves, max_depth, max_nsr, Mode::Sum, nullptr, 0, -1));
         // sums -> means
         for (size_t i = 0; i < n; ++i)
             if (cfgs[i].total_spp > 0)
@@ -990,7 +994,7 @@ sspt_status sspt_render_adjoint(sspt_ctx* ctx, sspt_scene* sc, const sspt_pt_set
         uint64_t offs[2] = {0, (uint64_t)num_iterations * np};
         HIP_TRY(ctx, hipMemcpyAsync(sc->d_recs, &R, sizeof(SceneRec), hipMemcpyHostToDevice, ctx->stream));
         HIP_TRY(ctx, hipMemcpyAsync(sc->d_path_offsets, offs, sizeof(offs), hipMemcpyHostToDevice, ctx->stream));
-        TRY(run_waves(ctx, sc, waves, cfg->max_depth, Mode::Adjoint, nullptr, 0, -1));
+        TRY(run_waves(ctx, sc, waves, cfg->max_depth, cfg->num_shadow_rays, Mode::Adjoint, nullptr, 0, -1));
     }
     return read_stats(ctx, sc, stats);
 }
@@ -1124,9 +1128,8 @@ sspt_status sspt_get_launch_config(sspt_ctx* ctx, uint32_t* trace_blocks, uint32
                                    uint32_t* trace_per_cu, uint32_t* shade_per_cu) {
     if (!ctx || !trace_blocks || !shade_blocks || !trace_per_cu || !shade_per_cu)
         return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "null argument");
-    LaunchConfig lc = launch_config(ctx);
-    *trace_blocks = lc.trace_blocks;
-    *shade_blocks = lc.shade_blocks;
+    *trace_blocks = ctx->trace_blocks;
+    *shade_blocks = ctx->shade_blocks;
     *trace_per_cu = (uint32_t)std::max(0, ctx->trace_per_cu);
     *shade_per_cu = (uint32_t)std::max(0, ctx->shade_per_cu);
     return SSPT_OK;

@@ -40,11 +40,17 @@ struct Wave {
     const WaveDesc* waves;         // per-wave work of the current render
 };
 
-// Grid sizes of the persistent kernels (blocks); every kernel loops over the
-// device-side count of its queue.
+// Grid sizes of the queue kernels. Each kernel reads its queue's length on the
+// device and walks the queue with a grid-stride loop. A grid size of 0 means
+// one block per chunk of the longest the queue can be in this render
+// (max_paths, max_shadow): blocks past the device-side length exit at once and
+// the hardware dispatcher balances the rest block by block. A non-zero size
+// gives persistent blocks that stride over the queue.
 struct LaunchConfig {
-    uint32_t trace_blocks;   // blocks of kTraceBlock threads for extend / shadow
-    uint32_t shade_blocks;   // blocks of 256 threads for generate / shade / accumulate
+    uint32_t trace_blocks;   // blocks of kTraceBlock threads for extend / shadow (0 = full grid)
+    uint32_t shade_blocks;   // blocks of 256 threads for generate / shade / accumulate (0 = full grid)
+    uint32_t max_paths;      // largest wave of the render (paths)
+    uint32_t max_shadow;     // largest shadow queue of the render (rays)
 };
 
 void launch_wave_begin(hipStream_t s, const Wave& w);

@@ -39,13 +39,6 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
     base = __shfl(base, leader);
     return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
 }
-// Dynamic work distribution: the wave takes the next 64 queue entries. Waves
-// that finish early take more, so the grid's blocks need not all be resident.
-__device__ __forceinline__ uint32_t wave_fetch(uint32_t* counter) {
-    uint32_t base = 0;
-    if (__lane_id() == 0) base = atomicAdd(counter, 64u);
-    return __shfl(base, 0);
-}
 __device__ __forceinline__ void wave_count(unsigned long long* stat, bool pred) {
     uint64_t mask = __ballot(pred);
     if (mask == 0) return;
@@ -311,9 +304,6 @@ __global__ void k_wave_end(Wave w) { w.counters[kCntWave] += 1; }
 __global__ void k_bounce_reset(Wave w, int next) {
     w.counters[next] = 0;
     w.counters[kCntShadow] = 0;
-    w.counters[kCntFetchExtend] = 0;
-    w.counters[kCntFetchShade] = 0;
-    w.counters[kCntFetchShadow] = 0;
 }
 
 // Scene of world-wide path g: last scene whose path offset is <= g.
@@ -365,10 +355,8 @@ k_extend(DevScene S, Wave w, int buf) {
     const float4* __restrict__ q_o = w.q_o[buf];
     const float4* __restrict__ q_d = w.q_d[buf];
     const uint32_t* __restrict__ q_slot = w.q_slot[buf];
-    for (;;) {
-        const uint32_t base = wave_fetch(&w.counters[kCntFetchExtend]);
-        if (base >= count) break;
-        uint32_t i = base + __lane_id();
+    for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += gridDim.x * blockDim.x) {
+        uint32_t i = base + threadIdx.x;
         bool active = i < count;
         HitOut h{0.0f, 0.0f, 0.0f, -1, 0};
         if (active) {
@@ -390,10 +378,8 @@ k_shadow(DevScene S, Wave w) {
     __shared__ int stack_node[kStackLds * kTraceBlock];
     __shared__ float stack_t[kStackLds * kTraceBlock];
     const uint32_t count = w.counters[kCntShadow];
-    for (;;) {
-        const uint32_t base = wave_fetch(&w.counters[kCntFetchShadow]);
-        if (base >= count) break;
-        uint32_t i = base + __lane_id();
+    for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += gridDim.x * blockDim.x) {
+        uint32_t i = base + threadIdx.x;
         bool active = i < count;
         bool occluded = false;
         if (active) {
@@ -475,10 +461,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ADJ ? 
 k_shade(DevScene S, Wave w, int buf) {
     const uint32_t count = w.counters[buf];
     const int nb = buf ^ 1;
-    for (;;) {
-        const uint32_t base = wave_fetch(&w.counters[kCntFetchShade]);
-        if (base >= count) break;
-        uint32_t i = base + __lane_id();
+    for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += gridDim.x * blockDim.x) {
+        uint32_t i = base + threadIdx.x;
         bool active = i < count;
         uint32_t qs = active ? w.q_slot[buf][i] : 0u;
         uint32_t slot = qs & kSlotMask, sc = qs >> kSlotBits;
@@ -821,33 +805,44 @@ void launch_wave_end(hipStream_t s, const Wave& w) { hipLaunchKernelGGL(k_wave_e
 void launch_bounce_reset(hipStream_t s, const Wave& w, int buf) {
     hipLaunchKernelGGL(k_bounce_reset, dim3(1), dim3(1), 0, s, w, buf ^ 1);
 }
+static uint32_t grid_of(uint32_t configured, uint64_t max_items, uint32_t block) {
+    if (configured) return configured;
+    return (uint32_t)std::max<uint64_t>(1, (max_items + block - 1) / block);
+}
+
 void launch_generate(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc) {
-    hipLaunchKernelGGL(k_generate, dim3(lc.shade_blocks), dim3(256), 0, s, sc, w);
+    hipLaunchKernelGGL(k_generate, dim3(grid_of(lc.shade_blocks, lc.max_paths, 256)), dim3(256), 0, s, sc, w);
 }
 void launch_extend(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int buf) {
-    hipLaunchKernelGGL(k_extend, dim3(lc.trace_blocks), dim3(kTraceBlock), 0, s, sc, w, buf);
+    hipLaunchKernelGGL(k_extend, dim3(grid_of(lc.trace_blocks, lc.max_paths, kTraceBlock)), dim3(kTraceBlock), 0, s,
+                       sc, w, buf);
 }
 void launch_shade(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int buf) {
-    hipLaunchKernelGGL(k_shade<false>, dim3(lc.shade_blocks), dim3(256), 0, s, sc, w, buf);
+    hipLaunchKernelGGL(k_shade<false>, dim3(grid_of(lc.shade_blocks, lc.max_paths, 256)), dim3(256), 0, s, sc, w,
+                       buf);
 }
 void launch_shadow(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc) {
-    hipLaunchKernelGGL(k_shadow<false>, dim3(lc.trace_blocks), dim3(kTraceBlock), 0, s, sc, w);
+    hipLaunchKernelGGL(k_shadow<false>, dim3(grid_of(lc.trace_blocks, lc.max_shadow, kTraceBlock)), dim3(kTraceBlock),
+                       0, s, sc, w);
 }
 void launch_shade_adjoint(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int buf) {
-    hipLaunchKernelGGL(k_shade<true>, dim3(lc.shade_blocks), dim3(256), 0, s, sc, w, buf);
+    hipLaunchKernelGGL(k_shade<true>, dim3(grid_of(lc.shade_blocks, lc.max_paths, 256)), dim3(256), 0, s, sc, w,
+                       buf);
 }
 void launch_shadow_adjoint(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc) {
-    hipLaunchKernelGGL(k_shadow<true>, dim3(lc.trace_blocks), dim3(kTraceBlock), 0, s, sc, w);
+    hipLaunchKernelGGL(k_shadow<true>, dim3(grid_of(lc.trace_blocks, lc.max_shadow, kTraceBlock)), dim3(kTraceBlock),
+                       0, s, sc, w);
 }
 void launch_adjoint_finalize(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc) {
-    hipLaunchKernelGGL(k_adjoint_finalize, dim3(lc.shade_blocks), dim3(256), 0, s, sc, w);
+    hipLaunchKernelGGL(k_adjoint_finalize, dim3(grid_of(lc.shade_blocks, lc.max_paths, 256)), dim3(256), 0, s, sc, w);
 }
 void launch_accumulate_mean(hipStream_t s, const Wave& w, float* image, uint32_t num_pixels) {
     hipLaunchKernelGGL(k_accumulate_mean, dim3(std::min(blocks(num_pixels, 256), 8192u)), dim3(256), 0, s, w, image,
                        num_pixels);
 }
 void launch_accumulate_sum(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, float* pool) {
-    hipLaunchKernelGGL(k_accumulate_sum, dim3(lc.shade_blocks), dim3(256), 0, s, sc, w, pool);
+    hipLaunchKernelGGL(k_accumulate_sum, dim3(grid_of(lc.shade_blocks, lc.max_paths, 256)), dim3(256), 0, s, sc, w,
+                       pool);
 }
 void resident_blocks_per_cu(int* trace, int* shade) {
     int e = 0, sh = 0, sd = 0;

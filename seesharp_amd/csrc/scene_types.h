@@ -134,10 +134,7 @@ enum Counter {
     kCntQueue1 = 1,  // extension queue 1 length
     kCntShadow = 2,  // shadow queue length
     kCntWave = 3,    // index of the wave being rendered
-    kCntFetchExtend = 4,  // work-distribution cursors of the bounce's kernels
-    kCntFetchShade = 5,
-    kCntFetchShadow = 6,
-    kCntNum = 8
+    kCntNum = 4
 };
 enum Stat {
     kStatRays = 0,

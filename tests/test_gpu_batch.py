@@ -61,3 +61,24 @@ def test_graph_and_direct_launches_agree(device):
         g.clear()
         g.render(cfg, 0, 8)
         assert np.array_equal(g.read_image(), direct)
+
+
+def test_persistent_and_full_grids_render_identical_images(device):
+    """Grid sizes only change which block walks which queue chunk: every path
+    computes the same values either way (one shadow ray of each kind per vertex,
+    so the per-slot accumulation order is fixed too)."""
+    s = scenes.large_scene(96, 64, num_spheres=30, grid=48)
+    g = GpuScene(device, s)
+    cfg = ol.settings(total_spp=8, max_depth=8)
+    g.clear()
+    g.render(cfg, 0, 8)
+    full = g.read_image()
+    resident = device.launch_config()
+    for tb, sb in ((resident["trace_per_cu"] * 256, resident["shade_per_cu"] * 256), (37, 5)):
+        device.set_launch_config(tb, sb, True)
+        try:
+            g.clear()
+            g.render(cfg, 0, 8)
+            assert np.array_equal(g.read_image(), full), (tb, sb)
+        finally:
+            device.set_launch_config(0, 0, True)
