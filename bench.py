@@ -11,12 +11,14 @@ oracle restatement of the reference, timed on a bounded pixel sample).
     python bench.py [--gpus 1] [--steps 3] [--warmup 1]
     python bench.py --workload batch     # 64 scenes in shared waves vs per-scene launches
     python bench.py --workload adjoint   # path-replay adjoint vs forward cost (512^2 textured box)
+    python bench.py --workload copy      # plain 1 GiB device copy: HBM counter calibration for rocprofv3 --pmc
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -32,12 +34,28 @@ HBM_PEAK_GBS = 8000.0
 # moves them: extension ray 32 B ray record + 16 B hit record (extend) and
 # 64 B path state (shade); shadow ray 32 B + 4 B visibility (shadow); hit
 # 3x12 B positions + 3x12 B normals + 3x8 B uvs + 4 B material (shade);
-# path 12 B pixel read + 12 B pixel write (accumulate).
+# path state written by generate and read by accumulate with the pixel.
 BYTES_EXTEND_PER_RAY = 48
 BYTES_SHADOW_PER_RAY = 36
 BYTES_SHADE_PER_RAY = 64
 BYTES_SHADE_PER_HIT = 100
-BYTES_PER_PATH = 24
+BYTES_GENERATE_PER_PATH = 124   # queue entry (36 B) + path state (88 B)
+BYTES_ACCUMULATE_PER_PATH = 72  # path radiance (48 B) + pixel read and write (24 B)
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "pt_large_scene_hbm_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per launch of each kernel on the default large-scene workload,
+    from the committed rocprofv3 PMC summary (FETCH_SIZE / WRITE_SIZE scaled by
+    the copy-kernel calibration). Empty if the summary is absent."""
+    try:
+        text = open(TRAFFIC_PROFILE).read()
+    except OSError:
+        return {}
+    m = re.search(r"^traffic_bytes_per_launch:(.*)$", text, re.M)
+    if not m:
+        return {}
+    return {k: int(v) for k, v in re.findall(r"(\w+)=(\d+)", m.group(1))}
 
 
 def log(msg):
@@ -211,6 +229,28 @@ def adjoint_main(args):
     print(json.dumps(out), flush=True)
 
 
+def copy_main(args):
+    """A plain elementwise device copy of known size (reads 1 GiB, writes 1 GiB
+    per launch). Under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE it calibrates
+    what the counters report per byte moved, so that the path tracer's kernel
+    counters can be converted to HBM bytes."""
+    import torch
+
+    n = 1 << 28  # float32 elements = 1 GiB
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(max(1, args.steps)):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1000.0 / max(1, args.steps)
+    print(json.dumps({"metric": "device copy bandwidth (read + write)", "value": round(2 * n * 4 / ms / 1e6, 1),
+                      "unit": "GB/s", "bytes_read_per_launch": n * 4, "bytes_written_per_launch": n * 4,
+                      "ms_per_launch": round(ms, 3), "launches": max(1, args.steps) + 1}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -221,7 +261,7 @@ def main():
     ap.add_argument("--max-depth", type=int, default=8)
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("large", "batch", "adjoint"), default="large")
+    ap.add_argument("--workload", choices=("large", "batch", "adjoint", "copy"), default="large")
     ap.add_argument("--autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -234,6 +274,9 @@ def main():
         return
     if args.workload == "adjoint":
         adjoint_main(args)
+        return
+    if args.workload == "copy":
+        copy_main(args)
         return
 
     from seesharp_amd import Device, GpuScene, scenes
@@ -289,8 +332,8 @@ def main():
         "extend": BYTES_EXTEND_PER_RAY * n_ext * per,
         "shade": (BYTES_SHADE_PER_RAY * n_ext + BYTES_SHADE_PER_HIT * n_hit) * per,
         "shadow": BYTES_SHADOW_PER_RAY * n_sh * per,
-        "accumulate": BYTES_PER_PATH * npaths,
-        "generate": 32 * npaths,
+        "accumulate": BYTES_ACCUMULATE_PER_PATH * npaths,
+        "generate": BYTES_GENERATE_PER_PATH * npaths,
     }
     dom = max(kernels, key=lambda k: kernels[k][1])
     launches, tot_ms = kernels[dom]
@@ -298,6 +341,8 @@ def main():
     bytes_per_launch = alg[dom] / max(1, launches)
     achieved = bytes_per_launch / (avg_ms / 1000.0) / 1e9
     kernel_ms = sum(v[1] for v in kernels.values())
+    default_workload = (args.res, args.spp, args.max_depth) == (1024, 64, 8)
+    traffic = measured_traffic().get(dom) if default_workload else None
 
     out = {
         "metric": METRIC,
@@ -334,7 +379,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": None,
+            "traffic": traffic,
             "avg_launch_ms": round(avg_ms, 4),
             "launches": launches,
             "alg_bytes_per_launch": int(bytes_per_launch),
