@@ -186,6 +186,15 @@ sspt_status sspt_scene_count(const sspt_scene* scene, uint32_t* n);
  * samples (fp32 sums in arbitrary order, divided by total_spp). */
 sspt_status sspt_render_batch(sspt_ctx* ctx, sspt_scene* scene, const sspt_pt_settings* settings,
                               sspt_ray_stats* stats);
+/* A view set: num_views cameras on one scene. The world holds the scene's
+ * geometry, BVH, materials, textures and emitters once and one record and
+ * framebuffer per view (desc's own camera is unused; every view has desc's
+ * width and height). Render it with sspt_render_batch (one settings entry per
+ * view) and read view i with sspt_framebuffer_read_scene; adjoint renders of
+ * all views add into the one set of texture gradients. No counterpart in the
+ * reference (SeeSharp renders one camera per Scene). */
+sspt_status sspt_views_create(sspt_ctx* ctx, const sspt_scene_desc* desc, const sspt_camera_desc* cameras,
+                              uint32_t num_views, sspt_scene** out_scene);
 /* Copies scene `index`'s W*H*3 framebuffer to host memory. */
 sspt_status sspt_framebuffer_read_scene(sspt_ctx* ctx, sspt_scene* scene, uint32_t index,
                                         float* host_rgb);
@@ -237,6 +246,15 @@ sspt_status sspt_render(sspt_ctx* ctx, sspt_scene* scene, const sspt_pt_settings
 sspt_status sspt_render_adjoint(sspt_ctx* ctx, sspt_scene* scene, const sspt_pt_settings* settings,
                                 const float* host_dloss_dimage, uint32_t first_iteration,
                                 uint32_t num_iterations, sspt_ray_stats* stats);
+/* The adjoint of sspt_render_batch for every scene / view of a world at once:
+ * host_dloss_dimages holds each scene's W*H*3 dLoss/dImage one after the other
+ * (scene order); scene i replays iterations first_iteration ..
+ * first_iteration + total_spp_i - 1. Gradients add into the world's texture
+ * gradients (shared by all views of a view set). No counterpart in the
+ * reference. */
+sspt_status sspt_render_adjoint_batch(sspt_ctx* ctx, sspt_scene* scene, const sspt_pt_settings* settings,
+                                      const float* host_dloss_dimages, uint32_t first_iteration,
+                                      sspt_ray_stats* stats);
 /* Zeroes every texture gradient of the scene. */
 sspt_status sspt_gradient_clear(sspt_ctx* ctx, sspt_scene* scene);
 /* Copies the gradient of texture `texture_index` (width*height*channels floats). */

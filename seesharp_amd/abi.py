@@ -152,6 +152,10 @@ SYMBOLS = {
     "sspt_framebuffer_read_scene": (C.c_int, [_vp, _vp, C.c_uint32, _fp]),
     "sspt_render_adjoint": (C.c_int, [_vp, _vp, C.POINTER(PtSettings), _fp, C.c_uint32, C.c_uint32,
                                       C.POINTER(RayStats)]),
+    "sspt_render_adjoint_batch": (C.c_int, [_vp, _vp, C.POINTER(PtSettings), _fp, C.c_uint32,
+                                            C.POINTER(RayStats)]),
+    "sspt_views_create": (C.c_int, [_vp, C.POINTER(SceneDesc), C.POINTER(CameraDesc), C.c_uint32,
+                                    C.POINTER(_vp)]),
     "sspt_gradient_clear": (C.c_int, [_vp, _vp]),
     "sspt_gradient_read": (C.c_int, [_vp, _vp, C.c_uint32, _fp]),
     "sspt_texture_write": (C.c_int, [_vp, _vp, C.c_uint32, _fp]),

@@ -470,14 +470,34 @@ sspt_status add_scene(sspt_ctx* ctx, WorldBuilder& W, const sspt_scene_desc* d) 
     return SSPT_OK;
 }
 
-sspt_status create_world(sspt_ctx* ctx, const sspt_scene_desc* descs, uint32_t n, sspt_scene** out) {
+// Builds a world from n scene descriptions. With `views`, n is 1 and the world
+// holds num_views records of that one scene (shared geometry, BVH, materials,
+// textures, emitters and environment), each with its own camera and framebuffer.
+sspt_status create_world(sspt_ctx* ctx, const sspt_scene_desc* descs, uint32_t n, const sspt_camera_desc* views,
+                         uint32_t num_views, sspt_scene** out) {
     if (!ctx || !descs || !out) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
     if (n == 0 || n > (uint32_t)kMaxScenes)
         return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "a batch holds 1..%d scenes", kMaxScenes);
+    if (views && (n != 1 || num_views == 0 || num_views > (uint32_t)kMaxScenes))
+        return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "a view set holds 1..%d views of one scene", kMaxScenes);
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     WorldBuilder W;
     for (uint32_t i = 0; i < n; ++i) TRY(add_scene(ctx, W, &descs[i]));
+    if (views) {
+        const SceneRec base = W.recs[0];
+        W.recs.clear();
+        W.fb_floats = 0;
+        for (uint32_t v = 0; v < num_views; ++v) {
+            SceneRec R = base;
+            if (!setup_camera(views[v], (int)descs[0].width, (int)descs[0].height, &R.cam))
+                return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "view %u: world to camera transform must be invertible",
+                            v);
+            R.fb_off = W.fb_floats;
+            W.fb_floats += (uint64_t)R.num_pixels * 3;
+            W.recs.push_back(R);
+        }
+    }
     if (W.materials.size() > (size_t)kInfoMaterialMask)
         return fail(ctx, SSPT_ERR_UNSUPPORTED, "too many materials");
 
@@ -824,11 +844,17 @@ void sspt_ctx_destroy(sspt_ctx* ctx) {
 const char* sspt_last_error(const sspt_ctx* ctx) { return ctx ? ctx->error.c_str() : "null context"; }
 
 sspt_status sspt_scene_create(sspt_ctx* ctx, const sspt_scene_desc* d, sspt_scene** out) {
-    return create_world(ctx, d, 1, out);
+    return create_world(ctx, d, 1, nullptr, 0, out);
 }
 
 sspt_status sspt_batch_create(sspt_ctx* ctx, const sspt_scene_desc* descs, uint32_t n, sspt_scene** out) {
-    return create_world(ctx, descs, n, out);
+    return create_world(ctx, descs, n, nullptr, 0, out);
+}
+
+sspt_status sspt_views_create(sspt_ctx* ctx, const sspt_scene_desc* desc, const sspt_camera_desc* cameras,
+                              uint32_t num_views, sspt_scene** out) {
+    if (!cameras) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "null argument");
+    return create_world(ctx, desc, 1, cameras, num_views, out);
 }
 
 void sspt_scene_destroy(sspt_scene* sc) {
@@ -918,47 +944,79 @@ sspt_status sspt_render(sspt_ctx* ctx, sspt_scene* sc, const sspt_pt_settings* c
     return read_stats(ctx, sc, stats);
 }
 
-sspt_status sspt_render_batch(sspt_ctx* ctx, sspt_scene* sc, const sspt_pt_settings* cfgs, sspt_ray_stats* stats) {
-    if (!ctx || !sc || !cfgs) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "null argument");
-    HIP_TRY(ctx, hipSetDevice(ctx->device));
+// Lays the paths of every scene of the world end to end (scene, then iteration,
+// then pixel), cuts them into waves of the wave capacity and uploads the
+// records and path offsets. Scene i renders iterations first_iteration ..
+// first_iteration + total_spp_i - 1.
+struct BatchPlan {
+    std::vector<WaveDesc> waves;
+    int max_depth = 1, max_nsr = 1;
+};
+sspt_status plan_batch(sspt_ctx* ctx, sspt_scene* sc, const sspt_pt_settings* cfgs, uint32_t first_iteration,
+                       bool adjoint, BatchPlan* plan) {
     const size_t n = sc->recs.size();
     std::vector<uint64_t> offs(n + 1, 0);
-    int max_depth = 1, max_nsr = 1;
     for (size_t i = 0; i < n; ++i) {
         if (!valid_settings(cfgs[i])) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "invalid settings for scene %zu", i);
         SceneRec& R = sc->recs[i];
         apply_settings(R, cfgs[i]);
-        R.first_iter = 0;
+        R.first_iter = first_iteration;
         R.path_off = offs[i];
+        R.adj_scale = cfgs[i].total_spp > 0 ? 1.0f / (float)cfgs[i].total_spp : 0.0f;
         offs[i + 1] = offs[i] + (uint64_t)R.num_pixels * (uint64_t)cfgs[i].total_spp;
-        max_depth = std::max(max_depth, cfgs[i].max_depth);
-        max_nsr = std::max(max_nsr, cfgs[i].num_shadow_rays);
+        plan->max_depth = std::max(plan->max_depth, cfgs[i].max_depth);
+        plan->max_nsr = std::max(plan->max_nsr, cfgs[i].num_shadow_rays);
     }
     const uint64_t total = offs[n];
     uint32_t cap = wave_capacity(ctx);
     if (total < cap) cap = (uint32_t)std::max<uint64_t>(total, 1);
     TRY(ensure_wave(ctx, cap));
-    TRY(ensure_shadow(ctx, (size_t)cap * 2 * max_nsr));
-    std::vector<WaveDesc> waves;
+    TRY(ensure_shadow(ctx, (size_t)cap * 2 * plan->max_nsr));
+    if (adjoint) TRY(ensure_adjoint(ctx, (uint32_t)plan->max_depth));
     for (uint64_t start = 0; start < total; start += cap)
-        waves.push_back(WaveDesc{start, (uint32_t)std::min<uint64_t>(cap, total - start), 0, 0, 0});
-    HIP_TRY(ctx, hipMemsetAsync(ctx->wave.stats, 0, 64, ctx->stream));
-    HIP_TRY(ctx, hipMemsetAsync(sc->image, 0, sc->image_floats * sizeof(float), ctx->stream));
-    sc->nan_count = 0;
-    if (!waves.empty()) {
-        TRY(ensure_wave_descs(ctx, waves.size()));
+        plan->waves.push_back(WaveDesc{start, (uint32_t)std::min<uint64_t>(cap, total - start), 0, 0, 0});
+    if (!plan->waves.empty()) {
+        TRY(ensure_wave_descs(ctx, plan->waves.size()));
         HIP_TRY(ctx, hipMemcpyAsync(sc->d_recs, sc->recs.data(), n * sizeof(SceneRec), hipMemcpyHostToDevice,
                                     ctx->stream));
         HIP_TRY(ctx, hipMemcpyAsync(sc->d_path_offsets, offs.data(), offs.size() * sizeof(uint64_t),
                                     hipMemcpyHostToDevice, ctx->stream));
-        TRY(run_waves(ctx, sc, waves, max_depth, max_nsr, Mode::Sum, nullptr, 0, -1));
+    }
+    return SSPT_OK;
+}
+
+sspt_status sspt_render_batch(sspt_ctx* ctx, sspt_scene* sc, const sspt_pt_settings* cfgs, sspt_ray_stats* stats) {
+    if (!ctx || !sc || !cfgs) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "null argument");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    BatchPlan plan;
+    TRY(plan_batch(ctx, sc, cfgs, 0, false, &plan));
+    HIP_TRY(ctx, hipMemsetAsync(ctx->wave.stats, 0, 64, ctx->stream));
+    HIP_TRY(ctx, hipMemsetAsync(sc->image, 0, sc->image_floats * sizeof(float), ctx->stream));
+    sc->nan_count = 0;
+    if (!plan.waves.empty()) {
+        TRY(run_waves(ctx, sc, plan.waves, plan.max_depth, plan.max_nsr, Mode::Sum, nullptr, 0, -1));
         // sums -> means
-        for (size_t i = 0; i < n; ++i)
+        for (size_t i = 0; i < sc->recs.size(); ++i)
             if (cfgs[i].total_spp > 0)
                 launch_scale(ctx->stream, sc->image + sc->recs[i].fb_off, (uint64_t)sc->recs[i].num_pixels * 3,
                              1.0f / (float)cfgs[i].total_spp);
         HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     }
+    return read_stats(ctx, sc, stats);
+}
+
+sspt_status sspt_render_adjoint_batch(sspt_ctx* ctx, sspt_scene* sc, const sspt_pt_settings* cfgs,
+                                      const float* host_dloss_dimages, uint32_t first_iteration,
+                                      sspt_ray_stats* stats) {
+    if (!ctx || !sc || !cfgs || !host_dloss_dimages) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "null argument");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    BatchPlan plan;
+    TRY(plan_batch(ctx, sc, cfgs, first_iteration, true, &plan));
+    HIP_TRY(ctx, hipMemsetAsync(ctx->wave.stats, 0, 64, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(sc->adj_image, host_dloss_dimages, sc->image_floats * sizeof(float),
+                                hipMemcpyHostToDevice, ctx->stream));
+    if (!plan.waves.empty())
+        TRY(run_waves(ctx, sc, plan.waves, plan.max_depth, plan.max_nsr, Mode::Adjoint, nullptr, 0, -1));
     return read_stats(ctx, sc, stats);
 }
 

@@ -70,7 +70,50 @@ class Device:
         return int(n.value), float(ms.value)
 
 
-class GpuScene:
+class _TextureParameters:
+    """Texture upload and gradient read-back of a device world (the
+    differentiable parameters). Textures are numbered in world order: the
+    textures of each scene description, scene after scene."""
+
+    def _texture_owners(self):
+        raise NotImplementedError
+
+    def texture_index(self, texture) -> int:
+        """Device index of a TextureRgb / TextureMono object of the world's scenes."""
+        base = 0
+        for scene in self._texture_owners():
+            for k, t in enumerate(scene.DeviceTextures):
+                if t is texture:
+                    return base + k
+            base += len(scene.DeviceTextures)
+        raise KeyError("texture is not part of this world (or has no image)")
+
+    def texture_shapes(self):
+        """(height, width[, channels]) of each texture, in world order."""
+        shapes = []
+        for scene in self._texture_owners():
+            for t in scene.DeviceTextures:
+                img = np.asarray(t.Image)
+                shapes.append(tuple(img.shape))
+        return shapes
+
+    def write_texture(self, index: int, texels: np.ndarray):
+        texels = np.ascontiguousarray(texels, np.float32)
+        shapes = self.texture_shapes()
+        if 0 <= index < len(shapes) and texels.size != int(np.prod(shapes[index])):
+            raise ValueError(f"texture {index} holds {int(np.prod(shapes[index]))} texels, got {texels.size}")
+        self.device.check(self.device.lib.sspt_texture_write(self.device.ptr, self.ptr, index, abi.fptr(texels)))
+
+    def clear_gradients(self):
+        self.device.check(self.device.lib.sspt_gradient_clear(self.device.ptr, self.ptr))
+
+    def gradient(self, index: int) -> np.ndarray:
+        out = np.zeros(self.texture_shapes()[index], np.float32)
+        self.device.check(self.device.lib.sspt_gradient_read(self.device.ptr, self.ptr, index, abi.fptr(out)))
+        return out
+
+
+class GpuScene(_TextureParameters):
     """The device copy of a prepared Scene: BVH, triangles, materials, emitters,
     environment sampling grid and the framebuffer (sspt_scene)."""
 
@@ -129,32 +172,8 @@ class GpuScene:
         return img
 
     # ---- differentiation (path-replay adjoint)
-    def texture_index(self, texture) -> int:
-        """Device index of a TextureRgb / TextureMono object of the scene."""
-        for k, t in enumerate(self.scene.DeviceTextures):
-            if t is texture:
-                return k
-        raise KeyError("texture is not part of this scene (or has no image)")
-
-    def texture_shapes(self):
-        """(height, width[, channels]) of each texture, in the order of the scene description."""
-        shapes = []
-        for k in range(self.desc.num_textures):
-            t = self.desc.textures[k]
-            shapes.append((t.height, t.width, t.channels) if t.channels > 1 else (t.height, t.width))
-        return shapes
-
-    def write_texture(self, index: int, texels: np.ndarray):
-        texels = np.ascontiguousarray(texels, np.float32)
-        self.device.check(self.device.lib.sspt_texture_write(self.device.ptr, self.ptr, index, abi.fptr(texels)))
-
-    def clear_gradients(self):
-        self.device.check(self.device.lib.sspt_gradient_clear(self.device.ptr, self.ptr))
-
-    def gradient(self, index: int) -> np.ndarray:
-        out = np.zeros(self.texture_shapes()[index], np.float32)
-        self.device.check(self.device.lib.sspt_gradient_read(self.device.ptr, self.ptr, index, abi.fptr(out)))
-        return out
+    def _texture_owners(self):
+        return [self.scene]
 
     def render_adjoint(self, settings: abi.PtSettings, dloss_dimage: np.ndarray, first_iteration: int,
                        num_iterations: int) -> dict:
@@ -189,19 +208,46 @@ class GpuScene:
         return rays, _hits(hits)
 
 
-class GpuBatch:
+class GpuBatch(_TextureParameters):
     """Many scenes in one device world, rendered together (sspt_batch_create /
     sspt_render_batch): waves are filled with paths of every scene, so small
-    scenes keep the GPU busy."""
+    scenes keep the GPU busy. `GpuBatch.views` builds the same for many cameras
+    on one scene (sspt_views_create): one copy of the geometry and textures,
+    one framebuffer per view."""
 
-    def __init__(self, device: Device, scenes):
+    def __init__(self, device: Device, scenes, _views=None):
         self.device = device
         self.scenes = list(scenes)
         descs = [s.to_desc() for s in self.scenes]
         self._descs = (abi.SceneDesc * len(descs))(*descs)
         h = C.c_void_p()
-        device.check(device.lib.sspt_batch_create(device.ptr, self._descs, len(descs), C.byref(h)))
+        if _views is None:
+            self._owners = self.scenes
+            self.sizes = [(s.FrameBuffer.Height, s.FrameBuffer.Width) for s in self.scenes]
+            device.check(device.lib.sspt_batch_create(device.ptr, self._descs, len(descs), C.byref(h)))
+        else:
+            cams = (abi.CameraDesc * len(_views))()
+            for i, cam in enumerate(_views):
+                cams[i].world_to_camera[:] = [float(x) for x in cam.WorldToCamera.reshape(16)]
+                cams[i].vertical_fov_degrees = cam.VerticalFieldOfView
+            self._cams = cams
+            self._owners = self.scenes[:1]
+            fb = self.scenes[0].FrameBuffer
+            self.sizes = [(fb.Height, fb.Width)] * len(_views)
+            device.check(device.lib.sspt_views_create(device.ptr, self._descs, cams, len(_views), C.byref(h)))
         self.ptr = h
+
+    @classmethod
+    def views(cls, device: Device, scene, cameras) -> "GpuBatch":
+        """One scene seen from each PerspectiveCamera in `cameras` (the scene's
+        own camera is not used), at the scene's framebuffer size."""
+        return cls(device, [scene], _views=list(cameras))
+
+    def __len__(self):
+        return len(self.sizes)
+
+    def _texture_owners(self):
+        return self._owners
 
     def close(self):
         if getattr(self, "ptr", None):
@@ -217,15 +263,29 @@ class GpuBatch:
         return dict(zip(("num_triangles", "num_emitters", "num_bvh_nodes", "device_bytes"), (x.value for x in v)))
 
     def render(self, settings) -> dict:
-        """settings: one abi.PtSettings per scene (total_spp iterations each)."""
-        arr = (abi.PtSettings * len(self.scenes))(*settings)
+        """settings: one abi.PtSettings per scene / view (total_spp iterations each)."""
+        arr = (abi.PtSettings * len(self))(*settings)
         stats = abi.RayStats()
         self.device.check(self.device.lib.sspt_render_batch(self.device.ptr, self.ptr, arr, C.byref(stats)))
         return stats.as_dict()
 
+    def render_adjoint(self, settings, dloss_dimages, first_iteration: int = 0) -> dict:
+        """Adds dLoss/dtexel of every scene / view to the world's texture gradients.
+        dloss_dimages: one (H, W, 3) array per scene / view; scene i replays
+        iterations first_iteration .. first_iteration + settings[i].total_spp - 1."""
+        arr = (abi.PtSettings * len(self))(*settings)
+        flat = np.concatenate([np.ascontiguousarray(g, np.float32).reshape(-1) for g in dloss_dimages])
+        expect = sum(h * w * 3 for h, w in self.sizes)
+        if flat.size != expect:
+            raise ValueError(f"dLoss/dImage holds {flat.size} values, the framebuffers {expect}")
+        stats = abi.RayStats()
+        self.device.check(self.device.lib.sspt_render_adjoint_batch(self.device.ptr, self.ptr, arr, abi.fptr(flat),
+                                                                    first_iteration, C.byref(stats)))
+        return stats.as_dict()
+
     def read_image(self, index: int) -> np.ndarray:
-        fb = self.scenes[index].FrameBuffer
-        img = np.zeros((fb.Height, fb.Width, 3), np.float32)
+        h, w = self.sizes[index]
+        img = np.zeros((h, w, 3), np.float32)
         self.device.check(self.device.lib.sspt_framebuffer_read_scene(self.device.ptr, self.ptr, index,
                                                                       abi.fptr(img)))
         return img

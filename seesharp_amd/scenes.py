@@ -307,8 +307,13 @@ def textured_materials(res: int, seed: int, uniform: float | None, spec_tint: fl
 
 def large_scene(width: int = 1024, height: int = 1024, num_spheres: int = 400, grid: int = 512,
                 scene_seed: int = 1, with_env: bool = True, camera_pos=(0.0, 3.0, 9.0),
-                camera_target=(0.0, 0.5, 0.0)) -> Scene:
-    """The 1 036 296-triangle scene (sceneSeed 1) with the default arguments."""
+                camera_target=(0.0, 0.5, 0.0), ground_textures: int = 0, ground_texture_seed: int = 5,
+                ground_uniform: float | None = None) -> Scene:
+    """The 1 036 296-triangle scene (sceneSeed 1) with the default arguments.
+
+    ground_textures > 0 gives the ground grid a GenericMaterial with a
+    ground_textures^2 RGB albedo and a mono roughness texture (UV-mapped over the
+    grid): smooth noise from ground_texture_seed, or the constant ground_uniform."""
     s = Scene()
     s.Name = "large"
     sv, sf = icosphere(3)  # 1280 faces
@@ -334,6 +339,17 @@ def large_scene(width: int = 1024, height: int = 1024, num_spheres: int = 400, g
     ground.Material = GenericMaterial(GenericMaterial.Parameters(
         BaseColor=(r.next_float(), r.next_float(), r.next_float()), Roughness=r.next_float(0.05, 1.0),
         Metallic=0.0, SpecularTintStrength=1.0, SpecularTransmittance=0.0, IndexOfRefraction=1.5))
+    if ground_textures:
+        n = int(ground_textures)
+        if ground_uniform is None:
+            alb = smooth_noise_texture(n, 3, ground_texture_seed, 0.05, 0.95)
+            rough = smooth_noise_texture(n, 1, ground_texture_seed + 50, 0.1, 0.9)
+        else:
+            alb = np.full((n, n, 3), ground_uniform, np.float32)
+            rough = np.full((n, n), ground_uniform, np.float32)
+        p = ground.Material.MaterialParameters
+        p.BaseColor = TextureRgb(alb)
+        p.Roughness = TextureMono(rough)
     s.Meshes.append(ground)
     black = DiffuseMaterial(DiffuseMaterial.Parameters(BaseColor=(0.0, 0.0, 0.0)))
     for (lx, lz) in ((-2.5, -2.5), (2.5, -2.5), (-2.5, 2.5), (2.5, 2.5)):
@@ -346,6 +362,17 @@ def large_scene(width: int = 1024, height: int = 1024, num_spheres: int = 400, g
     s.Camera = PerspectiveCamera(create_look_at(camera_pos, camera_target, (0, 1, 0)), 40.0)
     s.FrameBuffer = FrameBuffer(width, height)
     return s
+
+
+def circle_views(count: int = 32, radius: float = 8.0, height: float = 2.0, target=(0.0, 0.5, 0.0),
+                 fov: float = 40.0):
+    """`count` PerspectiveCameras evenly spaced on a horizontal circle, looking at `target`."""
+    cams = []
+    for k in range(count):
+        a = 2.0 * np.pi * k / count
+        pos = (radius * np.cos(a), height, radius * np.sin(a))
+        cams.append(PerspectiveCamera(create_look_at(pos, target, (0, 1, 0)), fov))
+    return cams
 
 
 BATCH_RESOLUTIONS = (128, 192, 256, 384, 512)
