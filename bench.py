@@ -129,6 +129,18 @@ def batch_main(args):
             g.render(c, 0, n)
     single_ms = (time.perf_counter() - t0) * 1000.0 / args.steps
     log(f"[bench] per-scene launches: {single_ms:.1f} ms per pass")
+    # per-scene plain launches: the single-kernel tracer, one render per scene
+    dev.set_execution_mode(True)
+    for g, c, n in zip(singles, cfgs, spp):
+        g.clear()
+        g.render(c, 0, n)
+    t0 = time.perf_counter()
+    for g, c, n in zip(singles, cfgs, spp):
+        g.clear()
+        g.render(c, 0, n)
+    plain_ms = (time.perf_counter() - t0) * 1000.0
+    dev.set_execution_mode(False)
+    log(f"[bench] per-scene plain single-kernel launches: {plain_ms:.1f} ms per pass")
     out = {
         "metric": "Mpaths/s of the batched wavefront (64 scenes, one world) vs per-scene launches",
         "value": round(npaths / (batch_ms / 1000.0) / 1e6, 3),
@@ -147,6 +159,8 @@ def batch_main(args):
                    "triangles": info["num_triangles"], "paths_per_step": npaths},
         "per_scene_launches_mpaths_per_s": round(npaths / (single_ms / 1000.0) / 1e6, 3),
         "speedup_vs_per_scene": round(single_ms / batch_ms, 3),
+        "per_scene_plain_single_kernel_mpaths_per_s": round(npaths / (plain_ms / 1000.0) / 1e6, 3),
+        "speedup_vs_per_scene_plain": round(plain_ms / batch_ms, 3),
         "autotuned": tuned,
     }
     print(json.dumps(out), flush=True)
@@ -469,6 +483,19 @@ def main():
         "image_mean": float(img.mean()),
         "nan_paths": gs.nan_count(),
     }
+    # the plain single-kernel tracer (one thread per path) on the same render
+    dev.set_execution_mode(True)
+    gs.clear()
+    gs.render(cfg, 0, args.spp)  # warm-up / graph capture
+    gs.clear()
+    t0 = time.perf_counter()
+    gs.render(cfg, 0, args.spp)
+    single_ms = (time.perf_counter() - t0) * 1000.0
+    dev.set_execution_mode(False)
+    log(f"[bench] single-kernel tracer: {single_ms:.1f} ms")
+    out["plain_single_kernel"] = {"mpaths_per_s": round(npaths / (single_ms / 1000.0) / 1e6, 3),
+                                  "ms_per_step": round(single_ms, 3),
+                                  "wavefront_speedup": round(single_ms / ms, 3)}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene, args.spp, args.max_depth, args.cpu_budget)
     print(json.dumps(out), flush=True)

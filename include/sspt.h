@@ -210,6 +210,16 @@ sspt_status sspt_framebuffer_read_scene(sspt_ctx* ctx, sspt_scene* scene, uint32
 sspt_status sspt_set_wave_capacity(sspt_ctx* ctx, uint32_t slots);
 sspt_status sspt_set_launch_config(sspt_ctx* ctx, uint32_t trace_blocks, uint32_t shade_blocks,
                                    int32_t use_graphs);
+/* Execution mode of sspt_render / sspt_render_batch: the wavefront tracer
+ * (default; generate / extend / shade / shadow kernels over queues), or the
+ * plain single-kernel tracer (one thread follows one path to its end, shadow
+ * rays traced where sampled): the same estimator and RNG draws, kept as the
+ * per-path baseline. Adjoint renders need the wavefront. */
+typedef enum sspt_exec_mode {
+    SSPT_EXEC_WAVEFRONT = 0,
+    SSPT_EXEC_SINGLE_KERNEL = 1
+} sspt_exec_mode;
+sspt_status sspt_set_execution_mode(sspt_ctx* ctx, int32_t mode);
 /* Configured grid sizes (0 = full grid) and the blocks per CU of the
  * traversal and shading kernels that are resident at once. */
 sspt_status sspt_get_launch_config(sspt_ctx* ctx, uint32_t* trace_blocks, uint32_t* shade_blocks,

@@ -25,6 +25,8 @@ STATUS_NAMES = {
     5: "SSPT_ERR_UNSUPPORTED",
     6: "SSPT_ERR_ZERO_AREA_EMITTER",
 }
+EXEC_WAVEFRONT = 0
+EXEC_SINGLE_KERNEL = 1
 MATERIAL_DIFFUSE = 0
 MATERIAL_GENERIC = 1
 
@@ -160,6 +162,7 @@ SYMBOLS = {
     "sspt_gradient_read": (C.c_int, [_vp, _vp, C.c_uint32, _fp]),
     "sspt_texture_write": (C.c_int, [_vp, _vp, C.c_uint32, _fp]),
     "sspt_set_wave_capacity": (C.c_int, [_vp, C.c_uint32]),
+    "sspt_set_execution_mode": (C.c_int, [_vp, C.c_int32]),
     "sspt_get_launch_config": (C.c_int, [_vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                          C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "sspt_set_launch_config": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_int32]),

@@ -87,6 +87,7 @@ struct sspt_ctx {
     uint32_t want_capacity = 0;
     uint32_t trace_blocks = 0, shade_blocks = 0;
     bool use_graphs = true;
+    int32_t exec_mode = SSPT_EXEC_WAVEFRONT;
     // adjoint buffers, grown on demand
     std::vector<void*> adj_allocs;
     uint64_t adj_records = 0;   // capacity * depth
@@ -686,6 +687,15 @@ void enqueue_wave(sspt_ctx* ctx, sspt_scene* sc, const LaunchConfig& lc, int max
     hipStream_t st = ctx->stream;
     const bool adj = mode == Mode::Adjoint;
     launch_wave_begin(st, w);
+    if (ctx->exec_mode == SSPT_EXEC_SINGLE_KERNEL && !adj) {
+        timed(ctx, "path", [&] { launch_path(st, S, w, lc); });
+        if (mode == Mode::Mean)
+            timed(ctx, "accumulate", [&] { launch_accumulate_mean(st, w, mean_image, mean_pixels); });
+        else
+            timed(ctx, "accumulate", [&] { launch_accumulate_sum(st, S, w, lc, sc->image); });
+        launch_wave_end(st, w);
+        return;
+    }
     timed(ctx, "generate", [&] { launch_generate(st, S, w, lc); });
     int buf = 0;
     for (int depth = 1; depth <= max_depth; ++depth) {
@@ -1009,6 +1019,8 @@ sspt_status sspt_render_adjoint_batch(sspt_ctx* ctx, sspt_scene* sc, const sspt_
                                       const float* host_dloss_dimages, uint32_t first_iteration,
                                       sspt_ray_stats* stats) {
     if (!ctx || !sc || !cfgs || !host_dloss_dimages) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "null argument");
+    if (ctx->exec_mode != SSPT_EXEC_WAVEFRONT)
+        return fail(ctx, SSPT_ERR_UNSUPPORTED, "adjoint renders run on the wavefront tracer only");
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     BatchPlan plan;
     TRY(plan_batch(ctx, sc, cfgs, first_iteration, true, &plan));
@@ -1026,6 +1038,8 @@ sspt_status sspt_render_adjoint(sspt_ctx* ctx, sspt_scene* sc, const sspt_pt_set
     if (!ctx || !sc || !cfg || !host_dloss_dimage) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "null argument");
     if (!valid_settings(*cfg)) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "invalid settings");
     if (sc->recs.size() != 1) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "sspt_render_adjoint renders a single scene");
+    if (ctx->exec_mode != SSPT_EXEC_WAVEFRONT)
+        return fail(ctx, SSPT_ERR_UNSUPPORTED, "adjoint renders run on the wavefront tracer only");
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     SceneRec& R = sc->recs[0];
     const uint32_t np = R.num_pixels;
@@ -1190,6 +1204,14 @@ sspt_status sspt_get_launch_config(sspt_ctx* ctx, uint32_t* trace_blocks, uint32
     *shade_blocks = ctx->shade_blocks;
     *trace_per_cu = (uint32_t)std::max(0, ctx->trace_per_cu);
     *shade_per_cu = (uint32_t)std::max(0, ctx->shade_per_cu);
+    return SSPT_OK;
+}
+
+sspt_status sspt_set_execution_mode(sspt_ctx* ctx, int32_t mode) {
+    if (!ctx) return SSPT_ERR_INVALID_ARGUMENT;
+    if (mode != SSPT_EXEC_WAVEFRONT && mode != SSPT_EXEC_SINGLE_KERNEL)
+        return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "unknown execution mode %d", mode);
+    ctx->exec_mode = mode;
     return SSPT_OK;
 }
 

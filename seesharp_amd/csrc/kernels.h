@@ -55,6 +55,8 @@ struct LaunchConfig {
 
 void launch_wave_begin(hipStream_t s, const Wave& w);
 void launch_generate(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc);
+// the single-kernel path tracer: a whole wave's paths, one thread per path
+void launch_path(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc);
 void launch_bounce_reset(hipStream_t s, const Wave& w, int buf);
 void launch_extend(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int buf);
 void launch_shade(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int buf);

@@ -318,27 +318,38 @@ __device__ __forceinline__ int scene_of(const DevScene& S, uint64_t g) {
 }
 
 // RNG(BaseSeed, row * W + col, iteration) (PathTracer.cs:236-237), pixel jitter
-// (:289-290), camera ray (:291) and the initial PathState (:293-302).
+// (:289-290) and camera ray (:291) of world-wide path g.
+struct PathStart {
+    int scene;
+    uint32_t pix;
+    Rng rng;
+    f3 o, dir;
+};
+__device__ __forceinline__ PathStart start_path(const DevScene& S, uint64_t g) {
+    PathStart p;
+    p.scene = scene_of(S, g);
+    const SceneRec& R = S.recs[p.scene];
+    uint64_t k = g - R.path_off;
+    uint32_t it = (uint32_t)(k / R.num_pixels);
+    p.pix = (uint32_t)(k - (uint64_t)it * R.num_pixels);
+    uint32_t row = p.pix / (uint32_t)R.cam.width;
+    uint32_t col = p.pix - row * (uint32_t)R.cam.width;
+    p.rng = Rng{hash_seed(R.base_seed, p.pix, R.first_iter + it)};
+    f2 off = p.rng.next_float2();
+    camera_ray(R.cam, (float)col + off.x, (float)row + off.y, &p.o, &p.dir);
+    return p;
+}
+
+// The first extension ray and the initial PathState (PathTracer.cs:293-302) of every path of the wave.
 __global__ void __launch_bounds__(256) k_generate(DevScene S, Wave w) {
     const WaveDesc d = cur_wave(w);
     for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < d.count; slot += gridDim.x * blockDim.x) {
-        uint64_t g = d.start + slot;
-        int sc = scene_of(S, g);
-        const SceneRec& R = S.recs[sc];
-        uint64_t k = g - R.path_off;
-        uint32_t it = (uint32_t)(k / R.num_pixels);
-        uint32_t pix = (uint32_t)(k - (uint64_t)it * R.num_pixels);
-        uint32_t row = pix / (uint32_t)R.cam.width;
-        uint32_t col = pix - row * (uint32_t)R.cam.width;
-        Rng rng{hash_seed(R.base_seed, pix, R.first_iter + it)};
-        f2 off = rng.next_float2();
-        f3 o, dir;
-        camera_ray(R.cam, (float)col + off.x, (float)row + off.y, &o, &dir);
-        w.q_o[0][slot] = f4_of(o, 0.0f);
-        w.q_d[0][slot] = f4_of(dir, INFINITY);
-        w.q_slot[0][slot] = slot | ((uint32_t)sc << kSlotBits);
-        w.st_pix[slot] = pix | ((uint32_t)sc << kSlotBits);
-        w.st_rng[slot] = rng.state;
+        PathStart p = start_path(S, d.start + slot);
+        w.q_o[0][slot] = f4_of(p.o, 0.0f);
+        w.q_d[0][slot] = f4_of(p.dir, INFINITY);
+        w.q_slot[0][slot] = slot | ((uint32_t)p.scene << kSlotBits);
+        w.st_pix[slot] = p.pix | ((uint32_t)p.scene << kSlotBits);
+        w.st_rng[slot] = p.rng.state;
         w.st_thr[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
         w.st_prev[slot] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(1));
         w.st_L[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -426,9 +437,6 @@ __device__ __forceinline__ uint32_t shadow_tag(uint32_t slot, uint32_t scene, ui
     return (scene << (kSlotBits + 1)) | (slot << 1) | kind;
 }
 
-// One bounce of EstimateIncidentRadiance (PathTracer.cs:315-382) for every path
-// of the queue: background / emitter hits with MIS, Russian roulette, next-event
-// estimation (environment, then area light) and BSDF sampling.
 // Texel float offsets of the shaded vertex's textured parameters (-1 = constant).
 __device__ __forceinline__ int2 vertex_texels(const DevScene& S, const Shader& sh, f2 uv) {
     int2 t = make_int2(-1, -1);
@@ -456,6 +464,164 @@ __device__ __forceinline__ float4 contribution_gradient(const DevScene& S, const
     return make_float4(g[0], g[1], g[2], g[3]);
 }
 
+// One vertex of EstimateIncidentRadiance (PathTracer.cs:315-382) for one path:
+// the hit of its extension ray (background / emitter with MIS, Russian
+// roulette), its next-event rays (environment, then area light, per shadow
+// ray) and its continuation. The wavefront shade kernel queues the rays a
+// vertex emits; the single-kernel tracer traces them at once. With ADJ the
+// vertex also computes the adjoint's per-ray gradients.
+template <bool ADJ>
+struct PathVertex {
+    const DevScene S;
+    const SceneRec& R;
+    Env E;
+    uint32_t slot, sc;
+    // path state (PathState, PathTracer.cs:143-194)
+    Rng rng{0};
+    f3 prefix = {0.0f, 0.0f, 0.0f};
+    f3 L = {0.0f, 0.0f, 0.0f};
+    f3 dir = {0.0f, 0.0f, 0.0f};  // the extension ray's direction
+    f3 prev_pos = {0.0f, 0.0f, 0.0f};
+    float prev_pdf = 0.0f;
+    int depth = 0;
+    // this vertex
+    Surf s;
+    Shader sh;
+    float q = 1.0f;
+    bool cont = false, do_nee = false;
+    f3 delta = {0.0f, 0.0f, 0.0f};  // ADJ: dLoss/dpixel / iterations
+    int2 vtex = {-1, -1};           // ADJ: texel offsets of the vertex's textured parameters
+    f3 next_dir = {0.0f, 0.0f, 0.0f};
+
+    __device__ PathVertex(const DevScene& S_, const SceneRec& R_, uint32_t slot_, uint32_t sc_)
+        : S(S_), R(R_), E(env_of(S_, R_)), slot(slot_), sc(sc_) {}
+
+    // OnBackgroundHit (PathTracer.cs:384-400), OnLightHit (:402-421) and the
+    // survival decision (:277-282, ApproxThroughput == PrefixWeight).
+    __device__ __forceinline__ void on_hit(float4 h) {
+        // decisions of this vertex (the single-kernel tracer reuses the object along the path)
+        q = 1.0f;
+        cont = false;
+        do_nee = false;
+        const int li = __float_as_int(h.w);
+        const float nsr = (float)R.num_shadow_rays;
+        if (li < 0) {
+            if (depth >= R.min_depth) {
+                float mis = 0.0f;
+                f3 contrib = gray(0.0f);
+                if (R.has_env && R.enable_bsdf_di) {
+                    mis = 1.0f;
+                    if (depth > 1) {
+                        float pdf_ne = env_pdf(E, dir) * nsr;
+                        mis = 1.0f / (1.0f + pdf_ne / prev_pdf);
+                    }
+                    contrib = env_radiance(E, dir);
+                }
+                L = L + prefix * mis * contrib;
+            }
+            return;
+        }
+        s = surface(S, li, h.y, h.z);
+        make_shader(S, s, -dir, &sh);
+        if (ADJ) vtex = vertex_texels(S, sh, s.uv);
+        int ei = S.tri_emitter[li];
+        if (ei >= 0 && depth >= R.min_depth) {
+            DevEmitter e = S.emitters[R.emitter_off + ei];
+            float mis = 1.0f;
+            if (depth > 1) {
+                float jac = area_to_solid_angle(prev_pos, s.pos, s.ng);
+                float pdf_ne = e.inv_area / (float)R.num_emitters * nsr / jac;
+                float ratio = pdf_ne / prev_pdf;
+                mis = 1.0f / (ratio + 1.0f);
+                if (!R.enable_bsdf_di) mis = 0.0f;
+            }
+            // DiffuseEmitter.EmittedRadiance: one-sided on the shading normal
+            f3 em = dot(s.ns, -dir) < 0.0f ? gray(0.0f) : mk3(e.r, e.g, e.b);
+            L = L + prefix * mis * em;
+        }
+        q = depth > 4 ? clampf(average(prefix), 0.05f, 0.95f) : 1.0f;
+        bool stop = rng.next_float() > q || depth == R.max_depth;
+        do_nee = !stop && depth + 1 >= R.min_depth;
+        cont = !stop;
+    }
+
+    // PerformBackgroundNextEvent (PathTracer.cs:423-452): one environment
+    // sample. Returns whether it yields a shadow ray with a non-zero
+    // contribution (a zero one cannot change the estimate and is not traced).
+    __device__ __forceinline__ bool env_next_event(float4* o, float4* d, float4* c, float4* g) {
+        const float nsr = (float)R.num_shadow_rays;
+        EnvSample es = env_sample(E, rng.next_float2());
+        f3 bc;
+        float pdf_bsdf;
+        eval_cos_pdf(sh, es.dir, &bc, &pdf_bsdf);
+        if (pdf_bsdf == 0.0f || es.pdf == 0.0f) return false;
+        float mis = R.enable_bsdf_di ? 1.0f / (1.0f + pdf_bsdf / (es.pdf * nsr)) : 1.0f;
+        f3 contrib = es.weight * bc / nsr;
+        f3 val = prefix * (mis * contrib) / q;
+        if (is_zero(val)) return false;
+        *o = f4_of(offset_point(s, es.dir), s.err);
+        *d = f4_of(es.dir, INFINITY);
+        *c = f4_of(val, __int_as_float((int)shadow_tag(slot, sc, 0)));
+        if (ADJ && (vtex.x >= 0 || vtex.y >= 0))
+            *g = contribution_gradient(S, sh, s.uv, vtex, sh.f.to_local(es.dir), val, delta);
+        return true;
+    }
+
+    // PerformNextEventEstimation (PathTracer.cs:454-501): one area-light sample.
+    __device__ __forceinline__ bool area_next_event(float4* o, float4* d, float4* c, float4* g) {
+        const float nsr = (float)R.num_shadow_rays;
+        int idx = (int)rng.next_int((uint32_t)R.num_emitters);
+        DevEmitter e = S.emitters[R.emitter_off + idx];
+        float sel = 1.0f / (float)R.num_emitters;
+        f2 b = to_uniform_triangle(rng.next_float2());
+        Surf lp = surface(S, e.tri, b.x, b.y);
+        f3 l2s = normalize(s.pos - lp.pos);
+        f3 em = dot(lp.ns, l2s) < 0.0f ? gray(0.0f) : mk3(e.r, e.g, e.b);
+        float jac = area_to_solid_angle(s.pos, lp.pos, lp.ng);
+        f3 bc;
+        float pdf_bsdf_sa;
+        eval_cos_pdf(sh, -l2s, &bc, &pdf_bsdf_sa);
+        float pdf_ne = e.inv_area * sel * nsr;
+        float pdf_bsdf = pdf_bsdf_sa * jac;
+        f3 val = gray(0.0f);
+        if (jac != 0.0f) {
+            float ratio = pdf_bsdf / pdf_ne;
+            float mis = R.enable_bsdf_di ? 1.0f / (ratio + 1.0f) : 1.0f;
+            float pdf = e.inv_area / jac * sel * nsr;
+            f3 contrib = em / pdf * bc;
+            val = prefix * (mis * contrib) / q;
+        }
+        if (is_zero(val)) return false;
+        f3 dd = lp.pos - s.pos;
+        f3 p0 = offset_point(s, dd);
+        f3 p1 = offset_point(lp, -dd);
+        if (ADJ && (vtex.x >= 0 || vtex.y >= 0))
+            *g = contribution_gradient(S, sh, s.uv, vtex, sh.f.to_local(-l2s), val, delta);
+        *o = f4_of(p0, 1e-5f);
+        *d = f4_of(p1 - p0, 1.0f - 1e-5f);
+        *c = f4_of(val, __int_as_float((int)shadow_tag(slot, sc, 1)));
+        return true;
+    }
+
+    // SampleDirection (PathTracer.cs:526-530) and the path-state update
+    // (:367-378). Returns false if the path ends here.
+    __device__ __forceinline__ bool sample_continuation(float4* no, float4* nd) {
+        float u_comp = rng.next_float();
+        f2 u_dir = rng.next_float2();
+        BsdfSample bs = sample_bsdf(sh, u_comp, u_dir);
+        if (bs.pdf == 0.0f || is_zero(bs.weight)) return false;
+        *no = f4_of(offset_point(s, bs.dir), s.err);
+        *nd = f4_of(bs.dir, INFINITY);
+        next_dir = bs.dir;
+        prefix = prefix * (bs.weight / q);
+        prev_pdf = bs.pdf * q;
+        prev_pos = s.pos;
+        depth = depth + 1;
+        return true;
+    }
+};
+
+// One bounce of EstimateIncidentRadiance for every path of the queue.
 template <bool ADJ>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ADJ ? 1 : SSPT_SHADE_WAVES)))
 k_shade(DevScene S, Wave w, int buf) {
@@ -467,112 +633,40 @@ k_shade(DevScene S, Wave w, int buf) {
         uint32_t qs = active ? w.q_slot[buf][i] : 0u;
         uint32_t slot = qs & kSlotMask, sc = qs >> kSlotBits;
         const SceneRec& R = S.recs[sc];
-        const int nsr_i = R.num_shadow_rays;
-        const float nsr = (float)nsr_i;
-        bool cont = false, do_nee = false;
-        Rng rng{0};
-        f3 prefix = gray(0.0f), L = gray(0.0f), dir = gray(0.0f);
-        Surf s;
-        Shader sh;
-        float q = 1.0f;
-        int depth = 0;
-        f3 delta = gray(0.0f);
-        int2 vtex = make_int2(-1, -1);
-        Env E = env_of(S, R);
+        PathVertex<ADJ> v(S, R, slot, sc);
         if (active) {
-            dir = f3_of(w.q_d[buf][i]);
-            float4 h = w.hits[i];
-            rng.state = w.st_rng[slot];
+            v.dir = f3_of(w.q_d[buf][i]);
+            v.rng.state = w.st_rng[slot];
             float4 thr = w.st_thr[slot];
-            prefix = f3_of(thr);
-            float prev_pdf = thr.w;
+            v.prefix = f3_of(thr);
+            v.prev_pdf = thr.w;
             float4 pv = w.st_prev[slot];
-            depth = __float_as_int(pv.w);
-            f3 prev_pos = f3_of(pv);
-            L = f3_of(w.st_L[slot]);
+            v.depth = __float_as_int(pv.w);
+            v.prev_pos = f3_of(pv);
+            v.L = f3_of(w.st_L[slot]);
             if (ADJ) {
                 // the previous vertex's next-event rays have resolved: snapshot A_{depth-1}
                 uint32_t pix = w.st_pix[slot] & kSlotMask;
                 const float* dl = S.adj_image + R.fb_off + 3ull * pix;
-                delta = mk3(dl[0], dl[1], dl[2]) * R.adj_scale;
-                if (depth >= 2) {
-                    f3 A = L + f3_of(w.st_Le[slot]) + f3_of(w.st_La[slot]);
-                    w.ad_A[(size_t)(depth - 2) * w.capacity + slot] = f4_of(A, 1.0f);
+                v.delta = mk3(dl[0], dl[1], dl[2]) * R.adj_scale;
+                if (v.depth >= 2) {
+                    f3 A = v.L + f3_of(w.st_Le[slot]) + f3_of(w.st_La[slot]);
+                    w.ad_A[(size_t)(v.depth - 2) * w.capacity + slot] = f4_of(A, 1.0f);
                 }
             }
-            const int li = __float_as_int(h.w);
-            if (li < 0) {
-                // OnBackgroundHit (PathTracer.cs:384-400)
-                if (depth >= R.min_depth) {
-                    float mis = 0.0f;
-                    f3 contrib = gray(0.0f);
-                    if (R.has_env && R.enable_bsdf_di) {
-                        mis = 1.0f;
-                        if (depth > 1) {
-                            float pdf_ne = env_pdf(E, dir) * nsr;
-                            mis = 1.0f / (1.0f + pdf_ne / prev_pdf);
-                        }
-                        contrib = env_radiance(E, dir);
-                    }
-                    L = L + prefix * mis * contrib;
-                }
-            } else {
-                s = surface(S, li, h.y, h.z);
-                make_shader(S, s, -dir, &sh);
-                if (ADJ) vtex = vertex_texels(S, sh, s.uv);
-                int ei = S.tri_emitter[li];
-                if (ei >= 0 && depth >= R.min_depth) {
-                    // OnLightHit (PathTracer.cs:402-421)
-                    DevEmitter e = S.emitters[R.emitter_off + ei];
-                    float mis = 1.0f;
-                    if (depth > 1) {
-                        float jac = area_to_solid_angle(prev_pos, s.pos, s.ng);
-                        float pdf_ne = e.inv_area / (float)R.num_emitters * nsr / jac;
-                        float ratio = pdf_ne / prev_pdf;
-                        mis = 1.0f / (ratio + 1.0f);
-                        if (!R.enable_bsdf_di) mis = 0.0f;
-                    }
-                    // DiffuseEmitter.EmittedRadiance: one-sided on the shading normal
-                    f3 em = dot(s.ns, -dir) < 0.0f ? gray(0.0f) : mk3(e.r, e.g, e.b);
-                    L = L + prefix * mis * em;
-                }
-                // ComputeSurvivalProbability (PathTracer.cs:277-282); ApproxThroughput == PrefixWeight
-                q = depth > 4 ? clampf(average(prefix), 0.05f, 0.95f) : 1.0f;
-                bool stop = rng.next_float() > q || depth == R.max_depth;
-                do_nee = !stop && depth + 1 >= R.min_depth;
-                cont = !stop;
-            }
+            v.on_hit(w.hits[i]);
         }
 
-        // Next-event estimation (PathTracer.cs:356-364): per shadow ray, environment
-        // then area light. A shadow ray whose contribution is exactly zero cannot
-        // change the estimate and is not traced; it still counts as an attempt.
-        // The loop bound is uniform over the wave (lanes of other scenes with fewer
-        // shadow rays simply do not take part).
+        // Next-event estimation (PathTracer.cs:356-364): per shadow ray,
+        // environment then area light. The loop bound is uniform over the wave
+        // (lanes of scenes with fewer shadow rays simply do not take part).
+        const int nsr_i = R.num_shadow_rays;
         int max_nsr = nsr_i;
         for (int off = 32; off > 0; off >>= 1) max_nsr = max(max_nsr, __shfl_xor(max_nsr, off));
         for (int k = 0; k < max_nsr; ++k) {
-            const bool nee_k = do_nee && k < nsr_i;
-            // PerformBackgroundNextEvent (PathTracer.cs:423-452)
-            bool env_ray = false;
+            const bool nee_k = v.do_nee && k < nsr_i;
             float4 eo, ed, ec, eg = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if (nee_k && R.has_env) {
-                EnvSample es = env_sample(E, rng.next_float2());
-                f3 bc;
-                float pdf_bsdf;
-                eval_cos_pdf(sh, es.dir, &bc, &pdf_bsdf);
-                if (!(pdf_bsdf == 0.0f || es.pdf == 0.0f)) {
-                    float mis = R.enable_bsdf_di ? 1.0f / (1.0f + pdf_bsdf / (es.pdf * nsr)) : 1.0f;
-                    f3 contrib = es.weight * bc / nsr;
-                    f3 val = prefix * (mis * contrib) / q;
-                    env_ray = !is_zero(val);
-                    eo = f4_of(offset_point(s, es.dir), s.err);
-                    ed = f4_of(es.dir, INFINITY);
-                    ec = f4_of(val, __int_as_float((int)shadow_tag(slot, sc, 0)));
-                    if (ADJ && env_ray && (vtex.x >= 0 || vtex.y >= 0))
-                        eg = contribution_gradient(S, sh, s.uv, vtex, sh.f.to_local(es.dir), val, delta);
-                }
-            }
+            const bool env_ray = nee_k && R.has_env && v.env_next_event(&eo, &ed, &ec, &eg);
             wave_count(&w.stats[kStatShadow], nee_k && R.has_env);
             uint32_t e_idx = wave_append(&w.counters[kCntShadow], env_ray);
             if (env_ray) {
@@ -581,44 +675,11 @@ k_shade(DevScene S, Wave w, int buf) {
                 w.s_c[e_idx] = ec;
                 if (ADJ) {
                     w.s_g[e_idx] = eg;
-                    w.s_t[e_idx] = vtex;
+                    w.s_t[e_idx] = v.vtex;
                 }
             }
-            // PerformNextEventEstimation (PathTracer.cs:454-501)
-            bool area_ray = false;
             float4 ao, ad, ac, ag = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if (nee_k && R.num_emitters > 0) {
-                int idx = (int)rng.next_int((uint32_t)R.num_emitters);
-                DevEmitter e = S.emitters[R.emitter_off + idx];
-                float sel = 1.0f / (float)R.num_emitters;
-                f2 b = to_uniform_triangle(rng.next_float2());
-                Surf lp = surface(S, e.tri, b.x, b.y);
-                f3 l2s = normalize(s.pos - lp.pos);
-                f3 em = dot(lp.ns, l2s) < 0.0f ? gray(0.0f) : mk3(e.r, e.g, e.b);
-                float jac = area_to_solid_angle(s.pos, lp.pos, lp.ng);
-                f3 bc;
-                float pdf_bsdf_sa;
-                eval_cos_pdf(sh, -l2s, &bc, &pdf_bsdf_sa);
-                float pdf_ne = e.inv_area * sel * nsr;
-                float pdf_bsdf = pdf_bsdf_sa * jac;
-                f3 val = gray(0.0f);
-                if (jac != 0.0f) {
-                    float ratio = pdf_bsdf / pdf_ne;
-                    float mis = R.enable_bsdf_di ? 1.0f / (ratio + 1.0f) : 1.0f;
-                    float pdf = e.inv_area / jac * sel * nsr;
-                    f3 contrib = em / pdf * bc;
-                    val = prefix * (mis * contrib) / q;
-                }
-                f3 dd = lp.pos - s.pos;
-                f3 p0 = offset_point(s, dd);
-                f3 p1 = offset_point(lp, -dd);
-                area_ray = !is_zero(val);
-                if (ADJ && area_ray && (vtex.x >= 0 || vtex.y >= 0))
-                    ag = contribution_gradient(S, sh, s.uv, vtex, sh.f.to_local(-l2s), val, delta);
-                ao = f4_of(p0, 1e-5f);
-                ad = f4_of(p1 - p0, 1.0f - 1e-5f);
-                ac = f4_of(val, __int_as_float((int)shadow_tag(slot, sc, 1)));
-            }
+            const bool area_ray = nee_k && R.num_emitters > 0 && v.area_next_event(&ao, &ad, &ac, &ag);
             wave_count(&w.stats[kStatShadow], nee_k && R.num_emitters > 0);
             uint32_t a_idx = wave_append(&w.counters[kCntShadow], area_ray);
             if (area_ray) {
@@ -627,37 +688,28 @@ k_shade(DevScene S, Wave w, int buf) {
                 w.s_c[a_idx] = ac;
                 if (ADJ) {
                     w.s_g[a_idx] = ag;
-                    w.s_t[a_idx] = vtex;
+                    w.s_t[a_idx] = v.vtex;
                 }
             }
         }
 
-        // SampleDirection (PathTracer.cs:526-530) and the path-state update (:367-378)
         float4 no, nd;
+        const bool cont = v.cont && v.sample_continuation(&no, &nd);
         if (cont) {
-            float u_comp = rng.next_float();
-            f2 u_dir = rng.next_float2();
-            BsdfSample bs = sample_bsdf(sh, u_comp, u_dir);
-            if (bs.pdf == 0.0f || is_zero(bs.weight)) {
-                cont = false;
-            } else {
-                no = f4_of(offset_point(s, bs.dir), s.err);
-                nd = f4_of(bs.dir, INFINITY);
-                prefix = prefix * (bs.weight / q);
-                w.st_thr[slot] = f4_of(prefix, bs.pdf * q);
-                w.st_prev[slot] = f4_of(s.pos, __int_as_float(depth + 1));
-                if (ADJ) {
-                    // d log of the sampled BSDF weight (the sampling pdf is detached)
-                    size_t rec = (size_t)(depth - 1) * w.capacity + slot;
-                    w.ad_tex[rec] = vtex;
-                    if (vtex.x >= 0 || vtex.y >= 0) {
-                        D3 f = material_value_dual(S, sh, s.uv, sh.f.to_local(bs.dir), vtex.x >= 0, vtex.y >= 0);
-                        float dl[12];
-                        dlog_of(f, dl);
-                        w.ad_dlog[rec * 3 + 0] = make_float4(dl[0], dl[1], dl[2], dl[3]);
-                        w.ad_dlog[rec * 3 + 1] = make_float4(dl[4], dl[5], dl[6], dl[7]);
-                        w.ad_dlog[rec * 3 + 2] = make_float4(dl[8], dl[9], dl[10], dl[11]);
-                    }
+            w.st_thr[slot] = f4_of(v.prefix, v.prev_pdf);
+            w.st_prev[slot] = f4_of(v.prev_pos, __int_as_float(v.depth));
+            if (ADJ) {
+                // d log of the sampled BSDF value (the sampling pdf is detached)
+                size_t rec = (size_t)(v.depth - 2) * w.capacity + slot;
+                w.ad_tex[rec] = v.vtex;
+                if (v.vtex.x >= 0 || v.vtex.y >= 0) {
+                    D3 f = material_value_dual(S, v.sh, v.s.uv, v.sh.f.to_local(v.next_dir), v.vtex.x >= 0,
+                                               v.vtex.y >= 0);
+                    float dl[12];
+                    dlog_of(f, dl);
+                    w.ad_dlog[rec * 3 + 0] = make_float4(dl[0], dl[1], dl[2], dl[3]);
+                    w.ad_dlog[rec * 3 + 1] = make_float4(dl[4], dl[5], dl[6], dl[7]);
+                    w.ad_dlog[rec * 3 + 2] = make_float4(dl[8], dl[9], dl[10], dl[11]);
                 }
             }
         }
@@ -668,16 +720,74 @@ k_shade(DevScene S, Wave w, int buf) {
             w.q_slot[nb][n_idx] = qs;
         }
         if (active) {
-            w.st_rng[slot] = rng.state;
-            w.st_L[slot] = f4_of(L, 0.0f);
+            w.st_rng[slot] = v.rng.state;
+            w.st_L[slot] = f4_of(v.L, 0.0f);
         }
     }
 }
 
-// Path-replay adjoint, per path after its last bounce: every vertex k that
-// continued passes the contributions collected after it, A_final - A_k, through
-// its sampled BSDF weight; d/dp of that sum is (A_final - A_k) * dlog f_k / dp.
-// Paths whose estimate is not finite are skipped, as the framebuffer skips them.
+// The plain path tracer: one thread follows one path from its camera ray to
+// its end (RenderPixel + EstimateIncidentRadiance in a single kernel, shadow
+// rays traced where they are sampled). Same arithmetic and RNG draws as the
+// wavefront, so both render the same image; it exists as the per-path-launch
+// baseline the wavefront is measured against. Writes the path's radiance sums
+// to the slot for the accumulate kernels.
+__global__ void __launch_bounds__(kTraceBlock) k_path(DevScene S, Wave w) {
+    __shared__ int stack_node[kStackLds * kTraceBlock];
+    __shared__ float stack_t[kStackLds * kTraceBlock];
+    const WaveDesc d = cur_wave(w);
+    for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < d.count; slot += gridDim.x * blockDim.x) {
+        PathStart p = start_path(S, d.start + slot);
+        const SceneRec& R = S.recs[p.scene];
+        PathVertex<false> v(S, R, slot, (uint32_t)p.scene);
+        v.rng = p.rng;
+        v.prefix = gray(1.0f);
+        v.depth = 1;
+        f3 Le = gray(0.0f), La = gray(0.0f);
+        float4 o4 = f4_of(p.o, 0.0f), d4 = f4_of(p.dir, INFINITY);
+        for (;;) {
+            HitOut h = traverse<false>(S, R.root, R.abs_max, f3_of(o4), f3_of(d4), o4.w, d4.w,
+                                       stack_node + threadIdx.x, stack_t + threadIdx.x);
+            wave_count(&w.stats[kStatRays], true);
+            wave_count(&w.stats[kStatHits], h.li >= 0);
+            v.dir = f3_of(d4);
+            v.on_hit(h.li >= 0 ? make_float4(h.t, h.u, h.v, __int_as_float(h.li))
+                               : make_float4(INFINITY, 0.0f, 0.0f, __int_as_float(-1)));
+            for (int k = 0; v.do_nee && k < R.num_shadow_rays; ++k) {
+                float4 so, sd, c, g;
+                if (R.has_env) {
+                    wave_count(&w.stats[kStatShadow], true);
+                    if (v.env_next_event(&so, &sd, &c, &g)) {
+                        bool occ = traverse<true>(S, R.root, R.abs_max, f3_of(so), f3_of(sd), so.w, sd.w,
+                                                  stack_node + threadIdx.x, stack_t + threadIdx.x)
+                                       .li >= 0;
+                        wave_count(&w.stats[kStatShadowTraced], true);
+                        wave_count(&w.stats[kStatOccluded], occ);
+                        if (!occ) Le = Le + f3_of(c);
+                    }
+                }
+                if (R.num_emitters > 0) {
+                    wave_count(&w.stats[kStatShadow], true);
+                    if (v.area_next_event(&so, &sd, &c, &g)) {
+                        bool occ = traverse<true>(S, R.root, R.abs_max, f3_of(so), f3_of(sd), so.w, sd.w,
+                                                  stack_node + threadIdx.x, stack_t + threadIdx.x)
+                                       .li >= 0;
+                        wave_count(&w.stats[kStatShadowTraced], true);
+                        wave_count(&w.stats[kStatOccluded], occ);
+                        if (!occ) La = La + f3_of(c);
+                    }
+                }
+            }
+            if (!v.cont || !v.sample_continuation(&o4, &d4)) break;
+        }
+        w.st_pix[slot] = p.pix | ((uint32_t)p.scene << kSlotBits);
+        w.st_prev[slot] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(v.depth));
+        w.st_L[slot] = f4_of(v.L, 0.0f);
+        w.st_Le[slot] = f4_of(Le, 0.0f);
+        w.st_La[slot] = f4_of(La, 0.0f);
+    }
+}
+
 __global__ void __launch_bounds__(256) k_adjoint_finalize(DevScene S, Wave w) {
     const WaveDesc d = cur_wave(w);
     for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < d.count; slot += gridDim.x * blockDim.x) {
@@ -808,6 +918,11 @@ void launch_bounce_reset(hipStream_t s, const Wave& w, int buf) {
 static uint32_t grid_of(uint32_t configured, uint64_t max_items, uint32_t block) {
     if (configured) return configured;
     return (uint32_t)std::max<uint64_t>(1, (max_items + block - 1) / block);
+}
+
+void launch_path(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc) {
+    hipLaunchKernelGGL(k_path, dim3(grid_of(lc.trace_blocks, lc.max_paths, kTraceBlock)), dim3(kTraceBlock), 0, s, sc,
+                       w);
 }
 
 void launch_generate(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc) {

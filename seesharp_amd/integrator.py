@@ -50,6 +50,11 @@ class Device:
         self.check(self.lib.sspt_set_launch_config(self.ptr, int(trace_blocks), int(shade_blocks),
                                                    1 if use_graphs else 0))
 
+    def set_execution_mode(self, single_kernel: bool):
+        """False: the wavefront tracer (default). True: the plain single-kernel tracer."""
+        self.check(self.lib.sspt_set_execution_mode(self.ptr, abi.EXEC_SINGLE_KERNEL if single_kernel
+                                                    else abi.EXEC_WAVEFRONT))
+
     def launch_config(self) -> dict:
         v = [C.c_uint32() for _ in range(4)]
         self.check(self.lib.sspt_get_launch_config(self.ptr, *[C.byref(x) for x in v]))
