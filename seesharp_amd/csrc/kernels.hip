@@ -133,8 +133,8 @@ __device__ __forceinline__ bool box_test(float lox, float hix, float loy, float 
     float y0 = (loy - o.y) * inv.y, y1 = (hiy - o.y) * inv.y;
     float z0 = (loz - o.z) * inv.z, z1 = (hiz - o.z) * inv.z;
     float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
-    float tf = fminf(fminf(fmaxf(x0, x1) * 1.0000004f, fmaxf(y0, y1) * 1.0000004f),
-                     fminf(fmaxf(z0, z1) * 1.0000004f, tmax));
+    // scaling the minimum equals the minimum of the scaled values (rounding is monotonic)
+    float tf = fminf(fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1)) * 1.0000004f, tmax);
     *tnear = tn;
     return tn <= tf;
 }
@@ -157,12 +157,14 @@ struct HitOut {
 
 // Per-lane traversal stack of (node, entry distance): the first kStackLds
 // entries in LDS ([depth][lane] layout, conflict free), the rest in a private
-// array that only deep trees reach.
+// array that only deep trees reach. The private arrays are separate locals of
+// the caller so that the traversal state around them stays in registers.
+constexpr int kStackSpill = kStackTotal - kStackLds;
 struct TraceStack {
     int* lds_node;    // + threadIdx.x, stride kTraceBlock
     float* lds_t;
-    int spill_node[kStackTotal - kStackLds];
-    float spill_t[kStackTotal - kStackLds];
+    int* spill_node;  // private arrays of kStackSpill entries, owned by the caller
+    float* spill_t;
     int sp;
     __device__ __forceinline__ void push(int node, float t) {
         // The builder rejects trees needing more than kStackTotal entries; the
@@ -199,23 +201,39 @@ __device__ __forceinline__ void cswap(float& ta, int& na, float& tb, int& nb) {
     na = n;
 }
 
-// 4-wide BVH traversal. Closest hit visits hit children nearest first and
-// skips popped nodes whose entry distance is beyond the best hit; ANY = true
-// stops at the first hit (shadow rays).
+// 4-wide BVH traversal of one ray, advanced one node at a time. Closest hit
+// visits hit children nearest first and skips popped nodes whose entry
+// distance is beyond the best hit; ANY = true stops at the first hit (shadow
+// rays). `traverse` runs a ray to its end.
 template <bool ANY>
-__device__ __forceinline__ HitOut traverse(const DevScene& S, int root, float abs_max, f3 o, f3 d, float tmin,
-                                           float tmax, int* lds_node, float* lds_t) {
-    RayPre p = precompute(d);
-    const float slack = ray_slack(abs_max, o, d);
-    const float tmin_box = tmin - slack;
-    HitOut best{tmax, 0.0f, 0.0f, -1, 0x7fffffff};
+struct Tracer {
+    f3 o;
+    RayPre p;
+    float tmin, slack, tmin_box;
+    HitOut best;
     TraceStack st;
-    st.lds_node = lds_node;
-    st.lds_t = lds_t;
-    st.sp = 0;
-    int node = root;
-    for (;;) {
-        if ((unsigned)node >= (unsigned)S.num_nodes) break;  // never taken for a valid tree
+    int node;
+    bool done;
+
+    __device__ __forceinline__ void start(int root, float abs_max, f3 o_, f3 d, float tmin_, float tmax) {
+        o = o_;
+        p = precompute(d);
+        slack = ray_slack(abs_max, o_, d);
+        tmin = tmin_;
+        tmin_box = tmin_ - slack;
+        best = HitOut{tmax, 0.0f, 0.0f, -1, 0x7fffffff};
+        st.sp = 0;
+        node = root;
+        done = false;
+    }
+
+    // Tests the current node's children, intersects the triangles of hit
+    // leaves, then moves to the nearest hit inner child or pops the stack.
+    __device__ __forceinline__ void step(const DevScene& S) {
+        if ((unsigned)node >= (unsigned)S.num_nodes) {  // never taken for a valid tree
+            done = true;
+            return;
+        }
         const float4* N = S.nodes + 8 * node;
         float4 lox = N[0], hix = N[1], loy = N[2], hiy = N[3], loz = N[4], hiz = N[5];
         int4 child = *reinterpret_cast<const int4*>(N + 6);
@@ -245,7 +263,10 @@ __device__ __forceinline__ HitOut traverse(const DevScene& S, int root, float ab
                     best.v = v;
                     best.li = tri;
                     best.gid = gid;
-                    if (ANY) return best;
+                    if (ANY) {
+                        done = true;
+                        return;
+                    }
                 }
             }
         }
@@ -268,21 +289,31 @@ __device__ __forceinline__ HitOut traverse(const DevScene& S, int root, float ab
             if (nhit > 2) st.push(n2, t2);
             if (nhit > 1) st.push(n1, t1);
             node = n0;
-            continue;
+            return;
         }
         // pop the next node still in front of the best hit
-        bool found = false;
         while (st.sp > 0) {
             float tn;
             st.pop(&node, &tn);
-            if (ANY || tn <= best.t + slack) {
-                found = true;
-                break;
-            }
+            if (ANY || tn <= best.t + slack) return;
         }
-        if (!found) break;
+        done = true;
     }
-    return best;
+};
+
+template <bool ANY>
+__device__ __forceinline__ HitOut traverse(const DevScene& S, int root, float abs_max, f3 o, f3 d, float tmin,
+                                           float tmax, int* lds_node, float* lds_t) {
+    int spill_node[kStackSpill];
+    float spill_t[kStackSpill];
+    Tracer<ANY> tr;
+    tr.st.lds_node = lds_node;
+    tr.st.lds_t = lds_t;
+    tr.st.spill_node = spill_node;
+    tr.st.spill_t = spill_t;
+    tr.start(root, abs_max, o, d, tmin, tmax);
+    while (!tr.done) tr.step(S);
+    return tr.best;
 }
 
 // ---------------------------------------------------------------- kernels --

@@ -82,3 +82,4 @@ def test_persistent_and_full_grids_render_identical_images(device):
             assert np.array_equal(g.read_image(), full), (tb, sb)
         finally:
             device.set_launch_config(0, 0, True)
+
