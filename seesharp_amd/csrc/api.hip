@@ -236,6 +236,7 @@ struct WorldBuilder {
     std::vector<DevEmitter> emitters;
     std::vector<float4> env_rgb;
     std::vector<float> env_cdf;
+    std::vector<int> env_guide;
     std::vector<SceneRec> recs;
     uint64_t fb_floats = 0;
     int stack_need = 0;
@@ -336,7 +337,7 @@ sspt_status add_scene(sspt_ctx* ctx, WorldBuilder& W, const sspt_scene_desc* d) 
 
     BvhOutput bvh;
     unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-    build_bvh(BvhInput{verts.data(), (int)ntri}, 4, (int)std::min(hw, 16u), &bvh);
+    build_bvh(BvhInput{verts.data(), (int)ntri}, kMaxLeafTriangles, (int)std::min(hw, 16u), &bvh);
     if (bvh.stack_need > kStackTotal)
         return fail(ctx, SSPT_ERR_UNSUPPORTED, "BVH needs a traversal stack of %d entries (limit %d)", bvh.stack_need,
                     kStackTotal);
@@ -466,6 +467,31 @@ sspt_status add_scene(sspt_ctx* ctx, WorldBuilder& W, const sspt_scene_desc* d) 
             }
             build_cdf(&density[(size_t)r * ew], ew, cdf);
         }
+        // guide tables: guide[j] = lower bound of j / g (the device searches
+        // between guide[j] and guide[j + 1] only)
+        auto pow2_at_least = [](int n) {
+            int g = 1;
+            while (g < n) g <<= 1;
+            return g;
+        };
+        auto build_guide = [](const float* cdf, int n, int g, int* guide) {
+            int i = 0;
+            for (int j = 0; j <= g; ++j) {
+                const float v = (float)j / (float)g;
+                while (i < n && cdf[i] < v) ++i;
+                guide[j] = i;
+            }
+        };
+        R.env_guide_row_g = pow2_at_least(eh);
+        R.env_guide_col_g = pow2_at_least(ew);
+        R.env_guide_row_off = (uint32_t)W.env_guide.size();
+        W.env_guide.resize(W.env_guide.size() + R.env_guide_row_g + 1);
+        build_guide(&W.env_cdf[R.env_row_off], eh, R.env_guide_row_g, &W.env_guide[R.env_guide_row_off]);
+        R.env_guide_col_off = (uint32_t)W.env_guide.size();
+        W.env_guide.resize(W.env_guide.size() + (size_t)eh * (R.env_guide_col_g + 1));
+        for (int r = 0; r < eh; ++r)
+            build_guide(&W.env_cdf[R.env_col_off + (size_t)r * ew], ew, R.env_guide_col_g,
+                        &W.env_guide[R.env_guide_col_off + (size_t)r * (R.env_guide_col_g + 1)]);
     }
     W.recs.push_back(R);
     return SSPT_OK;
@@ -518,6 +544,7 @@ sspt_status create_world(sspt_ctx* ctx, const sspt_scene_desc* descs, uint32_t n
     TRY(upload(ctx, sc.get(), W.emitters.data(), W.emitters.size(), &S.emitters));
     TRY(upload(ctx, sc.get(), W.env_rgb.data(), W.env_rgb.size(), &S.env_rgb));
     TRY(upload(ctx, sc.get(), W.env_cdf.data(), W.env_cdf.size(), &S.env_cdf));
+    TRY(upload(ctx, sc.get(), W.env_guide.data(), W.env_guide.size(), &S.env_guide));
     HIP_TRY(ctx, hipMalloc((void**)&sc->d_recs, W.recs.size() * sizeof(SceneRec)));
     sc->allocs.push_back(sc->d_recs);
     HIP_TRY(ctx, hipMalloc((void**)&sc->d_path_offsets, (W.recs.size() + 1) * sizeof(uint64_t)));

@@ -15,6 +15,12 @@
 namespace sspt {
 namespace {
 
+// SAH bins per axis and split.
+#ifndef SSPT_SAH_BINS
+#define SSPT_SAH_BINS 16
+#endif
+constexpr int kSahBins = SSPT_SAH_BINS;
+
 struct BNode {
     float lo[3], hi[3];
     int left, right;  // inner: child BNode indices
@@ -66,7 +72,7 @@ struct Builder {
                 n.left = n.right = -1;
                 return;
             }
-            constexpr int B = 16;
+            constexpr int B = kSahBins;
             float best_cost = INFINITY;
             int best_axis = -1, best_split = -1;
             float leaf_cost = half_area(n.lo, n.hi) * (float)count;

@@ -22,8 +22,15 @@ struct BvhOutput {
     float scene_lo[3], scene_hi[3];
 };
 
-// Binned SAH (16 bins on centroid bounds), leaves of at most `max_leaf`
-// triangles, depth bounded by kMaxBvhDepth. Uses up to `threads` host threads.
+// Triangles per leaf at most (the builder splits every larger range).
+#ifndef SSPT_MAX_LEAF
+#define SSPT_MAX_LEAF 2
+#endif
+constexpr int kMaxLeafTriangles = SSPT_MAX_LEAF;
+
+// Binned SAH (SSPT_SAH_BINS bins on centroid bounds), leaves of at most
+// `max_leaf` triangles, depth bounded by kMaxBvhDepth. Uses up to `threads`
+// host threads.
 void build_bvh(const BvhInput& in, int max_leaf, int threads, BvhOutput* out);
 
 }  // namespace sspt

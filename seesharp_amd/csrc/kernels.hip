@@ -247,15 +247,28 @@ struct Tracer {
         hc[3] = box_test(lox.w, hix.w, loy.w, hiy.w, loz.w, hiz.w, o, p.inv, tmin_box, tmax_box, &tc[3]);
         const int cidx[4] = {child.x, child.y, child.z, child.w};
         const int ccnt[4] = {count.x, count.y, count.z, count.w};
-        for (int k = 0; k < 4; ++k) hc[k] = hc[k] && ccnt[k] >= 0;  // absent slots
-        // leaf children: intersect their triangles right away
+        uint32_t leaves = 0;  // hit leaf children
         for (int k = 0; k < 4; ++k) {
-            if (!(hc[k] && ccnt[k] > 0)) continue;
-            hc[k] = false;
-            for (int tri = cidx[k]; tri < cidx[k] + ccnt[k]; ++tri) {
-                float4 a = S.tri_pos[3 * tri], b = S.tri_pos[3 * tri + 1], c = S.tri_pos[3 * tri + 2];
-                float t, u, v;
-                if (!intersect_tri(o, p, tmin, best.t, f3_of(a), f3_of(b), f3_of(c), &t, &u, &v)) continue;
+            hc[k] = hc[k] && ccnt[k] >= 0;  // absent slots
+            if (hc[k] && ccnt[k] > 0) {
+                leaves |= 1u << k;
+                hc[k] = false;
+            }
+        }
+        // The triangles of every hit leaf, in one loop per lane: lanes whose
+        // leaves sit in different child slots test their triangles side by side.
+        int tri = 0, left = 0;
+        for (;;) {
+            if (left == 0) {
+                if (leaves == 0) break;
+                const int k = __builtin_ctz(leaves);
+                leaves &= leaves - 1u;
+                tri = k == 0 ? cidx[0] : k == 1 ? cidx[1] : k == 2 ? cidx[2] : cidx[3];
+                left = k == 0 ? ccnt[0] : k == 1 ? ccnt[1] : k == 2 ? ccnt[2] : ccnt[3];
+            }
+            float4 a = S.tri_pos[3 * tri], b = S.tri_pos[3 * tri + 1], c = S.tri_pos[3 * tri + 2];
+            float t, u, v;
+            if (intersect_tri(o, p, tmin, best.t, f3_of(a), f3_of(b), f3_of(c), &t, &u, &v)) {
                 int gid = __float_as_int(a.w);
                 if (ANY || t < best.t || gid < best.gid) {
                     best.t = t;
@@ -269,6 +282,8 @@ struct Tracer {
                     }
                 }
             }
+            ++tri;
+            --left;
         }
         // inner children, nearest first. A hit child's key is clamped to FLT_MAX
         // (its entry distance is +inf when the ray runs parallel to a slab and no

@@ -86,6 +86,9 @@ struct SceneRec {
     uint32_t env_rgb_off;  // float4 index into the environment texel pool
     uint32_t env_row_off;  // float index into the CDF pool (env_h entries)
     uint32_t env_col_off;  // float index into the CDF pool (env_h * env_w entries)
+    uint32_t env_guide_row_off;  // int index into the guide pool: env_guide_row_g + 1 entries
+    uint32_t env_guide_col_off;  // env_h tables of env_guide_col_g + 1 entries
+    int env_guide_row_g, env_guide_col_g;  // guide buckets (powers of two)
     float abs_max;         // largest vertex coordinate magnitude (ray slack scale)
     uint32_t num_pixels;
     // settings (PathTracerBase fields)
@@ -109,6 +112,7 @@ struct DevScene {
     const DevEmitter* emitters;
     const float4* env_rgb;  // pooled environment texels (w unused)
     const float* env_cdf;   // pooled row and column CDFs
+    const int* env_guide;   // pooled CDF guide tables: guide[j] = lower bound of j / G in the CDF
     const SceneRec* recs;
     const uint64_t* path_offsets;  // num_scenes + 1 world-wide path offsets of this render
     const float* adj_image;        // adjoint renders: d loss / d image, framebuffer layout

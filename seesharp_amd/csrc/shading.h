@@ -277,10 +277,21 @@ struct Env {
     const float4* rgb;
     const float* row_cdf;
     const float* col_cdf;
+    const int* row_guide;
+    const int* col_guide;
     int w, h;
+    int row_g, col_g;
 };
 __device__ __forceinline__ Env env_of(const DevScene& S, const SceneRec& R) {
-    return Env{S.env_rgb + R.env_rgb_off, S.env_cdf + R.env_row_off, S.env_cdf + R.env_col_off, R.env_w, R.env_h};
+    return Env{S.env_rgb + R.env_rgb_off,
+               S.env_cdf + R.env_row_off,
+               S.env_cdf + R.env_col_off,
+               S.env_guide + R.env_guide_row_off,
+               S.env_guide + R.env_guide_col_off,
+               R.env_w,
+               R.env_h,
+               R.env_guide_row_g,
+               R.env_guide_col_g};
 }
 __device__ __forceinline__ f2 world_to_spherical(f3 dir) {
     dir = normalize(dir);
@@ -314,9 +325,15 @@ __device__ __forceinline__ float env_pdf(const Env& E, f3 dir) {
     f2 sp = world_to_spherical(dir);
     return grid_pdf(E, sp.x / (2.0f * kPi), sp.y / kPi) / (sinf(sp.y) * kPi * kPi * 2.0f);
 }
-// PiecewiseConstantPDF.Sample (SeeSharp/Sampling/PiecewiseConstant.cs:39-51): lower bound.
-__device__ __forceinline__ int lower_bound(const float* cdf, int n, float u) {
-    int lo = 0, len = n;
+// PiecewiseConstantPDF.Sample (SeeSharp/Sampling/PiecewiseConstant.cs:39-51): the
+// lower bound of u in the CDF (first entry >= u). The guide table holds the
+// lower bounds of j / g (g a power of two, so j = floor(u * g) is exact and
+// j / g <= u <= (j + 1) / g); the lower bound of u lies between guide[j] and
+// guide[j + 1] and is searched there only: the same index as a search of the
+// whole CDF, with a few dependent loads instead of log2(n).
+__device__ __forceinline__ int lower_bound(const float* cdf, const int* guide, int g, float u) {
+    const int j = min((int)(u * (float)g), g - 1);
+    int lo = guide[j], len = guide[j + 1] - lo;
     while (len > 0) {
         int half = len >> 1;
         if (cdf[lo + half] < u) {
@@ -335,11 +352,12 @@ struct EnvSample {
 };
 // EnvironmentMap.SampleDirection (EnvironmentMap.cs:40-73)
 __device__ EnvSample env_sample(const Env& E, f2 u) {
-    int ri = min(lower_bound(E.row_cdf, E.h, u.y), E.h - 1);
+    int ri = min(lower_bound(E.row_cdf, E.row_guide, E.row_g, u.y), E.h - 1);
     float rlo = ri == 0 ? 0.0f : E.row_cdf[ri - 1];
     float rrel = (u.y - rlo) / (E.row_cdf[ri] - rlo);
     const float* ccdf = E.col_cdf + (size_t)ri * E.w;
-    int ci = min(lower_bound(ccdf, E.w, u.x), E.w - 1);
+    const int* cguide = E.col_guide + (size_t)ri * (E.col_g + 1);
+    int ci = min(lower_bound(ccdf, cguide, E.col_g, u.x), E.w - 1);
     float clo = ci == 0 ? 0.0f : ccdf[ci - 1];
     float crel = (u.x - clo) / (ccdf[ci] - clo);
     float px = ((float)ci + crel) / (float)E.w;
