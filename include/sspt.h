@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define SSPT_ABI_VERSION 1
+#define SSPT_ABI_VERSION 2
 
 typedef enum sspt_status {
     SSPT_OK = 0,
@@ -129,6 +129,9 @@ typedef struct sspt_pt_settings {
     int32_t num_shadow_rays;     /* NumShadowRays */
     int32_t enable_bsdf_di;      /* EnableBsdfDI */
     int64_t max_render_time_ms;  /* MaximumRenderTimeMs, < 0 = unlimited */
+    int32_t enable_denoiser;     /* EnableDenoiser: writes the "albedo" and "normal" layers at primary
+                                    hits (DenoiseBuffers.LogPrimaryHit, PathTracer.cs:336-339); the
+                                    OIDN denoiser itself is not run */
 } sspt_pt_settings;
 
 /* TinyEmbree RayTracerStats as stored by PathTracer.cs:257: num_rays counts
@@ -231,6 +234,14 @@ sspt_status sspt_get_launch_config(sspt_ctx* ctx, uint32_t* trace_blocks, uint32
 sspt_status sspt_framebuffer_clear(sspt_ctx* ctx, sspt_scene* scene);
 /* Copies the W*H*3 fp32 running mean to host memory. */
 sspt_status sspt_framebuffer_read(sspt_ctx* ctx, sspt_scene* scene, float* host_rgb);
+/* The denoiser feature layers (DenoiseBuffers, SeeSharp/Integrators/Util/DenoiseBuffers.cs:10-32):
+ * "albedo" = Material.GetScatterStrength and "normal" = the shading normal at the
+ * primary hit, W*H*3 running means like the image (RgbLayer.Splat, Layer.OnStartIteration),
+ * written by renders with enable_denoiser set. `index` selects the scene / view of
+ * a batch (0 for a single scene). */
+typedef enum sspt_layer { SSPT_LAYER_ALBEDO = 0, SSPT_LAYER_NORMAL = 1 } sspt_layer;
+sspt_status sspt_framebuffer_read_layer(sspt_ctx* ctx, sspt_scene* scene, uint32_t index, int32_t layer,
+                                        float* host_rgb);
 /* Number of non-finite path estimates skipped so far (FrameBuffer.NaNWarnings count). */
 sspt_status sspt_framebuffer_nan_count(sspt_ctx* ctx, sspt_scene* scene, uint64_t* count);
 

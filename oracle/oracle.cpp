@@ -1841,6 +1841,49 @@ void or_render_pixel(const or_scene* s, const sspt_pt_settings* cfg, uint32_t ro
     out[0] = L.x, out[1] = L.y, out[2] = L.z;
 }
 
+// DenoiseBuffers.LogPrimaryHit (PathTracer.cs:336-339, DenoiseBuffers.cs:20-23): at the
+// primary hit, Material.GetScatterStrength (the base colour lookup, GenericMaterial.cs:65,
+// DiffuseMaterial.cs:37-38) and the shading normal, splatted into running-mean layers
+// (RgbLayer.Splat: value / n; Layer.OnStartIteration: scale by (n - 1) / n). A primary
+// miss splats nothing. Only the RNG's jitter draws and the primary ray are involved.
+void or_primary_aov(const or_scene* s, const sspt_pt_settings* cfg, uint32_t first, uint32_t num, float* albedo,
+                    float* normal, int32_t threads) {
+    int W = s->width, H = s->height;
+    int64_t total = (int64_t)W * H;
+    for (uint32_t it = first; it < first + num; ++it) {
+        float n = (float)(it + 1);
+        int64_t chunks = (total + 63) / 64;
+        parallel_for(chunks, threads, [&](int64_t ch) {
+            int64_t end = std::min<int64_t>(total, (ch + 1) * 64);
+            for (int64_t pix = ch * 64; pix < end; ++pix) {
+                uint32_t row = (uint32_t)(pix / W), col = (uint32_t)(pix % W);
+                float* a = albedo + pix * 3;
+                float* nm = normal + pix * 3;
+                if (it + 1 > 1) {
+                    float scale = (n - 1.0f) / n;
+                    for (int c = 0; c < 3; ++c) a[c] *= scale, nm[c] *= scale;
+                }
+                Rng rng{hash_seed(cfg->base_seed, row * (uint32_t)W + col, it)};
+                V2 off = rng.next_float2();
+                Ray ray;
+                s->camera.generate_ray((float)col + off.x, (float)row + off.y, &ray.o, &ray.d);
+                ray.tmin = 0.0f;
+                ray.tmax = INFINITY;
+                HitRec h = trace_closest(*s, ray, false);
+                if (h.tri < 0) continue;
+                SurfacePoint hit = make_point(*s, h.tri, {h.u, h.v}, h.t);
+                const Material& m = s->materials[s->tris[h.tri].material];
+                Rgb alb = lookup_rgb(s->textures, m.base_tex, m.base_color, hit.uv);
+                V3 sn = hit.shading_normal;
+                Rgb av = alb / n;
+                V3 nv = sn / n;
+                a[0] += av.x, a[1] += av.y, a[2] += av.z;
+                nm[0] += nv.x, nm[1] += nv.y, nm[2] += nv.z;
+            }
+        });
+    }
+}
+
 int32_t or_set_texture(or_scene* s, uint32_t index, const float* texels) {
     if (index >= s->textures.size()) return 1;
     Texture& t = s->textures[index];

@@ -94,6 +94,12 @@ uint64_t or_render(const or_scene* s, const sspt_pt_settings* settings, uint32_t
 void or_render_pixel(const or_scene* s, const sspt_pt_settings* settings, uint32_t row,
                      uint32_t col, uint32_t iteration, float* rgb3);
 
+/* DenoiseBuffers layers: running means of the primary hit's GetScatterStrength
+ * (albedo) and shading normal over iterations first..first+num-1, W*H*3 each,
+ * updated in place (start from zeros for a fresh framebuffer). */
+void or_primary_aov(const or_scene* s, const sspt_pt_settings* settings, uint32_t first_iteration,
+                    uint32_t num_iterations, float* albedo, float* normal, int32_t num_threads);
+
 /* ---- derivative reference for the path-replay adjoint ----------------------
  * Replaces the texels of texture `index` (scene-description order). Returns 0,
  * or 1 for an index out of range. */

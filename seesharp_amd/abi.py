@@ -13,7 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 # different occupancy targets); the default is the in-tree build.
 LIB_PATH = os.environ.get("SSPT_LIBRARY") or os.path.join(_HERE, "libsspt.so")
 
-ABI_VERSION = 1  # SSPT_ABI_VERSION of include/sspt.h
+ABI_VERSION = 2  # SSPT_ABI_VERSION of include/sspt.h
 SSPT_OK = 0
 INVALID_ARGUMENT = 1
 STATUS_NAMES = {
@@ -25,6 +25,8 @@ STATUS_NAMES = {
     5: "SSPT_ERR_UNSUPPORTED",
     6: "SSPT_ERR_ZERO_AREA_EMITTER",
 }
+LAYER_ALBEDO = 0
+LAYER_NORMAL = 1
 EXEC_WAVEFRONT = 0
 EXEC_SINGLE_KERNEL = 1
 MATERIAL_DIFFUSE = 0
@@ -110,6 +112,7 @@ class PtSettings(C.Structure):
         ("num_shadow_rays", C.c_int32),
         ("enable_bsdf_di", C.c_int32),
         ("max_render_time_ms", C.c_int64),
+        ("enable_denoiser", C.c_int32),
     ]
 
 
@@ -161,6 +164,7 @@ SYMBOLS = {
     "sspt_gradient_clear": (C.c_int, [_vp, _vp]),
     "sspt_gradient_read": (C.c_int, [_vp, _vp, C.c_uint32, _fp]),
     "sspt_texture_write": (C.c_int, [_vp, _vp, C.c_uint32, _fp]),
+    "sspt_framebuffer_read_layer": (C.c_int, [_vp, _vp, C.c_uint32, C.c_int32, _fp]),
     "sspt_set_wave_capacity": (C.c_int, [_vp, C.c_uint32]),
     "sspt_set_execution_mode": (C.c_int, [_vp, C.c_int32]),
     "sspt_get_launch_config": (C.c_int, [_vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),

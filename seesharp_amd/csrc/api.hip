@@ -111,6 +111,7 @@ struct sspt_scene {
     uint64_t texel_floats = 0;
     float* grad = nullptr;
     float* adj_image = nullptr;
+    float* layers = nullptr;  // albedo pool then normal pool, framebuffer layout each
 };
 
 namespace {
@@ -569,6 +570,12 @@ sspt_status create_world(sspt_ctx* ctx, const sspt_scene_desc* descs, uint32_t n
     }
     HIP_TRY(ctx, hipMalloc((void**)&sc->adj_image, W.fb_floats * sizeof(float)));
     sc->allocs.push_back(sc->adj_image);
+    HIP_TRY(ctx, hipMalloc((void**)&sc->layers, 2 * W.fb_floats * sizeof(float)));
+    sc->allocs.push_back(sc->layers);
+    sc->bytes += 2 * W.fb_floats * sizeof(float);
+    HIP_TRY(ctx, hipMemset(sc->layers, 0, 2 * W.fb_floats * sizeof(float)));
+    S.layers = sc->layers;
+    S.layer_stride = W.fb_floats;
     S.grad = sc->grad;
     S.adj_image = sc->adj_image;
     sc->num_tris = W.tri_pos.size() / 3;
@@ -615,6 +622,8 @@ sspt_status ensure_wave(sspt_ctx* ctx, uint32_t capacity) {
     TRY(alloc((void**)&w.st_L, P * 16));
     TRY(alloc((void**)&w.st_Le, P * 16));
     TRY(alloc((void**)&w.st_La, P * 16));
+    TRY(alloc((void**)&w.st_alb, P * 16));
+    TRY(alloc((void**)&w.st_nrm, P * 16));
     TRY(alloc((void**)&w.counters, 64));
     TRY(alloc((void**)&w.stats, 64));
     w.capacity = capacity;
@@ -708,7 +717,8 @@ LaunchConfig launch_config(const sspt_ctx* ctx, uint32_t max_paths, uint32_t max
 enum class Mode { Mean, Sum, Adjoint };
 
 void enqueue_wave(sspt_ctx* ctx, sspt_scene* sc, const LaunchConfig& lc, int max_depth, Mode mode,
-                  float* mean_image, uint32_t mean_pixels) {
+                  float* mean_image, uint32_t mean_pixels, float* mean_albedo) {
+    float* mean_normal = mean_albedo ? mean_albedo + sc->dev.layer_stride : nullptr;
     const DevScene& S = sc->dev;
     const Wave& w = ctx->wave;
     hipStream_t st = ctx->stream;
@@ -717,7 +727,8 @@ void enqueue_wave(sspt_ctx* ctx, sspt_scene* sc, const LaunchConfig& lc, int max
     if (ctx->exec_mode == SSPT_EXEC_SINGLE_KERNEL && !adj) {
         timed(ctx, "path", [&] { launch_path(st, S, w, lc); });
         if (mode == Mode::Mean)
-            timed(ctx, "accumulate", [&] { launch_accumulate_mean(st, w, mean_image, mean_pixels); });
+            timed(ctx, "accumulate",
+                  [&] { launch_accumulate_mean(st, w, mean_image, mean_pixels, mean_albedo, mean_normal); });
         else
             timed(ctx, "accumulate", [&] { launch_accumulate_sum(st, S, w, lc, sc->image); });
         launch_wave_end(st, w);
@@ -738,7 +749,8 @@ void enqueue_wave(sspt_ctx* ctx, sspt_scene* sc, const LaunchConfig& lc, int max
         buf ^= 1;
     }
     if (mode == Mode::Mean)
-        timed(ctx, "accumulate", [&] { launch_accumulate_mean(st, w, mean_image, mean_pixels); });
+        timed(ctx, "accumulate",
+              [&] { launch_accumulate_mean(st, w, mean_image, mean_pixels, mean_albedo, mean_normal); });
     else if (mode == Mode::Sum)
         timed(ctx, "accumulate", [&] { launch_accumulate_sum(st, S, w, lc, sc->image); });
     else
@@ -749,7 +761,8 @@ void enqueue_wave(sspt_ctx* ctx, sspt_scene* sc, const LaunchConfig& lc, int max
 // Runs the given waves: either every wave enqueued directly (kernel timing on,
 // or graphs off), or one wave captured as a graph and replayed.
 sspt_status run_waves(sspt_ctx* ctx, sspt_scene* sc, const std::vector<WaveDesc>& waves, int max_depth,
-                      int max_nsr, Mode mode, float* mean_image, uint32_t mean_pixels, int64_t budget_ms) {
+                      int max_nsr, Mode mode, float* mean_image, uint32_t mean_pixels, float* mean_albedo,
+                      int64_t budget_ms) {
     uint32_t max_paths = 0;
     for (const WaveDesc& d : waves) max_paths = std::max(max_paths, d.count);
     const LaunchConfig lc = launch_config(ctx, max_paths, (uint32_t)std::max(1, max_nsr));
@@ -764,7 +777,7 @@ sspt_status run_waves(sspt_ctx* ctx, sspt_scene* sc, const std::vector<WaveDesc>
     if (graphs) {
         hipGraph_t graph = nullptr;
         HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        enqueue_wave(ctx, sc, lc, max_depth, mode, mean_image, mean_pixels);
+        enqueue_wave(ctx, sc, lc, max_depth, mode, mean_image, mean_pixels, mean_albedo);
         HIP_TRY(ctx, hipStreamEndCapture(ctx->stream, &graph));
         hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
@@ -790,7 +803,7 @@ sspt_status run_waves(sspt_ctx* ctx, sspt_scene* sc, const std::vector<WaveDesc>
                 break;
             }
         } else {
-            enqueue_wave(ctx, sc, lc, max_depth, mode, mean_image, mean_pixels);
+            enqueue_wave(ctx, sc, lc, max_depth, mode, mean_image, mean_pixels, mean_albedo);
         }
         if (ctx->timing.enabled && (i % 4 == 3)) {
             (void)hipStreamSynchronize(ctx->stream);
@@ -831,6 +844,7 @@ void apply_settings(SceneRec& R, const sspt_pt_settings& s) {
     R.min_depth = s.min_depth;
     R.num_shadow_rays = s.num_shadow_rays;
     R.enable_bsdf_di = s.enable_bsdf_di;
+    R.enable_denoiser = s.enable_denoiser;
 }
 
 uint32_t wave_capacity(const sspt_ctx* ctx) {
@@ -921,8 +935,23 @@ sspt_status sspt_framebuffer_clear(sspt_ctx* ctx, sspt_scene* sc) {
     if (!ctx || !sc) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "null argument");
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     HIP_TRY(ctx, hipMemsetAsync(sc->image, 0, sc->image_floats * sizeof(float), ctx->stream));
+    HIP_TRY(ctx, hipMemsetAsync(sc->layers, 0, 2 * sc->image_floats * sizeof(float), ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     sc->nan_count = 0;
+    return SSPT_OK;
+}
+
+sspt_status sspt_framebuffer_read_layer(sspt_ctx* ctx, sspt_scene* sc, uint32_t index, int32_t layer, float* host) {
+    if (!ctx || !sc || !host) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "null argument");
+    if (index >= sc->recs.size()) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "scene index out of range");
+    if (layer != SSPT_LAYER_ALBEDO && layer != SSPT_LAYER_NORMAL)
+        return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "unknown layer %d", layer);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const SceneRec& R = sc->recs[index];
+    const float* src = sc->layers + (layer == SSPT_LAYER_NORMAL ? sc->image_floats : 0) + R.fb_off;
+    HIP_TRY(ctx, hipMemcpyAsync(host, src, (size_t)R.num_pixels * 3 * sizeof(float), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return SSPT_OK;
 }
 
@@ -975,8 +1004,9 @@ sspt_status sspt_render(sspt_ctx* ctx, sspt_scene* sc, const sspt_pt_settings* c
         uint64_t offs[2] = {0, (uint64_t)num_iterations * np};
         HIP_TRY(ctx, hipMemcpyAsync(sc->d_recs, &R, sizeof(SceneRec), hipMemcpyHostToDevice, ctx->stream));
         HIP_TRY(ctx, hipMemcpyAsync(sc->d_path_offsets, offs, sizeof(offs), hipMemcpyHostToDevice, ctx->stream));
+        float* mean_albedo = cfg->enable_denoiser ? sc->layers + R.fb_off : nullptr;
         TRY(run_waves(ctx, sc, waves, cfg->max_depth, cfg->num_shadow_rays, Mode::Mean, sc->image + R.fb_off, np,
-                      cfg->max_render_time_ms));
+                      mean_albedo, cfg->max_render_time_ms));
     }
     return read_stats(ctx, sc, stats);
 }
@@ -1029,14 +1059,21 @@ sspt_status sspt_render_batch(sspt_ctx* ctx, sspt_scene* sc, const sspt_pt_setti
     TRY(plan_batch(ctx, sc, cfgs, 0, false, &plan));
     HIP_TRY(ctx, hipMemsetAsync(ctx->wave.stats, 0, 64, ctx->stream));
     HIP_TRY(ctx, hipMemsetAsync(sc->image, 0, sc->image_floats * sizeof(float), ctx->stream));
+    HIP_TRY(ctx, hipMemsetAsync(sc->layers, 0, 2 * sc->image_floats * sizeof(float), ctx->stream));
     sc->nan_count = 0;
     if (!plan.waves.empty()) {
-        TRY(run_waves(ctx, sc, plan.waves, plan.max_depth, plan.max_nsr, Mode::Sum, nullptr, 0, -1));
+        TRY(run_waves(ctx, sc, plan.waves, plan.max_depth, plan.max_nsr, Mode::Sum, nullptr, 0, nullptr, -1));
         // sums -> means
-        for (size_t i = 0; i < sc->recs.size(); ++i)
-            if (cfgs[i].total_spp > 0)
-                launch_scale(ctx->stream, sc->image + sc->recs[i].fb_off, (uint64_t)sc->recs[i].num_pixels * 3,
-                             1.0f / (float)cfgs[i].total_spp);
+        for (size_t i = 0; i < sc->recs.size(); ++i) {
+            if (cfgs[i].total_spp <= 0) continue;
+            const uint64_t off = sc->recs[i].fb_off, n = (uint64_t)sc->recs[i].num_pixels * 3;
+            const float inv = 1.0f / (float)cfgs[i].total_spp;
+            launch_scale(ctx->stream, sc->image + off, n, inv);
+            if (cfgs[i].enable_denoiser) {
+                launch_scale(ctx->stream, sc->layers + off, n, inv);
+                launch_scale(ctx->stream, sc->layers + sc->image_floats + off, n, inv);
+            }
+        }
         HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     }
     return read_stats(ctx, sc, stats);
@@ -1055,7 +1092,7 @@ sspt_status sspt_render_adjoint_batch(sspt_ctx* ctx, sspt_scene* sc, const sspt_
     HIP_TRY(ctx, hipMemcpyAsync(sc->adj_image, host_dloss_dimages, sc->image_floats * sizeof(float),
                                 hipMemcpyHostToDevice, ctx->stream));
     if (!plan.waves.empty())
-        TRY(run_waves(ctx, sc, plan.waves, plan.max_depth, plan.max_nsr, Mode::Adjoint, nullptr, 0, -1));
+        TRY(run_waves(ctx, sc, plan.waves, plan.max_depth, plan.max_nsr, Mode::Adjoint, nullptr, 0, nullptr, -1));
     return read_stats(ctx, sc, stats);
 }
 
@@ -1093,7 +1130,7 @@ sspt_status sspt_render_adjoint(sspt_ctx* ctx, sspt_scene* sc, const sspt_pt_set
         uint64_t offs[2] = {0, (uint64_t)num_iterations * np};
         HIP_TRY(ctx, hipMemcpyAsync(sc->d_recs, &R, sizeof(SceneRec), hipMemcpyHostToDevice, ctx->stream));
         HIP_TRY(ctx, hipMemcpyAsync(sc->d_path_offsets, offs, sizeof(offs), hipMemcpyHostToDevice, ctx->stream));
-        TRY(run_waves(ctx, sc, waves, cfg->max_depth, cfg->num_shadow_rays, Mode::Adjoint, nullptr, 0, -1));
+        TRY(run_waves(ctx, sc, waves, cfg->max_depth, cfg->num_shadow_rays, Mode::Adjoint, nullptr, 0, nullptr, -1));
     }
     return read_stats(ctx, sc, stats);
 }

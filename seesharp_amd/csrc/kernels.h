@@ -28,6 +28,8 @@ struct Wave {
     float4* st_L;      // radiance from emitter / background hits
     float4* st_Le;     // environment next-event contributions
     float4* st_La;     // area-light next-event contributions
+    float4* st_alb;    // primary-hit albedo (scenes with denoiser layers)
+    float4* st_nrm;    // primary-hit shading normal
     // adjoint renders: per vertex (index depth-1) and slot, [depth * capacity + slot]
     float4* ad_dlog;   // 3 float4 per record: d log f / d(base r,g,b, roughness) per colour channel
     float4* ad_A;      // radiance accumulated after the vertex's next-event rays resolved
@@ -62,7 +64,8 @@ void launch_extend(hipStream_t s, const DevScene& sc, const Wave& w, const Launc
 void launch_shade(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int buf);
 void launch_shadow(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc);
 // running mean of one scene (waves of whole iterations)
-void launch_accumulate_mean(hipStream_t s, const Wave& w, float* image, uint32_t num_pixels);
+void launch_accumulate_mean(hipStream_t s, const Wave& w, float* image, uint32_t num_pixels, float* albedo,
+                            float* normal);
 // per-scene sums of a batch (any wave layout)
 void launch_accumulate_sum(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc,
                            float* image_pool);

@@ -401,6 +401,10 @@ __global__ void __launch_bounds__(256) k_generate(DevScene S, Wave w) {
         w.st_L[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         w.st_Le[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         w.st_La[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (S.recs[p.scene].enable_denoiser) {
+            w.st_alb[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            w.st_nrm[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
     }
 }
 
@@ -535,6 +539,7 @@ struct PathVertex {
     Shader sh;
     float q = 1.0f;
     bool cont = false, do_nee = false;
+    bool surface_hit = false;       // the extension ray hit a surface (s, sh are valid)
     f3 delta = {0.0f, 0.0f, 0.0f};  // ADJ: dLoss/dpixel / iterations
     int2 vtex = {-1, -1};           // ADJ: texel offsets of the vertex's textured parameters
     f3 next_dir = {0.0f, 0.0f, 0.0f};
@@ -549,6 +554,7 @@ struct PathVertex {
         q = 1.0f;
         cont = false;
         do_nee = false;
+        surface_hit = false;
         const int li = __float_as_int(h.w);
         const float nsr = (float)R.num_shadow_rays;
         if (li < 0) {
@@ -569,6 +575,7 @@ struct PathVertex {
         }
         s = surface(S, li, h.y, h.z);
         make_shader(S, s, -dir, &sh);
+        surface_hit = true;
         if (ADJ) vtex = vertex_texels(S, sh, s.uv);
         int ei = S.tri_emitter[li];
         if (ei >= 0 && depth >= R.min_depth) {
@@ -701,6 +708,11 @@ k_shade(DevScene S, Wave w, int buf) {
                 }
             }
             v.on_hit(w.hits[i]);
+            // DenoiseBuffers.LogPrimaryHit (PathTracer.cs:336-339): GetScatterStrength and the shading normal
+            if (R.enable_denoiser && v.depth == 1 && v.surface_hit) {
+                w.st_alb[slot] = f4_of(v.sh.base, 0.0f);
+                w.st_nrm[slot] = f4_of(v.s.ns, 0.0f);
+            }
         }
 
         // Next-event estimation (PathTracer.cs:356-364): per shadow ray,
@@ -790,6 +802,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_path(DevScene S, Wave w) {
         v.prefix = gray(1.0f);
         v.depth = 1;
         f3 Le = gray(0.0f), La = gray(0.0f);
+        f3 alb = gray(0.0f), nrm = gray(0.0f);
         float4 o4 = f4_of(p.o, 0.0f), d4 = f4_of(p.dir, INFINITY);
         for (;;) {
             HitOut h = traverse<false>(S, R.root, R.abs_max, f3_of(o4), f3_of(d4), o4.w, d4.w,
@@ -799,6 +812,10 @@ __global__ void __launch_bounds__(kTraceBlock) k_path(DevScene S, Wave w) {
             v.dir = f3_of(d4);
             v.on_hit(h.li >= 0 ? make_float4(h.t, h.u, h.v, __int_as_float(h.li))
                                : make_float4(INFINITY, 0.0f, 0.0f, __int_as_float(-1)));
+            if (v.depth == 1 && v.surface_hit) {
+                alb = v.sh.base;
+                nrm = v.s.ns;
+            }
             for (int k = 0; v.do_nee && k < R.num_shadow_rays; ++k) {
                 float4 so, sd, c, g;
                 if (R.has_env) {
@@ -831,6 +848,10 @@ __global__ void __launch_bounds__(kTraceBlock) k_path(DevScene S, Wave w) {
         w.st_L[slot] = f4_of(v.L, 0.0f);
         w.st_Le[slot] = f4_of(Le, 0.0f);
         w.st_La[slot] = f4_of(La, 0.0f);
+        if (R.enable_denoiser) {
+            w.st_alb[slot] = f4_of(alb, 0.0f);
+            w.st_nrm[slot] = f4_of(nrm, 0.0f);
+        }
     }
 }
 
@@ -868,10 +889,19 @@ __global__ void __launch_bounds__(256) k_adjoint_finalize(DevScene S, Wave w) {
 // FrameBuffer running mean of one scene (SeeSharp/Images/FrameBuffer.cs:89-160):
 // at iteration n the image is scaled by (n-1)/n and the finite estimate / n added.
 // The wave holds whole iterations, slot = iteration * W*H + pixel.
-__global__ void __launch_bounds__(256) k_accumulate_mean(Wave w, float* image, uint32_t num_pixels) {
+// FrameBuffer running mean over the wave's iterations (FrameBuffer.cs:89-160),
+// and of the albedo / normal layers when they are written (RgbLayer.Splat,
+// Layer.OnStartIteration: the same normalisation, no NaN filter).
+__global__ void __launch_bounds__(256) k_accumulate_mean(Wave w, float* image, uint32_t num_pixels, float* albedo,
+                                                          float* normal) {
     const WaveDesc d = cur_wave(w);
     for (uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x; pix < num_pixels; pix += gridDim.x * blockDim.x) {
         f3 img = mk3(image[3 * pix], image[3 * pix + 1], image[3 * pix + 2]);
+        f3 alb = gray(0.0f), nrm = gray(0.0f);
+        if (albedo) {
+            alb = mk3(albedo[3 * pix], albedo[3 * pix + 1], albedo[3 * pix + 2]);
+            nrm = mk3(normal[3 * pix], normal[3 * pix + 1], normal[3 * pix + 2]);
+        }
         uint32_t nans = 0;
         for (uint32_t s = 0; s < d.iters; ++s) {
             uint32_t slot = s * num_pixels + pix;
@@ -880,10 +910,26 @@ __global__ void __launch_bounds__(256) k_accumulate_mean(Wave w, float* image, u
             if (d.first_iter + s + 1 > 1) img = img * ((n - 1.0f) / n);
             if (isfinite(average(L))) img = img + L / n;
             else ++nans;
+            if (albedo) {
+                if (d.first_iter + s + 1 > 1) {
+                    alb = alb * ((n - 1.0f) / n);
+                    nrm = nrm * ((n - 1.0f) / n);
+                }
+                alb = alb + f3_of(w.st_alb[slot]) / n;
+                nrm = nrm + f3_of(w.st_nrm[slot]) / n;
+            }
         }
         image[3 * pix] = img.x;
         image[3 * pix + 1] = img.y;
         image[3 * pix + 2] = img.z;
+        if (albedo) {
+            albedo[3 * pix] = alb.x;
+            albedo[3 * pix + 1] = alb.y;
+            albedo[3 * pix + 2] = alb.z;
+            normal[3 * pix] = nrm.x;
+            normal[3 * pix + 1] = nrm.y;
+            normal[3 * pix + 2] = nrm.z;
+        }
         if (nans) atomicAdd(&w.stats[kStatNan], (unsigned long long)nans);
     }
 }
@@ -895,11 +941,23 @@ __global__ void __launch_bounds__(256) k_accumulate_sum(DevScene S, Wave w, floa
     for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < d.count; slot += gridDim.x * blockDim.x) {
         f3 L = f3_of(w.st_L[slot]) + f3_of(w.st_Le[slot]) + f3_of(w.st_La[slot]);
         uint32_t tag = w.st_pix[slot];
+        const SceneRec& R = S.recs[tag >> kSlotBits];
+        if (R.enable_denoiser) {
+            float* a = S.layers + R.fb_off + 3ull * (tag & kSlotMask);
+            float* nm = a + S.layer_stride;
+            float4 av = w.st_alb[slot], nv = w.st_nrm[slot];
+            atomicAdd(a, av.x);
+            atomicAdd(a + 1, av.y);
+            atomicAdd(a + 2, av.z);
+            atomicAdd(nm, nv.x);
+            atomicAdd(nm + 1, nv.y);
+            atomicAdd(nm + 2, nv.z);
+        }
         if (!isfinite(average(L))) {
             atomicAdd(&w.stats[kStatNan], 1ull);
             continue;
         }
-        float* px = pool + S.recs[tag >> kSlotBits].fb_off + 3ull * (tag & kSlotMask);
+        float* px = pool + R.fb_off + 3ull * (tag & kSlotMask);
         atomicAdd(px, L.x);
         atomicAdd(px + 1, L.y);
         atomicAdd(px + 2, L.z);
@@ -997,9 +1055,10 @@ void launch_shadow_adjoint(hipStream_t s, const DevScene& sc, const Wave& w, con
 void launch_adjoint_finalize(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc) {
     hipLaunchKernelGGL(k_adjoint_finalize, dim3(grid_of(lc.shade_blocks, lc.max_paths, 256)), dim3(256), 0, s, sc, w);
 }
-void launch_accumulate_mean(hipStream_t s, const Wave& w, float* image, uint32_t num_pixels) {
+void launch_accumulate_mean(hipStream_t s, const Wave& w, float* image, uint32_t num_pixels, float* albedo,
+                            float* normal) {
     hipLaunchKernelGGL(k_accumulate_mean, dim3(std::min(blocks(num_pixels, 256), 8192u)), dim3(256), 0, s, w, image,
-                       num_pixels);
+                       num_pixels, albedo, normal);
 }
 void launch_accumulate_sum(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, float* pool) {
     hipLaunchKernelGGL(k_accumulate_sum, dim3(grid_of(lc.shade_blocks, lc.max_paths, 256)), dim3(256), 0, s, sc, w,

@@ -94,6 +94,7 @@ struct SceneRec {
     // settings (PathTracerBase fields)
     uint32_t base_seed;
     int max_depth, min_depth, num_shadow_rays, enable_bsdf_di;
+    int enable_denoiser;   // write the albedo / normal layers at primary hits
     // current render
     uint32_t first_iter;   // iteration index of the scene's first path in this render
     float adj_scale;       // adjoint renders: 1 / number of iterations (d image / d sample)
@@ -117,6 +118,8 @@ struct DevScene {
     const uint64_t* path_offsets;  // num_scenes + 1 world-wide path offsets of this render
     const float* adj_image;        // adjoint renders: d loss / d image, framebuffer layout
     float* grad;                   // adjoint renders: d loss / d texel, texel-pool layout
+    float* layers;                 // albedo layers (framebuffer layout), then normal layers at + layer_stride
+    uint64_t layer_stride;
     int num_scenes;
     int num_nodes;
 };

@@ -153,6 +153,10 @@ class GpuScene(_TextureParameters):
         self.device.check(self.device.lib.sspt_framebuffer_read(self.device.ptr, self.ptr, abi.fptr(img)))
         return img
 
+    def read_layer(self, name: str) -> np.ndarray:
+        """The "albedo" or "normal" denoiser layer (H, W, 3) of renders with enable_denoiser."""
+        return _read_layer(self.device, self.ptr, 0, name, self.height, self.width)
+
     def nan_count(self) -> int:
         n = C.c_uint64()
         self.device.check(self.device.lib.sspt_framebuffer_nan_count(self.device.ptr, self.ptr, C.byref(n)))
@@ -288,12 +292,27 @@ class GpuBatch(_TextureParameters):
                                                                     first_iteration, C.byref(stats)))
         return stats.as_dict()
 
+    def read_layer(self, index: int, name: str) -> np.ndarray:
+        h, w = self.sizes[index]
+        return _read_layer(self.device, self.ptr, index, name, h, w)
+
     def read_image(self, index: int) -> np.ndarray:
         h, w = self.sizes[index]
         img = np.zeros((h, w, 3), np.float32)
         self.device.check(self.device.lib.sspt_framebuffer_read_scene(self.device.ptr, self.ptr, index,
                                                                       abi.fptr(img)))
         return img
+
+
+_LAYERS = {"albedo": abi.LAYER_ALBEDO, "normal": abi.LAYER_NORMAL}
+
+
+def _read_layer(device, ptr, index, name, h, w):
+    if name not in _LAYERS:
+        raise KeyError(f"unknown layer {name!r} (albedo, normal)")
+    img = np.zeros((h, w, 3), np.float32)
+    device.check(device.lib.sspt_framebuffer_read_layer(device.ptr, ptr, index, _LAYERS[name], abi.fptr(img)))
+    return img
 
 
 def _hits(hits):
@@ -326,10 +345,11 @@ class GpuPathTracer(Integrator):
         self.MaximumRenderTimeMs = None
         self.NumShadowRays = 1
         self.EnableBsdfDI = True
-        # The OIDN denoiser and the technique pyramid are separate products; the
-        # image the path tracer writes does not depend on them.
+        # EnableDenoiser (default true, PathTracer.cs:49) writes the "albedo" and
+        # "normal" feature layers; running the OIDN denoiser on them is a separate
+        # product and not part of this tracer. The technique pyramid is not supported.
         self.RenderTechniquePyramid = False
-        self.EnableDenoiser = False
+        self.EnableDenoiser = True
         self.device = device
         self._gpu_scene = None
 
@@ -337,7 +357,8 @@ class GpuPathTracer(Integrator):
         return abi.PtSettings(
             self.BaseSeed & 0xFFFFFFFF, self.TotalSpp, self.MaxDepth, self.MinDepth, self.NumShadowRays,
             1 if self.EnableBsdfDI else 0,
-            -1 if self.MaximumRenderTimeMs is None else int(self.MaximumRenderTimeMs))
+            -1 if self.MaximumRenderTimeMs is None else int(self.MaximumRenderTimeMs),
+            1 if self.EnableDenoiser else 0)
 
     def prepare(self, scene) -> GpuScene:
         """Scene.Prepare on the device (BVH build, emitter maps, env grid)."""
@@ -357,6 +378,9 @@ class GpuPathTracer(Integrator):
         stats = gs.render(self.settings(), 0, self.TotalSpp)
         render_ms = (time.perf_counter() - t0) * 1000.0
         fb.Image = gs.read_image()
+        if self.EnableDenoiser:
+            fb.Layers["albedo"] = gs.read_layer("albedo")
+            fb.Layers["normal"] = gs.read_layer("normal")
         fb.CurIteration = self.TotalSpp
         fb.NaNWarnings = gs.nan_count()
         fb.MetaData["RenderTime"] = render_ms

@@ -218,6 +218,9 @@ class FrameBuffer:
         self.CurIteration = 0
         self.NaNWarnings = 0
         self.MetaData = {}
+        # AOV layers by name (FrameBuffer.AddLayer); the GPU path tracer fills
+        # "albedo" and "normal" when EnableDenoiser is set (DenoiseBuffers).
+        self.Layers = {}
 
 
 class Scene:

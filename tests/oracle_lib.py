@@ -56,6 +56,7 @@ _SIGS = {
                                C.POINTER(abi.RayStats), C.POINTER(C.c_int32), C.c_int64, C.c_int32]),
     "or_render_pixel": (None, [_vp, C.POINTER(abi.PtSettings), C.c_uint32, C.c_uint32, C.c_uint32, _fp]),
     "or_set_texture": (C.c_int32, [_vp, C.c_uint32, _fp]),
+    "or_primary_aov": (None, [_vp, C.POINTER(abi.PtSettings), C.c_uint32, C.c_uint32, _fp, _fp, C.c_int32]),
     "or_render_tangent": (C.c_uint64, [_vp, C.POINTER(abi.PtSettings), C.c_uint32, C.c_uint32, C.c_uint32, _fp,
                                        C.c_float, _fp, C.POINTER(C.c_int32), C.c_int64, C.c_int32]),
 }
@@ -86,9 +87,9 @@ def fp(a):
 
 
 def settings(base_seed=0xC030114, total_spp=1, max_depth=5, min_depth=1, num_shadow_rays=1,
-             enable_bsdf_di=True, max_render_time_ms=-1):
+             enable_bsdf_di=True, max_render_time_ms=-1, enable_denoiser=False):
     return abi.PtSettings(base_seed, total_spp, max_depth, min_depth, num_shadow_rays,
-                          1 if enable_bsdf_di else 0, max_render_time_ms)
+                          1 if enable_bsdf_di else 0, max_render_time_ms, 1 if enable_denoiser else 0)
 
 
 class OracleScene:
@@ -122,6 +123,13 @@ class OracleScene:
             n = pixels.size
         nans = lib().or_render(self.ptr, C.byref(cfg), first, num, fp(image), C.byref(stats), px, n, threads)
         return image, stats, int(nans)
+
+    def primary_aov(self, cfg, first=0, num=1, threads=0):
+        """(albedo, normal) running-mean layers of iterations first..first+num-1."""
+        alb = np.zeros((self.height, self.width, 3), np.float32)
+        nrm = np.zeros((self.height, self.width, 3), np.float32)
+        lib().or_primary_aov(self.ptr, C.byref(cfg), first, num, fp(alb), fp(nrm), threads)
+        return alb, nrm
 
     def set_texture(self, index, texels):
         texels = np.ascontiguousarray(texels, np.float32)
