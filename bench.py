@@ -13,6 +13,7 @@ oracle restatement of the reference, timed on a bounded pixel sample).
     python bench.py --workload adjoint   # path-replay adjoint vs forward cost (512^2 textured box)
     python bench.py --workload views     # 32 views of the 1M-tri scene with 1024^2 ground textures: forward + adjoint
     python bench.py --workload copy      # plain 1 GiB device copy: HBM counter calibration for rocprofv3 --pmc
+    python bench.py --workload cornell   # the reference's CornellBox.json 512^2 512 spp render vs its recorded times
 """
 from __future__ import annotations
 
@@ -344,6 +345,65 @@ def copy_main(args):
                       "ms_per_launch": round(ms, 3), "launches": max(1, args.steps) + 1}), flush=True)
 
 
+CORNELL_SCENE = os.path.join(ROOT, "tests", "golden", "scenes", "CornellBox", "CornellBox.json")
+# Render times recorded with the reference's CornellBox reference renders
+# (Data/Scenes/CornellBox/References/MaxDepth{5,2}-Width512-Height512.json:2),
+# configured by Data/Scenes/CornellBox/References/Config.json:1-8.
+CORNELL_REFERENCE_MS = {5: 142532.0, 2: 26768.0}
+
+
+def cornell_main(args):
+    """The reference's recorded render: CornellBox.json (36 triangles) at 512^2,
+    512 spp, BaseSeed 571298512, MaxDepth 5 and 2, denoiser feature layers on
+    (the reference's default). Timed: the render loop only, scene already on the
+    device, as the reference's RenderTimer times it."""
+    from seesharp_amd import Device, GpuScene, abi, sceneio
+
+    dev = Device(0)
+    scene = sceneio.load_scene(CORNELL_SCENE, 512, 512)
+    g = GpuScene(dev, scene)
+    spp = 512
+    res = {}
+    for depth in (5, 2):
+        cfg = abi.PtSettings(571298512, spp, depth, 1, 1, 1, -1, 1)
+        for _ in range(max(0, args.warmup)):
+            g.clear()
+            g.render(cfg, 0, spp)
+        ms = []
+        for _ in range(max(1, args.steps)):
+            g.clear()
+            t0 = time.perf_counter()
+            st = g.render(cfg, 0, spp)
+            ms.append((time.perf_counter() - t0) * 1000.0)
+        res[depth] = (float(np.mean(ms)), st)
+        log(f"[bench] CornellBox MaxDepth {depth}: {res[depth][0]:.2f} ms per {spp} spp")
+    ms5, st5 = res[5]
+    npaths = 512 * 512 * spp
+    out = {
+        "metric": "CornellBox 512x512 512spp MaxDepth 5 render time",
+        "value": round(ms5, 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms5, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": round(ms5 / CORNELL_REFERENCE_MS[5], 6),
+        "dtype": "f32",
+        "data": "the reference's CornellBox.json scene file",
+        "config": {"workload": "CornellBox.json (36 triangles, 2 emitters) 512x512, 512 spp, BaseSeed 571298512, "
+                               "NEE + balance MIS, denoiser feature layers on",
+                   "max_depth": 5, "paths_per_step": npaths},
+        "mpaths_per_s": round(npaths / ms5 / 1e3, 3),
+        "ray_stats_per_step": st5,
+        "speedup_vs_reference_render_time": round(CORNELL_REFERENCE_MS[5] / ms5, 1),
+        "max_depth_2": {"ms": round(res[2][0], 3), "reference_ms": CORNELL_REFERENCE_MS[2],
+                        "speedup_vs_reference_render_time": round(CORNELL_REFERENCE_MS[2] / res[2][0], 1)},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -354,7 +414,7 @@ def main():
     ap.add_argument("--max-depth", type=int, default=8)
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("large", "batch", "adjoint", "views", "copy"), default="large")
+    ap.add_argument("--workload", choices=("large", "batch", "adjoint", "views", "copy", "cornell"), default="large")
     ap.add_argument("--autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -373,6 +433,9 @@ def main():
         return
     if args.workload == "views":
         views_main(args)
+        return
+    if args.workload == "cornell":
+        cornell_main(args)
         return
 
     from seesharp_amd import Device, GpuScene, scenes
