@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define SSPT_ABI_VERSION 2
+#define SSPT_ABI_VERSION 3
 
 typedef enum sspt_status {
     SSPT_OK = 0,
@@ -84,12 +84,23 @@ typedef struct sspt_texture_desc {
     const float* texels;   /* row-major, channels floats per texel */
 } sspt_texture_desc;
 
-/* DiffuseEmitter on one triangle (SeeSharp/Shading/Emitters/DiffuseEmitter.cs:6-37),
- * listed in Scene.Emitters order: next-event estimation indexes that order. */
+typedef enum sspt_emitter_kind {
+    /* DiffuseEmitter (SeeSharp/Shading/Emitters/DiffuseEmitter.cs:6-37): radiance
+     * on the side the shading normal faces */
+    SSPT_EMITTER_DIFFUSE = 0,
+    /* GlossyEmitter (SeeSharp/Shading/Emitters/GlossyEmitter.cs:7-40): radiance *
+     * cos^exponent * (exponent + 1) / (2 pi) on that side */
+    SSPT_EMITTER_GLOSSY = 1
+} sspt_emitter_kind;
+
+/* One emissive triangle, listed in Scene.Emitters order: next-event estimation
+ * indexes that order. */
 typedef struct sspt_emitter_desc {
     uint32_t mesh;
     uint32_t face;
     float radiance[3];
+    int32_t kind;          /* sspt_emitter_kind */
+    float exponent;        /* GlossyEmitter only: finite and > -1 */
 } sspt_emitter_desc;
 
 /* EnvironmentMap lat-long image (SeeSharp/Shading/Background/EnvironmentMap.cs:13-189). */

@@ -504,6 +504,9 @@ struct Emitter {
     int tri;
     Rgb radiance;
     float inv_area;
+    int kind;        // sspt_emitter_kind
+    float exponent;  // GlossyEmitter lobe exponent
+    float norm;      // GlossyEmitter ctor (GlossyEmitter.cs:14): (exponent + 1) / (2 pi)
 };
 
 struct EnvMap {
@@ -1218,8 +1221,14 @@ inline float area_to_solid_angle(const SurfacePoint& from, const SurfacePoint& t
     return std::fabs(dot(to.normal, -d)) / (d2 * std::sqrt(d2));
 }
 
-// DiffuseEmitter.EmittedRadiance (DiffuseEmitter.cs:33-37)
+// DiffuseEmitter.EmittedRadiance (DiffuseEmitter.cs:33-37) and
+// GlossyEmitter.EmittedRadiance (GlossyEmitter.cs:36-40)
 inline Rgb emitted(const Emitter& e, const SurfacePoint& p, V3 dir) {
+    if (e.kind == SSPT_EMITTER_GLOSSY) {
+        float cosine = dot(p.shading_normal, dir) / std::sqrt(dot(dir, dir));
+        if (cosine < 0.0f) return rgb(0.0f);
+        return e.radiance * std::pow(cosine, e.exponent) * e.norm;
+    }
     if (dot(p.shading_normal, dir) < 0.0f) return rgb(0.0f);
     return e.radiance;
 }
@@ -1695,7 +1704,20 @@ or_scene* or_scene_create(const sspt_scene_desc* d, char* err) {
             return nullptr;
         }
         s->tri_emitter[tri] = (int)i;
-        s->emitters.push_back({tri, v3(e.radiance[0], e.radiance[1], e.radiance[2]), 1.0f / area});
+        if (e.kind != SSPT_EMITTER_DIFFUSE && e.kind != SSPT_EMITTER_GLOSSY) {
+            set_err(err, "unknown emitter kind");
+            delete s;
+            return nullptr;
+        }
+        const bool glossy = e.kind == SSPT_EMITTER_GLOSSY;
+        if (glossy && !(std::isfinite(e.exponent) && e.exponent > -1.0f)) {
+            set_err(err, "glossy exponent must be finite and > -1");
+            delete s;
+            return nullptr;
+        }
+        const float kPiF = 3.14159265358979323846f;
+        s->emitters.push_back({tri, v3(e.radiance[0], e.radiance[1], e.radiance[2]), 1.0f / area, e.kind,
+                               glossy ? e.exponent : 0.0f, glossy ? (e.exponent + 1.0f) / (2.0f * kPiF) : 1.0f});
     }
     if (d->background) {
         s->has_env = true;

@@ -10,6 +10,7 @@ from .scene import (  # noqa: F401
     EnvironmentMap,
     FrameBuffer,
     GenericMaterial,
+    GlossyEmitter,
     Mesh,
     PerspectiveCamera,
     Scene,

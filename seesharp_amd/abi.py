@@ -13,7 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 # different occupancy targets); the default is the in-tree build.
 LIB_PATH = os.environ.get("SSPT_LIBRARY") or os.path.join(_HERE, "libsspt.so")
 
-ABI_VERSION = 2  # SSPT_ABI_VERSION of include/sspt.h
+ABI_VERSION = 3  # SSPT_ABI_VERSION of include/sspt.h
 SSPT_OK = 0
 INVALID_ARGUMENT = 1
 STATUS_NAMES = {
@@ -25,6 +25,8 @@ STATUS_NAMES = {
     5: "SSPT_ERR_UNSUPPORTED",
     6: "SSPT_ERR_ZERO_AREA_EMITTER",
 }
+EMITTER_DIFFUSE = 0
+EMITTER_GLOSSY = 1
 LAYER_ALBEDO = 0
 LAYER_NORMAL = 1
 EXEC_WAVEFRONT = 0
@@ -75,7 +77,8 @@ class TextureDesc(C.Structure):
 
 
 class EmitterDesc(C.Structure):
-    _fields_ = [("mesh", C.c_uint32), ("face", C.c_uint32), ("radiance", C.c_float * 3)]
+    _fields_ = [("mesh", C.c_uint32), ("face", C.c_uint32), ("radiance", C.c_float * 3), ("kind", C.c_int32),
+                ("exponent", C.c_float)]
 
 
 class EnvmapDesc(C.Structure):

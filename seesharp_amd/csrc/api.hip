@@ -333,7 +333,21 @@ sspt_status add_scene(sspt_ctx* ctx, WorldBuilder& W, const sspt_scene_desc* d) 
         float n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
         float area = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]) * 0.5f;
         if (area == 0.0f) return fail(ctx, SSPT_ERR_ZERO_AREA_EMITTER, "Zero-area triangles cannot be emitters");
-        emitters[i] = DevEmitter{(int)t, e.radiance[0], e.radiance[1], e.radiance[2], 1.0f / area, {0, 0, 0}};
+        if (e.kind != SSPT_EMITTER_DIFFUSE && e.kind != SSPT_EMITTER_GLOSSY)
+            return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "emitter %u has unknown kind %d", i, e.kind);
+        if (e.kind == SSPT_EMITTER_GLOSSY && !(std::isfinite(e.exponent) && e.exponent > -1.0f))
+            return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "emitter %u: glossy exponent must be finite and > -1", i);
+        DevEmitter de{};
+        de.tri = (int)t;
+        de.r = e.radiance[0];
+        de.g = e.radiance[1];
+        de.b = e.radiance[2];
+        de.inv_area = 1.0f / area;
+        de.kind = e.kind;
+        de.exponent = e.kind == SSPT_EMITTER_GLOSSY ? e.exponent : 0.0f;
+        // GlossyEmitter ctor (GlossyEmitter.cs:14): the lobe keeps a diffuse emitter's power
+        de.norm = e.kind == SSPT_EMITTER_GLOSSY ? (e.exponent + 1.0f) / (2.0f * 3.14159265358979323846f) : 1.0f;
+        emitters[i] = de;
     }
 
     BvhOutput bvh;

@@ -514,6 +514,18 @@ __device__ __forceinline__ float4 contribution_gradient(const DevScene& S, const
     return make_float4(g[0], g[1], g[2], g[3]);
 }
 
+// Emitter.EmittedRadiance at a point with shading normal ns, towards dir:
+// DiffuseEmitter (DiffuseEmitter.cs:33-37) is one-sided on ns; GlossyEmitter
+// (GlossyEmitter.cs:36-40) scales by cos^exponent * (exponent + 1) / (2 pi).
+__device__ __forceinline__ f3 emitted(const DevEmitter& e, f3 ns, f3 dir) {
+    if (e.kind == kEmitterGlossy) {
+        float cosine = dot(ns, dir) / length(dir);
+        if (cosine < 0.0f) return gray(0.0f);
+        return mk3(e.r, e.g, e.b) * powf(cosine, e.exponent) * e.norm;
+    }
+    return dot(ns, dir) < 0.0f ? gray(0.0f) : mk3(e.r, e.g, e.b);
+}
+
 // One vertex of EstimateIncidentRadiance (PathTracer.cs:315-382) for one path:
 // the hit of its extension ray (background / emitter with MIS, Russian
 // roulette), its next-event rays (environment, then area light, per shadow
@@ -588,8 +600,7 @@ struct PathVertex {
                 mis = 1.0f / (ratio + 1.0f);
                 if (!R.enable_bsdf_di) mis = 0.0f;
             }
-            // DiffuseEmitter.EmittedRadiance: one-sided on the shading normal
-            f3 em = dot(s.ns, -dir) < 0.0f ? gray(0.0f) : mk3(e.r, e.g, e.b);
+            f3 em = emitted(e, s.ns, -dir);
             L = L + prefix * mis * em;
         }
         q = depth > 4 ? clampf(average(prefix), 0.05f, 0.95f) : 1.0f;
@@ -629,7 +640,7 @@ struct PathVertex {
         f2 b = to_uniform_triangle(rng.next_float2());
         Surf lp = surface(S, e.tri, b.x, b.y);
         f3 l2s = normalize(s.pos - lp.pos);
-        f3 em = dot(lp.ns, l2s) < 0.0f ? gray(0.0f) : mk3(e.r, e.g, e.b);
+        f3 em = emitted(e, lp.ns, l2s);
         float jac = area_to_solid_angle(s.pos, lp.pos, lp.ng);
         f3 bc;
         float pdf_bsdf_sa;

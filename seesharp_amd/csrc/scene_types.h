@@ -60,11 +60,15 @@ struct DevTexture {
     int w, h, ch, border;
 };
 
+constexpr int kEmitterGlossy = 1;  // DevEmitter::kind of a GlossyEmitter (0 = DiffuseEmitter)
+
 struct DevEmitter {  // 32 B
     int tri;  // leaf-order triangle index (world-wide)
     float r, g, b;
     float inv_area;
-    int pad[3];
+    int kind;        // sspt_emitter_kind
+    float exponent;  // glossy lobe exponent
+    float norm;      // glossy: (exponent + 1) / (2 pi)
 };
 
 struct DevCamera {

@@ -158,6 +158,27 @@ class DiffuseEmitter:
         area = np.linalg.norm(np.cross(v[i[:, 1]] - v[i[:, 0]], v[i[:, 2]] - v[i[:, 0]]), axis=1) * 0.5
         return [DiffuseEmitter(mesh, f, radiance) for f in range(mesh.NumFaces) if area[f] != 0.0]
 
+    Kind = 0  # sspt_emitter_kind
+    Exponent = 0.0
+
+
+class GlossyEmitter(DiffuseEmitter):
+    """One emissive triangle whose emission is scaled by a cosine lobe about the
+    shading normal, radiance * cos^exponent * (exponent + 1) / (2 pi), which keeps
+    the total power of a DiffuseEmitter (SeeSharp/Shading/Emitters/GlossyEmitter.cs:7-40)."""
+
+    Kind = 1
+
+    def __init__(self, mesh: Mesh, face: int, radiance, exponent: float):
+        super().__init__(mesh, face, radiance)
+        self.Exponent = float(np.float32(exponent))
+
+    @staticmethod
+    def MakeFromMesh(mesh: Mesh, radiance, exponent: float):
+        """GlossyEmitter.cs:24-34: one emitter per face, zero-area faces skipped."""
+        return [GlossyEmitter(e.Mesh, e.FaceIndex, e.Radiance, exponent)
+                for e in DiffuseEmitter.MakeFromMesh(mesh, radiance)]
+
 
 class EnvironmentMap:
     """Lat-long RGB environment (EnvironmentMap.cs:13-17); +Y up, phi = atan2(z, x)."""
@@ -323,6 +344,8 @@ class Scene:
             emitters[k].mesh = mesh_index[id(e.Mesh)]
             emitters[k].face = e.FaceIndex
             emitters[k].radiance[:] = e.Radiance
+            emitters[k].kind = e.Kind
+            emitters[k].exponent = e.Exponent
         mats = (abi.MaterialDesc * max(1, len(materials)))(*materials)
         texs = (abi.TextureDesc * max(1, len(textures)))(*textures)
         d = abi.SceneDesc()
