@@ -22,8 +22,8 @@ in behaviour:
   equally named .obj materials (ObjConverter.cs:52-205);
 - background {"type": "image"} -> EnvironmentMap of the image.
 
-Images are read by seesharp_amd.imageio. Glossy emitters are not part of the
-path tracer and raise NotImplementedError.
+Images are read by seesharp_amd.imageio. Emissive materials make a
+DiffuseEmitter or, with "emissionIsGlossy", a GlossyEmitter per triangle.
 """
 from __future__ import annotations
 
@@ -41,6 +41,7 @@ from .scene import (
     EnvironmentMap,
     FrameBuffer,
     GenericMaterial,
+    GlossyEmitter,
     Mesh,
     PerspectiveCamera,
     Scene,
@@ -161,9 +162,8 @@ def read_materials(root: dict, base_dir: str):
         if "emission" in m:
             e = tuple(float(x) for x in m["emission"]["value"][:3])
             if any(c != 0.0 for c in e):
-                if m.get("emissionIsGlossy", False):
-                    raise NotImplementedError(f"material {name}: glossy emitters are not supported")
-                emissive[name] = e
+                # EmissionParameters (JsonScene.cs:93-104): glossy lobe exponent defaults to 50
+                emissive[name] = (e, bool(m.get("emissionIsGlossy", False)), float(m.get("emissionExponent", 50.0)))
     return named, emissive
 
 
@@ -171,6 +171,16 @@ def read_materials(root: dict, base_dir: str):
 def _f32(v, n):
     a = np.asarray(v, np.float32)
     return a.reshape(-1, n)
+
+
+def _material_emitters(mesh: Mesh, params):
+    """Emitters of a mesh whose material is emissive (TriMeshLoader.cs:53-56,
+    PlyLoader.cs:31-34, ObjConverter.cs:200-203): glossy or diffuse per the
+    material's emission parameters."""
+    radiance, glossy, exponent = params
+    if glossy:
+        return GlossyEmitter.MakeFromMesh(mesh, radiance, exponent)
+    return DiffuseEmitter.MakeFromMesh(mesh, radiance)
 
 
 def load_trimesh(obj: dict, named, emissive):
@@ -182,7 +192,7 @@ def load_trimesh(obj: dict, named, emissive):
     mesh.Material = named[name]
     emitters = []
     if name in emissive:
-        emitters = DiffuseEmitter.MakeFromMesh(mesh, emissive[name])
+        emitters = _material_emitters(mesh, emissive[name])
     elif "emission" in obj:
         emitters = DiffuseEmitter.MakeFromMesh(mesh, tuple(float(x) for x in obj["emission"]["value"][:3]))
     return [mesh], emitters
@@ -318,7 +328,7 @@ def load_obj(path: str, named, emissive):
                 mesh.Material = materials[name]
                 meshes.append(mesh)
                 if name in emissive:
-                    emitters += DiffuseEmitter.MakeFromMesh(mesh, emissive[name])
+                    emitters += _material_emitters(mesh, emissive[name])
                 elif name in obj_emission:
                     emitters += DiffuseEmitter.MakeFromMesh(mesh, obj_emission[name])
     return meshes, emitters
@@ -401,7 +411,7 @@ def load_ply(path: str, obj: dict, named, emissive):
     mesh.Material = named[name]
     emitters = []
     if name in emissive:
-        emitters = DiffuseEmitter.MakeFromMesh(mesh, emissive[name])
+        emitters = _material_emitters(mesh, emissive[name])
     elif "emission" in obj:
         emitters = DiffuseEmitter.MakeFromMesh(mesh, tuple(float(x) for x in obj["emission"]["value"][:3]))
     return [mesh], emitters

@@ -9,7 +9,7 @@ import zlib
 import numpy as np
 import pytest
 
-from seesharp_amd import DiffuseMaterial, GenericMaterial, TextureRgb, imageio, sceneio
+from seesharp_amd import DiffuseEmitter, DiffuseMaterial, GenericMaterial, GlossyEmitter, TextureRgb, imageio, sceneio
 
 SCENES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "scenes")
 
@@ -96,7 +96,7 @@ def test_obj_triangulation_and_uv_flip(tmp_path):
         "v 0 0 0\nv 1 0 0\nv 1 1 0\nv 0 1 0\nvt 0 0\nvt 1 0\nvt 1 1\nvt 0 1\n"
         "usemtl A\nf 1/1 2/2 3/3 4/4\no second\nusemtl B\nf -4/-4 -3/-3 -2/-2\n")
     named = {"A": GenericMaterial(GenericMaterial.Parameters()), "B": DiffuseMaterial(DiffuseMaterial.Parameters())}
-    meshes, emitters = sceneio.load_obj(str(tmp_path / "q.obj"), named, {"B": (1.0, 2.0, 3.0)})
+    meshes, emitters = sceneio.load_obj(str(tmp_path / "q.obj"), named, {"B": ((1.0, 2.0, 3.0), False, 50.0)})
     assert len(meshes) == 2 and len(emitters) == 1
     a, b = meshes
     assert a.Material is named["A"] and b.Material is named["B"]
@@ -177,3 +177,32 @@ def test_reference_exr_loads():
     sky = imageio.read_exr(os.path.join(SCENES, "sunsky.exr"))
     assert sky.shape == (900, 1600, 3) and np.all(np.isfinite(sky))
     assert sky.min() >= 0 and sky.max() > 100  # the sun
+
+
+def test_glossy_emissive_material(tmp_path):
+    """emissionIsGlossy / emissionExponent (JsonScene.cs:93-104, exponent default
+    50) make GlossyEmitters; plain emission makes DiffuseEmitters."""
+    scene = {
+        "transforms": [{"name": "cam", "position": [0, 0, 3]}],
+        "cameras": [{"name": "c", "type": "perspective", "fov": 40, "transform": "cam"}],
+        "materials": [
+            {"name": "g", "baseColor": {"type": "rgb", "value": [0, 0, 0]},
+             "emission": {"type": "rgb", "value": [5, 5, 5]}, "emissionIsGlossy": True, "emissionExponent": 12},
+            {"name": "h", "baseColor": {"type": "rgb", "value": [0, 0, 0]},
+             "emission": {"type": "rgb", "value": [1, 2, 3]}, "emissionIsGlossy": True},
+            {"name": "d", "baseColor": {"type": "rgb", "value": [0, 0, 0]},
+             "emission": {"type": "rgb", "value": [7, 7, 7]}},
+        ],
+        "objects": [
+            {"name": n, "material": n, "type": "trimesh", "indices": [0, 1, 2],
+             "vertices": [0, 0, z, 1, 0, z, 0, 1, z]} for n, z in (("g", 0), ("h", -1), ("d", -2))
+        ],
+    }
+    import json
+
+    (tmp_path / "s.json").write_text(json.dumps(scene))
+    s = sceneio.load_scene(str(tmp_path / "s.json"), 8, 8)
+    g, h, d = s.Emitters
+    assert isinstance(g, GlossyEmitter) and g.Exponent == 12.0 and g.Radiance == (5.0, 5.0, 5.0)
+    assert isinstance(h, GlossyEmitter) and h.Exponent == 50.0
+    assert type(d) is DiffuseEmitter and d.Radiance == (7.0, 7.0, 7.0)
