@@ -310,6 +310,42 @@ sspt_status sspt_primary_hits(sspt_ctx* ctx, sspt_scene* scene, uint32_t base_se
                               uint32_t iteration, float* host_rays_out,
                               sspt_hit* host_hits_out);
 
+/* ---- path replay: Integrator.ReplayPixel (SeeSharp/Integrators/Integrator.cs:28-29),
+ *      PathTracerBase.ReplayPixel (SeeSharp/Integrators/PathTracer.cs:61-69). ----
+ * Re-traces the one path of pixel (col, row) in `iteration` and returns its
+ * PathGraph (SeeSharp/Integrators/Util/PathGraph.cs) as a node array in the
+ * order RenderPixel / EstimateIncidentRadiance add the nodes (PathTracer.cs:303,
+ * 326, 346-348, 447, 496), each node pointing at its ancestor. */
+typedef enum sspt_node_kind {
+    SSPT_NODE_CAMERA = 0,       /* the root: the primary ray's origin */
+    SSPT_NODE_BSDF_SAMPLE = 1,  /* BSDFSampleNode: a surface hit of the path */
+    SSPT_NODE_NEXT_EVENT = 2,   /* NextEventNode: an unoccluded next-event sample with a non-black contribution */
+    SSPT_NODE_BACKGROUND = 3    /* BackgroundNode: the path left the scene at depth >= MinDepth */
+} sspt_node_kind;
+#define SSPT_NODE_HIT_EMITTER 1u    /* flags: a BSDF sample that hit an emitter (emission, mis_weight set) */
+#define SSPT_NODE_TO_BACKGROUND 2u  /* flags: a next-event sample of the background */
+typedef struct sspt_path_node {
+    int32_t kind;                /* sspt_node_kind */
+    int32_t ancestor;            /* index of the ancestor node, -1 for the root */
+    float position[3];           /* directions (background, background next events): ancestor position + direction */
+    float emission[3];           /* hit emitter / background radiance; next event: Emission */
+    float mis_weight;
+    float scatter_weight[3];     /* BSDF sample: PreviousScatterWeight (sample weight / survival probability) */
+    float survival_probability;  /* BSDF sample: PreviousSurvivalProbability */
+    float pdf;                   /* next event: Pdf */
+    float bsdf_cos[3];           /* next event: BsdfTimesCosine */
+    float prefix_weight[3];      /* next event: PrefixWeight */
+    uint32_t flags;
+} sspt_path_node;
+/* Scene / view `index` of the world (0 for a single scene). Writes at most
+ * `capacity` nodes; *num_nodes receives the path's node count (which may
+ * exceed capacity: 2 + max_depth * (1 + 2 * num_shadow_rays) always suffices)
+ * and estimate_rgb (may be NULL) the path's radiance estimate, the value the
+ * render splats for this sample. */
+sspt_status sspt_replay_pixel(sspt_ctx* ctx, sspt_scene* scene, uint32_t index, const sspt_pt_settings* settings,
+                              uint32_t col, uint32_t row, uint32_t iteration, sspt_path_node* host_nodes,
+                              uint32_t capacity, uint32_t* num_nodes, float* estimate_rgb);
+
 /* ---- measurement ------------------------------------------------------- */
 /* Enables HIP-event timing of every kernel launch on the context stream. */
 sspt_status sspt_set_kernel_timing(sspt_ctx* ctx, int32_t enable);

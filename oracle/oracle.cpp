@@ -1240,6 +1240,27 @@ struct PathCtx {
     Counters* c;
 };
 
+// PathGraph (PathGraph.cs) of one replayed path: nodes in the order
+// RenderPixel / EstimateIncidentRadiance add them, each with its ancestor.
+struct Graph {
+    std::vector<sspt_path_node> nodes;
+    int add(int kind, int ancestor, V3 pos) {
+        sspt_path_node n{};
+        n.kind = kind;
+        n.ancestor = ancestor;
+        n.position[0] = pos.x;
+        n.position[1] = pos.y;
+        n.position[2] = pos.z;
+        nodes.push_back(n);
+        return (int)nodes.size() - 1;
+    }
+    static void put(float* dst, V3 v) {
+        dst[0] = v.x;
+        dst[1] = v.y;
+        dst[2] = v.z;
+    }
+};
+
 // The estimators below run K texture sets along one path: every sample, pdf,
 // MIS weight and Russian-roulette decision comes from set 0 (the scene's own
 // texels); sets 1.. only change the BSDF values the estimate multiplies in.
@@ -1247,7 +1268,8 @@ struct PathCtx {
 
 // PathTracerBase.PerformBackgroundNextEvent (PathTracer.cs:423-452)
 template <int K>
-void background_nee(const PathCtx& pc, const Shader* sh, Rng& rng, Rgb* out) {
+void background_nee(const PathCtx& pc, const Shader* sh, Rng& rng, Rgb* out, Graph* graph = nullptr, int vertex = -1,
+                    Rgb prefix = {}) {
     const or_scene& s = *pc.s;
     for (int k = 0; k < K; ++k) out[k] = rgb(0.0f);
     if (!s.has_env) return;
@@ -1266,12 +1288,24 @@ void background_nee(const PathCtx& pc, const Shader* sh, Rng& rng, Rgb* out) {
         Rgb bsdf_cos = eval_cos(sh[k], smp.dir);
         Rgb contrib = smp.weight * bsdf_cos / nsr;
         out[k] = mis * contrib;
+        if (graph && k == 0 && !is_black(contrib)) {  // PathTracer.cs:446-447
+            const float* p = graph->nodes[vertex].position;
+            const V3 at = v3(p[0], p[1], p[2]) + smp.dir;  // NextEventNode(direction, ancestor, ...)
+            sspt_path_node& n = graph->nodes[graph->add(SSPT_NODE_NEXT_EVENT, vertex, at)];
+            Graph::put(n.emission, smp.weight * smp.pdf);
+            n.pdf = smp.pdf;
+            Graph::put(n.bsdf_cos, bsdf_cos);
+            n.mis_weight = mis;
+            Graph::put(n.prefix_weight, prefix);
+            n.flags = SSPT_NODE_TO_BACKGROUND;
+        }
     }
 }
 
 // PathTracerBase.PerformNextEventEstimation (PathTracer.cs:454-501)
 template <int K>
-void light_nee(const PathCtx& pc, const Shader* sh, Rng& rng, Rgb* out) {
+void light_nee(const PathCtx& pc, const Shader* sh, Rng& rng, Rgb* out, Graph* graph = nullptr, int vertex = -1,
+               Rgb prefix = {}) {
     const or_scene& s = *pc.s;
     for (int k = 0; k < K; ++k) out[k] = rgb(0.0f);
     if (s.emitters.empty()) return;
@@ -1304,13 +1338,21 @@ void light_nee(const PathCtx& pc, const Shader* sh, Rng& rng, Rgb* out) {
         Rgb bsdf_cos = eval_cos(sh[k], -l2s);
         Rgb contrib = em / pdf * bsdf_cos;
         out[k] = mis * contrib;
+        if (graph && k == 0 && !is_black(contrib)) {  // PathTracer.cs:495-496
+            sspt_path_node& n = graph->nodes[graph->add(SSPT_NODE_NEXT_EVENT, vertex, lp.position)];
+            Graph::put(n.emission, em);
+            n.pdf = pdf;
+            Graph::put(n.bsdf_cos, bsdf_cos);
+            n.mis_weight = mis;
+            Graph::put(n.prefix_weight, prefix);
+        }
     }
 }
 
 // PathTracerBase.RenderPixel + EstimateIncidentRadiance (PathTracer.cs:287-382)
 template <int K>
 void render_pixel_k(const PathCtx& pc, const TextureSet* const* sets, uint32_t row, uint32_t col,
-                    uint32_t iteration, Rgb* radiance) {
+                    uint32_t iteration, Rgb* radiance, Graph* graph = nullptr) {
     const or_scene& s = *pc.s;
     const sspt_pt_settings& cfg = *pc.cfg;
     Rng rng{hash_seed(cfg.base_seed, row * (uint32_t)s.width + col, iteration)};
@@ -1331,6 +1373,10 @@ void render_pixel_k(const PathCtx& pc, const TextureSet* const* sets, uint32_t r
     float prev_pdf = 0.0f;
     int count = (int)s.emitters.size();
     float nsr = (float)cfg.num_shadow_rays;
+    // ReplayPixel's graph (PathTracer.cs:303): the root, then the current vertex
+    int vertex = graph ? graph->add(SSPT_NODE_CAMERA, -1, ray.o) : -1;
+    Rgb prev_scatter = rgb(1.0f);  // PathState.PreviousScatterWeight / PreviousSurvivalProbability
+    float prev_survival = 1.0f;
 
     while (depth <= cfg.max_depth) {
         HitRec h = trace_closest(s, ray, pc.brute);
@@ -1349,6 +1395,12 @@ void render_pixel_k(const PathCtx& pc, const TextureSet* const* sets, uint32_t r
                     contrib = env_radiance(s.env, ray.d);
                 }
                 for (int k = 0; k < K; ++k) radiance[k] = radiance[k] + prefix[k] * mis * contrib;
+                if (graph) {  // PathTracer.cs:326
+                    const float* p = graph->nodes[vertex].position;
+                    sspt_path_node& n = graph->nodes[graph->add(SSPT_NODE_BACKGROUND, vertex, v3(p[0], p[1], p[2]) + ray.d)];
+                    Graph::put(n.emission, contrib);
+                    n.mis_weight = mis;
+                }
             }
             break;
         }
@@ -1375,6 +1427,19 @@ void render_pixel_k(const PathCtx& pc, const TextureSet* const* sets, uint32_t r
             }
             Rgb em = emitted(e, hit, -ray.d);
             for (int k = 0; k < K; ++k) radiance[k] = radiance[k] + prefix[k] * mis * em;
+            if (graph) {  // PathTracer.cs:346
+                vertex = graph->add(SSPT_NODE_BSDF_SAMPLE, vertex, hit.position);
+                sspt_path_node& n = graph->nodes[vertex];
+                Graph::put(n.scatter_weight, prev_scatter);
+                n.survival_probability = prev_survival;
+                Graph::put(n.emission, em);
+                n.mis_weight = mis;
+                n.flags = SSPT_NODE_HIT_EMITTER;
+            }
+        } else if (graph) {  // PathTracer.cs:348
+            vertex = graph->add(SSPT_NODE_BSDF_SAMPLE, vertex, hit.position);
+            Graph::put(graph->nodes[vertex].scatter_weight, prev_scatter);
+            graph->nodes[vertex].survival_probability = prev_survival;
         }
 
         // ComputeSurvivalProbability (PathTracer.cs:277-282)
@@ -1385,9 +1450,9 @@ void render_pixel_k(const PathCtx& pc, const TextureSet* const* sets, uint32_t r
             Rgb nee[K], part[K];
             for (int k = 0; k < K; ++k) nee[k] = rgb(0.0f);
             for (int i = 0; i < cfg.num_shadow_rays; ++i) {
-                background_nee<K>(pc, sh, rng, part);
+                background_nee<K>(pc, sh, rng, part, graph, vertex, prefix[0]);
                 for (int k = 0; k < K; ++k) nee[k] = nee[k] + part[k];
-                light_nee<K>(pc, sh, rng, part);
+                light_nee<K>(pc, sh, rng, part, graph, vertex, prefix[0]);
                 for (int k = 0; k < K; ++k) nee[k] = nee[k] + part[k];
             }
             for (int k = 0; k < K; ++k) radiance[k] = radiance[k] + prefix[k] * nee[k] / q;
@@ -1406,6 +1471,8 @@ void render_pixel_k(const PathCtx& pc, const TextureSet* const* sets, uint32_t r
         depth++;
         prev = hit;
         prev_pdf = bs.pdf * q;
+        prev_scatter = bs.weight / q;
+        prev_survival = q;
     }
 }
 
@@ -1911,6 +1978,23 @@ int32_t or_set_texture(or_scene* s, uint32_t index, const float* texels) {
     Texture& t = s->textures[index];
     std::copy(texels, texels + t.texels.size(), t.texels.begin());
     return 0;
+}
+
+// PathTracerBase.ReplayPixel (PathTracer.cs:61-69): the PathGraph and the
+// estimate of one path. Writes at most `capacity` nodes; returns the count.
+uint32_t or_replay_pixel(const or_scene* s, const sspt_pt_settings* cfg, uint32_t col, uint32_t row,
+                         uint32_t iteration, sspt_path_node* nodes, uint32_t capacity, float* estimate) {
+    Counters c;
+    PathCtx pc{s, cfg, false, &c};
+    const TextureSet* sets[1] = {&s->textures};
+    Graph g;
+    Rgb L;
+    render_pixel_k<1>(pc, sets, row, col, iteration, &L, &g);
+    for (uint32_t i = 0; i < std::min<uint32_t>(capacity, (uint32_t)g.nodes.size()); ++i) nodes[i] = g.nodes[i];
+    estimate[0] = L.x;
+    estimate[1] = L.y;
+    estimate[2] = L.z;
+    return (uint32_t)g.nodes.size();
 }
 
 uint64_t or_render_tangent(const or_scene* s, const sspt_pt_settings* cfg, uint32_t first, uint32_t num,

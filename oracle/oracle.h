@@ -97,6 +97,11 @@ void or_render_pixel(const or_scene* s, const sspt_pt_settings* settings, uint32
 /* DenoiseBuffers layers: running means of the primary hit's GetScatterStrength
  * (albedo) and shading normal over iterations first..first+num-1, W*H*3 each,
  * updated in place (start from zeros for a fresh framebuffer). */
+/* PathTracerBase.ReplayPixel: the path graph (sspt_path_node array, at most
+ * `capacity` written) and estimate of pixel (col, row) in `iteration`;
+ * returns the node count. */
+uint32_t or_replay_pixel(const or_scene* s, const sspt_pt_settings* settings, uint32_t col, uint32_t row,
+                         uint32_t iteration, sspt_path_node* nodes, uint32_t capacity, float* estimate_rgb);
 void or_primary_aov(const or_scene* s, const sspt_pt_settings* settings, uint32_t first_iteration,
                     uint32_t num_iterations, float* albedo, float* normal, int32_t num_threads);
 

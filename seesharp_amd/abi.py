@@ -136,6 +136,22 @@ class Hit(C.Structure):
     _fields_ = [("t", C.c_float), ("u", C.c_float), ("v", C.c_float), ("prim", C.c_int32)]
 
 
+NODE_CAMERA = 0
+NODE_BSDF_SAMPLE = 1
+NODE_NEXT_EVENT = 2
+NODE_BACKGROUND = 3
+NODE_HIT_EMITTER = 1
+NODE_TO_BACKGROUND = 2
+
+
+class PathNode(C.Structure):
+    """sspt_path_node: one node of a replayed path's PathGraph."""
+    _fields_ = [("kind", C.c_int32), ("ancestor", C.c_int32), ("position", C.c_float * 3),
+                ("emission", C.c_float * 3), ("mis_weight", C.c_float), ("scatter_weight", C.c_float * 3),
+                ("survival_probability", C.c_float), ("pdf", C.c_float), ("bsdf_cos", C.c_float * 3),
+                ("prefix_weight", C.c_float * 3), ("flags", C.c_uint32)]
+
+
 # Every symbol include/sspt.h declares, with (restype, argtypes).
 _vp = C.c_void_p
 SYMBOLS = {
@@ -177,6 +193,8 @@ SYMBOLS = {
     "sspt_get_kernel_timing": (C.c_int, [_vp, C.c_char_p, C.POINTER(C.c_uint64),
                                          C.POINTER(C.c_double)]),
     "sspt_reset_kernel_timing": (C.c_int, [_vp]),
+    "sspt_replay_pixel": (C.c_int, [_vp, _vp, C.c_uint32, C.POINTER(PtSettings), C.c_uint32, C.c_uint32,
+                                    C.c_uint32, C.POINTER(PathNode), C.c_uint32, C.POINTER(C.c_uint32), _fp]),
 }
 
 _lib = None

@@ -1259,6 +1259,50 @@ sspt_status sspt_primary_hits(sspt_ctx* ctx, sspt_scene* sc, uint32_t base_seed,
     return SSPT_OK;
 }
 
+static_assert(sizeof(PathNode) == sizeof(sspt_path_node), "PathNode mirrors sspt_path_node");
+
+sspt_status sspt_replay_pixel(sspt_ctx* ctx, sspt_scene* sc, uint32_t index, const sspt_pt_settings* cfg,
+                              uint32_t col, uint32_t row, uint32_t iteration, sspt_path_node* host_nodes,
+                              uint32_t capacity, uint32_t* num_nodes, float* estimate_rgb) {
+    if (!ctx || !sc || !cfg || !num_nodes) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "null argument");
+    if (capacity > 0 && !host_nodes) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "null node array");
+    if (!valid_settings(*cfg)) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "invalid settings");
+    if (index >= sc->recs.size()) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "scene index %u out of range", index);
+    SceneRec& R = sc->recs[index];
+    if (col >= (uint32_t)R.cam.width || row >= (uint32_t)R.cam.height)
+        return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "pixel (%u, %u) outside the %dx%d framebuffer", col, row,
+                    R.cam.width, R.cam.height);
+    // the most nodes a path can add: root, per vertex a BSDF sample and two next events per shadow ray, background
+    const uint64_t need = 2 + (uint64_t)cfg->max_depth * (1 + 2 * (uint64_t)cfg->num_shadow_rays);
+    const uint32_t cap = (uint32_t)std::min<uint64_t>(capacity, need);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    apply_settings(R, *cfg);
+    char* buf = nullptr;
+    const size_t bytes = (size_t)std::max(1u, cap) * sizeof(PathNode) + 64;
+    HIP_TRY(ctx, hipMalloc((void**)&buf, bytes));
+    PathNode* d_nodes = reinterpret_cast<PathNode*>(buf + 64);
+    uint32_t* d_count = reinterpret_cast<uint32_t*>(buf);
+    float* d_est = reinterpret_cast<float*>(buf + 16);
+    uint32_t count = 0;
+    float est[3] = {0.0f, 0.0f, 0.0f};
+    hipError_t e = hipMemcpyAsync(sc->d_recs + index, &R, sizeof(SceneRec), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) {
+        launch_replay(ctx->stream, sc->dev, (int)index, row * (uint32_t)R.cam.width + col, iteration, d_nodes, cap,
+                      d_count, d_est);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&count, d_count, sizeof(count), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(est, d_est, sizeof(est), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess && cap > 0)
+        e = hipMemcpy(host_nodes, d_nodes, std::min(count, cap) * sizeof(PathNode), hipMemcpyDeviceToHost);
+    (void)hipFree(buf);
+    if (e != hipSuccess) return fail(ctx, SSPT_ERR_DEVICE, "replay failed: %s", hipGetErrorString(e));
+    *num_nodes = count;
+    if (estimate_rgb) std::memcpy(estimate_rgb, est, sizeof(est));
+    return SSPT_OK;
+}
+
 sspt_status sspt_set_wave_capacity(sspt_ctx* ctx, uint32_t slots) {
     if (!ctx) return SSPT_ERR_INVALID_ARGUMENT;
     if (slots > kMaxSlots) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "wave capacity above %u", kMaxSlots);

@@ -87,4 +87,17 @@ void launch_trace_occluded(hipStream_t s, const DevScene& sc, int scene, const f
 void launch_primary_rays(hipStream_t s, const DevScene& sc, int scene, uint32_t num_pixels, uint32_t base_seed,
                          uint32_t iteration, float* rays);
 
+// One node of a replayed path (the layout of sspt_path_node).
+struct PathNode {
+    int kind, ancestor;
+    float position[3], emission[3], mis_weight, scatter_weight[3], survival_probability, pdf, bsdf_cos[3],
+        prefix_weight[3];
+    uint32_t flags;
+};
+// ReplayPixel of pixel `pix`, `iteration` of scene record `scene` (settings
+// already in the record): writes up to `cap` nodes, the node count and the
+// path's RGB estimate to device memory.
+void launch_replay(hipStream_t s, const DevScene& sc, int scene, uint32_t pix, uint32_t iteration, PathNode* nodes,
+                   uint32_t cap, uint32_t* count, float* estimate);
+
 }  // namespace sspt

@@ -526,6 +526,25 @@ __device__ __forceinline__ f3 emitted(const DevEmitter& e, f3 ns, f3 dir) {
     return dot(ns, dir) < 0.0f ? gray(0.0f) : mk3(e.r, e.g, e.b);
 }
 
+// What ReplayPixel (PathTracer.cs:61-69) records of one vertex: the values the
+// reference's PathGraph nodes carry (PathGraph.cs BSDFSampleNode,
+// NextEventNode, BackgroundNode). Filled only by the replay kernel; every other
+// user leaves PathVertex::log null and the recording compiles away.
+struct ReplayLog {
+    bool hit_light = false, hit_background = false;  // emitter hit / background hit at depth >= MinDepth
+    f3 hit_emission;
+    float hit_mis = 0.0f;
+    bool env_ok = false, env_nonzero = false;  // pdfs non-zero / contribution non-black
+    f3 env_dir, env_emission, env_bc;
+    float env_pdf = 0.0f, env_mis = 0.0f;
+    float4 env_o, env_d;                       // the environment shadow ray
+    bool area_ok = false, area_nonzero = false;
+    f3 area_pos, area_emission, area_bc;
+    float area_pdf = 0.0f, area_mis = 0.0f;
+    float4 area_o, area_d;                     // the area-light shadow ray
+    f3 scatter;                                // BSDF sample weight / survival probability
+};
+
 // One vertex of EstimateIncidentRadiance (PathTracer.cs:315-382) for one path:
 // the hit of its extension ray (background / emitter with MIS, Russian
 // roulette), its next-event rays (environment, then area light, per shadow
@@ -555,6 +574,7 @@ struct PathVertex {
     f3 delta = {0.0f, 0.0f, 0.0f};  // ADJ: dLoss/dpixel / iterations
     int2 vtex = {-1, -1};           // ADJ: texel offsets of the vertex's textured parameters
     f3 next_dir = {0.0f, 0.0f, 0.0f};
+    ReplayLog* log = nullptr;  // ReplayPixel only
 
     __device__ PathVertex(const DevScene& S_, const SceneRec& R_, uint32_t slot_, uint32_t sc_)
         : S(S_), R(R_), E(env_of(S_, R_)), slot(slot_), sc(sc_) {}
@@ -582,6 +602,11 @@ struct PathVertex {
                     contrib = env_radiance(E, dir);
                 }
                 L = L + prefix * mis * contrib;
+                if (log) {
+                    log->hit_background = true;
+                    log->hit_emission = contrib;
+                    log->hit_mis = mis;
+                }
             }
             return;
         }
@@ -602,6 +627,11 @@ struct PathVertex {
             }
             f3 em = emitted(e, s.ns, -dir);
             L = L + prefix * mis * em;
+            if (log) {
+                log->hit_light = true;
+                log->hit_emission = em;
+                log->hit_mis = mis;
+            }
         }
         q = depth > 4 ? clampf(average(prefix), 0.05f, 0.95f) : 1.0f;
         bool stop = rng.next_float() > q || depth == R.max_depth;
@@ -622,6 +652,17 @@ struct PathVertex {
         float mis = R.enable_bsdf_di ? 1.0f / (1.0f + pdf_bsdf / (es.pdf * nsr)) : 1.0f;
         f3 contrib = es.weight * bc / nsr;
         f3 val = prefix * (mis * contrib) / q;
+        if (log) {
+            log->env_ok = true;
+            log->env_nonzero = !is_zero(contrib);
+            log->env_dir = es.dir;
+            log->env_emission = es.weight * es.pdf;
+            log->env_pdf = es.pdf;
+            log->env_bc = bc;
+            log->env_mis = mis;
+            log->env_o = f4_of(offset_point(s, es.dir), s.err);
+            log->env_d = f4_of(es.dir, INFINITY);
+        }
         if (is_zero(val)) return false;
         *o = f4_of(offset_point(s, es.dir), s.err);
         *d = f4_of(es.dir, INFINITY);
@@ -654,11 +695,25 @@ struct PathVertex {
             float pdf = e.inv_area / jac * sel * nsr;
             f3 contrib = em / pdf * bc;
             val = prefix * (mis * contrib) / q;
+            if (log) {
+                log->area_ok = true;
+                log->area_nonzero = !is_zero(contrib);
+                log->area_pos = lp.pos;
+                log->area_emission = em;
+                log->area_pdf = pdf;
+                log->area_bc = bc;
+                log->area_mis = mis;
+            }
         }
-        if (is_zero(val)) return false;
+        if (!log && is_zero(val)) return false;
         f3 dd = lp.pos - s.pos;
         f3 p0 = offset_point(s, dd);
         f3 p1 = offset_point(lp, -dd);
+        if (log) {
+            log->area_o = f4_of(p0, 1e-5f);
+            log->area_d = f4_of(p1 - p0, 1.0f - 1e-5f);
+            if (is_zero(val)) return false;
+        }
         if (ADJ && (vtex.x >= 0 || vtex.y >= 0))
             *g = contribution_gradient(S, sh, s.uv, vtex, sh.f.to_local(-l2s), val, delta);
         *o = f4_of(p0, 1e-5f);
@@ -677,6 +732,7 @@ struct PathVertex {
         *no = f4_of(offset_point(s, bs.dir), s.err);
         *nd = f4_of(bs.dir, INFINITY);
         next_dir = bs.dir;
+        if (log) log->scatter = bs.weight / q;
         prefix = prefix * (bs.weight / q);
         prev_pdf = bs.pdf * q;
         prev_pos = s.pos;
@@ -864,6 +920,132 @@ __global__ void __launch_bounds__(kTraceBlock) k_path(DevScene S, Wave w) {
             w.st_nrm[slot] = f4_of(nrm, 0.0f);
         }
     }
+}
+
+// ReplayPixel (PathTracer.cs:61-69): thread 0 re-traces one path exactly as
+// k_path traces it (same vertex object, same RNG draws, same shadow rays) and
+// writes the PathGraph nodes RenderPixel / EstimateIncidentRadiance add
+// (PathTracer.cs:303, 326, 346-348, 447, 496). The block has kTraceBlock
+// threads only because the traversal stack's LDS layout is strided by it.
+__device__ __forceinline__ PathNode* replay_node(PathNode* nodes, uint32_t cap, uint32_t* n, PathNode* spare, int kind,
+                                                 int ancestor, f3 pos) {
+    PathNode* x = *n < cap ? nodes + *n : spare;
+    *n += 1;
+    *x = PathNode{};
+    x->kind = kind;
+    x->ancestor = ancestor;
+    x->position[0] = pos.x;
+    x->position[1] = pos.y;
+    x->position[2] = pos.z;
+    return x;
+}
+__device__ __forceinline__ void put3(float* dst, f3 v) {
+    dst[0] = v.x;
+    dst[1] = v.y;
+    dst[2] = v.z;
+}
+
+__global__ void __launch_bounds__(kTraceBlock) k_replay(DevScene S, int scene, uint32_t pix, uint32_t iteration,
+                                                        PathNode* nodes, uint32_t cap, uint32_t* count,
+                                                        float* estimate) {
+    __shared__ int stack_node[kStackLds * kTraceBlock];
+    __shared__ float stack_t[kStackLds * kTraceBlock];
+    if (threadIdx.x != 0) return;
+    const SceneRec& R = S.recs[scene];
+    const uint32_t row = pix / (uint32_t)R.cam.width, col = pix - row * (uint32_t)R.cam.width;
+    PathVertex<false> v(S, R, 0, (uint32_t)scene);
+    ReplayLog log;
+    v.log = &log;
+    v.rng = Rng{hash_seed(R.base_seed, pix, iteration)};
+    f2 off = v.rng.next_float2();
+    f3 o, d;
+    camera_ray(R.cam, (float)col + off.x, (float)row + off.y, &o, &d);
+    v.prefix = gray(1.0f);
+    v.depth = 1;
+    uint32_t n = 0;
+    PathNode spare;
+    replay_node(nodes, cap, &n, &spare, 0, -1, o);
+    int cur = 0;
+    f3 cur_pos = o, scatter = gray(1.0f);
+    float survival = 1.0f;
+    f3 Le = gray(0.0f), La = gray(0.0f);
+    float4 o4 = f4_of(o, 0.0f), d4 = f4_of(d, INFINITY);
+    int* sn = stack_node + threadIdx.x;
+    float* st = stack_t + threadIdx.x;
+    for (;;) {
+        HitOut h = traverse<false>(S, R.root, R.abs_max, f3_of(o4), f3_of(d4), o4.w, d4.w, sn, st);
+        v.dir = f3_of(d4);
+        log = ReplayLog{};
+        v.on_hit(h.li >= 0 ? make_float4(h.t, h.u, h.v, __int_as_float(h.li))
+                           : make_float4(INFINITY, 0.0f, 0.0f, __int_as_float(-1)));
+        if (!v.surface_hit) {
+            if (log.hit_background) {
+                PathNode* x = replay_node(nodes, cap, &n, &spare, 3, cur, cur_pos + v.dir);
+                put3(x->emission, log.hit_emission);
+                x->mis_weight = log.hit_mis;
+            }
+            break;
+        }
+        PathNode* x = replay_node(nodes, cap, &n, &spare, 1, cur, v.s.pos);
+        put3(x->scatter_weight, scatter);
+        x->survival_probability = survival;
+        if (log.hit_light) {
+            put3(x->emission, log.hit_emission);
+            x->mis_weight = log.hit_mis;
+            x->flags = 1u;
+        }
+        cur = (int)n - 1;
+        cur_pos = v.s.pos;
+        for (int k = 0; v.do_nee && k < R.num_shadow_rays; ++k) {
+            float4 so, sd, c, g;
+            if (R.has_env) {
+                log.env_ok = false;
+                bool traced = v.env_next_event(&so, &sd, &c, &g);
+                if (log.env_ok && log.env_nonzero) {
+                    bool occ = traverse<true>(S, R.root, R.abs_max, f3_of(log.env_o), f3_of(log.env_d), log.env_o.w,
+                                              log.env_d.w, sn, st)
+                                   .li >= 0;
+                    if (!occ) {
+                        PathNode* y = replay_node(nodes, cap, &n, &spare, 2, cur, cur_pos + log.env_dir);
+                        put3(y->emission, log.env_emission);
+                        y->pdf = log.env_pdf;
+                        put3(y->bsdf_cos, log.env_bc);
+                        y->mis_weight = log.env_mis;
+                        put3(y->prefix_weight, v.prefix);
+                        y->flags = 2u;
+                        if (traced) Le = Le + f3_of(c);
+                    }
+                }
+            }
+            if (R.num_emitters > 0) {
+                log.area_ok = false;
+                bool traced = v.area_next_event(&so, &sd, &c, &g);
+                if (log.area_ok && log.area_nonzero) {
+                    bool occ = traverse<true>(S, R.root, R.abs_max, f3_of(log.area_o), f3_of(log.area_d),
+                                              log.area_o.w, log.area_d.w, sn, st)
+                                   .li >= 0;
+                    if (!occ) {
+                        PathNode* y = replay_node(nodes, cap, &n, &spare, 2, cur, log.area_pos);
+                        put3(y->emission, log.area_emission);
+                        y->pdf = log.area_pdf;
+                        put3(y->bsdf_cos, log.area_bc);
+                        y->mis_weight = log.area_mis;
+                        put3(y->prefix_weight, v.prefix);
+                        if (traced) La = La + f3_of(c);
+                    }
+                }
+            }
+        }
+        if (!v.cont || !v.sample_continuation(&o4, &d4)) break;
+        scatter = log.scatter;
+        survival = v.q;
+    }
+    *count = n;
+    // the order k_accumulate_* sums a path's parts in
+    f3 total = v.L + Le + La;
+    estimate[0] = total.x;
+    estimate[1] = total.y;
+    estimate[2] = total.z;
 }
 
 __global__ void __launch_bounds__(256) k_adjoint_finalize(DevScene S, Wave w) {
@@ -1104,6 +1286,12 @@ void launch_primary_rays(hipStream_t s, const DevScene& sc, int scene, uint32_t 
                          uint32_t iteration, float* rays) {
     hipLaunchKernelGGL(k_primary_rays, dim3(blocks(num_pixels, 256)), dim3(256), 0, s, sc, scene, num_pixels,
                        base_seed, iteration, rays);
+}
+
+void launch_replay(hipStream_t s, const DevScene& sc, int scene, uint32_t pix, uint32_t iteration, PathNode* nodes,
+                   uint32_t cap, uint32_t* count, float* estimate) {
+    hipLaunchKernelGGL(k_replay, dim3(1), dim3(kTraceBlock), 0, s, sc, scene, pix, iteration, nodes, cap, count,
+                       estimate);
 }
 
 }  // namespace sspt

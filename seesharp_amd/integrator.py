@@ -208,6 +208,11 @@ class GpuScene(_TextureParameters):
             self.device.ptr, self.ptr, abi.fptr(rays), rays.shape[0], out.ctypes.data_as(C.POINTER(C.c_uint8))))
         return out
 
+    def replay_pixel(self, settings: abi.PtSettings, col: int, row: int, iteration: int):
+        """ReplayPixel (PathTracer.cs:61-69): the sspt_path_node records and the
+        RGB estimate of the path of pixel (col, row) in `iteration`."""
+        return _replay(self.device, self.ptr, 0, settings, col, row, iteration)
+
     def primary_hits(self, base_seed: int, iteration: int):
         n = self.width * self.height
         rays = np.zeros((n, 8), np.float32)
@@ -296,6 +301,10 @@ class GpuBatch(_TextureParameters):
         h, w = self.sizes[index]
         return _read_layer(self.device, self.ptr, index, name, h, w)
 
+    def replay_pixel(self, index: int, settings: abi.PtSettings, col: int, row: int, iteration: int):
+        """ReplayPixel of scene / view `index` of the world."""
+        return _replay(self.device, self.ptr, index, settings, col, row, iteration)
+
     def read_image(self, index: int) -> np.ndarray:
         h, w = self.sizes[index]
         img = np.zeros((h, w, 3), np.float32)
@@ -313,6 +322,19 @@ def _read_layer(device, ptr, index, name, h, w):
     img = np.zeros((h, w, 3), np.float32)
     device.check(device.lib.sspt_framebuffer_read_layer(device.ptr, ptr, index, _LAYERS[name], abi.fptr(img)))
     return img
+
+
+def _replay(device, ptr, index, settings, col, row, iteration):
+    """sspt_replay_pixel: (node array, RGB estimate) of one path."""
+    cap = 2 + settings.max_depth * (1 + 2 * max(0, settings.num_shadow_rays))
+    nodes = (abi.PathNode * cap)()
+    count = C.c_uint32()
+    est = np.zeros(3, np.float32)
+    device.check(device.lib.sspt_replay_pixel(device.ptr, ptr, index, C.byref(settings), col, row, iteration, nodes,
+                                              cap, C.byref(count), abi.fptr(est)))
+    if count.value > cap:
+        raise RuntimeError(f"replayed path has {count.value} nodes, more than the bound {cap}")
+    return list(nodes[:count.value]), est
 
 
 def _hits(hits):
@@ -389,3 +411,14 @@ class GpuPathTracer(Integrator):
             "NumRays": stats["num_rays"], "NumShadowRays": stats["num_shadow_rays"],
             "NumOccluded": stats["num_occluded"], "NumRayHits": stats["num_ray_hits"]}
         return fb
+
+    def ReplayPixel(self, scene, pixel, iteration: int):
+        """PathTracerBase.ReplayPixel (PathTracer.cs:61-69): re-traces the path of
+        pixel (col, row) in `iteration` and returns (PathGraph, estimate); the
+        framebuffer is not touched."""
+        from .pathgraph import PathGraph
+
+        col, row = pixel
+        gs = self.prepare(scene)
+        nodes, est = gs.replay_pixel(self.settings(), int(col), int(row), int(iteration))
+        return PathGraph.from_nodes(nodes), est

@@ -56,6 +56,8 @@ _SIGS = {
                                C.POINTER(abi.RayStats), C.POINTER(C.c_int32), C.c_int64, C.c_int32]),
     "or_render_pixel": (None, [_vp, C.POINTER(abi.PtSettings), C.c_uint32, C.c_uint32, C.c_uint32, _fp]),
     "or_set_texture": (C.c_int32, [_vp, C.c_uint32, _fp]),
+    "or_replay_pixel": (C.c_uint32, [_vp, C.POINTER(abi.PtSettings), C.c_uint32, C.c_uint32, C.c_uint32,
+                                     C.POINTER(abi.PathNode), C.c_uint32, _fp]),
     "or_primary_aov": (None, [_vp, C.POINTER(abi.PtSettings), C.c_uint32, C.c_uint32, _fp, _fp, C.c_int32]),
     "or_render_tangent": (C.c_uint64, [_vp, C.POINTER(abi.PtSettings), C.c_uint32, C.c_uint32, C.c_uint32, _fp,
                                        C.c_float, _fp, C.POINTER(C.c_int32), C.c_int64, C.c_int32]),
@@ -130,6 +132,20 @@ class OracleScene:
         nrm = np.zeros((self.height, self.width, 3), np.float32)
         lib().or_primary_aov(self.ptr, C.byref(cfg), first, num, fp(alb), fp(nrm), threads)
         return alb, nrm
+
+    def replay_pixel(self, cfg, col, row, iteration):
+        """ReplayPixel: (sspt_path_node list, RGB estimate) of one path."""
+        cap = 2 + cfg.max_depth * (1 + 2 * max(0, cfg.num_shadow_rays))
+        nodes = (abi.PathNode * cap)()
+        est = np.zeros(3, np.float32)
+        n = lib().or_replay_pixel(self.ptr, C.byref(cfg), col, row, iteration, nodes, cap, fp(est))
+        assert n <= cap
+        return list(nodes[:n]), est
+
+    def render_pixel(self, cfg, col, row, iteration):
+        out = np.zeros(3, np.float32)
+        lib().or_render_pixel(self.ptr, C.byref(cfg), row, col, iteration, fp(out))
+        return out
 
     def set_texture(self, index, texels):
         texels = np.ascontiguousarray(texels, np.float32)
