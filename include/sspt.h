@@ -253,6 +253,23 @@ sspt_status sspt_framebuffer_read(sspt_ctx* ctx, sspt_scene* scene, float* host_
 typedef enum sspt_layer { SSPT_LAYER_ALBEDO = 0, SSPT_LAYER_NORMAL = 1 } sspt_layer;
 sspt_status sspt_framebuffer_read_layer(sspt_ctx* ctx, sspt_scene* scene, uint32_t index, int32_t layer,
                                         float* host_rgb);
+/* FrameBuffer.NumOutliersToTrack and its OutlierReplayCache
+ * (SeeSharp/Images/FrameBuffer.cs:29, 186-189;
+ * SeeSharp/Integrators/Util/OutlierReplayCache.cs:9-22): per pixel, the n
+ * samples with the largest average estimate ((R+G+B)/3 of the splatted value)
+ * and the iterations that drew them; among equal weights the earliest
+ * iterations stay. n <= SSPT_MAX_OUTLIERS applies to every scene / view of the
+ * world; 0 (the default) turns tracking off. Tracking starts empty,
+ * sspt_framebuffer_clear and sspt_render_batch empty it, sspt_render adds to
+ * it. Non-finite samples are not tracked. */
+#define SSPT_MAX_OUTLIERS 4
+sspt_status sspt_framebuffer_track_outliers(sspt_ctx* ctx, sspt_scene* scene, uint32_t n);
+/* Scene / view `index`'s tracked samples: W*H*n weights and iterations (either
+ * may be NULL), per pixel in descending weight, the order FrameBuffer.WriteToFile
+ * writes its "outlier-i" layers; weight 0 and iteration -1 where a pixel has
+ * fewer than n samples. */
+sspt_status sspt_framebuffer_read_outliers(sspt_ctx* ctx, sspt_scene* scene, uint32_t index, float* weights,
+                                           int32_t* iterations);
 /* Number of non-finite path estimates skipped so far (FrameBuffer.NaNWarnings count). */
 sspt_status sspt_framebuffer_nan_count(sspt_ctx* ctx, sspt_scene* scene, uint64_t* count);
 

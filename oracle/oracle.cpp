@@ -1922,6 +1922,78 @@ uint64_t or_render(const or_scene* s, const sspt_pt_settings* cfg, uint32_t firs
     return nans;
 }
 
+// System.Collections.Generic.PriorityQueue<TElement, float> of .NET 6+ (the
+// runtime SeeSharp targets; not part of the reference's sources): an array
+// quaternary min-heap. Enqueue appends and moves the entry up while its
+// priority is strictly smaller than its parent's ((i - 1) / 4);
+// EnqueueDequeue returns the new entry at once unless its priority is strictly
+// larger than the root's, otherwise it replaces the root and moves down: into
+// the first strictly smallest of the children 4i+1..4i+4 while that child's
+// priority is strictly smaller. UnorderedItems enumerates the array in order.
+struct QuaternaryHeap {
+    std::vector<std::pair<float, int32_t>> a;  // (priority, element)
+    void enqueue(float p, int32_t e) {
+        size_t i = a.size();
+        a.push_back({p, e});
+        while (i > 0) {
+            size_t parent = (i - 1) >> 2;
+            if (!(p < a[parent].first)) break;
+            a[i] = a[parent];
+            i = parent;
+        }
+        a[i] = {p, e};
+    }
+    void enqueue_dequeue(float p, int32_t e) {
+        if (a.empty() || !(p > a[0].first)) return;
+        size_t i = 0;
+        for (;;) {
+            size_t c = (i << 2) + 1;
+            if (c >= a.size()) break;
+            size_t mc = c;
+            for (size_t k = c + 1; k < std::min(c + 4, a.size()); ++k)
+                if (a[k].first < a[mc].first) mc = k;
+            if (p <= a[mc].first) break;
+            a[i] = a[mc];
+            i = mc;
+        }
+        a[i] = {p, e};
+    }
+};
+
+// FrameBuffer's OutlierReplayCache (OutlierReplayCache.cs:9-22), fed by every
+// finite splat (FrameBuffer.cs:166-189) with Weight = the sample and
+// Iteration = CurIteration - 1, priority Weight.Average: Enqueue while the
+// queue holds fewer than n entries, else EnqueueDequeue. Writes per pixel the
+// queue's entries in WriteToFile's order (UnorderedItems stably sorted by
+// descending priority, FrameBuffer.cs:321-325), weight 0 / iteration -1 past
+// the queue's size.
+void or_render_outliers(const or_scene* s, const sspt_pt_settings* cfg, uint32_t first, uint32_t num, int32_t n,
+                        float* weights, int32_t* iterations, int32_t threads) {
+    const int64_t total = (int64_t)s->width * s->height;
+    parallel_for((total + 63) / 64, threads, [&](int64_t ch) {
+        Counters c;
+        PathCtx pc{s, cfg, false, &c};
+        const int64_t end = std::min<int64_t>(total, (ch + 1) * 64);
+        for (int64_t pix = ch * 64; pix < end; ++pix) {
+            QuaternaryHeap q;
+            for (uint32_t it = first; it < first + num; ++it) {
+                Rgb L = render_pixel(pc, (uint32_t)(pix / s->width), (uint32_t)(pix % s->width), it);
+                float w = average(L);
+                if (!std::isfinite(w)) continue;  // Splat returns before Notify (IgnoreNanAndInf)
+                if ((int32_t)q.a.size() < n) q.enqueue(w, (int32_t)it);
+                else q.enqueue_dequeue(w, (int32_t)it);
+            }
+            std::vector<std::pair<float, int32_t>> out = q.a;
+            std::stable_sort(out.begin(), out.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
+            for (int32_t j = 0; j < n; ++j) {
+                const bool have = j < (int32_t)out.size();
+                weights[pix * n + j] = have ? out[j].first : 0.0f;
+                iterations[pix * n + j] = have ? out[j].second : -1;
+            }
+        }
+    });
+}
+
 void or_render_pixel(const or_scene* s, const sspt_pt_settings* cfg, uint32_t row, uint32_t col,
                      uint32_t iteration, float* out) {
     Counters c;

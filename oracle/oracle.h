@@ -97,6 +97,11 @@ void or_render_pixel(const or_scene* s, const sspt_pt_settings* settings, uint32
 /* DenoiseBuffers layers: running means of the primary hit's GetScatterStrength
  * (albedo) and shading normal over iterations first..first+num-1, W*H*3 each,
  * updated in place (start from zeros for a fresh framebuffer). */
+/* The FrameBuffer's outlier cache after rendering iterations first..first+num-1:
+ * per pixel the n largest samples by average estimate, W*H*n weights and
+ * iterations in descending weight (weight 0 / iteration -1 past the kept count). */
+void or_render_outliers(const or_scene* s, const sspt_pt_settings* settings, uint32_t first, uint32_t num, int32_t n,
+                        float* weights, int32_t* iterations, int32_t threads);
 /* PathTracerBase.ReplayPixel: the path graph (sspt_path_node array, at most
  * `capacity` written) and estimate of pixel (col, row) in `iteration`;
  * returns the node count. */

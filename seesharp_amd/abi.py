@@ -25,6 +25,7 @@ STATUS_NAMES = {
     5: "SSPT_ERR_UNSUPPORTED",
     6: "SSPT_ERR_ZERO_AREA_EMITTER",
 }
+MAX_OUTLIERS = 4
 EMITTER_DIFFUSE = 0
 EMITTER_GLOSSY = 1
 LAYER_ALBEDO = 0
@@ -193,6 +194,8 @@ SYMBOLS = {
     "sspt_get_kernel_timing": (C.c_int, [_vp, C.c_char_p, C.POINTER(C.c_uint64),
                                          C.POINTER(C.c_double)]),
     "sspt_reset_kernel_timing": (C.c_int, [_vp]),
+    "sspt_framebuffer_track_outliers": (C.c_int, [_vp, _vp, C.c_uint32]),
+    "sspt_framebuffer_read_outliers": (C.c_int, [_vp, _vp, C.c_uint32, _fp, C.POINTER(C.c_int32)]),
     "sspt_replay_pixel": (C.c_int, [_vp, _vp, C.c_uint32, C.POINTER(PtSettings), C.c_uint32, C.c_uint32,
                                     C.c_uint32, C.POINTER(PathNode), C.c_uint32, C.POINTER(C.c_uint32), _fp]),
 }

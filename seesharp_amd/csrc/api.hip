@@ -112,6 +112,8 @@ struct sspt_scene {
     float* grad = nullptr;
     float* adj_image = nullptr;
     float* layers = nullptr;  // albedo pool then normal pool, framebuffer layout each
+    // outlier tracking: image_floats / 3 * dev.num_outliers keys, or null (off)
+    unsigned long long* outliers = nullptr;
 };
 
 namespace {
@@ -733,6 +735,9 @@ enum class Mode { Mean, Sum, Adjoint };
 void enqueue_wave(sspt_ctx* ctx, sspt_scene* sc, const LaunchConfig& lc, int max_depth, Mode mode,
                   float* mean_image, uint32_t mean_pixels, float* mean_albedo) {
     float* mean_normal = mean_albedo ? mean_albedo + sc->dev.layer_stride : nullptr;
+    // running-mean renders are single-scene: the scene's keys start at the pool's start
+    unsigned long long* mean_outl = sc->dev.outliers;
+    const int n_outl = sc->dev.num_outliers;
     const DevScene& S = sc->dev;
     const Wave& w = ctx->wave;
     hipStream_t st = ctx->stream;
@@ -742,7 +747,10 @@ void enqueue_wave(sspt_ctx* ctx, sspt_scene* sc, const LaunchConfig& lc, int max
         timed(ctx, "path", [&] { launch_path(st, S, w, lc); });
         if (mode == Mode::Mean)
             timed(ctx, "accumulate",
-                  [&] { launch_accumulate_mean(st, w, mean_image, mean_pixels, mean_albedo, mean_normal); });
+                  [&] {
+                      launch_accumulate_mean(st, w, mean_image, mean_pixels, mean_albedo, mean_normal, mean_outl,
+                                             n_outl);
+                  });
         else
             timed(ctx, "accumulate", [&] { launch_accumulate_sum(st, S, w, lc, sc->image); });
         launch_wave_end(st, w);
@@ -764,7 +772,9 @@ void enqueue_wave(sspt_ctx* ctx, sspt_scene* sc, const LaunchConfig& lc, int max
     }
     if (mode == Mode::Mean)
         timed(ctx, "accumulate",
-              [&] { launch_accumulate_mean(st, w, mean_image, mean_pixels, mean_albedo, mean_normal); });
+              [&] {
+                  launch_accumulate_mean(st, w, mean_image, mean_pixels, mean_albedo, mean_normal, mean_outl, n_outl);
+              });
     else if (mode == Mode::Sum)
         timed(ctx, "accumulate", [&] { launch_accumulate_sum(st, S, w, lc, sc->image); });
     else
@@ -852,6 +862,15 @@ bool valid_settings(const sspt_pt_settings& s) {
     return s.max_depth >= 1 && s.num_shadow_rays >= 0 && s.min_depth >= 0 && s.total_spp >= 0;
 }
 
+// Empties outlier tracking (an all-zero key is "no sample").
+sspt_status clear_outliers(sspt_ctx* ctx, sspt_scene* sc) {
+    if (!sc->outliers) return SSPT_OK;
+    HIP_TRY(ctx, hipMemsetAsync(sc->outliers, 0,
+                                (size_t)(sc->image_floats / 3) * sc->dev.num_outliers * sizeof(unsigned long long),
+                                ctx->stream));
+    return SSPT_OK;
+}
+
 void apply_settings(SceneRec& R, const sspt_pt_settings& s) {
     R.base_seed = s.base_seed;
     R.max_depth = s.max_depth;
@@ -927,6 +946,7 @@ void sspt_scene_destroy(sspt_scene* sc) {
     (void)hipSetDevice(sc->ctx->device);
     (void)hipStreamSynchronize(sc->ctx->stream);
     for (void* p : sc->allocs) (void)hipFree(p);
+    if (sc->outliers) (void)hipFree(sc->outliers);
     delete sc;
 }
 
@@ -950,8 +970,70 @@ sspt_status sspt_framebuffer_clear(sspt_ctx* ctx, sspt_scene* sc) {
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     HIP_TRY(ctx, hipMemsetAsync(sc->image, 0, sc->image_floats * sizeof(float), ctx->stream));
     HIP_TRY(ctx, hipMemsetAsync(sc->layers, 0, 2 * sc->image_floats * sizeof(float), ctx->stream));
+    TRY(clear_outliers(ctx, sc));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     sc->nan_count = 0;
+    return SSPT_OK;
+}
+
+sspt_status sspt_framebuffer_track_outliers(sspt_ctx* ctx, sspt_scene* sc, uint32_t n) {
+    if (!ctx || !sc) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "null argument");
+    if (n > SSPT_MAX_OUTLIERS)
+        return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "at most %d outliers per pixel are tracked", SSPT_MAX_OUTLIERS);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (sc->outliers) {
+        HIP_TRY(ctx, hipFree(sc->outliers));
+        sc->outliers = nullptr;
+    }
+    sc->dev.outliers = nullptr;
+    sc->dev.num_outliers = 0;
+    if (n == 0) return SSPT_OK;
+    const size_t bytes = (size_t)(sc->image_floats / 3) * n * sizeof(unsigned long long);
+    if (hipMalloc((void**)&sc->outliers, bytes) != hipSuccess) {
+        sc->outliers = nullptr;
+        return fail(ctx, SSPT_ERR_OUT_OF_MEMORY, "hipMalloc of %zu bytes for outlier tracking failed", bytes);
+    }
+    sc->dev.outliers = sc->outliers;
+    sc->dev.num_outliers = (int)n;
+    TRY(clear_outliers(ctx, sc));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return SSPT_OK;
+}
+
+sspt_status sspt_framebuffer_read_outliers(sspt_ctx* ctx, sspt_scene* sc, uint32_t index, float* weights,
+                                           int32_t* iterations) {
+    if (!ctx || !sc) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "null argument");
+    if (index >= sc->recs.size()) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "scene index out of range");
+    const int n = sc->dev.num_outliers;
+    if (n == 0) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "outlier tracking is off");
+    const SceneRec& R = sc->recs[index];
+    const size_t count = (size_t)R.num_pixels * n;
+    std::vector<unsigned long long> keys(count);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipMemcpyAsync(keys.data(), sc->outliers + (R.fb_off / 3) * n, count * sizeof(unsigned long long),
+                                hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    // FrameBuffer.WriteToFile's order: the queue's entries in heap-array order
+    // (UnorderedItems), stably sorted by descending weight (FrameBuffer.cs:321-325)
+    std::vector<std::pair<float, int32_t>> q;
+    for (uint32_t p = 0; p < R.num_pixels; ++p) {
+        q.clear();
+        for (int j = 0; j < n; ++j) {
+            const unsigned long long e = keys[(size_t)p * n + j];
+            if ((uint32_t)e == 0) continue;  // empty
+            float w;
+            const uint32_t b = (uint32_t)(e >> 32);
+            std::memcpy(&w, &b, sizeof(w));
+            q.push_back({w, (int32_t)((uint32_t)e - 1u)});
+        }
+        std::stable_sort(q.begin(), q.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+        for (int j = 0; j < n; ++j) {
+            const size_t o = (size_t)p * n + j;
+            if (weights) weights[o] = j < (int)q.size() ? q[j].first : 0.0f;
+            if (iterations) iterations[o] = j < (int)q.size() ? q[j].second : -1;
+        }
+    }
     return SSPT_OK;
 }
 
@@ -1074,6 +1156,7 @@ sspt_status sspt_render_batch(sspt_ctx* ctx, sspt_scene* sc, const sspt_pt_setti
     HIP_TRY(ctx, hipMemsetAsync(ctx->wave.stats, 0, 64, ctx->stream));
     HIP_TRY(ctx, hipMemsetAsync(sc->image, 0, sc->image_floats * sizeof(float), ctx->stream));
     HIP_TRY(ctx, hipMemsetAsync(sc->layers, 0, 2 * sc->image_floats * sizeof(float), ctx->stream));
+    TRY(clear_outliers(ctx, sc));
     sc->nan_count = 0;
     if (!plan.waves.empty()) {
         TRY(run_waves(ctx, sc, plan.waves, plan.max_depth, plan.max_nsr, Mode::Sum, nullptr, 0, nullptr, -1));

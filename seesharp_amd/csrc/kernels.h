@@ -65,7 +65,7 @@ void launch_shade(hipStream_t s, const DevScene& sc, const Wave& w, const Launch
 void launch_shadow(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc);
 // running mean of one scene (waves of whole iterations)
 void launch_accumulate_mean(hipStream_t s, const Wave& w, float* image, uint32_t num_pixels, float* albedo,
-                            float* normal);
+                            float* normal, unsigned long long* outliers, int num_outliers);
 // per-scene sums of a batch (any wave layout)
 void launch_accumulate_sum(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc,
                            float* image_pool);

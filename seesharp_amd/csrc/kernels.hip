@@ -1085,8 +1085,80 @@ __global__ void __launch_bounds__(256) k_adjoint_finalize(DevScene S, Wave w) {
 // FrameBuffer running mean over the wave's iterations (FrameBuffer.cs:89-160),
 // and of the albedo / normal layers when they are written (RgbLayer.Splat,
 // Layer.OnStartIteration: the same normalisation, no NaN filter).
+// OutlierReplayCache (SeeSharp/Integrators/Util/OutlierReplayCache.cs:9-22): per
+// pixel a .NET PriorityQueue<_, float> of at most n <= 4 samples keyed by the
+// sample's average estimate, fed in iteration order. The queue is a quaternary
+// min-heap (System.Collections.Generic.PriorityQueue, .NET 6+): Enqueue places
+// the new entry at the end and moves it above its parent while strictly
+// smaller; EnqueueDequeue (when full) replaces the root only if the new key is
+// strictly larger, then moves down into the first strictly smallest child
+// while that child is strictly smaller. With n <= 4 every entry is the root or
+// its child. An entry is (weight bits << 32) | (iteration + 1), 0 = empty; the
+// array is kept in heap order, which decides ties on read-out.
+struct OutlierHeap {
+    float w[4];
+    uint32_t it[4];  // iteration + 1, 0 = empty
+    int size;
+
+    __device__ __forceinline__ void load(const unsigned long long* p, int n) {
+        size = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            unsigned long long e = j < n ? p[j] : 0ull;
+            w[j] = __uint_as_float((uint32_t)(e >> 32));
+            it[j] = (uint32_t)e;
+            size += it[j] != 0u;
+        }
+    }
+    __device__ __forceinline__ void store(unsigned long long* p, int n) const {
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (j < n) p[j] = ((unsigned long long)__float_as_uint(w[j]) << 32) | it[j];
+    }
+    __device__ __forceinline__ void set(int j, float wv, uint32_t iv) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (k == j) {
+                w[k] = wv;
+                it[k] = iv;
+            }
+    }
+    __device__ __forceinline__ void add(float weight, uint32_t iteration, int n) {
+        const uint32_t tag = iteration + 1u;
+        if (size < n) {  // Enqueue: move up (the parent of entries 1..3 is entry 0)
+            int idx = size++;
+            if (idx > 0 && weight < w[0]) {
+                set(idx, w[0], it[0]);
+                idx = 0;
+            }
+            set(idx, weight, tag);
+        } else if (size > 0 && weight > w[0]) {  // EnqueueDequeue: move down from the root
+            int mc = 1;
+            float mw = size > 1 ? w[1] : 0.0f;
+            if (size > 2 && w[2] < mw) {
+                mc = 2;
+                mw = w[2];
+            }
+            if (size > 3 && w[3] < mw) {
+                mc = 3;
+                mw = w[3];
+            }
+            if (size > 1 && weight > mw) {
+                uint32_t mi = mc == 1 ? it[1] : mc == 2 ? it[2] : it[3];
+                w[0] = mw;
+                it[0] = mi;
+                set(mc, weight, tag);
+            } else {
+                w[0] = weight;
+                it[0] = tag;
+            }
+        }
+    }
+};
+
 __global__ void __launch_bounds__(256) k_accumulate_mean(Wave w, float* image, uint32_t num_pixels, float* albedo,
-                                                          float* normal) {
+                                                          float* normal, unsigned long long* outliers,
+                                                          int num_outliers) {
     const WaveDesc d = cur_wave(w);
     for (uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x; pix < num_pixels; pix += gridDim.x * blockDim.x) {
         f3 img = mk3(image[3 * pix], image[3 * pix + 1], image[3 * pix + 2]);
@@ -1095,14 +1167,21 @@ __global__ void __launch_bounds__(256) k_accumulate_mean(Wave w, float* image, u
             alb = mk3(albedo[3 * pix], albedo[3 * pix + 1], albedo[3 * pix + 2]);
             nrm = mk3(normal[3 * pix], normal[3 * pix + 1], normal[3 * pix + 2]);
         }
+        OutlierHeap heap;
+        unsigned long long* ok = outliers ? outliers + (size_t)pix * num_outliers : nullptr;
+        if (ok) heap.load(ok, num_outliers);
         uint32_t nans = 0;
         for (uint32_t s = 0; s < d.iters; ++s) {
             uint32_t slot = s * num_pixels + pix;
             f3 L = f3_of(w.st_L[slot]) + f3_of(w.st_Le[slot]) + f3_of(w.st_La[slot]);
             float n = (float)(d.first_iter + s + 1);
             if (d.first_iter + s + 1 > 1) img = img * ((n - 1.0f) / n);
-            if (isfinite(average(L))) img = img + L / n;
-            else ++nans;
+            if (isfinite(average(L))) {
+                img = img + L / n;
+                if (ok) heap.add(average(L), d.first_iter + s, num_outliers);
+            } else {
+                ++nans;
+            }
             if (albedo) {
                 if (d.first_iter + s + 1 > 1) {
                     alb = alb * ((n - 1.0f) / n);
@@ -1115,6 +1194,7 @@ __global__ void __launch_bounds__(256) k_accumulate_mean(Wave w, float* image, u
         image[3 * pix] = img.x;
         image[3 * pix + 1] = img.y;
         image[3 * pix + 2] = img.z;
+        if (ok) heap.store(ok, num_outliers);
         if (albedo) {
             albedo[3 * pix] = alb.x;
             albedo[3 * pix + 1] = alb.y;
@@ -1135,6 +1215,28 @@ __global__ void __launch_bounds__(256) k_accumulate_sum(DevScene S, Wave w, floa
         f3 L = f3_of(w.st_L[slot]) + f3_of(w.st_Le[slot]) + f3_of(w.st_La[slot]);
         uint32_t tag = w.st_pix[slot];
         const SceneRec& R = S.recs[tag >> kSlotBits];
+        if (S.outliers) {
+            // A scene's paths are laid out iteration after iteration, so the
+            // samples of one pixel in this wave sit num_pixels slots apart. The
+            // slot holding the pixel's first of them feeds the pixel's queue
+            // with all of them in iteration order (as the reference's
+            // iterations are) and no other thread touches that queue.
+            const uint64_t k = d.start + slot - R.path_off;  // the path's index within its scene
+            const uint32_t np = R.num_pixels;
+            if (k < np || slot < np) {
+                OutlierHeap heap;
+                unsigned long long* q = S.outliers + (R.fb_off / 3 + (tag & kSlotMask)) * (uint64_t)S.num_outliers;
+                heap.load(q, S.num_outliers);
+                const uint64_t scene_end = S.path_offsets[(tag >> kSlotBits) + 1] - d.start;  // in wave slots
+                const uint32_t it0 = R.first_iter + (uint32_t)(k / np);
+                uint32_t j = 0;
+                for (uint64_t t = slot; t < d.count && t < scene_end; t += np, ++j) {
+                    f3 Lt = f3_of(w.st_L[t]) + f3_of(w.st_Le[t]) + f3_of(w.st_La[t]);
+                    if (isfinite(average(Lt))) heap.add(average(Lt), it0 + j, S.num_outliers);
+                }
+                heap.store(q, S.num_outliers);
+            }
+        }
         if (R.enable_denoiser) {
             float* a = S.layers + R.fb_off + 3ull * (tag & kSlotMask);
             float* nm = a + S.layer_stride;
@@ -1249,9 +1351,9 @@ void launch_adjoint_finalize(hipStream_t s, const DevScene& sc, const Wave& w, c
     hipLaunchKernelGGL(k_adjoint_finalize, dim3(grid_of(lc.shade_blocks, lc.max_paths, 256)), dim3(256), 0, s, sc, w);
 }
 void launch_accumulate_mean(hipStream_t s, const Wave& w, float* image, uint32_t num_pixels, float* albedo,
-                            float* normal) {
+                            float* normal, unsigned long long* outliers, int num_outliers) {
     hipLaunchKernelGGL(k_accumulate_mean, dim3(std::min(blocks(num_pixels, 256), 8192u)), dim3(256), 0, s, w, image,
-                       num_pixels, albedo, normal);
+                       num_pixels, albedo, normal, outliers, num_outliers);
 }
 void launch_accumulate_sum(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, float* pool) {
     hipLaunchKernelGGL(k_accumulate_sum, dim3(grid_of(lc.shade_blocks, lc.max_paths, 256)), dim3(256), 0, s, sc, w,

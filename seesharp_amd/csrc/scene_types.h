@@ -124,6 +124,10 @@ struct DevScene {
     float* grad;                   // adjoint renders: d loss / d texel, texel-pool layout
     float* layers;                 // albedo layers (framebuffer layout), then normal layers at + layer_stride
     uint64_t layer_stride;
+    // FrameBuffer outlier tracking: per pixel of the framebuffer pool, num_outliers
+    // keys (average estimate, iteration) of the largest samples so far; null = off
+    unsigned long long* outliers;
+    int num_outliers;
     int num_scenes;
     int num_nodes;
 };

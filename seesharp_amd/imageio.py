@@ -172,7 +172,9 @@ def _read_exr_header(data: bytes):
     return attrs, p
 
 
-def read_exr(path: str) -> np.ndarray:
+def read_exr_channels(path: str) -> dict:
+    """Every channel of a scanline OpenEXR file (NONE / ZIPS / ZIP, HALF / FLOAT)
+    as an (h, w) float32 plane, by channel name."""
     data = open(path, "rb").read()
     attrs, p = _read_exr_header(data)
     comp = attrs["compression"][1][0]
@@ -216,12 +218,37 @@ def read_exr(path: str) -> np.ndarray:
                 vals = np.frombuffer(block[q:q + n], np.float16 if t == _HALF else np.float32)
                 planes[name][row] = vals.astype(np.float32)
                 q += n
-    names = [n for n, _ in channels]
-    if all(c in names for c in "RGB"):
-        return np.stack([planes["R"], planes["G"], planes["B"]], -1)
-    if "Y" in names:
-        return np.repeat(planes["Y"][..., None], 3, -1)
-    raise ValueError(f"{path}: no R, G, B or Y channels")
+    return planes
+
+
+def read_exr_layers(path: str) -> dict:
+    """The layers of an OpenEXR file: "" for the main image, otherwise the layer
+    name of "<layer>.<channel>" channels; RGB layers as (h, w, 3), single-channel
+    (Y) layers as (h, w)."""
+    planes = read_exr_channels(path)
+    groups: dict = {}
+    for name, plane in planes.items():
+        layer, _, ch = name.rpartition(".")
+        groups.setdefault(layer, {})[ch] = plane
+    out = {}
+    for layer, chans in groups.items():
+        if all(c in chans for c in "RGB"):
+            out[layer] = np.stack([chans["R"], chans["G"], chans["B"]], -1)
+        elif "Y" in chans:
+            out[layer] = chans["Y"]
+        elif len(chans) == 1:
+            out[layer] = next(iter(chans.values()))
+    return out
+
+
+def read_exr(path: str) -> np.ndarray:
+    """The main image of an OpenEXR file as (h, w, 3) float32 (R, G, B, or Y
+    repeated)."""
+    layers = read_exr_layers(path)
+    img = layers.get("")
+    if img is None:
+        raise ValueError(f"{path}: no R, G, B or Y channels")
+    return img if img.ndim == 3 else np.repeat(img[..., None], 3, -1)
 
 
 def read_image(path: str) -> np.ndarray:
@@ -237,14 +264,36 @@ def read_image(path: str) -> np.ndarray:
 def write_exr(path: str, rgb: np.ndarray, compression: str = "zip") -> None:
     """Writes an (h, w, 3) float32 image as a scanline OpenEXR file with FLOAT
     R, G, B channels, NONE or ZIP compression."""
-    rgb = np.ascontiguousarray(rgb, np.float32)
-    h, w = rgb.shape[:2]
+    write_exr_layers(path, [(None, rgb)], compression)
+
+
+def write_exr_layers(path: str, layers, compression: str = "zip") -> None:
+    """Writes several same-sized images into one scanline OpenEXR file with FLOAT
+    channels: the main image (name None or "") as R, G, B, a named RGB layer as
+    <name>.R/.G/.B and a single-channel (h, w) layer as <name>.Y; channels sorted
+    by name as the format requires. NONE or ZIP compression."""
+    planes = {}
+    shape = None
+    for name, img in layers:
+        img = np.asarray(img, np.float32)
+        prefix = f"{name}." if name else ""
+        if img.ndim == 3:
+            for k, c in enumerate("RGB"):
+                planes[prefix + c] = img[..., k]
+        else:
+            planes[prefix + "Y"] = img
+        if shape is None:
+            shape = img.shape[:2]
+        elif img.shape[:2] != shape:
+            raise ValueError("all layers of an OpenEXR file need the same size")
+    h, w = shape
+    names = sorted(planes, key=lambda n: n.encode())
     comp = {"none": _NONE, "zip": _ZIP}[compression]
 
     def attr(name, typ, payload):
         return name.encode() + b"\0" + typ.encode() + b"\0" + struct.pack("<i", len(payload)) + payload
 
-    chl = b"".join(c.encode() + b"\0" + struct.pack("<iBBBBii", _FLOAT, 0, 0, 0, 0, 1, 1) for c in "BGR") + b"\0"
+    chl = b"".join(c.encode() + b"\0" + struct.pack("<iBBBBii", _FLOAT, 0, 0, 0, 0, 1, 1) for c in names) + b"\0"
     header = struct.pack("<ii", _EXR_MAGIC, 2)
     header += attr("channels", "chlist", chl)
     header += attr("compression", "compression", bytes([comp]))
@@ -256,10 +305,10 @@ def write_exr(path: str, rgb: np.ndarray, compression: str = "zip") -> None:
     header += attr("screenWindowWidth", "float", struct.pack("<f", 1.0))
     header += b"\0"
     lines = 16 if comp == _ZIP else 1
+    cols = [np.ascontiguousarray(planes[n], np.float32) for n in names]
     blocks = []
     for y in range(0, h, lines):
-        rows = rgb[y:y + lines]
-        raw = b"".join(np.ascontiguousarray(r[:, c]).tobytes() for r in rows for c in (2, 1, 0))
+        raw = b"".join(c[r].tobytes() for r in range(y, min(h, y + lines)) for c in cols)
         if comp == _ZIP:
             packed = zlib.compress(_exr_predict(raw))
             if len(packed) < len(raw):

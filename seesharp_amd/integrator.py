@@ -17,7 +17,7 @@ import time
 
 import numpy as np
 
-from . import abi
+from . import abi, metrics
 
 
 class Device:
@@ -131,6 +131,7 @@ class GpuScene(_TextureParameters):
         self.ptr = h
         self.width = scene.FrameBuffer.Width
         self.height = scene.FrameBuffer.Height
+        self.num_outliers = 0
 
     def close(self):
         if getattr(self, "ptr", None):
@@ -207,6 +208,15 @@ class GpuScene(_TextureParameters):
         self.device.check(self.device.lib.sspt_trace_occluded(
             self.device.ptr, self.ptr, abi.fptr(rays), rays.shape[0], out.ctypes.data_as(C.POINTER(C.c_uint8))))
         return out
+
+    def track_outliers(self, n: int):
+        """Outlier tracking of n <= 4 samples per pixel (0 = off), starting empty."""
+        self.device.check(self.device.lib.sspt_framebuffer_track_outliers(self.device.ptr, self.ptr, n))
+        self.num_outliers = n
+
+    def read_outliers(self):
+        """(weights, iterations), each H x W x n, largest first."""
+        return _read_outliers(self.device, self.ptr, 0, self.height, self.width, self.num_outliers)
 
     def replay_pixel(self, settings: abi.PtSettings, col: int, row: int, iteration: int):
         """ReplayPixel (PathTracer.cs:61-69): the sspt_path_node records and the
@@ -301,6 +311,15 @@ class GpuBatch(_TextureParameters):
         h, w = self.sizes[index]
         return _read_layer(self.device, self.ptr, index, name, h, w)
 
+    def track_outliers(self, n: int):
+        """Outlier tracking of n <= 4 samples per pixel for every scene / view (0 = off)."""
+        self.device.check(self.device.lib.sspt_framebuffer_track_outliers(self.device.ptr, self.ptr, n))
+        self.num_outliers = n
+
+    def read_outliers(self, index: int):
+        h, w = self.sizes[index]
+        return _read_outliers(self.device, self.ptr, index, h, w, self.num_outliers)
+
     def replay_pixel(self, index: int, settings: abi.PtSettings, col: int, row: int, iteration: int):
         """ReplayPixel of scene / view `index` of the world."""
         return _replay(self.device, self.ptr, index, settings, col, row, iteration)
@@ -335,6 +354,14 @@ def _replay(device, ptr, index, settings, col, row, iteration):
     if count.value > cap:
         raise RuntimeError(f"replayed path has {count.value} nodes, more than the bound {cap}")
     return list(nodes[:count.value]), est
+
+
+def _read_outliers(device, ptr, index, h, w, n):
+    weights = np.zeros((h, w, n), np.float32)
+    its = np.zeros((h, w, n), np.int32)
+    device.check(device.lib.sspt_framebuffer_read_outliers(device.ptr, ptr, index, abi.fptr(weights),
+                                                           its.ctypes.data_as(C.POINTER(C.c_int32))))
+    return weights, its
 
 
 def _hits(hits):
@@ -395,14 +422,36 @@ class GpuPathTracer(Integrator):
             raise NotImplementedError("RenderTechniquePyramid is not supported by the GPU path tracer")
         gs = self.prepare(scene)
         fb = scene.FrameBuffer
+        n_outl = max(0, min(abi.MAX_OUTLIERS, int(getattr(fb, "NumOutliersToTrack", 0))))
+        if gs.num_outliers != n_outl:
+            gs.track_outliers(n_outl)
         gs.clear()
-        t0 = time.perf_counter()
-        stats = gs.render(self.settings(), 0, self.TotalSpp)
-        render_ms = (time.perf_counter() - t0) * 1000.0
+        cfg = self.settings()
+        if getattr(fb, "ReferenceImage", None) is None:
+            t0 = time.perf_counter()
+            stats = gs.render(cfg, 0, self.TotalSpp)
+            render_ms = (time.perf_counter() - t0) * 1000.0
+        else:
+            # FrameBuffer.EndIteration with a reference image (FrameBuffer.cs:267-291):
+            # one error entry per iteration, timed over the render calls only
+            stats = {k: 0 for k in ("num_rays", "num_shadow_rays", "num_occluded", "num_ray_hits",
+                                    "num_shadow_rays_traced")}
+            render_ms = 0.0
+            fb.Errors = []
+            for it in range(self.TotalSpp):
+                if self.MaximumRenderTimeMs is not None and it > 0 and render_ms >= self.MaximumRenderTimeMs:
+                    break
+                t0 = time.perf_counter()
+                st = gs.render(cfg, it, 1)
+                render_ms += (time.perf_counter() - t0) * 1000.0
+                for k in stats:
+                    stats[k] += st[k]
+                fb.Errors.append(metrics.ErrorMetric(render_ms, gs.read_image(), fb.ReferenceImage))
         fb.Image = gs.read_image()
         if self.EnableDenoiser:
             fb.Layers["albedo"] = gs.read_layer("albedo")
             fb.Layers["normal"] = gs.read_layer("normal")
+        fb.Outliers = gs.read_outliers() if n_outl else None
         fb.CurIteration = self.TotalSpp
         fb.NaNWarnings = gs.nan_count()
         fb.MetaData["RenderTime"] = render_ms

@@ -226,7 +226,13 @@ class PerspectiveCamera:
 
 
 class FrameBuffer:
-    """Running-mean RGB image (FrameBuffer.cs:78-160). ``Image`` is H x W x 3 fp32."""
+    """Running-mean RGB image (SeeSharp/Images/FrameBuffer.cs). ``Image`` is H x W x 3 fp32.
+
+    Beside the image it holds the named AOV layers, the outlier cache (per
+    pixel the NumOutliersToTrack largest samples and their iterations,
+    FrameBuffer.cs:29,186-189), the per-iteration error log against
+    ``ReferenceImage`` (``Errors``, FrameBuffer.cs:267-291,377-383) and the
+    metadata ``WriteToFile`` stores beside the image (FrameBuffer.cs:313-367)."""
 
     IgnoreNanAndInf = 16
     Recommended = IgnoreNanAndInf
@@ -242,6 +248,47 @@ class FrameBuffer:
         # AOV layers by name (FrameBuffer.AddLayer); the GPU path tracer fills
         # "albedo" and "normal" when EnableDenoiser is set (DenoiseBuffers).
         self.Layers = {}
+        self.NumOutliersToTrack = 4
+        # (weights, iterations), each H x W x NumOutliersToTrack, descending weight;
+        # weight 0 / iteration -1 where a pixel has fewer samples
+        self.Outliers = None
+        self.ReferenceImage = None
+        self.Errors: list[dict] = []
+
+    def GetPixelOutlier(self, col: int, row: int):
+        """OutlierCache.GetPixelOutlier: [(iteration, weight)] of the pixel, largest first."""
+        if self.Outliers is None:
+            return []
+        w, it = self.Outliers
+        return [(int(i), float(x)) for x, i in zip(w[row, col], it[row, col]) if i >= 0]
+
+    def WriteToFile(self, fname: str | None = None):
+        """FrameBuffer.WriteToFile (FrameBuffer.cs:313-367) for OpenEXR: the image,
+        every layer and one "outlier-i" layer per tracked sample (its iteration)
+        in one file, and the metadata as JSON beside it (<basename>.json)."""
+        import json
+        import os
+
+        from . import imageio
+
+        fname = fname or self.Filename
+        if not fname:
+            raise ValueError("no file name")
+        base, ext = os.path.splitext(fname)
+        if ext.lower() != ".exr":
+            raise NotImplementedError(f"{fname}: only OpenEXR output is supported")
+        layers = [(name, img) for name, img in self.Layers.items()] + [(None, self.Image)]
+        if self.Outliers is not None:
+            its = self.Outliers[1]
+            for i in range(its.shape[2]):
+                layers.append((f"outlier-{i}", np.maximum(its[..., i], 0).astype(np.float32)))
+        imageio.write_exr_layers(fname, layers)
+        meta = dict(self.MetaData)
+        if self.Errors:
+            meta["ErrorMetrics"] = self.Errors
+        meta["NaNWarnings"] = self.NaNWarnings
+        with open(base + ".json", "w") as f:
+            json.dump(meta, f, indent=2)
 
 
 class Scene:

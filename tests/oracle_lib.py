@@ -56,6 +56,8 @@ _SIGS = {
                                C.POINTER(abi.RayStats), C.POINTER(C.c_int32), C.c_int64, C.c_int32]),
     "or_render_pixel": (None, [_vp, C.POINTER(abi.PtSettings), C.c_uint32, C.c_uint32, C.c_uint32, _fp]),
     "or_set_texture": (C.c_int32, [_vp, C.c_uint32, _fp]),
+    "or_render_outliers": (None, [_vp, C.POINTER(abi.PtSettings), C.c_uint32, C.c_uint32, C.c_int32, _fp,
+                                  C.POINTER(C.c_int32), C.c_int32]),
     "or_replay_pixel": (C.c_uint32, [_vp, C.POINTER(abi.PtSettings), C.c_uint32, C.c_uint32, C.c_uint32,
                                      C.POINTER(abi.PathNode), C.c_uint32, _fp]),
     "or_primary_aov": (None, [_vp, C.POINTER(abi.PtSettings), C.c_uint32, C.c_uint32, _fp, _fp, C.c_int32]),
@@ -141,6 +143,14 @@ class OracleScene:
         n = lib().or_replay_pixel(self.ptr, C.byref(cfg), col, row, iteration, nodes, cap, fp(est))
         assert n <= cap
         return list(nodes[:n]), est
+
+    def render_outliers(self, cfg, first, num, n, threads=0):
+        """(weights, iterations), each H x W x n: the outlier cache after the render."""
+        w = np.zeros((self.height, self.width, n), np.float32)
+        it = np.zeros((self.height, self.width, n), np.int32)
+        lib().or_render_outliers(self.ptr, C.byref(cfg), first, num, n, fp(w),
+                                 it.ctypes.data_as(C.POINTER(C.c_int32)), threads)
+        return w, it
 
     def render_pixel(self, cfg, col, row, iteration):
         out = np.zeros(3, np.float32)
