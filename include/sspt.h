@@ -363,6 +363,16 @@ sspt_status sspt_replay_pixel(sspt_ctx* ctx, sspt_scene* scene, uint32_t index, 
                               uint32_t col, uint32_t row, uint32_t iteration, sspt_path_node* host_nodes,
                               uint32_t capacity, uint32_t* num_nodes, float* estimate_rgb);
 
+/* ---- DebugVisualizer (SeeSharp/Integrators/DebugVisualizer.cs:6-59) --------
+ * Renders iterations first_iteration .. first_iteration+num_iterations-1 of the
+ * grayscale "eye light" view of scene / view `index`: per sample the jittered
+ * primary ray (RNG seeded by (base_seed, row*W+col, iteration)), |cos| between
+ * the hit's face normal and the direction back to the camera, 0 on a miss,
+ * into the framebuffer with running-mean semantics (clear it first for a new
+ * image). */
+sspt_status sspt_render_debug(sspt_ctx* ctx, sspt_scene* scene, uint32_t index, uint32_t base_seed,
+                              uint32_t first_iteration, uint32_t num_iterations);
+
 /* ---- measurement ------------------------------------------------------- */
 /* Enables HIP-event timing of every kernel launch on the context stream. */
 sspt_status sspt_set_kernel_timing(sspt_ctx* ctx, int32_t enable);

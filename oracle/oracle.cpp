@@ -1994,6 +1994,43 @@ void or_render_outliers(const or_scene* s, const sspt_pt_settings* cfg, uint32_t
     });
 }
 
+// DebugVisualizer.Render (DebugVisualizer.cs:20-32), RenderPixel (:47-59) and
+// ComputeColor (:37-45): per iteration and pixel the jittered primary ray's
+// hit, |dot(face normal, -dir)| / |normal| / |dir| (0 on a miss), splatted
+// into the running-mean image (FrameBuffer.StartIteration / Splat).
+void or_render_debug(const or_scene* s, uint32_t base_seed, uint32_t first, uint32_t num, float* image,
+                     int32_t threads) {
+    const int64_t total = (int64_t)s->width * s->height;
+    for (uint32_t it = first; it < first + num; ++it) {
+        const float n = (float)(it + 1);
+        parallel_for((total + 63) / 64, threads, [&](int64_t ch) {
+            const int64_t end = std::min<int64_t>(total, (ch + 1) * 64);
+            for (int64_t pix = ch * 64; pix < end; ++pix) {
+                Rng rng{hash_seed(base_seed, (uint32_t)pix, it)};
+                V2 off = rng.next_float2();
+                Ray ray;
+                s->camera.generate_ray((float)(pix % s->width) + off.x, (float)(pix / s->width) + off.y, &ray.o,
+                                       &ray.d);
+                ray.tmin = 0.0f;
+                ray.tmax = INFINITY;
+                HitRec h = trace_closest(*s, ray, false);
+                float value = 0.0f;
+                if (h.tri >= 0) {
+                    SurfacePoint hit = make_point(*s, h.tri, {h.u, h.v}, h.t);
+                    V3 from = -ray.d;
+                    value = std::fabs(dot(hit.normal, from));
+                    value /= std::sqrt(dot(hit.normal, hit.normal));
+                    value /= std::sqrt(dot(from, from));
+                }
+                for (int c = 0; c < 3; ++c) {
+                    if (it + 1 > 1) image[pix * 3 + c] *= (n - 1.0f) / n;
+                    image[pix * 3 + c] += value / n;
+                }
+            }
+        });
+    }
+}
+
 void or_render_pixel(const or_scene* s, const sspt_pt_settings* cfg, uint32_t row, uint32_t col,
                      uint32_t iteration, float* out) {
     Counters c;

@@ -102,6 +102,10 @@ void or_render_pixel(const or_scene* s, const sspt_pt_settings* settings, uint32
  * iterations in descending weight (weight 0 / iteration -1 past the kept count). */
 void or_render_outliers(const or_scene* s, const sspt_pt_settings* settings, uint32_t first, uint32_t num, int32_t n,
                         float* weights, int32_t* iterations, int32_t threads);
+/* DebugVisualizer: the eye-light image of iterations first..first+num-1 into the
+ * running-mean W*H*3 image. */
+void or_render_debug(const or_scene* s, uint32_t base_seed, uint32_t first, uint32_t num, float* image,
+                     int32_t threads);
 /* PathTracerBase.ReplayPixel: the path graph (sspt_path_node array, at most
  * `capacity` written) and estimate of pixel (col, row) in `iteration`;
  * returns the node count. */

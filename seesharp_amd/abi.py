@@ -196,6 +196,7 @@ SYMBOLS = {
     "sspt_reset_kernel_timing": (C.c_int, [_vp]),
     "sspt_framebuffer_track_outliers": (C.c_int, [_vp, _vp, C.c_uint32]),
     "sspt_framebuffer_read_outliers": (C.c_int, [_vp, _vp, C.c_uint32, _fp, C.POINTER(C.c_int32)]),
+    "sspt_render_debug": (C.c_int, [_vp, _vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
     "sspt_replay_pixel": (C.c_int, [_vp, _vp, C.c_uint32, C.POINTER(PtSettings), C.c_uint32, C.c_uint32,
                                     C.c_uint32, C.POINTER(PathNode), C.c_uint32, C.POINTER(C.c_uint32), _fp]),
 }

@@ -1342,6 +1342,21 @@ sspt_status sspt_primary_hits(sspt_ctx* ctx, sspt_scene* sc, uint32_t base_seed,
     return SSPT_OK;
 }
 
+sspt_status sspt_render_debug(sspt_ctx* ctx, sspt_scene* sc, uint32_t index, uint32_t base_seed,
+                              uint32_t first_iteration, uint32_t num_iterations) {
+    if (!ctx || !sc) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "null argument");
+    if (index >= sc->recs.size()) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "scene index %u out of range", index);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const SceneRec& R = sc->recs[index];
+    if (num_iterations == 0) return SSPT_OK;
+    HIP_TRY(ctx, hipMemcpyAsync(sc->d_recs + index, &R, sizeof(SceneRec), hipMemcpyHostToDevice, ctx->stream));
+    launch_debug_visualize(ctx->stream, sc->dev, (int)index, R.num_pixels, base_seed, first_iteration,
+                           num_iterations, sc->image + R.fb_off);
+    HIP_TRY(ctx, hipGetLastError());
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return SSPT_OK;
+}
+
 static_assert(sizeof(PathNode) == sizeof(sspt_path_node), "PathNode mirrors sspt_path_node");
 
 sspt_status sspt_replay_pixel(sspt_ctx* ctx, sspt_scene* sc, uint32_t index, const sspt_pt_settings* cfg,

@@ -87,6 +87,11 @@ void launch_trace_occluded(hipStream_t s, const DevScene& sc, int scene, const f
 void launch_primary_rays(hipStream_t s, const DevScene& sc, int scene, uint32_t num_pixels, uint32_t base_seed,
                          uint32_t iteration, float* rays);
 
+// DebugVisualizer: eye-light shading of the primary hits of iterations
+// first..first+num-1 of scene record `scene` into its running-mean image.
+void launch_debug_visualize(hipStream_t s, const DevScene& sc, int scene, uint32_t num_pixels, uint32_t base_seed,
+                            uint32_t first, uint32_t num, float* image);
+
 // One node of a replayed path (the layout of sspt_path_node).
 struct PathNode {
     int kind, ancestor;

@@ -1048,6 +1048,44 @@ __global__ void __launch_bounds__(kTraceBlock) k_replay(DevScene S, int scene, u
     estimate[2] = total.z;
 }
 
+// DebugVisualizer.Render / RenderPixel / ComputeColor
+// (SeeSharp/Integrators/DebugVisualizer.cs:20-59): per pixel and iteration the
+// jittered primary ray's hit shaded by |cos| between the face normal and the
+// direction back to the camera ("eye light"), 0 on a miss, splatted into the
+// framebuffer's running mean. One thread per pixel walks the iterations in order.
+__global__ void __launch_bounds__(kTraceBlock) k_debug_visualize(DevScene S, int scene, uint32_t base_seed,
+                                                                 uint32_t first, uint32_t num, float* image) {
+    __shared__ int stack_node[kStackLds * kTraceBlock];
+    __shared__ float stack_t[kStackLds * kTraceBlock];
+    const SceneRec& R = S.recs[scene];
+    for (uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x; pix < R.num_pixels; pix += gridDim.x * blockDim.x) {
+        const uint32_t row = pix / (uint32_t)R.cam.width, col = pix - row * (uint32_t)R.cam.width;
+        f3 img = mk3(image[3 * pix], image[3 * pix + 1], image[3 * pix + 2]);
+        for (uint32_t it = first; it < first + num; ++it) {
+            Rng rng{hash_seed(base_seed, pix, it)};
+            f2 off = rng.next_float2();
+            f3 o, d;
+            camera_ray(R.cam, (float)col + off.x, (float)row + off.y, &o, &d);
+            HitOut h = traverse<false>(S, R.root, R.abs_max, o, d, 0.0f, INFINITY, stack_node + threadIdx.x,
+                                       stack_t + threadIdx.x);
+            float value = 0.0f;
+            if (h.li >= 0) {
+                const f3 from = -d;
+                const f3 n = surface(S, h.li, h.u, h.v).ng;
+                value = fabsf(dot(n, from));
+                value /= length(n);
+                value /= length(from);
+            }
+            const float cnt = (float)(it + 1);
+            if (it + 1 > 1) img = img * ((cnt - 1.0f) / cnt);
+            img = img + gray(value) / cnt;
+        }
+        image[3 * pix] = img.x;
+        image[3 * pix + 1] = img.y;
+        image[3 * pix + 2] = img.z;
+    }
+}
+
 __global__ void __launch_bounds__(256) k_adjoint_finalize(DevScene S, Wave w) {
     const WaveDesc d = cur_wave(w);
     for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < d.count; slot += gridDim.x * blockDim.x) {
@@ -1388,6 +1426,12 @@ void launch_primary_rays(hipStream_t s, const DevScene& sc, int scene, uint32_t 
                          uint32_t iteration, float* rays) {
     hipLaunchKernelGGL(k_primary_rays, dim3(blocks(num_pixels, 256)), dim3(256), 0, s, sc, scene, num_pixels,
                        base_seed, iteration, rays);
+}
+
+void launch_debug_visualize(hipStream_t s, const DevScene& sc, int scene, uint32_t num_pixels, uint32_t base_seed,
+                            uint32_t first, uint32_t num, float* image) {
+    hipLaunchKernelGGL(k_debug_visualize, dim3(blocks(num_pixels, kTraceBlock)), dim3(kTraceBlock), 0, s, sc, scene,
+                       base_seed, first, num, image);
 }
 
 void launch_replay(hipStream_t s, const DevScene& sc, int scene, uint32_t pix, uint32_t iteration, PathNode* nodes,

@@ -209,6 +209,11 @@ class GpuScene(_TextureParameters):
             self.device.ptr, self.ptr, abi.fptr(rays), rays.shape[0], out.ctypes.data_as(C.POINTER(C.c_uint8))))
         return out
 
+    def render_debug(self, base_seed: int, first_iteration: int, num_iterations: int):
+        """DebugVisualizer iterations into the running-mean framebuffer."""
+        self.device.check(self.device.lib.sspt_render_debug(self.device.ptr, self.ptr, 0, base_seed & 0xFFFFFFFF,
+                                                            first_iteration, num_iterations))
+
     def track_outliers(self, n: int):
         """Outlier tracking of n <= 4 samples per pixel (0 = off), starting empty."""
         self.device.check(self.device.lib.sspt_framebuffer_track_outliers(self.device.ptr, self.ptr, n))
@@ -471,3 +476,32 @@ class GpuPathTracer(Integrator):
         gs = self.prepare(scene)
         nodes, est = gs.replay_pixel(self.settings(), int(col), int(row), int(iteration))
         return PathGraph.from_nodes(nodes), est
+
+
+class DebugVisualizer(Integrator):
+    """A fast grayscale view of a scene (SeeSharp/Integrators/DebugVisualizer.cs:6-59):
+    per pixel and sample the jittered primary hit shaded by |cos| between its
+    face normal and the direction back to the camera ("eye light")."""
+
+    def __init__(self, device: Device | None = None):
+        super().__init__()
+        self.BaseSeed = 0xC030114
+        self.TotalSpp = 1
+        self.device = device
+        self._gpu_scene = None
+
+    def Render(self, scene):
+        if self.device is None:
+            self.device = Device(0)
+        if self._gpu_scene is None or self._gpu_scene.scene is not scene:
+            self._gpu_scene = GpuScene(self.device, scene)
+        gs = self._gpu_scene
+        fb = scene.FrameBuffer
+        gs.clear()
+        t0 = time.perf_counter()
+        gs.render_debug(self.BaseSeed, 0, self.TotalSpp)
+        fb.MetaData["RenderTime"] = (time.perf_counter() - t0) * 1000.0
+        fb.MetaData["NumIterations"] = self.TotalSpp
+        fb.Image = gs.read_image()
+        fb.CurIteration = self.TotalSpp
+        return fb

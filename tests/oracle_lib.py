@@ -58,6 +58,7 @@ _SIGS = {
     "or_set_texture": (C.c_int32, [_vp, C.c_uint32, _fp]),
     "or_render_outliers": (None, [_vp, C.POINTER(abi.PtSettings), C.c_uint32, C.c_uint32, C.c_int32, _fp,
                                   C.POINTER(C.c_int32), C.c_int32]),
+    "or_render_debug": (None, [_vp, C.c_uint32, C.c_uint32, C.c_uint32, _fp, C.c_int32]),
     "or_replay_pixel": (C.c_uint32, [_vp, C.POINTER(abi.PtSettings), C.c_uint32, C.c_uint32, C.c_uint32,
                                      C.POINTER(abi.PathNode), C.c_uint32, _fp]),
     "or_primary_aov": (None, [_vp, C.POINTER(abi.PtSettings), C.c_uint32, C.c_uint32, _fp, _fp, C.c_int32]),
@@ -151,6 +152,11 @@ class OracleScene:
         lib().or_render_outliers(self.ptr, C.byref(cfg), first, num, n, fp(w),
                                  it.ctypes.data_as(C.POINTER(C.c_int32)), threads)
         return w, it
+
+    def render_debug(self, base_seed, first, num, threads=0):
+        img = np.zeros((self.height, self.width, 3), np.float32)
+        lib().or_render_debug(self.ptr, base_seed, first, num, fp(img), threads)
+        return img
 
     def render_pixel(self, cfg, col, row, iteration):
         out = np.zeros(3, np.float32)
