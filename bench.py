@@ -109,7 +109,7 @@ def batch_main(args):
     log(f"[bench] batch world ready in {time.perf_counter() - t0:.1f} s: {info}")
     tuned = None
     if args.autotune:
-        tuned, _ = autotune.tune(dev, lambda: batch.render(cfgs), capacities=(1 << 20, 1 << 21, 1 << 22),
+        tuned, _ = autotune.tune(dev, lambda: batch.render(cfgs), capacities=(1 << 22, 1 << 23, 1 << 24),
                                  log=log)
         log(f"[bench] autotuned: {tuned}")
     batch.render(cfgs)  # warm-up
@@ -212,11 +212,11 @@ def adjoint_main(args):
     dev.set_kernel_timing(True)
     dev.reset_kernel_timing()
     forward()
-    kf = {k: round(dev.kernel_timing(k)[1], 3) for k in ("generate", "extend", "shade", "shadow", "accumulate")}
+    kf = {k: round(dev.kernel_timing(k)[1], 3) for k in ("generate", "extend", "trace", "shade", "shadow", "accumulate")}
     dev.reset_kernel_timing()
     adjoint(dimg)
     ka = {k: round(dev.kernel_timing(k)[1], 3)
-          for k in ("generate", "extend", "shade_adjoint", "shadow_adjoint", "adjoint_finalize")}
+          for k in ("generate", "extend", "trace_adjoint", "shade_adjoint", "shadow_adjoint", "adjoint_finalize")}
     dev.set_kernel_timing(False)
     out = {
         "metric": "adjoint/forward cost ratio (path-replay adjoint of an L2 image loss)",
@@ -478,7 +478,8 @@ def main():
     t0 = time.perf_counter()
     gs.render(cfg, 0, args.spp)
     timed_ms = (time.perf_counter() - t0) * 1000.0
-    kernels = {k: dev.kernel_timing(k) for k in ("generate", "extend", "shade", "shadow", "accumulate")}
+    kernels = {k: dev.kernel_timing(k) for k in ("generate", "extend", "trace", "shade", "shadow", "accumulate")}
+    kernels = {k: v for k, v in kernels.items() if v[0] > 0}
     dev.set_kernel_timing(False)
     log(f"[bench] event-timed render: {timed_ms:.1f} ms")
 
@@ -488,12 +489,21 @@ def main():
     # per render (the event-timed render is one render)
     per = 1.0 / args.steps
     alg = {
-        "extend": BYTES_EXTEND_PER_RAY * n_ext * per,
         "shade": (BYTES_SHADE_PER_RAY * n_ext + BYTES_SHADE_PER_HIT * n_hit) * per,
-        "shadow": BYTES_SHADOW_PER_RAY * n_sh * per,
         "accumulate": BYTES_ACCUMULATE_PER_PATH * npaths,
         "generate": BYTES_GENERATE_PER_PATH * npaths,
     }
+    if "trace" in kernels:
+        # fused schedule: "extend" traces the primary rays, "trace" each later
+        # bounce's continuation rays with the previous bounce's shadow rays, and
+        # "shadow" the last bounce's shadow queue (empty: the last bounce samples
+        # no next events)
+        alg["extend"] = BYTES_EXTEND_PER_RAY * npaths
+        alg["trace"] = BYTES_EXTEND_PER_RAY * (n_ext * per - npaths) + BYTES_SHADOW_PER_RAY * n_sh * per
+        alg["shadow"] = 0.0
+    else:
+        alg["extend"] = BYTES_EXTEND_PER_RAY * n_ext * per
+        alg["shadow"] = BYTES_SHADOW_PER_RAY * n_sh * per
     dom = max(kernels, key=lambda k: kernels[k][1])
     launches, tot_ms = kernels[dom]
     avg_ms = tot_ms / max(1, launches)

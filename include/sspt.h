@@ -214,7 +214,7 @@ sspt_status sspt_framebuffer_read_scene(sspt_ctx* ctx, sspt_scene* scene, uint32
                                         float* host_rgb);
 
 /* ---- execution knobs (the autotuner's search space) ---------------------
- * Wave capacity in path slots (0 = default 4 Mi); grid sizes of the
+ * Wave capacity in path slots (0 = default 16 Mi, the most a wave holds); grid sizes of the
  * traversal (extend, shadow) and shading kernels in blocks: 0 (the default)
  * launches one block per chunk of the longest its queue can be and lets the
  * hardware dispatcher balance the blocks, a non-zero size launches that many
@@ -376,8 +376,11 @@ sspt_status sspt_render_debug(sspt_ctx* ctx, sspt_scene* scene, uint32_t index, 
 /* ---- measurement ------------------------------------------------------- */
 /* Enables HIP-event timing of every kernel launch on the context stream. */
 sspt_status sspt_set_kernel_timing(sspt_ctx* ctx, int32_t enable);
-/* Accumulated launches and milliseconds of kernel `name` since the last reset
- * ("extend", "shadow", "shade", "generate", "accumulate"). */
+/* Accumulated launches and milliseconds of kernel `name` since the last reset:
+ * "generate", "extend" (primary rays), "shade", "trace" (a bounce's
+ * continuation rays with the previous bounce's shadow rays), "shadow" (the
+ * last bounce's shadow rays), "accumulate", "path" (single-kernel tracer),
+ * and "shade_adjoint", "trace_adjoint", "shadow_adjoint", "adjoint_finalize". */
 sspt_status sspt_get_kernel_timing(sspt_ctx* ctx, const char* name, uint64_t* launches,
                                    double* total_ms);
 sspt_status sspt_reset_kernel_timing(sspt_ctx* ctx);

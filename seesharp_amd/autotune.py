@@ -14,7 +14,7 @@ from __future__ import annotations
 import itertools
 import time
 
-WAVE_CAPACITIES = (1 << 18, 1 << 19, 1 << 20, 1 << 21, 1 << 22)
+WAVE_CAPACITIES = (1 << 20, 1 << 21, 1 << 22, 1 << 23, 1 << 24)
 TRACE_GRIDS = (0, 1, 2)
 SHADE_GRIDS = (0, 1, 2)
 

@@ -728,6 +728,12 @@ LaunchConfig launch_config(const sspt_ctx* ctx, uint32_t max_paths, uint32_t max
     return lc;
 }
 
+// Whether a bounce's shadow rays are traced in one launch with the next
+// bounce's continuation rays (1) or in launches of their own (0).
+#ifndef SSPT_FUSE_TRACE
+#define SSPT_FUSE_TRACE 1
+#endif
+
 // Enqueues the kernels of one wave; the wave's work is read on the device from
 // the descriptor the wave counter selects.
 enum class Mode { Mean, Sum, Adjoint };
@@ -758,17 +764,39 @@ void enqueue_wave(sspt_ctx* ctx, sspt_scene* sc, const LaunchConfig& lc, int max
     }
     timed(ctx, "generate", [&] { launch_generate(st, S, w, lc); });
     int buf = 0;
-    for (int depth = 1; depth <= max_depth; ++depth) {
+    if (SSPT_FUSE_TRACE) {
+        // bounce d's shade is followed by one launch tracing the continuation
+        // rays of bounce d + 1 and the shadow rays of bounce d
         launch_bounce_reset(st, w, buf);
         timed(ctx, "extend", [&] { launch_extend(st, S, w, lc, buf); });
-        if (adj) {
-            timed(ctx, "shade_adjoint", [&] { launch_shade_adjoint(st, S, w, lc, buf); });
-            timed(ctx, "shadow_adjoint", [&] { launch_shadow_adjoint(st, S, w, lc); });
-        } else {
-            timed(ctx, "shade", [&] { launch_shade(st, S, w, lc, buf); });
-            timed(ctx, "shadow", [&] { launch_shadow(st, S, w, lc); });
+        for (int depth = 1; depth <= max_depth; ++depth) {
+            if (adj)
+                timed(ctx, "shade_adjoint", [&] { launch_shade_adjoint(st, S, w, lc, buf); });
+            else
+                timed(ctx, "shade", [&] { launch_shade(st, S, w, lc, buf); });
+            if (depth < max_depth) {
+                timed(ctx, adj ? "trace_adjoint" : "trace", [&] { launch_trace(st, S, w, lc, buf ^ 1, adj); });
+                launch_bounce_reset(st, w, buf ^ 1);
+            } else if (adj) {
+                timed(ctx, "shadow_adjoint", [&] { launch_shadow_adjoint(st, S, w, lc); });
+            } else {
+                timed(ctx, "shadow", [&] { launch_shadow(st, S, w, lc); });
+            }
+            buf ^= 1;
         }
-        buf ^= 1;
+    } else {
+        for (int depth = 1; depth <= max_depth; ++depth) {
+            launch_bounce_reset(st, w, buf);
+            timed(ctx, "extend", [&] { launch_extend(st, S, w, lc, buf); });
+            if (adj) {
+                timed(ctx, "shade_adjoint", [&] { launch_shade_adjoint(st, S, w, lc, buf); });
+                timed(ctx, "shadow_adjoint", [&] { launch_shadow_adjoint(st, S, w, lc); });
+            } else {
+                timed(ctx, "shade", [&] { launch_shade(st, S, w, lc, buf); });
+                timed(ctx, "shadow", [&] { launch_shadow(st, S, w, lc); });
+            }
+            buf ^= 1;
+        }
     }
     if (mode == Mode::Mean)
         timed(ctx, "accumulate",
@@ -881,7 +909,7 @@ void apply_settings(SceneRec& R, const sspt_pt_settings& s) {
 }
 
 uint32_t wave_capacity(const sspt_ctx* ctx) {
-    uint32_t c = ctx->want_capacity ? ctx->want_capacity : (1u << 22);
+    uint32_t c = ctx->want_capacity ? ctx->want_capacity : kMaxSlots;
     return std::min(c, kMaxSlots);
 }
 

@@ -73,6 +73,8 @@ void launch_wave_end(hipStream_t s, const Wave& w);
 // adjoint (path replay) variants
 void launch_shade_adjoint(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int buf);
 void launch_shadow_adjoint(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc);
+// extension queue `buf` and the shadow queue in one grid (forward or adjoint shadow)
+void launch_trace(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int buf, bool adjoint);
 void launch_adjoint_finalize(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc);
 void launch_scale(hipStream_t s, float* data, uint64_t n, float factor);
 // Blocks per CU that fit at once (register / LDS limits) for the trace kernels

@@ -408,15 +408,15 @@ __global__ void __launch_bounds__(256) k_generate(DevScene S, Wave w) {
     }
 }
 
-__global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(SSPT_TRACE_WAVES)))
-k_extend(DevScene S, Wave w, int buf) {
-    __shared__ int stack_node[kStackLds * kTraceBlock];
-    __shared__ float stack_t[kStackLds * kTraceBlock];
+// Closest hits of the extension queue `buf`, over blocks [block, ...) of a
+// grid of `nblocks` blocks that stride over the device-side queue length.
+__device__ __forceinline__ void extend_rays(const DevScene& S, const Wave& w, int buf, uint32_t block,
+                                            uint32_t nblocks, int* stack_node, float* stack_t) {
     const uint32_t count = w.counters[buf];
     const float4* __restrict__ q_o = w.q_o[buf];
     const float4* __restrict__ q_d = w.q_d[buf];
     const uint32_t* __restrict__ q_slot = w.q_slot[buf];
-    for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += gridDim.x * blockDim.x) {
+    for (uint32_t base = block * blockDim.x; base < count; base += nblocks * blockDim.x) {
         uint32_t i = base + threadIdx.x;
         bool active = i < count;
         HitOut h{0.0f, 0.0f, 0.0f, -1, 0};
@@ -433,13 +433,13 @@ k_extend(DevScene S, Wave w, int buf) {
     }
 }
 
+// Any-hit tests of the shadow queue; a visible ray adds its contribution to
+// its path's next-event sum (and, with ADJ, its own gradient term).
 template <bool ADJ>
-__global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(SSPT_TRACE_WAVES)))
-k_shadow(DevScene S, Wave w) {
-    __shared__ int stack_node[kStackLds * kTraceBlock];
-    __shared__ float stack_t[kStackLds * kTraceBlock];
+__device__ __forceinline__ void shadow_rays(const DevScene& S, const Wave& w, uint32_t block, uint32_t nblocks,
+                                            int* stack_node, float* stack_t) {
     const uint32_t count = w.counters[kCntShadow];
-    for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += gridDim.x * blockDim.x) {
+    for (uint32_t base = block * blockDim.x; base < count; base += nblocks * blockDim.x) {
         uint32_t i = base + threadIdx.x;
         bool active = i < count;
         bool occluded = false;
@@ -474,6 +474,36 @@ k_shadow(DevScene S, Wave w) {
         wave_count(&w.stats[kStatShadowTraced], active);
         wave_count(&w.stats[kStatOccluded], active && occluded);
     }
+}
+
+__global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(SSPT_TRACE_WAVES)))
+k_extend(DevScene S, Wave w, int buf) {
+    __shared__ int stack_node[kStackLds * kTraceBlock];
+    __shared__ float stack_t[kStackLds * kTraceBlock];
+    extend_rays(S, w, buf, blockIdx.x, gridDim.x, stack_node, stack_t);
+}
+
+template <bool ADJ>
+__global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(SSPT_TRACE_WAVES)))
+k_shadow(DevScene S, Wave w) {
+    __shared__ int stack_node[kStackLds * kTraceBlock];
+    __shared__ float stack_t[kStackLds * kTraceBlock];
+    shadow_rays<ADJ>(S, w, blockIdx.x, gridDim.x, stack_node, stack_t);
+}
+
+// One launch for the two traversals that are ready after a bounce's shade:
+// the continuation rays of the next bounce (blocks [0, extend_blocks)) and
+// the bounce's shadow rays (the remaining blocks). Neither reads what the
+// other writes, and a single grid lets the tail of one overlap the other.
+template <bool ADJ>
+__global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(SSPT_TRACE_WAVES)))
+k_trace(DevScene S, Wave w, int buf, uint32_t extend_blocks) {
+    __shared__ int stack_node[kStackLds * kTraceBlock];
+    __shared__ float stack_t[kStackLds * kTraceBlock];
+    if (blockIdx.x < extend_blocks)
+        extend_rays(S, w, buf, blockIdx.x, extend_blocks, stack_node, stack_t);
+    else
+        shadow_rays<ADJ>(S, w, blockIdx.x - extend_blocks, gridDim.x - extend_blocks, stack_node, stack_t);
 }
 
 // TinyEmbree SpawnRay / IsOccluded offset rule: move along the face normal by
@@ -1380,6 +1410,14 @@ void launch_shadow(hipStream_t s, const DevScene& sc, const Wave& w, const Launc
 void launch_shade_adjoint(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int buf) {
     hipLaunchKernelGGL(k_shade<true>, dim3(grid_of(lc.shade_blocks, lc.max_paths, 256)), dim3(256), 0, s, sc, w,
                        buf);
+}
+void launch_trace(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int buf, bool adjoint) {
+    const uint32_t ne = grid_of(lc.trace_blocks, lc.max_paths, kTraceBlock);
+    const uint32_t ns = grid_of(lc.trace_blocks, lc.max_shadow, kTraceBlock);
+    if (adjoint)
+        hipLaunchKernelGGL(k_trace<true>, dim3(ne + ns), dim3(kTraceBlock), 0, s, sc, w, buf, ne);
+    else
+        hipLaunchKernelGGL(k_trace<false>, dim3(ne + ns), dim3(kTraceBlock), 0, s, sc, w, buf, ne);
 }
 void launch_shadow_adjoint(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc) {
     hipLaunchKernelGGL(k_shadow<true>, dim3(grid_of(lc.trace_blocks, lc.max_shadow, kTraceBlock)), dim3(kTraceBlock),
