@@ -539,6 +539,8 @@ def main():
         "mrays_per_s": round((n_ext + n_sh) / args.steps / (ms / 1000.0) / 1e6, 2),
         "ray_stats_per_step": {k: v // args.steps for k, v in totals.items()},
         "kernel_ms_per_step": {k: round(v[1], 3) for k, v in kernels.items()},
+        "kernel_launches_per_step": {k: int(v[0]) for k, v in kernels.items()},
+        "alg_bytes_per_step": {k: int(alg[k]) for k in kernels if k in alg},
         "event_timed_render_ms": round(timed_ms, 3),
         "kernel_sum_ms_per_step": round(kernel_ms, 3),
         "roofline": {

@@ -629,11 +629,13 @@ sspt_status ensure_wave(sspt_ctx* ctx, uint32_t capacity) {
         TRY(alloc((void**)&w.q_o[b], P * 16));
         TRY(alloc((void**)&w.q_d[b], P * 16));
         TRY(alloc((void**)&w.q_slot[b], P * 4));
+        TRY(alloc((void**)&w.q_rng[b], P * 4));
+        TRY(alloc((void**)&w.q_thr[b], P * 16));
+        TRY(alloc((void**)&w.q_prev[b], P * 16));
+        TRY(alloc((void**)&w.q_L[b], P * 16));
     }
     TRY(alloc((void**)&w.hits, P * 16));
-    TRY(alloc((void**)&w.st_rng, P * 4));
     TRY(alloc((void**)&w.st_pix, P * 4));
-    TRY(alloc((void**)&w.st_thr, P * 16));
     TRY(alloc((void**)&w.st_prev, P * 16));
     TRY(alloc((void**)&w.st_L, P * 16));
     TRY(alloc((void**)&w.st_Le, P * 16));

@@ -11,20 +11,23 @@ namespace sspt {
 // Wavefront buffers, all in HBM. P = path slots of one wave.
 struct Wave {
     uint32_t capacity;     // P
-    // extension queues (double buffered): origin+tmin, direction+tmax, (scene << 24 | slot)
+    // extension queues (double buffered): origin+tmin, direction+tmax, (scene << 24 | slot),
+    // and the live path state travelling with its ray (read and written in queue order)
     float4* q_o[2];
     float4* q_d[2];
     uint32_t* q_slot[2];
+    uint32_t* q_rng[2];  // RNG state
+    float4* q_thr[2];    // prefix weight rgb, previous pdf
+    float4* q_prev[2];   // previous hit position, depth (int bits)
+    float4* q_L[2];      // radiance from emitter / background hits so far
     float4* hits;          // (t, u, v, leaf triangle index bits), per queue entry
     // shadow queue: origin+tmin, direction+tmax, contribution rgb + (scene << 25 | slot << 1 | kind)
     float4* s_o;
     float4* s_d;
     float4* s_c;
-    // path state per slot
-    uint32_t* st_rng;
+    // per slot: written once, when the path ends (st_L, st_prev) or by other kernels
     uint32_t* st_pix;  // (scene << 24 | pixel)
-    float4* st_thr;    // prefix weight rgb, previous pdf
-    float4* st_prev;   // previous hit position, depth (int bits)
+    float4* st_prev;   // .w: the path's last depth (int bits)
     float4* st_L;      // radiance from emitter / background hits
     float4* st_Le;     // environment next-event contributions
     float4* st_La;     // area-light next-event contributions

@@ -395,10 +395,10 @@ __global__ void __launch_bounds__(256) k_generate(DevScene S, Wave w) {
         w.q_d[0][slot] = f4_of(p.dir, INFINITY);
         w.q_slot[0][slot] = slot | ((uint32_t)p.scene << kSlotBits);
         w.st_pix[slot] = p.pix | ((uint32_t)p.scene << kSlotBits);
-        w.st_rng[slot] = p.rng.state;
-        w.st_thr[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-        w.st_prev[slot] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(1));
-        w.st_L[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        w.q_rng[0][slot] = p.rng.state;
+        w.q_thr[0][slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+        w.q_prev[0][slot] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(1));
+        w.q_L[0][slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         w.st_Le[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         w.st_La[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (S.recs[p.scene].enable_denoiser) {
@@ -786,14 +786,14 @@ k_shade(DevScene S, Wave w, int buf) {
         PathVertex<ADJ> v(S, R, slot, sc);
         if (active) {
             v.dir = f3_of(w.q_d[buf][i]);
-            v.rng.state = w.st_rng[slot];
-            float4 thr = w.st_thr[slot];
+            v.rng.state = w.q_rng[buf][i];
+            float4 thr = w.q_thr[buf][i];
             v.prefix = f3_of(thr);
             v.prev_pdf = thr.w;
-            float4 pv = w.st_prev[slot];
+            float4 pv = w.q_prev[buf][i];
             v.depth = __float_as_int(pv.w);
             v.prev_pos = f3_of(pv);
-            v.L = f3_of(w.st_L[slot]);
+            v.L = f3_of(w.q_L[buf][i]);
             if (ADJ) {
                 // the previous vertex's next-event rays have resolved: snapshot A_{depth-1}
                 uint32_t pix = w.st_pix[slot] & kSlotMask;
@@ -851,8 +851,6 @@ k_shade(DevScene S, Wave w, int buf) {
         float4 no, nd;
         const bool cont = v.cont && v.sample_continuation(&no, &nd);
         if (cont) {
-            w.st_thr[slot] = f4_of(v.prefix, v.prev_pdf);
-            w.st_prev[slot] = f4_of(v.prev_pos, __int_as_float(v.depth));
             if (ADJ) {
                 // d log of the sampled BSDF value (the sampling pdf is detached)
                 size_t rec = (size_t)(v.depth - 2) * w.capacity + slot;
@@ -873,10 +871,14 @@ k_shade(DevScene S, Wave w, int buf) {
             w.q_o[nb][n_idx] = no;
             w.q_d[nb][n_idx] = nd;
             w.q_slot[nb][n_idx] = qs;
-        }
-        if (active) {
-            w.st_rng[slot] = v.rng.state;
+            w.q_rng[nb][n_idx] = v.rng.state;
+            w.q_thr[nb][n_idx] = f4_of(v.prefix, v.prev_pdf);
+            w.q_prev[nb][n_idx] = f4_of(v.prev_pos, __int_as_float(v.depth));
+            w.q_L[nb][n_idx] = f4_of(v.L, 0.0f);
+        } else if (active) {
+            // the path ends here: its radiance from hits and its last depth, by slot
             w.st_L[slot] = f4_of(v.L, 0.0f);
+            w.st_prev[slot] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(v.depth));
         }
     }
 }
