@@ -116,6 +116,16 @@ __device__ __forceinline__ bool intersect_tri(f3 o, const RayPre& p, float tmin,
     float det = U + V + W;
     if (det == 0.0f) return false;
     float T = U * (p.sz * az_) + V * (p.sz * bz_) + W * (p.sz * cz_);
+    // Division-free prefilter: t = T / det lies in [tmin, tmax] only if the
+    // sign-adjusted T lies in [tmin, tmax] * |det| widened by 2^-20 relative
+    // (the rounded t below differs from T / det by < 2^-22 relative), so it
+    // rejects only candidates the exact test rejects. A bound that underflows
+    // could reject wrongly: the upper test needs a normal bound, and the lower
+    // one only loosens when its bound underflows.
+    const float ad = fabsf(det), Ts = det < 0.0f ? -T : T;
+    const float lo = tmin * ad, hi = tmax * ad;
+    if (Ts < lo - fabsf(lo) * 9.5367431640625e-07f) return false;
+    if (hi >= 1e-37f && Ts > hi + hi * 9.5367431640625e-07f) return false;
     float rcp = 1.0f / det;
     float t = T * rcp;
     if (!(t >= tmin && t <= tmax)) return false;
@@ -247,14 +257,16 @@ struct Tracer {
         hc[3] = box_test(lox.w, hix.w, loy.w, hiy.w, loz.w, hiz.w, o, p.inv, tmin_box, tmax_box, &tc[3]);
         const int cidx[4] = {child.x, child.y, child.z, child.w};
         const int ccnt[4] = {count.x, count.y, count.z, count.w};
-        uint32_t leaves = 0;  // hit leaf children
-        for (int k = 0; k < 4; ++k) {
-            hc[k] = hc[k] && ccnt[k] >= 0;  // absent slots
-            if (hc[k] && ccnt[k] > 0) {
-                leaves |= 1u << k;
-                hc[k] = false;
-            }
-        }
+        // slot kinds as bit masks (count < 0: absent, 0: inner node, > 0: leaf)
+        const uint32_t hit = (uint32_t)hc[0] | ((uint32_t)hc[1] << 1) | ((uint32_t)hc[2] << 2) | ((uint32_t)hc[3] << 3);
+        const uint32_t leaf = (uint32_t)(count.x > 0) | ((uint32_t)(count.y > 0) << 1) |
+                              ((uint32_t)(count.z > 0) << 2) | ((uint32_t)(count.w > 0) << 3);
+        const uint32_t inner = (uint32_t)(count.x == 0) | ((uint32_t)(count.y == 0) << 1) |
+                               ((uint32_t)(count.z == 0) << 2) | ((uint32_t)(count.w == 0) << 3);
+        uint32_t leaves = hit & leaf;  // hit leaf children
+        const uint32_t inner_hit = hit & inner;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) hc[k] = (inner_hit >> k) & 1u;
         // The triangles of every hit leaf, in one loop per lane: lanes whose
         // leaves sit in different child slots test their triangles side by side.
         int tri = 0, left = 0;
@@ -293,7 +305,7 @@ struct Tracer {
         float t2 = hc[2] ? fminf(tc[2], 3.402823466e38f) : INFINITY;
         float t3 = hc[3] ? fminf(tc[3], 3.402823466e38f) : INFINITY;
         int n0 = cidx[0], n1 = cidx[1], n2 = cidx[2], n3 = cidx[3];
-        int nhit = (int)hc[0] + (int)hc[1] + (int)hc[2] + (int)hc[3];
+        int nhit = __builtin_popcount(inner_hit);
         if (nhit > 0) {
             cswap(t0, n0, t1, n1);
             cswap(t2, n2, t3, n3);
