@@ -643,7 +643,7 @@ sspt_status ensure_wave(sspt_ctx* ctx, uint32_t capacity) {
     TRY(alloc((void**)&w.st_alb, P * 16));
     TRY(alloc((void**)&w.st_nrm, P * 16));
     TRY(alloc((void**)&w.counters, 64));
-    TRY(alloc((void**)&w.stats, 64));
+    TRY(alloc((void**)&w.stats, kStatBytes));
     w.capacity = capacity;
     w.s_o = w.s_d = w.s_c = nullptr;
     ctx->wave_capacity = capacity;
@@ -725,6 +725,7 @@ LaunchConfig launch_config(const sspt_ctx* ctx, uint32_t max_paths, uint32_t max
     lc.trace_blocks = ctx->trace_blocks;
     lc.shade_blocks = ctx->shade_blocks;
     lc.max_paths = max_paths;
+    lc.max_nsr = (int)std::max(1u, max_nsr);
     const uint64_t shadow = (uint64_t)max_paths * 2u * std::max(1u, max_nsr);
     lc.max_shadow = (uint32_t)std::min<uint64_t>(shadow, ctx->shadow_capacity);
     return lc;
@@ -874,9 +875,12 @@ sspt_status run_waves(sspt_ctx* ctx, sspt_scene* sc, const std::vector<WaveDesc>
 }
 
 sspt_status read_stats(sspt_ctx* ctx, sspt_scene* sc, sspt_ray_stats* stats) {
-    unsigned long long hs[kStatNum];
-    HIP_TRY(ctx, hipMemcpyAsync(hs, ctx->wave.stats, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<unsigned long long> striped((size_t)kStatStripes * kStatStride);
+    HIP_TRY(ctx, hipMemcpyAsync(striped.data(), ctx->wave.stats, kStatBytes, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    unsigned long long hs[kStatNum] = {};
+    for (int k = 0; k < kStatStripes; ++k)
+        for (int j = 0; j < kStatNum; ++j) hs[j] += striped[(size_t)k * kStatStride + j];
     sc->nan_count += hs[kStatNan];
     if (stats) {
         stats->num_rays = hs[kStatRays];
@@ -1121,7 +1125,7 @@ sspt_status sspt_render(sspt_ctx* ctx, sspt_scene* sc, const sspt_pt_settings* c
         waves.push_back(WaveDesc{(uint64_t)done * np, it * np, first_iteration + done, it, 0});
         done += it;
     }
-    HIP_TRY(ctx, hipMemsetAsync(ctx->wave.stats, 0, 64, ctx->stream));
+    HIP_TRY(ctx, hipMemsetAsync(ctx->wave.stats, 0, kStatBytes, ctx->stream));
     if (!waves.empty()) {
         TRY(ensure_wave_descs(ctx, waves.size()));
         apply_settings(R, *cfg);
@@ -1183,7 +1187,7 @@ sspt_status sspt_render_batch(sspt_ctx* ctx, sspt_scene* sc, const sspt_pt_setti
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     BatchPlan plan;
     TRY(plan_batch(ctx, sc, cfgs, 0, false, &plan));
-    HIP_TRY(ctx, hipMemsetAsync(ctx->wave.stats, 0, 64, ctx->stream));
+    HIP_TRY(ctx, hipMemsetAsync(ctx->wave.stats, 0, kStatBytes, ctx->stream));
     HIP_TRY(ctx, hipMemsetAsync(sc->image, 0, sc->image_floats * sizeof(float), ctx->stream));
     HIP_TRY(ctx, hipMemsetAsync(sc->layers, 0, 2 * sc->image_floats * sizeof(float), ctx->stream));
     TRY(clear_outliers(ctx, sc));
@@ -1215,7 +1219,7 @@ sspt_status sspt_render_adjoint_batch(sspt_ctx* ctx, sspt_scene* sc, const sspt_
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     BatchPlan plan;
     TRY(plan_batch(ctx, sc, cfgs, first_iteration, true, &plan));
-    HIP_TRY(ctx, hipMemsetAsync(ctx->wave.stats, 0, 64, ctx->stream));
+    HIP_TRY(ctx, hipMemsetAsync(ctx->wave.stats, 0, kStatBytes, ctx->stream));
     HIP_TRY(ctx, hipMemcpyAsync(sc->adj_image, host_dloss_dimages, sc->image_floats * sizeof(float),
                                 hipMemcpyHostToDevice, ctx->stream));
     if (!plan.waves.empty())
@@ -1245,7 +1249,7 @@ sspt_status sspt_render_adjoint(sspt_ctx* ctx, sspt_scene* sc, const sspt_pt_set
         waves.push_back(WaveDesc{(uint64_t)done * np, it * np, first_iteration + done, it, 0});
         done += it;
     }
-    HIP_TRY(ctx, hipMemsetAsync(ctx->wave.stats, 0, 64, ctx->stream));
+    HIP_TRY(ctx, hipMemsetAsync(ctx->wave.stats, 0, kStatBytes, ctx->stream));
     HIP_TRY(ctx, hipMemcpyAsync(sc->adj_image + R.fb_off, host_dloss_dimage, (size_t)np * 12, hipMemcpyHostToDevice,
                                 ctx->stream));
     if (!waves.empty()) {

@@ -41,7 +41,7 @@ struct Wave {
     float4* s_g;       // per shadow ray: gradient of its contribution w.r.t. the vertex's 4 parameters
     int2* s_t;         // per shadow ray: texel offsets of the vertex
     uint32_t* counters;            // kCntNum
-    unsigned long long* stats;     // kStatNum
+    unsigned long long* stats;     // kStatStripes stripes of kStatStride (kStatNum used)
     const WaveDesc* waves;         // per-wave work of the current render
 };
 
@@ -56,6 +56,7 @@ struct LaunchConfig {
     uint32_t shade_blocks;   // blocks of 256 threads for generate / shade / accumulate (0 = full grid)
     uint32_t max_paths;      // largest wave of the render (paths)
     uint32_t max_shadow;     // largest shadow queue of the render (rays)
+    int max_nsr;             // most shadow rays per kind and vertex over the render's scenes
 };
 
 void launch_wave_begin(hipStream_t s, const Wave& w);

@@ -28,22 +28,39 @@
 namespace sspt {
 
 // ------------------------------------------------------------ wave helpers -
-// Appends one entry per lane with `pred` set: one atomic per wave64.
-__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
-    uint64_t mask = __ballot(pred);
-    if (mask == 0) return 0;
-    uint32_t lane = __lane_id();
-    int leader = __ffsll((unsigned long long)mask) - 1;
-    uint32_t base = 0;
-    if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-    base = __shfl(base, leader);
-    return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+// Adds the wave's count of `pred` to statistic `which`, on the stripe of this wave.
+__device__ __forceinline__ uint32_t stat_stripe() {
+    return (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) & (uint32_t)(kStatStripes - 1);
 }
-__device__ __forceinline__ void wave_count(unsigned long long* stat, bool pred) {
+__device__ __forceinline__ void wave_count(unsigned long long* stats, int which, bool pred) {
     uint64_t mask = __ballot(pred);
     if (mask == 0) return;
     int leader = __ffsll((unsigned long long)mask) - 1;
-    if ((int)__lane_id() == leader) atomicAdd(stat, (unsigned long long)__popcll(mask));
+    if ((int)__lane_id() == leader)
+        atomicAdd(&stats[stat_stripe() * kStatStride + which], (unsigned long long)__popcll(mask));
+}
+// Block-wide append: one atomic per block and call. Every thread of the block
+// must make the same calls (block-uniform control flow); `lds` holds one
+// uint32 per wave of the block plus one.
+__device__ __forceinline__ uint32_t block_append(uint32_t* counter, bool pred, uint32_t* lds) {
+    const uint64_t mask = __ballot(pred);
+    const uint32_t wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const uint32_t lane = __lane_id();
+    if (lane == 0) lds[wave] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t total = 0;
+        for (uint32_t k = 0; k < nw; ++k) {
+            uint32_t c = lds[k];
+            lds[k] = total;
+            total += c;
+        }
+        lds[nw] = total ? atomicAdd(counter, total) : 0u;
+    }
+    __syncthreads();
+    const uint32_t r = lds[nw] + lds[wave] + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    __syncthreads();  // the next call reuses lds
+    return r;
 }
 
 // ------------------------------------------------------------------ camera -
@@ -440,8 +457,8 @@ __device__ __forceinline__ void extend_rays(const DevScene& S, const Wave& w, in
             w.hits[i] = h.li >= 0 ? make_float4(h.t, h.u, h.v, __int_as_float(h.li))
                                   : make_float4(INFINITY, 0.0f, 0.0f, __int_as_float(-1));
         }
-        wave_count(&w.stats[kStatRays], active);
-        wave_count(&w.stats[kStatHits], active && h.li >= 0);
+        wave_count(w.stats, kStatRays, active);
+        wave_count(w.stats, kStatHits, active && h.li >= 0);
     }
 }
 
@@ -483,8 +500,8 @@ __device__ __forceinline__ void shadow_rays(const DevScene& S, const Wave& w, ui
                 }
             }
         }
-        wave_count(&w.stats[kStatShadowTraced], active);
-        wave_count(&w.stats[kStatOccluded], active && occluded);
+        wave_count(w.stats, kStatShadowTraced, active);
+        wave_count(w.stats, kStatOccluded, active && occluded);
     }
 }
 
@@ -786,7 +803,8 @@ struct PathVertex {
 // One bounce of EstimateIncidentRadiance for every path of the queue.
 template <bool ADJ>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ADJ ? 1 : SSPT_SHADE_WAVES)))
-k_shade(DevScene S, Wave w, int buf) {
+k_shade(DevScene S, Wave w, int buf, int max_nsr) {
+    __shared__ uint32_t app[256 / 64 + 1];  // block_append scratch
     const uint32_t count = w.counters[buf];
     const int nb = buf ^ 1;
     for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += gridDim.x * blockDim.x) {
@@ -825,17 +843,16 @@ k_shade(DevScene S, Wave w, int buf) {
         }
 
         // Next-event estimation (PathTracer.cs:356-364): per shadow ray,
-        // environment then area light. The loop bound is uniform over the wave
-        // (lanes of scenes with fewer shadow rays simply do not take part).
+        // environment then area light. The loop bound is the render's largest
+        // NumShadowRays, uniform over the block as block_append needs (lanes of
+        // scenes with fewer shadow rays simply do not take part).
         const int nsr_i = R.num_shadow_rays;
-        int max_nsr = nsr_i;
-        for (int off = 32; off > 0; off >>= 1) max_nsr = max(max_nsr, __shfl_xor(max_nsr, off));
         for (int k = 0; k < max_nsr; ++k) {
             const bool nee_k = v.do_nee && k < nsr_i;
             float4 eo, ed, ec, eg = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             const bool env_ray = nee_k && R.has_env && v.env_next_event(&eo, &ed, &ec, &eg);
-            wave_count(&w.stats[kStatShadow], nee_k && R.has_env);
-            uint32_t e_idx = wave_append(&w.counters[kCntShadow], env_ray);
+            wave_count(w.stats, kStatShadow, nee_k && R.has_env);
+            uint32_t e_idx = block_append(&w.counters[kCntShadow], env_ray, app);
             if (env_ray) {
                 w.s_o[e_idx] = eo;
                 w.s_d[e_idx] = ed;
@@ -847,8 +864,8 @@ k_shade(DevScene S, Wave w, int buf) {
             }
             float4 ao, ad, ac, ag = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             const bool area_ray = nee_k && R.num_emitters > 0 && v.area_next_event(&ao, &ad, &ac, &ag);
-            wave_count(&w.stats[kStatShadow], nee_k && R.num_emitters > 0);
-            uint32_t a_idx = wave_append(&w.counters[kCntShadow], area_ray);
+            wave_count(w.stats, kStatShadow, nee_k && R.num_emitters > 0);
+            uint32_t a_idx = block_append(&w.counters[kCntShadow], area_ray, app);
             if (area_ray) {
                 w.s_o[a_idx] = ao;
                 w.s_d[a_idx] = ad;
@@ -878,7 +895,7 @@ k_shade(DevScene S, Wave w, int buf) {
                 }
             }
         }
-        uint32_t n_idx = wave_append(&w.counters[nb], cont);
+        uint32_t n_idx = block_append(&w.counters[nb], cont, app);
         if (cont) {
             w.q_o[nb][n_idx] = no;
             w.q_d[nb][n_idx] = nd;
@@ -918,8 +935,8 @@ __global__ void __launch_bounds__(kTraceBlock) k_path(DevScene S, Wave w) {
         for (;;) {
             HitOut h = traverse<false>(S, R.root, R.abs_max, f3_of(o4), f3_of(d4), o4.w, d4.w,
                                        stack_node + threadIdx.x, stack_t + threadIdx.x);
-            wave_count(&w.stats[kStatRays], true);
-            wave_count(&w.stats[kStatHits], h.li >= 0);
+            wave_count(w.stats, kStatRays, true);
+            wave_count(w.stats, kStatHits, h.li >= 0);
             v.dir = f3_of(d4);
             v.on_hit(h.li >= 0 ? make_float4(h.t, h.u, h.v, __int_as_float(h.li))
                                : make_float4(INFINITY, 0.0f, 0.0f, __int_as_float(-1)));
@@ -930,24 +947,24 @@ __global__ void __launch_bounds__(kTraceBlock) k_path(DevScene S, Wave w) {
             for (int k = 0; v.do_nee && k < R.num_shadow_rays; ++k) {
                 float4 so, sd, c, g;
                 if (R.has_env) {
-                    wave_count(&w.stats[kStatShadow], true);
+                    wave_count(w.stats, kStatShadow, true);
                     if (v.env_next_event(&so, &sd, &c, &g)) {
                         bool occ = traverse<true>(S, R.root, R.abs_max, f3_of(so), f3_of(sd), so.w, sd.w,
                                                   stack_node + threadIdx.x, stack_t + threadIdx.x)
                                        .li >= 0;
-                        wave_count(&w.stats[kStatShadowTraced], true);
-                        wave_count(&w.stats[kStatOccluded], occ);
+                        wave_count(w.stats, kStatShadowTraced, true);
+                        wave_count(w.stats, kStatOccluded, occ);
                         if (!occ) Le = Le + f3_of(c);
                     }
                 }
                 if (R.num_emitters > 0) {
-                    wave_count(&w.stats[kStatShadow], true);
+                    wave_count(w.stats, kStatShadow, true);
                     if (v.area_next_event(&so, &sd, &c, &g)) {
                         bool occ = traverse<true>(S, R.root, R.abs_max, f3_of(so), f3_of(sd), so.w, sd.w,
                                                   stack_node + threadIdx.x, stack_t + threadIdx.x)
                                        .li >= 0;
-                        wave_count(&w.stats[kStatShadowTraced], true);
-                        wave_count(&w.stats[kStatOccluded], occ);
+                        wave_count(w.stats, kStatShadowTraced, true);
+                        wave_count(w.stats, kStatOccluded, occ);
                         if (!occ) La = La + f3_of(c);
                     }
                 }
@@ -1285,7 +1302,7 @@ __global__ void __launch_bounds__(256) k_accumulate_mean(Wave w, float* image, u
             normal[3 * pix + 1] = nrm.y;
             normal[3 * pix + 2] = nrm.z;
         }
-        if (nans) atomicAdd(&w.stats[kStatNan], (unsigned long long)nans);
+        if (nans) atomicAdd(&w.stats[(blockIdx.x & (kStatStripes - 1)) * kStatStride + kStatNan], (unsigned long long)nans);
     }
 }
 
@@ -1331,7 +1348,7 @@ __global__ void __launch_bounds__(256) k_accumulate_sum(DevScene S, Wave w, floa
             atomicAdd(nm + 2, nv.z);
         }
         if (!isfinite(average(L))) {
-            atomicAdd(&w.stats[kStatNan], 1ull);
+            atomicAdd(&w.stats[stat_stripe() * kStatStride + kStatNan], 1ull);
             continue;
         }
         float* px = pool + R.fb_off + 3ull * (tag & kSlotMask);
@@ -1415,7 +1432,7 @@ void launch_extend(hipStream_t s, const DevScene& sc, const Wave& w, const Launc
 }
 void launch_shade(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int buf) {
     hipLaunchKernelGGL(k_shade<false>, dim3(grid_of(lc.shade_blocks, lc.max_paths, 256)), dim3(256), 0, s, sc, w,
-                       buf);
+                       buf, lc.max_nsr);
 }
 void launch_shadow(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc) {
     hipLaunchKernelGGL(k_shadow<false>, dim3(grid_of(lc.trace_blocks, lc.max_shadow, kTraceBlock)), dim3(kTraceBlock),
@@ -1423,7 +1440,7 @@ void launch_shadow(hipStream_t s, const DevScene& sc, const Wave& w, const Launc
 }
 void launch_shade_adjoint(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int buf) {
     hipLaunchKernelGGL(k_shade<true>, dim3(grid_of(lc.shade_blocks, lc.max_paths, 256)), dim3(256), 0, s, sc, w,
-                       buf);
+                       buf, lc.max_nsr);
 }
 void launch_trace(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int buf, bool adjoint) {
     const uint32_t ne = grid_of(lc.trace_blocks, lc.max_paths, kTraceBlock);

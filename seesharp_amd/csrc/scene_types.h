@@ -160,5 +160,12 @@ enum Stat {
     kStatShadowTraced = 5,  // shadow rays traced (non-zero contribution)
     kStatNum = 6
 };
+// Ray statistics are striped over kStatStripes 128-byte lines: the waves of a
+// launch add to different addresses (device-wide atomics on one address
+// serialise across the XCDs), and a total is the sum over the stripes.
+constexpr int kStatStripes = 256;
+constexpr int kStatStride = 16;  // u64 per stripe (128 B)
+constexpr unsigned long kStatBytes = 8ul * kStatStripes * kStatStride;
+
 
 }  // namespace sspt
