@@ -17,7 +17,7 @@ namespace {
 
 // SAH bins per axis and split.
 #ifndef SSPT_SAH_BINS
-#define SSPT_SAH_BINS 16
+#define SSPT_SAH_BINS 64
 #endif
 constexpr int kSahBins = SSPT_SAH_BINS;
 

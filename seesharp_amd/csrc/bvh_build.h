@@ -24,7 +24,7 @@ struct BvhOutput {
 
 // Triangles per leaf at most (the builder splits every larger range).
 #ifndef SSPT_MAX_LEAF
-#define SSPT_MAX_LEAF 2
+#define SSPT_MAX_LEAF 1
 #endif
 constexpr int kMaxLeafTriangles = SSPT_MAX_LEAF;
 
