@@ -392,6 +392,26 @@ __device__ __forceinline__ int scene_of(const DevScene& S, uint64_t g) {
     return lo;
 }
 
+// Pixel of the k-th path of an iteration. The rows are walked in bands of
+// kTileRows rows and each band column by column, so the 64 consecutive paths
+// of a wavefront cover an 8 x 8 tile of the image: their camera rays, and the
+// rays of the first bounces that keep their queue order, share more BVH nodes
+// than a 64 x 1 strip of one row does. A bijection onto the W x H pixels for
+// any W, H (the last band is H mod 8 rows high). The order only decides which
+// slot a pixel's path occupies: every path keeps its own RNG stream.
+#ifndef SSPT_TILE_ROWS
+#define SSPT_TILE_ROWS 8
+#endif
+constexpr uint32_t kTileRows = SSPT_TILE_ROWS;
+__device__ __forceinline__ uint32_t path_pixel(uint32_t k, uint32_t W, uint32_t H) {
+    if (kTileRows <= 1) return k;
+    const uint32_t band = k / (kTileRows * W);
+    const uint32_t r = k - band * kTileRows * W;
+    const uint32_t bh = min(kTileRows, H - band * kTileRows);
+    const uint32_t col = r / bh;
+    return (band * kTileRows + (r - col * bh)) * W + col;
+}
+
 // RNG(BaseSeed, row * W + col, iteration) (PathTracer.cs:236-237), pixel jitter
 // (:289-290) and camera ray (:291) of world-wide path g.
 struct PathStart {
@@ -406,7 +426,7 @@ __device__ __forceinline__ PathStart start_path(const DevScene& S, uint64_t g) {
     const SceneRec& R = S.recs[p.scene];
     uint64_t k = g - R.path_off;
     uint32_t it = (uint32_t)(k / R.num_pixels);
-    p.pix = (uint32_t)(k - (uint64_t)it * R.num_pixels);
+    p.pix = path_pixel((uint32_t)(k - (uint64_t)it * R.num_pixels), (uint32_t)R.cam.width, (uint32_t)R.cam.height);
     uint32_t row = p.pix / (uint32_t)R.cam.width;
     uint32_t col = p.pix - row * (uint32_t)R.cam.width;
     p.rng = Rng{hash_seed(R.base_seed, p.pix, R.first_iter + it)};
@@ -1259,7 +1279,11 @@ __global__ void __launch_bounds__(256) k_accumulate_mean(Wave w, float* image, u
                                                           float* normal, unsigned long long* outliers,
                                                           int num_outliers) {
     const WaveDesc d = cur_wave(w);
-    for (uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x; pix < num_pixels; pix += gridDim.x * blockDim.x) {
+    // k: a path's index within its iteration; slot k of the wave's first
+    // iteration records which pixel that is, and every iteration of the wave
+    // lays its pixels out in the same order
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < num_pixels; k += gridDim.x * blockDim.x) {
+        const uint32_t pix = w.st_pix[k] & kSlotMask;
         f3 img = mk3(image[3 * pix], image[3 * pix + 1], image[3 * pix + 2]);
         f3 alb = gray(0.0f), nrm = gray(0.0f);
         if (albedo) {
@@ -1271,7 +1295,7 @@ __global__ void __launch_bounds__(256) k_accumulate_mean(Wave w, float* image, u
         if (ok) heap.load(ok, num_outliers);
         uint32_t nans = 0;
         for (uint32_t s = 0; s < d.iters; ++s) {
-            uint32_t slot = s * num_pixels + pix;
+            uint32_t slot = s * num_pixels + k;
             f3 L = f3_of(w.st_L[slot]) + f3_of(w.st_Le[slot]) + f3_of(w.st_La[slot]);
             float n = (float)(d.first_iter + s + 1);
             if (d.first_iter + s + 1 > 1) img = img * ((n - 1.0f) / n);
