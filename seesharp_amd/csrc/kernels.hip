@@ -228,10 +228,21 @@ __device__ __forceinline__ void cswap(float& ta, int& na, float& tb, int& nb) {
     na = n;
 }
 
-// 4-wide BVH traversal of one ray, advanced one node at a time. Closest hit
-// visits hit children nearest first and skips popped nodes whose entry
-// distance is beyond the best hit; ANY = true stops at the first hit (shadow
-// rays). `traverse` runs a ray to its end.
+// 4-wide BVH traversal of one ray. Closest hit visits hit children nearest
+// first and skips popped nodes whose entry distance is beyond the best hit;
+// ANY = true stops at the first hit (shadow rays).
+//
+// The triangles of hit leaves are not tested in the node step that finds
+// them: the leaf goes to a per-lane FIFO of kPendCap entries and the lane
+// keeps traversing. `traverse` runs triangle steps (one queued leaf per lane)
+// when most of the wave's live lanes have a leaf queued, when a lane's FIFO is
+// full, or when no lane has nodes left. A wave then spends its triangle tests
+// on many lanes at once instead of one node step's few leaves (the SIMD cost
+// of a traversal is its node steps plus its triangle steps, each paid by the
+// whole wave). The result is the same: a leaf is queued only while its box
+// meets [tmin, best], the closest hit is decided by distance and then by the
+// lower global id whatever the test order, and an any-hit query only asks
+// whether some triangle is hit.
 template <bool ANY>
 struct Tracer {
     f3 o;
@@ -239,8 +250,11 @@ struct Tracer {
     float tmin, slack, tmin_box;
     HitOut best;
     TraceStack st;
-    int node;
-    bool done;
+    int* pend;      // FIFO of queued leaves in LDS (+ threadIdx.x, stride kTraceBlock)
+    uint32_t ph;    // FIFO head
+    uint32_t pn;    // FIFO length
+    int node;       // next node to visit, -1 when none is left
+    bool done;      // any-hit found (or an invalid node index)
 
     __device__ __forceinline__ void start(int root, float abs_max, f3 o_, f3 d, float tmin_, float tmax) {
         o = o_;
@@ -250,13 +264,44 @@ struct Tracer {
         tmin_box = tmin_ - slack;
         best = HitOut{tmax, 0.0f, 0.0f, -1, 0x7fffffff};
         st.sp = 0;
+        ph = 0;
+        pn = 0;
         node = root;
         done = false;
     }
 
-    // Tests the current node's children, intersects the triangles of hit
-    // leaves, then moves to the nearest hit inner child or pops the stack.
-    __device__ __forceinline__ void step(const DevScene& S) {
+    // Lanes still working: nodes to visit or leaves queued.
+    __device__ __forceinline__ bool live() const { return !done && (node >= 0 || pn > 0); }
+    // A node step may queue four leaves.
+    __device__ __forceinline__ bool can_step() const { return !done && node >= 0 && pn <= (uint32_t)(kPendCap - 4); }
+
+    // Triangles tri .. tri + n - 1 of one leaf.
+    __device__ __forceinline__ void test_leaf(const DevScene& S, int tri, int n) {
+        for (int j = 0; j < n; ++j, ++tri) {
+            float4 a = S.tri_pos[3 * tri], b = S.tri_pos[3 * tri + 1], c = S.tri_pos[3 * tri + 2];
+            float t, u, v;
+            if (intersect_tri(o, p, tmin, best.t, f3_of(a), f3_of(b), f3_of(c), &t, &u, &v)) {
+                int gid = __float_as_int(a.w);
+                if (ANY || t < best.t || gid < best.gid) {
+                    best.t = t;
+                    best.u = u;
+                    best.v = v;
+                    best.li = tri;
+                    best.gid = gid;
+                    if (ANY) {
+                        done = true;
+                        return;
+                    }
+                }
+            }
+        }
+    }
+
+    // Box tests of the current node's children: hit leaves are queued (a leaf
+    // of more than four triangles, which only the build's depth bound makes,
+    // is tested here), hit inner children are pushed nearest first and the
+    // nearest becomes the next node, else the stack is popped.
+    __device__ __forceinline__ void node_step(const DevScene& S) {
         if ((unsigned)node >= (unsigned)S.num_nodes) {  // never taken for a valid tree
             done = true;
             return;
@@ -284,35 +329,19 @@ struct Tracer {
         const uint32_t inner_hit = hit & inner;
 #pragma unroll
         for (int k = 0; k < 4; ++k) hc[k] = (inner_hit >> k) & 1u;
-        // The triangles of every hit leaf, in one loop per lane: lanes whose
-        // leaves sit in different child slots test their triangles side by side.
-        int tri = 0, left = 0;
-        for (;;) {
-            if (left == 0) {
-                if (leaves == 0) break;
-                const int k = __builtin_ctz(leaves);
-                leaves &= leaves - 1u;
-                tri = k == 0 ? cidx[0] : k == 1 ? cidx[1] : k == 2 ? cidx[2] : cidx[3];
-                left = k == 0 ? ccnt[0] : k == 1 ? ccnt[1] : k == 2 ? ccnt[2] : ccnt[3];
+        while (leaves) {
+            const int k = __builtin_ctz(leaves);
+            leaves &= leaves - 1u;
+            const int tri = k == 0 ? cidx[0] : k == 1 ? cidx[1] : k == 2 ? cidx[2] : cidx[3];
+            const int n = k == 0 ? ccnt[0] : k == 1 ? ccnt[1] : k == 2 ? ccnt[2] : ccnt[3];
+            if (n <= 4) {
+                // entry: first triangle (< 2^30) and count - 1
+                pend[((ph + pn) & (uint32_t)(kPendCap - 1)) * kTraceBlock] = tri | ((n - 1) << 30);
+                ++pn;
+            } else {
+                test_leaf(S, tri, n);
+                if (ANY && done) return;
             }
-            float4 a = S.tri_pos[3 * tri], b = S.tri_pos[3 * tri + 1], c = S.tri_pos[3 * tri + 2];
-            float t, u, v;
-            if (intersect_tri(o, p, tmin, best.t, f3_of(a), f3_of(b), f3_of(c), &t, &u, &v)) {
-                int gid = __float_as_int(a.w);
-                if (ANY || t < best.t || gid < best.gid) {
-                    best.t = t;
-                    best.u = u;
-                    best.v = v;
-                    best.li = tri;
-                    best.gid = gid;
-                    if (ANY) {
-                        done = true;
-                        return;
-                    }
-                }
-            }
-            ++tri;
-            --left;
         }
         // inner children, nearest first. A hit child's key is clamped to FLT_MAX
         // (its entry distance is +inf when the ray runs parallel to a slab and no
@@ -341,10 +370,20 @@ struct Tracer {
             st.pop(&node, &tn);
             if (ANY || tn <= best.t + slack) return;
         }
-        done = true;
+        node = -1;
+    }
+
+    // Tests the oldest queued leaf.
+    __device__ __forceinline__ void tri_step(const DevScene& S) {
+        const uint32_t e = (uint32_t)pend[(ph & (uint32_t)(kPendCap - 1)) * kTraceBlock];
+        ++ph;
+        --pn;
+        test_leaf(S, (int)(e & 0x3fffffffu), (int)(e >> 30) + 1);
     }
 };
 
+// Runs one ray per lane to its end. The active lanes of a wave call it
+// together: the choice between node and triangle steps is made per wave.
 template <bool ANY>
 __device__ __forceinline__ HitOut traverse(const DevScene& S, int root, float abs_max, f3 o, f3 d, float tmin,
                                            float tmax, int* lds_node, float* lds_t) {
@@ -355,8 +394,23 @@ __device__ __forceinline__ HitOut traverse(const DevScene& S, int root, float ab
     tr.st.lds_t = lds_t;
     tr.st.spill_node = spill_node;
     tr.st.spill_t = spill_t;
+    tr.pend = lds_node + kStackLds * kTraceBlock;
     tr.start(root, abs_max, o, d, tmin, tmax);
-    while (!tr.done) tr.step(S);
+    for (;;) {
+        if (tr.can_step()) tr.node_step(S);
+        const bool live = tr.live();
+        const uint64_t act = __ballot(live);
+        if (act == 0) break;
+        const uint64_t queued = __ballot(live && tr.pn > 0);
+        const uint64_t walking = __ballot(live && tr.node >= 0);
+        const uint64_t can = __ballot(tr.can_step());
+        // triangle step: kPendPercent % of the live lanes have a leaf queued,
+        // a lane cannot step for a full FIFO, or no lane has a node left
+        if (queued != 0 &&
+            (can == 0 || walking != can || 100 * __popcll(queued) >= kPendPercent * __popcll(act))) {
+            if (live && tr.pn > 0) tr.tri_step(S);
+        }
+    }
     return tr.best;
 }
 
@@ -527,7 +581,7 @@ __device__ __forceinline__ void shadow_rays(const DevScene& S, const Wave& w, ui
 
 __global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(SSPT_TRACE_WAVES)))
 k_extend(DevScene S, Wave w, int buf) {
-    __shared__ int stack_node[kStackLds * kTraceBlock];
+    __shared__ int stack_node[kStackNodeLds];
     __shared__ float stack_t[kStackLds * kTraceBlock];
     extend_rays(S, w, buf, blockIdx.x, gridDim.x, stack_node, stack_t);
 }
@@ -535,7 +589,7 @@ k_extend(DevScene S, Wave w, int buf) {
 template <bool ADJ>
 __global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(SSPT_TRACE_WAVES)))
 k_shadow(DevScene S, Wave w) {
-    __shared__ int stack_node[kStackLds * kTraceBlock];
+    __shared__ int stack_node[kStackNodeLds];
     __shared__ float stack_t[kStackLds * kTraceBlock];
     shadow_rays<ADJ>(S, w, blockIdx.x, gridDim.x, stack_node, stack_t);
 }
@@ -547,7 +601,7 @@ k_shadow(DevScene S, Wave w) {
 template <bool ADJ>
 __global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(SSPT_TRACE_WAVES)))
 k_trace(DevScene S, Wave w, int buf, uint32_t extend_blocks) {
-    __shared__ int stack_node[kStackLds * kTraceBlock];
+    __shared__ int stack_node[kStackNodeLds];
     __shared__ float stack_t[kStackLds * kTraceBlock];
     if (blockIdx.x < extend_blocks)
         extend_rays(S, w, buf, blockIdx.x, extend_blocks, stack_node, stack_t);
@@ -939,7 +993,7 @@ k_shade(DevScene S, Wave w, int buf, int max_nsr) {
 // baseline the wavefront is measured against. Writes the path's radiance sums
 // to the slot for the accumulate kernels.
 __global__ void __launch_bounds__(kTraceBlock) k_path(DevScene S, Wave w) {
-    __shared__ int stack_node[kStackLds * kTraceBlock];
+    __shared__ int stack_node[kStackNodeLds];
     __shared__ float stack_t[kStackLds * kTraceBlock];
     const WaveDesc d = cur_wave(w);
     for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < d.count; slot += gridDim.x * blockDim.x) {
@@ -1029,7 +1083,7 @@ __device__ __forceinline__ void put3(float* dst, f3 v) {
 __global__ void __launch_bounds__(kTraceBlock) k_replay(DevScene S, int scene, uint32_t pix, uint32_t iteration,
                                                         PathNode* nodes, uint32_t cap, uint32_t* count,
                                                         float* estimate) {
-    __shared__ int stack_node[kStackLds * kTraceBlock];
+    __shared__ int stack_node[kStackNodeLds];
     __shared__ float stack_t[kStackLds * kTraceBlock];
     if (threadIdx.x != 0) return;
     const SceneRec& R = S.recs[scene];
@@ -1136,7 +1190,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_replay(DevScene S, int scene, u
 // framebuffer's running mean. One thread per pixel walks the iterations in order.
 __global__ void __launch_bounds__(kTraceBlock) k_debug_visualize(DevScene S, int scene, uint32_t base_seed,
                                                                  uint32_t first, uint32_t num, float* image) {
-    __shared__ int stack_node[kStackLds * kTraceBlock];
+    __shared__ int stack_node[kStackNodeLds];
     __shared__ float stack_t[kStackLds * kTraceBlock];
     const SceneRec& R = S.recs[scene];
     for (uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x; pix < R.num_pixels; pix += gridDim.x * blockDim.x) {
@@ -1389,7 +1443,7 @@ __global__ void k_scale(float* data, uint64_t n, float factor) {
 
 __global__ void __launch_bounds__(kTraceBlock) k_trace_closest(DevScene S, int scene, const float* rays, uint64_t n,
                                                                float4* out) {
-    __shared__ int stack_node[kStackLds * kTraceBlock];
+    __shared__ int stack_node[kStackNodeLds];
     __shared__ float stack_t[kStackLds * kTraceBlock];
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -1403,7 +1457,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_closest(DevScene S, int s
 
 __global__ void __launch_bounds__(kTraceBlock) k_trace_occluded(DevScene S, int scene, const float* rays,
                                                                 uint64_t n, uint8_t* out) {
-    __shared__ int stack_node[kStackLds * kTraceBlock];
+    __shared__ int stack_node[kStackNodeLds];
     __shared__ float stack_t[kStackLds * kTraceBlock];
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;

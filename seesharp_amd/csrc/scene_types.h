@@ -34,9 +34,26 @@ constexpr int kMaxBvhDepth = 40;
 // Traversal stack per lane: the top kStackLds entries live in LDS, the rest in
 // a private (scratch) array; scene creation rejects trees that need more than
 // kStackTotal entries.
-constexpr int kStackLds = 12;
+#ifndef SSPT_STACK_LDS
+#define SSPT_STACK_LDS 8
+#endif
+constexpr int kStackLds = SSPT_STACK_LDS;
 constexpr int kStackTotal = 64;
 constexpr int kTraceBlock = 128;
+// Leaves a lane may hold back for a later triangle phase (a power of two, at
+// least 4: one node step adds up to four). They sit in LDS after the lane's
+// stack entries, so a traversal kernel's node array holds kStackNodeLds ints.
+#ifndef SSPT_PEND_CAP
+#define SSPT_PEND_CAP 8
+#endif
+constexpr int kPendCap = SSPT_PEND_CAP;
+static_assert(kPendCap >= 4 && (kPendCap & (kPendCap - 1)) == 0, "kPendCap: a power of two >= 4");
+// A wave runs a triangle step once this percentage of its live lanes have a leaf queued.
+#ifndef SSPT_PEND_PERCENT
+#define SSPT_PEND_PERCENT 80
+#endif
+constexpr int kPendPercent = SSPT_PEND_PERCENT;
+constexpr int kStackNodeLds = (kStackLds + kPendCap) * kTraceBlock;
 
 // Queue entries pack (scene, slot): slot in the low 24 bits, scene above.
 constexpr int kSlotBits = 24;
