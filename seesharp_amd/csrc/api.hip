@@ -548,6 +548,9 @@ sspt_status create_world(sspt_ctx* ctx, const sspt_scene_desc* descs, uint32_t n
     // texel offsets travel as 32-bit ints through the shading and adjoint kernels
     if (W.texels.size() > (size_t)INT32_MAX)
         return fail(ctx, SSPT_ERR_UNSUPPORTED, "texture data exceeds 2^31 floats");
+    // the traversal addresses nodes by 32-bit byte offsets (128 B per node)
+    if (W.nodes.size() / 32 > (size_t)kMaxNodes)
+        return fail(ctx, SSPT_ERR_UNSUPPORTED, "BVH exceeds 2^25 nodes");
     std::unique_ptr<sspt_scene> sc(new sspt_scene());
     sc->ctx = ctx;
     DevScene& S = sc->dev;

@@ -154,14 +154,17 @@ __device__ __forceinline__ bool intersect_tri(f3 o, const RayPre& p, float tmin,
 
 // Robust slab test (Ize, JCGT 2013): far distances widened by 1 + 2 gamma(3).
 // The caller widens the query interval by the ray's slack (see traverse).
-__device__ __forceinline__ bool box_test(float lox, float hix, float loy, float hiy, float loz, float hiz,
-                                         f3 o, f3 inv, float tmin, float tmax, float* tnear) {
-    float x0 = (lox - o.x) * inv.x, x1 = (hix - o.x) * inv.x;
-    float y0 = (loy - o.y) * inv.y, y1 = (hiy - o.y) * inv.y;
-    float z0 = (loz - o.z) * inv.z, z1 = (hiz - o.z) * inv.z;
-    float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
+// nx / fx: the child's near and far x plane for this ray (lo if the direction's
+// x is positive, else hi; the sign bit decides, so -0 counts as negative). As
+// rounding is monotonic, (near - o) * inv is the smaller of the two slab
+// distances and (far - o) * inv the larger, the values min / max would pick;
+// the one exception is a zero direction component with the origin on a plane
+// (0 * inf = NaN), where fmaxf / fminf drop the NaN and the box is kept.
+__device__ __forceinline__ bool box_test(float nx, float fx, float ny, float fy, float nz, float fz, f3 o, f3 inv,
+                                         float tmin, float tmax, float* tnear) {
+    float tn = fmaxf(fmaxf((nx - o.x) * inv.x, (ny - o.y) * inv.y), fmaxf((nz - o.z) * inv.z, tmin));
     // scaling the minimum equals the minimum of the scaled values (rounding is monotonic)
-    float tf = fminf(fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1)) * 1.0000004f, tmax);
+    float tf = fminf(fminf(fminf((fx - o.x) * inv.x, (fy - o.y) * inv.y), (fz - o.z) * inv.z) * 1.0000004f, tmax);
     *tnear = tn;
     return tn <= tf;
 }
@@ -248,9 +251,10 @@ struct Tracer {
     f3 o;
     RayPre p;
     float tmin, slack, tmin_box;
+    uint32_t near_x, near_y, near_z;  // byte offsets of the near bound vectors in a node
     HitOut best;
     TraceStack st;
-    int* pend;      // FIFO of queued leaves in LDS (+ threadIdx.x, stride kTraceBlock)
+    int* pend;     // FIFO of queued leaves in LDS (+ threadIdx.x, stride kTraceBlock)
     uint32_t ph;    // FIFO head
     uint32_t pn;    // FIFO length
     int node;       // next node to visit, -1 when none is left
@@ -262,6 +266,9 @@ struct Tracer {
         slack = ray_slack(abs_max, o_, d);
         tmin = tmin_;
         tmin_box = tmin_ - slack;
+        near_x = signbit(p.inv.x) ? 16u : 0u;
+        near_y = signbit(p.inv.y) ? 48u : 32u;
+        near_z = signbit(p.inv.z) ? 80u : 64u;
         best = HitOut{tmax, 0.0f, 0.0f, -1, 0x7fffffff};
         st.sp = 0;
         ph = 0;
@@ -306,17 +313,23 @@ struct Tracer {
             done = true;
             return;
         }
-        const float4* N = S.nodes + 8 * node;
-        float4 lox = N[0], hix = N[1], loy = N[2], hiy = N[3], loz = N[4], hiz = N[5];
-        int4 child = *reinterpret_cast<const int4*>(N + 6);
-        int4 count = *reinterpret_cast<const int4*>(N + 7);
+        // the near / far bound vectors of each axis at the ray's byte offsets
+        // (far = near ^ 16); scene creation bounds node * 128 below 2^32
+        const char* B = reinterpret_cast<const char*>(S.nodes);
+        const uint32_t nb = (uint32_t)node << 7;
+        const uint32_t ax = nb + near_x, ay = nb + near_y, az = nb + near_z;
+        const float4 nx = *reinterpret_cast<const float4*>(B + ax), fx = *reinterpret_cast<const float4*>(B + (ax ^ 16u));
+        const float4 ny = *reinterpret_cast<const float4*>(B + ay), fy = *reinterpret_cast<const float4*>(B + (ay ^ 16u));
+        const float4 nz = *reinterpret_cast<const float4*>(B + az), fz = *reinterpret_cast<const float4*>(B + (az ^ 16u));
+        int4 child = *reinterpret_cast<const int4*>(B + nb + 96u);
+        int4 count = *reinterpret_cast<const int4*>(B + nb + 112u);
         const float tmax_box = best.t + slack;
         float tc[4];
         bool hc[4];
-        hc[0] = box_test(lox.x, hix.x, loy.x, hiy.x, loz.x, hiz.x, o, p.inv, tmin_box, tmax_box, &tc[0]);
-        hc[1] = box_test(lox.y, hix.y, loy.y, hiy.y, loz.y, hiz.y, o, p.inv, tmin_box, tmax_box, &tc[1]);
-        hc[2] = box_test(lox.z, hix.z, loy.z, hiy.z, loz.z, hiz.z, o, p.inv, tmin_box, tmax_box, &tc[2]);
-        hc[3] = box_test(lox.w, hix.w, loy.w, hiy.w, loz.w, hiz.w, o, p.inv, tmin_box, tmax_box, &tc[3]);
+        hc[0] = box_test(nx.x, fx.x, ny.x, fy.x, nz.x, fz.x, o, p.inv, tmin_box, tmax_box, &tc[0]);
+        hc[1] = box_test(nx.y, fx.y, ny.y, fy.y, nz.y, fz.y, o, p.inv, tmin_box, tmax_box, &tc[1]);
+        hc[2] = box_test(nx.z, fx.z, ny.z, fy.z, nz.z, fz.z, o, p.inv, tmin_box, tmax_box, &tc[2]);
+        hc[3] = box_test(nx.w, fx.w, ny.w, fy.w, nz.w, fz.w, o, p.inv, tmin_box, tmax_box, &tc[3]);
         const int cidx[4] = {child.x, child.y, child.z, child.w};
         const int ccnt[4] = {count.x, count.y, count.z, count.w};
         // slot kinds as bit masks (count < 0: absent, 0: inner node, > 0: leaf)

@@ -31,6 +31,8 @@ constexpr int kInfoHasUvs = 1 << 25;
 
 // Depth bound of the binary SAH build.
 constexpr int kMaxBvhDepth = 40;
+// Nodes of one world: the traversal forms 32-bit byte offsets into the node array.
+constexpr uint32_t kMaxNodes = 1u << 25;
 // Traversal stack per lane: the top kStackLds entries live in LDS, the rest in
 // a private (scratch) array; scene creation rejects trees that need more than
 // kStackTotal entries.
