@@ -24,6 +24,9 @@
 #ifndef SSPT_SHADE_WAVES
 #define SSPT_SHADE_WAVES 2
 #endif
+#ifndef SSPT_SHADE_ADJ_WAVES
+#define SSPT_SHADE_ADJ_WAVES 2
+#endif
 
 namespace sspt {
 
@@ -889,7 +892,7 @@ struct PathVertex {
 
 // One bounce of EstimateIncidentRadiance for every path of the queue.
 template <bool ADJ>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ADJ ? 1 : SSPT_SHADE_WAVES)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ADJ ? SSPT_SHADE_ADJ_WAVES : SSPT_SHADE_WAVES)))
 k_shade(DevScene S, Wave w, int buf, int max_nsr) {
     __shared__ uint32_t app[256 / 64 + 1];  // block_append scratch
     const uint32_t count = w.counters[buf];
