@@ -83,7 +83,7 @@ __device__ __forceinline__ D4 operator/(D4 a, float b) {
 __device__ __forceinline__ D4 operator/(float a, D4 b) { return dconst(a) / b; }
 __device__ __forceinline__ D4 dsqrt(D4 a) {
     D4 r;
-    r.v = sqrtf(a.v);
+    r.v = ssqrt(a.v);
     float g = r.v > 0.0f ? 0.5f / r.v : 0.0f;
 #pragma unroll
     for (int k = 0; k < 4; ++k) r.d[k] = a.d[k] * g;
@@ -152,7 +152,7 @@ __device__ D3 material_value_dual(const DevScene& S, const Shader& sh, f2 uv, f3
     D3 color_tint = lum.v > 0.0f ? base / lum : d3const(gray(1.0f));
     D3 spec_tint = d3lerp(m.spec_tint, gray(1.0f), color_tint);
     D4 rough = tex_rough ? dvar(sh.roughness, 3) : dconst(sh.roughness);
-    float aspect = sqrtf(1.0f - m.anisotropic * 0.9f);
+    float aspect = ssqrt(1.0f - m.anisotropic * 0.9f);
     GgxD ggx{dmax(0.001f, rough * rough / aspect), dmax(0.001f, rough * rough * aspect)};
     D3 spec_r0 = d3lerp(m.metallic, spec_tint * schlick_r0(m.ior), base);
     float dw = (1.0f - m.metallic) * (1.0f - m.spec_trans);

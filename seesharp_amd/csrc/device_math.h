@@ -5,8 +5,9 @@
 // library is compiled with -ffp-contract=off so that camera rays, the RNG
 // jitter and the watertight triangle test round exactly like the CPU):
 //   * vector sums are evaluated left to right, x*x + y*y + z*z;
-//   * normalize(v) = v / sqrt(dot(v, v)) per component (correctly rounded
-//     division and square root, HIP's default for fp32);
+//   * normalize_exact(v) = v / sqrt(dot(v, v)) per component (correctly
+//     rounded division and square root), for camera rays; shading uses the
+//     hardware forms (see ssqrt);
 //   * (int)float follows C#'s x86 conversion: truncation, out-of-range and
 //     NaN give INT_MIN.
 #pragma once
@@ -34,14 +35,45 @@ SSPT_HD f3 operator-(f3 a) { return {-a.x, -a.y, -a.z}; }
 SSPT_HD f3 operator*(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
 SSPT_HD f3 operator*(float s, f3 a) { return {s * a.x, s * a.y, s * a.z}; }
 SSPT_HD f3 operator*(f3 a, f3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
-SSPT_HD f3 operator/(f3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+// Shading arithmetic on the device uses the hardware square root and
+// reciprocal (within about 1 ulp) in place of the correctly rounded sequences
+// of a dozen instructions each: ssqrt, srcp, length, normalize and f3 / s.
+// Host code keeps the exact forms. Camera rays use normalize_exact and the
+// triangle test exact scalar division, so both still round like the CPU.
+SSPT_HD float ssqrt(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_sqrtf(x);
+#else
+    return sqrtf(x);
+#endif
+}
+SSPT_HD float srcp(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+SSPT_HD f3 operator/(f3 a, float s) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float r = srcp(s);
+    return {a.x * r, a.y * r, a.z * r};
+#else
+    return {a.x / s, a.y / s, a.z / s};
+#endif
+}
 SSPT_HD bool is_zero(f3 a) { return a.x == 0.0f && a.y == 0.0f && a.z == 0.0f; }
 SSPT_HD float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 SSPT_HD f3 cross(f3 a, f3 b) {
     return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
-SSPT_HD float length(f3 a) { return sqrtf(dot(a, a)); }
+SSPT_HD float length(f3 a) { return ssqrt(dot(a, a)); }
 SSPT_HD f3 normalize(f3 a) { return a / length(a); }
+// correctly rounded: v / sqrt(dot(v, v)) per component
+SSPT_HD f3 normalize_exact(f3 a) {
+    const float l = sqrtf(dot(a, a));
+    return {a.x / l, a.y / l, a.z / l};
+}
 SSPT_HD f3 vabs(f3 a) { return {fabsf(a.x), fabsf(a.y), fabsf(a.z)}; }
 SSPT_HD float comp(f3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
 SSPT_HD f3 f3_of(float4 v) { return {v.x, v.y, v.z}; }
@@ -115,12 +147,12 @@ struct Rng {
 // ---------------------------------------------------------------- warps ---
 // SampleWarp.ToUniformTriangle (SeeSharp/Sampling/SampleWarp.cs:9-14)
 SSPT_HD f2 to_uniform_triangle(f2 p) {
-    float s = sqrtf(p.x);
+    float s = ssqrt(p.x);
     return {1.0f - s, p.y * s};
 }
 // SampleWarp.ToCosHemisphere + SphericalToCartesian (SampleWarp.cs:89-96,130-137)
 SSPT_HD f3 to_cos_hemisphere(f2 p) {
-    float sin_t = sqrtf(1.0f - p.y), cos_t = sqrtf(p.y);
+    float sin_t = ssqrt(1.0f - p.y), cos_t = ssqrt(p.y);
     float phi = 2.0f * kPi * p.x;
     return {sin_t * cosf(phi), sin_t * sinf(phi), cos_t};
 }
@@ -128,7 +160,7 @@ SSPT_HD f3 to_cos_hemisphere(f2 p) {
 SSPT_HD float area_to_solid_angle(f3 from, f3 to, f3 to_normal) {
     f3 d = to - from;
     float d2 = dot(d, d);
-    return fabsf(dot(to_normal, -d)) / (d2 * sqrtf(d2));
+    return fabsf(dot(to_normal, -d)) / (d2 * ssqrt(d2));
 }
 
 // ---------------------------------------------------------- shading space -
@@ -139,10 +171,10 @@ struct Frame {
     SSPT_HD void build(f3 normal) {
         n = normal;
         if (fabsf(n.x) > fabsf(n.y)) {
-            float denom = sqrtf(n.x * n.x + n.z * n.z);
+            float denom = ssqrt(n.x * n.x + n.z * n.z);
             t = mk3(-n.z, 0.0f, n.x) / denom;
         } else {
-            float denom = sqrtf(n.y * n.y + n.z * n.z);
+            float denom = ssqrt(n.y * n.y + n.z * n.z);
             t = mk3(0.0f, n.z, -n.y) / denom;
         }
         b = cross(n, t);
@@ -153,7 +185,7 @@ struct Frame {
 
 SSPT_HD float cos2_theta(f3 d) { return d.z * d.z; }
 SSPT_HD float sin2_theta(f3 d) { return fmaxf(0.0f, 1.0f - cos2_theta(d)); }
-SSPT_HD float sin_theta(f3 d) { return sqrtf(sin2_theta(d)); }
+SSPT_HD float sin_theta(f3 d) { return ssqrt(sin2_theta(d)); }
 SSPT_HD float tan_theta(f3 d) { return sin_theta(d) / d.z; }
 SSPT_HD float tan2_theta(f3 d) { return sin2_theta(d) / cos2_theta(d); }
 SSPT_HD float cos_phi(f3 d) {
@@ -170,7 +202,7 @@ SSPT_HD bool refract(f3 i, f3 n, float eta, f3* out) {
     float sin2_i = fmaxf(0.0f, 1.0f - cos_i * cos_i);
     float sin2_t = eta * eta * sin2_i;
     if (sin2_t >= 1.0f) return false;
-    float cos_t = sqrtf(1.0f - sin2_t);
+    float cos_t = ssqrt(1.0f - sin2_t);
     *out = eta * -i + (eta * cos_i - cos_t) * n;
     return true;
 }
@@ -192,9 +224,9 @@ struct Ggx {
         float abs_tan = fabsf(tan_theta(o));
         if (isinf(abs_tan)) return 0.0f;
         float cp = cos_phi(o), sp = sin_phi(o);
-        float alpha = sqrtf(cp * cp * ax * ax + sp * sp * ay * ay);
+        float alpha = ssqrt(cp * cp * ax * ax + sp * sp * ay * ay);
         float a2t2 = alpha * abs_tan * (alpha * abs_tan);
-        return (-1.0f + sqrtf(1.0f + a2t2)) / 2.0f;
+        return (-1.0f + ssqrt(1.0f + a2t2)) / 2.0f;
     }
     SSPT_HD float g1(f3 o) const { return 1.0f / (1.0f + masking_ratio(o)); }
     SSPT_HD float g(f3 o, f3 i) const { return 1.0f / (1.0f + masking_ratio(o) + masking_ratio(i)); }
@@ -212,15 +244,15 @@ struct Ggx {
         }
         f3 vh = normalize(mk3(ax * o.x, ay * o.y, o.z));
         float lensq = vh.x * vh.x + vh.y * vh.y;
-        f3 t1 = lensq > 0.0f ? mk3(-vh.y, vh.x, 0.0f) * (1.0f / sqrtf(lensq)) : mk3(1, 0, 0);
+        f3 t1 = lensq > 0.0f ? mk3(-vh.y, vh.x, 0.0f) * (1.0f / ssqrt(lensq)) : mk3(1, 0, 0);
         f3 t2 = cross(vh, t1);
-        float r = sqrtf(p.x);
+        float r = ssqrt(p.x);
         float phi = 2.0f * kPi * p.y;
         float a1 = r * cosf(phi);
         float a2 = r * sinf(phi);
         float s = 0.5f * (1.0f + vh.z);
-        a2 = (1.0f - s) * sqrtf(1.0f - a1 * a1) + s * a2;
-        f3 nh = a1 * t1 + a2 * t2 + sqrtf(fmaxf(0.0f, 1.0f - a1 * a1 - a2 * a2)) * vh;
+        a2 = (1.0f - s) * ssqrt(1.0f - a1 * a1) + s * a2;
+        f3 nh = a1 * t1 + a2 * t2 + ssqrt(fmaxf(0.0f, 1.0f - a1 * a1 - a2 * a2)) * vh;
         f3 ne = normalize(mk3(ax * nh.x, ay * nh.y, fmaxf(0.0f, nh.z)));
         return flip ? -ne : ne;
     }
@@ -235,10 +267,10 @@ SSPT_HD float fresnel_dielectric(float cos_i, float eta_i, float eta_t) {
         eta_t = tmp;
         cos_i = fabsf(cos_i);
     }
-    float sin_i = sqrtf(fmaxf(0.0f, 1.0f - cos_i * cos_i));
+    float sin_i = ssqrt(fmaxf(0.0f, 1.0f - cos_i * cos_i));
     float sin_t = eta_i / eta_t * sin_i;
     if (sin_t >= 1.0f) return 1.0f;
-    float cos_t = sqrtf(fmaxf(0.0f, 1.0f - sin_t * sin_t));
+    float cos_t = ssqrt(fmaxf(0.0f, 1.0f - sin_t * sin_t));
     float rparl = (eta_t * cos_i - eta_i * cos_t) / (eta_t * cos_i + eta_i * cos_t);
     float rperp = (eta_i * cos_i - eta_t * cos_t) / (eta_i * cos_i + eta_t * cos_t);
     return (rparl * rparl + rperp * rperp) / 2.0f;

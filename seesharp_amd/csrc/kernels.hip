@@ -81,7 +81,7 @@ __device__ __forceinline__ void camera_ray(const DevCamera& c, float fx, float f
     f3 dd = mk3(l.x * W[0] + l.y * W[4] + l.z * W[8], l.x * W[1] + l.y * W[5] + l.z * W[9],
                 l.x * W[2] + l.y * W[6] + l.z * W[10]);
     *o = mk3(c.pos[0], c.pos[1], c.pos[2]);
-    *d = normalize(dd);
+    *d = normalize_exact(dd);
 }
 
 // ------------------------------------------------------------ intersection -

@@ -93,7 +93,7 @@ __device__ __forceinline__ void make_shader(const DevScene& S, const Surf& s, f3
         f3 color_tint = lum > 0.0f ? base / lum : gray(1.0f);
         sh->spec_tint = lerp(m.spec_tint, gray(1.0f), color_tint);
         sh->roughness = lookup_mono(S, m.rough_tex, m.roughness, s.uv);
-        float aspect = sqrtf(1.0f - m.anisotropic * 0.9f);
+        float aspect = ssqrt(1.0f - m.anisotropic * 0.9f);
         sh->ax = fmaxf(0.001f, sh->roughness * sh->roughness / aspect);
         sh->ay = fmaxf(0.001f, sh->roughness * sh->roughness * aspect);
         sh->spec_r0 = lerp(m.metallic, schlick_r0(m.ior) * sh->spec_tint, base);
@@ -295,7 +295,7 @@ __device__ __forceinline__ Env env_of(const DevScene& S, const SceneRec& R) {
 }
 __device__ __forceinline__ f2 world_to_spherical(f3 dir) {
     dir = normalize(dir);
-    f2 sp{atan2f(dir.z, dir.x), atan2f(sqrtf(dir.x * dir.x + dir.z * dir.z), dir.y)};
+    f2 sp{atan2f(dir.z, dir.x), atan2f(ssqrt(dir.x * dir.x + dir.z * dir.z), dir.y)};
     if (sp.x < 0.0f) sp.x += kPi * 2.0f;
     return sp;
 }
