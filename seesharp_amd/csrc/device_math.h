@@ -20,6 +20,8 @@
 namespace sspt {
 
 constexpr float kPi = 3.14159265358979323846f;
+constexpr float kInvPi = 0.318309886183790671538f;
+constexpr float kInv2Pi = 0.159154943091895335769f;
 
 struct f3 {
     float x, y, z;
@@ -54,6 +56,31 @@ SSPT_HD float srcp(float x) {
     return 1.0f / x;
 #endif
 }
+SSPT_HD float sdiv(float a, float b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return a * srcp(b);
+#else
+    return a / b;
+#endif
+}
+// sine and cosine of an angle in [0, 2 pi] (hardware forms on the device,
+// absolute error about 1e-6)
+SSPT_HD float ssin(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __sinf(x);
+#else
+    return sinf(x);
+#endif
+}
+SSPT_HD void ssincos(float x, float* s, float* c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    *s = __sinf(x);
+    *c = __cosf(x);
+#else
+    *s = sinf(x);
+    *c = cosf(x);
+#endif
+}
 SSPT_HD f3 operator/(f3 a, float s) {
 #if defined(__HIP_DEVICE_COMPILE__)
     const float r = srcp(s);
@@ -69,6 +96,8 @@ SSPT_HD f3 cross(f3 a, f3 b) {
 }
 SSPT_HD float length(f3 a) { return ssqrt(dot(a, a)); }
 SSPT_HD f3 normalize(f3 a) { return a / length(a); }
+// correctly rounded per-component division (FrameBuffer arithmetic)
+SSPT_HD f3 div_exact(f3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
 // correctly rounded: v / sqrt(dot(v, v)) per component
 SSPT_HD f3 normalize_exact(f3 a) {
     const float l = sqrtf(dot(a, a));
@@ -153,14 +182,15 @@ SSPT_HD f2 to_uniform_triangle(f2 p) {
 // SampleWarp.ToCosHemisphere + SphericalToCartesian (SampleWarp.cs:89-96,130-137)
 SSPT_HD f3 to_cos_hemisphere(f2 p) {
     float sin_t = ssqrt(1.0f - p.y), cos_t = ssqrt(p.y);
-    float phi = 2.0f * kPi * p.x;
-    return {sin_t * cosf(phi), sin_t * sinf(phi), cos_t};
+    float phi = 2.0f * kPi * p.x, sp, cp;
+    ssincos(phi, &sp, &cp);
+    return {sin_t * cp, sin_t * sp, cos_t};
 }
 // SampleWarp.SurfaceAreaToSolidAngle (SampleWarp.cs:290-295)
 SSPT_HD float area_to_solid_angle(f3 from, f3 to, f3 to_normal) {
     f3 d = to - from;
     float d2 = dot(d, d);
-    return fabsf(dot(to_normal, -d)) / (d2 * ssqrt(d2));
+    return sdiv(fabsf(dot(to_normal, -d)), d2 * ssqrt(d2));
 }
 
 // ---------------------------------------------------------- shading space -
@@ -186,15 +216,15 @@ struct Frame {
 SSPT_HD float cos2_theta(f3 d) { return d.z * d.z; }
 SSPT_HD float sin2_theta(f3 d) { return fmaxf(0.0f, 1.0f - cos2_theta(d)); }
 SSPT_HD float sin_theta(f3 d) { return ssqrt(sin2_theta(d)); }
-SSPT_HD float tan_theta(f3 d) { return sin_theta(d) / d.z; }
-SSPT_HD float tan2_theta(f3 d) { return sin2_theta(d) / cos2_theta(d); }
+SSPT_HD float tan_theta(f3 d) { return sdiv(sin_theta(d), d.z); }
+SSPT_HD float tan2_theta(f3 d) { return sdiv(sin2_theta(d), cos2_theta(d)); }
 SSPT_HD float cos_phi(f3 d) {
     float s = sin_theta(d);
-    return s == 0.0f ? 1.0f : clampf(d.x / s, -1.0f, 1.0f);
+    return s == 0.0f ? 1.0f : clampf(sdiv(d.x, s), -1.0f, 1.0f);
 }
 SSPT_HD float sin_phi(f3 d) {
     float s = sin_theta(d);
-    return s == 0.0f ? 0.0f : clampf(d.y / s, -1.0f, 1.0f);
+    return s == 0.0f ? 0.0f : clampf(sdiv(d.y, s), -1.0f, 1.0f);
 }
 SSPT_HD f3 reflect(f3 o, f3 n) { return -o + 2.0f * dot(o, n) * n; }
 SSPT_HD bool refract(f3 i, f3 n, float eta, f3* out) {
@@ -217,8 +247,8 @@ struct Ggx {
         if (isinf(tan2)) return 0.0f;
         float cos4 = cos2_theta(n) * cos2_theta(n);
         float cp = cos_phi(n), sp = sin_phi(n);
-        float e = tan2 * (cp * cp / (ax * ax) + sp * sp / (ay * ay));
-        return 1.0f / (kPi * ax * ay * cos4 * (1.0f + e) * (1.0f + e));
+        float e = tan2 * (sdiv(cp * cp, ax * ax) + sdiv(sp * sp, ay * ay));
+        return srcp(kPi * ax * ay * cos4 * (1.0f + e) * (1.0f + e));
     }
     SSPT_HD float masking_ratio(f3 o) const {
         float abs_tan = fabsf(tan_theta(o));
@@ -226,13 +256,13 @@ struct Ggx {
         float cp = cos_phi(o), sp = sin_phi(o);
         float alpha = ssqrt(cp * cp * ax * ax + sp * sp * ay * ay);
         float a2t2 = alpha * abs_tan * (alpha * abs_tan);
-        return (-1.0f + ssqrt(1.0f + a2t2)) / 2.0f;
+        return (-1.0f + ssqrt(1.0f + a2t2)) * 0.5f;
     }
-    SSPT_HD float g1(f3 o) const { return 1.0f / (1.0f + masking_ratio(o)); }
-    SSPT_HD float g(f3 o, f3 i) const { return 1.0f / (1.0f + masking_ratio(o) + masking_ratio(i)); }
+    SSPT_HD float g1(f3 o) const { return srcp(1.0f + masking_ratio(o)); }
+    SSPT_HD float g(f3 o, f3 i) const { return srcp(1.0f + masking_ratio(o) + masking_ratio(i)); }
     SSPT_HD float pdf(f3 o, f3 n) const {
         if (!same_hemisphere(o, n)) return 0.0f;
-        return g1(o) * fmaxf(0.0f, dot(o, n)) * ndf(n) / fabsf(o.z);
+        return sdiv(g1(o) * fmaxf(0.0f, dot(o, n)) * ndf(n), fabsf(o.z));
     }
     // Heitz 2018 visible-normal sampling (TrowbridgeReitzDistribution.cs:70-100);
     // the reciprocal square-root estimate is replaced by 1 / sqrt.
@@ -244,12 +274,13 @@ struct Ggx {
         }
         f3 vh = normalize(mk3(ax * o.x, ay * o.y, o.z));
         float lensq = vh.x * vh.x + vh.y * vh.y;
-        f3 t1 = lensq > 0.0f ? mk3(-vh.y, vh.x, 0.0f) * (1.0f / ssqrt(lensq)) : mk3(1, 0, 0);
+        f3 t1 = lensq > 0.0f ? mk3(-vh.y, vh.x, 0.0f) * srcp(ssqrt(lensq)) : mk3(1, 0, 0);
         f3 t2 = cross(vh, t1);
         float r = ssqrt(p.x);
-        float phi = 2.0f * kPi * p.y;
-        float a1 = r * cosf(phi);
-        float a2 = r * sinf(phi);
+        float phi = 2.0f * kPi * p.y, sphi, cphi;
+        ssincos(phi, &sphi, &cphi);
+        float a1 = r * cphi;
+        float a2 = r * sphi;
         float s = 0.5f * (1.0f + vh.z);
         a2 = (1.0f - s) * ssqrt(1.0f - a1 * a1) + s * a2;
         f3 nh = a1 * t1 + a2 * t2 + ssqrt(fmaxf(0.0f, 1.0f - a1 * a1 - a2 * a2)) * vh;
@@ -268,12 +299,12 @@ SSPT_HD float fresnel_dielectric(float cos_i, float eta_i, float eta_t) {
         cos_i = fabsf(cos_i);
     }
     float sin_i = ssqrt(fmaxf(0.0f, 1.0f - cos_i * cos_i));
-    float sin_t = eta_i / eta_t * sin_i;
+    float sin_t = sdiv(eta_i, eta_t) * sin_i;
     if (sin_t >= 1.0f) return 1.0f;
     float cos_t = ssqrt(fmaxf(0.0f, 1.0f - sin_t * sin_t));
-    float rparl = (eta_t * cos_i - eta_i * cos_t) / (eta_t * cos_i + eta_i * cos_t);
-    float rperp = (eta_i * cos_i - eta_t * cos_t) / (eta_i * cos_i + eta_t * cos_t);
-    return (rparl * rparl + rperp * rperp) / 2.0f;
+    float rparl = sdiv(eta_t * cos_i - eta_i * cos_t, eta_t * cos_i + eta_i * cos_t);
+    float rperp = sdiv(eta_i * cos_i - eta_t * cos_t, eta_i * cos_i + eta_t * cos_t);
+    return (rparl * rparl + rperp * rperp) * 0.5f;
 }
 SSPT_HD float schlick_weight(float c) {
     float m = clampf(1.0f - c, 0.0f, 1.0f);
@@ -281,7 +312,7 @@ SSPT_HD float schlick_weight(float c) {
 }
 SSPT_HD f3 schlick_fresnel(f3 r0, float c) { return lerp(schlick_weight(c), r0, gray(1.0f)); }
 SSPT_HD float schlick_r0(float eta) {
-    float r = (eta - 1.0f) / (eta + 1.0f);
+    float r = sdiv(eta - 1.0f, eta + 1.0f);
     return r * r;
 }
 

@@ -668,7 +668,7 @@ __device__ __forceinline__ float4 contribution_gradient(const DevScene& S, const
 // (GlossyEmitter.cs:36-40) scales by cos^exponent * (exponent + 1) / (2 pi).
 __device__ __forceinline__ f3 emitted(const DevEmitter& e, f3 ns, f3 dir) {
     if (e.kind == kEmitterGlossy) {
-        float cosine = dot(ns, dir) / length(dir);
+        float cosine = sdiv(dot(ns, dir), length(dir));
         if (cosine < 0.0f) return gray(0.0f);
         return mk3(e.r, e.g, e.b) * powf(cosine, e.exponent) * e.norm;
     }
@@ -746,7 +746,7 @@ struct PathVertex {
                     mis = 1.0f;
                     if (depth > 1) {
                         float pdf_ne = env_pdf(E, dir) * nsr;
-                        mis = 1.0f / (1.0f + pdf_ne / prev_pdf);
+                        mis = srcp(1.0f + sdiv(pdf_ne, prev_pdf));
                     }
                     contrib = env_radiance(E, dir);
                 }
@@ -769,9 +769,9 @@ struct PathVertex {
             float mis = 1.0f;
             if (depth > 1) {
                 float jac = area_to_solid_angle(prev_pos, s.pos, s.ng);
-                float pdf_ne = e.inv_area / (float)R.num_emitters * nsr / jac;
-                float ratio = pdf_ne / prev_pdf;
-                mis = 1.0f / (ratio + 1.0f);
+                float pdf_ne = sdiv(sdiv(e.inv_area, (float)R.num_emitters) * nsr, jac);
+                float ratio = sdiv(pdf_ne, prev_pdf);
+                mis = srcp(ratio + 1.0f);
                 if (!R.enable_bsdf_di) mis = 0.0f;
             }
             f3 em = emitted(e, s.ns, -dir);
@@ -798,7 +798,7 @@ struct PathVertex {
         float pdf_bsdf;
         eval_cos_pdf(sh, es.dir, &bc, &pdf_bsdf);
         if (pdf_bsdf == 0.0f || es.pdf == 0.0f) return false;
-        float mis = R.enable_bsdf_di ? 1.0f / (1.0f + pdf_bsdf / (es.pdf * nsr)) : 1.0f;
+        float mis = R.enable_bsdf_di ? srcp(1.0f + sdiv(pdf_bsdf, es.pdf * nsr)) : 1.0f;
         f3 contrib = es.weight * bc / nsr;
         f3 val = prefix * (mis * contrib) / q;
         if (log) {
@@ -826,7 +826,7 @@ struct PathVertex {
         const float nsr = (float)R.num_shadow_rays;
         int idx = (int)rng.next_int((uint32_t)R.num_emitters);
         DevEmitter e = S.emitters[R.emitter_off + idx];
-        float sel = 1.0f / (float)R.num_emitters;
+        float sel = srcp((float)R.num_emitters);
         f2 b = to_uniform_triangle(rng.next_float2());
         Surf lp = surface(S, e.tri, b.x, b.y);
         f3 l2s = normalize(s.pos - lp.pos);
@@ -839,9 +839,9 @@ struct PathVertex {
         float pdf_bsdf = pdf_bsdf_sa * jac;
         f3 val = gray(0.0f);
         if (jac != 0.0f) {
-            float ratio = pdf_bsdf / pdf_ne;
-            float mis = R.enable_bsdf_di ? 1.0f / (ratio + 1.0f) : 1.0f;
-            float pdf = e.inv_area / jac * sel * nsr;
+            float ratio = sdiv(pdf_bsdf, pdf_ne);
+            float mis = R.enable_bsdf_di ? srcp(ratio + 1.0f) : 1.0f;
+            float pdf = sdiv(e.inv_area, jac) * sel * nsr;
             f3 contrib = em / pdf * bc;
             val = prefix * (mis * contrib) / q;
             if (log) {
@@ -1229,7 +1229,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_debug_visualize(DevScene S, int
             }
             const float cnt = (float)(it + 1);
             if (it + 1 > 1) img = img * ((cnt - 1.0f) / cnt);
-            img = img + gray(value) / cnt;
+            img = img + div_exact(gray(value), cnt);
         }
         image[3 * pix] = img.x;
         image[3 * pix + 1] = img.y;
@@ -1370,7 +1370,7 @@ __global__ void __launch_bounds__(256) k_accumulate_mean(Wave w, float* image, u
             float n = (float)(d.first_iter + s + 1);
             if (d.first_iter + s + 1 > 1) img = img * ((n - 1.0f) / n);
             if (isfinite(average(L))) {
-                img = img + L / n;
+                img = img + div_exact(L, n);
                 if (ok) heap.add(average(L), d.first_iter + s, num_outliers);
             } else {
                 ++nans;
@@ -1380,8 +1380,8 @@ __global__ void __launch_bounds__(256) k_accumulate_mean(Wave w, float* image, u
                     alb = alb * ((n - 1.0f) / n);
                     nrm = nrm * ((n - 1.0f) / n);
                 }
-                alb = alb + f3_of(w.st_alb[slot]) / n;
-                nrm = nrm + f3_of(w.st_nrm[slot]) / n;
+                alb = alb + div_exact(f3_of(w.st_alb[slot]), n);
+                nrm = nrm + div_exact(f3_of(w.st_nrm[slot]), n);
             }
         }
         image[3 * pix] = img.x;

@@ -94,7 +94,7 @@ __device__ __forceinline__ void make_shader(const DevScene& S, const Surf& s, f3
         sh->spec_tint = lerp(m.spec_tint, gray(1.0f), color_tint);
         sh->roughness = lookup_mono(S, m.rough_tex, m.roughness, s.uv);
         float aspect = ssqrt(1.0f - m.anisotropic * 0.9f);
-        sh->ax = fmaxf(0.001f, sh->roughness * sh->roughness / aspect);
+        sh->ax = fmaxf(0.001f, sdiv(sh->roughness * sh->roughness, aspect));
         sh->ay = fmaxf(0.001f, sh->roughness * sh->roughness * aspect);
         sh->spec_r0 = lerp(m.metallic, schlick_r0(m.ior) * sh->spec_tint, base);
         float dw = (1.0f - m.metallic) * (1.0f - m.spec_trans);
@@ -123,7 +123,7 @@ __device__ void generic_value_pdf(const Shader& sh, f3 in, f3* value, float* pdf
     float f_out = schlick_weight(cos_o), f_in = schlick_weight(cos_i);
     if (same_hemi) {
         if (same_geo) val = val + sh.diffuse_refl / kPi * (1.0f - f_out * 0.5f) * (1.0f - f_in * 0.5f);
-        pdf0 = cos_i / kPi;
+        pdf0 = cos_i * kInvPi;
     }
     f3 h = out + in;
     if (!is_zero(h)) {
@@ -142,10 +142,10 @@ __device__ void generic_value_pdf(const Shader& sh, f3 in, f3* value, float* pdf
                 val = val + sh.spec_tint * ggx.ndf(h) * ggx.g(out, in) * fres / (4.0f * cos_i * cos_o);
             }
             float jf = fabsf(4.0f * c_out), jr = fabsf(4.0f * c_in);
-            if (jf != 0.0f && jr != 0.0f) pdf1 = ggx.pdf(out, h) / jf;
+            if (jf != 0.0f && jr != 0.0f) pdf1 = sdiv(ggx.pdf(out, h), jf);
         }
     }
-    float eta = out.z > 0.0f ? m.ior : (1.0f / m.ior);
+    float eta = out.z > 0.0f ? m.ior : srcp(m.ior);
     f3 ht = out + in * eta;
     if (cos_o != 0.0f && cos_i != 0.0f && !is_zero(ht) && m.spec_trans > 0.0f) {
         ht = normalize(ht);
@@ -158,10 +158,10 @@ __device__ void generic_value_pdf(const Shader& sh, f3 in, f3* value, float* pdf
             float num = ggx.ndf(wh) * ggx.g(out, in);
             num *= eta * eta * fmaxf(0.0f, dot(in, -wh)) * fmaxf(0.0f, dot(out, wh));
             float denom = in.z * out.z * sqrt_denom * sqrt_denom;
-            val = val + (gray(1.0f) - gray(F)) * sh.spec_trans * fabsf(num / denom);
+            val = val + (gray(1.0f) - gray(F)) * sh.spec_trans * fabsf(sdiv(num, denom));
         }
         if (sqrt_denom != 0.0f) {
-            float jac = eta * eta * fmaxf(0.0f, dot(in, -wh)) / (sqrt_denom * sqrt_denom);
+            float jac = sdiv(eta * eta * fmaxf(0.0f, dot(in, -wh)), sqrt_denom * sqrt_denom);
             pdf2 += ggx.pdf(out, wh) * jac;
         }
     }
@@ -188,9 +188,9 @@ __device__ __forceinline__ void diffuse_value_pdf(const Shader& sh, f3 in_world,
         if (same) v = sh.base / kPi;
     }
     *value = v;
-    float pr = same ? fabsf(in.z) / kPi : 0.0f;
+    float pr = same ? fabsf(in.z) * kInvPi : 0.0f;
     if (sh.m.transmitter) {
-        float pt = !same ? fabsf(in.z) / kPi : 0.0f;
+        float pt = !same ? fabsf(in.z) * kInvPi : 0.0f;
         *pdf = (pr + pt) * 0.5f;
     } else {
         *pdf = pr;
@@ -225,7 +225,7 @@ __device__ BsdfSample sample_bsdf(const Shader& sh, float pc, f2 pd) {
         f3 out = sh.out_local;
         if (out.z == 0.0f) return bad;
         Ggx ggx{sh.ax, sh.ay};
-        float eta = out.z > 0.0f ? (1.0f / m.ior) : m.ior;
+        float eta = out.z > 0.0f ? srcp(m.ior) : m.ior;
         float schl = average(schlick_fresnel(sh.spec_r0, fabsf(out.z)));
         float bias = fmaxf(1.0f - schl, 0.75f);
         float dw = bias * (1.0f - m.metallic) * (1.0f - m.spec_trans);
@@ -238,7 +238,7 @@ __device__ BsdfSample sample_bsdf(const Shader& sh, float pc, f2 pd) {
             float c_out = dot(out, h.z < 0.0f ? -h : h);
             float fres = fresnel_dielectric(c_out, 1.0f, m.ior);
             float sel_t = (1.0f - fres) * m.spec_trans;
-            if ((pc - dw) / (1.0f - dw) < sel_t) {
+            if (sdiv(pc - dw, 1.0f - dw) < sel_t) {
                 if (dot(out, h) < 0.0f) return bad;
                 f3 r;
                 if (!refract(out, h, eta, &r)) return bad;
@@ -306,7 +306,7 @@ __device__ __forceinline__ f3 env_pixel(const Env& E, int col, int row) {
 }
 __device__ __forceinline__ f3 env_radiance(const Env& E, f3 dir) {
     f2 sp = world_to_spherical(dir);
-    float px = sp.x / (2.0f * kPi), py = sp.y / kPi;
+    float px = sp.x * kInv2Pi, py = sp.y * kInvPi;
     return env_pixel(E, to_int(px * (float)E.w), to_int(py * (float)E.h));
 }
 __device__ __forceinline__ float cdf_prob(const float* cdf, int i) {
@@ -323,7 +323,7 @@ __device__ __forceinline__ float grid_pdf(const Env& E, float x, float y) {
 }
 __device__ __forceinline__ float env_pdf(const Env& E, f3 dir) {
     f2 sp = world_to_spherical(dir);
-    return grid_pdf(E, sp.x / (2.0f * kPi), sp.y / kPi) / (sinf(sp.y) * kPi * kPi * 2.0f);
+    return sdiv(grid_pdf(E, sp.x * kInv2Pi, sp.y * kInvPi), ssin(sp.y) * kPi * kPi * 2.0f);
 }
 // PiecewiseConstantPDF.Sample (SeeSharp/Sampling/PiecewiseConstant.cs:39-51): the
 // lower bound of u in the CDF (first entry >= u). The guide table holds the
@@ -354,20 +354,22 @@ struct EnvSample {
 __device__ EnvSample env_sample(const Env& E, f2 u) {
     int ri = min(lower_bound(E.row_cdf, E.row_guide, E.row_g, u.y), E.h - 1);
     float rlo = ri == 0 ? 0.0f : E.row_cdf[ri - 1];
-    float rrel = (u.y - rlo) / (E.row_cdf[ri] - rlo);
+    float rrel = sdiv(u.y - rlo, E.row_cdf[ri] - rlo);
     const float* ccdf = E.col_cdf + (size_t)ri * E.w;
     const int* cguide = E.col_guide + (size_t)ri * (E.col_g + 1);
     int ci = min(lower_bound(ccdf, cguide, E.col_g, u.x), E.w - 1);
     float clo = ci == 0 ? 0.0f : ccdf[ci - 1];
-    float crel = (u.x - clo) / (ccdf[ci] - clo);
-    float px = ((float)ci + crel) / (float)E.w;
-    float py = ((float)ri + rrel) / (float)E.h;
+    float crel = sdiv(u.x - clo, ccdf[ci] - clo);
+    float px = sdiv((float)ci + crel, (float)E.w);
+    float py = sdiv((float)ri + rrel, (float)E.h);
     float pdf = grid_pdf(E, px, py);
     float phi = px * 2.0f * kPi, theta = py * kPi;
-    float sin_t = sinf(theta);
+    float sin_t, cos_t, sin_p, cos_p;
+    ssincos(theta, &sin_t, &cos_t);
+    ssincos(phi, &sin_p, &cos_p);
     EnvSample r;
-    r.dir = mk3(sin_t * cosf(phi), cosf(theta), sin_t * sinf(phi));
-    float jac = sinf(theta) * kPi * kPi * 2.0f;
+    r.dir = mk3(sin_t * cos_p, cos_t, sin_t * sin_p);
+    float jac = sin_t * kPi * kPi * 2.0f;
     if (jac == 0.0f) {
         r.pdf = 0.0f;
         r.weight = gray(0.0f);
