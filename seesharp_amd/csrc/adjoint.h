@@ -45,8 +45,9 @@ __device__ __forceinline__ D4 operator*(D4 a, D4 b) {
 }
 __device__ __forceinline__ D4 operator/(D4 a, D4 b) {
     D4 r;
-    r.v = a.v / b.v;
-    float inv2 = 1.0f / (b.v * b.v);
+    const float ib = srcp(b.v);
+    r.v = a.v * ib;
+    const float inv2 = ib * ib;
 #pragma unroll
     for (int k = 0; k < 4; ++k) r.d[k] = (a.d[k] * b.v - a.v * b.d[k]) * inv2;
     return r;
@@ -75,16 +76,17 @@ __device__ __forceinline__ D4 operator*(D4 a, float b) {
 }
 __device__ __forceinline__ D4 operator*(float b, D4 a) { return a * b; }
 __device__ __forceinline__ D4 operator/(D4 a, float b) {
-    a.v /= b;
+    const float ib = srcp(b);
+    a.v *= ib;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) a.d[k] /= b;
+    for (int k = 0; k < 4; ++k) a.d[k] *= ib;
     return a;
 }
 __device__ __forceinline__ D4 operator/(float a, D4 b) { return dconst(a) / b; }
 __device__ __forceinline__ D4 dsqrt(D4 a) {
     D4 r;
     r.v = ssqrt(a.v);
-    float g = r.v > 0.0f ? 0.5f / r.v : 0.0f;
+    float g = r.v > 0.0f ? 0.5f * srcp(r.v) : 0.0f;
 #pragma unroll
     for (int k = 0; k < 4; ++k) r.d[k] = a.d[k] * g;
     return r;
@@ -101,7 +103,10 @@ __device__ __forceinline__ D3 operator*(D3 a, D4 s) { return {a.x * s, a.y * s, 
 __device__ __forceinline__ D3 operator*(D3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
 __device__ __forceinline__ D3 operator*(D3 a, D3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
 __device__ __forceinline__ D3 operator/(D3 a, D4 s) { return {a.x / s, a.y / s, a.z / s}; }
-__device__ __forceinline__ D3 operator/(D3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ D3 operator/(D3 a, float s) {
+    const float is = srcp(s);
+    return {a.x * is, a.y * is, a.z * is};
+}
 __device__ __forceinline__ D3 d3lerp(float w, D3 a, D3 b) { return a * (1.0f - w) + b * w; }
 __device__ __forceinline__ D3 d3lerp(float w, f3 a, D3 b) { return d3const(a) * (1.0f - w) + b * w; }
 __device__ __forceinline__ D3 d3lerp(float w, D3 a, f3 b) { return a * (1.0f - w) + d3const(b) * w; }
@@ -182,7 +187,7 @@ __device__ D3 material_value_dual(const DevScene& S, const Shader& sh, f2 uv, f3
             }
         }
     }
-    float eta = out.z > 0.0f ? m.ior : (1.0f / m.ior);
+    float eta = out.z > 0.0f ? m.ior : srcp(m.ior);
     f3 ht = out + in * eta;
     if (cos_o != 0.0f && cos_i != 0.0f && !is_zero(ht) && m.spec_trans > 0.0f) {
         ht = normalize(ht);
@@ -208,7 +213,7 @@ __device__ __forceinline__ void dlog_of(const D3& f, float* out12) {
     const D4* c[3] = {&f.x, &f.y, &f.z};
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch) {
-        float inv = c[ch]->v != 0.0f ? 1.0f / c[ch]->v : 0.0f;
+        float inv = c[ch]->v != 0.0f ? srcp(c[ch]->v) : 0.0f;
 #pragma unroll
         for (int k = 0; k < 4; ++k) out12[ch * 4 + k] = c[ch]->d[k] * inv;
     }
