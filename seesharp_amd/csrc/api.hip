@@ -369,10 +369,16 @@ sspt_status add_scene(sspt_ctx* ctx, WorldBuilder& W, const sspt_scene_desc* d) 
         float* node = &bvh.nodes[n * 32];
         int* idx = reinterpret_cast<int*>(node + 24);
         int* cnt = reinterpret_cast<int*>(node + 28);
+        uint32_t leaf = 0, inner = 0;
         for (int k = 0; k < 4; ++k) {
             if (cnt[k] > 0) idx[k] += (int)tri_base;
             else if (cnt[k] == 0) idx[k] += node_base;
+            if (cnt[k] >= (1 << 24)) return fail(ctx, SSPT_ERR_UNSUPPORTED, "BVH leaf of 2^24 or more triangles");
+            leaf |= (uint32_t)(cnt[k] > 0) << k;
+            inner |= (uint32_t)(cnt[k] == 0) << k;
         }
+        // the node's leaf and inner slot masks ride in the top byte of slot 0's count
+        cnt[0] = (int)(((uint32_t)cnt[0] & 0x00FFFFFFu) | ((leaf | (inner << 4)) << 24));
     }
     W.nodes.insert(W.nodes.end(), bvh.nodes.begin(), bvh.nodes.end());
 

@@ -334,13 +334,11 @@ struct Tracer {
         hc[2] = box_test(nx.z, fx.z, ny.z, fy.z, nz.z, fz.z, o, p.inv, tmin_box, tmax_box, &tc[2]);
         hc[3] = box_test(nx.w, fx.w, ny.w, fy.w, nz.w, fz.w, o, p.inv, tmin_box, tmax_box, &tc[3]);
         const int cidx[4] = {child.x, child.y, child.z, child.w};
-        const int ccnt[4] = {count.x, count.y, count.z, count.w};
-        // slot kinds as bit masks (count < 0: absent, 0: inner node, > 0: leaf)
+        const int ccnt[4] = {count.x & 0x00FFFFFF, count.y, count.z, count.w};
+        // slot kinds as bit masks, precomputed in the top byte of slot 0's count
         const uint32_t hit = (uint32_t)hc[0] | ((uint32_t)hc[1] << 1) | ((uint32_t)hc[2] << 2) | ((uint32_t)hc[3] << 3);
-        const uint32_t leaf = (uint32_t)(count.x > 0) | ((uint32_t)(count.y > 0) << 1) |
-                              ((uint32_t)(count.z > 0) << 2) | ((uint32_t)(count.w > 0) << 3);
-        const uint32_t inner = (uint32_t)(count.x == 0) | ((uint32_t)(count.y == 0) << 1) |
-                               ((uint32_t)(count.z == 0) << 2) | ((uint32_t)(count.w == 0) << 3);
+        const uint32_t meta = (uint32_t)count.x >> 24;
+        const uint32_t leaf = meta & 15u, inner = meta >> 4;
         uint32_t leaves = hit & leaf;  // hit leaf children
         const uint32_t inner_hit = hit & inner;
 #pragma unroll

@@ -17,7 +17,9 @@
 //   node[8*n+0..5] = lo.x, hi.x, lo.y, hi.y, lo.z, hi.z of children 0..3
 //   node[8*n+6]    = child index per slot (int bits): device node, or first leaf triangle
 //   node[8*n+7]    = triangle count per slot (int bits): 0 = inner node, > 0 = leaf,
-//                    -1 = absent slot (box [+inf, +inf]^3, skipped by the traversal)
+//                    -1 = absent slot (box [+inf, +inf]^3, skipped by the traversal);
+//                    slot 0 keeps its count in bits 0..23 (two's complement for -1) and
+//                    the node's slot masks in bits 24..31: leaf slots, then inner slots << 4
 #pragma once
 
 #include <hip/hip_runtime.h>
