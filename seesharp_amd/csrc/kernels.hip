@@ -65,6 +65,33 @@ __device__ __forceinline__ uint32_t block_append(uint32_t* counter, bool pred, u
     __syncthreads();  // the next call reuses lds
     return r;
 }
+// Two block_appends to one counter in one pass: the block's `pa` entries,
+// then its `pb` entries. `lds` holds two uint32 per wave of the block plus one.
+__device__ __forceinline__ uint2 block_append2(uint32_t* counter, bool pa, bool pb, uint32_t* lds) {
+    const uint64_t ma = __ballot(pa), mb = __ballot(pb);
+    const uint32_t wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const uint32_t lane = __lane_id();
+    if (lane == 0) {
+        lds[wave] = (uint32_t)__popcll(ma);
+        lds[nw + wave] = (uint32_t)__popcll(mb);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t total = 0;
+        for (uint32_t k = 0; k < 2 * nw; ++k) {
+            uint32_t c = lds[k];
+            lds[k] = total;
+            total += c;
+        }
+        lds[2 * nw] = total ? atomicAdd(counter, total) : 0u;
+    }
+    __syncthreads();
+    const uint64_t below = (1ull << lane) - 1ull;
+    const uint2 r = make_uint2(lds[2 * nw] + lds[wave] + (uint32_t)__popcll(ma & below),
+                               lds[2 * nw] + lds[nw + wave] + (uint32_t)__popcll(mb & below));
+    __syncthreads();  // the next call reuses lds
+    return r;
+}
 
 // ------------------------------------------------------------------ camera -
 // PerspectiveCamera.GenerateRay (SeeSharp/Cameras/PerspectiveCamera.cs:64-102), lensRadius = 0:
@@ -892,7 +919,7 @@ struct PathVertex {
 template <bool ADJ>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ADJ ? SSPT_SHADE_ADJ_WAVES : SSPT_SHADE_WAVES)))
 k_shade(DevScene S, Wave w, int buf, int max_nsr) {
-    __shared__ uint32_t app[256 / 64 + 1];  // block_append scratch
+    __shared__ uint32_t app[2 * (256 / 64) + 1];  // block_append / block_append2 scratch
     const uint32_t count = w.counters[buf];
     const int nb = buf ^ 1;
     for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += gridDim.x * blockDim.x) {
@@ -940,7 +967,12 @@ k_shade(DevScene S, Wave w, int buf, int max_nsr) {
             float4 eo, ed, ec, eg = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             const bool env_ray = nee_k && R.has_env && v.env_next_event(&eo, &ed, &ec, &eg);
             wave_count(w.stats, kStatShadow, nee_k && R.has_env);
-            uint32_t e_idx = block_append(&w.counters[kCntShadow], env_ray, app);
+            float4 ao, ad, ac, ag = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            const bool area_ray = nee_k && R.num_emitters > 0 && v.area_next_event(&ao, &ad, &ac, &ag);
+            wave_count(w.stats, kStatShadow, nee_k && R.num_emitters > 0);
+            // one block pass for both: the block's environment rays, then its area-light rays
+            const uint2 idx = block_append2(&w.counters[kCntShadow], env_ray, area_ray, app);
+            const uint32_t e_idx = idx.x, a_idx = idx.y;
             if (env_ray) {
                 w.s_o[e_idx] = eo;
                 w.s_d[e_idx] = ed;
@@ -950,10 +982,6 @@ k_shade(DevScene S, Wave w, int buf, int max_nsr) {
                     w.s_t[e_idx] = v.vtex;
                 }
             }
-            float4 ao, ad, ac, ag = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            const bool area_ray = nee_k && R.num_emitters > 0 && v.area_next_event(&ao, &ad, &ac, &ag);
-            wave_count(w.stats, kStatShadow, nee_k && R.num_emitters > 0);
-            uint32_t a_idx = block_append(&w.counters[kCntShadow], area_ray, app);
             if (area_ray) {
                 w.s_o[a_idx] = ao;
                 w.s_d[a_idx] = ad;
