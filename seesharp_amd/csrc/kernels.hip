@@ -92,6 +92,43 @@ __device__ __forceinline__ uint2 block_append2(uint32_t* counter, bool pa, bool 
     __syncthreads();  // the next call reuses lds
     return r;
 }
+// block_append2 to `counter_ab` and a block_append of `pc` to `counter_c`, in
+// one pass. `lds` holds three uint32 per wave of the block plus two.
+__device__ __forceinline__ uint3 block_append3(uint32_t* counter_ab, bool pa, bool pb, uint32_t* counter_c, bool pc,
+                                               uint32_t* lds) {
+    const uint64_t ma = __ballot(pa), mb = __ballot(pb), mc = __ballot(pc);
+    const uint32_t wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const uint32_t lane = __lane_id();
+    if (lane == 0) {
+        lds[wave] = (uint32_t)__popcll(ma);
+        lds[nw + wave] = (uint32_t)__popcll(mb);
+        lds[2 * nw + wave] = (uint32_t)__popcll(mc);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t total = 0;
+        for (uint32_t k = 0; k < 2 * nw; ++k) {
+            uint32_t c = lds[k];
+            lds[k] = total;
+            total += c;
+        }
+        lds[3 * nw] = total ? atomicAdd(counter_ab, total) : 0u;
+        total = 0;
+        for (uint32_t k = 2 * nw; k < 3 * nw; ++k) {
+            uint32_t c = lds[k];
+            lds[k] = total;
+            total += c;
+        }
+        lds[3 * nw + 1] = total ? atomicAdd(counter_c, total) : 0u;
+    }
+    __syncthreads();
+    const uint64_t below = (1ull << lane) - 1ull;
+    const uint3 r = make_uint3(lds[3 * nw] + lds[wave] + (uint32_t)__popcll(ma & below),
+                               lds[3 * nw] + lds[nw + wave] + (uint32_t)__popcll(mb & below),
+                               lds[3 * nw + 1] + lds[2 * nw + wave] + (uint32_t)__popcll(mc & below));
+    __syncthreads();  // the next call reuses lds
+    return r;
+}
 
 // ------------------------------------------------------------------ camera -
 // PerspectiveCamera.GenerateRay (SeeSharp/Cameras/PerspectiveCamera.cs:64-102), lensRadius = 0:
@@ -919,7 +956,7 @@ struct PathVertex {
 template <bool ADJ>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ADJ ? SSPT_SHADE_ADJ_WAVES : SSPT_SHADE_WAVES)))
 k_shade(DevScene S, Wave w, int buf, int max_nsr) {
-    __shared__ uint32_t app[2 * (256 / 64) + 1];  // block_append / block_append2 scratch
+    __shared__ uint32_t app[3 * (256 / 64) + 2];  // block_append / block_append2 / block_append3 scratch
     const uint32_t count = w.counters[buf];
     const int nb = buf ^ 1;
     for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += gridDim.x * blockDim.x) {
@@ -962,6 +999,10 @@ k_shade(DevScene S, Wave w, int buf, int max_nsr) {
         // NumShadowRays, uniform over the block as block_append needs (lanes of
         // scenes with fewer shadow rays simply do not take part).
         const int nsr_i = R.num_shadow_rays;
+        float4 no = make_float4(0.0f, 0.0f, 0.0f, 0.0f), nd = no;
+        bool cont = false;
+        uint32_t n_idx = 0;
+        bool cont_appended = false;  // the same in every thread (max_nsr is a kernel argument)
         for (int k = 0; k < max_nsr; ++k) {
             const bool nee_k = v.do_nee && k < nsr_i;
             float4 eo, ed, ec, eg = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -970,9 +1011,22 @@ k_shade(DevScene S, Wave w, int buf, int max_nsr) {
             float4 ao, ad, ac, ag = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             const bool area_ray = nee_k && R.num_emitters > 0 && v.area_next_event(&ao, &ad, &ac, &ag);
             wave_count(w.stats, kStatShadow, nee_k && R.num_emitters > 0);
-            // one block pass for both: the block's environment rays, then its area-light rays
-            const uint2 idx = block_append2(&w.counters[kCntShadow], env_ray, area_ray, app);
-            const uint32_t e_idx = idx.x, a_idx = idx.y;
+            // one block pass for both: the block's environment rays, then its area-light
+            // rays; the forward tracer's last pass also takes the continuation ray (its
+            // BSDF sample draws after the next-event samples, as in the reference)
+            uint32_t e_idx, a_idx;
+            if (!ADJ && k == max_nsr - 1) {
+                cont = v.cont && v.sample_continuation(&no, &nd);
+                const uint3 idx = block_append3(&w.counters[kCntShadow], env_ray, area_ray, &w.counters[nb], cont, app);
+                e_idx = idx.x;
+                a_idx = idx.y;
+                n_idx = idx.z;
+                cont_appended = true;
+            } else {
+                const uint2 idx = block_append2(&w.counters[kCntShadow], env_ray, area_ray, app);
+                e_idx = idx.x;
+                a_idx = idx.y;
+            }
             if (env_ray) {
                 w.s_o[e_idx] = eo;
                 w.s_d[e_idx] = ed;
@@ -993,8 +1047,7 @@ k_shade(DevScene S, Wave w, int buf, int max_nsr) {
             }
         }
 
-        float4 no, nd;
-        const bool cont = v.cont && v.sample_continuation(&no, &nd);
+        if (!cont_appended) cont = v.cont && v.sample_continuation(&no, &nd);
         if (cont) {
             if (ADJ) {
                 // d log of the sampled BSDF value (the sampling pdf is detached)
@@ -1011,7 +1064,7 @@ k_shade(DevScene S, Wave w, int buf, int max_nsr) {
                 }
             }
         }
-        uint32_t n_idx = block_append(&w.counters[nb], cont, app);
+        if (!cont_appended) n_idx = block_append(&w.counters[nb], cont, app);
         if (cont) {
             w.q_o[nb][n_idx] = no;
             w.q_d[nb][n_idx] = nd;
