@@ -43,7 +43,10 @@ constexpr uint32_t kMaxNodes = 1u << 25;
 #endif
 constexpr int kStackLds = SSPT_STACK_LDS;
 constexpr int kStackTotal = 64;
-constexpr int kTraceBlock = 128;
+#ifndef SSPT_TRACE_BLOCK
+#define SSPT_TRACE_BLOCK 128
+#endif
+constexpr int kTraceBlock = SSPT_TRACE_BLOCK;  // threads per traversal block
 // Leaves a lane may hold back for a later triangle phase (a power of two, at
 // least 4: one node step adds up to four). They sit in LDS after the lane's
 // stack entries, so a traversal kernel's node array holds kStackNodeLds ints.
