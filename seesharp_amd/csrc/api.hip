@@ -790,9 +790,9 @@ void enqueue_wave(sspt_ctx* ctx, sspt_scene* sc, const LaunchConfig& lc, int max
                 timed(ctx, adj ? "trace_adjoint" : "trace", [&] { launch_trace(st, S, w, lc, buf ^ 1, adj); });
                 launch_bounce_reset(st, w, buf ^ 1);
             } else if (adj) {
-                timed(ctx, "shadow_adjoint", [&] { launch_shadow_adjoint(st, S, w, lc); });
+                timed(ctx, "shadow_adjoint", [&] { launch_shadow_adjoint(st, S, w, lc, buf ^ 1); });
             } else {
-                timed(ctx, "shadow", [&] { launch_shadow(st, S, w, lc); });
+                timed(ctx, "shadow", [&] { launch_shadow(st, S, w, lc, buf ^ 1); });
             }
             buf ^= 1;
         }
@@ -802,10 +802,10 @@ void enqueue_wave(sspt_ctx* ctx, sspt_scene* sc, const LaunchConfig& lc, int max
             timed(ctx, "extend", [&] { launch_extend(st, S, w, lc, buf); });
             if (adj) {
                 timed(ctx, "shade_adjoint", [&] { launch_shade_adjoint(st, S, w, lc, buf); });
-                timed(ctx, "shadow_adjoint", [&] { launch_shadow_adjoint(st, S, w, lc); });
+                timed(ctx, "shadow_adjoint", [&] { launch_shadow_adjoint(st, S, w, lc, buf ^ 1); });
             } else {
                 timed(ctx, "shade", [&] { launch_shade(st, S, w, lc, buf); });
-                timed(ctx, "shadow", [&] { launch_shadow(st, S, w, lc); });
+                timed(ctx, "shadow", [&] { launch_shadow(st, S, w, lc, buf ^ 1); });
             }
             buf ^= 1;
         }

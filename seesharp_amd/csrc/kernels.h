@@ -66,7 +66,8 @@ void launch_path(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchC
 void launch_bounce_reset(hipStream_t s, const Wave& w, int buf);
 void launch_extend(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int buf);
 void launch_shade(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int buf);
-void launch_shadow(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc);
+// the shadow rays the shade pass appended with extension queue `slot`
+void launch_shadow(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int slot);
 // running mean of one scene (waves of whole iterations)
 void launch_accumulate_mean(hipStream_t s, const Wave& w, float* image, uint32_t num_pixels, float* albedo,
                             float* normal, unsigned long long* outliers, int num_outliers);
@@ -76,7 +77,7 @@ void launch_accumulate_sum(hipStream_t s, const DevScene& sc, const Wave& w, con
 void launch_wave_end(hipStream_t s, const Wave& w);
 // adjoint (path replay) variants
 void launch_shade_adjoint(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int buf);
-void launch_shadow_adjoint(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc);
+void launch_shadow_adjoint(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int slot);
 // extension queue `buf` and the shadow queue in one grid (forward or adjoint shadow)
 void launch_trace(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int buf, bool adjoint);
 void launch_adjoint_finalize(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc);

@@ -24,11 +24,17 @@
 #ifndef SSPT_SHADE_WAVES
 #define SSPT_SHADE_WAVES 2
 #endif
+#ifndef SSPT_SHADE_BLOCK
+#define SSPT_SHADE_BLOCK 256
+#endif
 #ifndef SSPT_SHADE_ADJ_WAVES
 #define SSPT_SHADE_ADJ_WAVES 2
 #endif
 
 namespace sspt {
+
+// threads per shade block: one queue atomic and one round of barriers per block and pass
+constexpr int kShadeBlock = SSPT_SHADE_BLOCK;
 
 // ------------------------------------------------------------ wave helpers -
 // Adds the wave's count of `pred` to statistic `which`, on the stripe of this wave.
@@ -42,60 +48,21 @@ __device__ __forceinline__ void wave_count(unsigned long long* stats, int which,
     if ((int)__lane_id() == leader)
         atomicAdd(&stats[stat_stripe() * kStatStride + which], (unsigned long long)__popcll(mask));
 }
-// Block-wide append: one atomic per block and call. Every thread of the block
-// must make the same calls (block-uniform control flow); `lds` holds one
-// uint32 per wave of the block plus one.
-__device__ __forceinline__ uint32_t block_append(uint32_t* counter, bool pred, uint32_t* lds) {
-    const uint64_t mask = __ballot(pred);
-    const uint32_t wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    const uint32_t lane = __lane_id();
-    if (lane == 0) lds[wave] = (uint32_t)__popcll(mask);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t total = 0;
-        for (uint32_t k = 0; k < nw; ++k) {
-            uint32_t c = lds[k];
-            lds[k] = total;
-            total += c;
-        }
-        lds[nw] = total ? atomicAdd(counter, total) : 0u;
-    }
-    __syncthreads();
-    const uint32_t r = lds[nw] + lds[wave] + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-    __syncthreads();  // the next call reuses lds
-    return r;
+// Queue counters: extension queue b's length and the shadow rays appended
+// with it, in one 64-bit word (low and high half).
+__device__ __forceinline__ uint32_t queue_count(const Wave& w, int b) { return w.counters[2 * b]; }
+__device__ __forceinline__ uint32_t shadow_count(const Wave& w, int b) { return w.counters[2 * b + 1]; }
+__device__ __forceinline__ unsigned long long* queue_word(const Wave& w, int b) {
+    return reinterpret_cast<unsigned long long*>(w.counters + 2 * b);
 }
-// Two block_appends to one counter in one pass: the block's `pa` entries,
-// then its `pb` entries. `lds` holds two uint32 per wave of the block plus one.
-__device__ __forceinline__ uint2 block_append2(uint32_t* counter, bool pa, bool pb, uint32_t* lds) {
-    const uint64_t ma = __ballot(pa), mb = __ballot(pb);
-    const uint32_t wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    const uint32_t lane = __lane_id();
-    if (lane == 0) {
-        lds[wave] = (uint32_t)__popcll(ma);
-        lds[nw + wave] = (uint32_t)__popcll(mb);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t total = 0;
-        for (uint32_t k = 0; k < 2 * nw; ++k) {
-            uint32_t c = lds[k];
-            lds[k] = total;
-            total += c;
-        }
-        lds[2 * nw] = total ? atomicAdd(counter, total) : 0u;
-    }
-    __syncthreads();
-    const uint64_t below = (1ull << lane) - 1ull;
-    const uint2 r = make_uint2(lds[2 * nw] + lds[wave] + (uint32_t)__popcll(ma & below),
-                               lds[2 * nw] + lds[nw + wave] + (uint32_t)__popcll(mb & below));
-    __syncthreads();  // the next call reuses lds
-    return r;
-}
-// block_append2 to `counter_ab` and a block_append of `pc` to `counter_c`, in
-// one pass. `lds` holds three uint32 per wave of the block plus two.
-__device__ __forceinline__ uint3 block_append3(uint32_t* counter_ab, bool pa, bool pb, uint32_t* counter_c, bool pc,
-                                               uint32_t* lds) {
+
+// Block-wide appends: one 64-bit atomic per block and call on a queue word,
+// the block's `pa` entries (then its `pb` entries) to the half at `shift`
+// (0: the extension queue, 32: the shadow rays), `pc` entries to the low half.
+// Every thread of the block must make the same calls (block-uniform control
+// flow); `lds` holds three uint32 per wave of the block plus two.
+__device__ __forceinline__ uint3 block_append_word(unsigned long long* word, int shift, bool pa, bool pb, bool pc,
+                                                   uint32_t* lds) {
     const uint64_t ma = __ballot(pa), mb = __ballot(pb), mc = __ballot(pc);
     const uint32_t wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const uint32_t lane = __lane_id();
@@ -106,20 +73,22 @@ __device__ __forceinline__ uint3 block_append3(uint32_t* counter_ab, bool pa, bo
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint32_t total = 0;
+        uint32_t ab = 0;
         for (uint32_t k = 0; k < 2 * nw; ++k) {
-            uint32_t c = lds[k];
-            lds[k] = total;
-            total += c;
+            const uint32_t c = lds[k];
+            lds[k] = ab;
+            ab += c;
         }
-        lds[3 * nw] = total ? atomicAdd(counter_ab, total) : 0u;
-        total = 0;
+        uint32_t cc = 0;
         for (uint32_t k = 2 * nw; k < 3 * nw; ++k) {
-            uint32_t c = lds[k];
-            lds[k] = total;
-            total += c;
+            const uint32_t c = lds[k];
+            lds[k] = cc;
+            cc += c;
         }
-        lds[3 * nw + 1] = total ? atomicAdd(counter_c, total) : 0u;
+        const unsigned long long add = ((unsigned long long)ab << shift) + (unsigned long long)cc;
+        const unsigned long long old = add ? atomicAdd(word, add) : 0ull;
+        lds[3 * nw] = (uint32_t)(old >> shift);
+        lds[3 * nw + 1] = (uint32_t)old;
     }
     __syncthreads();
     const uint64_t below = (1ull << lane) - 1ull;
@@ -502,15 +471,16 @@ __device__ __forceinline__ const WaveDesc& cur_wave(const Wave& w) { return w.wa
 __global__ void k_wave_begin(Wave w) {
     const WaveDesc& d = cur_wave(w);
     w.counters[kCntQueue0] = d.count;
+    w.counters[kCntShadow0] = 0;
     w.counters[kCntQueue1] = 0;
-    w.counters[kCntShadow] = 0;
+    w.counters[kCntShadow1] = 0;
 }
 
 __global__ void k_wave_end(Wave w) { w.counters[kCntWave] += 1; }
 
 __global__ void k_bounce_reset(Wave w, int next) {
-    w.counters[next] = 0;
-    w.counters[kCntShadow] = 0;
+    w.counters[2 * next] = 0;
+    w.counters[2 * next + 1] = 0;
 }
 
 // Scene of world-wide path g: last scene whose path offset is <= g.
@@ -593,7 +563,7 @@ __global__ void __launch_bounds__(256) k_generate(DevScene S, Wave w) {
 // grid of `nblocks` blocks that stride over the device-side queue length.
 __device__ __forceinline__ void extend_rays(const DevScene& S, const Wave& w, int buf, uint32_t block,
                                             uint32_t nblocks, int* stack_node, float* stack_t) {
-    const uint32_t count = w.counters[buf];
+    const uint32_t count = queue_count(w, buf);
     const float4* __restrict__ q_o = w.q_o[buf];
     const float4* __restrict__ q_d = w.q_d[buf];
     const uint32_t* __restrict__ q_slot = w.q_slot[buf];
@@ -617,9 +587,10 @@ __device__ __forceinline__ void extend_rays(const DevScene& S, const Wave& w, in
 // Any-hit tests of the shadow queue; a visible ray adds its contribution to
 // its path's next-event sum (and, with ADJ, its own gradient term).
 template <bool ADJ>
-__device__ __forceinline__ void shadow_rays(const DevScene& S, const Wave& w, uint32_t block, uint32_t nblocks,
+__device__ __forceinline__ void shadow_rays(const DevScene& S, const Wave& w, int slot, uint32_t block,
+                                            uint32_t nblocks,
                                             int* stack_node, float* stack_t) {
-    const uint32_t count = w.counters[kCntShadow];
+    const uint32_t count = shadow_count(w, slot);
     for (uint32_t base = block * blockDim.x; base < count; base += nblocks * blockDim.x) {
         uint32_t i = base + threadIdx.x;
         bool active = i < count;
@@ -666,10 +637,10 @@ k_extend(DevScene S, Wave w, int buf) {
 
 template <bool ADJ>
 __global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(SSPT_TRACE_WAVES)))
-k_shadow(DevScene S, Wave w) {
+k_shadow(DevScene S, Wave w, int slot) {
     __shared__ int stack_node[kStackNodeLds];
     __shared__ float stack_t[kStackLds * kTraceBlock];
-    shadow_rays<ADJ>(S, w, blockIdx.x, gridDim.x, stack_node, stack_t);
+    shadow_rays<ADJ>(S, w, slot, blockIdx.x, gridDim.x, stack_node, stack_t);
 }
 
 // One launch for the two traversals that are ready after a bounce's shade:
@@ -684,7 +655,7 @@ k_trace(DevScene S, Wave w, int buf, uint32_t extend_blocks) {
     if (blockIdx.x < extend_blocks)
         extend_rays(S, w, buf, blockIdx.x, extend_blocks, stack_node, stack_t);
     else
-        shadow_rays<ADJ>(S, w, blockIdx.x - extend_blocks, gridDim.x - extend_blocks, stack_node, stack_t);
+        shadow_rays<ADJ>(S, w, buf, blockIdx.x - extend_blocks, gridDim.x - extend_blocks, stack_node, stack_t);
 }
 
 // TinyEmbree SpawnRay / IsOccluded offset rule: move along the face normal by
@@ -954,10 +925,10 @@ struct PathVertex {
 
 // One bounce of EstimateIncidentRadiance for every path of the queue.
 template <bool ADJ>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ADJ ? SSPT_SHADE_ADJ_WAVES : SSPT_SHADE_WAVES)))
+__global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(ADJ ? SSPT_SHADE_ADJ_WAVES : SSPT_SHADE_WAVES)))
 k_shade(DevScene S, Wave w, int buf, int max_nsr) {
-    __shared__ uint32_t app[3 * (256 / 64) + 2];  // block_append / block_append2 / block_append3 scratch
-    const uint32_t count = w.counters[buf];
+    __shared__ uint32_t app[3 * (kShadeBlock / 64) + 2];  // block_append_word scratch
+    const uint32_t count = queue_count(w, buf);
     const int nb = buf ^ 1;
     for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += gridDim.x * blockDim.x) {
         uint32_t i = base + threadIdx.x;
@@ -996,7 +967,7 @@ k_shade(DevScene S, Wave w, int buf, int max_nsr) {
 
         // Next-event estimation (PathTracer.cs:356-364): per shadow ray,
         // environment then area light. The loop bound is the render's largest
-        // NumShadowRays, uniform over the block as block_append needs (lanes of
+        // NumShadowRays, uniform over the block as block_append_word needs (lanes of
         // scenes with fewer shadow rays simply do not take part).
         const int nsr_i = R.num_shadow_rays;
         float4 no = make_float4(0.0f, 0.0f, 0.0f, 0.0f), nd = no;
@@ -1017,13 +988,13 @@ k_shade(DevScene S, Wave w, int buf, int max_nsr) {
             uint32_t e_idx, a_idx;
             if (!ADJ && k == max_nsr - 1) {
                 cont = v.cont && v.sample_continuation(&no, &nd);
-                const uint3 idx = block_append3(&w.counters[kCntShadow], env_ray, area_ray, &w.counters[nb], cont, app);
+                const uint3 idx = block_append_word(queue_word(w, nb), 32, env_ray, area_ray, cont, app);
                 e_idx = idx.x;
                 a_idx = idx.y;
                 n_idx = idx.z;
                 cont_appended = true;
             } else {
-                const uint2 idx = block_append2(&w.counters[kCntShadow], env_ray, area_ray, app);
+                const uint3 idx = block_append_word(queue_word(w, nb), 32, env_ray, area_ray, false, app);
                 e_idx = idx.x;
                 a_idx = idx.y;
             }
@@ -1064,7 +1035,7 @@ k_shade(DevScene S, Wave w, int buf, int max_nsr) {
                 }
             }
         }
-        if (!cont_appended) n_idx = block_append(&w.counters[nb], cont, app);
+        if (!cont_appended) n_idx = block_append_word(queue_word(w, nb), 0, cont, false, false, app).x;
         if (cont) {
             w.q_o[nb][n_idx] = no;
             w.q_d[nb][n_idx] = nd;
@@ -1604,15 +1575,17 @@ void launch_extend(hipStream_t s, const DevScene& sc, const Wave& w, const Launc
                        sc, w, buf);
 }
 void launch_shade(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int buf) {
-    hipLaunchKernelGGL(k_shade<false>, dim3(grid_of(lc.shade_blocks, lc.max_paths, 256)), dim3(256), 0, s, sc, w,
+    hipLaunchKernelGGL(k_shade<false>, dim3(grid_of(lc.shade_blocks, lc.max_paths, kShadeBlock)), dim3(kShadeBlock), 0,
+                       s, sc, w,
                        buf, lc.max_nsr);
 }
-void launch_shadow(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc) {
+void launch_shadow(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int slot) {
     hipLaunchKernelGGL(k_shadow<false>, dim3(grid_of(lc.trace_blocks, lc.max_shadow, kTraceBlock)), dim3(kTraceBlock),
-                       0, s, sc, w);
+                       0, s, sc, w, slot);
 }
 void launch_shade_adjoint(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int buf) {
-    hipLaunchKernelGGL(k_shade<true>, dim3(grid_of(lc.shade_blocks, lc.max_paths, 256)), dim3(256), 0, s, sc, w,
+    hipLaunchKernelGGL(k_shade<true>, dim3(grid_of(lc.shade_blocks, lc.max_paths, kShadeBlock)), dim3(kShadeBlock), 0,
+                       s, sc, w,
                        buf, lc.max_nsr);
 }
 void launch_trace(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int buf, bool adjoint) {
@@ -1623,9 +1596,9 @@ void launch_trace(hipStream_t s, const DevScene& sc, const Wave& w, const Launch
     else
         hipLaunchKernelGGL(k_trace<false>, dim3(ne + ns), dim3(kTraceBlock), 0, s, sc, w, buf, ne);
 }
-void launch_shadow_adjoint(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc) {
+void launch_shadow_adjoint(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int slot) {
     hipLaunchKernelGGL(k_shadow<true>, dim3(grid_of(lc.trace_blocks, lc.max_shadow, kTraceBlock)), dim3(kTraceBlock),
-                       0, s, sc, w);
+                       0, s, sc, w, slot);
 }
 void launch_adjoint_finalize(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc) {
     hipLaunchKernelGGL(k_adjoint_finalize, dim3(grid_of(lc.shade_blocks, lc.max_paths, 256)), dim3(256), 0, s, sc, w);
@@ -1643,7 +1616,7 @@ void resident_blocks_per_cu(int* trace, int* shade) {
     int e = 0, sh = 0, sd = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&e, k_extend, kTraceBlock, 0) != hipSuccess) e = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&sh, k_shadow<false>, kTraceBlock, 0) != hipSuccess) sh = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&sd, k_shade<false>, 256, 0) != hipSuccess) sd = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&sd, k_shade<false>, kShadeBlock, 0) != hipSuccess) sd = 0;
     *trace = std::max(e, sh);
     *shade = sd;
 }

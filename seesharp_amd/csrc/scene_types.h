@@ -167,13 +167,17 @@ struct WaveDesc {
     uint32_t pad;
 };
 
-// Device-side counters.
+// Device-side counters. Extension queue b and the shadow rays of the shade
+// pass that fills it share the 64-bit word at counters + 2b (queue length in
+// the low half, shadow rays in the high half), so one 64-bit atomic appends
+// both.
 enum Counter {
-    kCntQueue0 = 0,  // extension queue 0 length
-    kCntQueue1 = 1,  // extension queue 1 length
-    kCntShadow = 2,  // shadow queue length
-    kCntWave = 3,    // index of the wave being rendered
-    kCntNum = 4
+    kCntQueue0 = 0,   // extension queue 0 length
+    kCntShadow0 = 1,  // shadow rays appended with queue 0
+    kCntQueue1 = 2,   // extension queue 1 length
+    kCntShadow1 = 3,  // shadow rays appended with queue 1
+    kCntWave = 4,     // index of the wave being rendered
+    kCntNum = 5
 };
 enum Stat {
     kStatRays = 0,
