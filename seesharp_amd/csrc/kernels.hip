@@ -61,29 +61,33 @@ __device__ __forceinline__ unsigned long long* queue_word(const Wave& w, int b) 
 // (0: the extension queue, 32: the shadow rays), `pc` entries to the low half.
 // Every thread of the block must make the same calls (block-uniform control
 // flow); `lds` holds three uint32 per wave of the block plus two.
+// HAS_B / HAS_C: whether the call has `pb` / `pc` entries at all.
+template <bool HAS_B, bool HAS_C>
 __device__ __forceinline__ uint3 block_append_word(unsigned long long* word, int shift, bool pa, bool pb, bool pc,
                                                    uint32_t* lds) {
-    const uint64_t ma = __ballot(pa), mb = __ballot(pb), mc = __ballot(pc);
+    const uint64_t ma = __ballot(pa), mb = HAS_B ? __ballot(pb) : 0ull, mc = HAS_C ? __ballot(pc) : 0ull;
     const uint32_t wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const uint32_t lane = __lane_id();
     if (lane == 0) {
         lds[wave] = (uint32_t)__popcll(ma);
-        lds[nw + wave] = (uint32_t)__popcll(mb);
-        lds[2 * nw + wave] = (uint32_t)__popcll(mc);
+        if (HAS_B) lds[nw + wave] = (uint32_t)__popcll(mb);
+        if (HAS_C) lds[2 * nw + wave] = (uint32_t)__popcll(mc);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t ab = 0;
-        for (uint32_t k = 0; k < 2 * nw; ++k) {
+        for (uint32_t k = 0; k < (HAS_B ? 2 : 1) * nw; ++k) {
             const uint32_t c = lds[k];
             lds[k] = ab;
             ab += c;
         }
         uint32_t cc = 0;
-        for (uint32_t k = 2 * nw; k < 3 * nw; ++k) {
-            const uint32_t c = lds[k];
-            lds[k] = cc;
-            cc += c;
+        if (HAS_C) {
+            for (uint32_t k = 2 * nw; k < 3 * nw; ++k) {
+                const uint32_t c = lds[k];
+                lds[k] = cc;
+                cc += c;
+            }
         }
         const unsigned long long add = ((unsigned long long)ab << shift) + (unsigned long long)cc;
         const unsigned long long old = add ? atomicAdd(word, add) : 0ull;
@@ -93,8 +97,8 @@ __device__ __forceinline__ uint3 block_append_word(unsigned long long* word, int
     __syncthreads();
     const uint64_t below = (1ull << lane) - 1ull;
     const uint3 r = make_uint3(lds[3 * nw] + lds[wave] + (uint32_t)__popcll(ma & below),
-                               lds[3 * nw] + lds[nw + wave] + (uint32_t)__popcll(mb & below),
-                               lds[3 * nw + 1] + lds[2 * nw + wave] + (uint32_t)__popcll(mc & below));
+                               HAS_B ? lds[3 * nw] + lds[nw + wave] + (uint32_t)__popcll(mb & below) : 0u,
+                               HAS_C ? lds[3 * nw + 1] + lds[2 * nw + wave] + (uint32_t)__popcll(mc & below) : 0u);
     __syncthreads();  // the next call reuses lds
     return r;
 }
@@ -988,13 +992,13 @@ k_shade(DevScene S, Wave w, int buf, int max_nsr) {
             uint32_t e_idx, a_idx;
             if (!ADJ && k == max_nsr - 1) {
                 cont = v.cont && v.sample_continuation(&no, &nd);
-                const uint3 idx = block_append_word(queue_word(w, nb), 32, env_ray, area_ray, cont, app);
+                const uint3 idx = block_append_word<true, true>(queue_word(w, nb), 32, env_ray, area_ray, cont, app);
                 e_idx = idx.x;
                 a_idx = idx.y;
                 n_idx = idx.z;
                 cont_appended = true;
             } else {
-                const uint3 idx = block_append_word(queue_word(w, nb), 32, env_ray, area_ray, false, app);
+                const uint3 idx = block_append_word<true, false>(queue_word(w, nb), 32, env_ray, area_ray, false, app);
                 e_idx = idx.x;
                 a_idx = idx.y;
             }
@@ -1035,7 +1039,7 @@ k_shade(DevScene S, Wave w, int buf, int max_nsr) {
                 }
             }
         }
-        if (!cont_appended) n_idx = block_append_word(queue_word(w, nb), 0, cont, false, false, app).x;
+        if (!cont_appended) n_idx = block_append_word<false, false>(queue_word(w, nb), 0, cont, false, false, app).x;
         if (cont) {
             w.q_o[nb][n_idx] = no;
             w.q_d[nb][n_idx] = nd;
