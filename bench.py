@@ -43,6 +43,8 @@ BYTES_SHADE_PER_RAY = 64
 BYTES_SHADE_PER_HIT = 100
 BYTES_GENERATE_PER_PATH = 124   # queue entry (36 B) + path state (88 B)
 BYTES_ACCUMULATE_PER_PATH = 72  # path radiance (48 B) + pixel read and write (24 B)
+# the fused primary launch: path state (88 B) + queue entry without the origin (20 B) + hit record (16 B)
+BYTES_PRIMARY_PER_PATH = 124
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "pt_large_scene_hbm_traffic.md")
 
 
@@ -498,7 +500,8 @@ def main():
         # bounce's continuation rays with the previous bounce's shadow rays, and
         # "shadow" the last bounce's shadow queue (empty: the last bounce samples
         # no next events)
-        alg["extend"] = BYTES_EXTEND_PER_RAY * npaths
+        # "extend" starts the paths and traces their camera rays in one launch
+        alg["extend"] = BYTES_PRIMARY_PER_PATH * npaths
         alg["trace"] = BYTES_EXTEND_PER_RAY * (n_ext * per - npaths) + BYTES_SHADOW_PER_RAY * n_sh * per
         alg["shadow"] = 0.0
     else:

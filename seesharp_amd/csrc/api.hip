@@ -774,13 +774,13 @@ void enqueue_wave(sspt_ctx* ctx, sspt_scene* sc, const LaunchConfig& lc, int max
         launch_wave_end(st, w);
         return;
     }
-    timed(ctx, "generate", [&] { launch_generate(st, S, w, lc); });
     int buf = 0;
     if (SSPT_FUSE_TRACE) {
-        // bounce d's shade is followed by one launch tracing the continuation
-        // rays of bounce d + 1 and the shadow rays of bounce d
+        // the paths start and their camera rays are traced in one launch; bounce
+        // d's shade is followed by one launch tracing the continuation rays of
+        // bounce d + 1 and the shadow rays of bounce d
         launch_bounce_reset(st, w, buf);
-        timed(ctx, "extend", [&] { launch_extend(st, S, w, lc, buf); });
+        timed(ctx, "extend", [&] { launch_extend_primary(st, S, w, lc); });
         for (int depth = 1; depth <= max_depth; ++depth) {
             if (adj)
                 timed(ctx, "shade_adjoint", [&] { launch_shade_adjoint(st, S, w, lc, buf); });
@@ -797,6 +797,7 @@ void enqueue_wave(sspt_ctx* ctx, sspt_scene* sc, const LaunchConfig& lc, int max
             buf ^= 1;
         }
     } else {
+        timed(ctx, "generate", [&] { launch_generate(st, S, w, lc); });
         for (int depth = 1; depth <= max_depth; ++depth) {
             launch_bounce_reset(st, w, buf);
             timed(ctx, "extend", [&] { launch_extend(st, S, w, lc, buf); });

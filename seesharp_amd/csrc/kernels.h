@@ -65,6 +65,8 @@ void launch_generate(hipStream_t s, const DevScene& sc, const Wave& w, const Lau
 void launch_path(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc);
 void launch_bounce_reset(hipStream_t s, const Wave& w, int buf);
 void launch_extend(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int buf);
+// launch_generate and launch_extend(buf 0) in one kernel
+void launch_extend_primary(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc);
 void launch_shade(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int buf);
 // the shadow rays the shade pass appended with extension queue `slot`
 void launch_shadow(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int slot);

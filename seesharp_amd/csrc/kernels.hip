@@ -541,25 +541,31 @@ __device__ __forceinline__ PathStart start_path(const DevScene& S, uint64_t g) {
     return p;
 }
 
-// The first extension ray and the initial PathState (PathTracer.cs:293-302) of every path of the wave.
+// The initial PathState (PathTracer.cs:293-302) of the path in `slot`, and
+// its entry in extension queue 0 (all but the ray origin).
+__device__ __forceinline__ void init_path(const DevScene& S, const Wave& w, uint32_t slot, const PathStart& p) {
+    w.q_d[0][slot] = f4_of(p.dir, INFINITY);
+    w.q_slot[0][slot] = slot | ((uint32_t)p.scene << kSlotBits);
+    w.st_pix[slot] = p.pix | ((uint32_t)p.scene << kSlotBits);
+    w.q_rng[0][slot] = p.rng.state;
+    w.q_thr[0][slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    w.q_prev[0][slot] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(1));
+    w.q_L[0][slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    w.st_Le[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    w.st_La[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (S.recs[p.scene].enable_denoiser) {
+        w.st_alb[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        w.st_nrm[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+}
+
+// The first extension ray and the initial PathState of every path of the wave.
 __global__ void __launch_bounds__(256) k_generate(DevScene S, Wave w) {
     const WaveDesc d = cur_wave(w);
     for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < d.count; slot += gridDim.x * blockDim.x) {
         PathStart p = start_path(S, d.start + slot);
         w.q_o[0][slot] = f4_of(p.o, 0.0f);
-        w.q_d[0][slot] = f4_of(p.dir, INFINITY);
-        w.q_slot[0][slot] = slot | ((uint32_t)p.scene << kSlotBits);
-        w.st_pix[slot] = p.pix | ((uint32_t)p.scene << kSlotBits);
-        w.q_rng[0][slot] = p.rng.state;
-        w.q_thr[0][slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-        w.q_prev[0][slot] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(1));
-        w.q_L[0][slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        w.st_Le[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        w.st_La[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (S.recs[p.scene].enable_denoiser) {
-            w.st_alb[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            w.st_nrm[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        }
+        init_path(S, w, slot, p);
     }
 }
 
@@ -629,6 +635,32 @@ __device__ __forceinline__ void shadow_rays(const DevScene& S, const Wave& w, in
         }
         wave_count(w.stats, kStatShadowTraced, active);
         wave_count(w.stats, kStatOccluded, active && occluded);
+    }
+}
+
+// k_generate and the closest hits of its queue in one pass: each thread starts
+// its path (RNG, jitter, camera ray), writes the path state and traces the
+// camera ray at once, so the ray never goes through the queue.
+__global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(SSPT_TRACE_WAVES)))
+k_extend_primary(DevScene S, Wave w) {
+    __shared__ int stack_node[kStackNodeLds];
+    __shared__ float stack_t[kStackLds * kTraceBlock];
+    const WaveDesc d = cur_wave(w);
+    for (uint32_t base = blockIdx.x * blockDim.x; base < d.count; base += gridDim.x * blockDim.x) {
+        const uint32_t slot = base + threadIdx.x;
+        const bool active = slot < d.count;
+        HitOut h{0.0f, 0.0f, 0.0f, -1, 0};
+        if (active) {
+            const PathStart p = start_path(S, d.start + slot);
+            init_path(S, w, slot, p);
+            const SceneRec& R = S.recs[p.scene];
+            h = traverse<false>(S, R.root, R.abs_max, p.o, p.dir, 0.0f, INFINITY, stack_node + threadIdx.x,
+                                stack_t + threadIdx.x);
+            w.hits[slot] = h.li >= 0 ? make_float4(h.t, h.u, h.v, __int_as_float(h.li))
+                                     : make_float4(INFINITY, 0.0f, 0.0f, __int_as_float(-1));
+        }
+        wave_count(w.stats, kStatRays, active);
+        wave_count(w.stats, kStatHits, active && h.li >= 0);
     }
 }
 
@@ -1573,6 +1605,10 @@ void launch_path(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchC
 
 void launch_generate(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc) {
     hipLaunchKernelGGL(k_generate, dim3(grid_of(lc.shade_blocks, lc.max_paths, 256)), dim3(256), 0, s, sc, w);
+}
+void launch_extend_primary(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc) {
+    hipLaunchKernelGGL(k_extend_primary, dim3(grid_of(lc.trace_blocks, lc.max_paths, kTraceBlock)), dim3(kTraceBlock),
+                       0, s, sc, w);
 }
 void launch_extend(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int buf) {
     hipLaunchKernelGGL(k_extend, dim3(grid_of(lc.trace_blocks, lc.max_paths, kTraceBlock)), dim3(kTraceBlock), 0, s,
