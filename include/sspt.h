@@ -377,7 +377,8 @@ sspt_status sspt_render_debug(sspt_ctx* ctx, sspt_scene* scene, uint32_t index, 
 /* Enables HIP-event timing of every kernel launch on the context stream. */
 sspt_status sspt_set_kernel_timing(sspt_ctx* ctx, int32_t enable);
 /* Accumulated launches and milliseconds of kernel `name` since the last reset:
- * "generate", "extend" (primary rays), "shade", "trace" (a bounce's
+ * "extend" (the paths' start and their camera rays; with the unfused schedule
+ * "generate" and "extend"), "shade", "trace" (a bounce's
  * continuation rays with the previous bounce's shadow rays), "shadow" (the
  * last bounce's shadow rays), "accumulate", "path" (single-kernel tracer),
  * and "shade_adjoint", "trace_adjoint", "shadow_adjoint", "adjoint_finalize". */
