@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define SSPT_ABI_VERSION 3
+#define SSPT_ABI_VERSION 4
 
 typedef enum sspt_status {
     SSPT_OK = 0,
@@ -139,7 +139,11 @@ typedef struct sspt_pt_settings {
     int32_t min_depth;           /* MinDepth */
     int32_t num_shadow_rays;     /* NumShadowRays */
     int32_t enable_bsdf_di;      /* EnableBsdfDI */
-    int64_t max_render_time_ms;  /* MaximumRenderTimeMs, < 0 = unlimited */
+    int64_t max_render_time_ms;  /* MaximumRenderTimeMs, < 0 = unlimited. With a budget sspt_render
+                                    renders one iteration per wave and, before each, stops once the
+                                    whole milliseconds spent so far plus their average per iteration
+                                    exceed it (RenderTimer, PathTracer.cs:221-227); stats.num_iterations
+                                    says how many it rendered */
     int32_t enable_denoiser;     /* EnableDenoiser: writes the "albedo" and "normal" layers at primary
                                     hits (DenoiseBuffers.LogPrimaryHit, PathTracer.cs:336-339); the
                                     OIDN denoiser itself is not run */
@@ -156,6 +160,11 @@ typedef struct sspt_ray_stats {
     uint64_t num_occluded;
     uint64_t num_ray_hits;
     uint64_t num_shadow_rays_traced;
+    /* Iterations the call rendered: sspt_render renders all it was asked for
+     * unless max_render_time_ms stopped it earlier (the FrameBuffer's
+     * CurIteration / NumIterations); sspt_render_adjoint the iterations it
+     * replayed; the batch calls the largest total_spp of their scenes. */
+    uint64_t num_iterations;
 } sspt_ray_stats;
 
 /* Closest-hit record of TinyEmbree `Hit` for the trace entry points. */

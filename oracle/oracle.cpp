@@ -1635,6 +1635,21 @@ void or_material_eval(const sspt_material_desc* mat, const sspt_texture_desc* te
     delete s;
 }
 
+void or_material_eval_many(const sspt_material_desc* mat, const sspt_texture_desc* textures,
+                           const float* normal, const float* uv, const float* out_dir, int64_t n,
+                           const float* in_dirs, float* value_cos, float* pdf) {
+    SurfacePoint p;
+    or_scene* s = flat_surface(mat, textures, normal, uv, &p);
+    Shader sh = make_shader(*s, p, v3(out_dir[0], out_dir[1], out_dir[2]));
+    for (int64_t i = 0; i < n; ++i) {
+        V3 in = v3(in_dirs[3 * i], in_dirs[3 * i + 1], in_dirs[3 * i + 2]);
+        Rgb vc = eval_cos(sh, in);
+        value_cos[3 * i] = vc.x, value_cos[3 * i + 1] = vc.y, value_cos[3 * i + 2] = vc.z;
+        if (pdf) pdf[i] = pdf_fwd(sh, in);
+    }
+    delete s;
+}
+
 void or_material_sample(const sspt_material_desc* mat, const sspt_texture_desc* textures,
                         const float* normal, const float* uv, const float* out_dir, float pc,
                         const float* p2, float* out) {

@@ -52,6 +52,11 @@ float or_fresnel_dielectric(float cos_i, float eta_i, float eta_t);
 void or_material_eval(const sspt_material_desc* mat, const sspt_texture_desc* textures,
                       const float* normal3, const float* uv2, const float* out_dir3,
                       const float* in_dir3, float* value3, float* value_cos3, float* pdf);
+/* f * |cos| (3 floats each) and the forward pdf (may be NULL) of n incident
+ * directions in_dirs (3 floats each) for one outgoing direction. */
+void or_material_eval_many(const sspt_material_desc* mat, const sspt_texture_desc* textures,
+                           const float* normal3, const float* uv2, const float* out_dir3, int64_t n,
+                           const float* in_dirs, float* value_cos, float* pdf);
 /* sample_out: dir3, pdf, weight3 -> 7 floats */
 void or_material_sample(const sspt_material_desc* mat, const sspt_texture_desc* textures,
                         const float* normal3, const float* uv2, const float* out_dir3,

@@ -13,7 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 # different occupancy targets); the default is the in-tree build.
 LIB_PATH = os.environ.get("SSPT_LIBRARY") or os.path.join(_HERE, "libsspt.so")
 
-ABI_VERSION = 3  # SSPT_ABI_VERSION of include/sspt.h
+ABI_VERSION = 4  # SSPT_ABI_VERSION of include/sspt.h
 SSPT_OK = 0
 INVALID_ARGUMENT = 1
 STATUS_NAMES = {
@@ -127,6 +127,7 @@ class RayStats(C.Structure):
         ("num_occluded", C.c_uint64),
         ("num_ray_hits", C.c_uint64),
         ("num_shadow_rays_traced", C.c_uint64),
+        ("num_iterations", C.c_uint64),
     ]
 
     def as_dict(self):

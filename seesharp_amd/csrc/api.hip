@@ -827,7 +827,7 @@ void enqueue_wave(sspt_ctx* ctx, sspt_scene* sc, const LaunchConfig& lc, int max
 // or graphs off), or one wave captured as a graph and replayed.
 sspt_status run_waves(sspt_ctx* ctx, sspt_scene* sc, const std::vector<WaveDesc>& waves, int max_depth,
                       int max_nsr, Mode mode, float* mean_image, uint32_t mean_pixels, float* mean_albedo,
-                      int64_t budget_ms) {
+                      int64_t budget_ms, uint32_t* waves_run = nullptr) {
     uint32_t max_paths = 0;
     for (const WaveDesc& d : waves) max_paths = std::max(max_paths, d.count);
     const LaunchConfig lc = launch_config(ctx, max_paths, (uint32_t)std::max(1, max_nsr));
@@ -848,18 +848,20 @@ sspt_status run_waves(sspt_ctx* ctx, sspt_scene* sc, const std::vector<WaveDesc>
         (void)hipGraphDestroy(graph);
         if (e != hipSuccess) return fail(ctx, SSPT_ERR_DEVICE, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
     }
-    auto t0 = std::chrono::steady_clock::now();
+    // MaximumRenderTimeMs (PathTracer.cs:221-227) with RenderTimer's arithmetic
+    // (SeeSharp/Integrators/Util/RenderTimer.cs:35-60): RenderTime sums each
+    // iteration's whole elapsed milliseconds, PerIterationCost = RenderTime / done,
+    // and the next wave runs only while RenderTime + PerIterationCost <= budget.
+    // Budgeted renders pass waves of one iteration, and each is waited for.
+    int64_t render_ms = 0;
+    uint32_t done = 0;
     sspt_status st = SSPT_OK;
     for (size_t i = 0; i < waves.size(); ++i) {
-        if (budget_ms >= 0 && i > 0) {
-            // MaximumRenderTimeMs (PathTracer.cs:221-227), checked per wave
-            hipError_t e = hipStreamSynchronize(ctx->stream);
-            if (e != hipSuccess) {
-                st = fail(ctx, SSPT_ERR_DEVICE, "render failed: %s", hipGetErrorString(e));
-                break;
-            }
-            double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-            if (ms + ms / (double)i > (double)budget_ms) break;
+        std::chrono::steady_clock::time_point t0;
+        if (budget_ms >= 0) {
+            const int64_t per_iteration = done ? render_ms / (int64_t)done : 0;
+            if (render_ms + per_iteration > budget_ms) break;
+            t0 = std::chrono::steady_clock::now();
         }
         if (graphs) {
             hipError_t e = hipGraphLaunch(exec, ctx->stream);
@@ -870,6 +872,16 @@ sspt_status run_waves(sspt_ctx* ctx, sspt_scene* sc, const std::vector<WaveDesc>
         } else {
             enqueue_wave(ctx, sc, lc, max_depth, mode, mean_image, mean_pixels, mean_albedo);
         }
+        ++done;
+        if (budget_ms >= 0) {
+            hipError_t e = hipStreamSynchronize(ctx->stream);
+            if (e != hipSuccess) {
+                st = fail(ctx, SSPT_ERR_DEVICE, "render failed: %s", hipGetErrorString(e));
+                break;
+            }
+            render_ms += std::chrono::duration_cast<std::chrono::milliseconds>(std::chrono::steady_clock::now() - t0)
+                             .count();
+        }
         if (ctx->timing.enabled && (i % 4 == 3)) {
             (void)hipStreamSynchronize(ctx->stream);
             ctx->timing.collect();
@@ -877,6 +889,7 @@ sspt_status run_waves(sspt_ctx* ctx, sspt_scene* sc, const std::vector<WaveDesc>
     }
     hipError_t e = hipStreamSynchronize(ctx->stream);
     if (exec) (void)hipGraphExecDestroy(exec);
+    if (waves_run) *waves_run = done;
     if (st != SSPT_OK) return st;
     if (e != hipSuccess) return fail(ctx, SSPT_ERR_DEVICE, "render failed: %s", hipGetErrorString(e));
     HIP_TRY(ctx, hipGetLastError());
@@ -898,6 +911,7 @@ sspt_status read_stats(sspt_ctx* ctx, sspt_scene* sc, sspt_ray_stats* stats) {
         stats->num_occluded = hs[kStatOccluded];
         stats->num_ray_hits = hs[kStatHits];
         stats->num_shadow_rays_traced = hs[kStatShadowTraced];
+        stats->num_iterations = 0;
     }
     return SSPT_OK;
 }
@@ -1124,8 +1138,10 @@ sspt_status sspt_render(sspt_ctx* ctx, sspt_scene* sc, const sspt_pt_settings* c
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     SceneRec& R = sc->recs[0];
     const uint32_t np = R.num_pixels;
-    // waves of whole iterations (running mean), as many as fit the wave capacity
+    // waves of whole iterations (running mean), as many as fit the wave capacity;
+    // one iteration per wave under a time budget, which is checked between waves
     uint32_t per_wave = std::max(1u, std::min(num_iterations, wave_capacity(ctx) / std::max(1u, np)));
+    if (cfg->max_render_time_ms >= 0) per_wave = 1;
     uint32_t cap = np * per_wave;
     TRY(ensure_wave(ctx, cap));
     TRY(ensure_shadow(ctx, (size_t)cap * 2 * std::max(1, cfg->num_shadow_rays)));
@@ -1136,6 +1152,7 @@ sspt_status sspt_render(sspt_ctx* ctx, sspt_scene* sc, const sspt_pt_settings* c
         done += it;
     }
     HIP_TRY(ctx, hipMemsetAsync(ctx->wave.stats, 0, kStatBytes, ctx->stream));
+    uint32_t waves_run = 0;
     if (!waves.empty()) {
         TRY(ensure_wave_descs(ctx, waves.size()));
         apply_settings(R, *cfg);
@@ -1146,9 +1163,11 @@ sspt_status sspt_render(sspt_ctx* ctx, sspt_scene* sc, const sspt_pt_settings* c
         HIP_TRY(ctx, hipMemcpyAsync(sc->d_path_offsets, offs, sizeof(offs), hipMemcpyHostToDevice, ctx->stream));
         float* mean_albedo = cfg->enable_denoiser ? sc->layers + R.fb_off : nullptr;
         TRY(run_waves(ctx, sc, waves, cfg->max_depth, cfg->num_shadow_rays, Mode::Mean, sc->image + R.fb_off, np,
-                      mean_albedo, cfg->max_render_time_ms));
+                      mean_albedo, cfg->max_render_time_ms, &waves_run));
     }
-    return read_stats(ctx, sc, stats);
+    TRY(read_stats(ctx, sc, stats));
+    if (stats) stats->num_iterations = (uint64_t)std::min(num_iterations, waves_run * per_wave);
+    return SSPT_OK;
 }
 
 // Lays the paths of every scene of the world end to end (scene, then iteration,
@@ -1217,7 +1236,11 @@ sspt_status sspt_render_batch(sspt_ctx* ctx, sspt_scene* sc, const sspt_pt_setti
         }
         HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     }
-    return read_stats(ctx, sc, stats);
+    TRY(read_stats(ctx, sc, stats));
+    if (stats)
+        for (size_t i = 0; i < sc->recs.size(); ++i)
+            stats->num_iterations = std::max<uint64_t>(stats->num_iterations, (uint64_t)std::max(0, cfgs[i].total_spp));
+    return SSPT_OK;
 }
 
 sspt_status sspt_render_adjoint_batch(sspt_ctx* ctx, sspt_scene* sc, const sspt_pt_settings* cfgs,
@@ -1234,7 +1257,11 @@ sspt_status sspt_render_adjoint_batch(sspt_ctx* ctx, sspt_scene* sc, const sspt_
                                 hipMemcpyHostToDevice, ctx->stream));
     if (!plan.waves.empty())
         TRY(run_waves(ctx, sc, plan.waves, plan.max_depth, plan.max_nsr, Mode::Adjoint, nullptr, 0, nullptr, -1));
-    return read_stats(ctx, sc, stats);
+    TRY(read_stats(ctx, sc, stats));
+    if (stats)
+        for (size_t i = 0; i < sc->recs.size(); ++i)
+            stats->num_iterations = std::max<uint64_t>(stats->num_iterations, (uint64_t)std::max(0, cfgs[i].total_spp));
+    return SSPT_OK;
 }
 
 sspt_status sspt_render_adjoint(sspt_ctx* ctx, sspt_scene* sc, const sspt_pt_settings* cfg,
@@ -1273,7 +1300,9 @@ sspt_status sspt_render_adjoint(sspt_ctx* ctx, sspt_scene* sc, const sspt_pt_set
         HIP_TRY(ctx, hipMemcpyAsync(sc->d_path_offsets, offs, sizeof(offs), hipMemcpyHostToDevice, ctx->stream));
         TRY(run_waves(ctx, sc, waves, cfg->max_depth, cfg->num_shadow_rays, Mode::Adjoint, nullptr, 0, nullptr, -1));
     }
-    return read_stats(ctx, sc, stats);
+    TRY(read_stats(ctx, sc, stats));
+    if (stats) stats->num_iterations = num_iterations;
+    return SSPT_OK;
 }
 
 sspt_status sspt_gradient_clear(sspt_ctx* ctx, sspt_scene* sc) {

@@ -433,22 +433,35 @@ class GpuPathTracer(Integrator):
         gs.clear()
         cfg = self.settings()
         if getattr(fb, "ReferenceImage", None) is None:
+            # the library checks MaximumRenderTimeMs before every iteration
             t0 = time.perf_counter()
             stats = gs.render(cfg, 0, self.TotalSpp)
             render_ms = (time.perf_counter() - t0) * 1000.0
+            iterations = stats["num_iterations"]
         else:
             # FrameBuffer.EndIteration with a reference image (FrameBuffer.cs:267-291):
-            # one error entry per iteration, timed over the render calls only
+            # one error entry per iteration, timed over the render calls only; the time
+            # budget follows RenderTimer (PathTracer.cs:221-227, RenderTimer.cs:35-60):
+            # whole milliseconds per iteration, and the next iteration only while the
+            # time so far plus its average per iteration stays within the budget
             stats = {k: 0 for k in ("num_rays", "num_shadow_rays", "num_occluded", "num_ray_hits",
                                     "num_shadow_rays_traced")}
             render_ms = 0.0
+            whole_ms = 0
+            iterations = 0
             fb.Errors = []
+            one = self.settings()
+            one.max_render_time_ms = -1
             for it in range(self.TotalSpp):
-                if self.MaximumRenderTimeMs is not None and it > 0 and render_ms >= self.MaximumRenderTimeMs:
+                per_iteration = whole_ms // iterations if iterations else 0
+                if self.MaximumRenderTimeMs is not None and whole_ms + per_iteration > self.MaximumRenderTimeMs:
                     break
                 t0 = time.perf_counter()
-                st = gs.render(cfg, it, 1)
-                render_ms += (time.perf_counter() - t0) * 1000.0
+                st = gs.render(one, it, 1)
+                elapsed = (time.perf_counter() - t0) * 1000.0
+                render_ms += elapsed
+                whole_ms += int(elapsed)
+                iterations += 1
                 for k in stats:
                     stats[k] += st[k]
                 fb.Errors.append(metrics.ErrorMetric(render_ms, gs.read_image(), fb.ReferenceImage))
@@ -457,10 +470,10 @@ class GpuPathTracer(Integrator):
             fb.Layers["albedo"] = gs.read_layer("albedo")
             fb.Layers["normal"] = gs.read_layer("normal")
         fb.Outliers = gs.read_outliers() if n_outl else None
-        fb.CurIteration = self.TotalSpp
+        fb.CurIteration = iterations
         fb.NaNWarnings = gs.nan_count()
         fb.MetaData["RenderTime"] = render_ms
-        fb.MetaData["NumIterations"] = self.TotalSpp
+        fb.MetaData["NumIterations"] = iterations
         fb.MetaData["RayTracerStats"] = {
             "NumRays": stats["num_rays"], "NumShadowRays": stats["num_shadow_rays"],
             "NumOccluded": stats["num_occluded"], "NumRayHits": stats["num_ray_hits"]}

@@ -41,6 +41,8 @@ _SIGS = {
                                 _fp, _fp, _fp, _fp]),
     "or_material_sample": (None, [C.POINTER(abi.MaterialDesc), C.POINTER(abi.TextureDesc), _fp, _fp, _fp,
                                   C.c_float, _fp, _fp]),
+    "or_material_eval_many": (None, [C.POINTER(abi.MaterialDesc), C.POINTER(abi.TextureDesc), _fp, _fp, _fp,
+                                     C.c_int64, _fp, _fp, _fp]),
     "or_material_integrate": (None, [C.POINTER(abi.MaterialDesc), C.POINTER(abi.TextureDesc), _fp, _fp, _fp,
                                      C.c_int64, C.c_uint32, C.c_int32, C.POINTER(C.c_double),
                                      C.POINTER(C.c_double)]),

@@ -86,6 +86,7 @@ def test_box_render_matches_oracle(box):
     sg = g.render(cfg, 0, 16)
     img_g = g.read_image()
     img_o, so, nan_o = o.render(cfg, 0, 16)
+    print(f"box 128^2 16 spp: relMSE {relmse(img_g, img_o):.3e}")
     assert g.nan_count() == nan_o == 0
     assert abs(img_g.mean() - img_o.mean()) <= 0.01 * img_o.mean()
     assert relmse(img_g, img_o) <= 1e-3
@@ -153,6 +154,7 @@ def test_large_render_matches_oracle(large):
     g.render(cfg, 0, 4)
     img_g = g.read_image()
     img_o, _, _ = o.render(cfg, 0, 4)
+    print(f"large 192^2 4 spp: relMSE {relmse(img_g, img_o):.3e}")
     assert abs(img_g.mean() - img_o.mean()) <= 0.01 * img_o.mean()
     assert relmse(img_g, img_o) <= 1e-3
 
@@ -163,16 +165,20 @@ def test_large_converged_relmse_against_cpu_reference(device):
     (tests/golden/make_converged_reference.py):
 
     * same seed as the GPU (0xC030114): the GPU follows the CPU path for path,
-      relMSE <= 1e-3;
-    * independent seed (571298512): the GPU's relMSE exceeds the CPU-vs-CPU relMSE
-      of the same two seeds by at most 25 % (no error beyond Monte Carlo noise).
-      The full relMSE of two independent 4096-spp images of this scene is ~7e-3
-      (fireflies: 1 % of the pixels hold most of it), so relMSE <= 1e-3 is
-      asserted with the worst 1 % of pixels rejected, as the reference's
-      RelMSE_OutlierRejection does (SeeSharp/Images/FrameBuffer.cs:378-383);
+      relMSE <= 1e-3 against the 4096-spp CPU reference (the primary gate; the
+      observed value is printed);
+    * independent seed (571298512): no error beyond Monte Carlo noise. Two
+      independent 4096-spp CPU images of these pixels differ by relMSE 6.7e-3
+      (2.6e-3 with 0.1 % of the pixels rejected: fireflies), so at this sample
+      count an independent-seed relMSE below 1e-3 is out of reach for any
+      renderer; instead the GPU's relMSE against the reference may exceed that
+      CPU-vs-CPU floor by at most 25 %, both plain and with
+      metrics.RelMSE_OutlierRejection (0.1 % rejected);
     * the image mean lies within 3 standard errors of the reference's.
     relMSE = mean((x - r)^2 / (r^2 + 1e-2)) per pixel and channel."""
     import os
+
+    from seesharp_amd import metrics
 
     ref = np.load(os.path.join(os.path.dirname(__file__), "golden", "large_scene_ref_4096spp.npz"))
     s = scenes.large_scene(1024, 1024)
@@ -186,13 +192,15 @@ def test_large_converged_relmse_against_cpu_reference(device):
     err_same = relmse(img, same)
     err = relmse(img, r)
     floor = relmse(same, r)
-    per_pixel = ((img - r) ** 2 / (r ** 2 + 1e-2)).mean(axis=1)
-    keep = np.sort(per_pixel)[: int(per_pixel.size * 0.99)]
-    print(f"relMSE same seed {err_same:.3e}, independent {err:.3e} (CPU floor {floor:.3e}), "
-          f"1%-outlier-rejected {keep.mean():.3e}")
+    # the subset as a column image for the metric's per-pixel rejection
+    col = lambda a: a.reshape(-1, 1, 3)  # noqa: E731
+    err_rej = metrics.RelMSE_OutlierRejection(col(img), col(r))
+    floor_rej = metrics.RelMSE_OutlierRejection(col(same), col(r))
+    print(f"relMSE same seed {err_same:.3e}; independent {err:.3e} (CPU floor {floor:.3e}); "
+          f"0.1%-outlier-rejected {err_rej:.3e} (CPU floor {floor_rej:.3e})")
     assert err_same <= 1e-3
     assert err <= 1.25 * floor
-    assert keep.mean() <= 1e-3
+    assert err_rej <= 1.25 * floor_rej
     n = r.shape[0]
     # standard error of the image mean, channels treated as fully correlated (conservative)
     se_mean = np.sqrt(np.sum(ref["stderr"].astype(np.float64) ** 2) * 3) / (n * 3)
