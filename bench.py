@@ -5,8 +5,11 @@ One step = one full render of the large scene (1 036 296 triangles, 1024 x 1024,
 scene already resident in HBM (BVH build and upload happen before timing).
 
 Prints one JSON line with the metric, the roofline of the dominant kernel
-(from HIP events on the library's own stream) and the CPU baseline (the
-oracle restatement of the reference, timed on a bounded pixel sample).
+(from HIP events on the library's own stream), the metric's second half (the
+adjoint / forward cost ratio on the 512^2 textured box and on 32 views of the
+textured large scene), the batching form's speed-up (64 scenes in one world vs
+per-scene launches) and the CPU baseline (the oracle restatement of the
+reference, timed on a bounded pixel sample).
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1]
     python bench.py --workload batch     # 64 scenes in shared waves vs per-scene launches
@@ -46,6 +49,7 @@ BYTES_ACCUMULATE_PER_PATH = 72  # path radiance (48 B) + pixel read and write (2
 # the fused primary launch: path state (88 B) + queue entry without the origin (20 B) + hit record (16 B)
 BYTES_PRIMARY_PER_PATH = 124
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "pt_large_scene_hbm_traffic.md")
+SQ_PROFILE = os.path.join(ROOT, "profiles", "pt_large_scene_sq_counters.md")
 
 
 def measured_traffic():
@@ -62,6 +66,19 @@ def measured_traffic():
     return {k: int(v) for k, v in re.findall(r"(\w+)=(\d+)", m.group(1))}
 
 
+def measured_valu_issue():
+    """Share of the SIMDs' VALU issue slots each kernel fills on the default
+    large-scene workload, from the committed rocprofv3 SQ counter summary."""
+    try:
+        text = open(SQ_PROFILE).read()
+    except OSError:
+        return {}
+    m = re.search(r"^valu_issue_share:(.*)$", text, re.M)
+    if not m:
+        return {}
+    return {k: float(v) for k, v in re.findall(r"(\w+)=([0-9.]+)", m.group(1))}
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -71,6 +88,7 @@ def cpu_baseline(scene, spp, max_depth, budget_s):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as ol
 
+    # one GPU's share of the host: at most 16 threads whatever the host has
     threads = min(16, len(os.sched_getaffinity(0)))
     t0 = time.perf_counter()
     o = ol.OracleScene(scene)
@@ -89,20 +107,40 @@ def cpu_baseline(scene, spp, max_depth, budget_s):
     o.render(cfg, 0, spp, pixels=pix, threads=threads)
     dt = time.perf_counter() - t0
     o.close()
-    return {"value": n * spp / dt / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port",
-            "sample": f"{n} seeded pixels x {spp} spp of the same scene ({n * spp} paths, {dt:.1f} s)"}
+    value = n * spp / dt / 1e6
+    host = host_cpu()
+    return {"value": value, "unit": "Mpaths/s", "cores": threads, "kind": "port",
+            "sample": f"{n} seeded pixels x {spp} spp of the same scene ({n * spp} paths, {dt:.1f} s)",
+            "per_core_mpaths_per_s": round(value / threads, 4),
+            "host_cpu_model": host["model"], "host_logical_cpus": host["logical"],
+            "host_cpus_available": host["available"],
+            "note": "threads = one GPU's share of the host (16 at most); per_core x host_logical_cpus "
+                    "bounds what the whole host would reach"}
 
 
-def batch_main(args):
+def host_cpu():
+    """CPU model (lscpu's "Model name", from /proc/cpuinfo) and logical CPU counts."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.lower().startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"model": model, "logical": os.cpu_count(), "available": len(os.sched_getaffinity(0))}
+
+
+def measure_batch(dev, steps, autotune_first=False):
     """The batching workload: 64 scenes (seeds 1000..1063, 1.8k-194k triangles,
     128^2-512^2, 4/16/64 spp, MaxDepth 8) rendered in shared waves, against the
-    same 64 scenes rendered one sspt_render call after the other."""
-    from seesharp_amd import Device, GpuBatch, GpuScene, abi, autotune, scenes
+    same 64 scenes rendered one sspt_render call after the other, with the
+    wavefront and with the plain single-kernel tracer."""
+    from seesharp_amd import GpuBatch, GpuScene, abi, autotune, scenes
 
     t0 = time.perf_counter()
     sc, spp = scenes.batch_scenes(64, 1000)
     log(f"[bench] 64 scenes generated in {time.perf_counter() - t0:.1f} s")
-    dev = Device(0)
     cfgs = [abi.PtSettings(0xC030114, n, 8, 1, 1, 1, -1) for n in spp]
     npaths = sum(s.FrameBuffer.Width * s.FrameBuffer.Height * n for s, n in zip(sc, spp))
     t0 = time.perf_counter()
@@ -110,74 +148,79 @@ def batch_main(args):
     info = batch.info()
     log(f"[bench] batch world ready in {time.perf_counter() - t0:.1f} s: {info}")
     tuned = None
-    if args.autotune:
-        tuned, _ = autotune.tune(dev, lambda: batch.render(cfgs), capacities=(1 << 22, 1 << 23, 1 << 24),
-                                 log=log)
+    if autotune_first:
+        tuned, _ = autotune.tune(dev, lambda: batch.render(cfgs), capacities=(1 << 22, 1 << 23, 1 << 24), log=log)
         log(f"[bench] autotuned: {tuned}")
     batch.render(cfgs)  # warm-up
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         batch.render(cfgs)
-    batch_ms = (time.perf_counter() - t0) * 1000.0 / args.steps
+    batch_ms = (time.perf_counter() - t0) * 1000.0 / steps
     log(f"[bench] batched: {batch_ms:.1f} ms per pass")
     # per-scene launches with the same execution settings
     singles = [GpuScene(dev, s) for s in sc]
-    for g, c, n in zip(singles, cfgs, spp):
-        g.clear()
-        g.render(c, 0, n)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
+
+    def per_scene():
         for g, c, n in zip(singles, cfgs, spp):
             g.clear()
             g.render(c, 0, n)
-    single_ms = (time.perf_counter() - t0) * 1000.0 / args.steps
+
+    per_scene()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        per_scene()
+    single_ms = (time.perf_counter() - t0) * 1000.0 / steps
     log(f"[bench] per-scene launches: {single_ms:.1f} ms per pass")
     # per-scene plain launches: the single-kernel tracer, one render per scene
     dev.set_execution_mode(True)
-    for g, c, n in zip(singles, cfgs, spp):
-        g.clear()
-        g.render(c, 0, n)
-    t0 = time.perf_counter()
-    for g, c, n in zip(singles, cfgs, spp):
-        g.clear()
-        g.render(c, 0, n)
-    plain_ms = (time.perf_counter() - t0) * 1000.0
-    dev.set_execution_mode(False)
+    try:
+        per_scene()
+        t0 = time.perf_counter()
+        per_scene()
+        plain_ms = (time.perf_counter() - t0) * 1000.0
+    finally:
+        dev.set_execution_mode(False)
     log(f"[bench] per-scene plain single-kernel launches: {plain_ms:.1f} ms per pass")
-    out = {
-        "metric": "Mpaths/s of the batched wavefront (64 scenes, one world) vs per-scene launches",
-        "value": round(npaths / (batch_ms / 1000.0) / 1e6, 3),
-        "unit": "Mpaths/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": 1,
-        "ms_per_step": round(batch_ms, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": "64 scenes, seeds 1000..1063, triangle budget log-uniform [1k, 200k], "
-                               "128^2..512^2, spp in {4,16,64}, MaxDepth 8",
-                   "triangles": info["num_triangles"], "paths_per_step": npaths},
+    for g in singles:
+        g.close()
+    batch.close()
+    return {
+        "workload": "64 scenes, seeds 1000..1063, triangle budget log-uniform [1k, 200k], 128^2..512^2, "
+                    "spp in {4,16,64}, MaxDepth 8",
+        "triangles": info["num_triangles"], "paths_per_pass": npaths,
+        "batched_ms": round(batch_ms, 3),
+        "batched_mpaths_per_s": round(npaths / (batch_ms / 1000.0) / 1e6, 3),
         "per_scene_launches_mpaths_per_s": round(npaths / (single_ms / 1000.0) / 1e6, 3),
         "speedup_vs_per_scene": round(single_ms / batch_ms, 3),
         "per_scene_plain_single_kernel_mpaths_per_s": round(npaths / (plain_ms / 1000.0) / 1e6, 3),
         "speedup_vs_per_scene_plain": round(plain_ms / batch_ms, 3),
         "autotuned": tuned,
     }
+
+
+def batch_main(args):
+    from seesharp_amd import Device
+
+    r = measure_batch(Device(0), args.steps, args.autotune)
+    out = {
+        "metric": "Mpaths/s of the batched wavefront (64 scenes, one world) vs per-scene launches",
+        "value": r["batched_mpaths_per_s"], "unit": "Mpaths/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+        "ms_per_step": r["batched_ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": r["workload"], "triangles": r["triangles"], "paths_per_step": r["paths_per_pass"]},
+    }
+    out.update({k: v for k, v in r.items() if k not in ("workload", "triangles", "paths_per_pass")})
     print(json.dumps(out), flush=True)
 
 
-def adjoint_main(args):
+def measure_adjoint(dev, steps, kernels=True):
     """The differentiation workload: the box at 512^2 with 256^2 RGB albedo and
     mono roughness textures on the back wall and floor, forward 32 spp and
-    path-replay adjoint 32 spp of an L2 loss, MaxDepth 6. Reports the adjoint /
-    forward cost ratio (device time of one render each, graphs on)."""
-    from seesharp_amd import Device, GpuScene, abi, scenes
+    path-replay adjoint 32 spp of an L2 loss, MaxDepth 6. The adjoint / forward
+    cost ratio compares the wall time of one render call each (graphs on)."""
+    from seesharp_amd import GpuScene, abi, scenes
 
     res, tex_res, spp, depth = 512, 256, 32, 6
-    dev = Device(0)
     target_scene = scenes.box_scene(res, res, textured=True, tex_res=tex_res)
     tg = GpuScene(dev, target_scene)
     cfg = abi.PtSettings(0xC030114, spp, depth, 1, 1, 1, -1)
@@ -201,7 +244,7 @@ def adjoint_main(args):
     dimg = (2.0 * (g.read_image() - target) / target.size).astype(np.float32)
     adjoint(dimg)
     fwd_ms, adj_ms = [], []
-    for _ in range(max(1, args.steps)):
+    for _ in range(max(1, steps)):
         t0 = time.perf_counter()
         forward()
         fwd_ms.append((time.perf_counter() - t0) * 1000.0)
@@ -209,59 +252,64 @@ def adjoint_main(args):
         stats = adjoint(dimg)
         adj_ms.append((time.perf_counter() - t0) * 1000.0)
     f_ms, a_ms = float(np.median(fwd_ms)), float(np.median(adj_ms))
-    log(f"[bench] forward {f_ms:.2f} ms, adjoint {a_ms:.2f} ms per {npaths} paths")
-    # per-kernel device time, one event-timed render of each
-    dev.set_kernel_timing(True)
-    dev.reset_kernel_timing()
-    forward()
-    kf = {k: round(dev.kernel_timing(k)[1], 3) for k in ("generate", "extend", "trace", "shade", "shadow", "accumulate")}
-    dev.reset_kernel_timing()
-    adjoint(dimg)
-    ka = {k: round(dev.kernel_timing(k)[1], 3)
-          for k in ("generate", "extend", "trace_adjoint", "shade_adjoint", "shadow_adjoint", "adjoint_finalize")}
-    dev.set_kernel_timing(False)
+    log(f"[bench] textured box: forward {f_ms:.2f} ms, adjoint {a_ms:.2f} ms per {npaths} paths")
     out = {
-        "metric": "adjoint/forward cost ratio (path-replay adjoint of an L2 image loss)",
-        "value": round(a_ms / f_ms, 3),
-        "unit": "x",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": 1,
-        "ms_per_step": round(f_ms + a_ms, 3),
-        "higher_is_better": False,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": "box scene 512x512, 256x256 RGB albedo + mono roughness textures on back wall "
-                               "and floor, forward 32 spp + adjoint 32 spp, MaxDepth 6",
-                   "paths_per_pass": npaths},
+        "workload": "box scene 512x512, 256x256 RGB albedo + mono roughness textures on back wall and floor, "
+                    "forward 32 spp + adjoint 32 spp, MaxDepth 6",
+        "paths_per_pass": npaths,
+        "ratio": round(a_ms / f_ms, 3),
         "forward_ms": round(f_ms, 3),
         "adjoint_ms": round(a_ms, 3),
         "forward_mpaths_per_s": round(npaths / f_ms / 1e3, 3),
         "adjoint_mpaths_per_s": round(npaths / a_ms / 1e3, 3),
-        "kernel_ms_forward": kf,
-        "kernel_ms_adjoint": ka,
         "adjoint_ray_stats": stats,
     }
+    if kernels:
+        # per-kernel device time, one event-timed render of each
+        dev.set_kernel_timing(True)
+        dev.reset_kernel_timing()
+        forward()
+        out["kernel_ms_forward"] = {k: round(dev.kernel_timing(k)[1], 3)
+                                    for k in ("extend", "trace", "shade", "shadow", "accumulate")}
+        dev.reset_kernel_timing()
+        adjoint(dimg)
+        out["kernel_ms_adjoint"] = {k: round(dev.kernel_timing(k)[1], 3)
+                                    for k in ("extend", "trace_adjoint", "shade_adjoint", "shadow_adjoint",
+                                              "adjoint_finalize")}
+        dev.set_kernel_timing(False)
+    g.close()
+    return out
+
+
+def adjoint_main(args):
+    from seesharp_amd import Device
+
+    r = measure_adjoint(Device(0), args.steps)
+    out = {
+        "metric": "adjoint/forward cost ratio (path-replay adjoint of an L2 image loss)",
+        "value": r["ratio"], "unit": "x", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+        "ms_per_step": round(r["forward_ms"] + r["adjoint_ms"], 3), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": r["workload"], "paths_per_pass": r["paths_per_pass"]},
+    }
+    out.update({k: v for k, v in r.items() if k not in ("workload", "paths_per_pass", "ratio")})
     print(json.dumps(out), flush=True)
 
 
-def views_main(args):
+def measure_views(dev, steps):
     """Differentiation + batching: the large scene with 1024^2 RGB albedo and
     1024^2 roughness textures on the ground grid, 32 views on a circle (radius 8,
     height 2) at 512^2 and 16 spp, MaxDepth 8. All views go forward in one
     batched render and backward in one batched adjoint whose gradients add into
     the shared texture buffers; per-view L2 loss against targets rendered from
     seeded noise textures (seed 5). Initial textures are uniform 0.5."""
-    from seesharp_amd import Device, GpuBatch, abi, scenes
+    from seesharp_amd import GpuBatch, abi, scenes
 
     res, spp, depth, nviews = 512, 16, 8, 32
     t0 = time.perf_counter()
     target_scene = scenes.large_scene(res, res, ground_textures=1024, ground_texture_seed=5)
     s = scenes.large_scene(res, res, ground_textures=1024, ground_uniform=0.5)
     cams = scenes.circle_views(nviews, radius=8.0, height=2.0)
-    dev = Device(0)
     views = GpuBatch.views(dev, s, cams)
     log(f"[bench] view set ready in {time.perf_counter() - t0:.1f} s: {views.info()}")
     cfgs = [abi.PtSettings(0xC030114, spp, depth, 1, 1, 1, -1) for _ in cams]
@@ -276,20 +324,17 @@ def views_main(args):
     views.write_texture(idx_rough, q.Roughness.Image)
     npaths = nviews * res * res * spp
 
-    def forward():
-        views.render(cfgs)
-        return [views.read_image(i) for i in range(nviews)]
-
     def adjoint(dl):
         views.clear_gradients()
         return views.render_adjoint(cfgs, dl, first_iteration=spp)
 
-    imgs = forward()
+    views.render(cfgs)
+    imgs = [views.read_image(i) for i in range(nviews)]
     npx = res * res * 3
     dl = [(2.0 * (imgs[i] - targets[i]) / (npx * nviews)).astype(np.float32) for i in range(nviews)]
     adjoint(dl)
     fwd_ms, adj_ms = [], []
-    for _ in range(max(1, args.steps)):
+    for _ in range(max(1, steps)):
         t0 = time.perf_counter()
         views.render(cfgs)
         fwd_ms.append((time.perf_counter() - t0) * 1000.0)
@@ -298,23 +343,13 @@ def views_main(args):
         adj_ms.append((time.perf_counter() - t0) * 1000.0)
     f_ms, a_ms = float(np.median(fwd_ms)), float(np.median(adj_ms))
     g_alb = views.gradient(idx_alb)
-    log(f"[bench] forward {f_ms:.1f} ms, adjoint {a_ms:.1f} ms per {npaths} paths")
-    out = {
-        "metric": "adjoint/forward cost ratio, 32 views in one batched launch (path-replay adjoint)",
-        "value": round(a_ms / f_ms, 3),
-        "unit": "x",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": 1,
-        "ms_per_step": round(f_ms + a_ms, 3),
-        "higher_is_better": False,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": "large scene (1036296 tris) with 1024x1024 RGB albedo + mono roughness on the ground, "
-                               "32 views on a circle (r 8, h 2) at 512x512, 16 spp, MaxDepth 8, L2 loss",
-                   "paths_per_pass": npaths},
+    log(f"[bench] 32 views: forward {f_ms:.1f} ms, adjoint {a_ms:.1f} ms per {npaths} paths")
+    views.close()
+    return {
+        "workload": "large scene (1036296 tris) with 1024x1024 RGB albedo + mono roughness on the ground, "
+                    "32 views on a circle (r 8, h 2) at 512x512, 16 spp, MaxDepth 8, L2 loss",
+        "paths_per_pass": npaths,
+        "ratio": round(a_ms / f_ms, 3),
         "forward_ms": round(f_ms, 3),
         "adjoint_ms": round(a_ms, 3),
         "forward_mpaths_per_s": round(npaths / f_ms / 1e3, 3),
@@ -322,6 +357,20 @@ def views_main(args):
         "albedo_gradient_abs_sum": float(np.abs(g_alb).sum()),
         "adjoint_ray_stats": stats,
     }
+
+
+def views_main(args):
+    from seesharp_amd import Device
+
+    r = measure_views(Device(0), args.steps)
+    out = {
+        "metric": "adjoint/forward cost ratio, 32 views in one batched launch (path-replay adjoint)",
+        "value": r["ratio"], "unit": "x", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+        "ms_per_step": round(r["forward_ms"] + r["adjoint_ms"], 3), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": r["workload"], "paths_per_pass": r["paths_per_pass"]},
+    }
+    out.update({k: v for k, v in r.items() if k not in ("workload", "paths_per_pass", "ratio")})
     print(json.dumps(out), flush=True)
 
 
@@ -416,6 +465,8 @@ def main():
     ap.add_argument("--max-depth", type=int, default=8)
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the adjoint (textured box, 32 views) and batching measurements")
     ap.add_argument("--workload", choices=("large", "batch", "adjoint", "views", "copy", "cornell"), default="large")
     ap.add_argument("--autotune", action="store_true")
     args = ap.parse_args()
@@ -515,6 +566,7 @@ def main():
     kernel_ms = sum(v[1] for v in kernels.values())
     default_workload = (args.res, args.spp, args.max_depth) == (1024, 64, 8)
     traffic = measured_traffic().get(dom) if default_workload else None
+    valu_share = measured_valu_issue().get(dom) if default_workload else None
 
     out = {
         "metric": METRIC,
@@ -557,6 +609,10 @@ def main():
             "avg_launch_ms": round(avg_ms, 4),
             "launches": launches,
             "alg_bytes_per_launch": int(bytes_per_launch),
+            # what does bound the kernel: its VALU instructions' share of the issue slots
+            # (rocprofv3 SQ counters of this command; the HBM fraction above is far from 1)
+            "limiter": "valu_issue" if valu_share else None,
+            "valu_issue_share": valu_share,
         },
         "image_mean": float(img.mean()),
         "nan_paths": gs.nan_count(),
@@ -574,6 +630,20 @@ def main():
     out["plain_single_kernel"] = {"mpaths_per_s": round(npaths / (single_ms / 1000.0) / 1e6, 3),
                                   "ms_per_step": round(single_ms, 3),
                                   "wavefront_speedup": round(single_ms / ms, 3)}
+    if default_workload and not args.no_extras:
+        # the metric's second half and the batching form, each timed in this run
+        adj = measure_adjoint(dev, 3, kernels=False)
+        vw = measure_views(dev, 1)
+        bt = measure_batch(dev, 1)
+        out["adjoint_forward_ratio"] = {"textured_box_512": adj["ratio"], "views_32x512_large": vw["ratio"]}
+        out["adjoint"] = {
+            "textured_box_512": {k: adj[k] for k in ("workload", "forward_ms", "adjoint_ms", "paths_per_pass")},
+            "views_32x512_large": {k: vw[k] for k in ("workload", "forward_ms", "adjoint_ms", "paths_per_pass")},
+        }
+        out["batching"] = {k: bt[k] for k in ("workload", "triangles", "paths_per_pass", "batched_mpaths_per_s",
+                                              "per_scene_launches_mpaths_per_s", "speedup_vs_per_scene",
+                                              "per_scene_plain_single_kernel_mpaths_per_s",
+                                              "speedup_vs_per_scene_plain")}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene, args.spp, args.max_depth, args.cpu_budget)
     print(json.dumps(out), flush=True)

@@ -1485,7 +1485,8 @@ Rgb render_pixel(const PathCtx& pc, uint32_t row, uint32_t col, uint32_t iterati
 
 template <class F>
 void parallel_for(int64_t n, int threads, F&& f) {
-    if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    // default: the hardware threads, at most 16 (one GPU's share of a shared host)
+    if (threads <= 0) threads = (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
     std::atomic<int64_t> next{0};
     auto work = [&]() {
         for (;;) {

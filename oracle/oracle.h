@@ -91,7 +91,7 @@ void or_primary_hits(const or_scene* s, uint32_t base_seed, uint32_t iteration,
 /* ---- PathTracerBase.Render (SeeSharp/Integrators/PathTracer.cs:199-382) ----
  * Renders iterations first..first+num-1 into image (W*H*3 running mean) for
  * all pixels, or only the `num_pixels` pixel indices listed in `pixels`.
- * num_threads <= 0 uses all hardware threads. Returns the NaN/Inf splat count. */
+ * num_threads <= 0 uses the hardware threads, at most 16. Returns the NaN/Inf splat count. */
 uint64_t or_render(const or_scene* s, const sspt_pt_settings* settings, uint32_t first_iteration,
                    uint32_t num_iterations, float* image, sspt_ray_stats* stats,
                    const int32_t* pixels, int64_t num_pixels, int32_t num_threads);

@@ -5,10 +5,14 @@ Bars:
 * closest-hit triangle ids and distances: identical to the oracle's brute force
   (watertight test, ties to the lower triangle id) on every ray;
 * images: the GPU and the oracle draw the same RNG streams, so per-pixel
-  estimates agree except where fp32 rounding of transcendental functions
-  sends a path another way. Tolerances: image means within 1 % (the
-  reference's own validation bar, SeeSharp.Validation/Validator.cs:24-27) and
-  relMSE <= 1e-3 with relMSE = mean((x - r)^2 / (r^2 + 1e-2)).
+  estimates agree except where the hardware square root and reciprocal of the
+  shading arithmetic (a few ulps) send a path another way. relMSE =
+  mean((x - r)^2 / (r^2 + 1e-2)); bars at about 10x the observed values: the
+  36-triangle box (observed 1.9e-14) <= 1e-10; the large scene at 192^2 and
+  4 spp (3.3e-5: its glossy GenericMaterials amplify ulp differences into
+  diverted paths) <= 3e-4 with at most 1 % of the pixels diverted; the
+  4096-spp converged subset (1.0e-6) <= 1e-5; image means within 1 % (the
+  reference's own validation bar, SeeSharp.Validation/Validator.cs:24-27).
 """
 import numpy as np
 import pytest
@@ -89,7 +93,7 @@ def test_box_render_matches_oracle(box):
     print(f"box 128^2 16 spp: relMSE {relmse(img_g, img_o):.3e}")
     assert g.nan_count() == nan_o == 0
     assert abs(img_g.mean() - img_o.mean()) <= 0.01 * img_o.mean()
-    assert relmse(img_g, img_o) <= 1e-3
+    assert relmse(img_g, img_o) <= 1e-10
     # the ray counters follow the same paths
     assert abs(sg["num_rays"] - so.num_rays) <= 0.01 * so.num_rays
     assert abs(sg["num_shadow_rays"] - so.num_shadow_rays) <= 0.01 * so.num_shadow_rays
@@ -154,9 +158,11 @@ def test_large_render_matches_oracle(large):
     g.render(cfg, 0, 4)
     img_g = g.read_image()
     img_o, _, _ = o.render(cfg, 0, 4)
-    print(f"large 192^2 4 spp: relMSE {relmse(img_g, img_o):.3e}")
+    div = float(np.mean(np.any(~np.isclose(img_g, img_o, rtol=1e-4, atol=1e-6), axis=-1)))
+    print(f"large 192^2 4 spp: relMSE {relmse(img_g, img_o):.3e}, diverted pixels {div:.3e}")
     assert abs(img_g.mean() - img_o.mean()) <= 0.01 * img_o.mean()
-    assert relmse(img_g, img_o) <= 1e-3
+    assert relmse(img_g, img_o) <= 3e-4
+    assert div <= 0.01
 
 
 def test_large_converged_relmse_against_cpu_reference(device):
@@ -165,8 +171,8 @@ def test_large_converged_relmse_against_cpu_reference(device):
     (tests/golden/make_converged_reference.py):
 
     * same seed as the GPU (0xC030114): the GPU follows the CPU path for path,
-      relMSE <= 1e-3 against the 4096-spp CPU reference (the primary gate; the
-      observed value is printed);
+      relMSE <= 1e-3 against the 4096-spp CPU reference (the north star's bar,
+      the primary gate), and <= 1e-5, 10x the observed 1.0e-6;
     * independent seed (571298512): no error beyond Monte Carlo noise. Two
       independent 4096-spp CPU images of these pixels differ by relMSE 6.7e-3
       (2.6e-3 with 0.1 % of the pixels rejected: fireflies), so at this sample
@@ -199,6 +205,7 @@ def test_large_converged_relmse_against_cpu_reference(device):
     print(f"relMSE same seed {err_same:.3e}; independent {err:.3e} (CPU floor {floor:.3e}); "
           f"0.1%-outlier-rejected {err_rej:.3e} (CPU floor {floor_rej:.3e})")
     assert err_same <= 1e-3
+    assert err_same <= 1e-5
     assert err <= 1.25 * floor
     assert err_rej <= 1.25 * floor_rej
     n = r.shape[0]

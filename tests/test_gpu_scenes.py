@@ -1,8 +1,9 @@
 """GPU renders of the reference's own scene files (tests/golden/scenes, loaded by
 seesharp_amd.sceneio) against the oracle on the same scene: the Cornell box
 of the reference's validation scenes, a textured .obj scene and an image
-background read from OpenEXR. Same bars as the parity tests: image means
-within 1 % and relMSE <= 1e-3."""
+background read from OpenEXR. Same seeds on both sides: image means within
+1 % and relMSE <= 1e-10 (observed 5e-16 .. 9e-15: the GPU follows every path
+of these diffuse scenes, differing from the oracle by a few ulps)."""
 import os
 
 import numpy as np
@@ -42,5 +43,5 @@ def test_reference_scene_matches_oracle(device, name, size, depth, spp):
     assert img_o.mean() > 0
     assert g.nan_count() == nan_o == 0
     assert abs(img_g.mean() - img_o.mean()) <= 0.01 * img_o.mean()
-    assert relmse(img_g, img_o) <= 1e-3
+    assert relmse(img_g, img_o) <= 1e-10
     assert abs(sg["num_rays"] - so.num_rays) <= 0.01 * so.num_rays

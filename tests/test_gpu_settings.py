@@ -7,10 +7,15 @@ appends, the 2 * NumShadowRays shadow capacity per path and the per-lane shadow-
 skips only run with these settings); a batch whose scenes use different
 NumShadowRays and MinDepth; and MaximumRenderTimeMs (PathTracer.cs:221-227).
 
-Same-seed bar: the GPU and the oracle draw the same RNG streams and differ only by
-the hardware square root / reciprocal in the shading arithmetic (a few ulps), so
-relMSE = mean((x - r)^2 / (r^2 + 1e-2)) stays below 1e-10 (observed values are
-printed), and every ray counter agrees to 0.1 %.
+Same-seed bar: the GPU and the oracle draw the same RNG streams and differ by the
+hardware square root / reciprocal in the shading arithmetic (a few ulps), which now
+and then sends one sample another way. Observed relMSE = mean((x - r)^2 / (r^2 + 1e-2))
+on the box: 1e-15 .. 6e-13, and 1.2e-7 for NumShadowRays 2 (one diverted sample).
+Bars: relMSE <= 1e-6, at most 0.1 % of the pixels off by more than rtol 1e-4 (a bias
+on 5 % of the pixels at 10 % would give relMSE ~5e-4 and fail both), and the
+closest-hit, shadow-ray and hit counters within 0.1 %. (num_occluded is not
+compared: the GPU counts only the shadow rays it traces, skipping those whose
+contribution is exactly zero, the oracle every IsOccluded call.)
 """
 import numpy as np
 import pytest
@@ -20,11 +25,17 @@ from seesharp_amd import GpuBatch, GpuPathTracer, GpuScene, scenes
 
 pytestmark = pytest.mark.gpu
 
-SAME_SEED_BAR = 1e-10
+SAME_SEED_BAR = 1e-6
+DIVERTED_PIXELS = 1e-3
 
 
 def relmse(x, r):
     return float(np.mean((x - r) ** 2 / (r ** 2 + 1e-2)))
+
+
+def diverted(x, r):
+    """Fraction of pixels whose estimate differs from the oracle's beyond rounding."""
+    return float(np.mean(np.any(~np.isclose(x, r, rtol=1e-4, atol=1e-6), axis=-1)))
 
 
 @pytest.fixture(scope="module")
@@ -46,20 +57,23 @@ def test_settings_match_oracle(box, nsr, bsdf_di, min_depth):
     sg = g.render(cfg, 0, 8)
     img_g = g.read_image()
     img_o, so, nan_o = o.render(cfg, 0, 8)
-    err = relmse(img_g, img_o)
-    print(f"  relMSE {err:.3e}, mean gpu {img_g.mean():.6f} oracle {img_o.mean():.6f}, rays {sg['num_rays']} "
-          f"{so.num_rays}, shadow {sg['num_shadow_rays']} {so.num_shadow_rays}")
+    err, div = relmse(img_g, img_o), diverted(img_g, img_o)
+    print(f"  relMSE {err:.3e}, diverted pixels {div:.2e}, mean gpu {img_g.mean():.6f} oracle {img_o.mean():.6f}, "
+          f"rays {sg['num_rays']} {so.num_rays}, shadow {sg['num_shadow_rays']} {so.num_shadow_rays}")
     assert g.nan_count() == nan_o == 0
-    assert err <= SAME_SEED_BAR
+    assert err <= SAME_SEED_BAR and div <= DIVERTED_PIXELS
     assert sg["num_iterations"] == 8
-    for k in ("num_rays", "num_shadow_rays", "num_occluded", "num_ray_hits"):
+    for k in ("num_rays", "num_shadow_rays", "num_ray_hits"):
         assert abs(sg[k] - getattr(so, k)) <= 1e-3 * max(1, getattr(so, k)), k
 
 
 def test_single_kernel_tracer_with_three_shadow_rays(device, box):
     """The plain single-kernel tracer traces its shadow rays where they are
-    sampled; with NumShadowRays 3 and MinDepth 2 it still renders the wavefront's
-    image bit for bit."""
+    sampled; with NumShadowRays 3 and MinDepth 2 it follows the wavefront's paths
+    exactly. The wavefront adds a vertex's three next-event contributions of a
+    kind into one sum with atomics in whatever order the shadow rays finish, so
+    the images agree to fp32 summation order, not bit for bit (paths never branch
+    on those sums)."""
     _, g, _ = box
     cfg = ol.settings(total_spp=4, max_depth=5, min_depth=2, num_shadow_rays=3)
     g.clear()
@@ -72,7 +86,8 @@ def test_single_kernel_tracer_with_three_shadow_rays(device, box):
         single = g.read_image()
     finally:
         device.set_execution_mode(0)
-    assert np.array_equal(wave, single)
+    assert np.allclose(wave, single, rtol=1e-5, atol=1e-7)
+    assert relmse(wave, single) <= 1e-12
 
 
 def test_batch_mixing_shadow_ray_counts_matches_oracle(device):
@@ -85,9 +100,10 @@ def test_batch_mixing_shadow_ray_counts_matches_oracle(device):
     assert st["num_iterations"] == 8
     for i, s in enumerate(sc):
         img_o, _, _ = ol.OracleScene(s).render(cfgs[i], 0, cfgs[i].total_spp)
-        err = relmse(b.read_image(i), img_o)
-        print(f"batch scene {i}: relMSE {err:.3e}")
-        assert err <= SAME_SEED_BAR, i
+        img_g = b.read_image(i)
+        err, div = relmse(img_g, img_o), diverted(img_g, img_o)
+        print(f"batch scene {i}: relMSE {err:.3e}, diverted pixels {div:.2e}")
+        assert err <= SAME_SEED_BAR and div <= DIVERTED_PIXELS, i
 
 
 @pytest.fixture(scope="module")

@@ -3,7 +3,8 @@ after SeeSharp.Validation) at the validator's resolution, sample count and
 depth, against the oracle:
 
 * same seed: the GPU and the oracle follow the same paths, so image means agree
-  within 1 % and relMSE <= 1e-3;
+  within 1 % and relMSE <= 1e-9 (observed 5e-16 .. 1.7e-11, the largest on the
+  rough dielectric of DirectIllumTransmit);
 * independent seeds: the validator's own bar, every image mean within 1 % of
   the first (SeeSharp.Validation/Validator.cs:10-30), GPU against oracle;
 * MultiLight against Lambert's closed form on every ground pixel.
@@ -49,7 +50,7 @@ def test_validation_scene_matches_oracle(device, name):
     assert g.nan_count() == nan_o == 0
     assert validation.image_mean(img_o) > 0
     assert abs(validation.image_mean(img_g) - validation.image_mean(img_o)) <= 0.01 * validation.image_mean(img_o)
-    assert relmse(img_g, img_o) <= 1e-3
+    assert relmse(img_g, img_o) <= 1e-9
     assert validation.means_agree([img_g, other])
 
 
