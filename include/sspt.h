@@ -295,6 +295,27 @@ sspt_status sspt_render(sspt_ctx* ctx, sspt_scene* scene, const sspt_pt_settings
                         uint32_t first_iteration, uint32_t num_iterations,
                         sspt_ray_stats* stats);
 
+/* ---- allocation-free rendering. No counterpart in the reference: it is for
+ *      callers that cannot allocate device memory (code under HIP graph capture,
+ *      tight serving loops, harnesses that forbid hipMalloc). -----------------
+ * sspt_render's semantics and results (the same images bit for bit, the same
+ * ray counters) with every wave buffer in one file-scope device array of
+ * SSPT_NOALLOC_SCRATCH_BYTES per GPU instead of device memory the context
+ * allocates; the call makes no hipMalloc / hipFree, no graph and no event.
+ * Waves hold as many path slots as fit the array next to a shadow queue of
+ * 2 * num_shadow_rays entries per slot (about 172 000 slots at one shadow ray);
+ * an image with more pixels is rendered in waves of part of an iteration.
+ * Limits: num_shadow_rays small enough to leave 256 slots (up to about 2 700),
+ * no time budget (max_render_time_ms must be < 0: SSPT_ERR_UNSUPPORTED).
+ * The scene's framebuffer (allocated by sspt_scene_create) receives the image.
+ * Every context on a GPU shares that GPU's array and each call leaves its last
+ * contents behind: calls that overlap on one GPU (two threads, or two contexts
+ * on one device) race and must be serialised by the caller. */
+#define SSPT_NOALLOC_SCRATCH_BYTES (64u << 20)
+sspt_status sspt_render_noalloc(sspt_ctx* ctx, sspt_scene* scene, const sspt_pt_settings* settings,
+                                uint32_t first_iteration, uint32_t num_iterations,
+                                sspt_ray_stats* stats);
+
 /* ---- differentiation: path-replay adjoint of sspt_render. No counterpart in
  *      the reference. Given dLoss/dImage for the image of iterations
  *      first_iteration .. first_iteration+num_iterations-1, replays those paths

@@ -14,6 +14,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("SSPT_LIBRARY") or os.path.join(_HERE, "libsspt.so")
 
 ABI_VERSION = 4  # SSPT_ABI_VERSION of include/sspt.h
+NOALLOC_SCRATCH_BYTES = 64 << 20  # SSPT_NOALLOC_SCRATCH_BYTES
 SSPT_OK = 0
 INVALID_ARGUMENT = 1
 STATUS_NAMES = {
@@ -169,6 +170,8 @@ SYMBOLS = {
     "sspt_framebuffer_nan_count": (C.c_int, [_vp, _vp, C.POINTER(C.c_uint64)]),
     "sspt_render": (C.c_int, [_vp, _vp, C.POINTER(PtSettings), C.c_uint32, C.c_uint32,
                               C.POINTER(RayStats)]),
+    "sspt_render_noalloc": (C.c_int, [_vp, _vp, C.POINTER(PtSettings), C.c_uint32, C.c_uint32,
+                                      C.POINTER(RayStats)]),
     "sspt_trace_closest": (C.c_int, [_vp, _vp, _fp, C.c_uint64, C.POINTER(Hit)]),
     "sspt_trace_occluded": (C.c_int, [_vp, _vp, _fp, C.c_uint64, C.POINTER(C.c_uint8)]),
     "sspt_primary_hits": (C.c_int, [_vp, _vp, C.c_uint32, C.c_uint32, _fp, C.POINTER(Hit)]),

@@ -11,6 +11,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <thread>
@@ -92,6 +93,9 @@ struct sspt_ctx {
     std::vector<void*> adj_allocs;
     uint64_t adj_records = 0;   // capacity * depth
     size_t adj_shadow = 0;
+    // this device's copy of the file-scope scratch array (sspt_render_noalloc), or null
+    unsigned char* scratch = nullptr;
+    hipError_t scratch_status = hipSuccess;
 };
 
 struct sspt_scene {
@@ -116,7 +120,33 @@ struct sspt_scene {
     unsigned long long* outliers = nullptr;
 };
 
+// Scratch memory of sspt_render_noalloc: every wave buffer of an allocation-free
+// render lives here. Each GPU the library runs on holds its own copy until the
+// process ends; in host code `scratch` only names the symbol, and its device
+// address (per device) comes from hipGetSymbolAddress.
+static_assert(SSPT_NOALLOC_SCRATCH_BYTES % 256 == 0, "scratch size is a multiple of 256 bytes");
+static __device__ __align__(256) unsigned char scratch[SSPT_NOALLOC_SCRATCH_BYTES];
+
 namespace {
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mutex;
+unsigned char* g_scratch_ptr[kMaxScratchDevices] = {};
+hipError_t g_scratch_status[kMaxScratchDevices] = {};
+
+// Looks up this device's scratch address once per device (the device current).
+hipError_t scratch_address(int device, unsigned char** out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (device < 0 || device >= kMaxScratchDevices) return hipErrorInvalidDevice;
+    if (!g_scratch_ptr[device] && g_scratch_status[device] == hipSuccess) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) g_scratch_status[device] = e != hipSuccess ? e : hipErrorInvalidSymbol;
+        else g_scratch_ptr[device] = static_cast<unsigned char*>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return g_scratch_status[device];
+}
 
 sspt_status fail(sspt_ctx* ctx, sspt_status st, const char* fmt, ...) {
     char buf[512];
@@ -750,14 +780,13 @@ LaunchConfig launch_config(const sspt_ctx* ctx, uint32_t max_paths, uint32_t max
 // the descriptor the wave counter selects.
 enum class Mode { Mean, Sum, Adjoint };
 
-void enqueue_wave(sspt_ctx* ctx, sspt_scene* sc, const LaunchConfig& lc, int max_depth, Mode mode,
+void enqueue_wave(sspt_ctx* ctx, sspt_scene* sc, const Wave& w, const LaunchConfig& lc, int max_depth, Mode mode,
                   float* mean_image, uint32_t mean_pixels, float* mean_albedo) {
     float* mean_normal = mean_albedo ? mean_albedo + sc->dev.layer_stride : nullptr;
     // running-mean renders are single-scene: the scene's keys start at the pool's start
     unsigned long long* mean_outl = sc->dev.outliers;
     const int n_outl = sc->dev.num_outliers;
     const DevScene& S = sc->dev;
-    const Wave& w = ctx->wave;
     hipStream_t st = ctx->stream;
     const bool adj = mode == Mode::Adjoint;
     launch_wave_begin(st, w);
@@ -842,7 +871,7 @@ sspt_status run_waves(sspt_ctx* ctx, sspt_scene* sc, const std::vector<WaveDesc>
     if (graphs) {
         hipGraph_t graph = nullptr;
         HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        enqueue_wave(ctx, sc, lc, max_depth, mode, mean_image, mean_pixels, mean_albedo);
+        enqueue_wave(ctx, sc, ctx->wave, lc, max_depth, mode, mean_image, mean_pixels, mean_albedo);
         HIP_TRY(ctx, hipStreamEndCapture(ctx->stream, &graph));
         hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
@@ -870,7 +899,7 @@ sspt_status run_waves(sspt_ctx* ctx, sspt_scene* sc, const std::vector<WaveDesc>
                 break;
             }
         } else {
-            enqueue_wave(ctx, sc, lc, max_depth, mode, mean_image, mean_pixels, mean_albedo);
+            enqueue_wave(ctx, sc, ctx->wave, lc, max_depth, mode, mean_image, mean_pixels, mean_albedo);
         }
         ++done;
         if (budget_ms >= 0) {
@@ -897,9 +926,11 @@ sspt_status run_waves(sspt_ctx* ctx, sspt_scene* sc, const std::vector<WaveDesc>
     return SSPT_OK;
 }
 
-sspt_status read_stats(sspt_ctx* ctx, sspt_scene* sc, sspt_ray_stats* stats) {
-    std::vector<unsigned long long> striped((size_t)kStatStripes * kStatStride);
-    HIP_TRY(ctx, hipMemcpyAsync(striped.data(), ctx->wave.stats, kStatBytes, hipMemcpyDeviceToHost, ctx->stream));
+sspt_status read_stats(sspt_ctx* ctx, sspt_scene* sc, sspt_ray_stats* stats,
+                       const unsigned long long* device_stats = nullptr) {
+    unsigned long long striped[(size_t)kStatStripes * kStatStride];
+    HIP_TRY(ctx, hipMemcpyAsync(striped, device_stats ? device_stats : ctx->wave.stats, kStatBytes,
+                                hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     unsigned long long hs[kStatNum] = {};
     for (int k = 0; k < kStatStripes; ++k)
@@ -943,6 +974,70 @@ uint32_t wave_capacity(const sspt_ctx* ctx) {
     return std::min(c, kMaxSlots);
 }
 
+// Wave buffers of sspt_render_noalloc carved out of the scratch array: the
+// counters, the striped ray statistics, a block of wave descriptors, then the
+// per-slot arrays of the largest wave (a multiple of 256 slots) that fits next
+// to a shadow queue of 2 * NumShadowRays entries per slot.
+constexpr uint32_t kScratchWaveDescs = 1024;
+struct ScratchWaves {
+    Wave wave{};
+    WaveDesc* descs = nullptr;
+    uint32_t capacity = 0;
+    size_t shadow_entries = 0;
+};
+bool carve_scratch(unsigned char* base, int num_shadow_rays, ScratchWaves* out) {
+    const size_t nsr = (size_t)std::max(1, num_shadow_rays);
+    auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t fixed = up(kCntNum * sizeof(uint32_t)) + up(kStatBytes) + up(kScratchWaveDescs * sizeof(WaveDesc));
+    // per slot: two extension queues (origin, direction, tag, RNG, prefix weight, previous
+    // hit, radiance), the hit record, pixel, last vertex, three radiance sums, two
+    // denoiser layers; the shadow queue's three float4 per entry
+    const size_t per_slot = 2 * (16 + 16 + 4 + 4 + 16 + 16 + 16) + 16 + 4 + 6 * 16 + 2 * nsr * 48;
+    const size_t arrays = 2 * 7 + 8 + 3;  // 256-byte alignment slack per array
+    const size_t avail = (size_t)SSPT_NOALLOC_SCRATCH_BYTES - fixed - arrays * 256;
+    size_t cap = avail / per_slot;
+    cap -= cap % 256;
+    cap = std::min<size_t>(cap, kMaxSlots);
+    if (cap < 256) return false;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        unsigned char* p = base + off;
+        off += up(bytes);
+        return p;
+    };
+    Wave& w = out->wave;
+    w.counters = reinterpret_cast<uint32_t*>(take(kCntNum * sizeof(uint32_t)));
+    w.stats = reinterpret_cast<unsigned long long*>(take(kStatBytes));
+    out->descs = reinterpret_cast<WaveDesc*>(take(kScratchWaveDescs * sizeof(WaveDesc)));
+    w.waves = out->descs;
+    for (int b = 0; b < 2; ++b) {
+        w.q_o[b] = reinterpret_cast<float4*>(take(cap * 16));
+        w.q_d[b] = reinterpret_cast<float4*>(take(cap * 16));
+        w.q_slot[b] = reinterpret_cast<uint32_t*>(take(cap * 4));
+        w.q_rng[b] = reinterpret_cast<uint32_t*>(take(cap * 4));
+        w.q_thr[b] = reinterpret_cast<float4*>(take(cap * 16));
+        w.q_prev[b] = reinterpret_cast<float4*>(take(cap * 16));
+        w.q_L[b] = reinterpret_cast<float4*>(take(cap * 16));
+    }
+    w.hits = reinterpret_cast<float4*>(take(cap * 16));
+    w.st_pix = reinterpret_cast<uint32_t*>(take(cap * 4));
+    w.st_prev = reinterpret_cast<float4*>(take(cap * 16));
+    w.st_L = reinterpret_cast<float4*>(take(cap * 16));
+    w.st_Le = reinterpret_cast<float4*>(take(cap * 16));
+    w.st_La = reinterpret_cast<float4*>(take(cap * 16));
+    w.st_alb = reinterpret_cast<float4*>(take(cap * 16));
+    w.st_nrm = reinterpret_cast<float4*>(take(cap * 16));
+    const size_t entries = cap * 2 * nsr;
+    w.s_o = reinterpret_cast<float4*>(take(entries * 16));
+    w.s_d = reinterpret_cast<float4*>(take(entries * 16));
+    w.s_c = reinterpret_cast<float4*>(take(entries * 16));
+    if (off > (size_t)SSPT_NOALLOC_SCRATCH_BYTES) return false;
+    w.capacity = (uint32_t)cap;
+    out->capacity = (uint32_t)cap;
+    out->shadow_entries = entries;
+    return true;
+}
+
 }  // namespace
 
 extern "C" {
@@ -967,6 +1062,9 @@ sspt_status sspt_ctx_create(int32_t device, sspt_ctx** out) {
         delete ctx;
         return SSPT_ERR_DEVICE;
     }
+    // the allocation-free render's scratch on this device (a failure only
+    // disables sspt_render_noalloc, which reports it)
+    ctx->scratch_status = scratch_address(device, &ctx->scratch);
     *out = ctx;
     return SSPT_OK;
 }
@@ -1167,6 +1265,77 @@ sspt_status sspt_render(sspt_ctx* ctx, sspt_scene* sc, const sspt_pt_settings* c
     }
     TRY(read_stats(ctx, sc, stats));
     if (stats) stats->num_iterations = (uint64_t)std::min(num_iterations, waves_run * per_wave);
+    return SSPT_OK;
+}
+
+sspt_status sspt_render_noalloc(sspt_ctx* ctx, sspt_scene* sc, const sspt_pt_settings* cfg,
+                                uint32_t first_iteration, uint32_t num_iterations, sspt_ray_stats* stats) {
+    if (!ctx || !sc || !cfg) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "null argument");
+    if (!valid_settings(*cfg)) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "invalid settings");
+    if (sc->recs.size() != 1)
+        return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "sspt_render_noalloc renders a single scene");
+    if (cfg->max_render_time_ms >= 0)
+        return fail(ctx, SSPT_ERR_UNSUPPORTED, "sspt_render_noalloc takes no time budget (max_render_time_ms < 0)");
+    if (!ctx->scratch)
+        return fail(ctx, SSPT_ERR_DEVICE, "the scratch array of device %d is unavailable: %s", ctx->device,
+                    hipGetErrorString(ctx->scratch_status));
+    ScratchWaves sw;
+    if (!carve_scratch(ctx->scratch, cfg->num_shadow_rays, &sw))
+        return fail(ctx, SSPT_ERR_INVALID_ARGUMENT,
+                    "num_shadow_rays %d leaves less than 256 path slots in the %u-byte scratch array",
+                    cfg->num_shadow_rays, (unsigned)SSPT_NOALLOC_SCRATCH_BYTES);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    SceneRec& R = sc->recs[0];
+    const uint32_t np = R.num_pixels;
+    const uint64_t total = (uint64_t)num_iterations * np;
+    // waves of whole iterations while one fits, else of at most one iteration's paths
+    uint32_t per_wave, whole;
+    if (np <= sw.capacity) {
+        const uint32_t iters = std::max(1u, std::min(num_iterations, sw.capacity / np));
+        per_wave = iters * np;
+        whole = np;
+    } else {
+        const uint32_t parts = (np + sw.capacity - 1) / sw.capacity;
+        per_wave = (np + parts - 1) / parts;
+        whole = 0;
+    }
+    const uint64_t num_waves = total ? (total + per_wave - 1) / per_wave : 0;
+    const bool timing = ctx->timing.enabled;
+    ctx->timing.enabled = false;  // no event objects: kernels are launched directly
+    const sspt_status st = [&]() -> sspt_status {
+        HIP_TRY(ctx, hipMemsetAsync(sw.wave.stats, 0, kStatBytes, ctx->stream));
+        if (!num_waves) return SSPT_OK;
+        apply_settings(R, *cfg);
+        R.first_iter = first_iteration;
+        R.path_off = 0;
+        const uint64_t offs[2] = {0, total};
+        HIP_TRY(ctx, hipMemcpyAsync(sc->d_recs, &R, sizeof(SceneRec), hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(ctx, hipMemcpyAsync(sc->d_path_offsets, offs, sizeof(offs), hipMemcpyHostToDevice, ctx->stream));
+        LaunchConfig lc;
+        lc.trace_blocks = ctx->trace_blocks;
+        lc.shade_blocks = ctx->shade_blocks;
+        lc.max_paths = per_wave;
+        lc.max_nsr = std::max(1, cfg->num_shadow_rays);
+        lc.max_shadow = (uint32_t)std::min<uint64_t>((uint64_t)per_wave * 2u * lc.max_nsr, sw.shadow_entries);
+        float* mean_albedo = cfg->enable_denoiser ? sc->layers + R.fb_off : nullptr;
+        for (uint64_t w0 = 0; w0 < num_waves; w0 += kScratchWaveDescs) {
+            const uint32_t n = (uint32_t)std::min<uint64_t>(kScratchWaveDescs, num_waves - w0);
+            // the descriptors of this block of waves, written on the device behind the previous
+            // block's waves; the counters (queues and wave index) start at zero
+            launch_fill_waves(ctx->stream, sw.descs, n, w0 * per_wave, per_wave, total,
+                              first_iteration, whole);
+            HIP_TRY(ctx, hipMemsetAsync(sw.wave.counters, 0, kCntNum * sizeof(uint32_t), ctx->stream));
+            for (uint32_t i = 0; i < n; ++i)
+                enqueue_wave(ctx, sc, sw.wave, lc, cfg->max_depth, Mode::Mean, sc->image + R.fb_off, np, mean_albedo);
+        }
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        HIP_TRY(ctx, hipGetLastError());
+        return SSPT_OK;
+    }();
+    ctx->timing.enabled = timing;
+    if (st != SSPT_OK) return st;
+    TRY(read_stats(ctx, sc, stats, sw.wave.stats));
+    if (stats) stats->num_iterations = num_iterations;
     return SSPT_OK;
 }
 

@@ -70,13 +70,18 @@ void launch_extend_primary(hipStream_t s, const DevScene& sc, const Wave& w, con
 void launch_shade(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int buf);
 // the shadow rays the shade pass appended with extension queue `slot`
 void launch_shadow(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int slot);
-// running mean of one scene (waves of whole iterations)
+// running mean of one scene: waves of whole iterations, or (WaveDesc.iters = 0)
+// waves of at most one iteration's pixel count
 void launch_accumulate_mean(hipStream_t s, const Wave& w, float* image, uint32_t num_pixels, float* albedo,
                             float* normal, unsigned long long* outliers, int num_outliers);
 // per-scene sums of a batch (any wave layout)
 void launch_accumulate_sum(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc,
                            float* image_pool);
 void launch_wave_end(hipStream_t s, const Wave& w);
+// n running-mean wave descriptors of a single-scene render written on the device
+// (whole = pixel count for waves of whole iterations, 0 for partial-iteration waves)
+void launch_fill_waves(hipStream_t s, WaveDesc* descs, uint32_t n, uint64_t first_path, uint32_t per_wave,
+                       uint64_t total, uint32_t first_iter, uint32_t whole);
 // adjoint (path replay) variants
 void launch_shade_adjoint(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int buf);
 void launch_shadow_adjoint(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int slot);

@@ -1431,58 +1431,98 @@ struct OutlierHeap {
     }
 };
 
+// FrameBuffer running-mean steps of one pixel (FrameBuffer.StartIteration's
+// scale by (n-1)/n, Splat's L/n, SeeSharp/Images/FrameBuffer.cs:89-160,232-255)
+// over `num` samples in iteration order: sample s is in wave slot slot0 + s * stride
+// and belongs to iteration first + s.
+__device__ __forceinline__ void mean_steps(const Wave& w, uint32_t pix, uint32_t slot0, uint32_t stride,
+                                           uint32_t num, uint32_t first, float* image, float* albedo,
+                                           float* normal, unsigned long long* outliers, int num_outliers,
+                                           uint32_t* nans) {
+    f3 img = mk3(image[3 * pix], image[3 * pix + 1], image[3 * pix + 2]);
+    f3 alb = gray(0.0f), nrm = gray(0.0f);
+    if (albedo) {
+        alb = mk3(albedo[3 * pix], albedo[3 * pix + 1], albedo[3 * pix + 2]);
+        nrm = mk3(normal[3 * pix], normal[3 * pix + 1], normal[3 * pix + 2]);
+    }
+    OutlierHeap heap;
+    unsigned long long* ok = outliers ? outliers + (size_t)pix * num_outliers : nullptr;
+    if (ok) heap.load(ok, num_outliers);
+    for (uint32_t s = 0; s < num; ++s) {
+        uint32_t slot = slot0 + s * stride;
+        f3 L = f3_of(w.st_L[slot]) + f3_of(w.st_Le[slot]) + f3_of(w.st_La[slot]);
+        float n = (float)(first + s + 1);
+        if (first + s + 1 > 1) img = img * ((n - 1.0f) / n);
+        if (isfinite(average(L))) {
+            img = img + div_exact(L, n);
+            if (ok) heap.add(average(L), first + s, num_outliers);
+        } else {
+            ++*nans;
+        }
+        if (albedo) {
+            if (first + s + 1 > 1) {
+                alb = alb * ((n - 1.0f) / n);
+                nrm = nrm * ((n - 1.0f) / n);
+            }
+            alb = alb + div_exact(f3_of(w.st_alb[slot]), n);
+            nrm = nrm + div_exact(f3_of(w.st_nrm[slot]), n);
+        }
+    }
+    image[3 * pix] = img.x;
+    image[3 * pix + 1] = img.y;
+    image[3 * pix + 2] = img.z;
+    if (ok) heap.store(ok, num_outliers);
+    if (albedo) {
+        albedo[3 * pix] = alb.x;
+        albedo[3 * pix + 1] = alb.y;
+        albedo[3 * pix + 2] = alb.z;
+        normal[3 * pix] = nrm.x;
+        normal[3 * pix + 1] = nrm.y;
+        normal[3 * pix + 2] = nrm.z;
+    }
+}
+
 __global__ void __launch_bounds__(256) k_accumulate_mean(Wave w, float* image, uint32_t num_pixels, float* albedo,
                                                           float* normal, unsigned long long* outliers,
                                                           int num_outliers) {
     const WaveDesc d = cur_wave(w);
-    // k: a path's index within its iteration; slot k of the wave's first
-    // iteration records which pixel that is, and every iteration of the wave
-    // lays its pixels out in the same order
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < num_pixels; k += gridDim.x * blockDim.x) {
-        const uint32_t pix = w.st_pix[k] & kSlotMask;
-        f3 img = mk3(image[3 * pix], image[3 * pix + 1], image[3 * pix + 2]);
-        f3 alb = gray(0.0f), nrm = gray(0.0f);
-        if (albedo) {
-            alb = mk3(albedo[3 * pix], albedo[3 * pix + 1], albedo[3 * pix + 2]);
-            nrm = mk3(normal[3 * pix], normal[3 * pix + 1], normal[3 * pix + 2]);
+    uint32_t nans = 0;
+    if (d.iters == 0) {
+        // a wave over part of one or two iterations: render paths start .. start + count - 1
+        // with count <= num_pixels, so no pixel occurs twice and each slot takes one step
+        for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < d.count;
+             slot += gridDim.x * blockDim.x) {
+            const uint32_t pix = w.st_pix[slot] & kSlotMask;
+            const uint32_t it = d.first_iter + (uint32_t)((d.start + slot) / num_pixels);
+            mean_steps(w, pix, slot, 0, 1, it, image, albedo, normal, outliers, num_outliers, &nans);
         }
-        OutlierHeap heap;
-        unsigned long long* ok = outliers ? outliers + (size_t)pix * num_outliers : nullptr;
-        if (ok) heap.load(ok, num_outliers);
-        uint32_t nans = 0;
-        for (uint32_t s = 0; s < d.iters; ++s) {
-            uint32_t slot = s * num_pixels + k;
-            f3 L = f3_of(w.st_L[slot]) + f3_of(w.st_Le[slot]) + f3_of(w.st_La[slot]);
-            float n = (float)(d.first_iter + s + 1);
-            if (d.first_iter + s + 1 > 1) img = img * ((n - 1.0f) / n);
-            if (isfinite(average(L))) {
-                img = img + div_exact(L, n);
-                if (ok) heap.add(average(L), d.first_iter + s, num_outliers);
-            } else {
-                ++nans;
-            }
-            if (albedo) {
-                if (d.first_iter + s + 1 > 1) {
-                    alb = alb * ((n - 1.0f) / n);
-                    nrm = nrm * ((n - 1.0f) / n);
-                }
-                alb = alb + div_exact(f3_of(w.st_alb[slot]), n);
-                nrm = nrm + div_exact(f3_of(w.st_nrm[slot]), n);
-            }
-        }
-        image[3 * pix] = img.x;
-        image[3 * pix + 1] = img.y;
-        image[3 * pix + 2] = img.z;
-        if (ok) heap.store(ok, num_outliers);
-        if (albedo) {
-            albedo[3 * pix] = alb.x;
-            albedo[3 * pix + 1] = alb.y;
-            albedo[3 * pix + 2] = alb.z;
-            normal[3 * pix] = nrm.x;
-            normal[3 * pix + 1] = nrm.y;
-            normal[3 * pix + 2] = nrm.z;
-        }
-        if (nans) atomicAdd(&w.stats[(blockIdx.x & (kStatStripes - 1)) * kStatStride + kStatNan], (unsigned long long)nans);
+    } else {
+        // k: a path's index within its iteration; slot k of the wave's first
+        // iteration records which pixel that is, and every iteration of the wave
+        // lays its pixels out in the same order
+        for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < num_pixels; k += gridDim.x * blockDim.x)
+            mean_steps(w, w.st_pix[k] & kSlotMask, k, num_pixels, d.iters, d.first_iter, image, albedo, normal,
+                       outliers, num_outliers, &nans);
+    }
+    if (nans) atomicAdd(&w.stats[(blockIdx.x & (kStatStripes - 1)) * kStatStride + kStatNan], (unsigned long long)nans);
+}
+
+// Descriptors of n running-mean waves of `per_wave` paths each (the last may be
+// shorter) from render path first_path on, of a single-scene render of `total`
+// paths starting at iteration first_iter: whole iterations when `whole` (the
+// pixel count) divides per_wave, else (whole = 0) partial-iteration waves
+// (iters = 0) of at most one iteration's paths.
+__global__ void k_fill_waves(WaveDesc* descs, uint32_t n, uint64_t first_path, uint32_t per_wave, uint64_t total,
+                             uint32_t first_iter, uint32_t whole) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint64_t start = first_path + (uint64_t)i * per_wave;
+        WaveDesc d;
+        d.start = start;
+        d.count = (uint32_t)min((uint64_t)per_wave, total - start);
+        d.first_iter = whole ? first_iter + (uint32_t)(start / whole) : first_iter;
+        d.iters = whole ? d.count / whole : 0;
+        d.pad = 0;
+        descs[i] = d;
     }
 }
 
@@ -1648,6 +1688,12 @@ void launch_accumulate_mean(hipStream_t s, const Wave& w, float* image, uint32_t
     hipLaunchKernelGGL(k_accumulate_mean, dim3(std::min(blocks(num_pixels, 256), 8192u)), dim3(256), 0, s, w, image,
                        num_pixels, albedo, normal, outliers, num_outliers);
 }
+void launch_fill_waves(hipStream_t s, WaveDesc* descs, uint32_t n, uint64_t first_path, uint32_t per_wave,
+                       uint64_t total, uint32_t first_iter, uint32_t whole) {
+    hipLaunchKernelGGL(k_fill_waves, dim3(std::max(1u, std::min(blocks(n, 256), 64u))), dim3(256), 0, s, descs, n,
+                       first_path, per_wave, total, first_iter, whole);
+}
+
 void launch_accumulate_sum(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, float* pool) {
     hipLaunchKernelGGL(k_accumulate_sum, dim3(grid_of(lc.shade_blocks, lc.max_paths, 256)), dim3(256), 0, s, sc, w,
                        pool);

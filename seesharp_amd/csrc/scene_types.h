@@ -158,7 +158,8 @@ struct DevScene {
 
 // Device-side per-wave work: the wave covers world-wide paths
 // [start, start + count); in running-mean mode it covers whole iterations
-// first_iter .. first_iter + iters - 1 of a single scene.
+// first_iter .. first_iter + iters - 1 of a single scene, or, with iters = 0,
+// at most one iteration's worth of paths whose iterations count from first_iter.
 struct WaveDesc {
     uint64_t start;
     uint32_t count;

@@ -169,6 +169,14 @@ class GpuScene(_TextureParameters):
                                                       first_iteration, num_iterations, C.byref(stats)))
         return stats.as_dict()
 
+    def render_noalloc(self, settings: abi.PtSettings, first_iteration: int, num_iterations: int) -> dict:
+        """render() without device allocation: the wave buffers live in the
+        library's per-GPU scratch array (sspt_render_noalloc)."""
+        stats = abi.RayStats()
+        self.device.check(self.device.lib.sspt_render_noalloc(self.device.ptr, self.ptr, C.byref(settings),
+                                                              first_iteration, num_iterations, C.byref(stats)))
+        return stats.as_dict()
+
     def render_mean(self, settings: abi.PtSettings, first_iteration: int, num_iterations: int) -> np.ndarray:
         """Mean of the estimates of iterations first..first+num-1 as an (H, W, 3)
         image. The framebuffer follows FrameBuffer's running mean, which after a
