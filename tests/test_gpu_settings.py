@@ -11,7 +11,7 @@ Same-seed bar: the GPU and the oracle draw the same RNG streams and differ by th
 hardware square root / reciprocal in the shading arithmetic (a few ulps), which now
 and then sends one sample another way. Observed relMSE = mean((x - r)^2 / (r^2 + 1e-2))
 on the box: 1e-15 .. 6e-13, and 1.2e-7 for NumShadowRays 2 (one diverted sample).
-Bars: relMSE <= 1e-6, at most 0.1 % of the pixels off by more than rtol 1e-4 (a bias
+Bars: relMSE <= 1e-6, at most 0.1 % of the pixels off by more than rtol 1e-3 (a bias
 on 5 % of the pixels at 10 % would give relMSE ~5e-4 and fail both), and the
 closest-hit, shadow-ray and hit counters within 0.1 %. (num_occluded is not
 compared: the GPU counts only the shadow rays it traces, skipping those whose
@@ -35,7 +35,7 @@ def relmse(x, r):
 
 def diverted(x, r):
     """Fraction of pixels whose estimate differs from the oracle's beyond rounding."""
-    return float(np.mean(np.any(~np.isclose(x, r, rtol=1e-4, atol=1e-6), axis=-1)))
+    return float(np.mean(np.any(~np.isclose(x, r, rtol=1e-3, atol=1e-5), axis=-1)))
 
 
 @pytest.fixture(scope="module")
