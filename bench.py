@@ -151,6 +151,8 @@ def measure_batch(dev, steps, autotune_first=False):
     if autotune_first:
         tuned, _ = autotune.tune(dev, lambda: batch.render(cfgs), capacities=(1 << 22, 1 << 23, 1 << 24), log=log)
         log(f"[bench] autotuned: {tuned}")
+        tuned = {k: tuned[k] for k in ("trace_block", "shade_block", "wave_capacity", "trace_blocks", "shade_blocks",
+                                       "ms")}
     batch.render(cfgs)  # warm-up
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -201,7 +203,7 @@ def measure_batch(dev, steps, autotune_first=False):
 def batch_main(args):
     from seesharp_amd import Device
 
-    r = measure_batch(Device(0), args.steps, args.autotune)
+    r = measure_batch(Device(0), args.steps, not args.no_autotune)
     out = {
         "metric": "Mpaths/s of the batched wavefront (64 scenes, one world) vs per-scene launches",
         "value": r["batched_mpaths_per_s"], "unit": "Mpaths/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
@@ -468,7 +470,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the adjoint (textured box, 32 views) and batching measurements")
     ap.add_argument("--workload", choices=("large", "batch", "adjoint", "views", "copy", "cornell"), default="large")
-    ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="batching: keep the default block shape, wave capacity and grids instead of tuning them")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -634,7 +637,7 @@ def main():
         # the metric's second half and the batching form, each timed in this run
         adj = measure_adjoint(dev, 3, kernels=False)
         vw = measure_views(dev, 1)
-        bt = measure_batch(dev, 1)
+        bt = measure_batch(dev, 1, autotune_first=not args.no_autotune)
         out["adjoint_forward_ratio"] = {"textured_box_512": adj["ratio"], "views_32x512_large": vw["ratio"]}
         out["adjoint"] = {
             "textured_box_512": {k: adj[k] for k in ("workload", "forward_ms", "adjoint_ms", "paths_per_pass")},
@@ -643,7 +646,7 @@ def main():
         out["batching"] = {k: bt[k] for k in ("workload", "triangles", "paths_per_pass", "batched_mpaths_per_s",
                                               "per_scene_launches_mpaths_per_s", "speedup_vs_per_scene",
                                               "per_scene_plain_single_kernel_mpaths_per_s",
-                                              "speedup_vs_per_scene_plain")}
+                                              "speedup_vs_per_scene_plain", "autotuned")}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene, args.spp, args.max_depth, args.cpu_budget)
     print(json.dumps(out), flush=True)

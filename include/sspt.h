@@ -243,8 +243,16 @@ typedef enum sspt_exec_mode {
     SSPT_EXEC_SINGLE_KERNEL = 1
 } sspt_exec_mode;
 sspt_status sspt_set_execution_mode(sspt_ctx* ctx, int32_t mode);
+/* Block shape: threads per block of the traversal kernels (64, 128 or 256)
+ * and of the forward shade kernel (128, 256 or 512), 0 = the defaults (128,
+ * 256); each is a prebuilt kernel variant. Grid sizes set with
+ * sspt_set_launch_config count blocks of these sizes. Images do not depend
+ * on the shape. */
+sspt_status sspt_set_block_shape(sspt_ctx* ctx, uint32_t trace_block, uint32_t shade_block);
+sspt_status sspt_get_block_shape(sspt_ctx* ctx, uint32_t* trace_block, uint32_t* shade_block);
 /* Configured grid sizes (0 = full grid) and the blocks per CU of the
- * traversal and shading kernels that are resident at once. */
+ * traversal and shading kernels (of the current block shape) that are
+ * resident at once. */
 sspt_status sspt_get_launch_config(sspt_ctx* ctx, uint32_t* trace_blocks, uint32_t* shade_blocks,
                                    uint32_t* trace_per_cu, uint32_t* shade_per_cu);
 

@@ -194,6 +194,8 @@ SYMBOLS = {
     "sspt_get_launch_config": (C.c_int, [_vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                          C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "sspt_set_launch_config": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_int32]),
+    "sspt_set_block_shape": (C.c_int, [_vp, C.c_uint32, C.c_uint32]),
+    "sspt_get_block_shape": (C.c_int, [_vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "sspt_set_kernel_timing": (C.c_int, [_vp, C.c_int32]),
     "sspt_get_kernel_timing": (C.c_int, [_vp, C.c_char_p, C.POINTER(C.c_uint64),
                                          C.POINTER(C.c_double)]),

@@ -87,6 +87,7 @@ struct sspt_ctx {
     // execution knobs (0 = default)
     uint32_t want_capacity = 0;
     uint32_t trace_blocks = 0, shade_blocks = 0;
+    uint32_t trace_block = 0, shade_block = 0;  // threads per block (0 = the default sizes)
     bool use_graphs = true;
     int32_t exec_mode = SSPT_EXEC_WAVEFRONT;
     // adjoint buffers, grown on demand
@@ -763,6 +764,8 @@ LaunchConfig launch_config(const sspt_ctx* ctx, uint32_t max_paths, uint32_t max
     LaunchConfig lc;
     lc.trace_blocks = ctx->trace_blocks;
     lc.shade_blocks = ctx->shade_blocks;
+    lc.trace_block = ctx->trace_block;
+    lc.shade_block = ctx->shade_block;
     lc.max_paths = max_paths;
     lc.max_nsr = (int)std::max(1u, max_nsr);
     const uint64_t shadow = (uint64_t)max_paths * 2u * std::max(1u, max_nsr);
@@ -1057,7 +1060,7 @@ sspt_status sspt_ctx_create(int32_t device, sspt_ctx** out) {
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
         ctx->num_cus = cus;
-    resident_blocks_per_cu(&ctx->trace_per_cu, &ctx->shade_per_cu);
+    resident_blocks_per_cu(0, 0, &ctx->trace_per_cu, &ctx->shade_per_cu);
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return SSPT_ERR_DEVICE;
@@ -1314,6 +1317,8 @@ sspt_status sspt_render_noalloc(sspt_ctx* ctx, sspt_scene* sc, const sspt_pt_set
         LaunchConfig lc;
         lc.trace_blocks = ctx->trace_blocks;
         lc.shade_blocks = ctx->shade_blocks;
+        lc.trace_block = ctx->trace_block;
+        lc.shade_block = ctx->shade_block;
         lc.max_paths = per_wave;
         lc.max_nsr = std::max(1, cfg->num_shadow_rays);
         lc.max_shadow = (uint32_t)std::min<uint64_t>((uint64_t)per_wave * 2u * lc.max_nsr, sw.shadow_entries);
@@ -1666,6 +1671,26 @@ sspt_status sspt_get_launch_config(sspt_ctx* ctx, uint32_t* trace_blocks, uint32
     *shade_blocks = ctx->shade_blocks;
     *trace_per_cu = (uint32_t)std::max(0, ctx->trace_per_cu);
     *shade_per_cu = (uint32_t)std::max(0, ctx->shade_per_cu);
+    return SSPT_OK;
+}
+
+sspt_status sspt_set_block_shape(sspt_ctx* ctx, uint32_t trace_block, uint32_t shade_block) {
+    if (!ctx) return SSPT_ERR_INVALID_ARGUMENT;
+    if (!valid_block_shape(trace_block, shade_block))
+        return fail(ctx, SSPT_ERR_INVALID_ARGUMENT,
+                    "block shape %u x %u: traversal blocks of 64, 128 or 256 threads, shade blocks of 128, 256 or 512",
+                    trace_block, shade_block);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    ctx->trace_block = trace_block;
+    ctx->shade_block = shade_block;
+    resident_blocks_per_cu(trace_block, shade_block, &ctx->trace_per_cu, &ctx->shade_per_cu);
+    return SSPT_OK;
+}
+
+sspt_status sspt_get_block_shape(sspt_ctx* ctx, uint32_t* trace_block, uint32_t* shade_block) {
+    if (!ctx || !trace_block || !shade_block) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "null argument");
+    *trace_block = ctx->trace_block ? ctx->trace_block : (uint32_t)kTraceBlock;
+    *shade_block = ctx->shade_block ? ctx->shade_block : 256u;
     return SSPT_OK;
 }
 

@@ -52,8 +52,10 @@ struct Wave {
 // the hardware dispatcher balances the rest block by block. A non-zero size
 // gives persistent blocks that stride over the queue.
 struct LaunchConfig {
-    uint32_t trace_blocks;   // blocks of kTraceBlock threads for extend / shadow (0 = full grid)
-    uint32_t shade_blocks;   // blocks of 256 threads for generate / shade / accumulate (0 = full grid)
+    uint32_t trace_blocks;   // blocks of trace_block threads for extend / shadow (0 = full grid)
+    uint32_t shade_blocks;   // blocks of shade_block threads for shade, 256 for generate / accumulate (0 = full grid)
+    uint32_t trace_block = 0;  // threads per traversal block: 64, 128 or 256 (0 = 128)
+    uint32_t shade_block = 0;  // threads per forward shade block: 128, 256 or 512 (0 = 256)
     uint32_t max_paths;      // largest wave of the render (paths)
     uint32_t max_shadow;     // largest shadow queue of the render (rays)
     int max_nsr;             // most shadow rays per kind and vertex over the render's scenes
@@ -90,8 +92,10 @@ void launch_trace(hipStream_t s, const DevScene& sc, const Wave& w, const Launch
 void launch_adjoint_finalize(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc);
 void launch_scale(hipStream_t s, float* data, uint64_t n, float factor);
 // Blocks per CU that fit at once (register / LDS limits) for the trace kernels
-// (extend, shadow) and the shade kernel; 0 if the query fails.
-void resident_blocks_per_cu(int* trace, int* shade);
+// (extend, shadow) and the shade kernel of the given block sizes; 0 if the query fails.
+void resident_blocks_per_cu(uint32_t trace_block, uint32_t shade_block, int* trace, int* shade);
+// Whether the block sizes are prebuilt ones (0 = the default size).
+bool valid_block_shape(uint32_t trace_block, uint32_t shade_block);
 
 // Ray queries on scene `scene` of the world: rays are 8 floats (ox,oy,oz,tmin,dx,dy,dz,tmax).
 void launch_trace_closest(hipStream_t s, const DevScene& sc, int scene, const float* rays, uint64_t n,

@@ -230,8 +230,9 @@ struct HitOut {
 // array that only deep trees reach. The private arrays are separate locals of
 // the caller so that the traversal state around them stays in registers.
 constexpr int kStackSpill = kStackTotal - kStackLds;
+template <int TB>  // threads per block: the stride of a lane's LDS entries
 struct TraceStack {
-    int* lds_node;    // + threadIdx.x, stride kTraceBlock
+    int* lds_node;    // + threadIdx.x, stride TB
     float* lds_t;
     int* spill_node;  // private arrays of kStackSpill entries, owned by the caller
     float* spill_t;
@@ -241,8 +242,8 @@ struct TraceStack {
         // guard keeps memory safe regardless.
         if (sp >= kStackTotal) return;
         if (sp < kStackLds) {
-            lds_node[sp * kTraceBlock] = node;
-            lds_t[sp * kTraceBlock] = t;
+            lds_node[sp * TB] = node;
+            lds_t[sp * TB] = t;
         } else {
             spill_node[sp - kStackLds] = node;
             spill_t[sp - kStackLds] = t;
@@ -252,8 +253,8 @@ struct TraceStack {
     __device__ __forceinline__ void pop(int* node, float* t) {
         --sp;
         if (sp < kStackLds) {
-            *node = lds_node[sp * kTraceBlock];
-            *t = lds_t[sp * kTraceBlock];
+            *node = lds_node[sp * TB];
+            *t = lds_t[sp * TB];
         } else {
             *node = spill_node[sp - kStackLds];
             *t = spill_t[sp - kStackLds];
@@ -286,15 +287,15 @@ __device__ __forceinline__ void cswap(float& ta, int& na, float& tb, int& nb) {
 // meets [tmin, best], the closest hit is decided by distance and then by the
 // lower global id whatever the test order, and an any-hit query only asks
 // whether some triangle is hit.
-template <bool ANY>
+template <bool ANY, int TB>
 struct Tracer {
     f3 o;
     RayPre p;
     float tmin, slack, tmin_box;
     uint32_t near_x, near_y, near_z;  // byte offsets of the near bound vectors in a node
     HitOut best;
-    TraceStack st;
-    int* pend;     // FIFO of queued leaves in LDS (+ threadIdx.x, stride kTraceBlock)
+    TraceStack<TB> st;
+    int* pend;     // FIFO of queued leaves in LDS (+ threadIdx.x, stride TB)
     uint32_t ph;    // FIFO head
     uint32_t pn;    // FIFO length
     int node;       // next node to visit, -1 when none is left
@@ -387,7 +388,7 @@ struct Tracer {
             const int n = k == 0 ? ccnt[0] : k == 1 ? ccnt[1] : k == 2 ? ccnt[2] : ccnt[3];
             if (n <= 4) {
                 // entry: first triangle (< 2^30) and count - 1
-                pend[((ph + pn) & (uint32_t)(kPendCap - 1)) * kTraceBlock] = tri | ((n - 1) << 30);
+                pend[((ph + pn) & (uint32_t)(kPendCap - 1)) * TB] = tri | ((n - 1) << 30);
                 ++pn;
             } else {
                 test_leaf(S, tri, n);
@@ -426,7 +427,7 @@ struct Tracer {
 
     // Tests the oldest queued leaf.
     __device__ __forceinline__ void tri_step(const DevScene& S) {
-        const uint32_t e = (uint32_t)pend[(ph & (uint32_t)(kPendCap - 1)) * kTraceBlock];
+        const uint32_t e = (uint32_t)pend[(ph & (uint32_t)(kPendCap - 1)) * TB];
         ++ph;
         --pn;
         test_leaf(S, (int)(e & 0x3fffffffu), (int)(e >> 30) + 1);
@@ -435,17 +436,17 @@ struct Tracer {
 
 // Runs one ray per lane to its end. The active lanes of a wave call it
 // together: the choice between node and triangle steps is made per wave.
-template <bool ANY>
+template <bool ANY, int TB = kTraceBlock>
 __device__ __forceinline__ HitOut traverse(const DevScene& S, int root, float abs_max, f3 o, f3 d, float tmin,
                                            float tmax, int* lds_node, float* lds_t) {
     int spill_node[kStackSpill];
     float spill_t[kStackSpill];
-    Tracer<ANY> tr;
+    Tracer<ANY, TB> tr;
     tr.st.lds_node = lds_node;
     tr.st.lds_t = lds_t;
     tr.st.spill_node = spill_node;
     tr.st.spill_t = spill_t;
-    tr.pend = lds_node + kStackLds * kTraceBlock;
+    tr.pend = lds_node + kStackLds * TB;
     tr.start(root, abs_max, o, d, tmin, tmax);
     for (;;) {
         if (tr.can_step()) tr.node_step(S);
@@ -571,6 +572,7 @@ __global__ void __launch_bounds__(256) k_generate(DevScene S, Wave w) {
 
 // Closest hits of the extension queue `buf`, over blocks [block, ...) of a
 // grid of `nblocks` blocks that stride over the device-side queue length.
+template <int TB>
 __device__ __forceinline__ void extend_rays(const DevScene& S, const Wave& w, int buf, uint32_t block,
                                             uint32_t nblocks, int* stack_node, float* stack_t) {
     const uint32_t count = queue_count(w, buf);
@@ -584,8 +586,8 @@ __device__ __forceinline__ void extend_rays(const DevScene& S, const Wave& w, in
         if (active) {
             float4 o4 = q_o[i], d4 = q_d[i];
             const SceneRec& R = S.recs[q_slot[i] >> kSlotBits];
-            h = traverse<false>(S, R.root, R.abs_max, f3_of(o4), f3_of(d4), o4.w, d4.w, stack_node + threadIdx.x,
-                                stack_t + threadIdx.x);
+            h = traverse<false, TB>(S, R.root, R.abs_max, f3_of(o4), f3_of(d4), o4.w, d4.w, stack_node + threadIdx.x,
+                                    stack_t + threadIdx.x);
             w.hits[i] = h.li >= 0 ? make_float4(h.t, h.u, h.v, __int_as_float(h.li))
                                   : make_float4(INFINITY, 0.0f, 0.0f, __int_as_float(-1));
         }
@@ -596,7 +598,7 @@ __device__ __forceinline__ void extend_rays(const DevScene& S, const Wave& w, in
 
 // Any-hit tests of the shadow queue; a visible ray adds its contribution to
 // its path's next-event sum (and, with ADJ, its own gradient term).
-template <bool ADJ>
+template <bool ADJ, int TB>
 __device__ __forceinline__ void shadow_rays(const DevScene& S, const Wave& w, int slot, uint32_t block,
                                             uint32_t nblocks,
                                             int* stack_node, float* stack_t) {
@@ -610,8 +612,8 @@ __device__ __forceinline__ void shadow_rays(const DevScene& S, const Wave& w, in
             float4 c = w.s_c[i];
             uint32_t tag = (uint32_t)__float_as_int(c.w);
             const SceneRec& R = S.recs[tag >> (kSlotBits + 1)];
-            HitOut h = traverse<true>(S, R.root, R.abs_max, f3_of(o4), f3_of(d4), o4.w, d4.w,
-                                      stack_node + threadIdx.x, stack_t + threadIdx.x);
+            HitOut h = traverse<true, TB>(S, R.root, R.abs_max, f3_of(o4), f3_of(d4), o4.w, d4.w,
+                                          stack_node + threadIdx.x, stack_t + threadIdx.x);
             occluded = h.li >= 0;
             if (!occluded) {
                 // the contribution is non-zero by construction
@@ -641,10 +643,11 @@ __device__ __forceinline__ void shadow_rays(const DevScene& S, const Wave& w, in
 // k_generate and the closest hits of its queue in one pass: each thread starts
 // its path (RNG, jitter, camera ray), writes the path state and traces the
 // camera ray at once, so the ray never goes through the queue.
-__global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(SSPT_TRACE_WAVES)))
+template <int TB>
+__global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(SSPT_TRACE_WAVES)))
 k_extend_primary(DevScene S, Wave w) {
-    __shared__ int stack_node[kStackNodeLds];
-    __shared__ float stack_t[kStackLds * kTraceBlock];
+    __shared__ int stack_node[(kStackLds + kPendCap) * TB];
+    __shared__ float stack_t[kStackLds * TB];
     const WaveDesc d = cur_wave(w);
     for (uint32_t base = blockIdx.x * blockDim.x; base < d.count; base += gridDim.x * blockDim.x) {
         const uint32_t slot = base + threadIdx.x;
@@ -654,7 +657,7 @@ k_extend_primary(DevScene S, Wave w) {
             const PathStart p = start_path(S, d.start + slot);
             init_path(S, w, slot, p);
             const SceneRec& R = S.recs[p.scene];
-            h = traverse<false>(S, R.root, R.abs_max, p.o, p.dir, 0.0f, INFINITY, stack_node + threadIdx.x,
+            h = traverse<false, TB>(S, R.root, R.abs_max, p.o, p.dir, 0.0f, INFINITY, stack_node + threadIdx.x,
                                 stack_t + threadIdx.x);
             w.hits[slot] = h.li >= 0 ? make_float4(h.t, h.u, h.v, __int_as_float(h.li))
                                      : make_float4(INFINITY, 0.0f, 0.0f, __int_as_float(-1));
@@ -664,34 +667,35 @@ k_extend_primary(DevScene S, Wave w) {
     }
 }
 
-__global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(SSPT_TRACE_WAVES)))
+template <int TB>
+__global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(SSPT_TRACE_WAVES)))
 k_extend(DevScene S, Wave w, int buf) {
-    __shared__ int stack_node[kStackNodeLds];
-    __shared__ float stack_t[kStackLds * kTraceBlock];
-    extend_rays(S, w, buf, blockIdx.x, gridDim.x, stack_node, stack_t);
+    __shared__ int stack_node[(kStackLds + kPendCap) * TB];
+    __shared__ float stack_t[kStackLds * TB];
+    extend_rays<TB>(S, w, buf, blockIdx.x, gridDim.x, stack_node, stack_t);
 }
 
-template <bool ADJ>
-__global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(SSPT_TRACE_WAVES)))
+template <bool ADJ, int TB>
+__global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(SSPT_TRACE_WAVES)))
 k_shadow(DevScene S, Wave w, int slot) {
-    __shared__ int stack_node[kStackNodeLds];
-    __shared__ float stack_t[kStackLds * kTraceBlock];
-    shadow_rays<ADJ>(S, w, slot, blockIdx.x, gridDim.x, stack_node, stack_t);
+    __shared__ int stack_node[(kStackLds + kPendCap) * TB];
+    __shared__ float stack_t[kStackLds * TB];
+    shadow_rays<ADJ, TB>(S, w, slot, blockIdx.x, gridDim.x, stack_node, stack_t);
 }
 
 // One launch for the two traversals that are ready after a bounce's shade:
 // the continuation rays of the next bounce (blocks [0, extend_blocks)) and
 // the bounce's shadow rays (the remaining blocks). Neither reads what the
 // other writes, and a single grid lets the tail of one overlap the other.
-template <bool ADJ>
-__global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(SSPT_TRACE_WAVES)))
+template <bool ADJ, int TB>
+__global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(SSPT_TRACE_WAVES)))
 k_trace(DevScene S, Wave w, int buf, uint32_t extend_blocks) {
-    __shared__ int stack_node[kStackNodeLds];
-    __shared__ float stack_t[kStackLds * kTraceBlock];
+    __shared__ int stack_node[(kStackLds + kPendCap) * TB];
+    __shared__ float stack_t[kStackLds * TB];
     if (blockIdx.x < extend_blocks)
-        extend_rays(S, w, buf, blockIdx.x, extend_blocks, stack_node, stack_t);
+        extend_rays<TB>(S, w, buf, blockIdx.x, extend_blocks, stack_node, stack_t);
     else
-        shadow_rays<ADJ>(S, w, buf, blockIdx.x - extend_blocks, gridDim.x - extend_blocks, stack_node, stack_t);
+        shadow_rays<ADJ, TB>(S, w, buf, blockIdx.x - extend_blocks, gridDim.x - extend_blocks, stack_node, stack_t);
 }
 
 // TinyEmbree SpawnRay / IsOccluded offset rule: move along the face normal by
@@ -960,10 +964,10 @@ struct PathVertex {
 };
 
 // One bounce of EstimateIncidentRadiance for every path of the queue.
-template <bool ADJ>
-__global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(ADJ ? SSPT_SHADE_ADJ_WAVES : SSPT_SHADE_WAVES)))
+template <bool ADJ, int SB>
+__global__ void __launch_bounds__(SB) __attribute__((amdgpu_waves_per_eu(ADJ ? SSPT_SHADE_ADJ_WAVES : SSPT_SHADE_WAVES)))
 k_shade(DevScene S, Wave w, int buf, int max_nsr) {
-    __shared__ uint32_t app[3 * (kShadeBlock / 64) + 2];  // block_append_word scratch
+    __shared__ uint32_t app[3 * (SB / 64) + 2];  // block_append_word scratch
     const uint32_t count = queue_count(w, buf);
     const int nb = buf ^ 1;
     for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += gridDim.x * blockDim.x) {
@@ -1638,6 +1642,27 @@ static uint32_t grid_of(uint32_t configured, uint64_t max_items, uint32_t block)
     return (uint32_t)std::max<uint64_t>(1, (max_items + block - 1) / block);
 }
 
+// The prebuilt block sizes (the autotuner's block-shape axis): traversal kernels
+// of 64, 128 (default) or 256 threads, forward shade blocks of 128, 256 (default)
+// or 512 threads. A block's lanes share nothing but LDS stack space and queue
+// appends, so every shape renders the same images.
+template <class F>
+static void with_trace_block(uint32_t tb, F&& f) {
+    if (tb == 64) f(std::integral_constant<int, 64>{});
+    else if (tb == 256) f(std::integral_constant<int, 256>{});
+    else f(std::integral_constant<int, kTraceBlock>{});
+}
+template <class F>
+static void with_shade_block(uint32_t sb, F&& f) {
+    if (sb == 128) f(std::integral_constant<int, 128>{});
+    else if (sb == 512) f(std::integral_constant<int, 512>{});
+    else f(std::integral_constant<int, kShadeBlock>{});
+}
+bool valid_block_shape(uint32_t trace_block, uint32_t shade_block) {
+    return (trace_block == 0 || trace_block == 64 || trace_block == 128 || trace_block == 256) &&
+           (shade_block == 0 || shade_block == 128 || shade_block == 256 || shade_block == 512);
+}
+
 void launch_path(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc) {
     hipLaunchKernelGGL(k_path, dim3(grid_of(lc.trace_blocks, lc.max_paths, kTraceBlock)), dim3(kTraceBlock), 0, s, sc,
                        w);
@@ -1647,38 +1672,54 @@ void launch_generate(hipStream_t s, const DevScene& sc, const Wave& w, const Lau
     hipLaunchKernelGGL(k_generate, dim3(grid_of(lc.shade_blocks, lc.max_paths, 256)), dim3(256), 0, s, sc, w);
 }
 void launch_extend_primary(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc) {
-    hipLaunchKernelGGL(k_extend_primary, dim3(grid_of(lc.trace_blocks, lc.max_paths, kTraceBlock)), dim3(kTraceBlock),
-                       0, s, sc, w);
+    with_trace_block(lc.trace_block, [&](auto tb) {
+        constexpr int TB = decltype(tb)::value;
+        hipLaunchKernelGGL(k_extend_primary<TB>, dim3(grid_of(lc.trace_blocks, lc.max_paths, TB)), dim3(TB), 0, s, sc,
+                           w);
+    });
 }
 void launch_extend(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int buf) {
-    hipLaunchKernelGGL(k_extend, dim3(grid_of(lc.trace_blocks, lc.max_paths, kTraceBlock)), dim3(kTraceBlock), 0, s,
-                       sc, w, buf);
+    with_trace_block(lc.trace_block, [&](auto tb) {
+        constexpr int TB = decltype(tb)::value;
+        hipLaunchKernelGGL(k_extend<TB>, dim3(grid_of(lc.trace_blocks, lc.max_paths, TB)), dim3(TB), 0, s, sc, w, buf);
+    });
 }
 void launch_shade(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int buf) {
-    hipLaunchKernelGGL(k_shade<false>, dim3(grid_of(lc.shade_blocks, lc.max_paths, kShadeBlock)), dim3(kShadeBlock), 0,
-                       s, sc, w,
-                       buf, lc.max_nsr);
+    with_shade_block(lc.shade_block, [&](auto sbk) {
+        constexpr int SB = decltype(sbk)::value;
+        hipLaunchKernelGGL((k_shade<false, SB>), dim3(grid_of(lc.shade_blocks, lc.max_paths, SB)), dim3(SB), 0, s, sc,
+                           w, buf, lc.max_nsr);
+    });
 }
 void launch_shadow(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int slot) {
-    hipLaunchKernelGGL(k_shadow<false>, dim3(grid_of(lc.trace_blocks, lc.max_shadow, kTraceBlock)), dim3(kTraceBlock),
-                       0, s, sc, w, slot);
+    with_trace_block(lc.trace_block, [&](auto tb) {
+        constexpr int TB = decltype(tb)::value;
+        hipLaunchKernelGGL((k_shadow<false, TB>), dim3(grid_of(lc.trace_blocks, lc.max_shadow, TB)), dim3(TB), 0, s, sc,
+                           w, slot);
+    });
 }
 void launch_shade_adjoint(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int buf) {
-    hipLaunchKernelGGL(k_shade<true>, dim3(grid_of(lc.shade_blocks, lc.max_paths, kShadeBlock)), dim3(kShadeBlock), 0,
-                       s, sc, w,
-                       buf, lc.max_nsr);
+    // the adjoint shade keeps its one block size (its register budget is tuned for it)
+    hipLaunchKernelGGL((k_shade<true, kShadeBlock>), dim3(grid_of(lc.shade_blocks, lc.max_paths, kShadeBlock)),
+                       dim3(kShadeBlock), 0, s, sc, w, buf, lc.max_nsr);
 }
 void launch_trace(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int buf, bool adjoint) {
-    const uint32_t ne = grid_of(lc.trace_blocks, lc.max_paths, kTraceBlock);
-    const uint32_t ns = grid_of(lc.trace_blocks, lc.max_shadow, kTraceBlock);
-    if (adjoint)
-        hipLaunchKernelGGL(k_trace<true>, dim3(ne + ns), dim3(kTraceBlock), 0, s, sc, w, buf, ne);
-    else
-        hipLaunchKernelGGL(k_trace<false>, dim3(ne + ns), dim3(kTraceBlock), 0, s, sc, w, buf, ne);
+    with_trace_block(lc.trace_block, [&](auto tb) {
+        constexpr int TB = decltype(tb)::value;
+        const uint32_t ne = grid_of(lc.trace_blocks, lc.max_paths, TB);
+        const uint32_t ns = grid_of(lc.trace_blocks, lc.max_shadow, TB);
+        if (adjoint)
+            hipLaunchKernelGGL((k_trace<true, TB>), dim3(ne + ns), dim3(TB), 0, s, sc, w, buf, ne);
+        else
+            hipLaunchKernelGGL((k_trace<false, TB>), dim3(ne + ns), dim3(TB), 0, s, sc, w, buf, ne);
+    });
 }
 void launch_shadow_adjoint(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int slot) {
-    hipLaunchKernelGGL(k_shadow<true>, dim3(grid_of(lc.trace_blocks, lc.max_shadow, kTraceBlock)), dim3(kTraceBlock),
-                       0, s, sc, w, slot);
+    with_trace_block(lc.trace_block, [&](auto tb) {
+        constexpr int TB = decltype(tb)::value;
+        hipLaunchKernelGGL((k_shadow<true, TB>), dim3(grid_of(lc.trace_blocks, lc.max_shadow, TB)), dim3(TB), 0, s, sc,
+                           w, slot);
+    });
 }
 void launch_adjoint_finalize(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc) {
     hipLaunchKernelGGL(k_adjoint_finalize, dim3(grid_of(lc.shade_blocks, lc.max_paths, 256)), dim3(256), 0, s, sc, w);
@@ -1698,11 +1739,17 @@ void launch_accumulate_sum(hipStream_t s, const DevScene& sc, const Wave& w, con
     hipLaunchKernelGGL(k_accumulate_sum, dim3(grid_of(lc.shade_blocks, lc.max_paths, 256)), dim3(256), 0, s, sc, w,
                        pool);
 }
-void resident_blocks_per_cu(int* trace, int* shade) {
+void resident_blocks_per_cu(uint32_t trace_block, uint32_t shade_block, int* trace, int* shade) {
     int e = 0, sh = 0, sd = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&e, k_extend, kTraceBlock, 0) != hipSuccess) e = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&sh, k_shadow<false>, kTraceBlock, 0) != hipSuccess) sh = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&sd, k_shade<false>, kShadeBlock, 0) != hipSuccess) sd = 0;
+    with_trace_block(trace_block, [&](auto tb) {
+        constexpr int TB = decltype(tb)::value;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&e, k_extend<TB>, TB, 0) != hipSuccess) e = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&sh, k_shadow<false, TB>, TB, 0) != hipSuccess) sh = 0;
+    });
+    with_shade_block(shade_block, [&](auto sbk) {
+        constexpr int SB = decltype(sbk)::value;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&sd, k_shade<false, SB>, SB, 0) != hipSuccess) sd = 0;
+    });
     *trace = std::max(e, sh);
     *shade = sd;
 }

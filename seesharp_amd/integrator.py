@@ -50,6 +50,16 @@ class Device:
         self.check(self.lib.sspt_set_launch_config(self.ptr, int(trace_blocks), int(shade_blocks),
                                                    1 if use_graphs else 0))
 
+    def set_block_shape(self, trace_block: int = 0, shade_block: int = 0):
+        """Threads per traversal block (64, 128, 256) and per forward shade block
+        (128, 256, 512); 0 = the defaults."""
+        self.check(self.lib.sspt_set_block_shape(self.ptr, int(trace_block), int(shade_block)))
+
+    def block_shape(self) -> tuple[int, int]:
+        t, s = C.c_uint32(), C.c_uint32()
+        self.check(self.lib.sspt_get_block_shape(self.ptr, C.byref(t), C.byref(s)))
+        return int(t.value), int(s.value)
+
     def set_execution_mode(self, single_kernel: bool):
         """False: the wavefront tracer (default). True: the plain single-kernel tracer."""
         self.check(self.lib.sspt_set_execution_mode(self.ptr, abi.EXEC_SINGLE_KERNEL if single_kernel

@@ -83,3 +83,54 @@ def test_persistent_and_full_grids_render_identical_images(device):
         finally:
             device.set_launch_config(0, 0, True)
 
+
+
+def test_block_shapes_render_identical_images(device):
+    """Every prebuilt block shape (traversal 64 / 128 / 256 threads, forward shade
+    128 / 256 / 512) renders the default shape's image bit for bit: a block only
+    shares LDS stack space and queue appends, never a path's arithmetic."""
+    s = scenes.large_scene(96, 64, num_spheres=30, grid=48)
+    g = GpuScene(device, s)
+    cfg = ol.settings(total_spp=6, max_depth=8)
+    g.clear()
+    g.render(cfg, 0, 6)
+    ref = g.read_image()
+    try:
+        for tb, sb in ((64, 128), (256, 512), (128, 512), (64, 256)):
+            device.set_block_shape(tb, sb)
+            assert device.block_shape() == (tb, sb)
+            g.clear()
+            g.render(cfg, 0, 6)
+            assert np.array_equal(g.read_image(), ref), (tb, sb)
+    finally:
+        device.set_block_shape(0, 0)
+    assert device.block_shape() == (128, 256)
+
+
+def test_autotuner_pick_renders_identical_images(device):
+    """The autotuner (block shape, wave capacity, grids by coordinate descent) leaves
+    its pick set on the device; a render with it equals the default render bit for bit."""
+    from seesharp_amd import autotune
+
+    s = scenes.large_scene(128, 96, num_spheres=40, grid=64)
+    g = GpuScene(device, s)
+    cfg = ol.settings(total_spp=8, max_depth=8)
+    g.clear()
+    g.render(cfg, 0, 8)
+    ref = g.read_image()
+
+    def run():
+        g.clear()
+        g.render(cfg, 0, 8)
+
+    try:
+        best, results = autotune.tune(device, run, capacities=(1 << 16, 1 << 18, 1 << 20))
+        assert len(results) == 9 + 3 + 9
+        assert device.block_shape() == (best["trace_block"], best["shade_block"])
+        g.clear()
+        g.render(cfg, 0, 8)
+        assert np.array_equal(g.read_image(), ref), best
+    finally:
+        device.set_block_shape(0, 0)
+        device.set_wave_capacity(0)
+        device.set_launch_config(0, 0, True)
