@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define SSPT_ABI_VERSION 4
+#define SSPT_ABI_VERSION 5
 
 typedef enum sspt_status {
     SSPT_OK = 0,
@@ -195,6 +195,28 @@ void sspt_scene_destroy(sspt_scene* scene);
 sspt_status sspt_scene_info(const sspt_scene* scene, uint64_t* num_triangles,
                             uint64_t* num_emitters, uint64_t* num_bvh_nodes,
                             uint64_t* device_bytes);
+
+/* BVH builder of the scenes a context creates (the CommitScene step,
+ * SeeSharp/Scene.cs:108-113): binned SAH on the device (default) or on the
+ * host. Both make the same tree: the same nodes, boxes and leaf order (a box
+ * bound may be -0 where the other has +0). */
+typedef enum sspt_bvh_builder {
+    SSPT_BVH_DEVICE = 0,
+    SSPT_BVH_HOST = 1
+} sspt_bvh_builder;
+sspt_status sspt_set_bvh_builder(sspt_ctx* ctx, int32_t builder);
+sspt_status sspt_get_bvh_builder(const sspt_ctx* ctx, int32_t* builder);
+/* The builder a scene was built with, the summed duration of its build
+ * kernels (0 for the host builder) and the wall time of its BVH builds. */
+sspt_status sspt_scene_bvh_build_time(const sspt_scene* scene, int32_t* builder,
+                                      double* kernel_ms, double* wall_ms);
+/* Copies the scene's 4-wide BVH as traversed: max_nodes >= num_bvh_nodes
+ * nodes of 32 floats (per axis the 4 child box minima then maxima, then 4
+ * child indices and 4 counts as int32, the top byte of the first count
+ * holding the node's leaf and inner slot masks), and max_tris >= num_triangles
+ * global triangle ids, one per leaf position. */
+sspt_status sspt_scene_read_bvh(sspt_ctx* ctx, sspt_scene* scene, float* nodes, uint64_t max_nodes,
+                                int32_t* leaf_triangles, uint64_t max_tris);
 
 /* ---- batching: many scenes (cameras, resolutions, triangle counts) in one
  *      device world, rendered in shared waves. No counterpart in the reference,

@@ -13,7 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 # different occupancy targets); the default is the in-tree build.
 LIB_PATH = os.environ.get("SSPT_LIBRARY") or os.path.join(_HERE, "libsspt.so")
 
-ABI_VERSION = 4  # SSPT_ABI_VERSION of include/sspt.h
+ABI_VERSION = 5  # SSPT_ABI_VERSION of include/sspt.h
 NOALLOC_SCRATCH_BYTES = 64 << 20  # SSPT_NOALLOC_SCRATCH_BYTES
 SSPT_OK = 0
 INVALID_ARGUMENT = 1
@@ -33,6 +33,8 @@ LAYER_ALBEDO = 0
 LAYER_NORMAL = 1
 EXEC_WAVEFRONT = 0
 EXEC_SINGLE_KERNEL = 1
+BVH_DEVICE = 0
+BVH_HOST = 1
 MATERIAL_DIFFUSE = 0
 MATERIAL_GENERIC = 1
 
@@ -196,6 +198,11 @@ SYMBOLS = {
     "sspt_set_launch_config": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_int32]),
     "sspt_set_block_shape": (C.c_int, [_vp, C.c_uint32, C.c_uint32]),
     "sspt_get_block_shape": (C.c_int, [_vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "sspt_set_bvh_builder": (C.c_int, [_vp, C.c_int32]),
+    "sspt_get_bvh_builder": (C.c_int, [_vp, C.POINTER(C.c_int32)]),
+    "sspt_scene_bvh_build_time": (C.c_int, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_double),
+                                            C.POINTER(C.c_double)]),
+    "sspt_scene_read_bvh": (C.c_int, [_vp, _vp, _fp, C.c_uint64, _ip, C.c_uint64]),
     "sspt_set_kernel_timing": (C.c_int, [_vp, C.c_int32]),
     "sspt_get_kernel_timing": (C.c_int, [_vp, C.c_char_p, C.POINTER(C.c_uint64),
                                          C.POINTER(C.c_double)]),

@@ -12,7 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libsspt.so")
-SOURCES = ["api.hip", "kernels.hip", "bvh_build.cpp"]
+SOURCES = ["api.hip", "kernels.hip", "bvh_build.cpp", "bvh_gpu.hip"]
 HEADERS = ["device_math.h", "scene_types.h", "shading.h", "adjoint.h", "kernels.h", "bvh_build.h"]
 
 HIPCC_FLAGS = [

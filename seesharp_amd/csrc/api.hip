@@ -90,6 +90,7 @@ struct sspt_ctx {
     uint32_t trace_block = 0, shade_block = 0;  // threads per block (0 = the default sizes)
     bool use_graphs = true;
     int32_t exec_mode = SSPT_EXEC_WAVEFRONT;
+    int32_t bvh_builder = SSPT_BVH_DEVICE;
     // adjoint buffers, grown on demand
     std::vector<void*> adj_allocs;
     uint64_t adj_records = 0;   // capacity * depth
@@ -111,6 +112,8 @@ struct sspt_scene {
     uint64_t num_tris = 0, num_emitters = 0, num_nodes = 0, bytes = 0;
     uint64_t nan_count = 0;
     int stack_need = 0;
+    int32_t bvh_builder = SSPT_BVH_DEVICE;
+    double bvh_kernel_ms = 0.0, bvh_wall_ms = 0.0;
     // adjoint: d loss / d texel (texel-pool layout) and d loss / d image (framebuffer layout)
     std::vector<DevTexture> textures;  // host copy of the texture table
     uint64_t texel_floats = 0;
@@ -274,6 +277,7 @@ struct WorldBuilder {
     std::vector<SceneRec> recs;
     uint64_t fb_floats = 0;
     int stack_need = 0;
+    double bvh_kernel_ms = 0.0, bvh_wall_ms = 0.0;
 };
 
 // Scene.IsValid (SeeSharp/Scene.cs:170-197) and structural checks of a description.
@@ -384,8 +388,21 @@ sspt_status add_scene(sspt_ctx* ctx, WorldBuilder& W, const sspt_scene_desc* d) 
     }
 
     BvhOutput bvh;
-    unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-    build_bvh(BvhInput{verts.data(), (int)ntri}, kMaxLeafTriangles, (int)std::min(hw, 16u), &bvh);
+    const auto build_start = std::chrono::steady_clock::now();
+    if (ctx->bvh_builder == SSPT_BVH_DEVICE) {
+        std::string err;
+        double kernel_ms = 0.0;
+        if (!build_bvh_device(ctx->stream, BvhInput{verts.data(), (int)ntri}, kMaxLeafTriangles, &bvh, &kernel_ms,
+                              &err))
+            return fail(ctx, err.find("out of memory") != std::string::npos ? SSPT_ERR_OUT_OF_MEMORY : SSPT_ERR_DEVICE,
+                        "device BVH build: %s", err.c_str());
+        W.bvh_kernel_ms += kernel_ms;
+    } else {
+        unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+        build_bvh(BvhInput{verts.data(), (int)ntri}, kMaxLeafTriangles, (int)std::min(hw, 16u), &bvh);
+    }
+    W.bvh_wall_ms +=
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - build_start).count();
     if (bvh.stack_need > kStackTotal)
         return fail(ctx, SSPT_ERR_UNSUPPORTED, "BVH needs a traversal stack of %d entries (limit %d)", bvh.stack_need,
                     kStackTotal);
@@ -638,6 +655,9 @@ sspt_status create_world(sspt_ctx* ctx, const sspt_scene_desc* descs, uint32_t n
     sc->num_emitters = W.emitters.size();
     sc->num_nodes = W.nodes.size() / 32;
     sc->stack_need = W.stack_need;
+    sc->bvh_builder = ctx->bvh_builder;
+    sc->bvh_kernel_ms = W.bvh_kernel_ms;
+    sc->bvh_wall_ms = W.bvh_wall_ms;
     *out = sc.release();
     return SSPT_OK;
 }
@@ -1115,6 +1135,47 @@ sspt_status sspt_scene_info(const sspt_scene* sc, uint64_t* ntri, uint64_t* nem,
     if (nem) *nem = sc->num_emitters;
     if (nnodes) *nnodes = sc->num_nodes;
     if (bytes) *bytes = sc->bytes;
+    return SSPT_OK;
+}
+
+sspt_status sspt_set_bvh_builder(sspt_ctx* ctx, int32_t builder) {
+    if (!ctx) return SSPT_ERR_INVALID_ARGUMENT;
+    if (builder != SSPT_BVH_DEVICE && builder != SSPT_BVH_HOST)
+        return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "unknown BVH builder %d", builder);
+    ctx->bvh_builder = builder;
+    return SSPT_OK;
+}
+
+sspt_status sspt_get_bvh_builder(const sspt_ctx* ctx, int32_t* builder) {
+    if (!ctx || !builder) return SSPT_ERR_INVALID_ARGUMENT;
+    *builder = ctx->bvh_builder;
+    return SSPT_OK;
+}
+
+sspt_status sspt_scene_bvh_build_time(const sspt_scene* sc, int32_t* builder, double* kernel_ms, double* wall_ms) {
+    if (!sc) return SSPT_ERR_INVALID_ARGUMENT;
+    if (builder) *builder = sc->bvh_builder;
+    if (kernel_ms) *kernel_ms = sc->bvh_kernel_ms;
+    if (wall_ms) *wall_ms = sc->bvh_wall_ms;
+    return SSPT_OK;
+}
+
+sspt_status sspt_scene_read_bvh(sspt_ctx* ctx, sspt_scene* sc, float* nodes, uint64_t max_nodes,
+                                int32_t* leaf_tris, uint64_t max_tris) {
+    if (!ctx || !sc || !nodes || !leaf_tris) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "null argument");
+    if (max_nodes < sc->num_nodes || max_tris < sc->num_tris)
+        return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "buffers hold %llu nodes and %llu triangles, the scene has %llu and %llu",
+                    (unsigned long long)max_nodes, (unsigned long long)max_tris, (unsigned long long)sc->num_nodes,
+                    (unsigned long long)sc->num_tris);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    std::vector<float4> pos(sc->num_tris * 3);
+    HIP_TRY(ctx, hipMemcpyAsync(nodes, sc->dev.nodes, sc->num_nodes * 32 * sizeof(float), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(pos.data(), sc->dev.tri_pos, pos.size() * sizeof(float4), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    // the first vertex record of each leaf triangle carries its global id in w
+    for (uint64_t i = 0; i < sc->num_tris; ++i) std::memcpy(&leaf_tris[i], &pos[i * 3].w, 4);
     return SSPT_OK;
 }
 

@@ -15,12 +15,6 @@
 namespace sspt {
 namespace {
 
-// SAH bins per axis and split.
-#ifndef SSPT_SAH_BINS
-#define SSPT_SAH_BINS 64
-#endif
-constexpr int kSahBins = SSPT_SAH_BINS;
-
 struct BNode {
     float lo[3], hi[3];
     int left, right;  // inner: child BNode indices
@@ -125,7 +119,9 @@ struct Builder {
                 int k = best_axis;
                 float scale = (float)B / (ch[k] - cl[k]);
                 float c0 = cl[k];
-                auto it = std::partition(order.begin() + begin, order.begin() + end, [&](int t) {
+                // stable: every node's range keeps ascending triangle ids, so the object
+                // median of coincident centroids takes the lower half of the ids
+                auto it = std::stable_partition(order.begin() + begin, order.begin() + end, [&](int t) {
                     return std::min(B - 1, (int)((cen[t * 3 + k] - c0) * scale)) <= best_split;
                 });
                 mid = (int)(it - order.begin());

@@ -3,6 +3,7 @@
 
 #include <stdint.h>
 
+#include <string>
 #include <vector>
 
 namespace sspt {
@@ -22,6 +23,12 @@ struct BvhOutput {
     float scene_lo[3], scene_hi[3];
 };
 
+// SAH bins per axis and split (both builders).
+#ifndef SSPT_SAH_BINS
+#define SSPT_SAH_BINS 64
+#endif
+constexpr int kSahBins = SSPT_SAH_BINS;
+
 // Triangles per leaf at most (the builder splits every larger range).
 #ifndef SSPT_MAX_LEAF
 #define SSPT_MAX_LEAF 1
@@ -32,5 +39,15 @@ constexpr int kMaxLeafTriangles = SSPT_MAX_LEAF;
 // `max_leaf` triangles, depth bounded by kMaxBvhDepth. Uses up to `threads`
 // host threads.
 void build_bvh(const BvhInput& in, int max_leaf, int threads, BvhOutput* out);
+
+// The same build on the device (bvh_gpu.hip): the same bins, split costs (fp32,
+// same operation order), first-minimum split choice and stable partitions
+// (every node's range keeps ascending triangle ids, which fixes the object-median
+// split of coincident centroids), hence the same tree, leaf order included; only
+// a bound may be -0 where the host has +0. Runs on `stream`; returns false and
+// fills `error` on a HIP failure. `device_ms` receives the summed duration of
+// the build kernels.
+bool build_bvh_device(void* stream, const BvhInput& in, int max_leaf, BvhOutput* out, double* device_ms,
+                      std::string* error);
 
 }  // namespace sspt

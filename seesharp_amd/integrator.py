@@ -60,6 +60,15 @@ class Device:
         self.check(self.lib.sspt_get_block_shape(self.ptr, C.byref(t), C.byref(s)))
         return int(t.value), int(s.value)
 
+    def set_bvh_builder(self, host: bool):
+        """BVH builder of the scenes created from now on: the device (default) or the host."""
+        self.check(self.lib.sspt_set_bvh_builder(self.ptr, abi.BVH_HOST if host else abi.BVH_DEVICE))
+
+    def bvh_builder(self) -> str:
+        b = C.c_int32()
+        self.check(self.lib.sspt_get_bvh_builder(self.ptr, C.byref(b)))
+        return "host" if b.value == abi.BVH_HOST else "device"
+
     def set_execution_mode(self, single_kernel: bool):
         """False: the wavefront tracer (default). True: the plain single-kernel tracer."""
         self.check(self.lib.sspt_set_execution_mode(self.ptr, abi.EXEC_SINGLE_KERNEL if single_kernel
@@ -155,6 +164,24 @@ class GpuScene(_TextureParameters):
         v = [C.c_uint64() for _ in range(4)]
         self.device.check(self.device.lib.sspt_scene_info(self.ptr, *[C.byref(x) for x in v]))
         return dict(zip(("num_triangles", "num_emitters", "num_bvh_nodes", "device_bytes"), (x.value for x in v)))
+
+    def bvh_build_time(self) -> dict:
+        """Builder, summed build-kernel milliseconds (device builder) and wall milliseconds of the BVH build."""
+        b, k, w = C.c_int32(), C.c_double(), C.c_double()
+        self.device.check(self.device.lib.sspt_scene_bvh_build_time(self.ptr, C.byref(b), C.byref(k), C.byref(w)))
+        return {"builder": "host" if b.value == abi.BVH_HOST else "device", "kernel_ms": k.value,
+                "wall_ms": w.value}
+
+    def read_bvh(self):
+        """The 4-wide BVH as traversed: nodes (N, 32) float32 (child box planes, then child
+        indices and counts as int32 bits) and the global triangle id of each leaf position."""
+        info = self.info()
+        nodes = np.zeros((info["num_bvh_nodes"], 32), np.float32)
+        tris = np.zeros(info["num_triangles"], np.int32)
+        self.device.check(self.device.lib.sspt_scene_read_bvh(
+            self.device.ptr, self.ptr, abi.fptr(nodes), nodes.shape[0],
+            tris.ctypes.data_as(C.POINTER(C.c_int32)), tris.size))
+        return nodes, tris
 
     def clear(self):
         self.device.check(self.device.lib.sspt_framebuffer_clear(self.device.ptr, self.ptr))
