@@ -17,6 +17,7 @@ reference, timed on a bounded pixel sample).
     python bench.py --workload views     # 32 views of the 1M-tri scene with 1024^2 ground textures: forward + adjoint
     python bench.py --workload copy      # plain 1 GiB device copy: HBM counter calibration for rocprofv3 --pmc
     python bench.py --workload cornell   # the reference's CornellBox.json 512^2 512 spp render vs its recorded times
+    python bench.py --workload bvh       # device BVH build of the 1M-tri scene vs the host builder
 """
 from __future__ import annotations
 
@@ -405,6 +406,61 @@ CORNELL_SCENE = os.path.join(ROOT, "tests", "golden", "scenes", "CornellBox", "C
 CORNELL_REFERENCE_MS = {5: 142532.0, 2: 26768.0}
 
 
+def bvh_main(args):
+    """The scene build's BVH step (CommitScene, SeeSharp/Scene.cs:108-113) on
+    the 1 036 296-triangle large scene: the device builder timed `steps` times
+    after `warmup` builds (summed kernel time from HIP events, and wall time of
+    the whole build incl. the triangle upload and node download), the host
+    builder once, and whether the two trees are identical."""
+    from seesharp_amd import Device, GpuScene, scenes
+
+    dev = Device(0)
+    scene = scenes.large_scene(args.res, args.res)
+    for i in range(max(0, args.warmup)):
+        GpuScene(dev, scene).close()
+    kernel, wall = [], []
+    g = None
+    for i in range(max(1, args.steps)):
+        if g is not None:
+            g.close()
+        g = GpuScene(dev, scene)
+        t = g.bvh_build_time()
+        kernel.append(t["kernel_ms"])
+        wall.append(t["wall_ms"])
+        log(f"[bench] device BVH build {i + 1}: kernels {t['kernel_ms']:.2f} ms, wall {t['wall_ms']:.1f} ms")
+    dev.set_bvh_builder(host=True)
+    h = GpuScene(dev, scene)
+    dev.set_bvh_builder(host=False)
+    host_wall = h.bvh_build_time()["wall_ms"]
+    nd, td = g.read_bvh()
+    nh, th = h.read_bvh()
+    same = bool(nd.shape == nh.shape and (nd[:, :24] == nh[:, :24]).all()
+                and np.array_equal(nd[:, 24:].view(np.int32), nh[:, 24:].view(np.int32))
+                and np.array_equal(td, th))
+    info = g.info()
+    out = {
+        "metric": "BVH build time, 1M-tri large scene (device builder kernels)",
+        "value": round(float(np.mean(kernel)), 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": max(1, args.steps),
+        "warmup": args.warmup,
+        "ms_per_step": round(float(np.mean(kernel)), 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"large scene ({info['num_triangles']} triangles), binned SAH 64 bins, "
+                               "leaves of 1 triangle, 4-wide collapse"},
+        "bvh_nodes": info["num_bvh_nodes"],
+        "device_wall_ms": round(float(np.mean(wall)), 2),
+        "host_builder_wall_ms": round(host_wall, 2),
+        "trees_identical": same,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def cornell_main(args):
     """The reference's recorded render: CornellBox.json (36 triangles) at 512^2,
     512 spp, BaseSeed 571298512, MaxDepth 5 and 2, denoiser feature layers on
@@ -469,7 +525,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the adjoint (textured box, 32 views) and batching measurements")
-    ap.add_argument("--workload", choices=("large", "batch", "adjoint", "views", "copy", "cornell"), default="large")
+    ap.add_argument("--workload", choices=("large", "batch", "adjoint", "views", "copy", "cornell", "bvh"),
+                    default="large")
     ap.add_argument("--no-autotune", action="store_true",
                     help="batching: keep the default block shape, wave capacity and grids instead of tuning them")
     args = ap.parse_args()
@@ -492,6 +549,9 @@ def main():
         return
     if args.workload == "cornell":
         cornell_main(args)
+        return
+    if args.workload == "bvh":
+        bvh_main(args)
         return
 
     from seesharp_amd import Device, GpuScene, scenes
@@ -619,6 +679,9 @@ def main():
         },
         "image_mean": float(img.mean()),
         "nan_paths": gs.nan_count(),
+        # scene preparation, outside the timed region: the BVH build (device builder)
+        "scene_build": {"bvh_" + k: (round(v, 3) if isinstance(v, float) else v)
+                        for k, v in gs.bvh_build_time().items()},
     }
     # the plain single-kernel tracer (one thread per path) on the same render
     dev.set_execution_mode(True)

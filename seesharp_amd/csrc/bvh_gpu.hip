@@ -38,8 +38,8 @@ namespace {
 constexpr int kBins = kSahBins;
 constexpr int kChunk = 2048;      // positions per block in the large-node levels
 constexpr int kBlock = 256;       // threads per large-node block
-constexpr int kSmallMax = 256;    // nodes of at most this many triangles go to phase 2
-constexpr int kSmallStack = 64;   // depth-first stack of one phase-2 wave (kMaxBvhDepth + 1 suffices)
+constexpr int kSmallMax = 128;    // nodes of at most this many triangles go to phase 2
+constexpr int kSmallStack = 48;   // depth-first stack of one phase-2 wave (kMaxBvhDepth + 1 suffices)
 static_assert(kBins == 64, "the phase-2 split sweep maps one bin to one lane of a wave");
 static_assert(kSmallStack > kMaxBvhDepth, "phase-2 stack");
 
@@ -214,41 +214,77 @@ __global__ void __launch_bounds__(kBlock) k_large_bins(const Chunk* chunks, cons
     }
 }
 
-// The host's sweep over one axis's bins (bvh_build.cpp, Builder::build).
-__device__ void sweep_axis(const int* cnt, const float (*bl)[3], const float (*bh)[3], int k, float* best_cost,
-                           int* best_axis, int* best_split) {
-    float ra[kBins];
-    int rc[kBins];
-    float rl[3] = {INFINITY, INFINITY, INFINITY}, rh[3] = {-INFINITY, -INFINITY, -INFINITY};
-    int rcount = 0;
-    for (int j = kBins - 1; j > 0; --j) {
-        rcount += cnt[j];
-        for (int q = 0; q < 3; ++q) rl[q] = smin(rl[q], bl[j][q]), rh[q] = smax(rh[q], bh[j][q]);
-        ra[j] = half_area(rl, rh);
-        rc[j] = rcount;
-    }
-    float ll[3] = {INFINITY, INFINITY, INFINITY}, lh[3] = {-INFINITY, -INFINITY, -INFINITY};
-    int lcount = 0;
-    for (int j = 0; j < kBins - 1; ++j) {
-        lcount += cnt[j];
-        for (int q = 0; q < 3; ++q) ll[q] = smin(ll[q], bl[j][q]), lh[q] = smax(lh[q], bh[j][q]);
-        if (lcount == 0 || rc[j + 1] == 0) continue;
-        const float cost = half_area(ll, lh) * (float)lcount + ra[j + 1] * (float)rc[j + 1];
-        if (cost < *best_cost) {
-            *best_cost = cost;
-            *best_axis = k;
-            *best_split = j;
+// One axis of the binned SAH sweep (bvh_build.cpp, Builder::build) by one wave:
+// `bins` holds the axis's kBins bins (count, then lo xyz and hi xyz as ord keys)
+// and lane j takes bin j; the left side of split j is bins 0..j (an inclusive
+// prefix scan), the right side bins j+1.. (the inclusive suffix scan of lane
+// j + 1). Costs round as the host's (the same operations in the same order) and
+// the choice is the host's: the first bin of the least cost, taken over the best
+// split so far only when strictly cheaper. All values wave-uniform.
+struct Split {
+    int axis, bin, lcount;
+};
+__device__ __forceinline__ void wave_sweep_axis(const uint32_t* bins, int k, float* best_cost, Split* best) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t* b = bins + lane * 7;
+    const int c = (int)b[0];
+    int pc = c, sc = c;
+    float pl[3], ph[3], sl[3], sh[3];
+    for (int q = 0; q < 3; ++q) pl[q] = sl[q] = unord(b[1 + q]), ph[q] = sh[q] = unord(b[4 + q]);
+    for (int s = 1; s < 64; s <<= 1) {
+        const int upc = __shfl_up(pc, s), dnc = __shfl_down(sc, s);
+        float upl[3], uph[3], dnl[3], dnh[3];
+        for (int q = 0; q < 3; ++q) {
+            upl[q] = __shfl_up(pl[q], s), uph[q] = __shfl_up(ph[q], s);
+            dnl[q] = __shfl_down(sl[q], s), dnh[q] = __shfl_down(sh[q], s);
         }
+        if (lane >= s) {
+            pc += upc;
+            for (int q = 0; q < 3; ++q) pl[q] = smin(upl[q], pl[q]), ph[q] = smax(uph[q], ph[q]);
+        }
+        if (lane + s < 64) {
+            sc += dnc;
+            for (int q = 0; q < 3; ++q) sl[q] = smin(sl[q], dnl[q]), sh[q] = smax(sh[q], dnh[q]);
+        }
+    }
+    // the right side of split `lane`: the suffix that starts at lane + 1
+    const int rc = __shfl_down(sc, 1);
+    float rl[3], rh[3];
+    for (int q = 0; q < 3; ++q) rl[q] = __shfl_down(sl[q], 1), rh[q] = __shfl_down(sh[q], 1);
+    float cost = INFINITY;
+    if (lane < kBins - 1 && pc != 0 && rc != 0) cost = half_area(pl, ph) * (float)pc + half_area(rl, rh) * (float)rc;
+    // the first minimum over the bins (the smallest bin among equal costs)
+    float mc = cost;
+    int mj = cost < INFINITY ? lane : 64;
+    for (int s = 1; s < 64; s <<= 1) {
+        const float oc = __shfl_xor(mc, s);
+        const int oj = __shfl_xor(mj, s);
+        if (oc < mc || (oc == mc && oj < mj)) mc = oc, mj = oj;
+    }
+    if (mj < 64 && mc < *best_cost) {
+        *best_cost = mc;
+        *best = Split{k, mj, __shfl(pc, mj)};
     }
 }
 
-// One thread per large node: bounds, leaf test, split choice, children.
-__global__ void k_large_split(const Task* tasks, int ntasks, const uint32_t* acc, GNode* nodes, int max_nodes,
-                              TaskOut* outs, int* node_counter, int max_leaf, int* max_depth) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// One wave per large node: leaf test, split choice over the node's bins, children.
+__global__ void __launch_bounds__(64) k_large_split(const Task* tasks, int ntasks, const uint32_t* acc, GNode* nodes,
+                                                    int max_nodes, TaskOut* outs, int* node_counter, int max_leaf,
+                                                    int* max_depth) {
+    const int i = blockIdx.x;
     if (i >= ntasks) return;
     const Task T = tasks[i];
     const uint32_t* A = acc + (size_t)i * kAccWords;
+    const int count = T.end - T.begin;
+    const bool leaf = count <= max_leaf || T.depth >= kMaxBvhDepth - 1;
+    Split best{-1, -1, 0};
+    if (!leaf) {
+        const Axes ax = axes_of(A);
+        float best_cost = INFINITY;
+        for (int k = 0; k < 3; ++k)
+            if (ax.ok[k]) wave_sweep_axis(A + kAccBounds + k * kBins * 7, k, &best_cost, &best);
+    }
+    if (threadIdx.x != 0) return;
     GNode& n = nodes[T.node];
     for (int k = 0; k < 3; ++k) {
         n.lo[k] = unord(A[k]);
@@ -256,45 +292,20 @@ __global__ void k_large_split(const Task* tasks, int ntasks, const uint32_t* acc
     }
     n.depth = T.depth;
     atomicMax(max_depth, T.depth);
-    const int count = T.end - T.begin;
     TaskOut o{-1, -1, T.begin, -1};
-    if (count <= max_leaf || T.depth >= kMaxBvhDepth - 1) {
+    const int l = leaf ? 0 : atomicAdd(node_counter, 2);
+    if (leaf || l + 1 >= max_nodes) {  // the second: never for a valid count (a binary tree has < 2n nodes)
         n.first = T.begin;
         n.count = count;
         n.left = n.right = -1;
         outs[i] = o;
         return;
     }
-    const Axes ax = axes_of(A);
-    float best_cost = INFINITY;
-    int best_axis = -1, best_split = -1;
-    for (int k = 0; k < 3; ++k) {
-        if (!ax.ok[k]) continue;
-        int cnt[kBins];
-        float bl[kBins][3], bh[kBins][3];
-        for (int j = 0; j < kBins; ++j) {
-            const uint32_t* b = A + kAccBounds + (k * kBins + j) * 7;
-            cnt[j] = (int)b[0];
-            for (int q = 0; q < 3; ++q) bl[j][q] = unord(b[1 + q]), bh[j][q] = unord(b[4 + q]);
-        }
-        sweep_axis(cnt, bl, bh, k, &best_cost, &best_axis, &best_split);
-    }
-    int mid = T.begin + count / 2;
+    int mid = T.begin + count / 2;  // all centroids coincide: object median
     o.split = -2;
-    if (best_axis >= 0) {
-        int lc = 0;
-        for (int j = 0; j <= best_split; ++j) lc += (int)A[kAccBounds + (best_axis * kBins + j) * 7];
-        mid = T.begin + lc;
-        o.split = best_axis * 256 + best_split;
-    }
-    const int l = atomicAdd(node_counter, 2);
-    if (l + 1 >= max_nodes) {  // never for a valid count (a binary tree has < 2n nodes)
-        n.first = T.begin;
-        n.count = count;
-        n.left = n.right = -1;
-        o.split = -1;
-        outs[i] = o;
-        return;
+    if (best.axis >= 0) {
+        mid = T.begin + best.lcount;
+        o.split = best.axis * 256 + best.bin;
     }
     n.left = l;
     n.right = l + 1;
@@ -403,180 +414,141 @@ __device__ __forceinline__ float wave_reduce(float v, F&& op) {
     return v;
 }
 
-// One wave per small node: its whole subtree, depth first.
+// One wave per small node: its whole subtree, depth first, out of LDS. The
+// node's triangles (boxes, centroids, ids) are staged once; node ranges are
+// ranges of a permutation of them, partitioned stably; the subtree's node
+// slots (at most 2 (n - 1) below its root) are reserved with one atomic, and
+// the final triangle order is written back at the end.
 __global__ void __launch_bounds__(64) k_small(const Task* tasks, int ntasks, GNode* nodes, int max_nodes, int* order,
-                                              int* order2, const float* tlo, const float* thi, const float* cen,
-                                              int* node_counter, int max_leaf, int* max_depth) {
+                                              const float* tlo, const float* thi, const float* cen, int* node_counter,
+                                              int max_leaf, int* max_depth) {
+    __shared__ float s_box[kSmallMax * 9];  // per staged triangle: lo xyz, hi xyz, centroid xyz
+    __shared__ int s_id[kSmallMax];         // its global triangle id
+    __shared__ int s_perm[kSmallMax];       // position -> staged triangle
+    __shared__ int s_tmp[kSmallMax];
+    __shared__ uint32_t bins[kBins * 7];    // one axis at a time
     __shared__ int4 stack[kSmallStack];
-    __shared__ uint32_t bins[3 * kBins * 7];
     const int lane = threadIdx.x;
     if ((int)blockIdx.x >= ntasks) return;
-    int sp = 0;
-    if (lane == 0) {
-        const Task T = tasks[blockIdx.x];
-        stack[0] = make_int4(T.node, T.begin, T.end, T.depth);
+    const Task T = tasks[blockIdx.x];
+    const int base = T.begin, total = T.end - T.begin;  // total <= kSmallMax
+    for (int i = lane; i < total; i += 64) {
+        const int t = order[base + i];
+        s_id[i] = t;
+        s_perm[i] = i;
+        for (int k = 0; k < 3; ++k) {
+            s_box[i * 9 + k] = tlo[t * 3 + k];
+            s_box[i * 9 + 3 + k] = thi[t * 3 + k];
+            s_box[i * 9 + 6 + k] = cen[t * 3 + k];
+        }
     }
-    sp = 1;
+    int next = 0;
+    if (lane == 0 && total > 1) next = atomicAdd(node_counter, 2 * (total - 1));
+    next = __shfl(next, 0);
+    const int limit = min(next + 2 * (total - 1), max_nodes);
+    int deepest = T.depth;
+    if (lane == 0) stack[0] = make_int4(T.node, 0, total, T.depth);
+    int sp = 1;
     __syncthreads();
     while (sp > 0) {
         const int4 e = stack[--sp];
         __syncthreads();
-        const int node = e.x, begin = e.y, end = e.z, depth = e.w;
-        const int count = end - begin;
-        // bounds
+        const int node = e.x, b = e.y, end = e.z, depth = e.w;  // positions b..end of the staged triangles
+        const int count = end - b;
+        deepest = max(deepest, depth);
         float v[12];
         for (int q = 0; q < 12; ++q) v[q] = ((q / 3) & 1) ? -INFINITY : INFINITY;
-        for (int p = begin + lane; p < end; p += 64) {
-            const int t = order[p];
+        for (int p = b + lane; p < end; p += 64) {
+            const float* t = s_box + s_perm[p] * 9;
             for (int k = 0; k < 3; ++k) {
-                v[k] = smin(v[k], tlo[t * 3 + k]);
-                v[3 + k] = smax(v[3 + k], thi[t * 3 + k]);
-                v[6 + k] = smin(v[6 + k], cen[t * 3 + k]);
-                v[9 + k] = smax(v[9 + k], cen[t * 3 + k]);
+                v[k] = smin(v[k], t[k]);
+                v[3 + k] = smax(v[3 + k], t[3 + k]);
+                v[6 + k] = smin(v[6 + k], t[6 + k]);
+                v[9 + k] = smax(v[9 + k], t[6 + k]);
             }
         }
         for (int q = 0; q < 12; ++q)
-            v[q] = ((q / 3) & 1) ? wave_reduce(v[q], [](float a, float b) { return smax(a, b); })
-                                 : wave_reduce(v[q], [](float a, float b) { return smin(a, b); });
+            v[q] = ((q / 3) & 1) ? wave_reduce(v[q], [](float x, float y) { return smax(x, y); })
+                                 : wave_reduce(v[q], [](float x, float y) { return smin(x, y); });
         if (lane == 0) {
             GNode& n = nodes[node];
             for (int k = 0; k < 3; ++k) n.lo[k] = v[k], n.hi[k] = v[3 + k];
             n.depth = depth;
-            atomicMax(max_depth, depth);
         }
-        if (count <= max_leaf || depth >= kMaxBvhDepth - 1) {
+        // (next + 2 > limit never holds for a valid count: a binary tree of n leaves has 2n - 1 nodes)
+        if (count <= max_leaf || depth >= kMaxBvhDepth - 1 || next + 2 > limit) {
             if (lane == 0) {
-                nodes[node].first = begin;
+                nodes[node].first = base + b;
                 nodes[node].count = count;
                 nodes[node].left = nodes[node].right = -1;
             }
             __syncthreads();
             continue;
         }
+        float best_cost = INFINITY;
+        Split best{-1, -1, 0};
         float cl[3], scale[3];
-        bool ok[3];
         for (int k = 0; k < 3; ++k) {
             cl[k] = v[6 + k];
             const float extent = v[9 + k] - v[6 + k];
-            ok[k] = extent > 0.0f;
-            scale[k] = ok[k] ? (float)kBins / extent : 0.0f;
-        }
-        // bins
-        for (int i = lane; i < 3 * kBins * 7; i += 64) {
-            const int f = i % 7;
-            bins[i] = f == 0 ? 0u : (f <= 3 ? ord(INFINITY) : ord(-INFINITY));
-        }
-        __syncthreads();
-        for (int p = begin + lane; p < end; p += 64) {
-            const int t = order[p];
-            const uint32_t lo[3] = {ord(tlo[t * 3]), ord(tlo[t * 3 + 1]), ord(tlo[t * 3 + 2])};
-            const uint32_t hi[3] = {ord(thi[t * 3]), ord(thi[t * 3 + 1]), ord(thi[t * 3 + 2])};
-            for (int k = 0; k < 3; ++k) {
-                if (!ok[k]) continue;
-                uint32_t* b = bins + (k * kBins + bin_of(cen[t * 3 + k], cl[k], scale[k])) * 7;
-                atomicAdd(&b[0], 1u);
+            if (!(extent > 0.0f)) continue;
+            scale[k] = (float)kBins / extent;
+            for (int i = lane; i < kBins * 7; i += 64) {
+                const int f = i % 7;
+                bins[i] = f == 0 ? 0u : (f <= 3 ? ord(INFINITY) : ord(-INFINITY));
+            }
+            __syncthreads();
+            for (int p = b + lane; p < end; p += 64) {
+                const float* t = s_box + s_perm[p] * 9;
+                uint32_t* bin = bins + bin_of(t[6 + k], cl[k], scale[k]) * 7;
+                atomicAdd(&bin[0], 1u);
                 for (int q = 0; q < 3; ++q) {
-                    atomicMin(&b[1 + q], lo[q]);
-                    atomicMax(&b[4 + q], hi[q]);
+                    atomicMin(&bin[1 + q], ord(t[q]));
+                    atomicMax(&bin[4 + q], ord(t[3 + q]));
                 }
             }
+            __syncthreads();
+            wave_sweep_axis(bins, k, &best_cost, &best);
+            __syncthreads();
         }
-        __syncthreads();
-        // split sweep: lane j holds bin j; the left side of split j is bins 0..j
-        // (inclusive prefix), the right side bins j+1.. (inclusive suffix of lane j+1)
-        float best_cost = INFINITY;
-        int best_axis = -1, best_split = -1, best_lcount = 0;
-        for (int k = 0; k < 3; ++k) {
-            if (!ok[k]) continue;
-            const uint32_t* b = bins + (k * kBins + lane) * 7;
-            int c = (int)b[0];
-            float l3[3] = {unord(b[1]), unord(b[2]), unord(b[3])}, h3[3] = {unord(b[4]), unord(b[5]), unord(b[6])};
-            int pc = c, sc = c;
-            float pl[3], ph[3], sl[3], sh[3];
-            for (int q = 0; q < 3; ++q) pl[q] = sl[q] = l3[q], ph[q] = sh[q] = h3[q];
-            for (int s = 1; s < 64; s <<= 1) {
-                const int upc = __shfl_up(pc, s), dnc = __shfl_down(sc, s);
-                float upl[3], uph[3], dnl[3], dnh[3];
-                for (int q = 0; q < 3; ++q) {
-                    upl[q] = __shfl_up(pl[q], s), uph[q] = __shfl_up(ph[q], s);
-                    dnl[q] = __shfl_down(sl[q], s), dnh[q] = __shfl_down(sh[q], s);
-                }
-                if (lane >= s) {
-                    pc += upc;
-                    for (int q = 0; q < 3; ++q) pl[q] = smin(upl[q], pl[q]), ph[q] = smax(uph[q], ph[q]);
-                }
-                if (lane + s < 64) {
-                    sc += dnc;
-                    for (int q = 0; q < 3; ++q) sl[q] = smin(sl[q], dnl[q]), sh[q] = smax(sh[q], dnh[q]);
-                }
-            }
-            // the right side of split `lane`: the suffix that starts at lane + 1
-            const int rc = __shfl_down(sc, 1);
-            float rl[3], rh[3];
-            for (int q = 0; q < 3; ++q) rl[q] = __shfl_down(sl[q], 1), rh[q] = __shfl_down(sh[q], 1);
-            float cost = INFINITY;
-            if (lane < kBins - 1 && pc != 0 && rc != 0)
-                cost = half_area(pl, ph) * (float)pc + half_area(rl, rh) * (float)rc;
-            // the first minimum over the bins (smallest bin among equal costs)
-            float mc = cost;
-            int mj = cost < INFINITY ? lane : 64;
-            for (int s = 1; s < 64; s <<= 1) {
-                const float oc = __shfl_xor(mc, s);
-                const int oj = __shfl_xor(mj, s);
-                if (oc < mc || (oc == mc && oj < mj)) mc = oc, mj = oj;
-            }
-            if (mj < 64 && mc < best_cost) {
-                best_cost = mc;
-                best_axis = k;
-                best_split = mj;
-                best_lcount = __shfl(pc, mj);
-            }
-        }
-        int mid = begin + count / 2;
-        if (best_axis >= 0) {
-            mid = begin + best_lcount;
-            // partition into order2, then back
-            int wl = begin, wr = mid;
-            for (int p0 = begin; p0 < end; p0 += 64) {
+        int mid = b + count / 2;  // all centroids coincide: object median
+        if (best.axis >= 0) {
+            const int k = best.axis;
+            mid = b + best.lcount;
+            // stable partition of the positions into s_tmp, then back
+            int wl = b, wr = mid;
+            for (int p0 = b; p0 < end; p0 += 64) {
                 const int p = p0 + lane;
-                const int t = p < end ? order[p] : -1;
-                const bool lf = t >= 0 && bin_of(cen[t * 3 + best_axis], cl[best_axis], scale[best_axis]) <= best_split;
-                const uint64_t ml = __ballot(t >= 0 && lf), mr = __ballot(t >= 0 && !lf);
+                const int t = p < end ? s_perm[p] : -1;
+                const bool lf = t >= 0 && bin_of(s_box[t * 9 + 6 + k], cl[k], scale[k]) <= best.bin;
+                const uint64_t ml = __ballot(lf), mr = __ballot(t >= 0 && !lf);
                 const uint64_t below = (1ull << lane) - 1ull;
                 if (t >= 0) {
-                    if (lf) order2[wl + __popcll(ml & below)] = t;
-                    else order2[wr + __popcll(mr & below)] = t;
+                    if (lf) s_tmp[wl + __popcll(ml & below)] = t;
+                    else s_tmp[wr + __popcll(mr & below)] = t;
                 }
                 wl += __popcll(ml);
                 wr += __popcll(mr);
             }
             __syncthreads();
-            for (int p = begin + lane; p < end; p += 64) order[p] = order2[p];
+            for (int p = b + lane; p < end; p += 64) s_perm[p] = s_tmp[p];
         }
-        int l = 0;
-        if (lane == 0) l = atomicAdd(node_counter, 2);
-        l = __shfl(l, 0);
-        if (l + 1 >= max_nodes) {  // never for a valid count
-            if (lane == 0) {
-                nodes[node].first = begin;
-                nodes[node].count = count;
-                nodes[node].left = nodes[node].right = -1;
-            }
-            __syncthreads();
-            continue;
-        }
+        const int l = next;
+        next += 2;
         if (lane == 0) {
             nodes[node].left = l;
             nodes[node].right = l + 1;
             nodes[node].first = -1;
             nodes[node].count = 0;
             stack[sp] = make_int4(l + 1, mid, end, depth + 1);
-            stack[sp + 1] = make_int4(l, begin, mid, depth + 1);
+            stack[sp + 1] = make_int4(l, b, mid, depth + 1);
         }
         // children are pushed only below depth kMaxBvhDepth - 1, which bounds sp by kMaxBvhDepth
         sp += 2;
         __syncthreads();
     }
+    for (int i = lane; i < total; i += 64) order[base + i] = s_id[s_perm[i]];
+    if (lane == 0) atomicMax(max_depth, deepest);
 }
 
 // ------------------------------------------------------------ phase 3 -------
@@ -871,7 +843,7 @@ bool build_bvh_device(void* stream, const BvhInput& in, int max_leaf, BvhOutput*
             hipLaunchKernelGGL(k_large_bins, dim3(nc), dim3(kBlock), 0, D.st, d_chunks, order, tlo, thi, cen, d_acc);
         });
         D.timed([&] {
-            hipLaunchKernelGGL(k_large_split, dim3(grid_for(nt, 64)), dim3(64), 0, D.st, d_tasks, nt, d_acc, nodes,
+            hipLaunchKernelGGL(k_large_split, dim3(nt), dim3(64), 0, D.st, d_tasks, nt, d_acc, nodes,
                                max_nodes, d_outs, counters, max_leaf, counters + 1);
         });
         D.timed([&] {
@@ -909,7 +881,7 @@ bool build_bvh_device(void* stream, const BvhInput& in, int max_leaf, BvhOutput*
                 "hipMemcpyAsync");
         D.timed([&] {
             hipLaunchKernelGGL(k_small, dim3((unsigned)small.size()), dim3(64), 0, D.st, d_small, (int)small.size(),
-                               nodes, max_nodes, order, order2, tlo, thi, cen, counters, max_leaf, counters + 1);
+                               nodes, max_nodes, order, tlo, thi, cen, counters, max_leaf, counters + 1);
         });
     }
     int host_counters[4] = {0, 0, 0, 0};
