@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define SSPT_ABI_VERSION 5
+#define SSPT_ABI_VERSION 6
 
 typedef enum sspt_status {
     SSPT_OK = 0,
@@ -432,6 +432,35 @@ sspt_status sspt_replay_pixel(sspt_ctx* ctx, sspt_scene* scene, uint32_t index, 
  * image). */
 sspt_status sspt_render_debug(sspt_ctx* ctx, sspt_scene* scene, uint32_t index, uint32_t base_seed,
                               uint32_t first_iteration, uint32_t num_iterations);
+
+/* ---- light tracing: BidirBase.Render (SeeSharp/Integrators/Bidir/BidirBase.cs:84-146)
+ *      of a VertexCacheBidir (SeeSharp/Integrators/Bidir/VertexCacheBidir.cs) set
+ *      up as a pure light tracer: NumConnections = 0, NumShadowRays = 0,
+ *      EnableHitting = false, EnableLightTracer = true. ---------------------
+ * Each iteration traces num_light_paths paths from the emitters, seeded
+ * RNG(base_seed_light, path, iteration) (TraceLightPaths / TraceEmitterPath,
+ * BidirBase.Light.cs:192-277: SelectLight, the emitter's SampleRay
+ * (DiffuseEmitter.cs:44-64, GlossyEmitter.cs:47-70), RandomWalk.StartFromEmitter
+ * and ContinueWalk, RandomWalk.cs:82-90,150-217), and connects every vertex
+ * of every path to the camera (ConnectLightVertexToCamera,
+ * BidirBase.Light.cs:35-92, PerspectiveCamera.SampleResponse). With the other
+ * techniques off the light tracer's MIS weight is 1 and the camera paths add
+ * nothing, so they are not traced. The splats go into the framebuffer of the
+ * (single) scene with running-mean semantics: each iteration n first scales
+ * the image by (n - 1) / n, then adds every finite splat / n in an
+ * unspecified order (fp32 atomics); non-finite splats are counted
+ * (sspt_framebuffer_nan_count). stats count the light paths' closest-hit
+ * traces and their camera connections' occlusion tests. Scenes with a
+ * background or without emitters, and batches, are SSPT_ERR_UNSUPPORTED. */
+typedef struct sspt_lt_settings {
+    uint32_t base_seed_light;  /* BaseSeedLight, default 0x13C0FEFE */
+    int32_t num_light_paths;   /* NumLightPaths, < 0 (default) = one per pixel */
+    int32_t max_depth;         /* MaxDepth (Integrator default 100): at most max_depth - 1 light vertices */
+    int32_t min_depth;         /* MinDepth (default 1): shorter full paths are not splatted */
+} sspt_lt_settings;
+sspt_status sspt_render_light_tracer(sspt_ctx* ctx, sspt_scene* scene, const sspt_lt_settings* settings,
+                                     uint32_t first_iteration, uint32_t num_iterations,
+                                     sspt_ray_stats* stats);
 
 /* ---- measurement ------------------------------------------------------- */
 /* Enables HIP-event timing of every kernel launch on the context stream. */

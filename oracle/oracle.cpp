@@ -153,6 +153,14 @@ inline DirSample to_cos_hemisphere(V2 p) {
     V3 d = spherical_to_cartesian(std::sqrt(1.0f - p.y), std::sqrt(p.y), 2.0f * kPi * p.x);
     return {d, d.z / kPi};
 }
+// SampleWarp.ToCosineLobe (SampleWarp.cs:171-182)
+inline DirSample to_cosine_lobe(float power, V2 p) {
+    float phi = kPi * 2.0f * p.x;
+    float cos_t = std::pow(p.y, 1.0f / (power + 1.0f));
+    float sin_t = std::sqrt(1.0f - cos_t * cos_t);
+    V3 d = spherical_to_cartesian(sin_t, cos_t, phi);
+    return {d, (power + 1.0f) * std::pow(cos_t, power) * 1.0f / (2.0f * kPi)};
+}
 
 // ------------------------------------------------------- piecewise const ---
 // PiecewiseConstantPDF (SeeSharp/Sampling/PiecewiseConstant.cs:12-70)
@@ -301,9 +309,10 @@ M4 perspective_fov(float fov, float aspect, float n, float f) {
 }
 
 struct Camera {
-    M4 world_to_camera, camera_to_world, view_to_camera;
+    M4 world_to_camera, camera_to_world, view_to_camera, camera_to_view;
     int width = 0, height = 0;
     V3 position;
+    float image_plane_distance = 0.0f;
     // Camera.WorldToCamera setter (Camera.cs:34-42) + PerspectiveCamera ctor and
     // UpdateResolution (PerspectiveCamera.cs:29-52).
     bool init(const float* w2c, float fov_deg, int w, int h) {
@@ -313,9 +322,39 @@ struct Camera {
         width = w;
         height = h;
         float aspect = (float)width / (float)height;
-        M4 camera_to_view = perspective_fov(fov_rad, aspect, 0.001f, 1000.0f);
+        camera_to_view = perspective_fov(fov_rad, aspect, 0.001f, 1000.0f);
         if (!invert(camera_to_view, &view_to_camera)) return false;
+        float tan_half = std::tan(fov_rad * 0.5f);
+        image_plane_distance = (float)height / (2.0f * tan_half);
         position = transform_point(v3(0, 0, 0), camera_to_world);
+        return true;
+    }
+    // PerspectiveCamera.SampleResponse (PerspectiveCamera.cs:111-146) without a
+    // lens: the pixel that sees `pos` (WorldToFilm, :148-166) and the weight
+    // SolidAngleToPixelJacobian(pos) (:177-191) * SurfaceAreaToSolidAngleJacobian
+    // (Camera.cs:91-98) of the surface normal. False if the point is behind the
+    // camera or outside the frustum.
+    bool sample_response(V3 pos, V3 normal, int* px, int* py, float* weight) const {
+        V3 local = transform_point(pos, world_to_camera);
+        if (local.z > 0.0f) return false;
+        // Vector4.Transform(local, cameraToView) with w = 1
+        const M4& P = camera_to_view;
+        float vx = local.x * P.at(0, 0) + local.y * P.at(1, 0) + local.z * P.at(2, 0) + P.at(3, 0);
+        float vy = local.x * P.at(0, 1) + local.y * P.at(1, 1) + local.z * P.at(2, 1) + P.at(3, 1);
+        float vw = local.x * P.at(0, 3) + local.y * P.at(1, 3) + local.z * P.at(2, 3) + P.at(3, 3);
+        float fx = (vx / vw + 1.0f) / 2.0f * (float)width;
+        float fy = (1.0f - vy / vw) / 2.0f * (float)height;
+        if (fx < 0.0f || fx > (float)width || fy < 0.0f || fy > (float)height) return false;
+        float cosine = local.z / length(local);
+        float d = image_plane_distance / cosine;
+        float jac = d * d / std::fabs(cosine);
+        V3 to_cam = position - pos;
+        float dist = length(to_cam);
+        float cos_cam = std::fabs(dot(normal, to_cam)) / dist;
+        jac *= dist == 0.0f ? 0.0f : cos_cam / (dist * dist);
+        *px = std::min(std::max(to_int(fx), 0), width - 1);
+        *py = std::min(std::max(to_int(fy), 0), height - 1);
+        *weight = jac;
         return true;
     }
     // PerspectiveCamera.GenerateRay (PerspectiveCamera.cs:64-102), lensRadius = 0.
@@ -964,6 +1003,7 @@ struct Shader {
     SurfacePoint p;
     Frame f;
     V3 out_local, out_world;
+    bool light = false;  // ShadingContext.IsOnLightSubpath
 };
 
 inline Shader make_shader(const or_scene& s, const SurfacePoint& p, V3 out_dir) {
@@ -1061,7 +1101,7 @@ void generic_value_pdf(const Shader& sh, V3 in, const GenericLocal& lp, Rgb* val
         float sqrt_denom = c_out + eta * c_in;
         if (!same_hemisphere(out, in) && c_out * c_in < 0.0f) {
             Rgb F = rgb(fresnel_dielectric(c_out, 1.0f, m.ior));
-            float factor = 1.0f;  // camera subpath
+            float factor = sh.light ? (1.0f / eta) : 1.0f;  // GenericMaterial.cs:255
             V3 wh = !same_hemisphere(out, ht) ? -ht : ht;
             float num = ggx.ndf(wh) * ggx.g(out, in);
             num *= eta * eta * std::max(0.0f, dot(in, -wh)) * std::max(0.0f, dot(out, wh));
@@ -1934,6 +1974,191 @@ uint64_t or_render(const or_scene* s, const sspt_pt_settings* cfg, uint32_t firs
         stats->num_shadow_rays = shadow;
         stats->num_occluded = occl;
         stats->num_ray_hits = hits;
+    }
+    return nans;
+}
+
+// ------------------------------------------------------------ light tracer -
+namespace {
+
+// TinyEmbree Raytracer.IsOccluded(from, Vector3 to) with a free end point (the
+// camera): only the surface end is offset, d = to - p0, t in (1e-5, 1 - 1e-5).
+inline Ray shadow_ray_to(const SurfacePoint& from, V3 to) {
+    V3 p0 = offset_point(from, to - from.position);
+    Ray r;
+    r.o = p0;
+    r.d = to - p0;
+    r.tmin = 1e-5f;
+    r.tmax = 1.0f - 1e-5f;
+    return r;
+}
+
+struct EmitterRay {
+    SurfacePoint p;
+    V3 dir;
+    float pdf;
+    Rgb weight;
+};
+// Emitter.SampleRay: DiffuseEmitter (DiffuseEmitter.cs:44-64), GlossyEmitter
+// (GlossyEmitter.cs:47-70); the point from Triangle.SampleUniformArea (Triangle.cs:42-56).
+EmitterRay emitter_sample_ray(const or_scene& s, const Emitter& e, V2 primary_pos, V2 primary_dir) {
+    EmitterRay r;
+    r.p = make_point(s, e.tri, to_uniform_triangle(primary_pos), 0.0f);
+    float pos_pdf = e.inv_area;
+    V3 n = r.p.shading_normal, t, b;
+    basis(n, &t, &b);
+    bool glossy = e.kind == SSPT_EMITTER_GLOSSY;
+    DirSample local = glossy ? to_cosine_lobe(e.exponent + 1.0f, primary_dir) : to_cos_hemisphere(primary_dir);
+    r.dir = local.dir.z * n + local.dir.x * t + local.dir.y * b;
+    r.pdf = local.pdf * pos_pdf;
+    if (glossy) {
+        Rgb w = e.radiance * std::pow(local.dir.z, e.exponent + 1.0f) * e.norm;
+        r.weight = w / pos_pdf / local.pdf;
+    } else {
+        r.weight = e.radiance / pos_pdf * kPi;
+    }
+    return r;
+}
+
+struct Splat {
+    int64_t pix;
+    Rgb value;
+};
+
+struct LightCtx {
+    const or_scene* s;
+    const sspt_lt_settings* cfg;
+    float num_paths;
+    Counters* c;
+};
+
+// BidirBase.ConnectLightVertexToCamera (BidirBase.Light.cs:35-92) with MIS weight 1.
+void connect_to_camera(const LightCtx& lc, const SurfacePoint& v, V3 ancestor, Rgb weight, int depth,
+                       std::vector<Splat>* out) {
+    const or_scene& s = *lc.s;
+    if (depth + 1 < lc.cfg->min_depth) return;
+    int px, py;
+    float jac;
+    if (!s.camera.sample_response(v.position, v.normal, &px, &py, &jac)) return;
+    lc.c->shadow++;
+    if (trace_any(s, shadow_ray_to(v, s.camera.position), false)) {
+        lc.c->occluded++;
+        return;
+    }
+    V3 to_cam = s.camera.position - v.position;
+    float dist = length(to_cam);
+    to_cam = to_cam / dist;
+    V3 to_anc = normalize(ancestor - v.position);
+    Shader sh = make_shader(s, v, to_cam);  // the camera side: IsOnLightSubpath = false
+    Rgb f = eval_nocos(sh, to_anc);
+    if (is_black(f)) return;
+    f = f * (std::fabs(dot(v.shading_normal, to_anc)) / std::fabs(dot(v.normal, to_anc)));
+    Rgb w = weight * f * rgb(jac) / lc.num_paths;
+    out->push_back({(int64_t)py * s.width + px, w});
+}
+
+// BidirBase.TraceLightPath / TraceEmitterPath (BidirBase.Light.cs:212-277) and
+// RandomWalk.StartFromEmitter / ContinueWalk (RandomWalk.cs:82-90,150-217) for
+// one light path, each vertex connected to the camera as LightPathWalk.OnHit
+// stores it (the connection draws no random numbers, so connecting during the
+// walk equals connecting the cached vertices afterwards).
+void light_path(const LightCtx& lc, uint32_t idx, uint32_t iteration, std::vector<Splat>* out) {
+    const or_scene& s = *lc.s;
+    const sspt_lt_settings& cfg = *lc.cfg;
+    Rng rng{hash_seed(cfg.base_seed_light, idx, iteration)};
+    // SelectLight (:103-110): without a background BackgroundProbability is 0 and draws nothing
+    int count = (int)s.emitters.size();
+    const Emitter& e = s.emitters[rng.next_int((uint32_t)count)];
+    float sel = (1.0f - 0.0f) / (float)count;
+    V2 ppos = rng.next_float2();
+    V2 pdir = rng.next_float2();
+    EmitterRay er = emitter_sample_ray(s, e, ppos, pdir);
+    er.pdf *= sel;
+    if (er.pdf == 0.0f || is_black(er.weight)) return;
+    Rgb prefix = er.weight / sel;
+    Rgb approx = rgb(1.0f);
+    Ray ray = spawn_ray(er.p, er.dir);
+    SurfacePoint prev = er.p;
+    float pdf_dir = er.pdf;
+    for (int depth = 1; depth < cfg.max_depth;) {
+        HitRec h = trace_closest(s, ray, false);
+        lc.c->rays++;
+        if (h.tri < 0) break;
+        lc.c->hits++;
+        SurfacePoint hit = make_point(s, h.tri, {h.u, h.v}, h.t);
+        float pdf_from = pdf_dir * area_to_solid_angle(prev, hit);
+        if (pdf_from == 0.0f) break;
+        connect_to_camera(lc, hit, prev.position, prefix, depth, out);
+        if (depth + 1 >= cfg.max_depth) break;
+        float q = depth > 4 ? std::min(std::max(average(approx), 0.05f), 0.95f) : 1.0f;
+        if (rng.next_float() > q) break;
+        Shader sh = make_shader(s, hit, -ray.d);
+        sh.light = true;
+        float pc = rng.next_float();
+        V2 pd = rng.next_float2();
+        BsdfSample bs = sample_bsdf(sh, pc, pd);
+        approx = approx * (bs.weight / q);
+        if (bs.pdf == 0.0f || is_black(bs.weight)) break;
+        // RandomWalk.cs:193-204: geometric instead of shading cosine, and the
+        // rendering equation's cosine ratio at the light subpath's vertex
+        Rgb w = bs.weight * (std::fabs(dot(hit.normal, bs.dir)) / std::fabs(dot(hit.shading_normal, bs.dir)));
+        w = w * (std::fabs(dot(hit.shading_normal, -ray.d)) / std::fabs(dot(hit.normal, -ray.d)));
+        prefix = prefix * (w / q);
+        ++depth;
+        pdf_dir = bs.pdf;
+        prev = hit;
+        ray = spawn_ray(hit, bs.dir);
+    }
+}
+
+}  // namespace
+
+uint64_t or_render_light_tracer(const or_scene* s, const sspt_lt_settings* cfg, uint32_t first, uint32_t num,
+                                float* image, sspt_ray_stats* stats, int32_t threads) {
+    if (s->has_env || s->emitters.empty()) return UINT64_MAX;
+    const int64_t W = s->width, H = s->height;
+    const uint32_t paths = cfg->num_light_paths < 0 ? (uint32_t)(W * H) : (uint32_t)cfg->num_light_paths;
+    std::atomic<uint64_t> rays{0}, shadow{0}, occl{0}, hits{0};
+    uint64_t nans = 0;
+    std::vector<std::vector<Splat>> splats(paths);
+    for (uint32_t it = first; it < first + num; ++it) {
+        // FrameBuffer.StartIteration / Normalize (FrameBuffer.cs:232-249)
+        float n = (float)(it + 1);
+        if (it + 1 > 1) {
+            float scale = (n - 1.0f) / n;
+            for (int64_t k = 0; k < W * H * 3; ++k) image[k] *= scale;
+        }
+        int64_t chunks = ((int64_t)paths + 63) / 64;
+        parallel_for(chunks, threads, [&](int64_t ch) {
+            Counters c;
+            LightCtx lc{s, cfg, (float)paths, &c};
+            int64_t end = std::min<int64_t>(paths, (ch + 1) * 64);
+            for (int64_t k = ch * 64; k < end; ++k) {
+                splats[k].clear();
+                light_path(lc, (uint32_t)k, it, &splats[k]);
+            }
+            rays += c.rays, shadow += c.shadow, occl += c.occluded, hits += c.hits;
+        });
+        // FrameBuffer.Splat (FrameBuffer.cs:160-191), IgnoreNanAndInf, in light-path order
+        for (uint32_t k = 0; k < paths; ++k)
+            for (const Splat& sp : splats[k]) {
+                if (!std::isfinite(average(sp.value))) {
+                    ++nans;
+                    continue;
+                }
+                Rgb add = sp.value / n;
+                image[sp.pix * 3 + 0] += add.x;
+                image[sp.pix * 3 + 1] += add.y;
+                image[sp.pix * 3 + 2] += add.z;
+            }
+    }
+    if (stats) {
+        stats->num_rays = rays;
+        stats->num_shadow_rays = shadow;
+        stats->num_occluded = occl;
+        stats->num_ray_hits = hits;
+        stats->num_shadow_rays_traced = shadow;
+        stats->num_iterations = num;
     }
     return nans;
 }

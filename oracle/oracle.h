@@ -116,6 +116,14 @@ void or_render_debug(const or_scene* s, uint32_t base_seed, uint32_t first, uint
  * returns the node count. */
 uint32_t or_replay_pixel(const or_scene* s, const sspt_pt_settings* settings, uint32_t col, uint32_t row,
                          uint32_t iteration, sspt_path_node* nodes, uint32_t capacity, float* estimate_rgb);
+/* ---- light tracing: BidirBase.Render (SeeSharp/Integrators/Bidir/BidirBase.cs:84-146,
+ * BidirBase.Light.cs:35-277, RandomWalk.cs) of a VertexCacheBidir set up as a pure
+ * light tracer, the contract of sspt_render_light_tracer; the splats of an
+ * iteration are added in light-path order. Returns the non-finite splat count,
+ * or UINT64_MAX for a scene with a background or without emitters. */
+uint64_t or_render_light_tracer(const or_scene* s, const sspt_lt_settings* settings, uint32_t first_iteration,
+                                uint32_t num_iterations, float* image, sspt_ray_stats* stats,
+                                int32_t num_threads);
 void or_primary_aov(const or_scene* s, const sspt_pt_settings* settings, uint32_t first_iteration,
                     uint32_t num_iterations, float* albedo, float* normal, int32_t num_threads);
 

@@ -18,4 +18,5 @@ from .scene import (  # noqa: F401
     TextureRgb,
     create_look_at,
 )
-from .integrator import DebugVisualizer, Device, GpuBatch, GpuPathTracer, GpuScene, Integrator  # noqa: F401
+from .integrator import (DebugVisualizer, Device, GpuBatch, GpuPathTracer, GpuScene, Integrator,  # noqa: F401
+                         VertexCacheBidirLightTracer)

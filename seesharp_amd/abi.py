@@ -13,7 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 # different occupancy targets); the default is the in-tree build.
 LIB_PATH = os.environ.get("SSPT_LIBRARY") or os.path.join(_HERE, "libsspt.so")
 
-ABI_VERSION = 5  # SSPT_ABI_VERSION of include/sspt.h
+ABI_VERSION = 6  # SSPT_ABI_VERSION of include/sspt.h
 NOALLOC_SCRATCH_BYTES = 64 << 20  # SSPT_NOALLOC_SCRATCH_BYTES
 SSPT_OK = 0
 INVALID_ARGUMENT = 1
@@ -123,6 +123,16 @@ class PtSettings(C.Structure):
     ]
 
 
+class LtSettings(C.Structure):
+    """sspt_lt_settings: the light tracer's BidirBase / Integrator fields."""
+    _fields_ = [
+        ("base_seed_light", C.c_uint32),
+        ("num_light_paths", C.c_int32),
+        ("max_depth", C.c_int32),
+        ("min_depth", C.c_int32),
+    ]
+
+
 class RayStats(C.Structure):
     _fields_ = [
         ("num_rays", C.c_uint64),
@@ -198,6 +208,8 @@ SYMBOLS = {
     "sspt_set_launch_config": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_int32]),
     "sspt_set_block_shape": (C.c_int, [_vp, C.c_uint32, C.c_uint32]),
     "sspt_get_block_shape": (C.c_int, [_vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "sspt_render_light_tracer": (C.c_int, [_vp, _vp, C.POINTER(LtSettings), C.c_uint32, C.c_uint32,
+                                           C.POINTER(RayStats)]),
     "sspt_set_bvh_builder": (C.c_int, [_vp, C.c_int32]),
     "sspt_get_bvh_builder": (C.c_int, [_vp, C.POINTER(C.c_int32)]),
     "sspt_scene_bvh_build_time": (C.c_int, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_double),

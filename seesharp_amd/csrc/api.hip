@@ -91,6 +91,7 @@ struct sspt_ctx {
     bool use_graphs = true;
     int32_t exec_mode = SSPT_EXEC_WAVEFRONT;
     int32_t bvh_builder = SSPT_BVH_DEVICE;
+    unsigned long long* lt_stats = nullptr;  // the light tracer's striped ray statistics
     // adjoint buffers, grown on demand
     std::vector<void*> adj_allocs;
     uint64_t adj_records = 0;   // capacity * depth
@@ -251,6 +252,10 @@ bool setup_camera(const sspt_camera_desc& cd, int w, int h, DevCamera* cam) {
     p[11] = -1.0f;
     p[14] = range * 0.001f;
     if (!invert4(p, cam->v2c)) return false;
+    std::memcpy(cam->w2c, cd.world_to_camera, sizeof(cam->w2c));
+    std::memcpy(cam->c2v, p, sizeof(cam->c2v));
+    // PerspectiveCamera.UpdateResolution (PerspectiveCamera.cs:48-51)
+    cam->image_plane = (float)h / (2.0f * std::tan(fov * 0.5f));
     const float* W = cam->c2w;
     // Transform((0,0,0), cameraToWorld) = translation row
     cam->pos[0] = 0.0f * W[0] + 0.0f * W[4] + 0.0f * W[8] + W[12];
@@ -1100,6 +1105,7 @@ void sspt_ctx_destroy(sspt_ctx* ctx) {
     for (void* p : ctx->wave_allocs) (void)hipFree(p);
     for (void* p : ctx->adj_allocs) (void)hipFree(p);
     if (ctx->wave_descs) (void)hipFree(ctx->wave_descs);
+    if (ctx->lt_stats) (void)hipFree(ctx->lt_stats);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -1662,6 +1668,41 @@ sspt_status sspt_render_debug(sspt_ctx* ctx, sspt_scene* sc, uint32_t index, uin
                            num_iterations, sc->image + R.fb_off);
     HIP_TRY(ctx, hipGetLastError());
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return SSPT_OK;
+}
+
+sspt_status sspt_render_light_tracer(sspt_ctx* ctx, sspt_scene* sc, const sspt_lt_settings* cfg,
+                                     uint32_t first_iteration, uint32_t num_iterations, sspt_ray_stats* stats) {
+    if (!ctx || !sc || !cfg) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "null argument");
+    if (sc->recs.size() != 1)
+        return fail(ctx, SSPT_ERR_UNSUPPORTED, "the light tracer renders a single scene, not a batch or view set");
+    const SceneRec& R = sc->recs[0];
+    if (R.has_env)
+        return fail(ctx, SSPT_ERR_UNSUPPORTED, "the light tracer does not sample the background (scene has one)");
+    if (R.num_emitters == 0) return fail(ctx, SSPT_ERR_UNSUPPORTED, "the light tracer needs emitters");
+    if (cfg->max_depth < 0 || cfg->min_depth < 0)
+        return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "max_depth and min_depth must be non-negative");
+    const uint64_t paths = cfg->num_light_paths < 0 ? (uint64_t)R.num_pixels : (uint64_t)cfg->num_light_paths;
+    if (paths > 0xFFFFFFFFull) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "too many light paths");
+    if ((uint64_t)first_iteration + num_iterations > 0xFFFFFFFFull)
+        return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "iteration index overflows");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    if (!ctx->lt_stats) HIP_TRY(ctx, hipMalloc((void**)&ctx->lt_stats, kStatBytes));
+    HIP_TRY(ctx, hipMemsetAsync(ctx->lt_stats, 0, kStatBytes, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(sc->d_recs, &R, sizeof(SceneRec), hipMemcpyHostToDevice, ctx->stream));
+    float* image = sc->image + R.fb_off;
+    for (uint32_t it = first_iteration; it < first_iteration + num_iterations; ++it) {
+        // FrameBuffer.StartIteration / Normalize (FrameBuffer.cs:232-249)
+        const float n = (float)(it + 1);
+        if (it + 1 > 1) launch_scale(ctx->stream, image, 3ull * R.num_pixels, (n - 1.0f) / n);
+        if (paths > 0) {
+            LightParams P{cfg->base_seed_light, it, (uint32_t)paths, cfg->max_depth, cfg->min_depth, (float)paths, n};
+            launch_light_paths(ctx->stream, sc->dev, 0, P, image, ctx->lt_stats);
+        }
+        HIP_TRY(ctx, hipGetLastError());
+    }
+    TRY(read_stats(ctx, sc, stats, ctx->lt_stats));
+    if (stats) stats->num_iterations = num_iterations;
     return SSPT_OK;
 }
 

@@ -107,6 +107,16 @@ void launch_primary_rays(hipStream_t s, const DevScene& sc, int scene, uint32_t 
 
 // DebugVisualizer: eye-light shading of the primary hits of iterations
 // first..first+num-1 of scene record `scene` into its running-mean image.
+// One iteration of the light tracer: num_paths light paths seeded
+// (base_seed, path, iteration), splats / n added to `image` (the scene's framebuffer).
+struct LightParams {
+    uint32_t base_seed, iteration, num_paths;
+    int max_depth, min_depth;
+    float num_paths_f;  // NumLightPaths as a float
+    float n;            // FrameBuffer.CurIteration
+};
+void launch_light_paths(hipStream_t s, const DevScene& sc, int scene, const LightParams& p, float* image,
+                        unsigned long long* stats);
 void launch_debug_visualize(hipStream_t s, const DevScene& sc, int scene, uint32_t num_pixels, uint32_t base_seed,
                             uint32_t first, uint32_t num, float* image);
 

@@ -1327,6 +1327,185 @@ __global__ void __launch_bounds__(kTraceBlock) k_debug_visualize(DevScene S, int
     }
 }
 
+// ------------------------------------------------------------ light tracer -
+// PerspectiveCamera.SampleResponse (PerspectiveCamera.cs:111-146) of the pinhole:
+// the pixel that sees `pos` (WorldToFilm, :148-166) and the importance weight
+// SolidAngleToPixelJacobian(pos) (:177-191) * Camera.SurfaceAreaToSolidAngleJacobian
+// (Camera.cs:91-98) of the surface normal. Correctly rounded divisions and
+// square roots, as for camera rays, so the film position picks the CPU's pixel.
+__device__ __forceinline__ bool camera_response(const DevCamera& c, f3 pos, f3 normal, uint32_t* pix, float* weight) {
+    const float* M = c.w2c;
+    const f3 l = mk3(pos.x * M[0] + pos.y * M[4] + pos.z * M[8] + M[12],
+                     pos.x * M[1] + pos.y * M[5] + pos.z * M[9] + M[13],
+                     pos.x * M[2] + pos.y * M[6] + pos.z * M[10] + M[14]);
+    if (l.z > 0.0f) return false;
+    const float* P = c.c2v;  // Vector4.Transform(local, cameraToView), w = 1
+    const float vx = l.x * P[0] + l.y * P[4] + l.z * P[8] + P[12];
+    const float vy = l.x * P[1] + l.y * P[5] + l.z * P[9] + P[13];
+    const float vw = l.x * P[3] + l.y * P[7] + l.z * P[11] + P[15];
+    const float fx = (vx / vw + 1.0f) / 2.0f * (float)c.width;
+    const float fy = (1.0f - vy / vw) / 2.0f * (float)c.height;
+    if (fx < 0.0f || fx > (float)c.width || fy < 0.0f || fy > (float)c.height) return false;
+    const float cosine = l.z / sqrtf(dot(l, l));
+    const float d = c.image_plane / cosine;
+    float jac = d * d / fabsf(cosine);
+    const f3 to_cam = mk3(c.pos[0], c.pos[1], c.pos[2]) - pos;
+    const float dist = sqrtf(dot(to_cam, to_cam));
+    const float cos_cam = fabsf(dot(normal, to_cam)) / dist;
+    jac *= dist == 0.0f ? 0.0f : cos_cam / (dist * dist);
+    const int px = min(max(to_int(fx), 0), c.width - 1), py = min(max(to_int(fy), 0), c.height - 1);
+    *pix = (uint32_t)py * (uint32_t)c.width + (uint32_t)px;
+    *weight = jac;
+    return true;
+}
+
+struct EmitterRay {
+    Surf p;
+    f3 dir;
+    float pdf;
+    f3 weight;
+};
+// Emitter.SampleRay: DiffuseEmitter (DiffuseEmitter.cs:44-64) and GlossyEmitter
+// (GlossyEmitter.cs:47-70, SampleWarp.ToCosineLobe SampleWarp.cs:171-182) from a
+// point of Triangle.SampleUniformArea (Triangle.cs:42-56).
+__device__ __forceinline__ EmitterRay emitter_ray(const DevScene& S, const DevEmitter& e, f2 primary_pos,
+                                                  f2 primary_dir) {
+    EmitterRay r;
+    const f2 b = to_uniform_triangle(primary_pos);
+    r.p = surface(S, e.tri, b.x, b.y);
+    const float pos_pdf = e.inv_area;
+    Frame f;
+    f.build(r.p.ns);
+    f3 local;
+    float pdf;
+    const f3 radiance = mk3(e.r, e.g, e.b);
+    if (e.kind == kEmitterGlossy) {
+        const float power = e.exponent + 1.0f;
+        const float phi = kPi * 2.0f * primary_dir.x;
+        const float cos_t = powf(primary_dir.y, srcp(power + 1.0f));
+        const float sin_t = ssqrt(1.0f - cos_t * cos_t);
+        float sp, cp;
+        ssincos(phi, &sp, &cp);
+        local = mk3(sin_t * cp, sin_t * sp, cos_t);
+        pdf = sdiv((power + 1.0f) * powf(cos_t, power) * 1.0f, 2.0f * kPi);
+        r.weight = radiance * powf(cos_t, e.exponent + 1.0f) * e.norm / pos_pdf / pdf;
+    } else {
+        local = to_cos_hemisphere(primary_dir);
+        pdf = sdiv(local.z, kPi);
+        r.weight = radiance / pos_pdf * kPi;
+    }
+    r.dir = f.to_world(local);
+    r.pdf = pdf * pos_pdf;
+    return r;
+}
+
+// SurfaceShader.Evaluate: the BSDF value without the cosine.
+__device__ __forceinline__ f3 material_value(const Shader& sh, f3 in_world) {
+    f3 v;
+    float pdf;
+    if (sh.m.kind == 1) generic_value_pdf(sh, sh.f.to_local(in_world), &v, &pdf);
+    else diffuse_value_pdf(sh, in_world, &v, &pdf);
+    return v;
+}
+
+// BidirBase.TraceLightPaths + SplatLightVertices for one iteration of a
+// VertexCacheBidir set up as a pure light tracer: one thread follows one light
+// path (SelectLight, TraceEmitterPath, BidirBase.Light.cs:103-110,212-277;
+// RandomWalk.StartFromEmitter / ContinueWalk, RandomWalk.cs:82-90,150-217) and
+// connects each vertex to the camera as the walk stores it
+// (ConnectLightVertexToCamera, BidirBase.Light.cs:35-92, MIS weight 1): the
+// connection draws no random numbers, so this equals splatting the cached
+// vertices afterwards. Splats are added to the running-mean image with fp32
+// atomics (FrameBuffer.Splat: value / CurIteration; non-finite ones counted).
+__global__ void __launch_bounds__(kTraceBlock) k_light_paths(DevScene S, int scene, LightParams P, float* image,
+                                                             unsigned long long* stats) {
+    __shared__ int stack_node[kStackNodeLds];
+    __shared__ float stack_t[kStackLds * kTraceBlock];
+    const SceneRec& R = S.recs[scene];
+    const f3 cam = mk3(R.cam.pos[0], R.cam.pos[1], R.cam.pos[2]);
+    for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < P.num_paths; idx += gridDim.x * blockDim.x) {
+        Rng rng{hash_seed(P.base_seed, idx, P.iteration)};
+        // SelectLight: no background, so BackgroundProbability = 0 draws nothing
+        const DevEmitter e = S.emitters[R.emitter_off + (int)rng.next_int((uint32_t)R.num_emitters)];
+        const float sel = (1.0f - 0.0f) / (float)R.num_emitters;
+        const f2 ppos = rng.next_float2();
+        const f2 pdir = rng.next_float2();
+        EmitterRay er = emitter_ray(S, e, ppos, pdir);
+        er.pdf *= sel;
+        if (er.pdf == 0.0f || is_zero(er.weight)) continue;
+        f3 prefix = er.weight / sel;
+        f3 approx = gray(1.0f);
+        Surf prev = er.p;
+        float pdf_dir = er.pdf;
+        f3 o = offset_point(er.p, er.dir), d = er.dir;
+        float tmin = er.p.err;
+        for (int depth = 1; depth < P.max_depth;) {
+            const HitOut h = traverse<false>(S, R.root, R.abs_max, o, d, tmin, INFINITY, stack_node + threadIdx.x,
+                                             stack_t + threadIdx.x);
+            wave_count(stats, kStatRays, true);
+            wave_count(stats, kStatHits, h.li >= 0);
+            if (h.li < 0) break;
+            const Surf s = surface(S, h.li, h.u, h.v);
+            if (pdf_dir * area_to_solid_angle(prev.pos, s.pos, s.ng) == 0.0f) break;
+            uint32_t pix;
+            float jac;
+            if (depth + 1 >= P.min_depth && camera_response(R.cam, s.pos, s.ng, &pix, &jac)) {
+                // Raytracer.IsOccluded(vertex, camera position): only the surface end is offset
+                const f3 p0 = offset_point(s, cam - s.pos);
+                const bool occ = traverse<true>(S, R.root, R.abs_max, p0, cam - p0, 1e-5f, 1.0f - 1e-5f,
+                                                stack_node + threadIdx.x, stack_t + threadIdx.x)
+                                     .li >= 0;
+                wave_count(stats, kStatShadow, true);
+                wave_count(stats, kStatShadowTraced, true);
+                wave_count(stats, kStatOccluded, occ);
+                if (!occ) {
+                    f3 to_cam = cam - s.pos;
+                    to_cam = to_cam / length(to_cam);
+                    const f3 to_anc = normalize(prev.pos - s.pos);
+                    Shader sh;
+                    make_shader(S, s, to_cam, &sh);  // the camera side: IsOnLightSubpath = false
+                    f3 f = material_value(sh, to_anc);
+                    if (!is_zero(f)) {
+                        f = f * sdiv(fabsf(dot(s.ns, to_anc)), fabsf(dot(s.ng, to_anc)));
+                        const f3 w = prefix * f * jac / P.num_paths_f;
+                        const bool bad = !isfinite(average(w));
+                        wave_count(stats, kStatNan, bad);
+                        if (!bad) {
+                            const f3 add = div_exact(w, P.n);
+                            float* px = image + 3ull * pix;
+                            atomicAdd(px, add.x);
+                            atomicAdd(px + 1, add.y);
+                            atomicAdd(px + 2, add.z);
+                        }
+                    }
+                }
+            }
+            if (depth + 1 >= P.max_depth) break;
+            const float q = depth > 4 ? clampf(average(approx), 0.05f, 0.95f) : 1.0f;
+            if (rng.next_float() > q) break;
+            Shader sh;
+            make_shader(S, s, -d, &sh);
+            sh.light = true;
+            const float pc = rng.next_float();
+            const f2 pd = rng.next_float2();
+            const BsdfSample bs = sample_bsdf(sh, pc, pd);
+            approx = approx * (bs.weight / q);
+            if (bs.pdf == 0.0f || is_zero(bs.weight)) break;
+            // RandomWalk.cs:193-204: the geometric instead of the shading cosine, and the
+            // rendering equation's cosine ratio at this vertex of the light subpath
+            f3 w = bs.weight * sdiv(fabsf(dot(s.ng, bs.dir)), fabsf(dot(s.ns, bs.dir)));
+            w = w * sdiv(fabsf(dot(s.ns, -d)), fabsf(dot(s.ng, -d)));
+            prefix = prefix * (w / q);
+            ++depth;
+            pdf_dir = bs.pdf;
+            prev = s;
+            o = offset_point(s, bs.dir);
+            d = bs.dir;
+            tmin = s.err;
+        }
+    }
+}
+
 __global__ void __launch_bounds__(256) k_adjoint_finalize(DevScene S, Wave w) {
     const WaveDesc d = cur_wave(w);
     for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < d.count; slot += gridDim.x * blockDim.x) {
@@ -1774,6 +1953,12 @@ void launch_primary_rays(hipStream_t s, const DevScene& sc, int scene, uint32_t 
                          uint32_t iteration, float* rays) {
     hipLaunchKernelGGL(k_primary_rays, dim3(blocks(num_pixels, 256)), dim3(256), 0, s, sc, scene, num_pixels,
                        base_seed, iteration, rays);
+}
+
+void launch_light_paths(hipStream_t s, const DevScene& sc, int scene, const LightParams& p, float* image,
+                        unsigned long long* stats) {
+    hipLaunchKernelGGL(k_light_paths, dim3(blocks(p.num_paths, kTraceBlock)), dim3(kTraceBlock), 0, s, sc, scene, p,
+                       image, stats);
 }
 
 void launch_debug_visualize(hipStream_t s, const DevScene& sc, int scene, uint32_t num_pixels, uint32_t base_seed,

@@ -98,7 +98,10 @@ struct DevEmitter {  // 32 B
 struct DevCamera {
     float v2c[16];  // view -> camera (Matrix4x4 row-major, row vectors)
     float c2w[16];  // camera -> world
+    float w2c[16];  // world -> camera (PerspectiveCamera.SampleResponse)
+    float c2v[16];  // camera -> view (the projection)
     float pos[3];
+    float image_plane;  // distance to the image plane of unit-area pixels
     int width, height;
 };
 

@@ -77,10 +77,12 @@ struct Shader {
     // GenericMaterial.LocalParams (GenericMaterial.cs:340-381); base color for diffuse.
     f3 base, spec_tint, diffuse_refl, spec_trans, spec_r0;
     float roughness, ax, ay;
+    bool light;  // ShadingContext.IsOnLightSubpath
 };
 
 __device__ __forceinline__ void make_shader(const DevScene& S, const Surf& s, f3 out_world, Shader* sh) {
     sh->m = S.materials[s.mat];
+    sh->light = false;
     sh->f.build(s.ns);
     sh->out_world = out_world;
     sh->out_local = sh->f.to_local(out_world);
@@ -157,6 +159,10 @@ __device__ void generic_value_pdf(const Shader& sh, f3 in, f3* value, float* pdf
             float F = fresnel_dielectric(c_out, 1.0f, m.ior);
             float num = ggx.ndf(wh) * ggx.g(out, in);
             num *= eta * eta * fmaxf(0.0f, dot(in, -wh)) * fmaxf(0.0f, dot(out, wh));
+            if (sh.light) {  // GenericMaterial.cs:255: importance, not radiance, is transported
+                const float factor = srcp(eta);
+                num *= factor * factor;
+            }
             float denom = in.z * out.z * sqrt_denom * sqrt_denom;
             val = val + (gray(1.0f) - gray(F)) * sh.spec_trans * fabsf(sdiv(num, denom));
         }

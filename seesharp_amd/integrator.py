@@ -254,6 +254,13 @@ class GpuScene(_TextureParameters):
             self.device.ptr, self.ptr, abi.fptr(rays), rays.shape[0], out.ctypes.data_as(C.POINTER(C.c_uint8))))
         return out
 
+    def render_light_tracer(self, settings: abi.LtSettings, first_iteration: int, num_iterations: int) -> dict:
+        """Iterations first..first+num-1 of the light tracer into the framebuffer (running mean)."""
+        stats = abi.RayStats()
+        self.device.check(self.device.lib.sspt_render_light_tracer(
+            self.device.ptr, self.ptr, C.byref(settings), first_iteration, num_iterations, C.byref(stats)))
+        return stats.as_dict()
+
     def render_debug(self, base_seed: int, first_iteration: int, num_iterations: int):
         """DebugVisualizer iterations into the running-mean framebuffer."""
         self.device.check(self.device.lib.sspt_render_debug(self.device.ptr, self.ptr, 0, base_seed & 0xFFFFFFFF,
@@ -562,4 +569,45 @@ class DebugVisualizer(Integrator):
         fb.MetaData["NumIterations"] = self.TotalSpp
         fb.Image = gs.read_image()
         fb.CurIteration = self.TotalSpp
+        return fb
+
+
+class VertexCacheBidirLightTracer(Integrator):
+    """Light tracing: a VertexCacheBidir (SeeSharp/Integrators/Bidir/VertexCacheBidir.cs)
+    with NumConnections = 0, NumShadowRays = 0, EnableHitting = false and
+    EnableLightTracer = true, rendered by BidirBase.Render
+    (SeeSharp/Integrators/Bidir/BidirBase.cs:84-146). Every iteration traces
+    NumLightPaths paths from the emitters and splats each vertex onto the camera
+    (BidirBase.Light.cs:35-277). Scenes with a background are not supported."""
+
+    def __init__(self, device: Device | None = None):
+        super().__init__()
+        self.NumIterations = 2
+        self.NumLightPaths = -1
+        self.BaseSeedLight = 0x13C0FEFE
+        self.device = device
+        self._gpu_scene = None
+
+    def settings(self) -> abi.LtSettings:
+        return abi.LtSettings(self.BaseSeedLight & 0xFFFFFFFF, int(self.NumLightPaths), self.MaxDepth,
+                              self.MinDepth)
+
+    def Render(self, scene):
+        if self.device is None:
+            self.device = Device(0)
+        if self._gpu_scene is None or self._gpu_scene.scene is not scene:
+            self._gpu_scene = GpuScene(self.device, scene)
+        gs = self._gpu_scene
+        fb = scene.FrameBuffer
+        gs.clear()
+        t0 = time.perf_counter()
+        stats = gs.render_light_tracer(self.settings(), 0, self.NumIterations)
+        fb.MetaData["RenderTime"] = (time.perf_counter() - t0) * 1000.0
+        fb.MetaData["NumIterations"] = self.NumIterations
+        fb.MetaData["RayTracerStats"] = {
+            "NumRays": stats["num_rays"], "NumShadowRays": stats["num_shadow_rays"],
+            "NumOccluded": stats["num_occluded"], "NumRayHits": stats["num_ray_hits"]}
+        fb.Image = gs.read_image()
+        fb.CurIteration = self.NumIterations
+        fb.NaNWarnings = gs.nan_count()
         return fb

@@ -56,6 +56,8 @@ _SIGS = {
     "or_primary_hits": (None, [_vp, C.c_uint32, C.c_uint32, _fp, C.POINTER(abi.Hit), C.c_int32]),
     "or_render": (C.c_uint64, [_vp, C.POINTER(abi.PtSettings), C.c_uint32, C.c_uint32, _fp,
                                C.POINTER(abi.RayStats), C.POINTER(C.c_int32), C.c_int64, C.c_int32]),
+    "or_render_light_tracer": (C.c_uint64, [_vp, C.POINTER(abi.LtSettings), C.c_uint32, C.c_uint32, _fp,
+                                            C.POINTER(abi.RayStats), C.c_int32]),
     "or_render_pixel": (None, [_vp, C.POINTER(abi.PtSettings), C.c_uint32, C.c_uint32, C.c_uint32, _fp]),
     "or_set_texture": (C.c_int32, [_vp, C.c_uint32, _fp]),
     "or_render_outliers": (None, [_vp, C.POINTER(abi.PtSettings), C.c_uint32, C.c_uint32, C.c_int32, _fp,
@@ -91,6 +93,11 @@ def lib():
 
 def fp(a):
     return a.ctypes.data_as(_fp)
+
+
+def lt_settings(base_seed_light=0x13C0FEFE, num_light_paths=-1, max_depth=5, min_depth=1):
+    # (MaxDepth defaults to 100 in the reference's Integrator; the tests pass small depths)
+    return abi.LtSettings(base_seed_light, num_light_paths, max_depth, min_depth)
 
 
 def settings(base_seed=0xC030114, total_spp=1, max_depth=5, min_depth=1, num_shadow_rays=1,
@@ -129,6 +136,17 @@ class OracleScene:
             px = pixels.ctypes.data_as(C.POINTER(C.c_int32))
             n = pixels.size
         nans = lib().or_render(self.ptr, C.byref(cfg), first, num, fp(image), C.byref(stats), px, n, threads)
+        return image, stats, int(nans)
+
+    def render_light_tracer(self, cfg, first=0, num=1, image=None, threads=0):
+        """The light tracer's running-mean image after iterations first..first+num-1,
+        its ray statistics and the non-finite splat count."""
+        if image is None:
+            image = np.zeros((self.height, self.width, 3), np.float32)
+        stats = abi.RayStats()
+        nans = lib().or_render_light_tracer(self.ptr, C.byref(cfg), first, num, fp(image), C.byref(stats), threads)
+        if nans == 2 ** 64 - 1:
+            raise ValueError("the light tracer needs emitters and no background")
         return image, stats, int(nans)
 
     def primary_aov(self, cfg, first=0, num=1, threads=0):
