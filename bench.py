@@ -277,8 +277,8 @@ def measure_adjoint(dev, steps, kernels=True):
         dev.reset_kernel_timing()
         adjoint(dimg)
         out["kernel_ms_adjoint"] = {k: round(dev.kernel_timing(k)[1], 3)
-                                    for k in ("extend", "trace_adjoint", "shade_adjoint", "shadow_adjoint",
-                                              "adjoint_finalize")}
+                                    for k in ("extend", "trace_adjoint", "shade_adjoint", "shade_grad",
+                                              "shadow_adjoint", "adjoint_finalize")}
         dev.set_kernel_timing(False)
     g.close()
     return out

@@ -470,7 +470,8 @@ sspt_status sspt_set_kernel_timing(sspt_ctx* ctx, int32_t enable);
  * "generate" and "extend"), "shade", "trace" (a bounce's
  * continuation rays with the previous bounce's shadow rays), "shadow" (the
  * last bounce's shadow rays), "accumulate", "path" (single-kernel tracer),
- * and "shade_adjoint", "trace_adjoint", "shadow_adjoint", "adjoint_finalize". */
+ * and "shade_adjoint", "shade_grad" (the adjoint's material derivatives),
+ * "trace_adjoint", "shadow_adjoint", "adjoint_finalize". */
 sspt_status sspt_get_kernel_timing(sspt_ctx* ctx, const char* name, uint64_t* launches,
                                    double* total_ms);
 sspt_status sspt_reset_kernel_timing(sspt_ctx* ctx);

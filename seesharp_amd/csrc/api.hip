@@ -839,10 +839,12 @@ void enqueue_wave(sspt_ctx* ctx, sspt_scene* sc, const Wave& w, const LaunchConf
         launch_bounce_reset(st, w, buf);
         timed(ctx, "extend", [&] { launch_extend_primary(st, S, w, lc); });
         for (int depth = 1; depth <= max_depth; ++depth) {
-            if (adj)
+            if (adj) {
                 timed(ctx, "shade_adjoint", [&] { launch_shade_adjoint(st, S, w, lc, buf); });
-            else
+                timed(ctx, "shade_grad", [&] { launch_shade_grad(st, S, w, lc, buf); });
+            } else {
                 timed(ctx, "shade", [&] { launch_shade(st, S, w, lc, buf); });
+            }
             if (depth < max_depth) {
                 timed(ctx, adj ? "trace_adjoint" : "trace", [&] { launch_trace(st, S, w, lc, buf ^ 1, adj); });
                 launch_bounce_reset(st, w, buf ^ 1);
@@ -860,6 +862,7 @@ void enqueue_wave(sspt_ctx* ctx, sspt_scene* sc, const Wave& w, const LaunchConf
             timed(ctx, "extend", [&] { launch_extend(st, S, w, lc, buf); });
             if (adj) {
                 timed(ctx, "shade_adjoint", [&] { launch_shade_adjoint(st, S, w, lc, buf); });
+                timed(ctx, "shade_grad", [&] { launch_shade_grad(st, S, w, lc, buf); });
                 timed(ctx, "shadow_adjoint", [&] { launch_shadow_adjoint(st, S, w, lc, buf ^ 1); });
             } else {
                 timed(ctx, "shade", [&] { launch_shade(st, S, w, lc, buf); });

@@ -39,6 +39,7 @@ struct Wave {
     int2* ad_tex;      // float offsets of the vertex's albedo / roughness texels (-1 = none)
     uint32_t ad_depth; // records allocated per slot
     float4* s_g;       // per shadow ray: gradient of its contribution w.r.t. the vertex's 4 parameters
+                       // (until k_shade_grad: the ray's local direction and queue entry)
     int2* s_t;         // per shadow ray: texel offsets of the vertex
     uint32_t* counters;            // kCntNum
     unsigned long long* stats;     // kStatStripes stripes of kStatStride (kStatNum used)
@@ -86,6 +87,8 @@ void launch_fill_waves(hipStream_t s, WaveDesc* descs, uint32_t n, uint64_t firs
                        uint64_t total, uint32_t first_iter, uint32_t whole);
 // adjoint (path replay) variants
 void launch_shade_adjoint(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int buf);
+// The adjoint's dual-number material derivatives of the bounce just shaded (before its trace).
+void launch_shade_grad(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int buf);
 void launch_shadow_adjoint(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int slot);
 // extension queue `buf` and the shadow queue in one grid (forward or adjoint shadow)
 void launch_trace(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int buf, bool adjoint);

@@ -723,19 +723,6 @@ __device__ __forceinline__ int2 vertex_texels(const DevScene& S, const Shader& s
     return t;
 }
 
-// Gradient of a contribution `val` reached through the vertex's material value
-// for direction `in` (local): g_p = sum_c delta_c * val_c * dlog f_c / dp.
-__device__ __forceinline__ float4 contribution_gradient(const DevScene& S, const Shader& sh, f2 uv, int2 tex, f3 in,
-                                                        f3 val, f3 delta) {
-    D3 f = material_value_dual(S, sh, uv, in, tex.x >= 0, tex.y >= 0);
-    float dl[12];
-    dlog_of(f, dl);
-    float g[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) g[k] = delta.x * val.x * dl[k] + delta.y * val.y * dl[4 + k] + delta.z * val.z * dl[8 + k];
-    return make_float4(g[0], g[1], g[2], g[3]);
-}
-
 // Emitter.EmittedRadiance at a point with shading normal ns, towards dir:
 // DiffuseEmitter (DiffuseEmitter.cs:33-37) is one-sided on ns; GlossyEmitter
 // (GlossyEmitter.cs:36-40) scales by cos^exponent * (exponent + 1) / (2 pi).
@@ -793,7 +780,6 @@ struct PathVertex {
     float q = 1.0f;
     bool cont = false, do_nee = false;
     bool surface_hit = false;       // the extension ray hit a surface (s, sh are valid)
-    f3 delta = {0.0f, 0.0f, 0.0f};  // ADJ: dLoss/dpixel / iterations
     int2 vtex = {-1, -1};           // ADJ: texel offsets of the vertex's textured parameters
     f3 next_dir = {0.0f, 0.0f, 0.0f};
     ReplayLog* log = nullptr;  // ReplayPixel only
@@ -889,8 +875,7 @@ struct PathVertex {
         *o = f4_of(offset_point(s, es.dir), s.err);
         *d = f4_of(es.dir, INFINITY);
         *c = f4_of(val, __int_as_float((int)shadow_tag(slot, sc, 0)));
-        if (ADJ && (vtex.x >= 0 || vtex.y >= 0))
-            *g = contribution_gradient(S, sh, s.uv, vtex, sh.f.to_local(es.dir), val, delta);
+        if (ADJ) *g = f4_of(sh.f.to_local(es.dir), 0.0f);  // the direction k_shade_grad differentiates at
         return true;
     }
 
@@ -936,8 +921,7 @@ struct PathVertex {
             log->area_d = f4_of(p1 - p0, 1.0f - 1e-5f);
             if (is_zero(val)) return false;
         }
-        if (ADJ && (vtex.x >= 0 || vtex.y >= 0))
-            *g = contribution_gradient(S, sh, s.uv, vtex, sh.f.to_local(-l2s), val, delta);
+        if (ADJ) *g = f4_of(sh.f.to_local(-l2s), 0.0f);  // the direction k_shade_grad differentiates at
         *o = f4_of(p0, 1e-5f);
         *d = f4_of(p1 - p0, 1.0f - 1e-5f);
         *c = f4_of(val, __int_as_float((int)shadow_tag(slot, sc, 1)));
@@ -989,9 +973,6 @@ k_shade(DevScene S, Wave w, int buf, int max_nsr) {
             v.L = f3_of(w.q_L[buf][i]);
             if (ADJ) {
                 // the previous vertex's next-event rays have resolved: snapshot A_{depth-1}
-                uint32_t pix = w.st_pix[slot] & kSlotMask;
-                const float* dl = S.adj_image + R.fb_off + 3ull * pix;
-                v.delta = mk3(dl[0], dl[1], dl[2]) * R.adj_scale;
                 if (v.depth >= 2) {
                     f3 A = v.L + f3_of(w.st_Le[slot]) + f3_of(w.st_La[slot]);
                     w.ad_A[(size_t)(v.depth - 2) * w.capacity + slot] = f4_of(A, 1.0f);
@@ -1042,8 +1023,8 @@ k_shade(DevScene S, Wave w, int buf, int max_nsr) {
                 w.s_o[e_idx] = eo;
                 w.s_d[e_idx] = ed;
                 w.s_c[e_idx] = ec;
-                if (ADJ) {
-                    w.s_g[e_idx] = eg;
+                if (ADJ) {  // the local direction and the queue entry, for k_shade_grad
+                    w.s_g[e_idx] = make_float4(eg.x, eg.y, eg.z, __int_as_float((int)i));
                     w.s_t[e_idx] = v.vtex;
                 }
             }
@@ -1052,27 +1033,22 @@ k_shade(DevScene S, Wave w, int buf, int max_nsr) {
                 w.s_d[a_idx] = ad;
                 w.s_c[a_idx] = ac;
                 if (ADJ) {
-                    w.s_g[a_idx] = ag;
+                    w.s_g[a_idx] = make_float4(ag.x, ag.y, ag.z, __int_as_float((int)i));
                     w.s_t[a_idx] = v.vtex;
                 }
             }
         }
 
         if (!cont_appended) cont = v.cont && v.sample_continuation(&no, &nd);
-        if (cont) {
-            if (ADJ) {
-                // d log of the sampled BSDF value (the sampling pdf is detached)
-                size_t rec = (size_t)(v.depth - 2) * w.capacity + slot;
-                w.ad_tex[rec] = v.vtex;
-                if (v.vtex.x >= 0 || v.vtex.y >= 0) {
-                    D3 f = material_value_dual(S, v.sh, v.s.uv, v.sh.f.to_local(v.next_dir), v.vtex.x >= 0,
-                                               v.vtex.y >= 0);
-                    float dl[12];
-                    dlog_of(f, dl);
-                    w.ad_dlog[rec * 3 + 0] = make_float4(dl[0], dl[1], dl[2], dl[3]);
-                    w.ad_dlog[rec * 3 + 1] = make_float4(dl[4], dl[5], dl[6], dl[7]);
-                    w.ad_dlog[rec * 3 + 2] = make_float4(dl[8], dl[9], dl[10], dl[11]);
-                }
+        if (ADJ && active) {
+            // the vertex's record (index depth - 1): its texels and, for k_shade_grad, the
+            // sampled direction whose d log f (the sampling pdf detached) it computes;
+            // (-1, -1) where the path ends here
+            const int vd = cont ? v.depth - 1 : v.depth;
+            if (vd >= 1) {
+                const size_t rec = (size_t)(vd - 1) * w.capacity + slot;
+                w.ad_tex[rec] = cont ? v.vtex : make_int2(-1, -1);
+                if (cont && (v.vtex.x >= 0 || v.vtex.y >= 0)) w.ad_dlog[rec * 3] = f4_of(v.next_dir, 0.0f);
             }
         }
         if (!cont_appended) n_idx = block_append_word<false, false>(queue_word(w, nb), 0, cont, false, false, app).x;
@@ -1088,6 +1064,66 @@ k_shade(DevScene S, Wave w, int buf, int max_nsr) {
             // the path ends here: its radiance from hits and its last depth, by slot
             w.st_L[slot] = f4_of(v.L, 0.0f);
             w.st_prev[slot] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(v.depth));
+        }
+    }
+}
+
+// The adjoint's material derivatives of one bounce, apart from the adjoint
+// shade so that their dual-number evaluations do not crowd its registers. Jobs
+// [0, queue length): the continuation of each textured vertex, d log f of its
+// sampled direction into the vertex's record; the rest: each next-event ray of
+// a textured vertex, the gradient of its contribution (val * delta * d log f,
+// summed over the colour channels) into s_g, which held the ray's local
+// direction and queue entry. The vertex's shading context is rebuilt from the
+// queue's hit and direction, so this runs after the bounce's shade and before
+// the next trace overwrites the hits.
+__global__ void __launch_bounds__(256) k_shade_grad(DevScene S, Wave w, int buf) {
+    const uint32_t qn = queue_count(w, buf), sn = shadow_count(w, buf ^ 1);
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < qn + sn; j += gridDim.x * blockDim.x) {
+        const bool shadow = j >= qn;
+        uint32_t i, r = 0;
+        size_t rec = 0;
+        int2 tex;
+        float4 aux;
+        if (!shadow) {
+            i = j;
+            const uint32_t slot = w.q_slot[buf][i] & kSlotMask;
+            const int depth = __float_as_int(w.q_prev[buf][i].w);
+            rec = (size_t)(depth - 1) * w.capacity + slot;
+            tex = w.ad_tex[rec];
+            if (tex.x < 0 && tex.y < 0) continue;
+            aux = w.ad_dlog[rec * 3];
+        } else {
+            r = j - qn;
+            tex = w.s_t[r];
+            if (tex.x < 0 && tex.y < 0) continue;
+            aux = w.s_g[r];
+            i = (uint32_t)__float_as_int(aux.w);
+        }
+        const float4 h = w.hits[i];
+        const Surf s = surface(S, __float_as_int(h.w), h.y, h.z);
+        Shader sh;
+        make_shader(S, s, -f3_of(w.q_d[buf][i]), &sh);
+        const f3 in = shadow ? f3_of(aux) : sh.f.to_local(f3_of(aux));
+        const D3 f = material_value_dual(S, sh, s.uv, in, tex.x >= 0, tex.y >= 0);
+        float dl[12];
+        dlog_of(f, dl);
+        if (!shadow) {
+            w.ad_dlog[rec * 3 + 0] = make_float4(dl[0], dl[1], dl[2], dl[3]);
+            w.ad_dlog[rec * 3 + 1] = make_float4(dl[4], dl[5], dl[6], dl[7]);
+            w.ad_dlog[rec * 3 + 2] = make_float4(dl[8], dl[9], dl[10], dl[11]);
+        } else {
+            const float4 c = w.s_c[r];
+            const uint32_t tag = (uint32_t)__float_as_int(c.w);
+            const SceneRec& R = S.recs[tag >> (kSlotBits + 1)];
+            const uint32_t slot = (tag >> 1) & kSlotMask;
+            const float* dp = S.adj_image + R.fb_off + 3ull * (w.st_pix[slot] & kSlotMask);
+            const f3 delta = mk3(dp[0], dp[1], dp[2]) * R.adj_scale;
+            float g[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                g[k] = delta.x * c.x * dl[k] + delta.y * c.y * dl[4 + k] + delta.z * c.z * dl[8 + k];
+            w.s_g[r] = make_float4(g[0], g[1], g[2], g[3]);
         }
     }
 }
@@ -1881,6 +1917,10 @@ void launch_shade_adjoint(hipStream_t s, const DevScene& sc, const Wave& w, cons
     // the adjoint shade keeps its one block size (its register budget is tuned for it)
     hipLaunchKernelGGL((k_shade<true, kShadeBlock>), dim3(grid_of(lc.shade_blocks, lc.max_paths, kShadeBlock)),
                        dim3(kShadeBlock), 0, s, sc, w, buf, lc.max_nsr);
+}
+void launch_shade_grad(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int buf) {
+    hipLaunchKernelGGL(k_shade_grad, dim3(grid_of(lc.shade_blocks, (uint64_t)lc.max_paths + lc.max_shadow, 256)),
+                       dim3(256), 0, s, sc, w, buf);
 }
 void launch_trace(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int buf, bool adjoint) {
     with_trace_block(lc.trace_block, [&](auto tb) {
