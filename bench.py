@@ -244,7 +244,11 @@ def measure_adjoint(dev, steps, kernels=True):
         return g.render_adjoint(cfg, dimg, spp, spp)
 
     forward()
-    dimg = (2.0 * (g.read_image() - target) / target.size).astype(np.float32)
+    # dLoss/dImage of an L2 loss, resident on the GPU as in an optimisation loop (the library
+    # copies it device to device)
+    import torch
+
+    dimg = torch.from_numpy((2.0 * (g.read_image() - target) / target.size).astype(np.float32)).to("cuda:0")
     adjoint(dimg)
     fwd_ms, adj_ms = [], []
     for _ in range(max(1, steps)):
@@ -334,7 +338,11 @@ def measure_views(dev, steps):
     views.render(cfgs)
     imgs = [views.read_image(i) for i in range(nviews)]
     npx = res * res * 3
-    dl = [(2.0 * (imgs[i] - targets[i]) / (npx * nviews)).astype(np.float32) for i in range(nviews)]
+    # dLoss/dImage of every view, resident on the GPU as in an optimisation loop
+    import torch
+
+    dl = torch.from_numpy(np.concatenate([(2.0 * (imgs[i] - targets[i]) / (npx * nviews)).astype(np.float32).ravel()
+                                          for i in range(nviews)])).to("cuda:0")
     adjoint(dl)
     fwd_ms, adj_ms = [], []
     for _ in range(max(1, steps)):
@@ -530,6 +538,10 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="batching: keep the default block shape, wave capacity and grids instead of tuning them")
     args = ap.parse_args()
+    # torch's HIP runtime must initialise before libsspt.so brings in the system one: the
+    # adjoint measurements hand the library device tensors
+    import torch  # noqa: F401
+
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
               "(this build renders on a single GPU; sharding by iteration is a design, not code).", file=sys.stderr)

@@ -351,18 +351,21 @@ sspt_status sspt_render_noalloc(sspt_ctx* ctx, sspt_scene* scene, const sspt_pt_
  *      first_iteration .. first_iteration+num_iterations-1, replays those paths
  *      from their seeds and adds dLoss/dtexel of every textured base colour and
  *      roughness to the scene's gradient buffers (BSDF values differentiated;
- *      sampling pdfs, MIS weights and Russian roulette held constant). --- */
+ *      sampling pdfs, MIS weights and Russian roulette held constant).
+ *      dLoss/dImage may be host memory or device memory of the context's
+ *      device (unified addressing: a loss computed on the GPU is passed by
+ *      its device pointer and copied device to device). --- */
 sspt_status sspt_render_adjoint(sspt_ctx* ctx, sspt_scene* scene, const sspt_pt_settings* settings,
-                                const float* host_dloss_dimage, uint32_t first_iteration,
+                                const float* dloss_dimage, uint32_t first_iteration,
                                 uint32_t num_iterations, sspt_ray_stats* stats);
 /* The adjoint of sspt_render_batch for every scene / view of a world at once:
- * host_dloss_dimages holds each scene's W*H*3 dLoss/dImage one after the other
+ * dloss_dimages (host or device memory) holds each scene's W*H*3 dLoss/dImage one after the other
  * (scene order); scene i replays iterations first_iteration ..
  * first_iteration + total_spp_i - 1. Gradients add into the world's texture
  * gradients (shared by all views of a view set). No counterpart in the
  * reference. */
 sspt_status sspt_render_adjoint_batch(sspt_ctx* ctx, sspt_scene* scene, const sspt_pt_settings* settings,
-                                      const float* host_dloss_dimages, uint32_t first_iteration,
+                                      const float* dloss_dimages, uint32_t first_iteration,
                                       sspt_ray_stats* stats);
 /* Zeroes every texture gradient of the scene. */
 sspt_status sspt_gradient_clear(sspt_ctx* ctx, sspt_scene* scene);

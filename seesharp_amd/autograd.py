@@ -53,7 +53,10 @@ class _RenderFunction(torch.autograd.Function):
         for idx, tex in zip(ctx.indices, textures):
             gs.write_texture(idx, _to_host(tex))
         gs.clear_gradients()
-        gs.render_adjoint(ctx.settings, _to_host(grad_image), ctx.backward_first_iteration, ctx.num_iterations)
+        g = grad_image.detach()
+        # a gradient on the GPU goes to the library by its device pointer, without a host round trip
+        g = g.to(torch.float32).contiguous() if g.is_cuda else _to_host(g)
+        gs.render_adjoint(ctx.settings, g, ctx.backward_first_iteration, ctx.num_iterations)
         grads = []
         for idx, tex, need in zip(ctx.indices, textures, ctx.needs_input_grad[6:]):
             grads.append(torch.from_numpy(gs.gradient(idx)).to(tex.device) if need else None)

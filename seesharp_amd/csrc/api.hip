@@ -1488,17 +1488,18 @@ sspt_status sspt_render_batch(sspt_ctx* ctx, sspt_scene* sc, const sspt_pt_setti
 }
 
 sspt_status sspt_render_adjoint_batch(sspt_ctx* ctx, sspt_scene* sc, const sspt_pt_settings* cfgs,
-                                      const float* host_dloss_dimages, uint32_t first_iteration,
+                                      const float* dloss_dimages, uint32_t first_iteration,
                                       sspt_ray_stats* stats) {
-    if (!ctx || !sc || !cfgs || !host_dloss_dimages) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "null argument");
+    if (!ctx || !sc || !cfgs || !dloss_dimages) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "null argument");
     if (ctx->exec_mode != SSPT_EXEC_WAVEFRONT)
         return fail(ctx, SSPT_ERR_UNSUPPORTED, "adjoint renders run on the wavefront tracer only");
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     BatchPlan plan;
     TRY(plan_batch(ctx, sc, cfgs, first_iteration, true, &plan));
     HIP_TRY(ctx, hipMemsetAsync(ctx->wave.stats, 0, kStatBytes, ctx->stream));
-    HIP_TRY(ctx, hipMemcpyAsync(sc->adj_image, host_dloss_dimages, sc->image_floats * sizeof(float),
-                                hipMemcpyHostToDevice, ctx->stream));
+    // host or device memory (unified addressing)
+    HIP_TRY(ctx, hipMemcpyAsync(sc->adj_image, dloss_dimages, sc->image_floats * sizeof(float),
+                                hipMemcpyDefault, ctx->stream));
     if (!plan.waves.empty())
         TRY(run_waves(ctx, sc, plan.waves, plan.max_depth, plan.max_nsr, Mode::Adjoint, nullptr, 0, nullptr, -1));
     TRY(read_stats(ctx, sc, stats));
@@ -1509,9 +1510,9 @@ sspt_status sspt_render_adjoint_batch(sspt_ctx* ctx, sspt_scene* sc, const sspt_
 }
 
 sspt_status sspt_render_adjoint(sspt_ctx* ctx, sspt_scene* sc, const sspt_pt_settings* cfg,
-                                const float* host_dloss_dimage, uint32_t first_iteration, uint32_t num_iterations,
+                                const float* dloss_dimage, uint32_t first_iteration, uint32_t num_iterations,
                                 sspt_ray_stats* stats) {
-    if (!ctx || !sc || !cfg || !host_dloss_dimage) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "null argument");
+    if (!ctx || !sc || !cfg || !dloss_dimage) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "null argument");
     if (!valid_settings(*cfg)) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "invalid settings");
     if (sc->recs.size() != 1) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "sspt_render_adjoint renders a single scene");
     if (ctx->exec_mode != SSPT_EXEC_WAVEFRONT)
@@ -1531,7 +1532,8 @@ sspt_status sspt_render_adjoint(sspt_ctx* ctx, sspt_scene* sc, const sspt_pt_set
         done += it;
     }
     HIP_TRY(ctx, hipMemsetAsync(ctx->wave.stats, 0, kStatBytes, ctx->stream));
-    HIP_TRY(ctx, hipMemcpyAsync(sc->adj_image + R.fb_off, host_dloss_dimage, (size_t)np * 12, hipMemcpyHostToDevice,
+    // host or device memory (unified addressing)
+    HIP_TRY(ctx, hipMemcpyAsync(sc->adj_image + R.fb_off, dloss_dimage, (size_t)np * 12, hipMemcpyDefault,
                                 ctx->stream));
     if (!waves.empty()) {
         TRY(ensure_wave_descs(ctx, waves.size()));

@@ -94,6 +94,23 @@ class Device:
         return int(n.value), float(ms.value)
 
 
+def _image_pointer(image, size: int):
+    """(float pointer, owner) of a float32 image: a torch tensor on the GPU by its device
+    pointer (the library copies device to device), anything else as a host array."""
+    if hasattr(image, "data_ptr") and getattr(image, "is_cuda", False):
+        import torch
+
+        if image.dtype != torch.float32 or not image.is_contiguous() or image.numel() != size:
+            raise ValueError(f"device dLoss/dImage must be a contiguous float32 tensor of {size} values")
+        # the library reads it on its own stream: torch's work on it must be done
+        torch.cuda.current_stream(image.device).synchronize()
+        return C.cast(C.c_void_p(image.data_ptr()), C.POINTER(C.c_float)), image
+    arr = np.ascontiguousarray(image, np.float32).reshape(-1)
+    if arr.size != size:
+        raise ValueError(f"dLoss/dImage holds {arr.size} values, the framebuffer {size}")
+    return abi.fptr(arr), arr
+
+
 class _TextureParameters:
     """Texture upload and gradient read-back of a device world (the
     differentiable parameters). Textures are numbered in world order: the
@@ -230,14 +247,17 @@ class GpuScene(_TextureParameters):
     def _texture_owners(self):
         return [self.scene]
 
-    def render_adjoint(self, settings: abi.PtSettings, dloss_dimage: np.ndarray, first_iteration: int,
+    def render_adjoint(self, settings: abi.PtSettings, dloss_dimage, first_iteration: int,
                        num_iterations: int) -> dict:
-        """Adds dLoss/dtexel for the image of iterations first..first+num-1 to the gradients."""
-        g = np.ascontiguousarray(dloss_dimage, np.float32).reshape(self.height, self.width, 3)
+        """Adds dLoss/dtexel for the image of iterations first..first+num-1 to the gradients.
+        dloss_dimage: an (H, W, 3) array, or a contiguous float32 torch tensor on the GPU
+        (passed by its device pointer)."""
+        ptr, keep = _image_pointer(dloss_dimage, self.height * self.width * 3)
         stats = abi.RayStats()
         self.device.check(self.device.lib.sspt_render_adjoint(self.device.ptr, self.ptr, C.byref(settings),
-                                                              abi.fptr(g), first_iteration, num_iterations,
+                                                              ptr, first_iteration, num_iterations,
                                                               C.byref(stats)))
+        del keep
         return stats.as_dict()
 
     def trace_closest(self, rays: np.ndarray) -> np.ndarray:
@@ -352,16 +372,20 @@ class GpuBatch(_TextureParameters):
 
     def render_adjoint(self, settings, dloss_dimages, first_iteration: int = 0) -> dict:
         """Adds dLoss/dtexel of every scene / view to the world's texture gradients.
-        dloss_dimages: one (H, W, 3) array per scene / view; scene i replays
+        dloss_dimages: one (H, W, 3) array per scene / view (or all of them concatenated in
+        one contiguous float32 torch tensor on the GPU); scene i replays
         iterations first_iteration .. first_iteration + settings[i].total_spp - 1."""
         arr = (abi.PtSettings * len(self))(*settings)
-        flat = np.concatenate([np.ascontiguousarray(g, np.float32).reshape(-1) for g in dloss_dimages])
         expect = sum(h * w * 3 for h, w in self.sizes)
-        if flat.size != expect:
-            raise ValueError(f"dLoss/dImage holds {flat.size} values, the framebuffers {expect}")
+        if hasattr(dloss_dimages, "data_ptr") and getattr(dloss_dimages, "is_cuda", False):
+            ptr, keep = _image_pointer(dloss_dimages, expect)  # all scenes' images, concatenated, on the GPU
+        else:
+            flat = np.concatenate([np.ascontiguousarray(g, np.float32).reshape(-1) for g in dloss_dimages])
+            ptr, keep = _image_pointer(flat, expect)
         stats = abi.RayStats()
-        self.device.check(self.device.lib.sspt_render_adjoint_batch(self.device.ptr, self.ptr, arr, abi.fptr(flat),
+        self.device.check(self.device.lib.sspt_render_adjoint_batch(self.device.ptr, self.ptr, arr, ptr,
                                                                     first_iteration, C.byref(stats)))
+        del keep
         return stats.as_dict()
 
     def read_layer(self, index: int, name: str) -> np.ndarray:

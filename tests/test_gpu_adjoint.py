@@ -266,3 +266,22 @@ def test_albedo_optimisation_recovers_target_albedo(device):
     loss1 = clean_loss(final)
     assert float(np.median(np.abs(final - 0.7))) < 0.15, final
     assert loss1 < 0.5 * loss0, (loss0, loss1)
+
+
+def test_device_dloss_matches_host_dloss(device):
+    """dLoss/dImage passed by its device pointer (a torch tensor on the GPU, copied
+    device to device) gives the gradients of the same values passed from host memory
+    (up to the order of the fp32 gradient atomics)."""
+    s = textured_box(diffuse=False, res=64, tex_res=16)
+    g = GpuScene(device, s)
+    cfg = ol.settings(total_spp=4, max_depth=5)
+    rng = np.random.default_rng(3)
+    dimg = rng.normal(0.0, 1.0, (64, 64, 3)).astype(np.float32)
+    grads = []
+    for src in (dimg, torch.from_numpy(dimg).to("cuda:0")):
+        g.clear_gradients()
+        g.render_adjoint(cfg, src, 0, 4)
+        grads.append([g.gradient(i) for i in range(len(g.texture_shapes()))])
+    for a, b in zip(*grads):
+        assert np.abs(a).max() > 0
+        np.testing.assert_allclose(b, a, rtol=1e-4, atol=1e-6 * np.abs(a).max())
