@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define SSPT_ABI_VERSION 6
+#define SSPT_ABI_VERSION 7
 
 typedef enum sspt_status {
     SSPT_OK = 0,
@@ -465,6 +465,45 @@ sspt_status sspt_render_light_tracer(sspt_ctx* ctx, sspt_scene* scene, const ssp
                                      uint32_t first_iteration, uint32_t num_iterations,
                                      sspt_ray_stats* stats);
 
+/* ---- bidirectional path tracing: ClassicBidir (SeeSharp/Integrators/Bidir/ClassicBidir.cs)
+ *      over BidirBase.Render (BidirBase.cs:84-146). --------------------------
+ * Each iteration first traces one light path per pixel (seeded
+ * RNG(base_seed_light, path, iteration), as sspt_render_light_tracer) and keeps
+ * its vertices (PathCache, LightPathWalk.OnHit / OnContinue,
+ * BidirBase.Light.cs:324-410); if enable_light_tracer, every vertex is
+ * connected to the camera (ConnectLightVertexToCamera, BidirBase.Light.cs:35-92).
+ * Then one camera path per pixel (RNG(base_seed_camera, row*W+col, iteration),
+ * BidirBase.RenderPixel, BidirBase.Camera.cs:97-106, a RandomWalk of
+ * max_depth + 1 vertices) adds at every vertex the emitter it hit
+ * (OnEmitterHit, :500-524), if enable_connections the connections to every
+ * vertex of the light path of its pixel (BidirConnections / Connect,
+ * :192-307), and num_shadow_rays next-event samples (PerformNextEventEstimation,
+ * :368-484), all weighted by the balance heuristic over the enabled techniques
+ * (ClassicBidir.EmitterHitMis / LightTracerMis / BidirConnectMis /
+ * NextEventMis / CameraPathReciprocals / LightPathReciprocals,
+ * ClassicBidir.cs:94-198, with the pdfs of BidirPathPdfs.cs). Splats of the light
+ * tracer and the camera paths' pixel values go into the framebuffer with
+ * running-mean semantics (as sspt_render_light_tracer: scale by (n-1)/n, then
+ * add every finite value / n, the light tracer's in an unspecified order);
+ * non-finite values are counted. The light-path cache holds every vertex (the
+ * reference's PathCache of min(MaxDepth + 1, 10) vertices per path drops whole
+ * paths only on overflow, which cannot happen for MaxDepth <= 9). Scenes with
+ * a background or without emitters, and batches, are SSPT_ERR_UNSUPPORTED;
+ * num_light_paths other than -1 or the pixel count is SSPT_ERR_INVALID
+ * (ClassicBidir.cs:72-75). */
+typedef struct sspt_bidir_settings {
+    uint32_t base_seed_camera;    /* BaseSeedCamera, default 0xC030114 */
+    uint32_t base_seed_light;     /* BaseSeedLight, default 0x13C0FEFE */
+    int32_t max_depth;            /* MaxDepth (Integrator default 100) */
+    int32_t min_depth;            /* MinDepth (default 1) */
+    int32_t num_shadow_rays;      /* NumShadowRays (default 1) */
+    int32_t enable_connections;   /* EnableConnections (default 1) */
+    int32_t enable_light_tracer;  /* EnableLightTracer (default 1) */
+    int32_t num_light_paths;      /* NumLightPaths: -1 (default) or width * height */
+} sspt_bidir_settings;
+sspt_status sspt_render_bidir(sspt_ctx* ctx, sspt_scene* scene, const sspt_bidir_settings* settings,
+                              uint32_t first_iteration, uint32_t num_iterations, sspt_ray_stats* stats);
+
 /* ---- measurement ------------------------------------------------------- */
 /* Enables HIP-event timing of every kernel launch on the context stream. */
 sspt_status sspt_set_kernel_timing(sspt_ctx* ctx, int32_t enable);
@@ -474,7 +513,8 @@ sspt_status sspt_set_kernel_timing(sspt_ctx* ctx, int32_t enable);
  * continuation rays with the previous bounce's shadow rays), "shadow" (the
  * last bounce's shadow rays), "accumulate", "path" (single-kernel tracer),
  * and "shade_adjoint", "shade_grad" (the adjoint's material derivatives),
- * "trace_adjoint", "shadow_adjoint", "adjoint_finalize". */
+ * "trace_adjoint", "shadow_adjoint", "adjoint_finalize"; "light_paths" (the
+ * light tracer), "bidir_light" and "bidir_camera" (bidirectional rendering). */
 sspt_status sspt_get_kernel_timing(sspt_ctx* ctx, const char* name, uint64_t* launches,
                                    double* total_ms);
 sspt_status sspt_reset_kernel_timing(sspt_ctx* ctx);

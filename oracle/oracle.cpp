@@ -358,13 +358,24 @@ struct Camera {
         return true;
     }
     // PerspectiveCamera.GenerateRay (PerspectiveCamera.cs:64-102), lensRadius = 0.
-    void generate_ray(float fx, float fy, V3* org, V3* dir) const {
+    // `raw` (optional) receives the unnormalised world direction.
+    void generate_ray(float fx, float fy, V3* org, V3* dir, V3* raw = nullptr) const {
         V3 view = v3(2.0f * fx / (float)width - 1.0f, 1.0f - 2.0f * fy / (float)height, 0.0f);
         V3 local = transform_point(view, view_to_camera);
         V3 d = transform_dir(local, camera_to_world);
         *org = position;
         *dir = normalize(d);
+        if (raw) *raw = d;
     }
+    // PerspectiveCamera.SolidAngleToPixelJacobian (PerspectiveCamera.cs:177-191)
+    float pixel_jacobian(V3 pos) const {
+        V3 local = transform_point(pos, world_to_camera);
+        float cosine = local.z / length(local);
+        float d = image_plane_distance / cosine;
+        return d * d / std::fabs(cosine);
+    }
+    // Camera.Direction (Camera.cs:16-21)
+    V3 direction() const { return transform_dir(v3(0, 0, -1), camera_to_world); }
 };
 
 // ----------------------------------------------------------- shading space -
@@ -1049,8 +1060,10 @@ GenericLocal generic_local(const Shader& sh) {
     return r;
 }
 
-// GenericMaterial.ComputeValueAndPdf (GenericMaterial.cs:152-338), forward pdf only.
-void generic_value_pdf(const Shader& sh, V3 in, const GenericLocal& lp, Rgb* value, float* pdf_out) {
+// GenericMaterial.ComputeValueAndPdf (GenericMaterial.cs:152-338): the value, the
+// forward pdf and, if `pdf_rev_out` is given, the reverse pdf.
+void generic_value_pdf(const Shader& sh, V3 in, const GenericLocal& lp, Rgb* value, float* pdf_out,
+                       float* pdf_rev_out = nullptr) {
     const Material& m = *sh.m;
     Ggx ggx{lp.ax, lp.ay};
     V3 out = sh.out_local;
@@ -1058,6 +1071,7 @@ void generic_value_pdf(const Shader& sh, V3 in, const GenericLocal& lp, Rgb* val
     bool same_geo = should_reflect(sh.p, sh.out_world, sh.f.to_world(in));
     Rgb val = rgb(0.0f);
     float pdf0 = 0.0f, pdf1 = 0.0f, pdf2 = 0.0f;
+    float rev0 = 0.0f, rev1 = 0.0f, rev2 = 0.0f;  // pdfsRev
 
     float f_out = schlick_weight(cos_o), f_in = schlick_weight(cos_i);
     if (same_hemisphere(out, in)) {
@@ -1066,6 +1080,7 @@ void generic_value_pdf(const Shader& sh, V3 in, const GenericLocal& lp, Rgb* val
             val = val + diffuse;
         }
         pdf0 = cos_i / kPi;
+        rev0 = cos_o / kPi;
     }
 
     V3 h = out + in;
@@ -1088,7 +1103,10 @@ void generic_value_pdf(const Shader& sh, V3 in, const GenericLocal& lp, Rgb* val
                 val = val + refl;
             }
             float jf = std::fabs(4.0f * c_out), jr = std::fabs(4.0f * c_in);
-            if (jf != 0.0f && jr != 0.0f) pdf1 = ggx.pdf(out, h) / jf;
+            if (jf != 0.0f && jr != 0.0f) {
+                pdf1 = ggx.pdf(out, h) / jf;
+                rev1 = ggx.pdf(in, h) / jr;
+            }
         }
     }
 
@@ -1115,6 +1133,18 @@ void generic_value_pdf(const Shader& sh, V3 in, const GenericLocal& lp, Rgb* val
             float jac = eta * eta * std::max(0.0f, dot(in, -wh)) / (sqrt_denom * sqrt_denom);
             pdf2 += ggx.pdf(out, wh) * jac;
         }
+        // the reverse transmission pdf, from the half vector of the swapped pair (:277-295)
+        float eta_in = in.z > 0.0f ? m.ior : (1.0f / m.ior);
+        V3 hin = normalize(out * eta_in + in);
+        hin = hin.z < 0.0f ? -hin : hin;
+        if (hin != v3(0, 0, 0)) {
+            V3 wh = !same_hemisphere(in, hin) ? -hin : hin;
+            float sqrt_denom_in = dot(hin, in) + eta_in * dot(out, hin);
+            if (sqrt_denom_in != 0.0f) {
+                float jac = eta_in * eta_in * std::max(0.0f, dot(out, -wh)) / (sqrt_denom_in * sqrt_denom_in);
+                rev2 += ggx.pdf(in, wh) * jac;
+            }
+        }
     }
 
     float schl = average(schlick_fresnel(lp.spec_r0, std::fabs(out.z)));
@@ -1126,12 +1156,15 @@ void generic_value_pdf(const Shader& sh, V3 in, const GenericLocal& lp, Rgb* val
     float sel_trans = (1.0f - dw) * (1.0f - ft) * m.spec_trans;
     *value = val;
     *pdf_out = pdf0 * dw + pdf1 * sel_refl + pdf2 * sel_trans;
+    // selectTransmitIn (:311) repeats selectTransmit: both use fresnelT of the out direction
+    if (pdf_rev_out) *pdf_rev_out = rev0 * dw + rev1 * sel_refl + rev2 * sel_trans;
 }
 
 struct BsdfSample {
     V3 dir;
     float pdf;
     Rgb weight;
+    float pdf_rev = 0.0f;  // BsdfSample.PdfReverse
 };
 
 // GenericMaterial.Sample (GenericMaterial.cs:92-150)
@@ -1164,10 +1197,11 @@ BsdfSample generic_sample(const Shader& sh, float pc, V2 pd) {
         }
     }
     Rgb val;
-    float pdf;
-    generic_value_pdf(sh, in, lp, &val, &pdf);
+    float pdf, pdf_rev;
+    generic_value_pdf(sh, in, lp, &val, &pdf, &pdf_rev);
     BsdfSample bs;
     bs.pdf = pdf;
+    bs.pdf_rev = pdf_rev;
     bs.weight = val * std::fabs(in.z) / pdf;
     bs.dir = sh.f.to_world(in);
     return bs;
@@ -1187,14 +1221,21 @@ Rgb diffuse_eval(const Shader& sh, V3 in_world) {
     }
     return rgb(0.0f);
 }
-// DiffuseMaterial.Pdf (DiffuseMaterial.cs:167-188), forward pdf.
-float diffuse_pdf(const Shader& sh, V3 in_world) {
+// DiffuseMaterial.Pdf (DiffuseMaterial.cs:167-188) with DiffuseBsdf.Pdf /
+// DiffuseTransmission.Pdf (:63-71,95-103): the forward pdf and, if `rev` is
+// given, the reverse pdf.
+float diffuse_pdf(const Shader& sh, V3 in_world, float* rev = nullptr) {
     V3 in = sh.f.to_local(in_world);
-    float refl = same_hemisphere(sh.out_local, in) ? abs_cos_theta(in) / kPi : 0.0f;
+    bool same = same_hemisphere(sh.out_local, in);
+    float refl = same ? abs_cos_theta(in) / kPi : 0.0f;
+    float refl_rev = same ? abs_cos_theta(sh.out_local) / kPi : 0.0f;
     if (sh.m->transmitter) {
-        float trans = !same_hemisphere(sh.out_local, in) ? abs_cos_theta(in) / kPi : 0.0f;
+        float trans = !same ? abs_cos_theta(in) / kPi : 0.0f;
+        float trans_rev = !same ? abs_cos_theta(sh.out_local) / kPi : 0.0f;
+        if (rev) *rev = (refl_rev + trans_rev) * 0.5f;
         return (refl + trans) * 0.5f;
     }
+    if (rev) *rev = refl_rev;
     return refl;
 }
 
@@ -1244,9 +1285,21 @@ BsdfSample sample_bsdf(const Shader& sh, float pc, V2 pd) {
     // ShadingToWorld(context.Normal, sample) recomputes the same basis.
     V3 dir = sh.f.to_world(d);
     Rgb value = eval_cos(sh, dir);
-    float pdf = diffuse_pdf(sh, dir);
+    float rev;
+    float pdf = diffuse_pdf(sh, dir, &rev);
     if (pdf == 0.0f) return {v3(0, 0, 0), 0.0f, rgb(0.0f)};
-    return {dir, pdf, value / pdf};
+    return {dir, pdf, value / pdf, rev};
+}
+
+// SurfaceShader.Pdf(inDir) (SurfaceShader.cs:64-67): forward and reverse solid-angle pdfs.
+void pdf_pair(const Shader& sh, V3 in_world, float* fwd, float* rev) {
+    if (sh.m->kind == SSPT_MATERIAL_GENERIC) {
+        V3 in = sh.f.to_local(in_world);
+        Rgb v;
+        generic_value_pdf(sh, in, generic_local(sh), &v, fwd, rev);
+        return;
+    }
+    *fwd = diffuse_pdf(sh, in_world, rev);
 }
 
 // ----------------------------------------------------------- path tracer ---
@@ -1676,6 +1729,15 @@ void or_material_eval(const sspt_material_desc* mat, const sspt_texture_desc* te
     delete s;
 }
 
+void or_material_pdf_pair(const sspt_material_desc* mat, const sspt_texture_desc* textures, const float* normal,
+                          const float* uv, const float* out_dir, const float* in_dir, float* pdf, float* pdf_rev) {
+    SurfacePoint p;
+    or_scene* s = flat_surface(mat, textures, normal, uv, &p);
+    Shader sh = make_shader(*s, p, v3(out_dir[0], out_dir[1], out_dir[2]));
+    pdf_pair(sh, v3(in_dir[0], in_dir[1], in_dir[2]), pdf, pdf_rev);
+    delete s;
+}
+
 void or_material_eval_many(const sspt_material_desc* mat, const sspt_texture_desc* textures,
                            const float* normal, const float* uv, const float* out_dir, int64_t n,
                            const float* in_dirs, float* value_cos, float* pdf) {
@@ -1701,6 +1763,7 @@ void or_material_sample(const sspt_material_desc* mat, const sspt_texture_desc* 
     out[0] = bs.dir.x, out[1] = bs.dir.y, out[2] = bs.dir.z;
     out[3] = bs.pdf;
     out[4] = bs.weight.x, out[5] = bs.weight.y, out[6] = bs.weight.z;
+    out[7] = bs.pdf_rev;
     delete s;
 }
 
@@ -2151,6 +2214,481 @@ uint64_t or_render_light_tracer(const or_scene* s, const sspt_lt_settings* cfg, 
                 image[sp.pix * 3 + 1] += add.y;
                 image[sp.pix * 3 + 2] += add.z;
             }
+    }
+    if (stats) {
+        stats->num_rays = rays;
+        stats->num_shadow_rays = shadow;
+        stats->num_occluded = occl;
+        stats->num_ray_hits = hits;
+        stats->num_shadow_rays_traced = shadow;
+        stats->num_iterations = num;
+    }
+    return nans;
+}
+
+// --------------------------------------------- bidirectional path tracing -
+namespace {
+
+// Emitter.PdfRay: DiffuseEmitter.cs:39-42, GlossyEmitter.cs:42-45 with
+// SampleWarp.ToCosineLobeJacobian (SampleWarp.cs:192-194).
+float emitter_pdf_ray(const Emitter& e, const SurfacePoint& p, V3 dir) {
+    float cosine = dot(p.shading_normal, dir) / length(dir);
+    if (e.kind == SSPT_EMITTER_GLOSSY) {
+        float power = e.exponent + 1.0f;
+        float jac = cosine > 0.0f ? ((power + 1.0f) * std::pow(cosine, power) * 1.0f / (2.0f * kPi)) : 0.0f;
+        return e.inv_area * jac;
+    }
+    return e.inv_area * std::max(cosine, 0.0f) / kPi;
+}
+
+// One stored light-path vertex: the fields of PathVertex (PathVertex.cs:6-66)
+// that ClassicBidir reads. The vertex at index k of a path has Depth k.
+struct CachedVertex {
+    SurfacePoint p;
+    float pdf_from = 0.0f;  // PdfFromAncestor
+    float pdf_rev = 0.0f;   // PdfReverseAncestor
+    float pdf_ne = 0.0f;    // PdfNextEventAncestor
+    Rgb weight = rgb(0.0f);
+    int depth = 0;
+};
+
+// CameraPath.Vertices (BidirBase.Camera.cs:7-18)
+struct PdfPair {
+    float from = 0.0f, to = 0.0f;
+};
+
+// BidirPathPdfs (BidirPathPdfs.cs:9-114): per-vertex area pdfs of both sampling
+// directions, reused across the calls of one thread.
+struct Pdfs {
+    std::vector<float> c2l, l2c;  // PdfsCameraToLight, PdfsLightToCamera
+    float ne = 0.0f;              // PdfNextEvent
+    void reset(int n) {
+        c2l.assign((size_t)n, 0.0f);
+        l2c.assign((size_t)n, 0.0f);
+        ne = 0.0f;
+    }
+    int n() const { return (int)c2l.size(); }
+    // GatherCameraPdfs (:46-53)
+    void gather_camera(const std::vector<PdfPair>& v, int last) {
+        for (int i = 0; i < last; ++i) {
+            c2l[(size_t)i] = v[(size_t)i].from;
+            if (i < last - 1) l2c[(size_t)i] = v[(size_t)i + 1].to;
+        }
+    }
+    // GatherLightPdfs (:82-92): from `vertex` back along its path
+    void gather_light(const std::vector<CachedVertex>& path, const CachedVertex& vertex, int last) {
+        const CachedVertex* next = &vertex;
+        for (int i = last + 1; i < n() - 2; ++i) {
+            l2c[(size_t)i] = next->pdf_from;
+            c2l[(size_t)i + 2] = next->pdf_rev;
+            ne += next->pdf_ne;
+            next = &path[(size_t)next->depth - 1];
+        }
+        l2c[(size_t)n() - 2] = next->pdf_from;
+        l2c[(size_t)n() - 1] = 1.0f;
+    }
+};
+
+struct BidirCtx {
+    const or_scene* s;
+    const sspt_bidir_settings* cfg;
+    float num_paths;  // NumLightPaths
+    const std::vector<std::vector<CachedVertex>>* cache;
+    Counters* c;
+    Pdfs pdfs;
+};
+
+// ClassicBidir.CameraPathReciprocals (ClassicBidir.cs:164-176)
+float camera_reciprocals(const BidirCtx& bc, const Pdfs& p, int last) {
+    float sum = 0.0f, next = 1.0f;
+    for (int i = last; i > 0; --i) {
+        next *= p.l2c[(size_t)i] / p.c2l[(size_t)i];
+        if (bc.cfg->enable_connections) sum += next;
+    }
+    if (bc.cfg->enable_light_tracer) sum += next * p.l2c[0] / p.c2l[0] * bc.num_paths;
+    return sum;
+}
+// ClassicBidir.LightPathReciprocals (ClassicBidir.cs:187-198)
+float light_reciprocals(const BidirCtx& bc, const Pdfs& p, int last) {
+    float sum = 0.0f, next = 1.0f;
+    const int n = p.n();
+    for (int i = last + 1; i < n; ++i) {
+        if (i == n - 1) sum += next * p.ne / p.l2c[(size_t)i];
+        next *= p.c2l[(size_t)i] / p.l2c[(size_t)i];
+        if (bc.cfg->enable_connections && i < n - 2) sum += next;
+    }
+    sum += next;
+    return sum;
+}
+
+// ClassicBidir.NextEventPdf (ClassicBidir.cs:42-44) over BidirBase.NextEventPdf
+// (BidirBase.Camera.cs:338-347) of a point on an emitter, without a background.
+float next_event_pdf(const BidirCtx& bc, const SurfacePoint& to) {
+    const or_scene& s = *bc.s;
+    const Emitter& e = s.emitters[(size_t)s.tri_emitter[(size_t)to.tri]];
+    float pmf = 1.0f / (float)s.emitters.size();
+    return e.inv_area * pmf * (1.0f - 0.0f) * (float)bc.cfg->num_shadow_rays;
+}
+// BidirBase.ComputeEmitterPdf (BidirBase.Light.cs:160-166), SelectLightPmf (:116-122)
+float emitter_pdf(const BidirCtx& bc, const Emitter& e, const SurfacePoint& p, V3 light_to_surface,
+                  float jacobian) {
+    float pdf = emitter_pdf_ray(e, p, light_to_surface);
+    pdf *= jacobian;
+    pdf *= (1.0f - 0.0f) / (float)bc.s->emitters.size();
+    return pdf;
+}
+
+// TraceEmitterPath (BidirBase.Light.cs:268-282) with RandomWalk.StartFromEmitter /
+// ContinueWalk (RandomWalk.cs:82-90,150-217) and LightPathWalk (BidirBase.Light.cs:324-410):
+// the vertices of light path `idx` (none if the emitter sample is invalid).
+void bidir_light_path(const BidirCtx& bc, uint32_t idx, uint32_t iteration, std::vector<CachedVertex>* out) {
+    const or_scene& s = *bc.s;
+    const sspt_bidir_settings& cfg = *bc.cfg;
+    out->clear();
+    Rng rng{hash_seed(cfg.base_seed_light, idx, iteration)};
+    // SelectLight (:103-110): no background, so nothing is drawn for it
+    int count = (int)s.emitters.size();
+    const Emitter& e = s.emitters[rng.next_int((uint32_t)count)];
+    float sel = (1.0f - 0.0f) / (float)count;
+    V2 ppos = rng.next_float2();
+    V2 pdir = rng.next_float2();
+    EmitterRay er = emitter_sample_ray(s, e, ppos, pdir);
+    er.pdf *= sel;
+    if (er.pdf == 0.0f || is_black(er.weight)) return;
+    Rgb prefix = er.weight / sel;
+    CachedVertex v0;
+    v0.p = er.p;
+    out->push_back(v0);
+    Rgb approx = rgb(1.0f);
+    Ray ray = spawn_ray(er.p, er.dir);
+    SurfacePoint prev = er.p;
+    float pdf_dir = er.pdf, next_rev = 0.0f;
+    for (int depth = 1; depth < cfg.max_depth;) {
+        HitRec h = trace_closest(s, ray, false);
+        bc.c->rays++;
+        if (h.tri < 0) break;
+        bc.c->hits++;
+        SurfacePoint hit = make_point(s, h.tri, {h.u, h.v}, h.t);
+        float pdf_from = pdf_dir * area_to_solid_angle(prev, hit);
+        if (pdf_from == 0.0f) break;
+        // LightPathWalk.OnHit: the next-event pdf of the edge to the light is
+        // stored with the vertex of depth 2
+        CachedVertex v;
+        v.p = hit;
+        v.pdf_from = pdf_from;
+        v.pdf_rev = next_rev;
+        v.pdf_ne = depth == 2 ? next_event_pdf(bc, (*out)[0].p) : 0.0f;
+        v.weight = prefix;
+        v.depth = depth;
+        out->push_back(v);
+        if (depth + 1 >= cfg.max_depth) break;
+        float q = depth > 4 ? std::min(std::max(average(approx), 0.05f), 0.95f) : 1.0f;
+        if (rng.next_float() > q) break;
+        Shader sh = make_shader(s, hit, -ray.d);
+        sh.light = true;
+        float pc = rng.next_float();
+        V2 pd = rng.next_float2();
+        BsdfSample bs = sample_bsdf(sh, pc, pd);
+        approx = approx * (bs.weight / q);
+        next_rev = bs.pdf_rev * area_to_solid_angle(hit, prev);  // OnContinue
+        if (bs.pdf == 0.0f || is_black(bs.weight)) break;
+        Rgb w = bs.weight * (std::fabs(dot(hit.normal, bs.dir)) / std::fabs(dot(hit.shading_normal, bs.dir)));
+        w = w * (std::fabs(dot(hit.shading_normal, -ray.d)) / std::fabs(dot(hit.normal, -ray.d)));
+        prefix = prefix * (w / q);
+        ++depth;
+        pdf_dir = bs.pdf;
+        prev = hit;
+        ray = spawn_ray(hit, bs.dir);
+    }
+}
+
+// ConnectLightVertexToCamera (BidirBase.Light.cs:35-92) with
+// ClassicBidir.LightTracerMis (ClassicBidir.cs:131-134).
+void bidir_splat(BidirCtx& bc, const std::vector<CachedVertex>& path, size_t k, std::vector<Splat>* out) {
+    const or_scene& s = *bc.s;
+    const CachedVertex& v = path[k];
+    const CachedVertex& anc = path[k - 1];
+    V3 to_anc = normalize(anc.p.position - v.p.position);
+    if (v.depth + 1 < bc.cfg->min_depth) return;
+    int px, py;
+    float jac;
+    if (!s.camera.sample_response(v.p.position, v.p.normal, &px, &py, &jac)) return;
+    bc.c->shadow++;
+    if (trace_any(s, shadow_ray_to(v.p, s.camera.position), false)) {
+        bc.c->occluded++;
+        return;
+    }
+    V3 to_cam = s.camera.position - v.p.position;
+    to_cam = to_cam / length(to_cam);
+    Shader sh = make_shader(s, v.p, to_cam);
+    Rgb f = eval_nocos(sh, to_anc);
+    if (is_black(f)) return;
+    f = f * (std::fabs(dot(v.p.shading_normal, to_anc)) / std::fabs(dot(v.p.normal, to_anc)));
+    float pdf_rev, unused;
+    pdf_pair(sh, to_anc, &pdf_rev, &unused);
+    pdf_rev *= area_to_solid_angle(v.p, anc.p);
+    Pdfs& p = bc.pdfs;
+    p.reset(v.depth + 1);
+    p.gather_light(path, v, -1);
+    p.c2l[0] = 1.0f * jac;  // response.PdfEmit = invLensArea * jacobian
+    p.c2l[1] = pdf_rev;
+    if (v.depth == 1) p.ne = next_event_pdf(bc, anc.p);
+    float mis = 1.0f / (1.0f + light_reciprocals(bc, p, -1) / bc.num_paths);
+    Rgb w = v.weight * f * rgb(jac) / bc.num_paths;
+    out->push_back({(int64_t)py * s.width + px, mis * w});
+}
+
+// BidirBase.OnEmitterHit (BidirBase.Camera.cs:500-524) with ClassicBidir.EmitterHitMis (ClassicBidir.cs:119-128)
+Rgb bidir_emitter_hit(BidirCtx& bc, const Emitter& e, const SurfacePoint& hit, V3 out_dir,
+                      const std::vector<PdfPair>& v, float rev_jac) {
+    Rgb emission = emitted(e, hit, out_dir);
+    float pdf_emit = emitter_pdf(bc, e, hit, out_dir, rev_jac);
+    float pdf_ne = next_event_pdf(bc, hit);
+    const int n = (int)v.size();
+    Pdfs& p = bc.pdfs;
+    p.reset(n);
+    p.gather_camera(v, n - 1);
+    if (n > 1) p.l2c[(size_t)n - 2] = pdf_emit;
+    p.ne = pdf_ne;
+    p.c2l[(size_t)n - 1] = v.back().from;
+    float mis = 1.0f;
+    if (n > 1) {
+        float pdf_this = v.back().from;
+        float sum = 1.0f;
+        sum += p.ne / pdf_this;
+        sum += camera_reciprocals(bc, p, n - 2) / pdf_this;
+        mis = 1.0f / sum;
+    }
+    return mis * emission;
+}
+
+// BidirBase.PerformNextEventEstimation (BidirBase.Camera.cs:368-484), emitter
+// branch, with ClassicBidir.SampleNextEvent / NextEventMis (ClassicBidir.cs:47-51,147-153).
+Rgb bidir_next_event(BidirCtx& bc, const Shader& sh, Rng& rng, const std::vector<PdfPair>& v, float rev_jac) {
+    const or_scene& s = *bc.s;
+    const int n = (int)v.size() + 1;
+    Pdfs& p = bc.pdfs;
+    p.reset(n);
+    p.gather_camera(v, n - 2);
+    p.c2l[(size_t)n - 2] = v.back().from;
+    const float bg = 0.0f;  // ComputeNextEventBackgroundProbability without a background
+    if (rng.next_float() < bg) return rgb(0.0f);
+    if (s.emitters.empty()) return rgb(0.0f);
+    // SampleNextEvent: SelectLight (BidirBase.Camera.cs:70-73), SampleUniformArea
+    int count = (int)s.emitters.size();
+    const Emitter& e = s.emitters[rng.next_int((uint32_t)count)];
+    float prob = 1.0f / (float)count;
+    SurfacePoint lp = make_point(s, e.tri, to_uniform_triangle(rng.next_float2()), 0.0f);
+    float lpdf = e.inv_area;
+    lpdf *= prob;
+    lpdf *= (float)bc.cfg->num_shadow_rays;
+    lpdf *= (1.0f - bg);
+    if (lpdf == 0.0f) return rgb(0.0f);
+    bc.c->shadow++;
+    if (trace_any(s, shadow_ray(sh.p, lp), false)) {
+        bc.c->occluded++;
+        return rgb(0.0f);
+    }
+    V3 l2s = normalize(sh.p.position - lp.position);
+    Rgb emission = emitted(e, lp, l2s);
+    if (is_black(emission)) return rgb(0.0f);
+    Rgb bsdf_cos = eval_cos(sh, -l2s);
+    if (is_black(bsdf_cos)) return rgb(0.0f);
+    float jac = area_to_solid_angle(sh.p, lp);
+    if (jac == 0.0f) return rgb(0.0f);
+    float fwd, rev;
+    pdf_pair(sh, -l2s, &fwd, &rev);
+    fwd *= area_to_solid_angle(sh.p, lp);
+    rev *= rev_jac;
+    if (fwd == 0.0f || rev == 0.0f) return rgb(0.0f);
+    float pdf_emit = emitter_pdf(bc, e, lp, l2s, area_to_solid_angle(lp, sh.p));
+    p.l2c[(size_t)n - 2] = pdf_emit;
+    if (n > 2) p.l2c[(size_t)n - 3] = rev;
+    p.ne = lpdf;
+    p.c2l[(size_t)n - 1] = fwd;
+    float sum = 1.0f;
+    sum += p.c2l[(size_t)n - 1] / p.ne;
+    sum += camera_reciprocals(bc, p, (int)v.size() - 1) / p.ne;
+    float mis = 1.0f / sum;
+    Rgb w = emission * bsdf_cos * (jac / lpdf);
+    return mis * w;
+}
+
+// BidirBase.Connect (BidirBase.Camera.cs:192-270) with ClassicBidir.BidirConnectMis (ClassicBidir.cs:137-144)
+Rgb bidir_connect(BidirCtx& bc, const Shader& sh, const std::vector<CachedVertex>& path, size_t k,
+                  const std::vector<PdfPair>& v, float rev_jac) {
+    const or_scene& s = *bc.s;
+    const CachedVertex& lv = path[k];
+    const CachedVertex& anc = path[k - 1];
+    V3 to_anc = normalize(anc.p.position - lv.p.position);
+    int depth = lv.depth + (int)v.size() + 1;
+    if (depth > bc.cfg->max_depth || depth < bc.cfg->min_depth) return rgb(0.0f);
+    bc.c->shadow++;
+    if (trace_any(s, shadow_ray(lv.p, sh.p), false)) {
+        bc.c->occluded++;
+        return rgb(0.0f);
+    }
+    V3 dir = normalize(lv.p.position - sh.p.position);  // camera vertex -> light vertex
+    Shader ls = make_shader(s, lv.p, to_anc);
+    ls.light = true;
+    Rgb f_light = eval_nocos(ls, -dir) * std::fabs(dot(lv.p.normal, -dir));
+    f_light = f_light * (std::fabs(dot(lv.p.shading_normal, to_anc)) / std::fabs(dot(lv.p.normal, to_anc)));
+    Rgb f_cam = eval_cos(sh, dir);
+    if (is_black(f_cam) || is_black(f_light)) return rgb(0.0f);
+    float pdf_cam_to_light, pdf_cam_rev;
+    pdf_pair(sh, dir, &pdf_cam_to_light, &pdf_cam_rev);
+    pdf_cam_rev *= rev_jac;
+    pdf_cam_to_light *= area_to_solid_angle(sh.p, lv.p);
+    if (pdf_cam_to_light == 0.0f) return rgb(0.0f);
+    float pdf_light_to_cam, pdf_light_rev;
+    pdf_pair(ls, -dir, &pdf_light_to_cam, &pdf_light_rev);
+    pdf_light_rev *= area_to_solid_angle(lv.p, anc.p);  // the ancestor is on a surface
+    pdf_light_to_cam *= area_to_solid_angle(lv.p, sh.p);
+    const int last = (int)v.size() - 1;
+    Pdfs& p = bc.pdfs;
+    p.reset((int)v.size() + lv.depth + 1);
+    p.gather_camera(v, last);
+    p.gather_light(path, lv, last);
+    if (lv.depth == 1) p.ne = next_event_pdf(bc, anc.p);
+    if (last > 0) p.l2c[(size_t)last - 1] = pdf_cam_rev;
+    p.c2l[(size_t)last] = v.back().from;
+    p.l2c[(size_t)last] = pdf_light_to_cam;
+    p.c2l[(size_t)last + 1] = pdf_cam_to_light;
+    p.c2l[(size_t)last + 2] = pdf_light_rev;
+    float sum = 1.0f;
+    sum += camera_reciprocals(bc, p, last);
+    sum += light_reciprocals(bc, p, last);
+    float mis = 1.0f / sum;
+    V3 d = sh.p.position - lv.p.position;
+    float dist2 = dot(d, d);
+    if (dist2 == 0.0f) return rgb(0.0f);
+    Rgb w = lv.weight * f_light * f_cam / dist2 / 1.0f;
+    return mis * w;
+}
+
+// BidirBase.RenderPixel (BidirBase.Camera.cs:97-106) and the camera's RandomWalk
+// (RandomWalk.StartFromCamera / ContinueWalk, RandomWalk.cs:73-80,150-217;
+// CameraRandomWalk, BidirBase.Camera.cs:588-653) with ClassicBidir.OnCameraHit
+// (ClassicBidir.cs:94-116): the pixel value of one camera path.
+Rgb bidir_camera_path(BidirCtx& bc, uint32_t row, uint32_t col, uint32_t iteration) {
+    const or_scene& s = *bc.s;
+    const sspt_bidir_settings& cfg = *bc.cfg;
+    const uint32_t pixel = row * (uint32_t)s.width + col;
+    Rng rng{hash_seed(cfg.base_seed_camera, pixel, iteration)};
+    V2 off = rng.next_float2();
+    Ray ray;
+    V3 raw;
+    s.camera.generate_ray((float)col + off.x, (float)row + off.y, &ray.o, &ray.d, &raw);
+    ray.tmin = 0.0f;
+    ray.tmax = INFINITY;
+    // CameraRaySample: Point (position, normal = view direction), PdfRay
+    SurfacePoint prev;
+    prev.position = s.camera.position;
+    prev.normal = s.camera.direction();
+    float pdf_dir = s.camera.pixel_jacobian(ray.o + raw);
+    const std::vector<CachedVertex>& light = (*bc.cache)[pixel];  // SelectBidirPath (:179-184)
+    std::vector<PdfPair> v;
+    Rgb prefix = rgb(1.0f), approx = rgb(1.0f), L = rgb(0.0f);
+    for (int depth = 1; depth < cfg.max_depth + 1;) {
+        HitRec h = trace_closest(s, ray, false);
+        bc.c->rays++;
+        if (h.tri < 0) break;  // OnBackgroundHit without a background adds nothing
+        bc.c->hits++;
+        SurfacePoint hit = make_point(s, h.tri, {h.u, h.v}, h.t);
+        float pdf_from = pdf_dir * area_to_solid_angle(prev, hit);
+        if (pdf_from == 0.0f) break;
+        float rev_jac = area_to_solid_angle(hit, prev);
+        Shader sh = make_shader(s, hit, -ray.d);
+        v.push_back({pdf_from, 0.0f});
+        Rgb value = rgb(0.0f);
+        int ei = s.tri_emitter[(size_t)h.tri];
+        if (ei >= 0 && depth >= cfg.min_depth)
+            value = value + prefix * bidir_emitter_hit(bc, s.emitters[(size_t)ei], hit, -ray.d, v, rev_jac);
+        if (depth < cfg.max_depth && cfg.enable_connections) {
+            Rgb conn = rgb(0.0f);
+            for (size_t k = 1; k < light.size(); ++k) conn = conn + bidir_connect(bc, sh, light, k, v, rev_jac);
+            value = value + prefix * conn;
+        }
+        if (depth < cfg.max_depth && depth + 1 >= cfg.min_depth)
+            for (int i = 0; i < cfg.num_shadow_rays; ++i)
+                value = value + prefix * bidir_next_event(bc, sh, rng, v, rev_jac);
+        L = L + value;
+        if (depth + 1 >= cfg.max_depth + 1) break;
+        float q = depth > 4 ? std::min(std::max(average(approx), 0.05f), 0.95f) : 1.0f;
+        if (rng.next_float() > q) break;
+        float pc = rng.next_float();
+        V2 pd = rng.next_float2();
+        BsdfSample bs = sample_bsdf(sh, pc, pd);
+        approx = approx * (bs.weight / q);
+        v.back().to = bs.pdf_rev * rev_jac;  // OnContinue
+        if (bs.pdf == 0.0f || is_black(bs.weight)) break;
+        prefix = prefix * (bs.weight / q);
+        ++depth;
+        pdf_dir = bs.pdf;
+        prev = hit;
+        ray = spawn_ray(hit, bs.dir);
+    }
+    return L;
+}
+
+}  // namespace
+
+uint64_t or_render_bidir(const or_scene* s, const sspt_bidir_settings* cfg, uint32_t first, uint32_t num,
+                         float* image, sspt_ray_stats* stats, int32_t threads) {
+    const int64_t W = s->width, H = s->height;
+    if (s->has_env || s->emitters.empty()) return UINT64_MAX;
+    if (cfg->num_light_paths >= 0 && (int64_t)cfg->num_light_paths != W * H) return UINT64_MAX;
+    const uint32_t paths = (uint32_t)(W * H);
+    std::atomic<uint64_t> rays{0}, shadow{0}, occl{0}, hits{0};
+    uint64_t nans = 0;
+    std::vector<std::vector<CachedVertex>> cache(paths);
+    std::vector<std::vector<Splat>> splats(paths);
+    std::vector<Rgb> values((size_t)(W * H));
+    const int64_t chunks = ((int64_t)paths + 63) / 64;
+    for (uint32_t it = first; it < first + num; ++it) {
+        // FrameBuffer.StartIteration / Normalize (FrameBuffer.cs:232-249)
+        float n = (float)(it + 1);
+        if (it + 1 > 1) {
+            float scale = (n - 1.0f) / n;
+            for (int64_t k = 0; k < W * H * 3; ++k) image[k] *= scale;
+        }
+        auto run = [&](int phase) {
+            parallel_for(chunks, threads, [&](int64_t ch) {
+                Counters c;
+                BidirCtx bc{s, cfg, (float)paths, &cache, &c, {}};
+                int64_t end = std::min<int64_t>(paths, (ch + 1) * 64);
+                for (int64_t k = ch * 64; k < end; ++k) {
+                    if (phase == 0) {
+                        bidir_light_path(bc, (uint32_t)k, it, &cache[(size_t)k]);
+                    } else if (phase == 1) {
+                        splats[(size_t)k].clear();
+                        for (size_t i = 1; i < cache[(size_t)k].size(); ++i)
+                            bidir_splat(bc, cache[(size_t)k], i, &splats[(size_t)k]);
+                    } else {
+                        values[(size_t)k] = bidir_camera_path(bc, (uint32_t)(k / W), (uint32_t)(k % W), it);
+                    }
+                }
+                rays += c.rays, shadow += c.shadow, occl += c.occluded, hits += c.hits;
+            });
+        };
+        run(0);  // TraceLightPaths
+        if (cfg->enable_light_tracer) run(1);  // ProcessPathCache: SplatLightVertices
+        run(2);  // TraceAllCameraPaths
+        // FrameBuffer.Splat (FrameBuffer.cs:160-191), IgnoreNanAndInf
+        auto add = [&](int64_t pix, Rgb value) {
+            if (!std::isfinite(average(value))) {
+                ++nans;
+                return;
+            }
+            Rgb a = value / n;
+            image[pix * 3 + 0] += a.x;
+            image[pix * 3 + 1] += a.y;
+            image[pix * 3 + 2] += a.z;
+        };
+        if (cfg->enable_light_tracer)
+            for (uint32_t k = 0; k < paths; ++k)
+                for (const Splat& sp : splats[k]) add(sp.pix, sp.value);
+        for (int64_t k = 0; k < W * H; ++k) add(k, values[(size_t)k]);
     }
     if (stats) {
         stats->num_rays = rays;

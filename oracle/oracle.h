@@ -52,12 +52,16 @@ float or_fresnel_dielectric(float cos_i, float eta_i, float eta_t);
 void or_material_eval(const sspt_material_desc* mat, const sspt_texture_desc* textures,
                       const float* normal3, const float* uv2, const float* out_dir3,
                       const float* in_dir3, float* value3, float* value_cos3, float* pdf);
+/* SurfaceShader.Pdf(inDir) (SurfaceShader.cs:64-67): the forward and the reverse pdf. */
+void or_material_pdf_pair(const sspt_material_desc* mat, const sspt_texture_desc* textures, const float* normal3,
+                          const float* uv2, const float* out_dir3, const float* in_dir3, float* pdf,
+                          float* pdf_rev);
 /* f * |cos| (3 floats each) and the forward pdf (may be NULL) of n incident
  * directions in_dirs (3 floats each) for one outgoing direction. */
 void or_material_eval_many(const sspt_material_desc* mat, const sspt_texture_desc* textures,
                            const float* normal3, const float* uv2, const float* out_dir3, int64_t n,
                            const float* in_dirs, float* value_cos, float* pdf);
-/* sample_out: dir3, pdf, weight3 -> 7 floats */
+/* sample_out: dir3, pdf, weight3, reverse pdf -> 8 floats */
 void or_material_sample(const sspt_material_desc* mat, const sspt_texture_desc* textures,
                         const float* normal3, const float* uv2, const float* out_dir3,
                         float primary_component, const float* primary2, float* sample_out);
@@ -124,6 +128,14 @@ uint32_t or_replay_pixel(const or_scene* s, const sspt_pt_settings* settings, ui
 uint64_t or_render_light_tracer(const or_scene* s, const sspt_lt_settings* settings, uint32_t first_iteration,
                                 uint32_t num_iterations, float* image, sspt_ray_stats* stats,
                                 int32_t num_threads);
+/* ---- bidirectional path tracing: ClassicBidir (SeeSharp/Integrators/Bidir/ClassicBidir.cs),
+ * the contract of sspt_render_bidir; per iteration the light tracer's splats
+ * are added in light-path order, then the camera paths' values in pixel
+ * order. Returns the non-finite value count, UINT64_MAX for a scene with a
+ * background or without emitters or for num_light_paths other than -1 or
+ * the pixel count. */
+uint64_t or_render_bidir(const or_scene* s, const sspt_bidir_settings* settings, uint32_t first_iteration,
+                         uint32_t num_iterations, float* image, sspt_ray_stats* stats, int32_t num_threads);
 void or_primary_aov(const or_scene* s, const sspt_pt_settings* settings, uint32_t first_iteration,
                     uint32_t num_iterations, float* albedo, float* normal, int32_t num_threads);
 

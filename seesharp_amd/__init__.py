@@ -18,5 +18,5 @@ from .scene import (  # noqa: F401
     TextureRgb,
     create_look_at,
 )
-from .integrator import (DebugVisualizer, Device, GpuBatch, GpuPathTracer, GpuScene, Integrator,  # noqa: F401
-                         VertexCacheBidirLightTracer)
+from .integrator import (ClassicBidir, DebugVisualizer, Device, GpuBatch, GpuPathTracer, GpuScene,  # noqa: F401
+                         Integrator, VertexCacheBidirLightTracer)

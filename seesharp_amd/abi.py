@@ -13,7 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 # different occupancy targets); the default is the in-tree build.
 LIB_PATH = os.environ.get("SSPT_LIBRARY") or os.path.join(_HERE, "libsspt.so")
 
-ABI_VERSION = 6  # SSPT_ABI_VERSION of include/sspt.h
+ABI_VERSION = 7  # SSPT_ABI_VERSION of include/sspt.h
 NOALLOC_SCRATCH_BYTES = 64 << 20  # SSPT_NOALLOC_SCRATCH_BYTES
 SSPT_OK = 0
 INVALID_ARGUMENT = 1
@@ -133,6 +133,20 @@ class LtSettings(C.Structure):
     ]
 
 
+class BidirSettings(C.Structure):
+    """sspt_bidir_settings: ClassicBidir's BidirBase / Integrator fields."""
+    _fields_ = [
+        ("base_seed_camera", C.c_uint32),
+        ("base_seed_light", C.c_uint32),
+        ("max_depth", C.c_int32),
+        ("min_depth", C.c_int32),
+        ("num_shadow_rays", C.c_int32),
+        ("enable_connections", C.c_int32),
+        ("enable_light_tracer", C.c_int32),
+        ("num_light_paths", C.c_int32),
+    ]
+
+
 class RayStats(C.Structure):
     _fields_ = [
         ("num_rays", C.c_uint64),
@@ -210,6 +224,8 @@ SYMBOLS = {
     "sspt_get_block_shape": (C.c_int, [_vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "sspt_render_light_tracer": (C.c_int, [_vp, _vp, C.POINTER(LtSettings), C.c_uint32, C.c_uint32,
                                            C.POINTER(RayStats)]),
+    "sspt_render_bidir": (C.c_int, [_vp, _vp, C.POINTER(BidirSettings), C.c_uint32, C.c_uint32,
+                                    C.POINTER(RayStats)]),
     "sspt_set_bvh_builder": (C.c_int, [_vp, C.c_int32]),
     "sspt_get_bvh_builder": (C.c_int, [_vp, C.POINTER(C.c_int32)]),
     "sspt_scene_bvh_build_time": (C.c_int, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_double),

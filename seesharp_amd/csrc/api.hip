@@ -123,6 +123,9 @@ struct sspt_scene {
     float* layers = nullptr;  // albedo pool then normal pool, framebuffer layout each
     // outlier tracking: image_floats / 3 * dev.num_outliers keys, or null (off)
     unsigned long long* outliers = nullptr;
+    // bidirectional rendering: the light-path cache and camera pdf levels, grown on demand
+    char* bidir = nullptr;
+    uint64_t bidir_bytes = 0;
 };
 
 // Scratch memory of sspt_render_noalloc: every wave buffer of an allocation-free
@@ -1135,6 +1138,7 @@ void sspt_scene_destroy(sspt_scene* sc) {
     (void)hipStreamSynchronize(sc->ctx->stream);
     for (void* p : sc->allocs) (void)hipFree(p);
     if (sc->outliers) (void)hipFree(sc->outliers);
+    if (sc->bidir) (void)hipFree(sc->bidir);
     delete sc;
 }
 
@@ -1702,7 +1706,87 @@ sspt_status sspt_render_light_tracer(sspt_ctx* ctx, sspt_scene* sc, const sspt_l
         if (it + 1 > 1) launch_scale(ctx->stream, image, 3ull * R.num_pixels, (n - 1.0f) / n);
         if (paths > 0) {
             LightParams P{cfg->base_seed_light, it, (uint32_t)paths, cfg->max_depth, cfg->min_depth, (float)paths, n};
-            launch_light_paths(ctx->stream, sc->dev, 0, P, image, ctx->lt_stats);
+            timed(ctx, "light_paths", [&] { launch_light_paths(ctx->stream, sc->dev, 0, P, image, ctx->lt_stats); });
+        }
+        HIP_TRY(ctx, hipGetLastError());
+    }
+    TRY(read_stats(ctx, sc, stats, ctx->lt_stats));
+    if (stats) stats->num_iterations = num_iterations;
+    return SSPT_OK;
+}
+
+sspt_status sspt_render_bidir(sspt_ctx* ctx, sspt_scene* sc, const sspt_bidir_settings* cfg,
+                              uint32_t first_iteration, uint32_t num_iterations, sspt_ray_stats* stats) {
+    if (!ctx || !sc || !cfg) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "null argument");
+    if (sc->recs.size() != 1)
+        return fail(ctx, SSPT_ERR_UNSUPPORTED, "bidirectional rendering takes a single scene, not a batch or view set");
+    const SceneRec& R = sc->recs[0];
+    if (R.has_env)
+        return fail(ctx, SSPT_ERR_UNSUPPORTED, "bidirectional rendering does not sample the background (scene has one)");
+    if (R.num_emitters == 0) return fail(ctx, SSPT_ERR_UNSUPPORTED, "bidirectional rendering needs emitters");
+    if (cfg->max_depth < 0 || cfg->min_depth < 0 || cfg->num_shadow_rays < 0)
+        return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "max_depth, min_depth and num_shadow_rays must be non-negative");
+    const uint64_t paths = (uint64_t)R.num_pixels;
+    // ClassicBidir.Render (ClassicBidir.cs:72-75): one light path per camera path
+    if (cfg->num_light_paths >= 0 && (uint64_t)cfg->num_light_paths != paths)
+        return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "ClassicBidir requires exactly one light path per pixel (%llu)",
+                    (unsigned long long)paths);
+    if ((uint64_t)first_iteration + num_iterations > 0xFFFFFFFFull)
+        return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "iteration index overflows");
+    // light cache: max_depth levels of two float4 per path, the lengths and next-event pdfs;
+    // camera pdfs: max_depth levels of a float2 per pixel
+    const uint64_t levels = (uint64_t)std::max(cfg->max_depth, 1);
+    const uint64_t need = levels * paths * (2 * sizeof(float4) + sizeof(float2)) + paths * 8 + 256 * 4;
+    if (need > (64ull << 30)) return fail(ctx, SSPT_ERR_OUT_OF_MEMORY, "bidirectional buffers exceed 64 GiB");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    if (sc->bidir_bytes < need) {
+        if (sc->bidir) {
+            HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+            (void)hipFree(sc->bidir);
+            sc->bidir = nullptr;
+            sc->bidir_bytes = 0;
+        }
+        if (hipMalloc((void**)&sc->bidir, need) != hipSuccess) {
+            sc->bidir = nullptr;
+            return fail(ctx, SSPT_ERR_OUT_OF_MEMORY, "bidirectional buffers (%llu bytes)", (unsigned long long)need);
+        }
+        sc->bidir_bytes = need;
+    }
+    auto carve = [&](uint64_t* off, uint64_t bytes) {
+        char* p = sc->bidir + *off;
+        *off += (bytes + 255) & ~255ull;
+        return p;
+    };
+    uint64_t off = 0;
+    BidirParams P{};
+    P.lv_a = reinterpret_cast<float4*>(carve(&off, levels * paths * sizeof(float4)));
+    P.lv_b = reinterpret_cast<float4*>(carve(&off, levels * paths * sizeof(float4)));
+    P.cam = reinterpret_cast<float2*>(carve(&off, levels * paths * sizeof(float2)));
+    P.lv_len = reinterpret_cast<int*>(carve(&off, paths * sizeof(int)));
+    P.lv_ne = reinterpret_cast<float*>(carve(&off, paths * sizeof(float)));
+    P.seed_camera = cfg->base_seed_camera;
+    P.seed_light = cfg->base_seed_light;
+    P.num_paths = (uint32_t)paths;
+    P.max_depth = cfg->max_depth;
+    P.min_depth = cfg->min_depth;
+    P.num_shadow_rays = cfg->num_shadow_rays;
+    P.connections = cfg->enable_connections ? 1 : 0;
+    P.light_tracer = cfg->enable_light_tracer ? 1 : 0;
+    P.num_paths_f = (float)paths;
+    if (!ctx->lt_stats) HIP_TRY(ctx, hipMalloc((void**)&ctx->lt_stats, kStatBytes));
+    HIP_TRY(ctx, hipMemsetAsync(ctx->lt_stats, 0, kStatBytes, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(sc->d_recs, &R, sizeof(SceneRec), hipMemcpyHostToDevice, ctx->stream));
+    float* image = sc->image + R.fb_off;
+    for (uint32_t it = first_iteration; it < first_iteration + num_iterations; ++it) {
+        // FrameBuffer.StartIteration / Normalize (FrameBuffer.cs:232-249)
+        const float n = (float)(it + 1);
+        if (it + 1 > 1) launch_scale(ctx->stream, image, 3ull * R.num_pixels, (n - 1.0f) / n);
+        P.iteration = it;
+        P.n = n;
+        if (paths > 0) {
+            timed(ctx, "bidir_light", [&] { launch_bidir_light(ctx->stream, sc->dev, 0, P, image, ctx->lt_stats); });
+            timed(ctx, "bidir_camera",
+                  [&] { launch_bidir_camera(ctx->stream, sc->dev, 0, P, image, ctx->lt_stats); });
         }
         HIP_TRY(ctx, hipGetLastError());
     }

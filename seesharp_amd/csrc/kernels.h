@@ -120,6 +120,29 @@ struct LightParams {
 };
 void launch_light_paths(hipStream_t s, const DevScene& sc, int scene, const LightParams& p, float* image,
                         unsigned long long* stats);
+// One iteration of ClassicBidir over the pixels of scene record `scene`
+// (num_paths = width * height light paths and camera paths).
+struct BidirParams {
+    uint32_t seed_camera, seed_light, iteration, num_paths;
+    int max_depth, min_depth, num_shadow_rays;
+    int connections, light_tracer;  // EnableConnections, EnableLightTracer
+    float num_paths_f;              // NumLightPaths as a float
+    float n;                        // FrameBuffer.CurIteration
+    // The light-path cache, one level per vertex depth: vertex k of path p at
+    // [k * num_paths + p] (k < max_depth).
+    float4* lv_a;   // prefix weight rgb, PdfFromAncestor
+    float4* lv_b;   // barycentric u, v, PdfReverseAncestor, triangle (int bits)
+    int* lv_len;    // vertices of path p (0: none)
+    float* lv_ne;   // next-event pdf of path p's emitter point (PdfNextEventAncestor)
+    // Camera vertex k (depth k + 1) of pixel p at [k * num_paths + p]:
+    // PdfFromAncestor, PdfToAncestor.
+    float2* cam;
+};
+// TraceLightPaths (+ SplatLightVertices if light_tracer), then the camera paths.
+void launch_bidir_light(hipStream_t s, const DevScene& sc, int scene, const BidirParams& p, float* image,
+                        unsigned long long* stats);
+void launch_bidir_camera(hipStream_t s, const DevScene& sc, int scene, const BidirParams& p, float* image,
+                         unsigned long long* stats);
 void launch_debug_visualize(hipStream_t s, const DevScene& sc, int scene, uint32_t num_pixels, uint32_t base_seed,
                             uint32_t first, uint32_t num, float* image);
 

@@ -107,7 +107,8 @@ __device__ __forceinline__ uint3 block_append_word(unsigned long long* word, int
 // PerspectiveCamera.GenerateRay (SeeSharp/Cameras/PerspectiveCamera.cs:64-102), lensRadius = 0:
 // view = (2x/W - 1, 1 - 2y/H, 0); local = Transform(view, viewToCamera) (w = 1);
 // dir = normalize(Transform((local, 0), cameraToWorld)); origin = camera position.
-__device__ __forceinline__ void camera_ray(const DevCamera& c, float fx, float fy, f3* o, f3* d) {
+__device__ __forceinline__ void camera_ray(const DevCamera& c, float fx, float fy, f3* o, f3* d,
+                                           f3* raw = nullptr) {
     float vx = 2.0f * fx / (float)c.width - 1.0f;
     float vy = 1.0f - 2.0f * fy / (float)c.height;
     float vz = 0.0f;
@@ -119,6 +120,7 @@ __device__ __forceinline__ void camera_ray(const DevCamera& c, float fx, float f
                 l.x * W[2] + l.y * W[6] + l.z * W[10]);
     *o = mk3(c.pos[0], c.pos[1], c.pos[2]);
     *d = normalize_exact(dd);
+    if (raw) *raw = dd;  // the unnormalised direction
 }
 
 // ------------------------------------------------------------ intersection -
@@ -1542,6 +1544,399 @@ __global__ void __launch_bounds__(kTraceBlock) k_light_paths(DevScene S, int sce
     }
 }
 
+// ---------------------------------------------- bidirectional path tracing -
+// ClassicBidir (ClassicBidir.cs) in two kernels per iteration: k_bidir_light
+// traces one light path per pixel into a cache in HBM, one level per vertex
+// depth so that the lanes of a wave (neighbouring paths) touch neighbouring
+// entries, and splats its vertices onto the camera; k_bidir_camera then
+// follows one camera path per pixel and combines its emitter hits, next-event
+// samples and connections to the vertices of its pixel's light path. One thread
+// per path: the MIS weights read the pdfs along both subpaths from the cache and
+// the per-pixel camera pdf levels instead of per-thread arrays.
+
+// Emitter.PdfRay: DiffuseEmitter.cs:39-42, GlossyEmitter.cs:42-45 (with
+// SampleWarp.ToCosineLobeJacobian, SampleWarp.cs:192-194).
+__device__ __forceinline__ float emitter_pdf_ray(const DevEmitter& e, f3 ns, f3 dir) {
+    const float cosine = sdiv(dot(ns, dir), length(dir));
+    if (e.kind == kEmitterGlossy) {
+        const float power = e.exponent + 1.0f;
+        const float jac = cosine > 0.0f ? sdiv((power + 1.0f) * powf(cosine, power) * 1.0f, 2.0f * kPi) : 0.0f;
+        return e.inv_area * jac;
+    }
+    return e.inv_area * fmaxf(cosine, 0.0f) * kInvPi;
+}
+
+// PerspectiveCamera.SolidAngleToPixelJacobian (PerspectiveCamera.cs:177-191)
+__device__ __forceinline__ float pixel_jacobian(const DevCamera& c, f3 pos) {
+    const float* M = c.w2c;
+    const f3 l = mk3(pos.x * M[0] + pos.y * M[4] + pos.z * M[8] + M[12],
+                     pos.x * M[1] + pos.y * M[5] + pos.z * M[9] + M[13],
+                     pos.x * M[2] + pos.y * M[6] + pos.z * M[10] + M[14]);
+    const float cosine = l.z / sqrtf(dot(l, l));
+    const float d = c.image_plane / cosine;
+    return d * d / fabsf(cosine);
+}
+
+struct BidirCtx {
+    const DevScene& S;
+    const SceneRec& R;
+    const BidirParams& P;
+    uint32_t path;  // the light path / pixel
+    __device__ __forceinline__ float4 va(int k) const { return P.lv_a[(size_t)k * P.num_paths + path]; }
+    __device__ __forceinline__ float4 vb(int k) const { return P.lv_b[(size_t)k * P.num_paths + path]; }
+    __device__ __forceinline__ float2 cam(int k) const { return P.cam[(size_t)k * P.num_paths + path]; }
+    __device__ __forceinline__ Surf vertex(int k) const {
+        const float4 b = vb(k);
+        return surface(S, __float_as_int(b.w), b.x, b.y);
+    }
+    // ClassicBidir.NextEventPdf (ClassicBidir.cs:42-44) over BidirBase.NextEventPdf
+    // (BidirBase.Camera.cs:338-347) of a point on emitter e, without a background
+    __device__ __forceinline__ float next_event_pdf(const DevEmitter& e) const {
+        return e.inv_area * (1.0f / (float)R.num_emitters) * (1.0f - 0.0f) * (float)P.num_shadow_rays;
+    }
+    // BidirBase.ComputeEmitterPdf (BidirBase.Light.cs:160-166) with SelectLightPmf (:116-122)
+    __device__ __forceinline__ float emitter_pdf(const DevEmitter& e, f3 ns, f3 light_to_surface, float jac) const {
+        float pdf = emitter_pdf_ray(e, ns, light_to_surface);
+        pdf *= jac;
+        pdf *= (1.0f - 0.0f) / (float)R.num_emitters;
+        return pdf;
+    }
+    // ClassicBidir.CameraPathReciprocals (ClassicBidir.cs:164-176) over the
+    // camera prefix: PdfsCameraToLight[i] = camera vertex i's PdfFromAncestor,
+    // PdfsLightToCamera[i] = camera vertex i+1's PdfToAncestor, except
+    // [last] = x_last and [last - 1] = x_prev (GatherCameraPdfs and the
+    // technique's own values, BidirPathPdfs.cs:46-53).
+    __device__ float camera_reciprocals(int last, float x_last, float x_prev) const {
+        float sum = 0.0f, next = 1.0f;
+        for (int i = last; i > 0; --i) {
+            const float l2c = i == last ? x_last : (i == last - 1 ? x_prev : cam(i + 1).y);
+            next *= sdiv(l2c, cam(i).x);
+            if (P.connections) sum += next;
+        }
+        if (P.light_tracer) {
+            const float l2c = last == 0 ? x_last : (last == 1 ? x_prev : cam(1).y);
+            sum += sdiv(next * l2c, cam(0).x) * P.num_paths_f;
+        }
+        return sum;
+    }
+    // ClassicBidir.LightPathReciprocals (ClassicBidir.cs:187-198) of the full
+    // path of c camera vertices and light vertices s .. 0 (N = c + s + 1 pdfs):
+    // PdfsCameraToLight[c] = x0, [c + 1] = x1, [i >= c + 2] = light vertex
+    // (s + c + 2 - i)'s PdfReverseAncestor; PdfsLightToCamera[i < N - 1] =
+    // light vertex (s - i + c)'s PdfFromAncestor, [N - 1] = 1
+    // (GatherLightPdfs, BidirPathPdfs.cs:82-92).
+    __device__ float light_reciprocals(int c, int s, float x0, float x1, float ne) const {
+        const int n = c + s + 1;
+        float sum = 0.0f, next = 1.0f;
+        for (int i = c; i < n; ++i) {
+            const float l2c = i == n - 1 ? 1.0f : va(s - (i - c)).w;
+            if (i == n - 1) sum += sdiv(next * ne, l2c);
+            const float c2l = i == c ? x0 : (i == c + 1 ? x1 : vb(s + c + 2 - i).z);
+            next *= sdiv(c2l, l2c);
+            if (P.connections && i < n - 2) sum += next;
+        }
+        sum += next;
+        return sum;
+    }
+};
+
+// TraceLightPaths / TraceEmitterPath (BidirBase.Light.cs:197-282) with
+// RandomWalk.StartFromEmitter / ContinueWalk (RandomWalk.cs:82-90,150-217) and
+// LightPathWalk (BidirBase.Light.cs:324-410) storing the vertices; with the
+// light tracer on, every vertex is connected to the camera as it is stored
+// (ConnectLightVertexToCamera, BidirBase.Light.cs:35-92, with
+// ClassicBidir.LightTracerMis, ClassicBidir.cs:131-134): the connection draws
+// no random numbers and reads only the vertices stored before it.
+__global__ void __launch_bounds__(kTraceBlock) k_bidir_light(DevScene S, int scene, BidirParams P, float* image,
+                                                             unsigned long long* stats) {
+    __shared__ int stack_node[kStackNodeLds];
+    __shared__ float stack_t[kStackLds * kTraceBlock];
+    const SceneRec& R = S.recs[scene];
+    const f3 cam = mk3(R.cam.pos[0], R.cam.pos[1], R.cam.pos[2]);
+    for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < P.num_paths; idx += gridDim.x * blockDim.x) {
+        BidirCtx B{S, R, P, idx};
+        Rng rng{hash_seed(P.seed_light, idx, P.iteration)};
+        // SelectLight (:103-110): no background, so nothing is drawn for it
+        const DevEmitter e = S.emitters[R.emitter_off + (int)rng.next_int((uint32_t)R.num_emitters)];
+        const float sel = (1.0f - 0.0f) / (float)R.num_emitters;
+        const f2 ppos = rng.next_float2();
+        const f2 pdir = rng.next_float2();
+        EmitterRay er = emitter_ray(S, e, ppos, pdir);
+        er.pdf *= sel;
+        const float ne = B.next_event_pdf(e);
+        P.lv_ne[idx] = ne;
+        if (er.pdf == 0.0f || is_zero(er.weight)) {
+            P.lv_len[idx] = 0;
+            continue;
+        }
+        const f2 b0 = to_uniform_triangle(ppos);
+        P.lv_a[idx] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        P.lv_b[idx] = make_float4(b0.x, b0.y, 0.0f, __int_as_float(e.tri));
+        int len = 1;
+        f3 prefix = er.weight / sel;
+        f3 approx = gray(1.0f);
+        Surf prev = er.p;
+        float pdf_dir = er.pdf, next_rev = 0.0f;
+        f3 o = offset_point(er.p, er.dir), d = er.dir;
+        float tmin = er.p.err;
+        for (int depth = 1; depth < P.max_depth;) {
+            const HitOut h = traverse<false>(S, R.root, R.abs_max, o, d, tmin, INFINITY, stack_node + threadIdx.x,
+                                             stack_t + threadIdx.x);
+            wave_count(stats, kStatRays, true);
+            wave_count(stats, kStatHits, h.li >= 0);
+            if (h.li < 0) break;
+            const Surf s = surface(S, h.li, h.u, h.v);
+            const float pdf_from = pdf_dir * area_to_solid_angle(prev.pos, s.pos, s.ng);
+            if (pdf_from == 0.0f) break;
+            const size_t at = (size_t)depth * P.num_paths + idx;
+            P.lv_a[at] = f4_of(prefix, pdf_from);
+            P.lv_b[at] = make_float4(h.u, h.v, next_rev, __int_as_float(h.li));
+            len = depth + 1;
+            uint32_t pix;
+            float jac;
+            if (P.light_tracer && depth + 1 >= P.min_depth && camera_response(R.cam, s.pos, s.ng, &pix, &jac)) {
+                const f3 p0 = offset_point(s, cam - s.pos);
+                const bool occ = traverse<true>(S, R.root, R.abs_max, p0, cam - p0, 1e-5f, 1.0f - 1e-5f,
+                                                stack_node + threadIdx.x, stack_t + threadIdx.x)
+                                     .li >= 0;
+                wave_count(stats, kStatShadow, true);
+                wave_count(stats, kStatShadowTraced, true);
+                wave_count(stats, kStatOccluded, occ);
+                if (!occ) {
+                    f3 to_cam = cam - s.pos;
+                    to_cam = to_cam / length(to_cam);
+                    const f3 to_anc = normalize(prev.pos - s.pos);
+                    Shader sh;
+                    make_shader(S, s, to_cam, &sh);  // the camera side: IsOnLightSubpath = false
+                    f3 f = material_value(sh, to_anc);
+                    if (!is_zero(f)) {
+                        f = f * sdiv(fabsf(dot(s.ns, to_anc)), fabsf(dot(s.ng, to_anc)));
+                        float pdf_rev, unused;
+                        pdf_pair(sh, to_anc, &pdf_rev, &unused);
+                        pdf_rev *= area_to_solid_angle(s.pos, prev.pos, prev.ng);
+                        // PdfEmit = invLensArea * jacobian, then the vertex's own reverse pdf
+                        const float mis = srcp(1.0f + sdiv(B.light_reciprocals(0, depth, 1.0f * jac, pdf_rev, ne),
+                                                           P.num_paths_f));
+                        const f3 w = prefix * f * jac / P.num_paths_f;
+                        const f3 v = mis * w;
+                        const bool bad = !isfinite(average(v));
+                        wave_count(stats, kStatNan, bad);
+                        if (!bad) {
+                            const f3 add = div_exact(v, P.n);
+                            float* px = image + 3ull * pix;
+                            atomicAdd(px, add.x);
+                            atomicAdd(px + 1, add.y);
+                            atomicAdd(px + 2, add.z);
+                        }
+                    }
+                }
+            }
+            if (depth + 1 >= P.max_depth) break;
+            const float q = depth > 4 ? clampf(average(approx), 0.05f, 0.95f) : 1.0f;
+            if (rng.next_float() > q) break;
+            Shader sh;
+            make_shader(S, s, -d, &sh);
+            sh.light = true;
+            const float pc = rng.next_float();
+            const f2 pd = rng.next_float2();
+            const BsdfSample bs = sample_bsdf<true>(sh, pc, pd);
+            approx = approx * (bs.weight / q);
+            next_rev = bs.pdf_rev * area_to_solid_angle(s.pos, prev.pos, prev.ng);  // OnContinue
+            if (bs.pdf == 0.0f || is_zero(bs.weight)) break;
+            f3 w = bs.weight * sdiv(fabsf(dot(s.ng, bs.dir)), fabsf(dot(s.ns, bs.dir)));
+            w = w * sdiv(fabsf(dot(s.ns, -d)), fabsf(dot(s.ng, -d)));
+            prefix = prefix * (w / q);
+            ++depth;
+            pdf_dir = bs.pdf;
+            prev = s;
+            o = offset_point(s, bs.dir);
+            d = bs.dir;
+            tmin = s.err;
+        }
+        P.lv_len[idx] = len;
+    }
+}
+
+// BidirBase.RenderPixel (BidirBase.Camera.cs:97-106) and the camera's RandomWalk
+// (RandomWalk.cs:73-80,150-217, CameraRandomWalk BidirBase.Camera.cs:588-653) of
+// MaxDepth + 1 vertices with ClassicBidir.OnCameraHit (ClassicBidir.cs:94-116):
+// OnEmitterHit (BidirBase.Camera.cs:500-524), the connections to every vertex
+// of the pixel's light path (BidirConnections / Connect, :192-307) and
+// NumShadowRays next-event samples (PerformNextEventEstimation, :368-484), each
+// weighted by the balance heuristic (ClassicBidir.cs:119-153). The pixel value
+// is added to the running mean after the light tracer's splats.
+__global__ void __launch_bounds__(kTraceBlock) k_bidir_camera(DevScene S, int scene, BidirParams P, float* image,
+                                                              unsigned long long* stats) {
+    __shared__ int stack_node[kStackNodeLds];
+    __shared__ float stack_t[kStackLds * kTraceBlock];
+    const SceneRec& R = S.recs[scene];
+    const DevCamera& C = R.cam;
+    for (uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x; pix < P.num_paths; pix += gridDim.x * blockDim.x) {
+        BidirCtx B{S, R, P, pix};
+        const uint32_t row = pix / (uint32_t)C.width, col = pix % (uint32_t)C.width;
+        Rng rng{hash_seed(P.seed_camera, pix, P.iteration)};
+        const f2 off = rng.next_float2();
+        f3 o, d, raw;
+        camera_ray(C, (float)col + off.x, (float)row + off.y, &o, &d, &raw);
+        float tmin = 0.0f;
+        // CameraRaySample.Point (position, normal = Camera.Direction) and PdfRay
+        f3 prev_pos = o;
+        f3 prev_n = mk3(-C.c2w[8], -C.c2w[9], -C.c2w[10]);
+        float pdf_dir = pixel_jacobian(C, o + raw);
+        const int light_len = P.lv_len[pix];
+        const float ne = P.lv_ne[pix];
+        f3 L = gray(0.0f), prefix = gray(1.0f), approx = gray(1.0f);
+        for (int depth = 1; depth < P.max_depth + 1;) {
+            const HitOut h = traverse<false>(S, R.root, R.abs_max, o, d, tmin, INFINITY, stack_node + threadIdx.x,
+                                             stack_t + threadIdx.x);
+            wave_count(stats, kStatRays, true);
+            wave_count(stats, kStatHits, h.li >= 0);
+            if (h.li < 0) break;  // OnBackgroundHit without a background adds nothing
+            const Surf s = surface(S, h.li, h.u, h.v);
+            const float pdf_from = pdf_dir * area_to_solid_angle(prev_pos, s.pos, s.ng);
+            if (pdf_from == 0.0f) break;
+            const float rev_jac = area_to_solid_angle(s.pos, prev_pos, prev_n);
+            Shader sh;
+            make_shader(S, s, -d, &sh);
+            const int c = depth;  // camera vertices, this one included
+            P.cam[(size_t)(c - 1) * P.num_paths + pix] = make_float2(pdf_from, 0.0f);
+            f3 value = gray(0.0f);
+            const int ei = S.tri_emitter[h.li];
+            if (ei >= 0 && depth >= P.min_depth) {
+                const DevEmitter e = S.emitters[R.emitter_off + ei];
+                const f3 emission = emitted(e, s.ns, -d);
+                float mis = 1.0f;
+                if (c > 1) {
+                    const float pdf_emit = B.emitter_pdf(e, s.ns, -d, rev_jac);
+                    const float x_prev = c - 2 >= 1 ? B.cam(c - 2).y : 0.0f;
+                    float sum = 1.0f;
+                    sum += sdiv(B.next_event_pdf(e), pdf_from);
+                    sum += sdiv(B.camera_reciprocals(c - 2, pdf_emit, x_prev), pdf_from);
+                    mis = srcp(sum);
+                }
+                value = value + prefix * (mis * emission);
+            }
+            if (depth < P.max_depth && P.connections && light_len > 1) {
+                f3 conn = gray(0.0f);
+                Surf anc = B.vertex(0);
+                for (int k = 1; k < light_len; ++k) {
+                    const Surf lv = B.vertex(k);
+                    const int total = k + c + 1;
+                    if (total <= P.max_depth && total >= P.min_depth) {
+                        const f3 dd = s.pos - lv.pos;
+                        const f3 p0 = offset_point(lv, dd), p1 = offset_point(s, -dd);
+                        const bool occ = traverse<true>(S, R.root, R.abs_max, p0, p1 - p0, 1e-5f, 1.0f - 1e-5f,
+                                                        stack_node + threadIdx.x, stack_t + threadIdx.x)
+                                             .li >= 0;
+                        wave_count(stats, kStatShadow, true);
+                        wave_count(stats, kStatShadowTraced, true);
+                        wave_count(stats, kStatOccluded, occ);
+                        if (!occ) {
+                            const f3 to_anc = normalize(anc.pos - lv.pos);
+                            const f3 dir = normalize(lv.pos - s.pos);  // camera vertex -> light vertex
+                            Shader ls;
+                            make_shader(S, lv, to_anc, &ls);
+                            ls.light = true;
+                            f3 f_light = material_value(ls, -dir) * fabsf(dot(lv.ng, -dir));
+                            f_light = f_light * sdiv(fabsf(dot(lv.ns, to_anc)), fabsf(dot(lv.ng, to_anc)));
+                            f3 f_cam;
+                            float unused;
+                            eval_cos_pdf(sh, dir, &f_cam, &unused);
+                            float pcl, pcr, plc, plr;
+                            pdf_pair(sh, dir, &pcl, &pcr);
+                            pcr *= rev_jac;
+                            pcl *= area_to_solid_angle(s.pos, lv.pos, lv.ng);
+                            pdf_pair(ls, -dir, &plc, &plr);
+                            plr *= area_to_solid_angle(lv.pos, anc.pos, anc.ng);
+                            plc *= area_to_solid_angle(lv.pos, s.pos, s.ng);
+                            const float dist2 = dot(dd, dd);
+                            if (!is_zero(f_cam) && !is_zero(f_light) && pcl != 0.0f && dist2 != 0.0f) {
+                                float sum = 1.0f;
+                                sum += B.camera_reciprocals(c - 1, plc, pcr);
+                                sum += B.light_reciprocals(c, k, pcl, plr, ne);
+                                const float mis = srcp(sum);
+                                const f3 w = f3_of(B.va(k)) * f_light * f_cam / dist2 / 1.0f;
+                                conn = conn + mis * w;
+                            }
+                        }
+                    }
+                    anc = lv;
+                }
+                value = value + prefix * conn;
+            }
+            if (depth < P.max_depth && depth + 1 >= P.min_depth) {
+                for (int r = 0; r < P.num_shadow_rays; ++r) {
+                    const float bg = 0.0f;  // ComputeNextEventBackgroundProbability without a background
+                    if (rng.next_float() < bg) continue;
+                    const DevEmitter e = S.emitters[R.emitter_off + (int)rng.next_int((uint32_t)R.num_emitters)];
+                    const float prob = 1.0f / (float)R.num_emitters;
+                    const f2 b = to_uniform_triangle(rng.next_float2());
+                    const Surf lp = surface(S, e.tri, b.x, b.y);
+                    float lpdf = e.inv_area;
+                    lpdf *= prob;
+                    lpdf *= (float)P.num_shadow_rays;
+                    lpdf *= (1.0f - bg);
+                    if (lpdf == 0.0f) continue;
+                    const f3 dd = lp.pos - s.pos;
+                    const f3 p0 = offset_point(s, dd), p1 = offset_point(lp, -dd);
+                    const bool occ = traverse<true>(S, R.root, R.abs_max, p0, p1 - p0, 1e-5f, 1.0f - 1e-5f,
+                                                    stack_node + threadIdx.x, stack_t + threadIdx.x)
+                                         .li >= 0;
+                    wave_count(stats, kStatShadow, true);
+                    wave_count(stats, kStatShadowTraced, true);
+                    wave_count(stats, kStatOccluded, occ);
+                    if (occ) continue;
+                    const f3 l2s = normalize(s.pos - lp.pos);
+                    const f3 emission = emitted(e, lp.ns, l2s);
+                    f3 bc;
+                    float unused;
+                    eval_cos_pdf(sh, -l2s, &bc, &unused);
+                    const float jac = area_to_solid_angle(s.pos, lp.pos, lp.ng);
+                    float fwd, rev;
+                    pdf_pair(sh, -l2s, &fwd, &rev);
+                    fwd *= jac;
+                    rev *= rev_jac;
+                    if (is_zero(emission) || is_zero(bc) || jac == 0.0f || fwd == 0.0f || rev == 0.0f) continue;
+                    const float pdf_emit = B.emitter_pdf(e, lp.ns, l2s, area_to_solid_angle(lp.pos, s.pos, s.ng));
+                    float sum = 1.0f;
+                    sum += sdiv(fwd, lpdf);
+                    sum += sdiv(B.camera_reciprocals(c - 1, pdf_emit, rev), lpdf);
+                    const float mis = srcp(sum);
+                    const f3 w = emission * bc * sdiv(jac, lpdf);
+                    value = value + prefix * (mis * w);
+                }
+            }
+            L = L + value;
+            if (depth + 1 >= P.max_depth + 1) break;
+            const float q = depth > 4 ? clampf(average(approx), 0.05f, 0.95f) : 1.0f;
+            if (rng.next_float() > q) break;
+            const float pc = rng.next_float();
+            const f2 pd = rng.next_float2();
+            const BsdfSample bs = sample_bsdf<true>(sh, pc, pd);
+            approx = approx * (bs.weight / q);
+            P.cam[(size_t)(c - 1) * P.num_paths + pix].y = bs.pdf_rev * rev_jac;  // OnContinue
+            if (bs.pdf == 0.0f || is_zero(bs.weight)) break;
+            prefix = prefix * (bs.weight / q);
+            ++depth;
+            pdf_dir = bs.pdf;
+            prev_pos = s.pos;
+            prev_n = s.ng;
+            o = offset_point(s, bs.dir);
+            d = bs.dir;
+            tmin = s.err;
+        }
+        const bool bad = !isfinite(average(L));
+        wave_count(stats, kStatNan, bad);
+        if (!bad) {
+            const f3 add = div_exact(L, P.n);
+            float* px = image + 3ull * pix;
+            px[0] += add.x;
+            px[1] += add.y;
+            px[2] += add.z;
+        }
+    }
+}
+
 __global__ void __launch_bounds__(256) k_adjoint_finalize(DevScene S, Wave w) {
     const WaveDesc d = cur_wave(w);
     for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < d.count; slot += gridDim.x * blockDim.x) {
@@ -1998,6 +2393,18 @@ void launch_primary_rays(hipStream_t s, const DevScene& sc, int scene, uint32_t 
 void launch_light_paths(hipStream_t s, const DevScene& sc, int scene, const LightParams& p, float* image,
                         unsigned long long* stats) {
     hipLaunchKernelGGL(k_light_paths, dim3(blocks(p.num_paths, kTraceBlock)), dim3(kTraceBlock), 0, s, sc, scene, p,
+                       image, stats);
+}
+
+void launch_bidir_light(hipStream_t s, const DevScene& sc, int scene, const BidirParams& p, float* image,
+                        unsigned long long* stats) {
+    hipLaunchKernelGGL(k_bidir_light, dim3(blocks(p.num_paths, kTraceBlock)), dim3(kTraceBlock), 0, s, sc, scene, p,
+                       image, stats);
+}
+
+void launch_bidir_camera(hipStream_t s, const DevScene& sc, int scene, const BidirParams& p, float* image,
+                         unsigned long long* stats) {
+    hipLaunchKernelGGL(k_bidir_camera, dim3(blocks(p.num_paths, kTraceBlock)), dim3(kTraceBlock), 0, s, sc, scene, p,
                        image, stats);
 }
 

@@ -182,6 +182,52 @@ __device__ void generic_value_pdf(const Shader& sh, f3 in, f3* value, float* pdf
     *pdf_out = pdf0 * dw + pdf1 * sel_refl + pdf2 * sel_trans;
 }
 
+// The reverse pdf of GenericMaterial.ComputeValueAndPdf (GenericMaterial.cs:152-338):
+// the density of sampling `out` from `in` per lobe (pdfsRev), weighted with the
+// selection probabilities of the forward direction (selectTransmitIn repeats
+// (1 - fresnelT) of `out`, :311). Only the bidirectional MIS weights read it.
+__device__ float generic_pdf_rev(const Shader& sh, f3 in) {
+    const DevMaterial& m = sh.m;
+    Ggx ggx{sh.ax, sh.ay};
+    f3 out = sh.out_local;
+    float cos_o = fabsf(out.z), cos_i = fabsf(in.z);
+    float rev0 = 0.0f, rev1 = 0.0f, rev2 = 0.0f;
+    if (same_hemisphere(out, in)) rev0 = cos_o * kInvPi;
+    f3 h = out + in;
+    if (!is_zero(h)) {
+        h = normalize(h);
+        if (cos_i != 0.0f && cos_o != 0.0f) {
+            float jf = fabsf(4.0f * dot(out, h)), jr = fabsf(4.0f * dot(in, h));
+            if (jf != 0.0f && jr != 0.0f) rev1 = sdiv(ggx.pdf(in, h), jr);
+        }
+    }
+    float eta = out.z > 0.0f ? m.ior : srcp(m.ior);
+    f3 ht = out + in * eta;
+    if (cos_o != 0.0f && cos_i != 0.0f && !is_zero(ht) && m.spec_trans > 0.0f) {
+        ht = normalize(ht);
+        ht = ht.z < 0.0f ? -ht : ht;
+        float eta_in = in.z > 0.0f ? m.ior : srcp(m.ior);
+        f3 hin = normalize(out * eta_in + in);
+        hin = hin.z < 0.0f ? -hin : hin;
+        if (!is_zero(hin)) {
+            f3 wh = !same_hemisphere(in, hin) ? -hin : hin;
+            float sqrt_denom_in = dot(hin, in) + eta_in * dot(out, hin);
+            if (sqrt_denom_in != 0.0f) {
+                float jac = sdiv(eta_in * eta_in * fmaxf(0.0f, dot(out, -wh)), sqrt_denom_in * sqrt_denom_in);
+                rev2 += ggx.pdf(in, wh) * jac;
+            }
+        }
+    }
+    float schl = average(schlick_fresnel(sh.spec_r0, fabsf(out.z)));
+    float bias = fmaxf(1.0f - schl, 0.75f);
+    float dw = bias * (1.0f - m.metallic) * (1.0f - m.spec_trans);
+    float fr = fresnel_dielectric(dot(out, h.z < 0.0f ? -h : h), 1.0f, m.ior);
+    float sel_refl = (1.0f - dw) * (1.0f - (1.0f - fr) * m.spec_trans);
+    float ft = fresnel_dielectric(dot(out, ht), 1.0f, m.ior);
+    float sel_trans = (1.0f - dw) * (1.0f - ft) * m.spec_trans;
+    return rev0 * dw + rev1 * sel_refl + rev2 * sel_trans;
+}
+
 // DiffuseMaterial.Evaluate / Pdf (DiffuseMaterial.cs:127-139,167-188)
 __device__ __forceinline__ void diffuse_value_pdf(const Shader& sh, f3 in_world, f3* value, float* pdf) {
     bool refl = should_reflect(sh.ng, sh.out_world, in_world);
@@ -203,6 +249,32 @@ __device__ __forceinline__ void diffuse_value_pdf(const Shader& sh, f3 in_world,
     }
 }
 
+// DiffuseBsdf.Pdf / DiffuseTransmission.Pdf (DiffuseMaterial.cs:63-71,95-103): the
+// reverse pdf, |cos| of the outgoing direction / pi on the lobe's side.
+__device__ __forceinline__ float diffuse_pdf_rev(const Shader& sh, f3 in_world) {
+    f3 in = sh.f.to_local(in_world);
+    bool same = same_hemisphere(sh.out_local, in);
+    float pr = same ? fabsf(sh.out_local.z) * kInvPi : 0.0f;
+    if (sh.m.transmitter) {
+        float pt = !same ? fabsf(sh.out_local.z) * kInvPi : 0.0f;
+        return (pr + pt) * 0.5f;
+    }
+    return pr;
+}
+
+// SurfaceShader.Pdf(inDir) (SurfaceShader.cs:64-67): forward and reverse pdfs.
+__device__ __forceinline__ void pdf_pair(const Shader& sh, f3 in_world, float* fwd, float* rev) {
+    f3 v;
+    if (sh.m.kind == 1) {
+        f3 in = sh.f.to_local(in_world);
+        generic_value_pdf(sh, in, &v, fwd);
+        *rev = generic_pdf_rev(sh, in);
+    } else {
+        diffuse_value_pdf(sh, in_world, &v, fwd);
+        *rev = diffuse_pdf_rev(sh, in_world);
+    }
+}
+
 // SurfaceShader.EvaluateWithCosine + Pdf(..).Pdf for a world direction.
 __device__ __forceinline__ void eval_cos_pdf(const Shader& sh, f3 in_world, f3* value_cos, float* pdf) {
     if (sh.m.kind == 1) {
@@ -221,11 +293,14 @@ struct BsdfSample {
     f3 dir;
     float pdf;
     f3 weight;
+    float pdf_rev;  // BsdfSample.PdfReverse: set by sample_bsdf<true> only
 };
 
-// SurfaceShader.Sample (GenericMaterial.cs:92-150, DiffuseMaterial.cs:141-165)
+// SurfaceShader.Sample (GenericMaterial.cs:92-150, DiffuseMaterial.cs:141-165);
+// REV also computes the reverse pdf of the sampled direction.
+template <bool REV = false>
 __device__ BsdfSample sample_bsdf(const Shader& sh, float pc, f2 pd) {
-    BsdfSample bad{gray(0.0f), 0.0f, gray(0.0f)};
+    BsdfSample bad{gray(0.0f), 0.0f, gray(0.0f), 0.0f};
     const DevMaterial& m = sh.m;
     if (m.kind == 1) {
         f3 out = sh.out_local;
@@ -260,6 +335,7 @@ __device__ BsdfSample sample_bsdf(const Shader& sh, float pc, f2 pd) {
         bs.pdf = pdf;
         bs.weight = val * fabsf(in.z) / pdf;
         bs.dir = sh.f.to_world(in);
+        bs.pdf_rev = REV ? generic_pdf_rev(sh, in) : 0.0f;
         return bs;
     }
     f3 d = to_cos_hemisphere(pd);
@@ -273,7 +349,7 @@ __device__ BsdfSample sample_bsdf(const Shader& sh, float pc, f2 pd) {
     float pdf;
     eval_cos_pdf(sh, dir, &vc, &pdf);
     if (pdf == 0.0f) return bad;
-    return {dir, pdf, vc / pdf};
+    return {dir, pdf, vc / pdf, REV ? diffuse_pdf_rev(sh, dir) : 0.0f};
 }
 
 // ------------------------------------------------------------ environment --

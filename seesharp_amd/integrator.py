@@ -281,6 +281,13 @@ class GpuScene(_TextureParameters):
             self.device.ptr, self.ptr, C.byref(settings), first_iteration, num_iterations, C.byref(stats)))
         return stats.as_dict()
 
+    def render_bidir(self, settings: abi.BidirSettings, first_iteration: int, num_iterations: int) -> dict:
+        """Iterations first..first+num-1 of ClassicBidir into the framebuffer (running mean)."""
+        stats = abi.RayStats()
+        self.device.check(self.device.lib.sspt_render_bidir(
+            self.device.ptr, self.ptr, C.byref(settings), first_iteration, num_iterations, C.byref(stats)))
+        return stats.as_dict()
+
     def render_debug(self, base_seed: int, first_iteration: int, num_iterations: int):
         """DebugVisualizer iterations into the running-mean framebuffer."""
         self.device.check(self.device.lib.sspt_render_debug(self.device.ptr, self.ptr, 0, base_seed & 0xFFFFFFFF,
@@ -626,6 +633,54 @@ class VertexCacheBidirLightTracer(Integrator):
         gs.clear()
         t0 = time.perf_counter()
         stats = gs.render_light_tracer(self.settings(), 0, self.NumIterations)
+        fb.MetaData["RenderTime"] = (time.perf_counter() - t0) * 1000.0
+        fb.MetaData["NumIterations"] = self.NumIterations
+        fb.MetaData["RayTracerStats"] = {
+            "NumRays": stats["num_rays"], "NumShadowRays": stats["num_shadow_rays"],
+            "NumOccluded": stats["num_occluded"], "NumRayHits": stats["num_ray_hits"]}
+        fb.Image = gs.read_image()
+        fb.CurIteration = self.NumIterations
+        fb.NaNWarnings = gs.nan_count()
+        return fb
+
+
+class ClassicBidir(Integrator):
+    """Classic bidirectional path tracing (SeeSharp/Integrators/Bidir/ClassicBidir.cs)
+    rendered by BidirBase.Render (SeeSharp/Integrators/Bidir/BidirBase.cs:84-146):
+    every iteration traces one light path per pixel, splats its vertices onto
+    the camera (EnableLightTracer), and traces one camera path per pixel that
+    adds emitter hits, NumShadowRays next-event samples and, with
+    EnableConnections, connections to every vertex of its pixel's light path,
+    all combined by the balance heuristic. Scenes with a background are not
+    supported."""
+
+    def __init__(self, device: Device | None = None):
+        super().__init__()
+        self.NumIterations = 2
+        self.NumLightPaths = -1
+        self.BaseSeedCamera = 0xC030114
+        self.BaseSeedLight = 0x13C0FEFE
+        self.NumShadowRays = 1
+        self.EnableConnections = True
+        self.EnableLightTracer = True
+        self.device = device
+        self._gpu_scene = None
+
+    def settings(self) -> abi.BidirSettings:
+        return abi.BidirSettings(self.BaseSeedCamera & 0xFFFFFFFF, self.BaseSeedLight & 0xFFFFFFFF, self.MaxDepth,
+                                 self.MinDepth, self.NumShadowRays, 1 if self.EnableConnections else 0,
+                                 1 if self.EnableLightTracer else 0, int(self.NumLightPaths))
+
+    def Render(self, scene):
+        if self.device is None:
+            self.device = Device(0)
+        if self._gpu_scene is None or self._gpu_scene.scene is not scene:
+            self._gpu_scene = GpuScene(self.device, scene)
+        gs = self._gpu_scene
+        fb = scene.FrameBuffer
+        gs.clear()
+        t0 = time.perf_counter()
+        stats = gs.render_bidir(self.settings(), 0, self.NumIterations)
         fb.MetaData["RenderTime"] = (time.perf_counter() - t0) * 1000.0
         fb.MetaData["NumIterations"] = self.NumIterations
         fb.MetaData["RayTracerStats"] = {

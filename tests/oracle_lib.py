@@ -41,6 +41,8 @@ _SIGS = {
                                 _fp, _fp, _fp, _fp]),
     "or_material_sample": (None, [C.POINTER(abi.MaterialDesc), C.POINTER(abi.TextureDesc), _fp, _fp, _fp,
                                   C.c_float, _fp, _fp]),
+    "or_material_pdf_pair": (None, [C.POINTER(abi.MaterialDesc), C.POINTER(abi.TextureDesc), _fp, _fp, _fp,
+                                    _fp, _fp, _fp]),
     "or_material_eval_many": (None, [C.POINTER(abi.MaterialDesc), C.POINTER(abi.TextureDesc), _fp, _fp, _fp,
                                      C.c_int64, _fp, _fp, _fp]),
     "or_material_integrate": (None, [C.POINTER(abi.MaterialDesc), C.POINTER(abi.TextureDesc), _fp, _fp, _fp,
@@ -58,6 +60,8 @@ _SIGS = {
                                C.POINTER(abi.RayStats), C.POINTER(C.c_int32), C.c_int64, C.c_int32]),
     "or_render_light_tracer": (C.c_uint64, [_vp, C.POINTER(abi.LtSettings), C.c_uint32, C.c_uint32, _fp,
                                             C.POINTER(abi.RayStats), C.c_int32]),
+    "or_render_bidir": (C.c_uint64, [_vp, C.POINTER(abi.BidirSettings), C.c_uint32, C.c_uint32, _fp,
+                                     C.POINTER(abi.RayStats), C.c_int32]),
     "or_render_pixel": (None, [_vp, C.POINTER(abi.PtSettings), C.c_uint32, C.c_uint32, C.c_uint32, _fp]),
     "or_set_texture": (C.c_int32, [_vp, C.c_uint32, _fp]),
     "or_render_outliers": (None, [_vp, C.POINTER(abi.PtSettings), C.c_uint32, C.c_uint32, C.c_int32, _fp,
@@ -98,6 +102,12 @@ def fp(a):
 def lt_settings(base_seed_light=0x13C0FEFE, num_light_paths=-1, max_depth=5, min_depth=1):
     # (MaxDepth defaults to 100 in the reference's Integrator; the tests pass small depths)
     return abi.LtSettings(base_seed_light, num_light_paths, max_depth, min_depth)
+
+
+def bidir_settings(base_seed_camera=0xC030114, base_seed_light=0x13C0FEFE, max_depth=5, min_depth=1,
+                   num_shadow_rays=1, enable_connections=True, enable_light_tracer=True, num_light_paths=-1):
+    return abi.BidirSettings(base_seed_camera, base_seed_light, max_depth, min_depth, num_shadow_rays,
+                             1 if enable_connections else 0, 1 if enable_light_tracer else 0, num_light_paths)
 
 
 def settings(base_seed=0xC030114, total_spp=1, max_depth=5, min_depth=1, num_shadow_rays=1,
@@ -147,6 +157,17 @@ class OracleScene:
         nans = lib().or_render_light_tracer(self.ptr, C.byref(cfg), first, num, fp(image), C.byref(stats), threads)
         if nans == 2 ** 64 - 1:
             raise ValueError("the light tracer needs emitters and no background")
+        return image, stats, int(nans)
+
+    def render_bidir(self, cfg, first=0, num=1, image=None, threads=0):
+        """ClassicBidir's running-mean image after iterations first..first+num-1,
+        its ray statistics and the non-finite value count."""
+        if image is None:
+            image = np.zeros((self.height, self.width, 3), np.float32)
+        stats = abi.RayStats()
+        nans = lib().or_render_bidir(self.ptr, C.byref(cfg), first, num, fp(image), C.byref(stats), threads)
+        if nans == 2 ** 64 - 1:
+            raise ValueError("ClassicBidir needs emitters, no background and one light path per pixel")
         return image, stats, int(nans)
 
     def primary_aov(self, cfg, first=0, num=1, threads=0):

@@ -249,7 +249,7 @@ def evaluate(m, out_dir, in_dir):
 
 
 def sample(m, out_dir, pc, p2):
-    out = arr(7)
+    out = arr(8)
     L.or_material_sample(C.byref(m), None, f3(*NORMAL), f3(*UV), f3(*out_dir), pc, f3(*p2), out)
     return np.array(out[:3]), out[3], np.array(out[4:7])
 
