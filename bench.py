@@ -18,6 +18,7 @@ reference, timed on a bounded pixel sample).
     python bench.py --workload copy      # plain 1 GiB device copy: HBM counter calibration for rocprofv3 --pmc
     python bench.py --workload cornell   # the reference's CornellBox.json 512^2 512 spp render vs its recorded times
     python bench.py --workload bvh       # device BVH build of the 1M-tri scene vs the host builder
+    python bench.py --workload bidir     # ClassicBidir and the light tracer on the 1M-tri scene without its sky
 """
 from __future__ import annotations
 
@@ -469,6 +470,87 @@ def bvh_main(args):
     print(json.dumps(out), flush=True)
 
 
+def bidir_main(args):
+    """Bidirectional rendering on the 1 036 296-triangle large scene without its
+    sky (its area lights only), 1024^2, MaxDepth 8: ClassicBidir with every
+    technique on (one light path and one camera path per pixel and iteration,
+    NumShadowRays 1), `spp` iterations per step; the light tracer alone
+    (VertexCacheBidir as a pure light tracer) beside it. Kernel times from HIP
+    events; the oracle's ClassicBidir timed on one iteration of the same scene."""
+    from seesharp_amd import Device, GpuScene, abi, scenes
+
+    dev = Device(0)
+    scene = scenes.large_scene(args.res, args.res, with_env=False)
+    g = GpuScene(dev, scene)
+    iters = max(1, args.spp // 8)
+    cfg = abi.BidirSettings(0xC030114, 0x13C0FEFE, args.max_depth, 1, 1, 1, 1, -1)
+    for _ in range(max(0, args.warmup)):
+        g.clear()
+        g.render_bidir(cfg, 0, iters)
+    dev.set_kernel_timing(True)
+    dev.reset_kernel_timing()
+    ms = []
+    for _ in range(max(1, args.steps)):
+        g.clear()
+        t0 = time.perf_counter()
+        st = g.render_bidir(cfg, 0, iters)
+        ms.append((time.perf_counter() - t0) * 1000.0)
+    kt = {k: dev.kernel_timing(k) for k in ("bidir_light", "bidir_camera")}
+    dev.reset_kernel_timing()
+    lt = abi.LtSettings(0x13C0FEFE, -1, args.max_depth, 1)
+    g.clear()
+    g.render_light_tracer(lt, 0, 1)
+    dev.reset_kernel_timing()
+    g.clear()
+    lst = g.render_light_tracer(lt, 0, iters)
+    lt_ms = dev.kernel_timing("light_paths")
+    dev.set_kernel_timing(False)
+    npx = args.res * args.res
+    step_ms = float(np.mean(ms))
+    rays = st["num_rays"] + st["num_shadow_rays"]
+    log(f"[bench] ClassicBidir {iters} iterations: {step_ms:.2f} ms; light kernel "
+        f"{kt['bidir_light'][1] / max(1, kt['bidir_light'][0]):.2f} ms, camera kernel "
+        f"{kt['bidir_camera'][1] / max(1, kt['bidir_camera'][0]):.2f} ms per iteration")
+    out = {
+        "metric": "ClassicBidir samples per second (one light + one camera path per pixel sample)",
+        "value": round(npx * iters / step_ms / 1e3, 3),
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": max(1, args.steps),
+        "warmup": args.warmup,
+        "ms_per_step": round(step_ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"large scene ({g.info()['num_triangles']} triangles) without its sky, "
+                               f"{args.res}x{args.res}, ClassicBidir MaxDepth {args.max_depth}, connections + "
+                               f"light tracer + 1 shadow ray, {iters} iterations per step",
+                   "max_depth": args.max_depth, "iterations_per_step": iters},
+        "ray_stats_per_step": st,
+        "mrays_per_s": round(rays / step_ms / 1e3, 1),
+        "kernel_ms_per_iteration": {k: round(v[1] / max(1, v[0]), 3) for k, v in kt.items()},
+        "light_tracer": {"ms_per_iteration": round(lt_ms[1] / max(1, lt_ms[0]), 3),
+                         "mpaths_per_s": round(npx * lt_ms[0] / lt_ms[1] / 1e3, 2) if lt_ms[1] else None,
+                         "ray_stats_per_step": lst},
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as ol
+
+        threads = min(16, len(os.sched_getaffinity(0)))
+        o = ol.OracleScene(scene)
+        t0 = time.perf_counter()
+        o.render_bidir(ol.bidir_settings(max_depth=args.max_depth), 0, 1, threads=threads)
+        dt = time.perf_counter() - t0
+        o.close()
+        out["cpu_baseline"] = {"value": round(npx / dt / 1e6, 4), "unit": "Msamples/s", "cores": threads,
+                               "kind": "port", "sample": f"one iteration of the same render ({npx} samples, "
+                                                         f"{dt:.1f} s)"}
+    print(json.dumps(out), flush=True)
+
+
 def cornell_main(args):
     """The reference's recorded render: CornellBox.json (36 triangles) at 512^2,
     512 spp, BaseSeed 571298512, MaxDepth 5 and 2, denoiser feature layers on
@@ -533,7 +615,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the adjoint (textured box, 32 views) and batching measurements")
-    ap.add_argument("--workload", choices=("large", "batch", "adjoint", "views", "copy", "cornell", "bvh"),
+    ap.add_argument("--workload", choices=("large", "batch", "adjoint", "views", "copy", "cornell", "bvh", "bidir"),
                     default="large")
     ap.add_argument("--no-autotune", action="store_true",
                     help="batching: keep the default block shape, wave capacity and grids instead of tuning them")
@@ -562,6 +644,8 @@ def main():
     if args.workload == "cornell":
         cornell_main(args)
         return
+    if args.workload == "bidir":
+        return bidir_main(args)
     if args.workload == "bvh":
         bvh_main(args)
         return

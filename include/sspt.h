@@ -504,6 +504,57 @@ typedef struct sspt_bidir_settings {
 sspt_status sspt_render_bidir(sspt_ctx* ctx, sspt_scene* scene, const sspt_bidir_settings* settings,
                               uint32_t first_iteration, uint32_t num_iterations, sspt_ray_stats* stats);
 
+/* ---- light vertex cache and merging: VertexCacheBidir
+ *      (SeeSharp/Integrators/Bidir/VertexCacheBidir.cs) and VertexConnectionAndMerging
+ *      (SeeSharp/Integrators/Bidir/VertexConnectionAndMerging.cs) over BidirBase.Render. --
+ * Light paths and the light tracer as sspt_render_bidir, but num_light_paths of
+ * them (< 0: one per pixel) and the light tracer weighted against the
+ * techniques below. Each camera vertex below MaxDepth makes num_connections
+ * connections, each to one vertex drawn uniformly (RNG.NextInt of the camera
+ * path's generator) from all cached vertices in path order, light points
+ * included, which connect to nothing (VertexSelector.cs,
+ * BidirBase.Camera.cs:279-291); the connection's MIS density is
+ * num_connections * num_light_paths / vertex count (BidirSelectDensity,
+ * VertexCacheBidir.cs:71-82). enable_hitting switches the emitter hits of
+ * camera paths (VCM keeps those of primary hits). SSPT_VCM adds merging
+ * (PerformMerging / MergeHelper / Merge, VertexConnectionAndMerging.cs:329-516): at every
+ * camera vertex past the primary hit (or at it too with merge_primary), the
+ * max_num_photons light vertices (of depth >= 1 and non-zero weight) nearest
+ * to it within the radius of the camera path's primary pixel footprint
+ * (sqrt(1 / area pdf of the primary hit)) are merged with the Epanechnikov
+ * kernel, whose radius is the distance of the furthest found photon when
+ * max_num_photons were found. The k nearest are the stored points with the k
+ * smallest squared distances (fp32, dx*dx + dy*dy + dz*dz) below the squared
+ * radius, ties to the earlier photon: the query of TinyEmbree 1.2.0's
+ * NearestNeighborSearch.ForAllNearest, restated (the package is not part of
+ * the reference's sources). All techniques are combined by the balance
+ * heuristic, for SSPT_VCM with the correlation-aware merge weights (Grittmann
+ * et al. 2021, CorrelAwareRatios / MergeMis :580-697) unless disable_correl_aware_mis.
+ * Scenes with a background or without emitters, batches, and
+ * max_num_photons outside 1..32 are rejected. */
+typedef enum sspt_vcm_algorithm {
+    SSPT_VERTEX_CACHE_BIDIR = 0,  /* VertexCacheBidir */
+    SSPT_VCM = 1                  /* VertexConnectionAndMerging */
+} sspt_vcm_algorithm;
+typedef struct sspt_vcm_settings {
+    int32_t algorithm;                 /* sspt_vcm_algorithm */
+    uint32_t base_seed_camera;         /* BaseSeedCamera, default 0xC030114 */
+    uint32_t base_seed_light;          /* BaseSeedLight, default 0x13C0FEFE */
+    int32_t max_depth;                 /* MaxDepth (Integrator default 100) */
+    int32_t min_depth;                 /* MinDepth (default 1) */
+    int32_t num_light_paths;           /* NumLightPaths, < 0 (default) = one per pixel */
+    int32_t num_connections;           /* NumConnections (default 1) */
+    int32_t num_shadow_rays;           /* NumShadowRays (default 1) */
+    int32_t enable_light_tracer;       /* EnableLightTracer (default 1) */
+    int32_t enable_hitting;            /* EnableHitting (default 1) */
+    int32_t enable_merging;            /* VCM: EnableMerging (default 1) */
+    int32_t merge_primary;             /* VCM: MergePrimary (default 0) */
+    int32_t max_num_photons;           /* VCM: MaxNumPhotons (default 8) */
+    int32_t disable_correl_aware_mis;  /* VCM: DisableCorrelAwareMIS (default 0) */
+} sspt_vcm_settings;
+sspt_status sspt_render_vcm(sspt_ctx* ctx, sspt_scene* scene, const sspt_vcm_settings* settings,
+                            uint32_t first_iteration, uint32_t num_iterations, sspt_ray_stats* stats);
+
 /* ---- measurement ------------------------------------------------------- */
 /* Enables HIP-event timing of every kernel launch on the context stream. */
 sspt_status sspt_set_kernel_timing(sspt_ctx* ctx, int32_t enable);

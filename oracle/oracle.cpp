@@ -31,6 +31,7 @@
 #include <cstdio>
 #include <cstring>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 namespace {
@@ -2675,6 +2676,591 @@ uint64_t or_render_bidir(const or_scene* s, const sspt_bidir_settings* cfg, uint
         if (cfg->enable_light_tracer) run(1);  // ProcessPathCache: SplatLightVertices
         run(2);  // TraceAllCameraPaths
         // FrameBuffer.Splat (FrameBuffer.cs:160-191), IgnoreNanAndInf
+        auto add = [&](int64_t pix, Rgb value) {
+            if (!std::isfinite(average(value))) {
+                ++nans;
+                return;
+            }
+            Rgb a = value / n;
+            image[pix * 3 + 0] += a.x;
+            image[pix * 3 + 1] += a.y;
+            image[pix * 3 + 2] += a.z;
+        };
+        if (cfg->enable_light_tracer)
+            for (uint32_t k = 0; k < paths; ++k)
+                for (const Splat& sp : splats[k]) add(sp.pix, sp.value);
+        for (int64_t k = 0; k < W * H; ++k) add(k, values[(size_t)k]);
+    }
+    if (stats) {
+        stats->num_rays = rays;
+        stats->num_shadow_rays = shadow;
+        stats->num_occluded = occl;
+        stats->num_ray_hits = hits;
+        stats->num_shadow_rays_traced = shadow;
+        stats->num_iterations = num;
+    }
+    return nans;
+}
+
+// ----------------------------- light vertex cache and vertex merging (VCM) -
+namespace {
+
+struct VcmCtx {
+    const or_scene* s;
+    const sspt_vcm_settings* cfg;
+    sspt_bidir_settings light_cfg;  // the fields the shared light-path and next-event helpers read
+    bool vcm;                       // VertexConnectionAndMerging semantics
+    float num_paths;                // NumLightPaths
+    const std::vector<std::vector<CachedVertex>>* cache;
+    const std::vector<int64_t>* cum;  // PathCache.cumPathLen: inclusive prefix sums of the path lengths
+    int64_t num_vertices;             // PathCache.NumVertices
+    Counters* c;
+    Pdfs pdfs;
+    // the photon map (NearestNeighborSearch over the positions of the light vertices of
+    // depth >= 1 with non-zero weight, in path order) as a uniform grid
+    const std::vector<V3>* photon_pos;
+    const std::vector<std::pair<int32_t, int32_t>>* photon_ref;  // (path, vertex)
+    const std::unordered_map<int64_t, std::vector<int32_t>>* grid;
+    float cell;
+    BidirCtx light() const { return BidirCtx{s, &light_cfg, num_paths, cache, c, {}}; }
+};
+
+// VertexCacheBidir.BidirSelectDensity (VertexCacheBidir.cs:71-82)
+float select_density(const VcmCtx& vc) {
+    if (vc.num_vertices == 0) return 0.0f;
+    float select_prob = 1.0f / (float)vc.num_vertices;
+    float num_samples = (float)(vc.cfg->num_connections * (int32_t)vc.num_paths);
+    return select_prob * num_samples;
+}
+
+// VertexConnectionAndMerging.CorrelAwareRatios (VertexConnectionAndMerging.cs:580-646)
+struct CorrelRatios {
+    std::vector<float> c2l, l2c;
+    void init(const Pdfs& p, float dist_to_cam, bool from_background) {
+        float radius = dist_to_cam * 0.0174550649f;  // distance * tan(1 degree)
+        float accept_area = radius * radius * kPi;
+        int nsv = p.n() - 1;
+        c2l.assign((size_t)std::max(nsv, 0), 0.0f);
+        l2c.assign((size_t)std::max(nsv, 0), 0.0f);
+        float product = 1.0f;
+        for (int i = 0; i < nsv; ++i) {
+            product *= std::min(p.c2l[(size_t)i] * accept_area, 1.0f);
+            c2l[(size_t)i] = product;
+        }
+        product = 1.0f;
+        for (int i = nsv - 1; i >= 0; --i) {
+            float next = p.l2c[(size_t)i] * accept_area;
+            if (i == nsv - 1 && !from_background) next *= accept_area;
+            product *= std::min(next, 1.0f);
+            l2c[(size_t)i] = product;
+        }
+    }
+    float operator[](int idx) const {
+        if (idx == 0) return 1.0f;  // primary merges are not correlated
+        float light = l2c[(size_t)idx], cam = c2l[(size_t)idx];
+        if (cam == 0.0f && light == 0.0f) return 1.0f;
+        return cam / (cam + light - cam * light);
+    }
+};
+
+// CameraPathReciprocals: VertexCacheBidir.cs:210-223, VertexConnectionAndMerging.cs:839-889
+float vcm_camera_reciprocals(const VcmCtx& vc, const Pdfs& p, int last, float radius, const CorrelRatios& cr) {
+    const sspt_vcm_settings& cfg = *vc.cfg;
+    const bool merging = vc.vcm && cfg.enable_merging;
+    float sum = 0.0f, next = 1.0f;
+    for (int i = last; i > 0; --i) {
+        if (merging) {
+            float accept = p.l2c[(size_t)i] * kPi * radius * radius;
+            if (!cfg.disable_correl_aware_mis) accept *= cr[i];
+            sum += next * vc.num_paths * accept;
+        }
+        next *= p.l2c[(size_t)i] / p.c2l[(size_t)i];
+        if (cfg.num_connections > 0) sum += next * select_density(vc);
+    }
+    if (cfg.enable_light_tracer) sum += next * p.l2c[0] / p.c2l[0] * vc.num_paths;
+    if (merging && cfg.merge_primary) sum += next * vc.num_paths * p.l2c[0] * kPi * radius * radius;
+    return sum;
+}
+
+// LightPathReciprocals: VertexCacheBidir.cs:225-237, VertexConnectionAndMerging.cs:891-929
+float vcm_light_reciprocals(const VcmCtx& vc, const Pdfs& p, int last, float radius, const CorrelRatios& cr) {
+    const sspt_vcm_settings& cfg = *vc.cfg;
+    const bool merging = vc.vcm && cfg.enable_merging;
+    const int n = p.n();
+    float sum = 0.0f, next = 1.0f;
+    for (int i = last + 1; i < n; ++i) {
+        if (i == n - 1) sum += next * p.ne / p.l2c[(size_t)i];
+        if (merging && i < n - 1 && (cfg.merge_primary || i > 0)) {
+            float accept = p.c2l[(size_t)i] * kPi * radius * radius;
+            if (!cfg.disable_correl_aware_mis) accept *= cr[i];
+            sum += next * vc.num_paths * accept;
+        }
+        next *= p.c2l[(size_t)i] / p.l2c[(size_t)i];
+        if (i < n - 2 && cfg.num_connections > 0) sum += next * select_density(vc);
+    }
+    if (cfg.enable_hitting) sum += next;
+    return sum;
+}
+
+// ConnectLightVertexToCamera (BidirBase.Light.cs:35-92) with LightTracerMis
+// (VertexCacheBidir.cs:183-186, VertexConnectionAndMerging.cs:735-766)
+void vcm_splat(VcmCtx& vc, const std::vector<CachedVertex>& path, size_t k, std::vector<Splat>* out) {
+    const or_scene& s = *vc.s;
+    const CachedVertex& v = path[k];
+    const CachedVertex& anc = path[k - 1];
+    V3 to_anc = normalize(anc.p.position - v.p.position);
+    if (v.depth + 1 < vc.cfg->min_depth) return;
+    int px, py;
+    float jac;
+    if (!s.camera.sample_response(v.p.position, v.p.normal, &px, &py, &jac)) return;
+    vc.c->shadow++;
+    if (trace_any(s, shadow_ray_to(v.p, s.camera.position), false)) {
+        vc.c->occluded++;
+        return;
+    }
+    V3 to_cam = s.camera.position - v.p.position;
+    float dist = length(to_cam);
+    to_cam = to_cam / dist;
+    Shader sh = make_shader(s, v.p, to_cam);
+    Rgb f = eval_nocos(sh, to_anc);
+    if (is_black(f)) return;
+    f = f * (std::fabs(dot(v.p.shading_normal, to_anc)) / std::fabs(dot(v.p.normal, to_anc)));
+    float pdf_rev, unused;
+    pdf_pair(sh, to_anc, &pdf_rev, &unused);
+    pdf_rev *= area_to_solid_angle(v.p, anc.p);
+    Pdfs& p = vc.pdfs;
+    p.reset(v.depth + 1);
+    p.gather_light(path, v, -1);
+    p.c2l[0] = 1.0f * jac;
+    p.c2l[1] = pdf_rev;
+    if (v.depth == 1) p.ne = next_event_pdf(vc.light(), anc.p);
+    float mis;
+    if (vc.vcm) {
+        CorrelRatios cr;
+        cr.init(p, dist, false);
+        float footprint = std::sqrt(1.0f / p.c2l[0]);
+        float sum = vcm_light_reciprocals(vc, p, -1, footprint, cr);
+        sum /= vc.num_paths;
+        sum += 1.0f;
+        mis = 1.0f / sum;
+    } else {
+        CorrelRatios none;
+        mis = 1.0f / (1.0f + vcm_light_reciprocals(vc, p, -1, 0.0f, none) / vc.num_paths);
+    }
+    Rgb w = v.weight * f * rgb(jac) / vc.num_paths;
+    out->push_back({(int64_t)py * s.width + px, mis * w});
+}
+
+// The camera path's state the techniques read (CameraPath, BidirBase.Camera.cs:24-68)
+struct VcmPath {
+    std::vector<PdfPair> v;
+    float footprint = 0.0f;  // FootprintRadius
+    float dist0 = 0.0f;      // Distances[0]
+};
+
+// OnEmitterHit (BidirBase.Camera.cs:500-524) with EmitterHitMis (VertexCacheBidir.cs:171-180,
+// VertexConnectionAndMerging.cs:700-732)
+Rgb vcm_emitter_hit(VcmCtx& vc, const Emitter& e, const SurfacePoint& hit, V3 out_dir, const VcmPath& cp,
+                    float rev_jac) {
+    Rgb emission = emitted(e, hit, out_dir);
+    BidirCtx bc = vc.light();
+    float pdf_emit = emitter_pdf(bc, e, hit, out_dir, rev_jac);
+    float pdf_ne = next_event_pdf(bc, hit);
+    const int n = (int)cp.v.size();
+    Pdfs& p = vc.pdfs;
+    p.reset(n);
+    p.gather_camera(cp.v, n - 1);
+    if (n > 1) p.l2c[(size_t)n - 2] = pdf_emit;
+    p.ne = pdf_ne;
+    p.c2l[(size_t)n - 1] = cp.v.back().from;
+    float mis = 1.0f;
+    if (n > 1) {
+        CorrelRatios cr;
+        if (vc.vcm) cr.init(p, cp.dist0, false);
+        float pdf_this = p.c2l[(size_t)n - 1];
+        float sum = 1.0f;
+        sum += p.ne / pdf_this;
+        sum += vcm_camera_reciprocals(vc, p, n - 2, cp.footprint, cr) / pdf_this;
+        mis = 1.0f / sum;
+    }
+    return mis * emission;
+}
+
+// PerformNextEventEstimation (BidirBase.Camera.cs:368-484) with NextEventMis
+// (VertexCacheBidir.cs:200-207, VertexConnectionAndMerging.cs:802-836)
+Rgb vcm_next_event(VcmCtx& vc, const Shader& sh, Rng& rng, const VcmPath& cp, float rev_jac) {
+    const or_scene& s = *vc.s;
+    const int n = (int)cp.v.size() + 1;
+    Pdfs& p = vc.pdfs;
+    p.reset(n);
+    p.gather_camera(cp.v, n - 2);
+    p.c2l[(size_t)n - 2] = cp.v.back().from;
+    const float bg = 0.0f;
+    if (rng.next_float() < bg) return rgb(0.0f);
+    int count = (int)s.emitters.size();
+    const Emitter& e = s.emitters[rng.next_int((uint32_t)count)];
+    float prob = 1.0f / (float)count;
+    SurfacePoint lp = make_point(s, e.tri, to_uniform_triangle(rng.next_float2()), 0.0f);
+    float lpdf = e.inv_area;
+    lpdf *= prob;
+    lpdf *= (float)vc.cfg->num_shadow_rays;
+    lpdf *= (1.0f - bg);
+    if (lpdf == 0.0f) return rgb(0.0f);
+    vc.c->shadow++;
+    if (trace_any(s, shadow_ray(sh.p, lp), false)) {
+        vc.c->occluded++;
+        return rgb(0.0f);
+    }
+    V3 l2s = normalize(sh.p.position - lp.position);
+    Rgb emission = emitted(e, lp, l2s);
+    if (is_black(emission)) return rgb(0.0f);
+    Rgb bsdf_cos = eval_cos(sh, -l2s);
+    if (is_black(bsdf_cos)) return rgb(0.0f);
+    float jac = area_to_solid_angle(sh.p, lp);
+    if (jac == 0.0f) return rgb(0.0f);
+    float fwd, rev;
+    pdf_pair(sh, -l2s, &fwd, &rev);
+    fwd *= area_to_solid_angle(sh.p, lp);
+    rev *= rev_jac;
+    if (fwd == 0.0f || rev == 0.0f) return rgb(0.0f);
+    float pdf_emit = emitter_pdf(vc.light(), e, lp, l2s, area_to_solid_angle(lp, sh.p));
+    p.l2c[(size_t)n - 2] = pdf_emit;
+    if (n > 2) p.l2c[(size_t)n - 3] = rev;
+    p.ne = lpdf;
+    p.c2l[(size_t)n - 1] = fwd;
+    CorrelRatios cr;
+    if (vc.vcm) cr.init(p, cp.dist0, false);
+    float sum = 1.0f;
+    if (vc.cfg->enable_hitting) sum += p.c2l[(size_t)n - 1] / p.ne;
+    sum += vcm_camera_reciprocals(vc, p, (int)cp.v.size() - 1, cp.footprint, cr) / p.ne;
+    float mis = 1.0f / sum;
+    Rgb w = emission * bsdf_cos * (jac / lpdf);
+    return mis * w;
+}
+
+// PathCache.GetVertex (PathCache.cs:40-56): the global index in path order
+std::pair<size_t, size_t> global_vertex(const VcmCtx& vc, int64_t g) {
+    const std::vector<int64_t>& cum = *vc.cum;
+    size_t path = (size_t)(std::upper_bound(cum.begin(), cum.end(), g) - cum.begin());
+    int64_t first = path == 0 ? 0 : cum[path - 1];
+    return {path, (size_t)(g - first)};
+}
+
+// BidirConnections with a vertex from VertexSelector (BidirBase.Camera.cs:279-295) and
+// Connect (:192-270) with BidirConnectMis (VertexCacheBidir.cs:189-197, VertexConnectionAndMerging.cs:768-800)
+Rgb vcm_connect(VcmCtx& vc, const Shader& sh, Rng& rng, const VcmPath& cp, float rev_jac) {
+    const or_scene& s = *vc.s;
+    int64_t g = (int64_t)rng.next_int((uint32_t)vc.num_vertices);
+    float density = select_density(vc);
+    if (g <= 0) return rgb(0.0f);
+    auto [pi, k] = global_vertex(vc, g);
+    const std::vector<CachedVertex>& path = (*vc.cache)[pi];
+    const CachedVertex& lv = path[k];
+    if (lv.depth < 1) return rgb(0.0f);
+    const CachedVertex& anc = path[k - 1];
+    V3 to_anc = normalize(anc.p.position - lv.p.position);
+    int depth = lv.depth + (int)cp.v.size() + 1;
+    if (depth > vc.cfg->max_depth || depth < vc.cfg->min_depth) return rgb(0.0f);
+    vc.c->shadow++;
+    if (trace_any(s, shadow_ray(lv.p, sh.p), false)) {
+        vc.c->occluded++;
+        return rgb(0.0f);
+    }
+    V3 dir = normalize(lv.p.position - sh.p.position);
+    Shader ls = make_shader(s, lv.p, to_anc);
+    ls.light = true;
+    Rgb f_light = eval_nocos(ls, -dir) * std::fabs(dot(lv.p.normal, -dir));
+    f_light = f_light * (std::fabs(dot(lv.p.shading_normal, to_anc)) / std::fabs(dot(lv.p.normal, to_anc)));
+    Rgb f_cam = eval_cos(sh, dir);
+    if (is_black(f_cam) || is_black(f_light)) return rgb(0.0f);
+    float pcl, pcr;
+    pdf_pair(sh, dir, &pcl, &pcr);
+    pcr *= rev_jac;
+    pcl *= area_to_solid_angle(sh.p, lv.p);
+    if (pcl == 0.0f) return rgb(0.0f);
+    float plc, plr;
+    pdf_pair(ls, -dir, &plc, &plr);
+    plr *= area_to_solid_angle(lv.p, anc.p);
+    plc *= area_to_solid_angle(lv.p, sh.p);
+    const int last = (int)cp.v.size() - 1;
+    Pdfs& p = vc.pdfs;
+    p.reset((int)cp.v.size() + lv.depth + 1);
+    p.gather_camera(cp.v, last);
+    p.gather_light(path, lv, last);
+    if (lv.depth == 1) p.ne = next_event_pdf(vc.light(), anc.p);
+    if (last > 0) p.l2c[(size_t)last - 1] = pcr;
+    p.c2l[(size_t)last] = cp.v.back().from;
+    p.l2c[(size_t)last] = plc;
+    p.c2l[(size_t)last + 1] = pcl;
+    p.c2l[(size_t)last + 2] = plr;
+    CorrelRatios cr;
+    if (vc.vcm) cr.init(p, cp.dist0, false);
+    float sum = 1.0f;
+    sum += vcm_camera_reciprocals(vc, p, last, cp.footprint, cr) / density;
+    sum += vcm_light_reciprocals(vc, p, last, cp.footprint, cr) / density;
+    float mis = 1.0f / sum;
+    V3 d = sh.p.position - lv.p.position;
+    float dist2 = dot(d, d);
+    if (dist2 == 0.0f) return rgb(0.0f);
+    Rgb w = lv.weight * f_light * f_cam / dist2 / density;
+    return mis * w;
+}
+
+// VertexConnectionAndMerging.Merge (VertexConnectionAndMerging.cs:329-425) with MergeMis (:648-697)
+Rgb vcm_merge(VcmCtx& vc, const Shader& sh, const VcmPath& cp, float rev_jac, int32_t photon, float dist_sqr,
+              float radius_sqr) {
+    auto [pi, k] = (*vc.photon_ref)[(size_t)photon];
+    const std::vector<CachedVertex>& path = (*vc.cache)[(size_t)pi];
+    const CachedVertex& ph = path[(size_t)k];
+    int depth = (int)cp.v.size() + ph.depth;
+    if (depth > vc.cfg->max_depth || depth < vc.cfg->min_depth) return rgb(0.0f);
+    if (std::fabs(dot(sh.p.normal, ph.p.normal)) < 0.4f) return rgb(0.0f);
+    const CachedVertex& anc = path[(size_t)k - 1];
+    V3 to_anc = normalize(anc.p.position - sh.p.position);
+    Rgb bsdf = eval_nocos(sh, to_anc);
+    bsdf = bsdf * (std::fabs(dot(sh.p.shading_normal, to_anc)) / std::fabs(dot(ph.p.normal, to_anc)));
+    Rgb contrib = ph.weight * bsdf / vc.num_paths;
+    if (is_black(contrib)) return rgb(0.0f);
+    if (std::fabs(dot(to_anc, sh.p.normal)) < 1e-4f) return rgb(0.0f);
+    float pdf_light_rev, pdf_cam_rev;
+    pdf_pair(sh, to_anc, &pdf_light_rev, &pdf_cam_rev);
+    pdf_cam_rev *= rev_jac;
+    pdf_light_rev *= area_to_solid_angle(sh.p, anc.p);  // the ancestor is on a surface
+    const int last = (int)cp.v.size() - 1;
+    Pdfs& p = vc.pdfs;
+    p.reset((int)cp.v.size() + ph.depth);
+    p.gather_camera(cp.v, last);
+    p.gather_light(path, ph, last - 1);
+    if (last > 0) p.l2c[(size_t)last - 1] = pdf_cam_rev;
+    p.l2c[(size_t)last] = ph.pdf_from;
+    p.c2l[(size_t)last] = cp.v.back().from;
+    p.c2l[(size_t)last + 1] = pdf_light_rev;
+    if (ph.depth == 1) p.ne = next_event_pdf(vc.light(), anc.p);
+    // MergeMis
+    float mis;
+    {
+        const sspt_vcm_settings& cfg = *vc.cfg;
+        float radius = cp.footprint;
+        float approx = p.l2c[(size_t)last] * kPi * radius * radius * vc.num_paths;
+        CorrelRatios cr;
+        cr.init(p, cp.dist0, false);
+        if (!cfg.disable_correl_aware_mis) approx *= cr[last];
+        if (approx == 0.0f) {
+            mis = 0.0f;
+        } else {
+            float sum = 0.0f;
+            sum += vcm_camera_reciprocals(vc, p, last, radius, cr) / approx;
+            sum += vcm_light_reciprocals(vc, p, last, radius, cr) / approx;
+            if (ph.depth > 1 && cfg.num_connections > 0) sum += select_density(vc) / approx;
+            mis = 1.0f / sum;
+        }
+    }
+    if (pdf_cam_rev == 0.0f || pdf_light_rev == 0.0f) return rgb(0.0f);
+    float kernel = 2.0f * (radius_sqr - dist_sqr) / (kPi * radius_sqr * radius_sqr);
+    return contrib * kernel * mis;
+}
+
+// PerformMerging (VertexConnectionAndMerging.cs:469-516): the merges with the
+// max_num_photons nearest photons within the footprint radius.
+Rgb vcm_merging(VcmCtx& vc, const Shader& sh, const VcmPath& cp, float rev_jac) {
+    const sspt_vcm_settings& cfg = *vc.cfg;
+    if (!cfg.enable_merging) return rgb(0.0f);
+    if (!cfg.merge_primary && cp.v.size() == 1) return rgb(0.0f);
+    float radius = cp.footprint;  // ComputeLocalMergeRadius
+    float r2 = radius * radius;
+    V3 q = sh.p.position;
+    // NearestNeighborSearch.ForAllNearest(q, MaxNumPhotons, radius): the photons with the
+    // smallest squared distances below radius^2, ties to the earlier photon
+    std::vector<std::pair<float, int32_t>> found;
+    const float h = vc.cell;
+    const double span = 2.0 * (double)radius / (double)h + 2.0;
+    if (!(span * span * span <= (double)std::max<size_t>(vc.grid->size(), 1))) {
+        // a query box of more cells than there are occupied cells (or a non-finite radius): every photon
+        for (size_t ph = 0; ph < vc.photon_pos->size(); ++ph) {
+            V3 d = (*vc.photon_pos)[ph] - q;
+            float d2 = d.x * d.x + d.y * d.y + d.z * d.z;
+            if (d2 < r2) found.push_back({d2, (int32_t)ph});
+        }
+    }
+    int64_t lo[3] = {0, 0, 0}, hi[3] = {-1, -1, -1};
+    if (found.empty() && span * span * span <= (double)std::max<size_t>(vc.grid->size(), 1))
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = (int64_t)std::floor((get(q, a) - radius) / h);
+            hi[a] = (int64_t)std::floor((get(q, a) + radius) / h);
+        }
+    for (int64_t x = lo[0]; x <= hi[0]; ++x)
+        for (int64_t y = lo[1]; y <= hi[1]; ++y)
+            for (int64_t z = lo[2]; z <= hi[2]; ++z) {
+                auto it = vc.grid->find((x * 73856093) ^ (y * 19349663) ^ (z * 83492791));
+                if (it == vc.grid->end()) continue;
+                for (int32_t ph : it->second) {
+                    V3 pp = (*vc.photon_pos)[(size_t)ph];
+                    if ((int64_t)std::floor(pp.x / h) != x || (int64_t)std::floor(pp.y / h) != y ||
+                        (int64_t)std::floor(pp.z / h) != z)
+                        continue;
+                    V3 d = pp - q;
+                    float d2 = d.x * d.x + d.y * d.y + d.z * d.z;
+                    if (d2 < r2) found.push_back({d2, ph});
+                }
+            }
+    std::sort(found.begin(), found.end());
+    const int k = cfg.max_num_photons;
+    if ((int)found.size() > k) found.resize((size_t)k);
+    const int num_found = (int)found.size();
+    Rgb est = rgb(0.0f);
+    if (num_found == 0) return est;
+    float furthest = std::sqrt(found.back().first);
+    for (const auto& f : found) {
+        float distance = std::sqrt(f.first);
+        float radius_sqr = num_found == k ? furthest * furthest : r2;  // MergeHelper (:494-516)
+        est = est + vcm_merge(vc, sh, cp, rev_jac, f.second, distance * distance, radius_sqr);
+    }
+    return est;
+}
+
+// BidirBase.RenderPixel (BidirBase.Camera.cs:97-106), the camera's RandomWalk and
+// OnCameraHit of VertexCacheBidir (VertexCacheBidir.cs:143-168) /
+// VertexConnectionAndMerging (VertexConnectionAndMerging.cs:525-572).
+Rgb vcm_camera_path(VcmCtx& vc, uint32_t row, uint32_t col, uint32_t iteration) {
+    const or_scene& s = *vc.s;
+    const sspt_vcm_settings& cfg = *vc.cfg;
+    const uint32_t pixel = row * (uint32_t)s.width + col;
+    Rng rng{hash_seed(cfg.base_seed_camera, pixel, iteration)};
+    V2 off = rng.next_float2();
+    Ray ray;
+    V3 raw;
+    s.camera.generate_ray((float)col + off.x, (float)row + off.y, &ray.o, &ray.d, &raw);
+    ray.tmin = 0.0f;
+    ray.tmax = INFINITY;
+    SurfacePoint prev;
+    prev.position = s.camera.position;
+    prev.normal = s.camera.direction();
+    float pdf_dir = s.camera.pixel_jacobian(ray.o + raw);
+    VcmPath cp;
+    Rgb prefix = rgb(1.0f), approx = rgb(1.0f), L = rgb(0.0f);
+    for (int depth = 1; depth < cfg.max_depth + 1;) {
+        HitRec h = trace_closest(s, ray, false);
+        vc.c->rays++;
+        if (h.tri < 0) break;
+        vc.c->hits++;
+        SurfacePoint hit = make_point(s, h.tri, {h.u, h.v}, h.t);
+        float pdf_from = pdf_dir * area_to_solid_angle(prev, hit);
+        if (pdf_from == 0.0f) break;
+        float rev_jac = area_to_solid_angle(hit, prev);
+        Shader sh = make_shader(s, hit, -ray.d);
+        if (depth == 1) {
+            cp.footprint = std::sqrt(1.0f / pdf_from);
+            cp.dist0 = hit.distance;
+        }
+        cp.v.push_back({pdf_from, 0.0f});
+        Rgb value = rgb(0.0f);
+        int ei = s.tri_emitter[(size_t)h.tri];
+        bool hitting = vc.vcm ? (cfg.enable_hitting || depth == 1) : (bool)cfg.enable_hitting;
+        if (ei >= 0 && hitting && depth >= cfg.min_depth)
+            value = value + prefix * vcm_emitter_hit(vc, s.emitters[(size_t)ei], hit, -ray.d, cp, rev_jac);
+        if (depth < cfg.max_depth) {
+            for (int i = 0; i < cfg.num_connections; ++i) value = value + prefix * vcm_connect(vc, sh, rng, cp, rev_jac);
+            if (vc.vcm) value = value + prefix * vcm_merging(vc, sh, cp, rev_jac);
+        }
+        if (depth < cfg.max_depth && depth + 1 >= cfg.min_depth)
+            for (int i = 0; i < cfg.num_shadow_rays; ++i) value = value + prefix * vcm_next_event(vc, sh, rng, cp, rev_jac);
+        L = L + value;
+        if (depth + 1 >= cfg.max_depth + 1) break;
+        float q = depth > 4 ? std::min(std::max(average(approx), 0.05f), 0.95f) : 1.0f;
+        if (rng.next_float() > q) break;
+        float pc = rng.next_float();
+        V2 pd = rng.next_float2();
+        BsdfSample bs = sample_bsdf(sh, pc, pd);
+        approx = approx * (bs.weight / q);
+        cp.v.back().to = bs.pdf_rev * rev_jac;
+        if (bs.pdf == 0.0f || is_black(bs.weight)) break;
+        prefix = prefix * (bs.weight / q);
+        ++depth;
+        pdf_dir = bs.pdf;
+        prev = hit;
+        ray = spawn_ray(hit, bs.dir);
+    }
+    return L;
+}
+
+}  // namespace
+
+uint64_t or_render_vcm(const or_scene* s, const sspt_vcm_settings* cfg, float cell_size, uint32_t first,
+                       uint32_t num, float* image, sspt_ray_stats* stats, int32_t threads) {
+    const int64_t W = s->width, H = s->height;
+    if (s->has_env || s->emitters.empty()) return UINT64_MAX;
+    if (cfg->algorithm != SSPT_VERTEX_CACHE_BIDIR && cfg->algorithm != SSPT_VCM) return UINT64_MAX;
+    if (cfg->algorithm == SSPT_VCM && (cfg->max_num_photons < 1 || cfg->max_num_photons > 32)) return UINT64_MAX;
+    const uint32_t paths = cfg->num_light_paths < 0 ? (uint32_t)(W * H) : (uint32_t)cfg->num_light_paths;
+    sspt_bidir_settings lcfg{};
+    lcfg.base_seed_light = cfg->base_seed_light;
+    lcfg.max_depth = cfg->max_depth;
+    lcfg.num_shadow_rays = cfg->num_shadow_rays;
+    std::atomic<uint64_t> rays{0}, shadow{0}, occl{0}, hits{0};
+    uint64_t nans = 0;
+    std::vector<std::vector<CachedVertex>> cache(paths);
+    std::vector<std::vector<Splat>> splats(paths);
+    std::vector<Rgb> values((size_t)(W * H));
+    std::vector<int64_t> cum(paths);
+    std::vector<V3> photon_pos;
+    std::vector<std::pair<int32_t, int32_t>> photon_ref;
+    std::unordered_map<int64_t, std::vector<int32_t>> grid;
+    const bool vcm = cfg->algorithm == SSPT_VCM;
+    for (uint32_t it = first; it < first + num; ++it) {
+        float n = (float)(it + 1);
+        if (it + 1 > 1) {
+            float scale = (n - 1.0f) / n;
+            for (int64_t k = 0; k < W * H * 3; ++k) image[k] *= scale;
+        }
+        int64_t num_vertices = 0;
+        auto run = [&](int phase, int64_t count) {
+            parallel_for((count + 63) / 64, threads, [&](int64_t ch) {
+                Counters c;
+                VcmCtx vc{s, cfg, lcfg, vcm, (float)paths, &cache, &cum, num_vertices, &c, {},
+                          &photon_pos, &photon_ref, &grid, cell_size};
+                BidirCtx bc = vc.light();
+                int64_t end = std::min<int64_t>(count, (ch + 1) * 64);
+                for (int64_t k = ch * 64; k < end; ++k) {
+                    if (phase == 0) {
+                        bidir_light_path(bc, (uint32_t)k, it, &cache[(size_t)k]);
+                    } else if (phase == 1) {
+                        splats[(size_t)k].clear();
+                        for (size_t i = 1; i < cache[(size_t)k].size(); ++i)
+                            vcm_splat(vc, cache[(size_t)k], i, &splats[(size_t)k]);
+                    } else {
+                        values[(size_t)k] = vcm_camera_path(vc, (uint32_t)(k / W), (uint32_t)(k % W), it);
+                    }
+                }
+                rays += c.rays, shadow += c.shadow, occl += c.occluded, hits += c.hits;
+            });
+        };
+        run(0, paths);  // TraceLightPaths
+        // PathCache.Prepare (PathCache.cs:87-94)
+        int64_t sum = 0;
+        for (uint32_t k = 0; k < paths; ++k) {
+            sum += (int64_t)cache[k].size();
+            cum[k] = sum;
+        }
+        num_vertices = sum;
+        // ProcessPathCache: the photon map (VertexConnectionAndMerging.cs:297-316)
+        photon_pos.clear();
+        photon_ref.clear();
+        grid.clear();
+        if (vcm && cfg->enable_merging) {
+            for (uint32_t k = 0; k < paths; ++k)
+                for (size_t i = 1; i < cache[k].size(); ++i) {
+                    if (is_black(cache[k][i].weight)) continue;
+                    V3 pp = cache[k][i].p.position;
+                    int32_t id = (int32_t)photon_pos.size();
+                    photon_pos.push_back(pp);
+                    photon_ref.push_back({(int32_t)k, (int32_t)i});
+                    int64_t x = (int64_t)std::floor(pp.x / cell_size), y = (int64_t)std::floor(pp.y / cell_size),
+                            z = (int64_t)std::floor(pp.z / cell_size);
+                    grid[(x * 73856093) ^ (y * 19349663) ^ (z * 83492791)].push_back(id);
+                }
+        }
+        if (cfg->enable_light_tracer) run(1, paths);
+        run(2, W * H);
         auto add = [&](int64_t pix, Rgb value) {
             if (!std::isfinite(average(value))) {
                 ++nans;

@@ -136,6 +136,16 @@ uint64_t or_render_light_tracer(const or_scene* s, const sspt_lt_settings* setti
  * the pixel count. */
 uint64_t or_render_bidir(const or_scene* s, const sspt_bidir_settings* settings, uint32_t first_iteration,
                          uint32_t num_iterations, float* image, sspt_ray_stats* stats, int32_t num_threads);
+/* ---- VertexCacheBidir / VertexConnectionAndMerging (SeeSharp/Integrators/Bidir/
+ * VertexCacheBidir.cs, VertexConnectionAndMerging.cs), the contract of
+ * sspt_render_vcm; the photon map is a uniform grid of cells of cell_size
+ * (any size gives the same photons, it only changes the work). Splats in
+ * light-path order, then the camera paths' values in pixel order. Returns the
+ * non-finite value count, UINT64_MAX for a scene with a background or without
+ * emitters, or invalid settings. */
+uint64_t or_render_vcm(const or_scene* s, const sspt_vcm_settings* settings, float cell_size,
+                       uint32_t first_iteration, uint32_t num_iterations, float* image, sspt_ray_stats* stats,
+                       int32_t num_threads);
 void or_primary_aov(const or_scene* s, const sspt_pt_settings* settings, uint32_t first_iteration,
                     uint32_t num_iterations, float* albedo, float* normal, int32_t num_threads);
 

@@ -19,4 +19,4 @@ from .scene import (  # noqa: F401
     create_look_at,
 )
 from .integrator import (ClassicBidir, DebugVisualizer, Device, GpuBatch, GpuPathTracer, GpuScene,  # noqa: F401
-                         Integrator, VertexCacheBidirLightTracer)
+                         Integrator, VertexCacheBidir, VertexCacheBidirLightTracer, VertexConnectionAndMerging)

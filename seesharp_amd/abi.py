@@ -147,6 +147,30 @@ class BidirSettings(C.Structure):
     ]
 
 
+VERTEX_CACHE_BIDIR = 0  # sspt_vcm_algorithm
+VCM = 1
+
+
+class VcmSettings(C.Structure):
+    """sspt_vcm_settings: VertexCacheBidir / VertexConnectionAndMerging fields."""
+    _fields_ = [
+        ("algorithm", C.c_int32),
+        ("base_seed_camera", C.c_uint32),
+        ("base_seed_light", C.c_uint32),
+        ("max_depth", C.c_int32),
+        ("min_depth", C.c_int32),
+        ("num_light_paths", C.c_int32),
+        ("num_connections", C.c_int32),
+        ("num_shadow_rays", C.c_int32),
+        ("enable_light_tracer", C.c_int32),
+        ("enable_hitting", C.c_int32),
+        ("enable_merging", C.c_int32),
+        ("merge_primary", C.c_int32),
+        ("max_num_photons", C.c_int32),
+        ("disable_correl_aware_mis", C.c_int32),
+    ]
+
+
 class RayStats(C.Structure):
     _fields_ = [
         ("num_rays", C.c_uint64),
@@ -226,6 +250,8 @@ SYMBOLS = {
                                            C.POINTER(RayStats)]),
     "sspt_render_bidir": (C.c_int, [_vp, _vp, C.POINTER(BidirSettings), C.c_uint32, C.c_uint32,
                                     C.POINTER(RayStats)]),
+    "sspt_render_vcm": (C.c_int, [_vp, _vp, C.POINTER(VcmSettings), C.c_uint32, C.c_uint32,
+                                  C.POINTER(RayStats)]),
     "sspt_set_bvh_builder": (C.c_int, [_vp, C.c_int32]),
     "sspt_get_bvh_builder": (C.c_int, [_vp, C.POINTER(C.c_int32)]),
     "sspt_scene_bvh_build_time": (C.c_int, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_double),

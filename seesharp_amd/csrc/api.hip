@@ -1767,6 +1767,8 @@ sspt_status sspt_render_bidir(sspt_ctx* ctx, sspt_scene* sc, const sspt_bidir_se
     P.seed_camera = cfg->base_seed_camera;
     P.seed_light = cfg->base_seed_light;
     P.num_paths = (uint32_t)paths;
+    P.num_pixels = (uint32_t)paths;
+    P.mode = kModeClassic;
     P.max_depth = cfg->max_depth;
     P.min_depth = cfg->min_depth;
     P.num_shadow_rays = cfg->num_shadow_rays;
@@ -1788,6 +1790,112 @@ sspt_status sspt_render_bidir(sspt_ctx* ctx, sspt_scene* sc, const sspt_bidir_se
             timed(ctx, "bidir_camera",
                   [&] { launch_bidir_camera(ctx->stream, sc->dev, 0, P, image, ctx->lt_stats); });
         }
+        HIP_TRY(ctx, hipGetLastError());
+    }
+    TRY(read_stats(ctx, sc, stats, ctx->lt_stats));
+    if (stats) stats->num_iterations = num_iterations;
+    return SSPT_OK;
+}
+
+sspt_status sspt_render_vcm(sspt_ctx* ctx, sspt_scene* sc, const sspt_vcm_settings* cfg, uint32_t first_iteration,
+                            uint32_t num_iterations, sspt_ray_stats* stats) {
+    if (!ctx || !sc || !cfg) return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "null argument");
+    if (sc->recs.size() != 1)
+        return fail(ctx, SSPT_ERR_UNSUPPORTED, "bidirectional rendering takes a single scene, not a batch or view set");
+    const SceneRec& R = sc->recs[0];
+    if (R.has_env)
+        return fail(ctx, SSPT_ERR_UNSUPPORTED, "bidirectional rendering does not sample the background (scene has one)");
+    if (R.num_emitters == 0) return fail(ctx, SSPT_ERR_UNSUPPORTED, "bidirectional rendering needs emitters");
+    if (cfg->algorithm != SSPT_VERTEX_CACHE_BIDIR && cfg->algorithm != SSPT_VCM)
+        return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "unknown algorithm %d", cfg->algorithm);
+    if (cfg->max_depth < 0 || cfg->min_depth < 0 || cfg->num_shadow_rays < 0 || cfg->num_connections < 0)
+        return fail(ctx, SSPT_ERR_INVALID_ARGUMENT,
+                    "max_depth, min_depth, num_shadow_rays and num_connections must be non-negative");
+    const bool vcm = cfg->algorithm == SSPT_VCM;
+    if (vcm && (cfg->max_num_photons < 1 || cfg->max_num_photons > kMaxPhotons))
+        return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "max_num_photons must be in 1..%d", kMaxPhotons);
+    const uint64_t pixels = (uint64_t)R.num_pixels;
+    const uint64_t paths = cfg->num_light_paths < 0 ? pixels : (uint64_t)cfg->num_light_paths;
+    const uint64_t levels = (uint64_t)std::max(cfg->max_depth, 1);
+    if (paths * levels > 0x7FFFFFFFull)
+        return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "too many light path vertices (paths x max_depth >= 2^31)");
+    if ((uint64_t)first_iteration + num_iterations > 0xFFFFFFFFull)
+        return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "iteration index overflows");
+    const bool merging = vcm && cfg->enable_merging && cfg->max_depth >= 2;
+    uint64_t table = 1024;
+    while (merging && table < 2 * paths) table <<= 1;
+    const uint64_t max_photons = merging ? (levels - 1) * paths : 0;
+    const uint64_t scan_words = scan_tmp_words((uint32_t)std::max<uint64_t>(paths, table));
+    auto align = [](uint64_t b) { return (b + 255) & ~255ull; };
+    const uint64_t need = 2 * align(levels * paths * sizeof(float4)) + align(levels * pixels * sizeof(float2)) +
+                          3 * align(paths * 4) + align(scan_words * 4) + 2 * align((table + 1) * 4) +
+                          align(max_photons * sizeof(float4)) + 256;
+    if (need > (64ull << 30)) return fail(ctx, SSPT_ERR_OUT_OF_MEMORY, "bidirectional buffers exceed 64 GiB");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    if (sc->bidir_bytes < need) {
+        if (sc->bidir) {
+            HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+            (void)hipFree(sc->bidir);
+            sc->bidir = nullptr;
+            sc->bidir_bytes = 0;
+        }
+        if (hipMalloc((void**)&sc->bidir, need) != hipSuccess) {
+            sc->bidir = nullptr;
+            return fail(ctx, SSPT_ERR_OUT_OF_MEMORY, "bidirectional buffers (%llu bytes)", (unsigned long long)need);
+        }
+        sc->bidir_bytes = need;
+    }
+    uint64_t off = 0;
+    auto carve = [&](uint64_t bytes) {
+        char* p = sc->bidir + off;
+        off += align(bytes);
+        return p;
+    };
+    BidirParams P{};
+    P.lv_a = reinterpret_cast<float4*>(carve(levels * paths * sizeof(float4)));
+    P.lv_b = reinterpret_cast<float4*>(carve(levels * paths * sizeof(float4)));
+    P.cam = reinterpret_cast<float2*>(carve(levels * pixels * sizeof(float2)));
+    P.lv_len = reinterpret_cast<int*>(carve(paths * 4));
+    P.lv_ne = reinterpret_cast<float*>(carve(paths * 4));
+    P.cum = reinterpret_cast<int*>(carve(paths * 4));
+    int* scan_tmp = reinterpret_cast<int*>(carve(scan_words * 4));
+    P.bucket_start = reinterpret_cast<uint32_t*>(carve((table + 1) * 4));
+    P.bucket_fill = reinterpret_cast<uint32_t*>(carve((table + 1) * 4));
+    P.photons = reinterpret_cast<float4*>(carve(max_photons * sizeof(float4)));
+    P.seed_camera = cfg->base_seed_camera;
+    P.seed_light = cfg->base_seed_light;
+    P.num_paths = (uint32_t)paths;
+    P.num_pixels = (uint32_t)pixels;
+    P.max_depth = cfg->max_depth;
+    P.min_depth = cfg->min_depth;
+    P.num_shadow_rays = cfg->num_shadow_rays;
+    P.light_tracer = cfg->enable_light_tracer ? 1 : 0;
+    P.num_paths_f = (float)paths;
+    P.mode = vcm ? kModeVcm : kModeVertexCache;
+    P.num_connections = cfg->num_connections;
+    P.hitting = cfg->enable_hitting ? 1 : 0;
+    P.merging = merging ? 1 : 0;
+    P.merge_primary = cfg->merge_primary ? 1 : 0;
+    P.max_photons = vcm ? cfg->max_num_photons : 1;
+    P.correl = cfg->disable_correl_aware_mis ? 0 : 1;
+    // grid cells of about a pixel's footprint at the scene's scale; any size finds the same photons
+    P.cell = std::max(1e-6f, 1.7320508f * R.abs_max / (float)std::max(R.cam.width, R.cam.height));
+    P.table_mask = (uint32_t)(table - 1);
+    if (!ctx->lt_stats) HIP_TRY(ctx, hipMalloc((void**)&ctx->lt_stats, kStatBytes));
+    HIP_TRY(ctx, hipMemsetAsync(ctx->lt_stats, 0, kStatBytes, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(sc->d_recs, &R, sizeof(SceneRec), hipMemcpyHostToDevice, ctx->stream));
+    float* image = sc->image + R.fb_off;
+    for (uint32_t it = first_iteration; it < first_iteration + num_iterations; ++it) {
+        const float n = (float)(it + 1);
+        if (it + 1 > 1) launch_scale(ctx->stream, image, 3ull * R.num_pixels, (n - 1.0f) / n);
+        P.iteration = it;
+        P.n = n;
+        if (paths > 0)
+            timed(ctx, "vcm_light", [&] { launch_bidir_light(ctx->stream, sc->dev, 0, P, image, ctx->lt_stats); });
+        else
+            HIP_TRY(ctx, hipMemsetAsync(P.lv_len, 0, 4, ctx->stream));
+        timed(ctx, "vcm_prepare", [&] { launch_vcm_prepare(ctx->stream, sc->dev, 0, P, scan_tmp); });
+        timed(ctx, "vcm_camera", [&] { launch_vcm_camera(ctx->stream, sc->dev, 0, P, image, ctx->lt_stats); });
         HIP_TRY(ctx, hipGetLastError());
     }
     TRY(read_stats(ctx, sc, stats, ctx->lt_stats));

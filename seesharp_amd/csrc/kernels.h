@@ -134,15 +134,36 @@ struct BidirParams {
     float4* lv_b;   // barycentric u, v, PdfReverseAncestor, triangle (int bits)
     int* lv_len;    // vertices of path p (0: none)
     float* lv_ne;   // next-event pdf of path p's emitter point (PdfNextEventAncestor)
-    // Camera vertex k (depth k + 1) of pixel p at [k * num_paths + p]:
+    // Camera vertex k (depth k + 1) of pixel p at [k * num_pixels + p]:
     // PdfFromAncestor, PdfToAncestor.
     float2* cam;
+    uint32_t num_pixels;  // camera paths (ClassicBidir: num_paths)
+    // VertexCacheBidir / VertexConnectionAndMerging
+    int mode;             // kModeClassic, kModeVertexCache, kModeVcm
+    int num_connections, hitting, merging, merge_primary, max_photons, correl;
+    int* cum;             // inclusive prefix sums of lv_len (PathCache.cumPathLen)
+    // photon map: a hash grid of cells of size `cell`; bucket b holds
+    // photons[bucket_start[b] .. bucket_start[b + 1]) (position, global vertex index bits)
+    float cell;
+    uint32_t table_mask;
+    uint32_t* bucket_start;
+    uint32_t* bucket_fill;
+    float4* photons;
 };
+constexpr int kModeClassic = 0, kModeVertexCache = 1, kModeVcm = 2;
+constexpr int kMaxPhotons = 32;  // largest MaxNumPhotons of a merge
 // TraceLightPaths (+ SplatLightVertices if light_tracer), then the camera paths.
 void launch_bidir_light(hipStream_t s, const DevScene& sc, int scene, const BidirParams& p, float* image,
                         unsigned long long* stats);
 void launch_bidir_camera(hipStream_t s, const DevScene& sc, int scene, const BidirParams& p, float* image,
                          unsigned long long* stats);
+// VertexCacheBidir / VCM: the light paths' global vertex indices (p.cum), the
+// photon grid (VCM with merging) and the camera paths. `scan_tmp` holds
+// scan_tmp_words(max(num_paths, table size)) ints.
+uint32_t scan_tmp_words(uint32_t n);
+void launch_vcm_prepare(hipStream_t s, const DevScene& sc, int scene, const BidirParams& p, int* scan_tmp);
+void launch_vcm_camera(hipStream_t s, const DevScene& sc, int scene, const BidirParams& p, float* image,
+                       unsigned long long* stats);
 void launch_debug_visualize(hipStream_t s, const DevScene& sc, int scene, uint32_t num_pixels, uint32_t base_seed,
                             uint32_t first, uint32_t num, float* image);
 

@@ -1584,7 +1584,7 @@ struct BidirCtx {
     uint32_t path;  // the light path / pixel
     __device__ __forceinline__ float4 va(int k) const { return P.lv_a[(size_t)k * P.num_paths + path]; }
     __device__ __forceinline__ float4 vb(int k) const { return P.lv_b[(size_t)k * P.num_paths + path]; }
-    __device__ __forceinline__ float2 cam(int k) const { return P.cam[(size_t)k * P.num_paths + path]; }
+    __device__ __forceinline__ float2 cam(int k) const { return P.cam[(size_t)k * P.num_pixels + path]; }
     __device__ __forceinline__ Surf vertex(int k) const {
         const float4 b = vb(k);
         return surface(S, __float_as_int(b.w), b.x, b.y);
@@ -1694,7 +1694,8 @@ __global__ void __launch_bounds__(kTraceBlock) k_bidir_light(DevScene S, int sce
             len = depth + 1;
             uint32_t pix;
             float jac;
-            if (P.light_tracer && depth + 1 >= P.min_depth && camera_response(R.cam, s.pos, s.ng, &pix, &jac)) {
+            if (P.mode == kModeClassic && P.light_tracer && depth + 1 >= P.min_depth &&
+                camera_response(R.cam, s.pos, s.ng, &pix, &jac)) {
                 const f3 p0 = offset_point(s, cam - s.pos);
                 const bool occ = traverse<true>(S, R.root, R.abs_max, p0, cam - p0, 1e-5f, 1.0f - 1e-5f,
                                                 stack_node + threadIdx.x, stack_t + threadIdx.x)
@@ -1799,7 +1800,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_bidir_camera(DevScene S, int sc
             Shader sh;
             make_shader(S, s, -d, &sh);
             const int c = depth;  // camera vertices, this one included
-            P.cam[(size_t)(c - 1) * P.num_paths + pix] = make_float2(pdf_from, 0.0f);
+            P.cam[(size_t)(c - 1) * P.num_pixels + pix] = make_float2(pdf_from, 0.0f);
             f3 value = gray(0.0f);
             const int ei = S.tri_emitter[h.li];
             if (ei >= 0 && depth >= P.min_depth) {
@@ -1914,7 +1915,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_bidir_camera(DevScene S, int sc
             const f2 pd = rng.next_float2();
             const BsdfSample bs = sample_bsdf<true>(sh, pc, pd);
             approx = approx * (bs.weight / q);
-            P.cam[(size_t)(c - 1) * P.num_paths + pix].y = bs.pdf_rev * rev_jac;  // OnContinue
+            P.cam[(size_t)(c - 1) * P.num_pixels + pix].y = bs.pdf_rev * rev_jac;  // OnContinue
             if (bs.pdf == 0.0f || is_zero(bs.weight)) break;
             prefix = prefix * (bs.weight / q);
             ++depth;
@@ -1936,6 +1937,609 @@ __global__ void __launch_bounds__(kTraceBlock) k_bidir_camera(DevScene S, int sc
         }
     }
 }
+
+// ------------------------------- light vertex cache and merging (VCM) -------
+// VertexCacheBidir (VertexCacheBidir.cs) and VertexConnectionAndMerging
+// (VertexConnectionAndMerging.cs) over the same light-path cache: the light
+// kernel is k_bidir_light (its light-tracer weight per mode); k_vcm_prepare
+// numbers the cached vertices in path order (PathCache.cumPathLen) and builds
+// the photon grid; k_vcm_camera follows the camera paths.
+
+// The pdfs of one full path as BidirPathPdfs (BidirPathPdfs.cs) holds them,
+// read where they live instead of gathered into arrays: C2L[j] (camera ->
+// light) is camera vertex j's PdfFromAncestor for j < c, else light vertex
+// (n + 1 - j)'s PdfReverseAncestor; L2C[j] (light -> camera) is 1 for
+// j = n - 1, camera vertex j + 1's PdfToAncestor for j < c - 2, else light
+// vertex (n - 1 - j)'s PdfFromAncestor; a technique overrides up to four
+// entries with its own values (GatherCameraPdfs / GatherLightPdfs and the
+// assignments after them, BidirBase.Camera.cs:192-270,368-524,
+// BidirBase.Light.cs:35-92, VertexConnectionAndMerging.cs:329-425).
+struct PdfView {
+    const BidirParams* P;
+    uint32_t pixel, lpath;
+    int n, c;
+    float ne;  // PdfNextEvent
+    int no = 0;
+    int oi[4];
+    bool oc[4];  // the override is a C2L entry
+    float ov[4];
+    __device__ __forceinline__ void set_c2l(int j, float v) {
+        oi[no] = j;
+        oc[no] = true;
+        ov[no++] = v;
+    }
+    __device__ __forceinline__ void set_l2c(int j, float v) {
+        oi[no] = j;
+        oc[no] = false;
+        ov[no++] = v;
+    }
+    __device__ __forceinline__ float c2l(int j) const {
+        for (int k = 0; k < no; ++k)
+            if (oc[k] && oi[k] == j) return ov[k];
+        if (j < c) return P->cam[(size_t)j * P->num_pixels + pixel].x;
+        return P->lv_b[(size_t)(n + 1 - j) * P->num_paths + lpath].z;
+    }
+    __device__ __forceinline__ float l2c(int j) const {
+        for (int k = 0; k < no; ++k)
+            if (!oc[k] && oi[k] == j) return ov[k];
+        if (j == n - 1) return 1.0f;
+        if (j < c - 2) return P->cam[(size_t)(j + 1) * P->num_pixels + pixel].y;
+        return P->lv_a[(size_t)(n - 1 - j) * P->num_paths + lpath].w;
+    }
+};
+
+// VertexConnectionAndMerging.CorrelAwareRatios (VertexConnectionAndMerging.cs:580-646),
+// evaluated per index from the same prefix / suffix products.
+struct Correl {
+    const PdfView* v;
+    float accept_area;
+    __device__ __forceinline__ Correl(const PdfView& view, float dist_to_cam) : v(&view) {
+        const float radius = dist_to_cam * 0.0174550649f;
+        accept_area = radius * radius * kPi;
+    }
+    __device__ float operator[](int idx) const {
+        if (idx == 0) return 1.0f;
+        const int nsv = v->n - 1;
+        float cam = 1.0f;
+        for (int i = 0; i <= idx; ++i) cam *= fminf(v->c2l(i) * accept_area, 1.0f);
+        float light = 1.0f;
+        for (int i = nsv - 1; i >= idx; --i) {
+            float next = v->l2c(i) * accept_area;
+            if (i == nsv - 1) next *= accept_area;  // not from the background
+            light *= fminf(next, 1.0f);
+        }
+        if (cam == 0.0f && light == 0.0f) return 1.0f;
+        return sdiv(cam, cam + light - cam * light);
+    }
+};
+
+// VertexCacheBidir.BidirSelectDensity (VertexCacheBidir.cs:71-82)
+__device__ __forceinline__ float select_density(const BidirParams& P, int num_vertices) {
+    if (num_vertices == 0) return 0.0f;
+    return (1.0f / (float)num_vertices) * (float)(P.num_connections * (int)P.num_paths);
+}
+
+// CameraPathReciprocals: VertexCacheBidir.cs:210-223, VertexConnectionAndMerging.cs:839-889
+__device__ float vcm_camera_reciprocals(const BidirParams& P, const PdfView& v, int last, float radius,
+                                        const Correl& cr, float density) {
+    const bool merging = P.mode == kModeVcm && P.merging;
+    float sum = 0.0f, next = 1.0f;
+    for (int i = last; i > 0; --i) {
+        if (merging) {
+            float accept = v.l2c(i) * kPi * radius * radius;
+            if (P.correl) accept *= cr[i];
+            sum += next * P.num_paths_f * accept;
+        }
+        next *= sdiv(v.l2c(i), v.c2l(i));
+        if (P.num_connections > 0) sum += next * density;
+    }
+    if (P.light_tracer) sum += sdiv(next * v.l2c(0), v.c2l(0)) * P.num_paths_f;
+    if (merging && P.merge_primary) sum += next * P.num_paths_f * v.l2c(0) * kPi * radius * radius;
+    return sum;
+}
+
+// LightPathReciprocals: VertexCacheBidir.cs:225-237, VertexConnectionAndMerging.cs:891-929
+__device__ float vcm_light_reciprocals(const BidirParams& P, const PdfView& v, int last, float radius,
+                                       const Correl& cr, float density) {
+    const bool merging = P.mode == kModeVcm && P.merging;
+    const int n = v.n;
+    float sum = 0.0f, next = 1.0f;
+    for (int i = last + 1; i < n; ++i) {
+        if (i == n - 1) sum += sdiv(next * v.ne, v.l2c(i));
+        if (merging && i < n - 1 && (P.merge_primary || i > 0)) {
+            float accept = v.c2l(i) * kPi * radius * radius;
+            if (P.correl) accept *= cr[i];
+            sum += next * P.num_paths_f * accept;
+        }
+        next *= sdiv(v.c2l(i), v.l2c(i));
+        if (i < n - 2 && P.num_connections > 0) sum += next * density;
+    }
+    if (P.hitting) sum += next;
+    return sum;
+}
+
+// The inclusive scan of n ints in three passes: per-block scans, one block
+// scanning the block sums in chunks with a running carry, the carries added.
+constexpr int kScanBlock = 1024;
+uint32_t scan_tmp_words(uint32_t n) { return (n + kScanBlock - 1) / kScanBlock + 1; }
+__device__ __forceinline__ int block_scan_incl(int v, int* lds) {
+    lds[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < kScanBlock; off <<= 1) {
+        const int add = threadIdx.x >= (unsigned)off ? lds[threadIdx.x - off] : 0;
+        __syncthreads();
+        lds[threadIdx.x] += add;
+        __syncthreads();
+    }
+    const int r = lds[threadIdx.x];
+    __syncthreads();
+    return r;
+}
+__global__ void __launch_bounds__(kScanBlock) k_vscan_local(const int* in, int* out, uint32_t n, int* block_sums) {
+    __shared__ int lds[kScanBlock];
+    const uint32_t i = blockIdx.x * kScanBlock + threadIdx.x;
+    const int r = block_scan_incl(i < n ? in[i] : 0, lds);
+    if (i < n) out[i] = r;
+    if (threadIdx.x == kScanBlock - 1) block_sums[blockIdx.x] = r;
+}
+__global__ void __launch_bounds__(kScanBlock) k_vscan_carry(int* block_sums, uint32_t nb) {
+    __shared__ int lds[kScanBlock];
+    __shared__ int total;
+    int carry = 0;
+    for (uint32_t base = 0; base < nb; base += kScanBlock) {
+        const uint32_t i = base + threadIdx.x;
+        const int v = i < nb ? block_sums[i] : 0;
+        const int r = block_scan_incl(v, lds);
+        if (i < nb) block_sums[i] = carry + r - v;  // exclusive: the sum of the blocks before block i
+        if (threadIdx.x == kScanBlock - 1) total = r;
+        __syncthreads();
+        carry += total;
+        __syncthreads();
+    }
+}
+__global__ void __launch_bounds__(kScanBlock) k_vscan_add(int* out, uint32_t n, const int* block_sums) {
+    const uint32_t i = blockIdx.x * kScanBlock + threadIdx.x;
+    if (i < n) out[i] += block_sums[blockIdx.x];
+}
+
+// The photon map (VertexConnectionAndMerging.ProcessPathCache, :297-316): the
+// cached vertices of depth >= 1 with non-zero weight, hashed by grid cell.
+__device__ __forceinline__ int3 grid_cell(f3 p, float h) {
+    return make_int3((int)floorf(p.x / h), (int)floorf(p.y / h), (int)floorf(p.z / h));
+}
+__device__ __forceinline__ uint32_t cell_bucket(int3 c, uint32_t mask) {
+    return ((uint32_t)c.x * 73856093u ^ (uint32_t)c.y * 19349663u ^ (uint32_t)c.z * 83492791u) & mask;
+}
+template <bool SCATTER>
+__global__ void __launch_bounds__(256) k_photon_grid(DevScene S, BidirParams P) {
+    const uint64_t total = (uint64_t)(P.max_depth - 1) * P.num_paths;
+    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < total;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        const int k = (int)(t / P.num_paths) + 1;
+        const uint32_t p = (uint32_t)(t % P.num_paths);
+        if (k >= P.lv_len[p]) continue;
+        const size_t at = (size_t)k * P.num_paths + p;
+        const float4 a = P.lv_a[at];
+        if (a.x == 0.0f && a.y == 0.0f && a.z == 0.0f) continue;
+        const float4 b = P.lv_b[at];
+        const f3 pos = surface(S, __float_as_int(b.w), b.x, b.y).pos;
+        const uint32_t bucket = cell_bucket(grid_cell(pos, P.cell), P.table_mask);
+        if (!SCATTER) {
+            atomicAdd(P.bucket_fill + bucket, 1u);
+        } else {
+            const uint32_t slot = atomicAdd(P.bucket_fill + bucket, 1u);
+            const int g = (p == 0 ? 0 : P.cum[p - 1]) + k;  // global vertex index, the photon's order
+            P.photons[P.bucket_start[bucket] + slot] = f4_of(pos, __int_as_float(g));
+        }
+    }
+}
+__global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+// The light vertex of global index g (PathCache.GetVertex, PathCache.cs:40-56):
+// its path (the first with cum > g) and its index along the path.
+__device__ __forceinline__ void global_vertex(const BidirParams& P, int g, uint32_t* path, int* k) {
+    uint32_t lo = 0, len = P.num_paths;
+    while (len > 0) {
+        const uint32_t half = len >> 1;
+        if (P.cum[lo + half] <= g) {
+            lo += half + 1;
+            len -= half + 1;
+        } else {
+            len = half;
+        }
+    }
+    *path = lo;
+    *k = g - (lo == 0 ? 0 : P.cum[lo - 1]);
+}
+
+// NearestNeighborSearch.ForAllNearest (TinyEmbree 1.2.0) as restated: the
+// (at most) kmax photons with the smallest squared distances (fp32,
+// dx*dx + dy*dy + dz*dz) below r2, ties to the smaller global index, in
+// ascending order. Returns the count.
+__device__ __forceinline__ void consider_photon(float4 ph, f3 q, float r2, int kmax, float* d2s, int* ids,
+                                                int* found) {
+    const float dx = ph.x - q.x, dy = ph.y - q.y, dz = ph.z - q.z;
+    const float d2 = dx * dx + dy * dy + dz * dz;
+    if (!(d2 < r2)) return;
+    const int id = __float_as_int(ph.w);
+    if (*found == kmax && (d2 > d2s[kmax - 1] || (d2 == d2s[kmax - 1] && id > ids[kmax - 1]))) return;
+    int at = *found < kmax ? (*found)++ : kmax - 1;
+    while (at > 0 && (d2s[at - 1] > d2 || (d2s[at - 1] == d2 && ids[at - 1] > id))) {
+        d2s[at] = d2s[at - 1];
+        ids[at] = ids[at - 1];
+        --at;
+    }
+    d2s[at] = d2;
+    ids[at] = id;
+}
+__device__ int nearest_photons(const BidirParams& P, f3 q, float radius, float r2, int kmax, float* d2s, int* ids) {
+    int found = 0;
+    const float h = P.cell;
+    const float span = 2.0f * radius / h + 2.0f;  // cells per axis the query box can touch
+    if (!(span * span * span <= (float)(P.table_mask + 1))) {
+        // a query box of more cells than buckets (or a non-finite radius): every bucket once
+        const uint32_t end = P.bucket_start[P.table_mask + 1];
+        for (uint32_t e = 0; e < end; ++e) consider_photon(P.photons[e], q, r2, kmax, d2s, ids, &found);
+        return found;
+    }
+    const int3 lo = grid_cell(mk3(q.x - radius, q.y - radius, q.z - radius), h);
+    const int3 hi = grid_cell(mk3(q.x + radius, q.y + radius, q.z + radius), h);
+    for (int x = lo.x; x <= hi.x; ++x)
+        for (int y = lo.y; y <= hi.y; ++y)
+            for (int z = lo.z; z <= hi.z; ++z) {
+                const uint32_t b = cell_bucket(make_int3(x, y, z), P.table_mask);
+                const uint32_t end = P.bucket_start[b + 1];
+                for (uint32_t e = P.bucket_start[b]; e < end; ++e) {
+                    const float4 ph = P.photons[e];
+                    const int3 pc = grid_cell(f3_of(ph), h);
+                    if (pc.x != x || pc.y != y || pc.z != z) continue;  // another cell in the same bucket
+                    consider_photon(ph, q, r2, kmax, d2s, ids, &found);
+                }
+            }
+    return found;
+}
+
+// SplatLightVertices (BidirBase.Light.cs:29-92) of VertexCacheBidir / VCM, after
+// the cache is numbered: the light tracer's weight counts the connections,
+// whose density needs the vertex count (LightTracerMis, VertexCacheBidir.cs:183-186,
+// VertexConnectionAndMerging.cs:735-766).
+__global__ void __launch_bounds__(kTraceBlock) k_vcm_splat(DevScene S, int scene, BidirParams P, float* image,
+                                                           unsigned long long* stats) {
+    __shared__ int stack_node[kStackNodeLds];
+    __shared__ float stack_t[kStackLds * kTraceBlock];
+    const SceneRec& R = S.recs[scene];
+    const f3 cam = mk3(R.cam.pos[0], R.cam.pos[1], R.cam.pos[2]);
+    const int num_vertices = P.num_paths > 0 ? P.cum[P.num_paths - 1] : 0;
+    const float density = select_density(P, num_vertices);
+    for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < P.num_paths; idx += gridDim.x * blockDim.x) {
+        const int len = P.lv_len[idx];
+        if (len < 2) continue;
+        const float ne = P.lv_ne[idx];
+        float4 b = P.lv_b[idx];
+        Surf prev = surface(S, __float_as_int(b.w), b.x, b.y);
+        for (int depth = 1; depth < len; ++depth) {
+            const size_t at = (size_t)depth * P.num_paths + idx;
+            const float4 a = P.lv_a[at];
+            b = P.lv_b[at];
+            const Surf s = surface(S, __float_as_int(b.w), b.x, b.y);
+            uint32_t pix;
+            float jac;
+            if (depth + 1 >= P.min_depth && camera_response(R.cam, s.pos, s.ng, &pix, &jac)) {
+                const f3 p0 = offset_point(s, cam - s.pos);
+                const bool occ = traverse<true>(S, R.root, R.abs_max, p0, cam - p0, 1e-5f, 1.0f - 1e-5f,
+                                                stack_node + threadIdx.x, stack_t + threadIdx.x)
+                                     .li >= 0;
+                wave_count(stats, kStatShadow, true);
+                wave_count(stats, kStatShadowTraced, true);
+                wave_count(stats, kStatOccluded, occ);
+                if (!occ) {
+                    f3 to_cam = cam - s.pos;
+                    const float dist = length(to_cam);
+                    to_cam = to_cam / dist;
+                    const f3 to_anc = normalize(prev.pos - s.pos);
+                    Shader sh;
+                    make_shader(S, s, to_cam, &sh);
+                    f3 f = material_value(sh, to_anc);
+                    if (!is_zero(f)) {
+                        f = f * sdiv(fabsf(dot(s.ns, to_anc)), fabsf(dot(s.ng, to_anc)));
+                        float pdf_rev, unused;
+                        pdf_pair(sh, to_anc, &pdf_rev, &unused);
+                        pdf_rev *= area_to_solid_angle(s.pos, prev.pos, prev.ng);
+                        PdfView v{&P, 0u, idx, depth + 1, 0, ne};
+                        v.set_c2l(0, 1.0f * jac);
+                        v.set_c2l(1, pdf_rev);
+                        float mis;
+                        if (P.mode == kModeVcm) {
+                            const Correl cr(v, dist);
+                            const float footprint = sqrtf(1.0f / v.c2l(0));
+                            float sum = vcm_light_reciprocals(P, v, -1, footprint, cr, density);
+                            sum = sdiv(sum, P.num_paths_f);
+                            sum += 1.0f;
+                            mis = srcp(sum);
+                        } else {
+                            const Correl cr(v, dist);
+                            mis = srcp(1.0f + sdiv(vcm_light_reciprocals(P, v, -1, 0.0f, cr, density),
+                                                   P.num_paths_f));
+                        }
+                        const f3 w = f3_of(a) * f * jac / P.num_paths_f;
+                        const f3 val = mis * w;
+                        const bool bad = !isfinite(average(val));
+                        wave_count(stats, kStatNan, bad);
+                        if (!bad) {
+                            const f3 add = div_exact(val, P.n);
+                            float* px = image + 3ull * pix;
+                            atomicAdd(px, add.x);
+                            atomicAdd(px + 1, add.y);
+                            atomicAdd(px + 2, add.z);
+                        }
+                    }
+                }
+            }
+            prev = s;
+        }
+    }
+}
+
+// BidirBase.RenderPixel (BidirBase.Camera.cs:97-106) and the camera's RandomWalk
+// with OnCameraHit of VertexCacheBidir (VertexCacheBidir.cs:143-168) /
+// VertexConnectionAndMerging (VertexConnectionAndMerging.cs:525-572): emitter
+// hits, NumConnections connections to vertices drawn from the whole cache,
+// merging with the nearest photons (VCM) and NumShadowRays next-event samples,
+// each weighted against all enabled techniques (VertexCacheBidir.cs:171-237,
+// VertexConnectionAndMerging.cs:648-929).
+__global__ void __launch_bounds__(kTraceBlock) k_vcm_camera(DevScene S, int scene, BidirParams P, float* image,
+                                                            unsigned long long* stats) {
+    __shared__ int stack_node[kStackNodeLds];
+    __shared__ float stack_t[kStackLds * kTraceBlock];
+    const SceneRec& R = S.recs[scene];
+    const DevCamera& C = R.cam;
+    const bool vcm = P.mode == kModeVcm;
+    const int num_vertices = P.num_paths > 0 ? P.cum[P.num_paths - 1] : 0;
+    const float density = select_density(P, num_vertices);
+    const float pmf_light = (1.0f - 0.0f) / (float)R.num_emitters;
+    for (uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x; pix < P.num_pixels; pix += gridDim.x * blockDim.x) {
+        const uint32_t row = pix / (uint32_t)C.width, col = pix % (uint32_t)C.width;
+        Rng rng{hash_seed(P.seed_camera, pix, P.iteration)};
+        const f2 off = rng.next_float2();
+        f3 o, d, raw;
+        camera_ray(C, (float)col + off.x, (float)row + off.y, &o, &d, &raw);
+        float tmin = 0.0f;
+        f3 prev_pos = o;
+        f3 prev_n = mk3(-C.c2w[8], -C.c2w[9], -C.c2w[10]);
+        float pdf_dir = pixel_jacobian(C, o + raw);
+        float footprint = 0.0f, dist0 = 0.0f;
+        f3 L = gray(0.0f), prefix = gray(1.0f), approx = gray(1.0f);
+        for (int depth = 1; depth < P.max_depth + 1;) {
+            const HitOut h = traverse<false>(S, R.root, R.abs_max, o, d, tmin, INFINITY, stack_node + threadIdx.x,
+                                             stack_t + threadIdx.x);
+            wave_count(stats, kStatRays, true);
+            wave_count(stats, kStatHits, h.li >= 0);
+            if (h.li < 0) break;
+            const Surf s = surface(S, h.li, h.u, h.v);
+            const float pdf_from = pdf_dir * area_to_solid_angle(prev_pos, s.pos, s.ng);
+            if (pdf_from == 0.0f) break;
+            const float rev_jac = area_to_solid_angle(s.pos, prev_pos, prev_n);
+            Shader sh;
+            make_shader(S, s, -d, &sh);
+            if (depth == 1) {
+                footprint = sqrtf(1.0f / pdf_from);  // CameraRandomWalk.OnHit: the pixel footprint
+                dist0 = h.t;
+            }
+            const int c = depth;
+            P.cam[(size_t)(c - 1) * P.num_pixels + pix] = make_float2(pdf_from, 0.0f);
+            f3 value = gray(0.0f);
+            // emitter hits
+            const int ei = S.tri_emitter[h.li];
+            const bool hitting = vcm ? (P.hitting || depth == 1) : P.hitting != 0;
+            if (ei >= 0 && hitting && depth >= P.min_depth) {
+                const DevEmitter e = S.emitters[R.emitter_off + ei];
+                const f3 emission = emitted(e, s.ns, -d);
+                float mis = 1.0f;
+                if (c > 1) {
+                    PdfView v{&P, pix, 0u, c, c, e.inv_area * (1.0f / (float)R.num_emitters) * (1.0f - 0.0f) *
+                                                     (float)P.num_shadow_rays};
+                    v.set_l2c(c - 2, emitter_pdf_ray(e, s.ns, -d) * rev_jac * pmf_light);
+                    const Correl cr(v, dist0);
+                    const float pdf_this = pdf_from;
+                    float sum = 1.0f;
+                    sum += sdiv(v.ne, pdf_this);
+                    sum += sdiv(vcm_camera_reciprocals(P, v, c - 2, footprint, cr, density), pdf_this);
+                    mis = srcp(sum);
+                }
+                value = value + prefix * (mis * emission);
+            }
+            if (depth < P.max_depth) {
+                // connections (BidirConnections with VertexSelector, BidirBase.Camera.cs:279-295)
+                for (int r = 0; r < P.num_connections; ++r) {
+                    const int g = (int)rng.next_int((uint32_t)num_vertices);
+                    if (g <= 0) continue;
+                    uint32_t lp;
+                    int k;
+                    global_vertex(P, g, &lp, &k);
+                    if (k < 1) continue;  // a light point
+                    const int total = k + c + 1;
+                    if (total > P.max_depth || total < P.min_depth) continue;
+                    const float4 la = P.lv_a[(size_t)k * P.num_paths + lp];
+                    const float4 lb = P.lv_b[(size_t)k * P.num_paths + lp];
+                    const float4 ab = P.lv_b[(size_t)(k - 1) * P.num_paths + lp];
+                    const Surf lv = surface(S, __float_as_int(lb.w), lb.x, lb.y);
+                    const Surf anc = surface(S, __float_as_int(ab.w), ab.x, ab.y);
+                    const f3 dd = s.pos - lv.pos;
+                    const f3 p0 = offset_point(lv, dd), p1 = offset_point(s, -dd);
+                    const bool occ = traverse<true>(S, R.root, R.abs_max, p0, p1 - p0, 1e-5f, 1.0f - 1e-5f,
+                                                    stack_node + threadIdx.x, stack_t + threadIdx.x)
+                                         .li >= 0;
+                    wave_count(stats, kStatShadow, true);
+                    wave_count(stats, kStatShadowTraced, true);
+                    wave_count(stats, kStatOccluded, occ);
+                    if (occ) continue;
+                    const f3 to_anc = normalize(anc.pos - lv.pos);
+                    const f3 dir = normalize(lv.pos - s.pos);
+                    Shader ls;
+                    make_shader(S, lv, to_anc, &ls);
+                    ls.light = true;
+                    f3 f_light = material_value(ls, -dir) * fabsf(dot(lv.ng, -dir));
+                    f_light = f_light * sdiv(fabsf(dot(lv.ns, to_anc)), fabsf(dot(lv.ng, to_anc)));
+                    f3 f_cam;
+                    float unused;
+                    eval_cos_pdf(sh, dir, &f_cam, &unused);
+                    float pcl, pcr, plc, plr;
+                    pdf_pair(sh, dir, &pcl, &pcr);
+                    pcr *= rev_jac;
+                    pcl *= area_to_solid_angle(s.pos, lv.pos, lv.ng);
+                    pdf_pair(ls, -dir, &plc, &plr);
+                    plr *= area_to_solid_angle(lv.pos, anc.pos, anc.ng);
+                    plc *= area_to_solid_angle(lv.pos, s.pos, s.ng);
+                    const float dist2 = dot(dd, dd);
+                    if (is_zero(f_cam) || is_zero(f_light) || pcl == 0.0f || dist2 == 0.0f) continue;
+                    PdfView v{&P, pix, lp, c + k + 1, c, P.lv_ne[lp]};
+                    if (c > 1) v.set_l2c(c - 2, pcr);
+                    v.set_l2c(c - 1, plc);
+                    v.set_c2l(c, pcl);
+                    v.set_c2l(c + 1, plr);
+                    const Correl cr(v, dist0);
+                    float sum = 1.0f;
+                    sum += sdiv(vcm_camera_reciprocals(P, v, c - 1, footprint, cr, density), density);
+                    sum += sdiv(vcm_light_reciprocals(P, v, c - 1, footprint, cr, density), density);
+                    const float mis = srcp(sum);
+                    const f3 w = f3_of(la) * f_light * f_cam / dist2 / density;
+                    value = value + prefix * (mis * w);
+                }
+                // merging (PerformMerging / Merge, VertexConnectionAndMerging.cs:329-516)
+                if (vcm && P.merging && (P.merge_primary || c > 1)) {
+                    const float r2 = footprint * footprint;
+                    float d2s[kMaxPhotons];
+                    int ids[kMaxPhotons];
+                    const int found = nearest_photons(P, s.pos, footprint, r2, P.max_photons, d2s, ids);
+                    f3 est = gray(0.0f);
+                    const float furthest = found > 0 ? sqrtf(d2s[found - 1]) : 0.0f;
+                    for (int f = 0; f < found; ++f) {
+                        const float distance = sqrtf(d2s[f]);
+                        const float dist_sqr = distance * distance;
+                        const float radius_sqr = found == P.max_photons ? furthest * furthest : r2;
+                        uint32_t lp;
+                        int k;
+                        global_vertex(P, ids[f], &lp, &k);
+                        const int total = c + k;
+                        if (total > P.max_depth || total < P.min_depth) continue;
+                        const float4 la = P.lv_a[(size_t)k * P.num_paths + lp];
+                        const float4 lb = P.lv_b[(size_t)k * P.num_paths + lp];
+                        const float4 ab = P.lv_b[(size_t)(k - 1) * P.num_paths + lp];
+                        const Surf ph = surface(S, __float_as_int(lb.w), lb.x, lb.y);
+                        if (fabsf(dot(s.ng, ph.ng)) < 0.4f) continue;
+                        const Surf anc = surface(S, __float_as_int(ab.w), ab.x, ab.y);
+                        const f3 to_anc = normalize(anc.pos - s.pos);
+                        f3 bsdf = material_value(sh, to_anc);
+                        bsdf = bsdf * sdiv(fabsf(dot(s.ns, to_anc)), fabsf(dot(ph.ng, to_anc)));
+                        const f3 contrib = f3_of(la) * bsdf / P.num_paths_f;
+                        if (is_zero(contrib)) continue;
+                        if (fabsf(dot(to_anc, s.ng)) < 1e-4f) continue;
+                        float pdf_light_rev, pdf_cam_rev;
+                        pdf_pair(sh, to_anc, &pdf_light_rev, &pdf_cam_rev);
+                        pdf_cam_rev *= rev_jac;
+                        pdf_light_rev *= area_to_solid_angle(s.pos, anc.pos, anc.ng);
+                        if (pdf_cam_rev == 0.0f || pdf_light_rev == 0.0f) continue;
+                        PdfView v{&P, pix, lp, c + k, c, P.lv_ne[lp]};
+                        if (c > 1) v.set_l2c(c - 2, pdf_cam_rev);
+                        v.set_c2l(c, pdf_light_rev);
+                        const Correl cr(v, dist0);
+                        // MergeMis (:648-697)
+                        float approx_m = v.l2c(c - 1) * kPi * footprint * footprint * P.num_paths_f;
+                        if (P.correl) approx_m *= cr[c - 1];
+                        float mis = 0.0f;
+                        if (approx_m != 0.0f) {
+                            float sum = 0.0f;
+                            sum += sdiv(vcm_camera_reciprocals(P, v, c - 1, footprint, cr, density), approx_m);
+                            sum += sdiv(vcm_light_reciprocals(P, v, c - 1, footprint, cr, density), approx_m);
+                            if (k > 1 && P.num_connections > 0) sum += sdiv(density, approx_m);
+                            mis = srcp(sum);
+                        }
+                        const float kernel = sdiv(2.0f * (radius_sqr - dist_sqr), kPi * radius_sqr * radius_sqr);
+                        est = est + contrib * kernel * mis;
+                    }
+                    value = value + prefix * est;
+                }
+            }
+            if (depth < P.max_depth && depth + 1 >= P.min_depth) {
+                for (int r = 0; r < P.num_shadow_rays; ++r) {
+                    const float bg = 0.0f;
+                    if (rng.next_float() < bg) continue;
+                    const DevEmitter e = S.emitters[R.emitter_off + (int)rng.next_int((uint32_t)R.num_emitters)];
+                    const float prob = 1.0f / (float)R.num_emitters;
+                    const f2 b = to_uniform_triangle(rng.next_float2());
+                    const Surf lp = surface(S, e.tri, b.x, b.y);
+                    float lpdf = e.inv_area;
+                    lpdf *= prob;
+                    lpdf *= (float)P.num_shadow_rays;
+                    lpdf *= (1.0f - bg);
+                    if (lpdf == 0.0f) continue;
+                    const f3 dd = lp.pos - s.pos;
+                    const f3 p0 = offset_point(s, dd), p1 = offset_point(lp, -dd);
+                    const bool occ = traverse<true>(S, R.root, R.abs_max, p0, p1 - p0, 1e-5f, 1.0f - 1e-5f,
+                                                    stack_node + threadIdx.x, stack_t + threadIdx.x)
+                                         .li >= 0;
+                    wave_count(stats, kStatShadow, true);
+                    wave_count(stats, kStatShadowTraced, true);
+                    wave_count(stats, kStatOccluded, occ);
+                    if (occ) continue;
+                    const f3 l2s = normalize(s.pos - lp.pos);
+                    const f3 emission = emitted(e, lp.ns, l2s);
+                    f3 bc;
+                    float unused;
+                    eval_cos_pdf(sh, -l2s, &bc, &unused);
+                    const float jac = area_to_solid_angle(s.pos, lp.pos, lp.ng);
+                    float fwd, rev;
+                    pdf_pair(sh, -l2s, &fwd, &rev);
+                    fwd *= jac;
+                    rev *= rev_jac;
+                    if (is_zero(emission) || is_zero(bc) || jac == 0.0f || fwd == 0.0f || rev == 0.0f) continue;
+                    PdfView v{&P, pix, 0u, c + 1, c, lpdf};
+                    if (c > 1) v.set_l2c(c - 2, rev);
+                    v.set_l2c(c - 1, emitter_pdf_ray(e, lp.ns, l2s) * area_to_solid_angle(lp.pos, s.pos, s.ng) *
+                                         pmf_light);
+                    v.set_c2l(c, fwd);
+                    const Correl cr(v, dist0);
+                    float sum = 1.0f;
+                    if (P.hitting) sum += sdiv(fwd, lpdf);
+                    sum += sdiv(vcm_camera_reciprocals(P, v, c - 1, footprint, cr, density), lpdf);
+                    const float mis = srcp(sum);
+                    const f3 w = emission * bc * sdiv(jac, lpdf);
+                    value = value + prefix * (mis * w);
+                }
+            }
+            L = L + value;
+            if (depth + 1 >= P.max_depth + 1) break;
+            const float q = depth > 4 ? clampf(average(approx), 0.05f, 0.95f) : 1.0f;
+            if (rng.next_float() > q) break;
+            const float pc = rng.next_float();
+            const f2 pd = rng.next_float2();
+            const BsdfSample bs = sample_bsdf<true>(sh, pc, pd);
+            approx = approx * (bs.weight / q);
+            P.cam[(size_t)(c - 1) * P.num_pixels + pix].y = bs.pdf_rev * rev_jac;  // OnContinue
+            if (bs.pdf == 0.0f || is_zero(bs.weight)) break;
+            prefix = prefix * (bs.weight / q);
+            ++depth;
+            pdf_dir = bs.pdf;
+            prev_pos = s.pos;
+            prev_n = s.ng;
+            o = offset_point(s, bs.dir);
+            d = bs.dir;
+            tmin = s.err;
+        }
+        const bool bad = !isfinite(average(L));
+        wave_count(stats, kStatNan, bad);
+        if (!bad) {
+            const f3 add = div_exact(L, P.n);
+            float* px = image + 3ull * pix;
+            px[0] += add.x;
+            px[1] += add.y;
+            px[2] += add.z;
+        }
+    }
+}
+
 
 __global__ void __launch_bounds__(256) k_adjoint_finalize(DevScene S, Wave w) {
     const WaveDesc d = cur_wave(w);
@@ -2405,6 +3009,38 @@ void launch_bidir_light(hipStream_t s, const DevScene& sc, int scene, const Bidi
 void launch_bidir_camera(hipStream_t s, const DevScene& sc, int scene, const BidirParams& p, float* image,
                          unsigned long long* stats) {
     hipLaunchKernelGGL(k_bidir_camera, dim3(blocks(p.num_paths, kTraceBlock)), dim3(kTraceBlock), 0, s, sc, scene, p,
+                       image, stats);
+}
+
+void launch_vcm_prepare(hipStream_t s, const DevScene& sc, int scene, const BidirParams& p, int* scan_tmp) {
+    (void)scene;
+    auto scan = [&](const int* in, int* out, uint32_t n) {
+        if (n == 0) return;
+        const uint32_t nb = (n + kScanBlock - 1) / kScanBlock;
+        hipLaunchKernelGGL(k_vscan_local, dim3(nb), dim3(kScanBlock), 0, s, in, out, n, scan_tmp);
+        hipLaunchKernelGGL(k_vscan_carry, dim3(1), dim3(kScanBlock), 0, s, scan_tmp, nb);
+        hipLaunchKernelGGL(k_vscan_add, dim3(nb), dim3(kScanBlock), 0, s, out, n, scan_tmp);
+    };
+    scan(p.lv_len, p.cum, p.num_paths);  // PathCache.Prepare (PathCache.cs:87-94)
+    if (p.mode != kModeVcm || !p.merging || p.max_depth < 2) return;
+    const uint32_t table = p.table_mask + 1;
+    const uint64_t photons = (uint64_t)(p.max_depth - 1) * p.num_paths;
+    if (photons == 0) return;
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(blocks(photons, 256), 65536);
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks(table, 256)), dim3(256), 0, s, p.bucket_fill, (uint64_t)table, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(1), 0, s, p.bucket_start, (uint64_t)1, 0u);
+    hipLaunchKernelGGL(k_photon_grid<false>, dim3(grid), dim3(256), 0, s, sc, p);
+    scan(reinterpret_cast<const int*>(p.bucket_fill), reinterpret_cast<int*>(p.bucket_start + 1), table);
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks(table, 256)), dim3(256), 0, s, p.bucket_fill, (uint64_t)table, 0u);
+    hipLaunchKernelGGL(k_photon_grid<true>, dim3(grid), dim3(256), 0, s, sc, p);
+}
+
+void launch_vcm_camera(hipStream_t s, const DevScene& sc, int scene, const BidirParams& p, float* image,
+                       unsigned long long* stats) {
+    if (p.light_tracer && p.num_paths > 0)
+        hipLaunchKernelGGL(k_vcm_splat, dim3(blocks(p.num_paths, kTraceBlock)), dim3(kTraceBlock), 0, s, sc, scene, p,
+                           image, stats);
+    hipLaunchKernelGGL(k_vcm_camera, dim3(blocks(p.num_pixels, kTraceBlock)), dim3(kTraceBlock), 0, s, sc, scene, p,
                        image, stats);
 }
 

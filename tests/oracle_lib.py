@@ -62,6 +62,8 @@ _SIGS = {
                                             C.POINTER(abi.RayStats), C.c_int32]),
     "or_render_bidir": (C.c_uint64, [_vp, C.POINTER(abi.BidirSettings), C.c_uint32, C.c_uint32, _fp,
                                      C.POINTER(abi.RayStats), C.c_int32]),
+    "or_render_vcm": (C.c_uint64, [_vp, C.POINTER(abi.VcmSettings), C.c_float, C.c_uint32, C.c_uint32, _fp,
+                                   C.POINTER(abi.RayStats), C.c_int32]),
     "or_render_pixel": (None, [_vp, C.POINTER(abi.PtSettings), C.c_uint32, C.c_uint32, C.c_uint32, _fp]),
     "or_set_texture": (C.c_int32, [_vp, C.c_uint32, _fp]),
     "or_render_outliers": (None, [_vp, C.POINTER(abi.PtSettings), C.c_uint32, C.c_uint32, C.c_int32, _fp,
@@ -108,6 +110,15 @@ def bidir_settings(base_seed_camera=0xC030114, base_seed_light=0x13C0FEFE, max_d
                    num_shadow_rays=1, enable_connections=True, enable_light_tracer=True, num_light_paths=-1):
     return abi.BidirSettings(base_seed_camera, base_seed_light, max_depth, min_depth, num_shadow_rays,
                              1 if enable_connections else 0, 1 if enable_light_tracer else 0, num_light_paths)
+
+
+def vcm_settings(algorithm=abi.VCM, base_seed_camera=0xC030114, base_seed_light=0x13C0FEFE, max_depth=5,
+                 min_depth=1, num_light_paths=-1, num_connections=1, num_shadow_rays=1, enable_light_tracer=True,
+                 enable_hitting=True, enable_merging=True, merge_primary=False, max_num_photons=8,
+                 disable_correl_aware_mis=False):
+    return abi.VcmSettings(algorithm, base_seed_camera, base_seed_light, max_depth, min_depth, num_light_paths,
+                           num_connections, num_shadow_rays, int(enable_light_tracer), int(enable_hitting),
+                           int(enable_merging), int(merge_primary), max_num_photons, int(disable_correl_aware_mis))
 
 
 def settings(base_seed=0xC030114, total_spp=1, max_depth=5, min_depth=1, num_shadow_rays=1,
@@ -168,6 +179,21 @@ class OracleScene:
         nans = lib().or_render_bidir(self.ptr, C.byref(cfg), first, num, fp(image), C.byref(stats), threads)
         if nans == 2 ** 64 - 1:
             raise ValueError("ClassicBidir needs emitters, no background and one light path per pixel")
+        return image, stats, int(nans)
+
+    def render_vcm(self, cfg, first=0, num=1, image=None, threads=0, cell=None):
+        """VertexCacheBidir / VCM running-mean image after iterations first..first+num-1, its
+        ray statistics and the non-finite value count. `cell`: the photon grid's cell size
+        (default: the scene's bounding-box diagonal over twice the larger image side)."""
+        if image is None:
+            image = np.zeros((self.height, self.width, 3), np.float32)
+        if cell is None:
+            pts = np.concatenate([m.Vertices for m in self.scene.Meshes])
+            cell = float(np.linalg.norm(pts.max(axis=0) - pts.min(axis=0))) / (2 * max(self.width, self.height))
+        stats = abi.RayStats()
+        nans = lib().or_render_vcm(self.ptr, C.byref(cfg), cell, first, num, fp(image), C.byref(stats), threads)
+        if nans == 2 ** 64 - 1:
+            raise ValueError("VertexCacheBidir / VCM need emitters, no background and valid settings")
         return image, stats, int(nans)
 
     def primary_aov(self, cfg, first=0, num=1, threads=0):

@@ -1,0 +1,181 @@
+"""The oracle's VertexCacheBidir and VertexConnectionAndMerging
+(VertexCacheBidir.cs, VertexConnectionAndMerging.cs over BidirBase.*.cs,
+PathCache.cs, VertexSelector.cs).
+
+VertexCacheBidir is unbiased for any subset of its techniques, like
+ClassicBidir: it is compared with the path tracer and with itself (technique
+subsets at independent seeds, and at the same seeds where the switched
+technique draws no random numbers). Its special case of no connections, no
+next-event samples and no emitter hits is the light tracer, pixel for pixel.
+
+Merging is biased (a kernel density estimate): VCM is compared with itself
+without merging at the same seeds, where the light and camera paths and every
+other sample are shared, and the difference is the photon-mapping bias of
+about -0.6 % of the image on these scenes. Planted mistakes in the merge
+weights (a dropped NumLightPaths factor, a correlation ratio of the wrong
+vertex, merging terms without their correlation ratio) or in the kernel's
+normalisation move it to below -1.6 % or above +2 %. The photon query is the
+restated k-nearest search of TinyEmbree's NearestNeighborSearch; the grid that
+answers it is checked against a single cell holding every photon, i.e. a brute
+force search, and against other cell sizes: the images are bit-identical."""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from seesharp_amd import abi, scenes
+from seesharp_amd.scene import EnvironmentMap
+from test_oracle_bidir import check, glossy_floor_box, pt_batches
+from test_oracle_light_tracer import spheres_box, tilted_floor_box
+
+BATCHES = 8
+VCB = abi.VERTEX_CACHE_BIDIR
+
+
+def vcm_batches(o, iters, seed=0, **kw):
+    out = []
+    for b in range(BATCHES):
+        cfg = ol.vcm_settings(base_seed_camera=0x5EED0000 + 7919 * b + seed,
+                              base_seed_light=0x13C0FEFE + 7919 * b + seed, **kw)
+        img, _, nans = o.render_vcm(cfg, 0, iters)
+        assert nans == 0
+        out.append(img.astype(np.float64))
+    return np.stack(out)
+
+
+SCENES = {  # scene, MaxDepth, iterations per batch
+    "box": (lambda: scenes.box_scene(32, 32), 5, 16),
+    "glass_metal": (lambda: spheres_box(32, 32), 6, 32),
+    "tilted_normals": (lambda: tilted_floor_box(32, 32), 4, 16),
+    "glossy_light": (lambda: spheres_box(32, 32, glossy_light=True), 4, 32),
+}
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+def test_vertex_cache_bidir_matches_path_tracer(name):
+    make, depth, iters = SCENES[name]
+    o = ol.OracleScene(make())
+    check(vcm_batches(o, iters, algorithm=VCB, max_depth=depth), pt_batches(o, iters, depth))
+
+
+# (without emitter hits nothing samples the directly visible emitters: MinDepth 2 for both)
+VARIANTS = [dict(num_connections=0), dict(num_connections=3), dict(num_light_paths=300),
+            dict(enable_hitting=False, min_depth=2), dict(enable_light_tracer=False), dict(num_shadow_rays=0),
+            dict(num_shadow_rays=2), dict(min_depth=2)]
+
+
+@pytest.mark.parametrize("variant", VARIANTS, ids=lambda v: "-".join(f"{k}={x}" for k, x in v.items()))
+def test_vertex_cache_bidir_techniques_agree(variant):
+    o = ol.OracleScene(spheres_box(32, 32))
+    md = variant.get("min_depth", 1)
+    full = vcm_batches(o, 24, algorithm=VCB, max_depth=6, min_depth=md)
+    part = vcm_batches(o, 24, seed=0x51DE, algorithm=VCB, max_depth=6, **variant)
+    check(full, part)
+
+
+@pytest.mark.parametrize("off", [dict(enable_light_tracer=False), dict(enable_hitting=False)],
+                         ids=["no_light_tracer", "no_hitting"])
+def test_vertex_cache_bidir_paired_subsets(off):
+    # the light tracer and emitter hits draw no random numbers: same paths, same samples
+    o = ol.OracleScene(glossy_floor_box(32, 32))
+    diffs = []
+    for b in range(BATCHES):
+        kw = dict(algorithm=VCB, base_seed_camera=0x5EED0000 + 7919 * b, base_seed_light=0x13C0FEFE + 7919 * b,
+                  max_depth=5, min_depth=2)
+        full, _, _ = o.render_vcm(ol.vcm_settings(**kw), 0, 24)
+        part, _, _ = o.render_vcm(ol.vcm_settings(**kw, **off), 0, 24)
+        diffs.append(float((full.astype(np.float64) - part).mean()))
+    diffs = np.array(diffs)
+    z = diffs.mean() / (diffs.std(ddof=1) / np.sqrt(BATCHES))
+    print(f"mean difference {diffs.mean():.3e}, z {z:.2f}")
+    assert abs(z) < 3.5
+
+
+def test_vertex_cache_bidir_reduces_to_light_tracer():
+    # NumConnections 0, NumShadowRays 0, EnableHitting false: only the light tracer
+    # contributes, with weight 1 (the configuration of the light-tracer entry point)
+    s = spheres_box(32, 32)
+    o = ol.OracleScene(s)
+    img, _, _ = o.render_vcm(ol.vcm_settings(algorithm=VCB, max_depth=6, num_connections=0, num_shadow_rays=0,
+                                             enable_hitting=False), 0, 3)
+    lt, _, _ = o.render_light_tracer(ol.lt_settings(max_depth=6), 0, 3)
+    assert lt.mean() > 0
+    np.testing.assert_array_equal(img, lt)
+
+
+def test_vcm_without_merging_is_vertex_cache_bidir():
+    o = ol.OracleScene(spheres_box(32, 32))
+    a, _, _ = o.render_vcm(ol.vcm_settings(max_depth=6, enable_merging=False), 0, 3)
+    b, _, _ = o.render_vcm(ol.vcm_settings(algorithm=VCB, max_depth=6), 0, 3)
+    np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("name", ["box", "glass_metal", "glossy_floor"])
+def test_vcm_merging_bias(name):
+    make = {"box": lambda: scenes.box_scene(32, 32), "glass_metal": lambda: spheres_box(32, 32),
+            "glossy_floor": lambda: glossy_floor_box(32, 32)}[name]
+    depth = 6 if name == "glass_metal" else 5
+    o = ol.OracleScene(make())
+    diffs, means = [], []
+    for b in range(BATCHES):
+        kw = dict(base_seed_camera=0x5EED0000 + 7919 * b, base_seed_light=0x13C0FEFE + 7919 * b, max_depth=depth)
+        full, _, _ = o.render_vcm(ol.vcm_settings(**kw), 0, 16)
+        plain, _, _ = o.render_vcm(ol.vcm_settings(enable_merging=False, **kw), 0, 16)
+        diffs.append(float((full.astype(np.float64) - plain).mean()))
+        means.append(float(plain.mean()))
+    rel = float(np.mean(diffs) / np.mean(means))
+    print(f"{name}: merging changes the image mean by {rel:+.4f}")
+    assert -0.012 < rel < 0.002
+
+
+def test_vcm_matches_path_tracer_within_merging_bias():
+    o = ol.OracleScene(spheres_box(32, 32))
+    vcm = vcm_batches(o, 32, max_depth=6)
+    pt = pt_batches(o, 32, 6)
+    rel = (vcm.mean() - pt.mean()) / pt.mean()
+    print(f"VCM {vcm.mean():.5f} PT {pt.mean():.5f} ({rel:+.4f})")
+    assert abs(rel) < 0.03
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(max_num_photons=1), dict(max_num_photons=32), dict(merge_primary=True),
+                                dict(disable_correl_aware_mis=True)],
+                         ids=["default", "one_photon", "32_photons", "merge_primary", "no_correl"])
+def test_vcm_photon_grid_is_exact(kw):
+    # one cell holding every photon is a brute-force k-nearest search; other cell
+    # sizes must find the same photons, merged in the same order
+    s = spheres_box(32, 32)
+    o = ol.OracleScene(s)
+    cfg = ol.vcm_settings(max_depth=6, **kw)
+    ref, _, _ = o.render_vcm(cfg, 0, 2, cell=1e6)
+    for cell in (0.004, 0.02, 0.1):
+        img, _, _ = o.render_vcm(cfg, 0, 2, cell=cell)
+        np.testing.assert_array_equal(img, ref)
+
+
+def test_vcm_settings_change_the_image():
+    # each merge setting reaches the estimate
+    o = ol.OracleScene(spheres_box(32, 32))
+    base, _, _ = o.render_vcm(ol.vcm_settings(max_depth=6), 0, 2)
+    for kw in (dict(max_num_photons=2), dict(merge_primary=True), dict(disable_correl_aware_mis=True)):
+        img, _, _ = o.render_vcm(ol.vcm_settings(max_depth=6, **kw), 0, 2)
+        assert not np.array_equal(img, base), kw
+
+
+def test_vcm_running_mean_split():
+    o = ol.OracleScene(scenes.box_scene(16, 16))
+    cfg = ol.vcm_settings()
+    full, _, _ = o.render_vcm(cfg, 0, 4)
+    part, _, _ = o.render_vcm(cfg, 0, 2)
+    part, _, _ = o.render_vcm(cfg, 2, 2, image=part)
+    assert np.array_equal(full, part)
+
+
+def test_vcm_rejects_unsupported():
+    s = scenes.box_scene(16, 16)
+    o = ol.OracleScene(s)
+    with pytest.raises(ValueError):
+        o.render_vcm(ol.vcm_settings(max_num_photons=0), 0, 1)
+    with pytest.raises(ValueError):
+        o.render_vcm(ol.vcm_settings(algorithm=7), 0, 1)
+    s.Background = EnvironmentMap(np.ones((8, 16, 3), np.float32))
+    with pytest.raises(ValueError):
+        ol.OracleScene(s).render_vcm(ol.vcm_settings(), 0, 1)
