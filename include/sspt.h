@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define SSPT_ABI_VERSION 7
+#define SSPT_ABI_VERSION 8
 
 typedef enum sspt_status {
     SSPT_OK = 0,
@@ -206,11 +206,19 @@ typedef enum sspt_bvh_builder {
 } sspt_bvh_builder;
 sspt_status sspt_set_bvh_builder(sspt_ctx* ctx, int32_t builder);
 sspt_status sspt_get_bvh_builder(const sspt_ctx* ctx, int32_t* builder);
+/* Triangles per BVH leaf at most (1..16, default 1) for the scenes a context
+ * creates, both builders. No counterpart in the reference (Embree picks its own
+ * leaves); it changes the tree, never a hit: the closest hit is decided by
+ * distance, then by the lower triangle id. */
+sspt_status sspt_set_bvh_leaf_size(sspt_ctx* ctx, int32_t max_triangles);
+sspt_status sspt_get_bvh_leaf_size(const sspt_ctx* ctx, int32_t* max_triangles);
 /* The builder a scene was built with, the summed duration of its build
  * kernels (0 for the host builder) and the wall time of its BVH builds. */
 sspt_status sspt_scene_bvh_build_time(const sspt_scene* scene, int32_t* builder,
                                       double* kernel_ms, double* wall_ms);
-/* Copies the scene's 4-wide BVH as traversed: max_nodes >= num_bvh_nodes
+/* Copies the scene's 4-wide BVH as the builder made it (the traversal reads
+ * the same tree, node for node, with its child planes re-encoded as fp16
+ * offsets rounded outward): max_nodes >= num_bvh_nodes
  * nodes of 32 floats (per axis the 4 child box minima then maxima, then 4
  * child indices and 4 counts as int32, the top byte of the first count
  * holding the node's leaf and inner slot masks), and max_tris >= num_triangles

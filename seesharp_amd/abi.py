@@ -13,7 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 # different occupancy targets); the default is the in-tree build.
 LIB_PATH = os.environ.get("SSPT_LIBRARY") or os.path.join(_HERE, "libsspt.so")
 
-ABI_VERSION = 7  # SSPT_ABI_VERSION of include/sspt.h
+ABI_VERSION = 8  # SSPT_ABI_VERSION of include/sspt.h
 NOALLOC_SCRATCH_BYTES = 64 << 20  # SSPT_NOALLOC_SCRATCH_BYTES
 SSPT_OK = 0
 INVALID_ARGUMENT = 1
@@ -253,6 +253,8 @@ SYMBOLS = {
     "sspt_render_vcm": (C.c_int, [_vp, _vp, C.POINTER(VcmSettings), C.c_uint32, C.c_uint32,
                                   C.POINTER(RayStats)]),
     "sspt_set_bvh_builder": (C.c_int, [_vp, C.c_int32]),
+    "sspt_set_bvh_leaf_size": (C.c_int, [_vp, C.c_int32]),
+    "sspt_get_bvh_leaf_size": (C.c_int, [_vp, C.POINTER(C.c_int32)]),
     "sspt_get_bvh_builder": (C.c_int, [_vp, C.POINTER(C.c_int32)]),
     "sspt_scene_bvh_build_time": (C.c_int, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_double),
                                             C.POINTER(C.c_double)]),

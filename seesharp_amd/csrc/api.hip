@@ -91,6 +91,7 @@ struct sspt_ctx {
     bool use_graphs = true;
     int32_t exec_mode = SSPT_EXEC_WAVEFRONT;
     int32_t bvh_builder = SSPT_BVH_DEVICE;
+    int32_t bvh_leaf_size = kMaxLeafTriangles;
     unsigned long long* lt_stats = nullptr;  // the light tracer's striped ray statistics
     // adjoint buffers, grown on demand
     std::vector<void*> adj_allocs;
@@ -400,14 +401,14 @@ sspt_status add_scene(sspt_ctx* ctx, WorldBuilder& W, const sspt_scene_desc* d) 
     if (ctx->bvh_builder == SSPT_BVH_DEVICE) {
         std::string err;
         double kernel_ms = 0.0;
-        if (!build_bvh_device(ctx->stream, BvhInput{verts.data(), (int)ntri}, kMaxLeafTriangles, &bvh, &kernel_ms,
+        if (!build_bvh_device(ctx->stream, BvhInput{verts.data(), (int)ntri}, ctx->bvh_leaf_size, &bvh, &kernel_ms,
                               &err))
             return fail(ctx, err.find("out of memory") != std::string::npos ? SSPT_ERR_OUT_OF_MEMORY : SSPT_ERR_DEVICE,
                         "device BVH build: %s", err.c_str());
         W.bvh_kernel_ms += kernel_ms;
     } else {
         unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-        build_bvh(BvhInput{verts.data(), (int)ntri}, kMaxLeafTriangles, (int)std::min(hw, 16u), &bvh);
+        build_bvh(BvhInput{verts.data(), (int)ntri}, ctx->bvh_leaf_size, (int)std::min(hw, 16u), &bvh);
     }
     W.bvh_wall_ms +=
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - build_start).count();
@@ -610,13 +611,29 @@ sspt_status create_world(sspt_ctx* ctx, const sspt_scene_desc* descs, uint32_t n
     // texel offsets travel as 32-bit ints through the shading and adjoint kernels
     if (W.texels.size() > (size_t)INT32_MAX)
         return fail(ctx, SSPT_ERR_UNSUPPORTED, "texture data exceeds 2^31 floats");
-    // the traversal addresses nodes by 32-bit byte offsets (128 B per node)
+    // the traversal addresses nodes by 32-bit byte offsets (80 B per traversal node)
     if (W.nodes.size() / 32 > (size_t)kMaxNodes)
         return fail(ctx, SSPT_ERR_UNSUPPORTED, "BVH exceeds 2^25 nodes");
     std::unique_ptr<sspt_scene> sc(new sspt_scene());
     sc->ctx = ctx;
     DevScene& S = sc->dev;
     TRY(upload(ctx, sc.get(), reinterpret_cast<const float4*>(W.nodes.data()), W.nodes.size() / 4, &S.nodes));
+    {
+        // the traversal's copy of the tree (fp16 child planes, 80 B per node); its
+        // encoding counts as BVH build time
+        const auto enc_start = std::chrono::steady_clock::now();
+        std::vector<uint32_t> qwords;
+        std::vector<int> big;
+        std::string err;
+        const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+        if (!encode_traversal_nodes(W.nodes, (int)std::min(hw, 16u), &qwords, &big, &err))
+            return fail(ctx, SSPT_ERR_UNSUPPORTED, "%s", err.c_str());
+        if (big.empty()) big.assign(2, 0);
+        TRY(upload(ctx, sc.get(), qwords.data(), qwords.size(), &S.qnodes));
+        TRY(upload(ctx, sc.get(), big.data(), big.size(), &S.big_leaves));
+        W.bvh_wall_ms +=
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - enc_start).count();
+    }
     TRY(upload(ctx, sc.get(), W.tri_pos.data(), W.tri_pos.size(), &S.tri_pos));
     TRY(upload(ctx, sc.get(), W.tri_aux.data(), W.tri_aux.size(), &S.tri_aux));
     TRY(upload(ctx, sc.get(), W.tri_emitter.data(), W.tri_emitter.size(), &S.tri_emitter));
@@ -1162,6 +1179,20 @@ sspt_status sspt_set_bvh_builder(sspt_ctx* ctx, int32_t builder) {
 sspt_status sspt_get_bvh_builder(const sspt_ctx* ctx, int32_t* builder) {
     if (!ctx || !builder) return SSPT_ERR_INVALID_ARGUMENT;
     *builder = ctx->bvh_builder;
+    return SSPT_OK;
+}
+
+sspt_status sspt_set_bvh_leaf_size(sspt_ctx* ctx, int32_t max_triangles) {
+    if (!ctx) return SSPT_ERR_INVALID_ARGUMENT;
+    if (max_triangles < 1 || max_triangles > 16)
+        return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "BVH leaf size %d outside 1..16", max_triangles);
+    ctx->bvh_leaf_size = max_triangles;
+    return SSPT_OK;
+}
+
+sspt_status sspt_get_bvh_leaf_size(const sspt_ctx* ctx, int32_t* max_triangles) {
+    if (!ctx || !max_triangles) return SSPT_ERR_INVALID_ARGUMENT;
+    *max_triangles = ctx->bvh_leaf_size;
     return SSPT_OK;
 }
 

@@ -275,4 +275,155 @@ void build_bvh(const BvhInput& in, int max_leaf, int threads, BvhOutput* out) {
     }
 }
 
+namespace {
+
+// fp16 helpers for traversal-node planes: values are non-negative, finite and
+// below 2^16; only zero and normal fp16 numbers are produced.
+uint16_t f16_bits(double v) {  // v: zero or an exactly representable normal fp16
+    if (v == 0.0) return 0;
+    int p;
+    const double m = std::frexp(v, &p);  // v = m 2^p, m in [0.5, 1)
+    const uint32_t mant = (uint32_t)((2.0 * m - 1.0) * 1024.0);
+    return (uint16_t)(((uint32_t)(p - 1 + 15) << 10) | mant);
+}
+double f16_value(uint16_t b) {
+    if (b == 0) return 0.0;
+    return std::ldexp(1.0 + (double)(b & 1023u) / 1024.0, (int)(b >> 10) - 15);
+}
+// largest zero-or-normal fp16 <= y
+uint16_t f16_floor(double y) {
+    if (!(y >= 0x1p-14)) return 0;
+    int p;
+    std::frexp(y, &p);
+    const double q = std::ldexp(1.0, p - 1 - 10);
+    return f16_bits(std::floor(y / q) * q);
+}
+// smallest normal fp16 >= y (y > 0), 0 for y <= 0
+uint16_t f16_ceil(double y) {
+    if (y <= 0.0) return 0;
+    if (y <= 0x1p-14) return f16_bits(0x1p-14);
+    int p;
+    std::frexp(y, &p);
+    const double q = std::ldexp(1.0, p - 1 - 10);
+    return f16_bits(std::ceil(y / q) * q);
+}
+// a - b as s + err exactly (Knuth's TwoSum in double)
+double diff_exact(float a, float b, double* err) {
+    const double x = (double)a, y = -(double)b;
+    const double s = x + y, bb = s - x;
+    *err = (x - (s - bb)) + (y - bb);
+    return s;
+}
+
+}  // namespace
+
+bool encode_traversal_nodes(const std::vector<float>& nodes, int threads, std::vector<uint32_t>* words,
+                            std::vector<int>* big_leaves, std::string* error) {
+    const size_t nn = nodes.size() / 32;
+    auto count_of = [&](size_t n, int k) -> int {
+        int32_t c;
+        std::memcpy(&c, &nodes[n * 32 + 28 + k], 4);
+        if (k == 0) c = (c << 8) >> 8;  // slot 0 keeps the slot masks in its top byte
+        return c;
+    };
+    auto child_of = [&](size_t n, int k) -> int32_t {
+        int32_t c;
+        std::memcpy(&c, &nodes[n * 32 + 24 + k], 4);
+        return c;
+    };
+    // leaves of more than four triangles: numbered in node order
+    std::vector<int> big_index(nn * 4, -1);
+    big_leaves->clear();
+    for (size_t n = 0; n < nn; ++n)
+        for (int k = 0; k < 4; ++k) {
+            const int c = count_of(n, k);
+            if (c > 4) {
+                big_index[n * 4 + k] = (int)(big_leaves->size() / 2);
+                big_leaves->push_back(child_of(n, k));
+                big_leaves->push_back(c);
+            } else if (c > 0 && (uint32_t)child_of(n, k) >= (1u << 30)) {
+                if (error) *error = "a leaf's first triangle index exceeds 2^30";
+                return false;
+            }
+        }
+    words->assign(nn * kTravNodeWords, 0u);
+    auto encode = [&](size_t n0, size_t n1) {
+        for (size_t n = n0; n < n1; ++n) {
+            const float* f = &nodes[n * 32];
+            uint32_t* w = &(*words)[n * kTravNodeWords];
+            int kind[4];
+            uint32_t leafm = 0, innerm = 0;
+            for (int k = 0; k < 4; ++k) {
+                const int c = count_of(n, k);
+                kind[k] = c < 0 ? -1 : c;  // -1 absent, 0 inner, > 0 leaf of c triangles
+                if (c == 0) innerm |= 1u << k;
+                if (c > 0 && c <= 4) leafm |= 1u << k;
+                if (c > 4) {
+                    leafm |= 1u << k;
+                    innerm |= 1u << k;
+                }
+            }
+            uint32_t meta = (leafm | (innerm << 4)) << 24;
+            for (int a = 0; a < 3; ++a) {
+                const float* lo = f + 4 * (2 * a);
+                const float* hi = f + 4 * (2 * a + 1);
+                float origin = 0.0f;
+                bool any = false;
+                for (int k = 0; k < 4; ++k)
+                    if (kind[k] >= 0) {
+                        origin = any ? std::min(origin, lo[k]) : lo[k];
+                        any = true;
+                    }
+                double extent = 0.0;
+                for (int k = 0; k < 4; ++k)
+                    if (kind[k] >= 0) extent = std::max(extent, (double)hi[k] - (double)origin);
+                int e = 0;
+                if (extent > 0.0) {
+                    int p;
+                    std::frexp(extent, &p);  // extent in [2^(p-1), 2^p): offsets * 2^-e below 2^15
+                    e = std::min(127, std::max(-128, p - 15));
+                }
+                std::memcpy(&w[a], &origin, 4);
+                meta |= (uint32_t)(uint8_t)(int8_t)e << (8 * a);
+                uint16_t hl[4] = {0, 0, 0, 0}, hh[4] = {0, 0, 0, 0};
+                for (int k = 0; k < 4; ++k) {
+                    if (kind[k] < 0) continue;
+                    double err;
+                    // lower plane rounded down: origin + h 2^e <= lo
+                    double sl = diff_exact(lo[k], origin, &err);
+                    double x = std::ldexp(sl, -e);
+                    uint16_t h = f16_floor(x);
+                    if (err < 0.0 && f16_value(h) == x) h = h > 0x0400 ? (uint16_t)(h - 1) : (uint16_t)0;
+                    hl[k] = h;
+                    // upper plane rounded up: origin + h 2^e >= hi
+                    double sh = diff_exact(hi[k], origin, &err);
+                    x = std::ldexp(sh, -e);
+                    h = f16_ceil(x);
+                    if (err > 0.0 && f16_value(h) == x) h = h == 0 ? (uint16_t)0x0400 : (uint16_t)(h + 1);
+                    hh[k] = h;
+                }
+                w[4 + 4 * a] = (uint32_t)hl[0] | ((uint32_t)hl[1] << 16);
+                w[5 + 4 * a] = (uint32_t)hl[2] | ((uint32_t)hl[3] << 16);
+                w[6 + 4 * a] = (uint32_t)hh[0] | ((uint32_t)hh[1] << 16);
+                w[7 + 4 * a] = (uint32_t)hh[2] | ((uint32_t)hh[3] << 16);
+            }
+            w[3] = meta;
+            for (int k = 0; k < 4; ++k) {
+                int32_t c;
+                if (kind[k] < 0) c = -1;
+                else if (kind[k] == 0) c = child_of(n, k);
+                else if (kind[k] <= 4) c = (int32_t)((uint32_t)child_of(n, k) | ((uint32_t)(kind[k] - 1) << 30));
+                else c = big_index[n * 4 + k];
+                std::memcpy(&w[16 + k], &c, 4);
+            }
+        }
+    };
+    threads = std::max(1, std::min(threads, (int)(nn / 4096) + 1));
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads; ++t)
+        pool.emplace_back(encode, nn * t / threads, nn * (t + 1) / threads);
+    for (auto& th : pool) th.join();
+    return true;
+}
+
 }  // namespace sspt

@@ -50,4 +50,12 @@ void build_bvh(const BvhInput& in, int max_leaf, int threads, BvhOutput* out);
 bool build_bvh_device(void* stream, const BvhInput& in, int max_leaf, BvhOutput* out, double* device_ms,
                       std::string* error);
 
+// Traversal nodes (scene_types.h): re-encodes a 4-wide float tree (32 floats
+// per node, the builders' layout) as kTravNodeWords words per node, same node
+// indices. Leaves of more than four triangles get numbered (first, count)
+// pairs in `big_leaves`. Fails (filling `error`) if a leaf starts at a
+// triangle index of 2^30 or more.
+bool encode_traversal_nodes(const std::vector<float>& nodes, int threads, std::vector<uint32_t>* words,
+                            std::vector<int>* big_leaves, std::string* error);
+
 }  // namespace sspt

@@ -127,7 +127,6 @@ __device__ __forceinline__ void camera_ray(const DevCamera& c, float fx, float f
 struct RayPre {
     int kx, ky, kz;
     float sx, sy, sz;
-    f3 inv;
 };
 __device__ __forceinline__ RayPre precompute(f3 d) {
     RayPre p;
@@ -144,7 +143,6 @@ __device__ __forceinline__ RayPre precompute(f3 d) {
     p.sx = comp(d, p.kx) / dz;
     p.sy = comp(d, p.ky) / dz;
     p.sz = 1.0f / dz;
-    p.inv = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
     return p;
 }
 
@@ -194,21 +192,93 @@ __device__ __forceinline__ bool intersect_tri(f3 o, const RayPre& p, float tmin,
     return true;
 }
 
-// Robust slab test (Ize, JCGT 2013): far distances widened by 1 + 2 gamma(3).
-// The caller widens the query interval by the ray's slack (see traverse).
-// nx / fx: the child's near and far x plane for this ray (lo if the direction's
-// x is positive, else hi; the sign bit decides, so -0 counts as negative). As
-// rounding is monotonic, (near - o) * inv is the smaller of the two slab
-// distances and (far - o) * inv the larger, the values min / max would pick;
-// the one exception is a zero direction component with the origin on a plane
-// (0 * inf = NaN), where fmaxf / fminf drop the NaN and the box is kept.
-__device__ __forceinline__ bool box_test(float nx, float fx, float ny, float fy, float nz, float fz, f3 o, f3 inv,
-                                         float tmin, float tmax, float* tnear) {
-    float tn = fmaxf(fmaxf((nx - o.x) * inv.x, (ny - o.y) * inv.y), fmaxf((nz - o.z) * inv.z, tmin));
-    // scaling the minimum equals the minimum of the scaled values (rounding is monotonic)
-    float tf = fminf(fminf(fminf((fx - o.x) * inv.x, (fy - o.y) * inv.y), (fz - o.z) * inv.z) * 1.0000004f, tmax);
-    *tnear = tn;
-    return tn <= tf;
+// One traversal node's four child boxes (traversal node layout, scene_types.h):
+// per axis the planes are origin + h * 2^e with fp16 offsets h, so the slab
+// distance of a plane is h * (2^e * inv) + (origin * inv - q), one fma with an
+// fp16 operand (v_fma_mix) per plane, where q is the ray's o * inv shifted by a
+// margin per axis (BoxRay). The near plane of an axis is the lower one if the
+// direction's component is positive, else the upper one (the sign bit decides,
+// so -0 counts as negative). The margins make the computed near distance no
+// larger and the far distance no smaller than the exact slab distances of the
+// decoded planes, which enclose the children's boxes, so a box the ray meets
+// within [tmin, tmax] is never culled. Returns the hit slots as a bit mask and
+// their entry distances in tc (each in [tmin, tmax] for a hit slot).
+__device__ __forceinline__ float h16(uint32_t w, int upper) {
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)(upper ? w >> 16 : w & 0xffffu));
+}
+template <class BR>
+__device__ __forceinline__ uint32_t child_boxes(const uint32_t (&w)[kTravNodeWords], const BR& box, float tmin,
+                                                float tmax, float (&tc)[4]) {
+    const uint32_t meta = w[3];
+    const float ox = __uint_as_float(w[0]), oy = __uint_as_float(w[1]), oz = __uint_as_float(w[2]);
+    const float sx = ldexpf(box.inv.x, (int)(int8_t)(meta & 0xffu));
+    const float sy = ldexpf(box.inv.y, (int)(int8_t)((meta >> 8) & 0xffu));
+    const float sz = ldexpf(box.inv.z, (int)(int8_t)((meta >> 16) & 0xffu));
+    const float bnx = fmaf(ox, box.inv.x, -box.qn.x), bfx = fmaf(ox, box.inv.x, -box.qf.x);
+    const float bny = fmaf(oy, box.inv.y, -box.qn.y), bfy = fmaf(oy, box.inv.y, -box.qf.y);
+    const float bnz = fmaf(oz, box.inv.z, -box.qn.z), bfz = fmaf(oz, box.inv.z, -box.qf.z);
+    const bool gx = signbit(box.inv.x), gy = signbit(box.inv.y), gz = signbit(box.inv.z);
+    const uint32_t nx0 = gx ? w[6] : w[4], nx1 = gx ? w[7] : w[5], fx0 = gx ? w[4] : w[6], fx1 = gx ? w[5] : w[7];
+    const uint32_t ny0 = gy ? w[10] : w[8], ny1 = gy ? w[11] : w[9], fy0 = gy ? w[8] : w[10], fy1 = gy ? w[9] : w[11];
+    const uint32_t nz0 = gz ? w[14] : w[12], nz1 = gz ? w[15] : w[13], fz0 = gz ? w[12] : w[14],
+                   fz1 = gz ? w[13] : w[15];
+    uint32_t hit = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t nxw = k < 2 ? nx0 : nx1, fxw = k < 2 ? fx0 : fx1;
+        const uint32_t nyw = k < 2 ? ny0 : ny1, fyw = k < 2 ? fy0 : fy1;
+        const uint32_t nzw = k < 2 ? nz0 : nz1, fzw = k < 2 ? fz0 : fz1;
+        const int u = k & 1;
+        const float tn = fmaxf(fmaxf(fmaf(h16(nxw, u), sx, bnx), fmaf(h16(nyw, u), sy, bny)),
+                               fmaxf(fmaf(h16(nzw, u), sz, bnz), tmin));
+        const float tf = fminf(fminf(fmaf(h16(fxw, u), sx, bfx), fmaf(h16(fyw, u), sy, bfy)),
+                               fminf(fmaf(h16(fzw, u), sz, bfz), tmax));
+        tc[k] = tn;
+        hit |= (uint32_t)(tn <= tf) << k;
+    }
+    return hit;
+}
+
+// Per-ray constants of child_boxes. Let M = max(|o|, R), R the scene's largest
+// vertex coordinate magnitude: node origins lie within R and decoded planes P
+// within 1.004 R (an upper plane rounds up by at most 2^-9 of its node's
+// extent, which is below 2 R). With A = 2^-20 * M * |inv| per axis,
+// qn = o * inv + A and qf = o * inv - A (each one rounding, u = 2^-24). A
+// computed near distance h * s + (origin * inv - qn), s = 2^e * inv exact,
+// differs from (P - o) * inv - A by at most 5.1 u M |inv| (the roundings of qn,
+// of the base and of the fma), and (P - o) * inv from the exact (P - o) / d by
+// at most 2.1 u M |inv| (the rounded reciprocal), so it is at least 8 u M |inv|
+// below the exact near distance, and a far distance as far above the exact one.
+// Direction components below 2^-40 of the largest one are replaced by +-2^-40
+// of it (sign kept): the altered ray stays within t * 2^-40 * |d|max of the
+// true one along that axis, and where the true ray can be inside a box
+// (t * |d|max <= 2 M) that is far below the margin's 2^-20 M, so the
+// substitution keeps the test conservative. A direction shorter than 2^-60 M
+// (or not finite) gets inv = 0, qn = +inf, qf = -inf: every box is entered.
+// Otherwise M |inv| <= 2^100 and every distance is finite.
+struct BoxRay {
+    f3 inv, qn, qf;
+};
+__device__ __forceinline__ BoxRay box_ray(float abs_max, f3 o, f3 d) {
+    const float dmax = fmaxf(fmaxf(fabsf(d.x), fabsf(d.y)), fabsf(d.z));
+    const float lo = dmax * 0x1p-40f;
+    const float cx = fabsf(d.x) < lo ? copysignf(lo, d.x) : d.x;
+    const float cy = fabsf(d.y) < lo ? copysignf(lo, d.y) : d.y;
+    const float cz = fabsf(d.z) < lo ? copysignf(lo, d.z) : d.z;
+    BoxRay b;
+    const float mm = fmaxf(abs_max, fmaxf(fmaxf(fabsf(o.x), fabsf(o.y)), fabsf(o.z)));
+    if (!(dmax > mm * 0x1p-60f)) {
+        b.inv = mk3(0.0f, 0.0f, 0.0f);
+        b.qn = mk3(INFINITY, INFINITY, INFINITY);
+        b.qf = mk3(-INFINITY, -INFINITY, -INFINITY);
+        return b;
+    }
+    b.inv = mk3(1.0f / cx, 1.0f / cy, 1.0f / cz);
+    const float m = mm * 0x1p-20f;
+    const f3 a = mk3(m * fabsf(b.inv.x), m * fabsf(b.inv.y), m * fabsf(b.inv.z));
+    b.qn = mk3(fmaf(o.x, b.inv.x, a.x), fmaf(o.y, b.inv.y, a.y), fmaf(o.z, b.inv.z, a.z));
+    b.qf = mk3(fmaf(o.x, b.inv.x, -a.x), fmaf(o.y, b.inv.y, -a.y), fmaf(o.z, b.inv.z, -a.z));
+    return b;
 }
 
 // The watertight t carries an absolute error of a few ulps of the coordinate
@@ -293,8 +363,8 @@ template <bool ANY, int TB>
 struct Tracer {
     f3 o;
     RayPre p;
+    BoxRay box;
     float tmin, slack, tmin_box;
-    uint32_t near_x, near_y, near_z;  // byte offsets of the near bound vectors in a node
     HitOut best;
     TraceStack<TB> st;
     int* pend;     // FIFO of queued leaves in LDS (+ threadIdx.x, stride TB)
@@ -306,12 +376,10 @@ struct Tracer {
     __device__ __forceinline__ void start(int root, float abs_max, f3 o_, f3 d, float tmin_, float tmax) {
         o = o_;
         p = precompute(d);
+        box = box_ray(abs_max, o_, d);
         slack = ray_slack(abs_max, o_, d);
         tmin = tmin_;
         tmin_box = tmin_ - slack;
-        near_x = signbit(p.inv.x) ? 16u : 0u;
-        near_y = signbit(p.inv.y) ? 48u : 32u;
-        near_z = signbit(p.inv.z) ? 80u : 64u;
         best = HitOut{tmax, 0.0f, 0.0f, -1, 0x7fffffff};
         st.sp = 0;
         ph = 0;
@@ -350,61 +418,85 @@ struct Tracer {
     // Box tests of the current node's children: hit leaves are queued (a leaf
     // of more than four triangles, which only the build's depth bound makes,
     // is tested here), hit inner children are pushed nearest first and the
-    // nearest becomes the next node, else the stack is popped.
+    // nearest becomes the next node, else the stack is popped. The node comes
+    // through the scalar data cache when every stepping lane of the wave is at
+    // it, else with five 16-byte vector loads per lane: the traversal's time
+    // follows its vector memory instructions (a 64-lane scattered load costs
+    // the texture units about the same whatever its width), not the bytes.
     __device__ __forceinline__ void node_step(const DevScene& S) {
         if ((unsigned)node >= (unsigned)S.num_nodes) {  // never taken for a valid tree
             done = true;
             return;
         }
-        // the near / far bound vectors of each axis at the ray's byte offsets
-        // (far = near ^ 16); scene creation bounds node * 128 below 2^32
-        const char* B = reinterpret_cast<const char*>(S.nodes);
-        const uint32_t nb = (uint32_t)node << 7;
-        const uint32_t ax = nb + near_x, ay = nb + near_y, az = nb + near_z;
-        const float4 nx = *reinterpret_cast<const float4*>(B + ax), fx = *reinterpret_cast<const float4*>(B + (ax ^ 16u));
-        const float4 ny = *reinterpret_cast<const float4*>(B + ay), fy = *reinterpret_cast<const float4*>(B + (ay ^ 16u));
-        const float4 nz = *reinterpret_cast<const float4*>(B + az), fz = *reinterpret_cast<const float4*>(B + (az ^ 16u));
-        int4 child = *reinterpret_cast<const int4*>(B + nb + 96u);
-        int4 count = *reinterpret_cast<const int4*>(B + nb + 112u);
-        const float tmax_box = best.t + slack;
+        // finite, so that a hit child's entry distance (<= tmax_box) is finite
+        const float tmax_box = fminf(best.t + slack, 3.402823466e38f);
         float tc[4];
+        uint32_t hit, kinds;
+        int cw[4];
+        const int nu = __builtin_amdgcn_readfirstlane(node);
+        if (__ballot(node == nu) == __ballot(true)) {
+            // every stepping lane of the wave visits the same node: scalar loads
+            // (the scalar data cache), no vector memory instructions
+            typedef const __attribute__((address_space(4))) uint32_t ConstU32;
+            ConstU32* q = (ConstU32*)(S.qnodes + (size_t)nu * kTravNodeWords);
+            uint32_t w[kTravNodeWords];
+#pragma unroll
+            for (int i = 0; i < kTravNodeWords; ++i) w[i] = q[i];
+            hit = child_boxes(w, box, tmin_box, tmax_box, tc);
+            kinds = w[3] >> 24;
+            cw[0] = (int)w[16];
+            cw[1] = (int)w[17];
+            cw[2] = (int)w[18];
+            cw[3] = (int)w[19];
+        } else {
+            // 80 B per node; scene creation bounds node * 80 below 2^32
+            const char* B = reinterpret_cast<const char*>(S.qnodes);
+            const uint32_t nb = (uint32_t)node * 80u;
+            uint32_t w[kTravNodeWords];
+#pragma unroll
+            for (int i = 0; i < 5; ++i) {
+                const uint4 v = *reinterpret_cast<const uint4*>(B + nb + 16u * i);
+                w[4 * i] = v.x;
+                w[4 * i + 1] = v.y;
+                w[4 * i + 2] = v.z;
+                w[4 * i + 3] = v.w;
+            }
+            hit = child_boxes(w, box, tmin_box, tmax_box, tc);
+            kinds = w[3] >> 24;
+            cw[0] = (int)w[16];
+            cw[1] = (int)w[17];
+            cw[2] = (int)w[18];
+            cw[3] = (int)w[19];
+        }
+        const uint32_t leafk = kinds & 15u, innerk = kinds >> 4;
+        uint32_t leaves = hit & leafk & ~innerk;  // hit leaves of at most four triangles
+        uint32_t bigs = hit & leafk & innerk;     // hit leaves of more triangles
+        const uint32_t inner_hit = hit & innerk & ~leafk;
         bool hc[4];
-        hc[0] = box_test(nx.x, fx.x, ny.x, fy.x, nz.x, fz.x, o, p.inv, tmin_box, tmax_box, &tc[0]);
-        hc[1] = box_test(nx.y, fx.y, ny.y, fy.y, nz.y, fz.y, o, p.inv, tmin_box, tmax_box, &tc[1]);
-        hc[2] = box_test(nx.z, fx.z, ny.z, fy.z, nz.z, fz.z, o, p.inv, tmin_box, tmax_box, &tc[2]);
-        hc[3] = box_test(nx.w, fx.w, ny.w, fy.w, nz.w, fz.w, o, p.inv, tmin_box, tmax_box, &tc[3]);
-        const int cidx[4] = {child.x, child.y, child.z, child.w};
-        const int ccnt[4] = {count.x & 0x00FFFFFF, count.y, count.z, count.w};
-        // slot kinds as bit masks, precomputed in the top byte of slot 0's count
-        const uint32_t hit = (uint32_t)hc[0] | ((uint32_t)hc[1] << 1) | ((uint32_t)hc[2] << 2) | ((uint32_t)hc[3] << 3);
-        const uint32_t meta = (uint32_t)count.x >> 24;
-        const uint32_t leaf = meta & 15u, inner = meta >> 4;
-        uint32_t leaves = hit & leaf;  // hit leaf children
-        const uint32_t inner_hit = hit & inner;
 #pragma unroll
         for (int k = 0; k < 4; ++k) hc[k] = (inner_hit >> k) & 1u;
         while (leaves) {
             const int k = __builtin_ctz(leaves);
             leaves &= leaves - 1u;
-            const int tri = k == 0 ? cidx[0] : k == 1 ? cidx[1] : k == 2 ? cidx[2] : cidx[3];
-            const int n = k == 0 ? ccnt[0] : k == 1 ? ccnt[1] : k == 2 ? ccnt[2] : ccnt[3];
-            if (n <= 4) {
-                // entry: first triangle (< 2^30) and count - 1
-                pend[((ph + pn) & (uint32_t)(kPendCap - 1)) * TB] = tri | ((n - 1) << 30);
-                ++pn;
-            } else {
-                test_leaf(S, tri, n);
-                if (ANY && done) return;
-            }
+            // the child word is the FIFO entry: first triangle (< 2^30) | (count - 1) << 30
+            pend[((ph + pn) & (uint32_t)(kPendCap - 1)) * TB] = k == 0 ? cw[0] : k == 1 ? cw[1] : k == 2 ? cw[2] : cw[3];
+            ++pn;
         }
-        // inner children, nearest first. A hit child's key is clamped to FLT_MAX
-        // (its entry distance is +inf when the ray runs parallel to a slab and no
-        // hit bounds tmax yet) so every hit child sorts before every other slot.
-        float t0 = hc[0] ? fminf(tc[0], 3.402823466e38f) : INFINITY;
-        float t1 = hc[1] ? fminf(tc[1], 3.402823466e38f) : INFINITY;
-        float t2 = hc[2] ? fminf(tc[2], 3.402823466e38f) : INFINITY;
-        float t3 = hc[3] ? fminf(tc[3], 3.402823466e38f) : INFINITY;
-        int n0 = cidx[0], n1 = cidx[1], n2 = cidx[2], n3 = cidx[3];
+        while (bigs) {
+            const int k = __builtin_ctz(bigs);
+            bigs &= bigs - 1u;
+            const int b = k == 0 ? cw[0] : k == 1 ? cw[1] : k == 2 ? cw[2] : cw[3];
+            test_leaf(S, S.big_leaves[2 * b], S.big_leaves[2 * b + 1]);
+            if (ANY && done) return;
+        }
+        // inner children, nearest first. A hit child's entry distance lies in
+        // [tmin_box, tmax_box] (finite), so every hit child sorts before every
+        // other slot (key +inf).
+        float t0 = hc[0] ? tc[0] : INFINITY;
+        float t1 = hc[1] ? tc[1] : INFINITY;
+        float t2 = hc[2] ? tc[2] : INFINITY;
+        float t3 = hc[3] ? tc[3] : INFINITY;
+        int n0 = cw[0], n1 = cw[1], n2 = cw[2], n3 = cw[3];
         int nhit = __builtin_popcount(inner_hit);
         if (nhit > 0) {
             cswap(t0, n0, t1, n1);

@@ -20,6 +20,20 @@
 //                    -1 = absent slot (box [+inf, +inf]^3, skipped by the traversal);
 //                    slot 0 keeps its count in bits 0..23 (two's complement for -1) and
 //                    the node's slot masks in bits 24..31: leaf slots, then inner slots << 4
+// Traversal nodes: the same tree re-encoded for the traversal kernels in
+// kTravNodeWords = 20 words (80 B, five 16-byte loads) per node, same indices:
+//   w[0..2]   node origin (float): the lowest lower child plane per axis
+//   w[3]      bytes 0..2: exponent e per axis (int8); byte 3: slot kinds, bits 0..3
+//             "leaf" slots, bits 4..7 "inner" slots: leaf only = leaf of 1..4
+//             triangles, inner only = inner node, both = leaf of more triangles,
+//             neither = absent slot
+//   w[4..7]   x planes as fp16 offsets h: lower planes of children 0..3 (w4 = c0 |
+//             c1 << 16, w5 = c2 | c3 << 16), then upper planes (w6, w7); the plane
+//             is origin + h * 2^e, rounded outward (lower planes down, upper up)
+//   w[8..11]  y planes, w[12..15] z planes, as x
+//   w[16..19] child words: inner node index; leaf of n <= 4 triangles: first
+//             triangle | (n - 1) << 30; larger leaf: index of its (first, count)
+//             pair in the big-leaf table; absent: -1
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -43,6 +57,7 @@ constexpr uint32_t kMaxNodes = 1u << 25;
 #endif
 constexpr int kStackLds = SSPT_STACK_LDS;
 constexpr int kStackTotal = 64;
+constexpr int kTravNodeWords = 20;
 #ifndef SSPT_TRACE_BLOCK
 #define SSPT_TRACE_BLOCK 128
 #endif
@@ -134,7 +149,9 @@ struct SceneRec {
 };
 
 struct DevScene {
-    const float4* nodes;
+    const float4* nodes;      // the builders' 4-wide float tree (read back by sspt_scene_read_bvh)
+    const uint32_t* qnodes;   // the same tree as traversal nodes
+    const int* big_leaves;    // (first triangle, count) of the leaves of more than four triangles
     const float4* tri_pos;
     const float4* tri_aux;
     const int* tri_emitter;

@@ -64,6 +64,15 @@ class Device:
         """BVH builder of the scenes created from now on: the device (default) or the host."""
         self.check(self.lib.sspt_set_bvh_builder(self.ptr, abi.BVH_HOST if host else abi.BVH_DEVICE))
 
+    def set_bvh_leaf_size(self, max_triangles: int):
+        """Triangles per BVH leaf at most (1..16, default 1) for the scenes created from now on."""
+        self.check(self.lib.sspt_set_bvh_leaf_size(self.ptr, int(max_triangles)))
+
+    def bvh_leaf_size(self) -> int:
+        n = C.c_int32()
+        self.check(self.lib.sspt_get_bvh_leaf_size(self.ptr, C.byref(n)))
+        return n.value
+
     def bvh_builder(self) -> str:
         b = C.c_int32()
         self.check(self.lib.sspt_get_bvh_builder(self.ptr, C.byref(b)))
@@ -190,7 +199,8 @@ class GpuScene(_TextureParameters):
                 "wall_ms": w.value}
 
     def read_bvh(self):
-        """The 4-wide BVH as traversed: nodes (N, 32) float32 (child box planes, then child
+        """The 4-wide BVH as the builder made it (the traversal reads the same tree with
+        fp16 child planes rounded outward): nodes (N, 32) float32 (child box planes, then child
         indices and counts as int32 bits) and the global triangle id of each leaf position."""
         info = self.info()
         nodes = np.zeros((info["num_bvh_nodes"], 32), np.float32)
