@@ -68,14 +68,15 @@ def measured_traffic():
     return {k: int(v) for k, v in re.findall(r"(\w+)=(\d+)", m.group(1))}
 
 
-def measured_valu_issue():
-    """Share of the SIMDs' VALU issue slots each kernel fills on the default
-    large-scene workload, from the committed rocprofv3 SQ counter summary."""
+def measured_share(key):
+    """Per-kernel shares on the default large-scene workload from the committed
+    rocprofv3 counter summary: "valu_issue_share" (VALU issue slots filled) or
+    "td_busy_share" (the texture data units' busy cycles)."""
     try:
         text = open(SQ_PROFILE).read()
     except OSError:
         return {}
-    m = re.search(r"^valu_issue_share:(.*)$", text, re.M)
+    m = re.search(r"^" + key + r":(.*)$", text, re.M)
     if not m:
         return {}
     return {k: float(v) for k, v in re.findall(r"(\w+)=([0-9.]+)", m.group(1))}
@@ -725,7 +726,8 @@ def main():
     kernel_ms = sum(v[1] for v in kernels.values())
     default_workload = (args.res, args.spp, args.max_depth) == (1024, 64, 8)
     traffic = measured_traffic().get(dom) if default_workload else None
-    valu_share = measured_valu_issue().get(dom) if default_workload else None
+    valu_share = measured_share("valu_issue_share").get(dom) if default_workload else None
+    td_share = measured_share("td_busy_share").get(dom) if default_workload else None
 
     out = {
         "metric": METRIC,
@@ -768,10 +770,12 @@ def main():
             "avg_launch_ms": round(avg_ms, 4),
             "launches": launches,
             "alg_bytes_per_launch": int(bytes_per_launch),
-            # what does bound the kernel: its VALU instructions' share of the issue slots
-            # (rocprofv3 SQ counters of this command; the HBM fraction above is far from 1)
+            # what does bound the kernel (rocprofv3 counters of this command; the HBM
+            # fraction above is far from 1): its VALU instructions' share of the issue
+            # slots and the texture data units' busy share (scattered node loads)
             "limiter": "valu_issue" if valu_share else None,
             "valu_issue_share": valu_share,
+            "td_busy_share": td_share,
         },
         "image_mean": float(img.mean()),
         "nan_paths": gs.nan_count(),

@@ -302,7 +302,9 @@ struct HitOut {
 // array that only deep trees reach. The private arrays are separate locals of
 // the caller so that the traversal state around them stays in registers.
 constexpr int kStackSpill = kStackTotal - kStackLds;
-template <int TB>  // threads per block: the stride of a lane's LDS entries
+// KEEP_T: entries carry the node's entry distance (closest-hit traversal culls
+// popped nodes behind the best hit); an any-hit traversal keeps nodes only.
+template <int TB, bool KEEP_T>  // TB: threads per block, the stride of a lane's LDS entries
 struct TraceStack {
     int* lds_node;    // + threadIdx.x, stride TB
     float* lds_t;
@@ -315,21 +317,27 @@ struct TraceStack {
         if (sp >= kStackTotal) return;
         if (sp < kStackLds) {
             lds_node[sp * TB] = node;
-            lds_t[sp * TB] = t;
+            if (KEEP_T) lds_t[sp * TB] = t;
         } else {
             spill_node[sp - kStackLds] = node;
-            spill_t[sp - kStackLds] = t;
+            if (KEEP_T) spill_t[sp - kStackLds] = t;
         }
+        ++sp;
+    }
+    // push when the lane's LDS part has room (the caller checked sp < kStackLds)
+    __device__ __forceinline__ void push_lds(int node, float t) {
+        lds_node[sp * TB] = node;
+        if (KEEP_T) lds_t[sp * TB] = t;
         ++sp;
     }
     __device__ __forceinline__ void pop(int* node, float* t) {
         --sp;
         if (sp < kStackLds) {
             *node = lds_node[sp * TB];
-            *t = lds_t[sp * TB];
+            *t = KEEP_T ? lds_t[sp * TB] : 0.0f;
         } else {
             *node = spill_node[sp - kStackLds];
-            *t = spill_t[sp - kStackLds];
+            *t = KEEP_T ? spill_t[sp - kStackLds] : 0.0f;
         }
     }
 };
@@ -366,7 +374,7 @@ struct Tracer {
     BoxRay box;
     float tmin, slack, tmin_box;
     HitOut best;
-    TraceStack<TB> st;
+    TraceStack<TB, !ANY> st;
     int* pend;     // FIFO of queued leaves in LDS (+ threadIdx.x, stride TB)
     uint32_t ph;    // FIFO head
     uint32_t pn;    // FIFO length
@@ -482,12 +490,14 @@ struct Tracer {
             pend[((ph + pn) & (uint32_t)(kPendCap - 1)) * TB] = k == 0 ? cw[0] : k == 1 ? cw[1] : k == 2 ? cw[2] : cw[3];
             ++pn;
         }
-        while (bigs) {
-            const int k = __builtin_ctz(bigs);
-            bigs &= bigs - 1u;
-            const int b = k == 0 ? cw[0] : k == 1 ? cw[1] : k == 2 ? cw[2] : cw[3];
-            test_leaf(S, S.big_leaves[2 * b], S.big_leaves[2 * b + 1]);
-            if (ANY && done) return;
+        if (__ballot(bigs != 0)) {  // only trees with leaves of more than four triangles
+            while (bigs) {
+                const int k = __builtin_ctz(bigs);
+                bigs &= bigs - 1u;
+                const int b = k == 0 ? cw[0] : k == 1 ? cw[1] : k == 2 ? cw[2] : cw[3];
+                test_leaf(S, S.big_leaves[2 * b], S.big_leaves[2 * b + 1]);
+                if (ANY && done) return;
+            }
         }
         // inner children, nearest first. A hit child's entry distance lies in
         // [tmin_box, tmax_box] (finite), so every hit child sorts before every
@@ -499,14 +509,37 @@ struct Tracer {
         int n0 = cw[0], n1 = cw[1], n2 = cw[2], n3 = cw[3];
         int nhit = __builtin_popcount(inner_hit);
         if (nhit > 0) {
-            cswap(t0, n0, t1, n1);
-            cswap(t2, n2, t3, n3);
-            cswap(t0, n0, t2, n2);
-            cswap(t1, n1, t3, n3);
-            cswap(t1, n1, t2, n2);
-            if (nhit > 3) st.push(n3, t3);
-            if (nhit > 2) st.push(n2, t2);
-            if (nhit > 1) st.push(n1, t1);
+            if (!ANY) {
+                cswap(t0, n0, t1, n1);
+                cswap(t2, n2, t3, n3);
+                cswap(t0, n0, t2, n2);
+                cswap(t1, n1, t3, n3);
+                cswap(t1, n1, t2, n2);
+            } else {
+                // any-hit: an occluder found in any order ends the ray, so the hit
+                // children only move to the front (node indices < 2^25 sort
+                // before the absent key), in index order
+                uint32_t k0 = hc[0] ? (uint32_t)n0 : 0xffffffffu, k1 = hc[1] ? (uint32_t)n1 : 0xffffffffu;
+                uint32_t k2 = hc[2] ? (uint32_t)n2 : 0xffffffffu, k3 = hc[3] ? (uint32_t)n3 : 0xffffffffu;
+                uint32_t a = min(k0, k1), b = max(k0, k1), c = min(k2, k3), d = max(k2, k3);
+                k0 = min(a, c);
+                k2 = max(a, c);
+                k1 = min(b, d);
+                k3 = max(b, d);
+                n0 = (int)k0;
+                n1 = (int)min(k1, k2);
+                n2 = (int)max(k1, k2);
+                n3 = (int)k3;
+            }
+            if (__ballot(st.sp + 3 > kStackLds) == 0) {  // every lane's pushes fit in LDS
+                if (nhit > 3) st.push_lds(n3, t3);
+                if (nhit > 2) st.push_lds(n2, t2);
+                if (nhit > 1) st.push_lds(n1, t1);
+            } else {
+                if (nhit > 3) st.push(n3, t3);
+                if (nhit > 2) st.push(n2, t2);
+                if (nhit > 1) st.push(n1, t1);
+            }
             node = n0;
             return;
         }
