@@ -462,7 +462,7 @@ def bvh_main(args):
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"large scene ({info['num_triangles']} triangles), binned SAH 64 bins, "
-                               "leaves of 1 triangle, 4-wide collapse"},
+                               f"leaves of at most {dev.bvh_leaf_size()} triangles, 4-wide collapse"},
         "bvh_nodes": info["num_bvh_nodes"],
         "device_wall_ms": round(float(np.mean(wall)), 2),
         "host_builder_wall_ms": round(host_wall, 2),

@@ -206,7 +206,7 @@ typedef enum sspt_bvh_builder {
 } sspt_bvh_builder;
 sspt_status sspt_set_bvh_builder(sspt_ctx* ctx, int32_t builder);
 sspt_status sspt_get_bvh_builder(const sspt_ctx* ctx, int32_t* builder);
-/* Triangles per BVH leaf at most (1..16, default 1) for the scenes a context
+/* Triangles per BVH leaf at most (1..16, default 2) for the scenes a context
  * creates, both builders. No counterpart in the reference (Embree picks its own
  * leaves); it changes the tree, never a hit: the closest hit is decided by
  * distance, then by the lower triangle id. */

@@ -29,9 +29,10 @@ struct BvhOutput {
 #endif
 constexpr int kSahBins = SSPT_SAH_BINS;
 
-// Triangles per leaf at most (the builder splits every larger range).
+// Triangles per leaf at most by default (the builder splits every larger range;
+// sspt_set_bvh_leaf_size changes it per context).
 #ifndef SSPT_MAX_LEAF
-#define SSPT_MAX_LEAF 1
+#define SSPT_MAX_LEAF 2
 #endif
 constexpr int kMaxLeafTriangles = SSPT_MAX_LEAF;
 

@@ -509,6 +509,13 @@ struct Tracer {
         int n0 = cw[0], n1 = cw[1], n2 = cw[2], n3 = cw[3];
         int nhit = __builtin_popcount(inner_hit);
         if (nhit > 0) {
+            if (!ANY && __ballot(nhit > 1) == 0) {
+                // no lane of the wave has two hit children: visit the one, no sort
+                const uint32_t k0 = hc[0] ? (uint32_t)n0 : 0xffffffffu, k1 = hc[1] ? (uint32_t)n1 : 0xffffffffu;
+                const uint32_t k2 = hc[2] ? (uint32_t)n2 : 0xffffffffu, k3 = hc[3] ? (uint32_t)n3 : 0xffffffffu;
+                node = (int)min(min(k0, k1), min(k2, k3));
+                return;
+            }
             if (!ANY) {
                 cswap(t0, n0, t1, n1);
                 cswap(t2, n2, t3, n3);
@@ -586,7 +593,8 @@ __device__ __forceinline__ HitOut traverse(const DevScene& S, int root, float ab
         // triangle step: kPendPercent % of the live lanes have a leaf queued,
         // a lane cannot step for a full FIFO, or no lane has a node left
         if (queued != 0 &&
-            (can == 0 || walking != can || 100 * __popcll(queued) >= kPendPercent * __popcll(act))) {
+            (can == 0 || walking != can ||
+             100 * __popcll(queued) >= (ANY ? kPendPercentAny : kPendPercent) * __popcll(act))) {
             if (live && tr.pn > 0) tr.tri_step(S);
         }
     }

@@ -75,6 +75,11 @@ static_assert(kPendCap >= 4 && (kPendCap & (kPendCap - 1)) == 0, "kPendCap: a po
 #define SSPT_PEND_PERCENT 80
 #endif
 constexpr int kPendPercent = SSPT_PEND_PERCENT;
+// The same for any-hit (shadow) rays: a queued leaf may end the ray.
+#ifndef SSPT_PEND_PERCENT_ANY
+#define SSPT_PEND_PERCENT_ANY 80
+#endif
+constexpr int kPendPercentAny = SSPT_PEND_PERCENT_ANY;
 constexpr int kStackNodeLds = (kStackLds + kPendCap) * kTraceBlock;
 
 // Queue entries pack (scene, slot): slot in the low 24 bits, scene above.

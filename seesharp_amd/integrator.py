@@ -65,7 +65,7 @@ class Device:
         self.check(self.lib.sspt_set_bvh_builder(self.ptr, abi.BVH_HOST if host else abi.BVH_DEVICE))
 
     def set_bvh_leaf_size(self, max_triangles: int):
-        """Triangles per BVH leaf at most (1..16, default 1) for the scenes created from now on."""
+        """Triangles per BVH leaf at most (1..16, default 2) for the scenes created from now on."""
         self.check(self.lib.sspt_set_bvh_leaf_size(self.ptr, int(max_triangles)))
 
     def bvh_leaf_size(self) -> int:

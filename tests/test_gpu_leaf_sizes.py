@@ -1,9 +1,9 @@
 """BVH leaf sizes (sspt_set_bvh_leaf_size) through the traversal.
 
-With the default single-triangle leaves every queued leaf holds one triangle.
-Larger leaf sizes make the traversal nodes' other leaf kinds: leaves of two to
-four triangles, queued as one entry, and leaves of more than four, read from
-the big-leaf table and tested in place. The closest hit is decided by distance
+Single-triangle leaves queue one triangle per FIFO entry; larger leaf sizes
+(the default is two) make the traversal nodes' other leaf kinds: leaves of two
+to four triangles, queued as one entry, and leaves of more than four, read
+from the big-leaf table and tested in place. The closest hit is decided by distance
 and then by the lower triangle id whatever the tree, so every leaf size must
 give brute force's closest hits and occlusion answers, and renders identical
 bit for bit to the single-triangle tree's, with the same ray counts."""
@@ -21,11 +21,12 @@ SIZES = (2, 4, 8, 16)
 
 
 def build(device, scene, leaf):
+    before = device.bvh_leaf_size()
     device.set_bvh_leaf_size(leaf)
     try:
         return GpuScene(device, scene)
     finally:
-        device.set_bvh_leaf_size(1)
+        device.set_bvh_leaf_size(before)
 
 
 def leaf_counts(g):
@@ -64,6 +65,7 @@ def test_soup_rays_match_brute_force(device, soup, leaf):
 
 
 def test_large_scene_renders_equal_single_triangle_leaves(device):
+    # leaf size 1 is the reference tree here; SIZES include the default 2
     s = scenes.large_scene(96, 96, num_spheres=60, grid=64)
     cfg = ol.settings(total_spp=4, max_depth=8)
     base = build(device, s, 1)
@@ -84,7 +86,9 @@ def test_large_scene_renders_equal_single_triangle_leaves(device):
 def test_leaf_size_is_checked(device):
     from seesharp_amd import abi
 
+    before = device.bvh_leaf_size()
+    assert before == 2  # the library's default
     for bad in (0, 17, -1):
         with pytest.raises(abi.SsptError):
             device.set_bvh_leaf_size(bad)
-    assert device.bvh_leaf_size() == 1
+    assert device.bvh_leaf_size() == before
