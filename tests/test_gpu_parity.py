@@ -152,6 +152,39 @@ def test_large_positive_octant_and_axis_parallel_rays(large):
     assert np.array_equal(g.trace_occluded(rays), o.trace_occluded(rays))
 
 
+def test_box_degenerate_and_extreme_directions(box):
+    """The slab test's margins scale with the direction: directions shorter than
+    2^-60 of the scene's scale (and zero or non-finite ones) enter every box,
+    components below 2^-40 of the largest are replaced by +-2^-40 of it, and
+    very long directions make tiny distances. The closest hit and occlusion
+    must still equal brute force's."""
+    _, g, o = box
+    rng = np.random.default_rng(5)
+    base = random_rays(rng, 3000, -0.2, 1.2)
+    groups = []
+    for scale in (1e-25, 1e-12, 1e12, 1e20):
+        r = base.copy()
+        r[:, 4:7] *= np.float32(scale)
+        groups.append(r)
+    r = base[:500].copy()
+    r[:, 4:7] = 0.0  # zero direction: no hit
+    groups.append(r)
+    r = base[:500].copy()
+    r[:, 5] = np.float32(1e-30)  # one component far below the others (and subnormal-range products)
+    r[:, 6] *= np.float32(1e-41)
+    groups.append(r)
+    rays = np.concatenate(groups)
+    hg = g.trace_closest(rays)
+    ho = o.trace_closest(rays, brute=True)
+    bad = np.flatnonzero(hg["prim"] != ho["prim"])
+    assert bad.size == 0, f"{bad.size} rays differ, first {bad[:5]}: gpu {hg[bad[:5]]} oracle {ho[bad[:5]]}"
+    hit = ho["prim"] >= 0
+    assert hit.sum() > 1000
+    assert np.array_equal(hg["t"][hit].view(np.uint32), ho["t"][hit].view(np.uint32))
+    assert np.all(hg["prim"][3000 * 4:3000 * 4 + 500] < 0)
+    assert np.array_equal(g.trace_occluded(rays), o.trace_occluded(rays, brute=True))
+
+
 def test_large_render_matches_oracle(large):
     _, g, o = large
     cfg = ol.settings(total_spp=4, max_depth=8)
