@@ -895,6 +895,11 @@ struct ReplayLog {
 // ray) and its continuation. The wavefront shade kernel queues the rays a
 // vertex emits; the single-kernel tracer traces them at once. With ADJ the
 // vertex also computes the adjoint's per-ray gradients.
+// Emitters a shade block stages in LDS (single-scene worlds with at most this many).
+constexpr int kLdsEmitters = 64;
+constexpr int kEmitterWords = 9;  // float4: DevEmitter (2), tri_pos (3), tri_aux (4)
+static_assert(sizeof(DevEmitter) == 32, "DevEmitter is two float4");
+
 template <bool ADJ>
 struct PathVertex {
     const DevScene S;
@@ -916,6 +921,9 @@ struct PathVertex {
     bool cont = false, do_nee = false;
     bool surface_hit = false;       // the extension ray hit a surface (s, sh are valid)
     int2 vtex = {-1, -1};           // ADJ: texel offsets of the vertex's textured parameters
+    // the scene's emitters staged in LDS (kEmitterWords float4 each: the record,
+    // then its triangle's 3 tri_pos and 4 tri_aux entries), or null
+    const float4* em_tab = nullptr;
     f3 next_dir = {0.0f, 0.0f, 0.0f};
     ReplayLog* log = nullptr;  // ReplayPixel only
 
@@ -1018,10 +1026,18 @@ struct PathVertex {
     __device__ __forceinline__ bool area_next_event(float4* o, float4* d, float4* c, float4* g) {
         const float nsr = (float)R.num_shadow_rays;
         int idx = (int)rng.next_int((uint32_t)R.num_emitters);
-        DevEmitter e = S.emitters[R.emitter_off + idx];
+        DevEmitter e;
         float sel = srcp((float)R.num_emitters);
         f2 b = to_uniform_triangle(rng.next_float2());
-        Surf lp = surface(S, e.tri, b.x, b.y);
+        Surf lp;
+        if (em_tab) {
+            const float4* r = em_tab + kEmitterWords * idx;
+            e = *reinterpret_cast<const DevEmitter*>(r);
+            lp = surface_at(r + 2, r + 5, b.x, b.y);
+        } else {
+            e = S.emitters[R.emitter_off + idx];
+            lp = surface(S, e.tri, b.x, b.y);
+        }
         f3 l2s = normalize(s.pos - lp.pos);
         f3 em = emitted(e, lp.ns, l2s);
         float jac = area_to_solid_angle(s.pos, lp.pos, lp.ng);
@@ -1095,14 +1111,28 @@ __device__ __forceinline__ void load_rec_uniform(const SceneRec* src, SceneRec* 
 // One bounce of EstimateIncidentRadiance for every path of the queue. UNI: the
 // world holds one scene record (not a batch or a view set), whose fields are
 // then read once through the scalar data cache instead of by per-lane loads
-// (shade 26.5 -> 24.8 ms per large-scene render).
+// (shade 26.5 -> 24.8 ms per large-scene render); a scene of at most
+// kLdsEmitters emitters has them staged in LDS for the area-light samples.
 template <bool ADJ, int SB, bool UNI>
 __global__ void __launch_bounds__(SB) __attribute__((amdgpu_waves_per_eu(ADJ ? SSPT_SHADE_ADJ_WAVES : SSPT_SHADE_WAVES)))
 k_shade(DevScene S, Wave w, int buf, int max_nsr) {
     __shared__ uint32_t app[3 * (SB / 64) + 2];  // block_append_word scratch
     SceneRec rec_u;
     if (UNI) load_rec_uniform(S.recs, &rec_u);
+    __shared__ float4 em_lds[UNI ? kLdsEmitters * kEmitterWords : 1];
     const uint32_t count = queue_count(w, buf);
+    // blocks beyond the queue's length exit at once and stage nothing
+    const bool em_staged = UNI && rec_u.num_emitters <= kLdsEmitters && blockIdx.x * blockDim.x < count;
+    if (em_staged) {
+        for (int t = threadIdx.x; t < rec_u.num_emitters * kEmitterWords; t += blockDim.x) {
+            const int e = t / kEmitterWords, k = t - e * kEmitterWords;
+            const DevEmitter* E = S.emitters + rec_u.emitter_off + e;
+            em_lds[t] = k < 2   ? reinterpret_cast<const float4*>(E)[k]
+                        : k < 5 ? S.tri_pos[3 * E->tri + (k - 2)]
+                                : S.tri_aux[4 * E->tri + (k - 5)];
+        }
+        __syncthreads();
+    }
     const int nb = buf ^ 1;
     for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += gridDim.x * blockDim.x) {
         uint32_t i = base + threadIdx.x;
@@ -1111,6 +1141,7 @@ k_shade(DevScene S, Wave w, int buf, int max_nsr) {
         uint32_t slot = qs & kSlotMask, sc = UNI ? 0u : qs >> kSlotBits;
         const SceneRec& R = UNI ? rec_u : S.recs[sc];
         PathVertex<ADJ> v(S, R, slot, sc);
+        if (em_staged) v.em_tab = em_lds;
         if (active) {
             v.dir = f3_of(w.q_d[buf][i]);
             v.rng.state = w.q_rng[buf][i];

@@ -21,15 +21,15 @@ struct Surf {
     int mat;
 };
 
-__device__ __forceinline__ Surf surface(const DevScene& S, int li, float u, float v) {
+// pos: the triangle's 3 tri_pos entries, aux: its 4 tri_aux entries (HBM or LDS)
+__device__ __forceinline__ Surf surface_at(const float4* pos, const float4* aux, float u, float v) {
     Surf s;
-    float4 a = S.tri_pos[3 * li], b = S.tri_pos[3 * li + 1], c = S.tri_pos[3 * li + 2];
+    float4 a = pos[0], b = pos[1], c = pos[2];
     f3 p0 = f3_of(a), p1 = f3_of(b), p2 = f3_of(c);
     float w0 = 1.0f - u - v;
     s.pos = u * p1 + v * p2 + w0 * p0;
     s.ng = normalize(cross(p1 - p0, p2 - p0));
-    float4 x0 = S.tri_aux[4 * li], x1 = S.tri_aux[4 * li + 1], x2 = S.tri_aux[4 * li + 2],
-           x3 = S.tri_aux[4 * li + 3];
+    float4 x0 = aux[0], x1 = aux[1], x2 = aux[2], x3 = aux[3];
     int info = __float_as_int(x3.w);
     s.mat = info & kInfoMaterialMask;
     s.ns = (info & kInfoHasNormals) ? normalize(u * f3_of(x1) + v * f3_of(x2) + w0 * f3_of(x0)) : s.ng;
@@ -42,6 +42,9 @@ __device__ __forceinline__ Surf surface(const DevScene& S, int li, float u, floa
     f3 diag = vabs(u * p1) + vabs(v * p2) + vabs(w0 * p0);
     s.err = length(diag) * 32.0f * 1.19209e-07f;
     return s;
+}
+__device__ __forceinline__ Surf surface(const DevScene& S, int li, float u, float v) {
+    return surface_at(S.tri_pos + 3 * li, S.tri_aux + 4 * li, u, v);
 }
 
 // ImageTexture.ComputeTexel (SeeSharp/Images/ImageTexture.cs:27-44): nearest texel.
