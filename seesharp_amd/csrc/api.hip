@@ -865,13 +865,11 @@ void enqueue_wave(sspt_ctx* ctx, sspt_scene* sc, const Wave& w, const LaunchConf
             } else {
                 timed(ctx, "shade", [&] { launch_shade(st, S, w, lc, buf); });
             }
+            // the last bounce samples no next events (every path still alive is at its
+            // scene's MaxDepth, PathTracer.cs:367-368), so its shadow queue stays empty
             if (depth < max_depth) {
                 timed(ctx, adj ? "trace_adjoint" : "trace", [&] { launch_trace(st, S, w, lc, buf ^ 1, adj); });
                 launch_bounce_reset(st, w, buf ^ 1);
-            } else if (adj) {
-                timed(ctx, "shadow_adjoint", [&] { launch_shadow_adjoint(st, S, w, lc, buf ^ 1); });
-            } else {
-                timed(ctx, "shadow", [&] { launch_shadow(st, S, w, lc, buf ^ 1); });
             }
             buf ^= 1;
         }
