@@ -828,6 +828,23 @@ LaunchConfig launch_config(const sspt_ctx* ctx, uint32_t max_paths, uint32_t max
 // the descriptor the wave counter selects.
 enum class Mode { Mean, Sum, Adjoint };
 
+// Whether every scene record agrees on the fields the shade kernel reads
+// (PathVertex, env_of, the denoiser flag), so that it may read record 0 for all.
+bool records_alike(const std::vector<SceneRec>& recs) {
+    for (size_t i = 1; i < recs.size(); ++i) {
+        const SceneRec &a = recs[0], &b = recs[i];
+        if (a.num_shadow_rays != b.num_shadow_rays || a.enable_bsdf_di != b.enable_bsdf_di ||
+            a.min_depth != b.min_depth || a.max_depth != b.max_depth || a.enable_denoiser != b.enable_denoiser ||
+            a.num_emitters != b.num_emitters || a.emitter_off != b.emitter_off || a.has_env != b.has_env ||
+            a.env_w != b.env_w || a.env_h != b.env_h || a.env_rgb_off != b.env_rgb_off ||
+            a.env_row_off != b.env_row_off || a.env_col_off != b.env_col_off ||
+            a.env_guide_row_off != b.env_guide_row_off || a.env_guide_col_off != b.env_guide_col_off ||
+            a.env_guide_row_g != b.env_guide_row_g || a.env_guide_col_g != b.env_guide_col_g)
+            return false;
+    }
+    return true;
+}
+
 void enqueue_wave(sspt_ctx* ctx, sspt_scene* sc, const Wave& w, const LaunchConfig& lc, int max_depth, Mode mode,
                   float* mean_image, uint32_t mean_pixels, float* mean_albedo) {
     float* mean_normal = mean_albedo ? mean_albedo + sc->dev.layer_stride : nullptr;
@@ -909,6 +926,7 @@ sspt_status run_waves(sspt_ctx* ctx, sspt_scene* sc, const std::vector<WaveDesc>
     uint32_t max_paths = 0;
     for (const WaveDesc& d : waves) max_paths = std::max(max_paths, d.count);
     const LaunchConfig lc = launch_config(ctx, max_paths, (uint32_t)std::max(1, max_nsr));
+    sc->dev.recs_alike = records_alike(sc->recs) ? 1 : 0;
     HIP_TRY(ctx, hipMemcpyAsync(ctx->wave_descs, waves.data(), waves.size() * sizeof(WaveDesc),
                                 hipMemcpyHostToDevice, ctx->stream));
     // counters: queues and wave index 0 (issued before any capture: the
@@ -1414,6 +1432,7 @@ sspt_status sspt_render_noalloc(sspt_ctx* ctx, sspt_scene* sc, const sspt_pt_set
         apply_settings(R, *cfg);
         R.first_iter = first_iteration;
         R.path_off = 0;
+        sc->dev.recs_alike = 1;  // every path of this render is record 0's
         const uint64_t offs[2] = {0, total};
         HIP_TRY(ctx, hipMemcpyAsync(sc->d_recs, &R, sizeof(SceneRec), hipMemcpyHostToDevice, ctx->stream));
         HIP_TRY(ctx, hipMemcpyAsync(sc->d_path_offsets, offs, sizeof(offs), hipMemcpyHostToDevice, ctx->stream));

@@ -1108,11 +1108,13 @@ __device__ __forceinline__ void load_rec_uniform(const SceneRec* src, SceneRec* 
     for (int k = 0; k < (int)(sizeof(SceneRec) / 4); ++k) d[k] = q[k];
 }
 
-// One bounce of EstimateIncidentRadiance for every path of the queue. UNI: the
-// world holds one scene record (not a batch or a view set), whose fields are
-// then read once through the scalar data cache instead of by per-lane loads
-// (shade 26.5 -> 24.8 ms per large-scene render); a scene of at most
+// One bounce of EstimateIncidentRadiance for every path of the queue. UNI: every
+// record agrees on what this kernel reads (a world of one scene, or a view set
+// rendered with one setting; DevScene::recs_alike), so record 0's fields are
+// read once through the scalar data cache instead of by per-lane loads (shade
+// 26.5 -> 24.8 ms per large-scene render), and a scene of at most
 // kLdsEmitters emitters has them staged in LDS for the area-light samples.
+// Shadow rays keep their path's own record index in their tags.
 template <bool ADJ, int SB, bool UNI>
 __global__ void __launch_bounds__(SB) __attribute__((amdgpu_waves_per_eu(ADJ ? SSPT_SHADE_ADJ_WAVES : SSPT_SHADE_WAVES)))
 k_shade(DevScene S, Wave w, int buf, int max_nsr) {
@@ -1138,7 +1140,7 @@ k_shade(DevScene S, Wave w, int buf, int max_nsr) {
         uint32_t i = base + threadIdx.x;
         bool active = i < count;
         uint32_t qs = active ? w.q_slot[buf][i] : 0u;
-        uint32_t slot = qs & kSlotMask, sc = UNI ? 0u : qs >> kSlotBits;
+        uint32_t slot = qs & kSlotMask, sc = qs >> kSlotBits;
         const SceneRec& R = UNI ? rec_u : S.recs[sc];
         PathVertex<ADJ> v(S, R, slot, sc);
         if (em_staged) v.em_tab = em_lds;
@@ -3080,7 +3082,7 @@ void launch_extend(hipStream_t s, const DevScene& sc, const Wave& w, const Launc
 void launch_shade(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int buf) {
     with_shade_block(lc.shade_block, [&](auto sbk) {
         constexpr int SB = decltype(sbk)::value;
-        if (sc.num_scenes == 1)
+        if (sc.num_scenes == 1 || sc.recs_alike)
             hipLaunchKernelGGL((k_shade<false, SB, true>), dim3(grid_of(lc.shade_blocks, lc.max_paths, SB)), dim3(SB),
                                0, s, sc, w, buf, lc.max_nsr);
         else
@@ -3097,7 +3099,7 @@ void launch_shadow(hipStream_t s, const DevScene& sc, const Wave& w, const Launc
 }
 void launch_shade_adjoint(hipStream_t s, const DevScene& sc, const Wave& w, const LaunchConfig& lc, int buf) {
     // the adjoint shade keeps its one block size (its register budget is tuned for it)
-    if (sc.num_scenes == 1)
+    if (sc.num_scenes == 1 || sc.recs_alike)
         hipLaunchKernelGGL((k_shade<true, kShadeBlock, true>), dim3(grid_of(lc.shade_blocks, lc.max_paths, kShadeBlock)),
                            dim3(kShadeBlock), 0, s, sc, w, buf, lc.max_nsr);
     else

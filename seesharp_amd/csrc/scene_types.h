@@ -179,6 +179,10 @@ struct DevScene {
     int num_outliers;
     int num_scenes;
     int num_nodes;
+    // set by the host before a render: every record agrees on what the shade
+    // kernel reads (settings, emitters, environment), as in a world of one scene
+    // or a view set rendered with one setting
+    int recs_alike;
 };
 
 // Device-side per-wave work: the wave covers world-wide paths
