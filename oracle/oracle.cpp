@@ -591,6 +591,8 @@ struct or_scene {
     std::vector<int> tri_emitter;  // global tri -> emitter index or -1
     bool has_env = false;
     EnvMap env;
+    V3 scene_center{};         // Background.SceneCenter (Scene.cs:114-141)
+    float scene_radius = 0.0f;  // Background.SceneRadius
     Camera camera;
     int width = 0, height = 0;
     float abs_max = 0.0f;  // largest vertex coordinate magnitude
@@ -1005,6 +1007,29 @@ EnvSample env_sample(const EnvMap& e, V2 primary) {  // :40-73
     pdf /= jac;
     Rgb w = e.pixel(to_int(pp.x * (float)e.w), to_int(pp.y * (float)e.h)) / pdf;
     return {dir, pdf, w};
+}
+
+// SampleWarp.ToConcentricDisc (SampleWarp.cs:219-247): Shirley and Chiu's
+// concentric map of the unit square onto the unit disc.
+inline V2 to_concentric_disc(V2 p) {
+    float phi, r;
+    const float a = 2.0f * p.x - 1.0f, b = 2.0f * p.y - 1.0f;
+    if (a > -b) {
+        if (a > b) {
+            r = a;
+            phi = (kPi * 0.25f) * (b / a);
+        } else {
+            r = b;
+            phi = (kPi * 0.25f) * (2.0f - (a / b));
+        }
+    } else if (a < b) {
+        r = -a;
+        phi = (kPi * 0.25f) * (4.0f + (b / a));
+    } else {
+        r = -b;
+        phi = b != 0.0f ? (kPi * 0.25f) * (6.0f - (a / b)) : 0.0f;
+    }
+    return {r * std::cos(phi), r * std::sin(phi)};
 }
 
 // ------------------------------------------------------------- materials ---
@@ -1917,6 +1942,27 @@ or_scene* or_scene_create(const sspt_scene_desc* d, char* err) {
         for (int r = 0; r < e.h; ++r)
             for (int c = 0; c < e.w; ++c) e.grid.splat(c, r, luminance(e.pixel(c, r)));
         e.grid.normalize();
+        // Scene.Prepare (Scene.cs:114-141): the centre is the mean of every mesh vertex
+        // (summed in mesh and vertex order), the radius the distance to the furthest one
+        V3 sum = v3(0.0f, 0.0f, 0.0f);
+        uint64_t total = 0;
+        for (uint32_t mi = 0; mi < d->num_meshes; ++mi) {
+            const sspt_mesh_desc& m = d->meshes[mi];
+            for (uint32_t v = 0; v < m.num_vertices; ++v)
+                sum = sum + v3(m.positions[v * 3], m.positions[v * 3 + 1], m.positions[v * 3 + 2]);
+            total += m.num_vertices;
+        }
+        const float count = (float)total;
+        s->scene_center = v3(sum.x / count, sum.y / count, sum.z / count);
+        float r2 = 0.0f;
+        for (uint32_t mi = 0; mi < d->num_meshes; ++mi) {
+            const sspt_mesh_desc& m = d->meshes[mi];
+            for (uint32_t v = 0; v < m.num_vertices; ++v) {
+                V3 o = v3(m.positions[v * 3], m.positions[v * 3 + 1], m.positions[v * 3 + 2]) - s->scene_center;
+                r2 = std::max(dot(o, o), r2);
+            }
+        }
+        s->scene_radius = std::sqrt(r2);
     }
     if (!s->camera.init(d->camera.world_to_camera, d->camera.vertical_fov_degrees, s->width, s->height)) {
         set_err(err, "World to camera transform must be invertible.");
@@ -2251,6 +2297,7 @@ struct CachedVertex {
     float pdf_ne = 0.0f;    // PdfNextEventAncestor
     Rgb weight = rgb(0.0f);
     int depth = 0;
+    bool from_background = false;  // FromBackground; vertex 0 of such a path is the ray origin, on no mesh
 };
 
 // CameraPath.Vertices (BidirBase.Camera.cs:7-18)
@@ -2322,49 +2369,143 @@ float light_reciprocals(const BidirCtx& bc, const Pdfs& p, int last) {
     return sum;
 }
 
-// ClassicBidir.NextEventPdf (ClassicBidir.cs:42-44) over BidirBase.NextEventPdf
-// (BidirBase.Camera.cs:338-347) of a point on an emitter, without a background.
-float next_event_pdf(const BidirCtx& bc, const SurfacePoint& to) {
+// BidirBase.BackgroundProbability (BidirBase.Light.cs:132-133), which is also
+// ComputeNextEventBackgroundProbability (BidirBase.Camera.cs:354-355)
+float background_probability(const or_scene& s) {
+    return s.has_env ? 1.0f / (1.0f + (float)s.emitters.size()) : 0.0f;
+}
+
+// ClassicBidir.NextEventPdf (ClassicBidir.cs:42-44; VertexCacheBidir.cs:51-53) over
+// BidirBase.NextEventPdf (BidirBase.Camera.cs:338-347) of the vertex `to` seen from
+// `from`: the background's direction pdf towards the (unnormalised) offset, or the
+// emitter's area pdf.
+float next_event_pdf(const BidirCtx& bc, V3 from, const SurfacePoint& to) {
     const or_scene& s = *bc.s;
+    const float bg = background_probability(s);
+    if (to.tri < 0) return env_pdf(s.env, to.position - from) * bg * (float)bc.cfg->num_shadow_rays;
     const Emitter& e = s.emitters[(size_t)s.tri_emitter[(size_t)to.tri]];
     float pmf = 1.0f / (float)s.emitters.size();
-    return e.inv_area * pmf * (1.0f - 0.0f) * (float)bc.cfg->num_shadow_rays;
+    return e.inv_area * pmf * (1.0f - bg) * (float)bc.cfg->num_shadow_rays;
 }
 // BidirBase.ComputeEmitterPdf (BidirBase.Light.cs:160-166), SelectLightPmf (:116-122)
 float emitter_pdf(const BidirCtx& bc, const Emitter& e, const SurfacePoint& p, V3 light_to_surface,
                   float jacobian) {
     float pdf = emitter_pdf_ray(e, p, light_to_surface);
     pdf *= jacobian;
-    pdf *= (1.0f - 0.0f) / (float)bc.s->emitters.size();
+    pdf *= (1.0f - background_probability(*bc.s)) / (float)bc.s->emitters.size();
     return pdf;
 }
 
-// TraceEmitterPath (BidirBase.Light.cs:268-282) with RandomWalk.StartFromEmitter /
-// ContinueWalk (RandomWalk.cs:82-90,150-217) and LightPathWalk (BidirBase.Light.cs:324-410):
-// the vertices of light path `idx` (none if the emitter sample is invalid).
+// EnvironmentMap.SampleRay (EnvironmentMap.cs:87-103): a direction of the map and a
+// point on the disc of radius SceneRadius about SceneCenter + SceneRadius * direction,
+// orthogonal to it; the ray runs back along the direction.
+struct EnvRay {
+    Ray ray;
+    Rgb weight;
+    float pdf;
+};
+EnvRay env_sample_ray(const or_scene& s, V2 primary_pos, V2 primary_dir) {
+    EnvSample ds = env_sample(s.env, primary_dir);
+    V2 disc = to_concentric_disc(primary_pos);
+    V3 t, b;
+    basis(ds.dir, &t, &b);
+    V3 pos = s.scene_center + s.scene_radius * (ds.dir + t * disc.x + b * disc.y);
+    float pos_pdf = (1.0f / kPi) / (s.scene_radius * s.scene_radius);  // ToConcentricDiscJacobian / R^2
+    EnvRay r;
+    r.ray.o = pos;
+    r.ray.d = -ds.dir;
+    r.ray.tmin = 0.0f;
+    r.ray.tmax = INFINITY;
+    r.weight = ds.weight / pos_pdf;
+    r.pdf = pos_pdf * ds.pdf;
+    return r;
+}
+// BidirBase.ComputeBackgroundPdf (BidirBase.Light.cs:174-178) over EnvironmentMap.RayPdf
+// (EnvironmentMap.cs:127-130): the pdf of the background ray along `light_to_surface`
+float background_pdf(const or_scene& s, V3 light_to_surface) {
+    float dir_pdf = env_pdf(s.env, -light_to_surface);
+    float pos_pdf = (1.0f / kPi) / (s.scene_radius * s.scene_radius);
+    float pdf = pos_pdf * dir_pdf;
+    return pdf * background_probability(s);
+}
+
+// TraceLightPath (BidirBase.Light.cs:212-219): SelectLight (:103-111) picks the
+// background with BackgroundProbability, else an emitter; TraceEmitterPath /
+// TraceBackgroundPath (:268-299) with RandomWalk.StartFromEmitter /
+// StartFromBackground / ContinueWalk (RandomWalk.cs:82-128,150-217) and LightPathWalk
+// (BidirBase.Light.cs:324-410): the vertices of light path `idx` (none if the
+// emitter or background sample is invalid).
 void bidir_light_path(const BidirCtx& bc, uint32_t idx, uint32_t iteration, std::vector<CachedVertex>* out) {
     const or_scene& s = *bc.s;
     const sspt_bidir_settings& cfg = *bc.cfg;
     out->clear();
     Rng rng{hash_seed(cfg.base_seed_light, idx, iteration)};
-    // SelectLight (:103-110): no background, so nothing is drawn for it
-    int count = (int)s.emitters.size();
-    const Emitter& e = s.emitters[rng.next_int((uint32_t)count)];
-    float sel = (1.0f - 0.0f) / (float)count;
-    V2 ppos = rng.next_float2();
-    V2 pdir = rng.next_float2();
-    EmitterRay er = emitter_sample_ray(s, e, ppos, pdir);
-    er.pdf *= sel;
-    if (er.pdf == 0.0f || is_black(er.weight)) return;
-    Rgb prefix = er.weight / sel;
-    CachedVertex v0;
-    v0.p = er.p;
-    out->push_back(v0);
-    Rgb approx = rgb(1.0f);
-    Ray ray = spawn_ray(er.p, er.dir);
-    SurfacePoint prev = er.p;
-    float pdf_dir = er.pdf, next_rev = 0.0f;
-    for (int depth = 1; depth < cfg.max_depth;) {
+    const float bg = background_probability(s);
+    const bool from_bg = bg > 0.0f && rng.next_float() <= bg;
+    Rgb prefix, approx = rgb(1.0f);
+    Ray ray;
+    SurfacePoint prev;
+    float pdf_dir, next_rev = 0.0f;
+    int depth = 1;
+    if (from_bg) {
+        // SampleBackground: primaryPos, then primaryDir
+        V2 ppos = rng.next_float2();
+        V2 pdir = rng.next_float2();
+        EnvRay er = env_sample_ray(s, ppos, pdir);
+        er.pdf *= bg;
+        er.weight = er.weight / bg;
+        if (er.pdf == 0.0f || is_black(er.weight)) return;
+        CachedVertex v0;
+        v0.p.position = er.ray.o;
+        v0.from_background = true;
+        out->push_back(v0);
+        // StartFromBackground: the first hit samples its direction before it is
+        // stored; its pdfs stay per solid angle and its weight is the initial one
+        HitRec h = trace_closest(s, er.ray, false);
+        bc.c->rays++;
+        if (h.tri < 0) return;
+        bc.c->hits++;
+        SurfacePoint hit = make_point(s, h.tri, {h.u, h.v}, h.t);
+        Shader sh = make_shader(s, hit, -er.ray.d);
+        sh.light = true;
+        float pc = rng.next_float();
+        V2 pd = rng.next_float2();
+        BsdfSample bs = sample_bsdf(sh, pc, pd);
+        approx = approx * bs.weight;
+        CachedVertex v;
+        v.p = hit;
+        v.pdf_from = er.pdf;
+        v.pdf_rev = 0.0f;
+        v.weight = er.weight;
+        v.depth = 1;
+        v.from_background = true;
+        out->push_back(v);
+        next_rev = bs.pdf_rev;  // OnContinue
+        if (cfg.max_depth <= 1) return;
+        if (bs.pdf == 0.0f || is_black(bs.weight)) return;
+        prefix = er.weight * bs.weight;
+        depth = 2;
+        pdf_dir = bs.pdf;
+        prev = hit;
+        ray = spawn_ray(hit, bs.dir);
+    } else {
+        int count = (int)s.emitters.size();
+        const Emitter& e = s.emitters[rng.next_int((uint32_t)count)];
+        float sel = (1.0f - bg) / (float)count;
+        V2 ppos = rng.next_float2();
+        V2 pdir = rng.next_float2();
+        EmitterRay er = emitter_sample_ray(s, e, ppos, pdir);
+        er.pdf *= sel;
+        if (er.pdf == 0.0f || is_black(er.weight)) return;
+        prefix = er.weight / sel;
+        CachedVertex v0;
+        v0.p = er.p;
+        out->push_back(v0);
+        ray = spawn_ray(er.p, er.dir);
+        prev = er.p;
+        pdf_dir = er.pdf;
+    }
+    for (; depth < cfg.max_depth;) {
         HitRec h = trace_closest(s, ray, false);
         bc.c->rays++;
         if (h.tri < 0) break;
@@ -2378,9 +2519,10 @@ void bidir_light_path(const BidirCtx& bc, uint32_t idx, uint32_t iteration, std:
         v.p = hit;
         v.pdf_from = pdf_from;
         v.pdf_rev = next_rev;
-        v.pdf_ne = depth == 2 ? next_event_pdf(bc, (*out)[0].p) : 0.0f;
+        v.pdf_ne = depth == 2 ? next_event_pdf(bc, (*out)[1].p.position, (*out)[0].p) : 0.0f;
         v.weight = prefix;
         v.depth = depth;
+        v.from_background = from_bg;
         out->push_back(v);
         if (depth + 1 >= cfg.max_depth) break;
         float q = depth > 4 ? std::min(std::max(average(approx), 0.05f), 0.95f) : 1.0f;
@@ -2427,13 +2569,13 @@ void bidir_splat(BidirCtx& bc, const std::vector<CachedVertex>& path, size_t k, 
     f = f * (std::fabs(dot(v.p.shading_normal, to_anc)) / std::fabs(dot(v.p.normal, to_anc)));
     float pdf_rev, unused;
     pdf_pair(sh, to_anc, &pdf_rev, &unused);
-    pdf_rev *= area_to_solid_angle(v.p, anc.p);
+    if (anc.p.tri >= 0) pdf_rev *= area_to_solid_angle(v.p, anc.p);  // not from the background
     Pdfs& p = bc.pdfs;
     p.reset(v.depth + 1);
     p.gather_light(path, v, -1);
     p.c2l[0] = 1.0f * jac;  // response.PdfEmit = invLensArea * jacobian
     p.c2l[1] = pdf_rev;
-    if (v.depth == 1) p.ne = next_event_pdf(bc, anc.p);
+    if (v.depth == 1) p.ne = next_event_pdf(bc, v.p.position, anc.p);
     float mis = 1.0f / (1.0f + light_reciprocals(bc, p, -1) / bc.num_paths);
     Rgb w = v.weight * f * rgb(jac) / bc.num_paths;
     out->push_back({(int64_t)py * s.width + px, mis * w});
@@ -2444,7 +2586,7 @@ Rgb bidir_emitter_hit(BidirCtx& bc, const Emitter& e, const SurfacePoint& hit, V
                       const std::vector<PdfPair>& v, float rev_jac) {
     Rgb emission = emitted(e, hit, out_dir);
     float pdf_emit = emitter_pdf(bc, e, hit, out_dir, rev_jac);
-    float pdf_ne = next_event_pdf(bc, hit);
+    float pdf_ne = next_event_pdf(bc, V3{}, hit);  // NextEventPdf(new SurfacePoint(), hit)
     const int n = (int)v.size();
     Pdfs& p = bc.pdfs;
     p.reset(n);
@@ -2463,6 +2605,59 @@ Rgb bidir_emitter_hit(BidirCtx& bc, const Emitter& e, const SurfacePoint& hit, V
     return mis * emission;
 }
 
+// BidirBase.OnBackgroundHit (BidirBase.Camera.cs:532-561) with ClassicBidir.EmitterHitMis:
+// `v` ends with the escaped ray's entry (OnInvalidHit, :604-612: PdfFromAncestor is
+// the direction's solid-angle pdf). No MinDepth test, as in the reference.
+Rgb bidir_background_hit(BidirCtx& bc, V3 dir, const std::vector<PdfPair>& v) {
+    const or_scene& s = *bc.s;
+    float pdf_emit = background_pdf(s, -dir);
+    float pdf_ne = env_pdf(s.env, dir) * background_probability(s);
+    const int n = (int)v.size();
+    Pdfs& p = bc.pdfs;
+    p.reset(n);
+    p.gather_camera(v, n - 1);
+    if (n > 1) p.l2c[(size_t)n - 2] = pdf_emit;
+    p.ne = pdf_ne;
+    p.c2l[(size_t)n - 1] = v.back().from;
+    float mis = 1.0f;
+    if (n > 1) {
+        float pdf_this = v.back().from;
+        float sum = 1.0f;
+        sum += p.ne / pdf_this;
+        sum += camera_reciprocals(bc, p, n - 2) / pdf_this;
+        mis = 1.0f / sum;
+    }
+    return mis * env_radiance(s.env, dir);
+}
+
+// PerformNextEventEstimation's background branch (BidirBase.Camera.cs:381-418): a
+// direction of the map (its pdf without the NumShadowRays factor of the emitter
+// branch), an any-hit ray to infinity (Raytracer.LeavesScene), the solid-angle pdfs
+// of both directions. Fills the technique's pdfs in `p`; returns the unweighted
+// sample (black if it contributes nothing).
+Rgb background_next_event(const or_scene& s, Counters* c, const Shader& sh, Rng& rng, float bg, float rev_jac,
+                          Pdfs& p, int n) {
+    EnvSample smp = env_sample(s.env, rng.next_float2());
+    smp.pdf *= bg;
+    smp.weight = smp.weight / bg;
+    if (smp.pdf == 0.0f) return rgb(0.0f);
+    c->shadow++;
+    if (trace_any(s, spawn_ray(sh.p, smp.dir), false)) {
+        c->occluded++;
+        return rgb(0.0f);
+    }
+    Rgb bsdf_cos = eval_cos(sh, smp.dir);
+    float fwd, rev;
+    pdf_pair(sh, smp.dir, &fwd, &rev);
+    rev *= rev_jac;
+    if (fwd == 0.0f || rev == 0.0f || is_black(bsdf_cos)) return rgb(0.0f);
+    p.l2c[(size_t)n - 2] = background_pdf(s, -smp.dir);
+    if (n > 2) p.l2c[(size_t)n - 3] = rev;
+    p.ne = smp.pdf;
+    p.c2l[(size_t)n - 1] = fwd;
+    return smp.weight * bsdf_cos;
+}
+
 // BidirBase.PerformNextEventEstimation (BidirBase.Camera.cs:368-484), emitter
 // branch, with ClassicBidir.SampleNextEvent / NextEventMis (ClassicBidir.cs:47-51,147-153).
 Rgb bidir_next_event(BidirCtx& bc, const Shader& sh, Rng& rng, const std::vector<PdfPair>& v, float rev_jac) {
@@ -2472,8 +2667,15 @@ Rgb bidir_next_event(BidirCtx& bc, const Shader& sh, Rng& rng, const std::vector
     p.reset(n);
     p.gather_camera(v, n - 2);
     p.c2l[(size_t)n - 2] = v.back().from;
-    const float bg = 0.0f;  // ComputeNextEventBackgroundProbability without a background
-    if (rng.next_float() < bg) return rgb(0.0f);
+    const float bg = background_probability(s);  // ComputeNextEventBackgroundProbability
+    if (rng.next_float() < bg) {
+        Rgb w = background_next_event(s, bc.c, sh, rng, bg, rev_jac, p, n);
+        if (is_black(w)) return rgb(0.0f);
+        float sum = 1.0f;  // ClassicBidir.NextEventMis (ClassicBidir.cs:147-153)
+        sum += p.c2l[(size_t)n - 1] / p.ne;
+        sum += camera_reciprocals(bc, p, (int)v.size() - 1) / p.ne;
+        return (1.0f / sum) * w;
+    }
     if (s.emitters.empty()) return rgb(0.0f);
     // SampleNextEvent: SelectLight (BidirBase.Camera.cs:70-73), SampleUniformArea
     int count = (int)s.emitters.size();
@@ -2543,14 +2745,14 @@ Rgb bidir_connect(BidirCtx& bc, const Shader& sh, const std::vector<CachedVertex
     if (pdf_cam_to_light == 0.0f) return rgb(0.0f);
     float pdf_light_to_cam, pdf_light_rev;
     pdf_pair(ls, -dir, &pdf_light_to_cam, &pdf_light_rev);
-    pdf_light_rev *= area_to_solid_angle(lv.p, anc.p);  // the ancestor is on a surface
+    if (anc.p.tri >= 0) pdf_light_rev *= area_to_solid_angle(lv.p, anc.p);  // not from the background
     pdf_light_to_cam *= area_to_solid_angle(lv.p, sh.p);
     const int last = (int)v.size() - 1;
     Pdfs& p = bc.pdfs;
     p.reset((int)v.size() + lv.depth + 1);
     p.gather_camera(v, last);
     p.gather_light(path, lv, last);
-    if (lv.depth == 1) p.ne = next_event_pdf(bc, anc.p);
+    if (lv.depth == 1) p.ne = next_event_pdf(bc, lv.p.position, anc.p);
     if (last > 0) p.l2c[(size_t)last - 1] = pdf_cam_rev;
     p.c2l[(size_t)last] = v.back().from;
     p.l2c[(size_t)last] = pdf_light_to_cam;
@@ -2593,7 +2795,13 @@ Rgb bidir_camera_path(BidirCtx& bc, uint32_t row, uint32_t col, uint32_t iterati
     for (int depth = 1; depth < cfg.max_depth + 1;) {
         HitRec h = trace_closest(s, ray, false);
         bc.c->rays++;
-        if (h.tri < 0) break;  // OnBackgroundHit without a background adds nothing
+        if (h.tri < 0) {
+            if (s.has_env) {  // CameraRandomWalk.OnInvalidHit (BidirBase.Camera.cs:604-612)
+                v.push_back({pdf_dir, 0.0f});
+                L = L + prefix * bidir_background_hit(bc, ray.d, v);
+            }
+            break;
+        }
         bc.c->hits++;
         SurfacePoint hit = make_point(s, h.tri, {h.u, h.v}, h.t);
         float pdf_from = pdf_dir * area_to_solid_angle(prev, hit);
@@ -2637,7 +2845,6 @@ Rgb bidir_camera_path(BidirCtx& bc, uint32_t row, uint32_t col, uint32_t iterati
 uint64_t or_render_bidir(const or_scene* s, const sspt_bidir_settings* cfg, uint32_t first, uint32_t num,
                          float* image, sspt_ray_stats* stats, int32_t threads) {
     const int64_t W = s->width, H = s->height;
-    if (s->has_env || s->emitters.empty()) return UINT64_MAX;
     if (cfg->num_light_paths >= 0 && (int64_t)cfg->num_light_paths != W * H) return UINT64_MAX;
     const uint32_t paths = (uint32_t)(W * H);
     std::atomic<uint64_t> rays{0}, shadow{0}, occl{0}, hits{0};
@@ -2827,17 +3034,17 @@ void vcm_splat(VcmCtx& vc, const std::vector<CachedVertex>& path, size_t k, std:
     f = f * (std::fabs(dot(v.p.shading_normal, to_anc)) / std::fabs(dot(v.p.normal, to_anc)));
     float pdf_rev, unused;
     pdf_pair(sh, to_anc, &pdf_rev, &unused);
-    pdf_rev *= area_to_solid_angle(v.p, anc.p);
+    if (anc.p.tri >= 0) pdf_rev *= area_to_solid_angle(v.p, anc.p);  // not from the background
     Pdfs& p = vc.pdfs;
     p.reset(v.depth + 1);
     p.gather_light(path, v, -1);
     p.c2l[0] = 1.0f * jac;
     p.c2l[1] = pdf_rev;
-    if (v.depth == 1) p.ne = next_event_pdf(vc.light(), anc.p);
+    if (v.depth == 1) p.ne = next_event_pdf(vc.light(), v.p.position, anc.p);
     float mis;
     if (vc.vcm) {
         CorrelRatios cr;
-        cr.init(p, dist, false);
+        cr.init(p, dist, v.from_background);
         float footprint = std::sqrt(1.0f / p.c2l[0]);
         float sum = vcm_light_reciprocals(vc, p, -1, footprint, cr);
         sum /= vc.num_paths;
@@ -2865,7 +3072,7 @@ Rgb vcm_emitter_hit(VcmCtx& vc, const Emitter& e, const SurfacePoint& hit, V3 ou
     Rgb emission = emitted(e, hit, out_dir);
     BidirCtx bc = vc.light();
     float pdf_emit = emitter_pdf(bc, e, hit, out_dir, rev_jac);
-    float pdf_ne = next_event_pdf(bc, hit);
+    float pdf_ne = next_event_pdf(bc, V3{}, hit);  // NextEventPdf(new SurfacePoint(), hit)
     const int n = (int)cp.v.size();
     Pdfs& p = vc.pdfs;
     p.reset(n);
@@ -2886,6 +3093,31 @@ Rgb vcm_emitter_hit(VcmCtx& vc, const Emitter& e, const SurfacePoint& hit, V3 ou
     return mis * emission;
 }
 
+// OnBackgroundHit (BidirBase.Camera.cs:532-561) with EmitterHitMis (isBackground)
+Rgb vcm_background_hit(VcmCtx& vc, V3 dir, const VcmPath& cp) {
+    const or_scene& s = *vc.s;
+    float pdf_emit = background_pdf(s, -dir);
+    float pdf_ne = env_pdf(s.env, dir) * background_probability(s);
+    const int n = (int)cp.v.size();
+    Pdfs& p = vc.pdfs;
+    p.reset(n);
+    p.gather_camera(cp.v, n - 1);
+    if (n > 1) p.l2c[(size_t)n - 2] = pdf_emit;
+    p.ne = pdf_ne;
+    p.c2l[(size_t)n - 1] = cp.v.back().from;
+    float mis = 1.0f;
+    if (n > 1) {
+        CorrelRatios cr;
+        if (vc.vcm) cr.init(p, cp.dist0, true);
+        float pdf_this = p.c2l[(size_t)n - 1];
+        float sum = 1.0f;
+        sum += p.ne / pdf_this;
+        sum += vcm_camera_reciprocals(vc, p, n - 2, cp.footprint, cr) / pdf_this;
+        mis = 1.0f / sum;
+    }
+    return mis * env_radiance(s.env, dir);
+}
+
 // PerformNextEventEstimation (BidirBase.Camera.cs:368-484) with NextEventMis
 // (VertexCacheBidir.cs:200-207, VertexConnectionAndMerging.cs:802-836)
 Rgb vcm_next_event(VcmCtx& vc, const Shader& sh, Rng& rng, const VcmPath& cp, float rev_jac) {
@@ -2895,8 +3127,18 @@ Rgb vcm_next_event(VcmCtx& vc, const Shader& sh, Rng& rng, const VcmPath& cp, fl
     p.reset(n);
     p.gather_camera(cp.v, n - 2);
     p.c2l[(size_t)n - 2] = cp.v.back().from;
-    const float bg = 0.0f;
-    if (rng.next_float() < bg) return rgb(0.0f);
+    const float bg = background_probability(s);
+    if (rng.next_float() < bg) {
+        Rgb w = background_next_event(s, vc.c, sh, rng, bg, rev_jac, p, n);
+        if (is_black(w)) return rgb(0.0f);
+        CorrelRatios cr;
+        if (vc.vcm) cr.init(p, cp.dist0, true);
+        float sum = 1.0f;
+        if (vc.cfg->enable_hitting) sum += p.c2l[(size_t)n - 1] / p.ne;
+        sum += vcm_camera_reciprocals(vc, p, (int)cp.v.size() - 1, cp.footprint, cr) / p.ne;
+        return (1.0f / sum) * w;
+    }
+    if (s.emitters.empty()) return rgb(0.0f);
     int count = (int)s.emitters.size();
     const Emitter& e = s.emitters[rng.next_int((uint32_t)count)];
     float prob = 1.0f / (float)count;
@@ -2980,21 +3222,21 @@ Rgb vcm_connect(VcmCtx& vc, const Shader& sh, Rng& rng, const VcmPath& cp, float
     if (pcl == 0.0f) return rgb(0.0f);
     float plc, plr;
     pdf_pair(ls, -dir, &plc, &plr);
-    plr *= area_to_solid_angle(lv.p, anc.p);
+    if (anc.p.tri >= 0) plr *= area_to_solid_angle(lv.p, anc.p);  // not from the background
     plc *= area_to_solid_angle(lv.p, sh.p);
     const int last = (int)cp.v.size() - 1;
     Pdfs& p = vc.pdfs;
     p.reset((int)cp.v.size() + lv.depth + 1);
     p.gather_camera(cp.v, last);
     p.gather_light(path, lv, last);
-    if (lv.depth == 1) p.ne = next_event_pdf(vc.light(), anc.p);
+    if (lv.depth == 1) p.ne = next_event_pdf(vc.light(), lv.p.position, anc.p);
     if (last > 0) p.l2c[(size_t)last - 1] = pcr;
     p.c2l[(size_t)last] = cp.v.back().from;
     p.l2c[(size_t)last] = plc;
     p.c2l[(size_t)last + 1] = pcl;
     p.c2l[(size_t)last + 2] = plr;
     CorrelRatios cr;
-    if (vc.vcm) cr.init(p, cp.dist0, false);
+    if (vc.vcm) cr.init(p, cp.dist0, lv.from_background);
     float sum = 1.0f;
     sum += vcm_camera_reciprocals(vc, p, last, cp.footprint, cr) / density;
     sum += vcm_light_reciprocals(vc, p, last, cp.footprint, cr) / density;
@@ -3025,7 +3267,8 @@ Rgb vcm_merge(VcmCtx& vc, const Shader& sh, const VcmPath& cp, float rev_jac, in
     float pdf_light_rev, pdf_cam_rev;
     pdf_pair(sh, to_anc, &pdf_light_rev, &pdf_cam_rev);
     pdf_cam_rev *= rev_jac;
-    pdf_light_rev *= area_to_solid_angle(sh.p, anc.p);  // the ancestor is on a surface
+    // at the first hit from the background the pdf stays per solid angle
+    if (!(ph.depth == 1 && ph.from_background)) pdf_light_rev *= area_to_solid_angle(sh.p, anc.p);
     const int last = (int)cp.v.size() - 1;
     Pdfs& p = vc.pdfs;
     p.reset((int)cp.v.size() + ph.depth);
@@ -3035,7 +3278,7 @@ Rgb vcm_merge(VcmCtx& vc, const Shader& sh, const VcmPath& cp, float rev_jac, in
     p.l2c[(size_t)last] = ph.pdf_from;
     p.c2l[(size_t)last] = cp.v.back().from;
     p.c2l[(size_t)last + 1] = pdf_light_rev;
-    if (ph.depth == 1) p.ne = next_event_pdf(vc.light(), anc.p);
+    if (ph.depth == 1) p.ne = next_event_pdf(vc.light(), sh.p.position, anc.p);
     // MergeMis
     float mis;
     {
@@ -3043,7 +3286,7 @@ Rgb vcm_merge(VcmCtx& vc, const Shader& sh, const VcmPath& cp, float rev_jac, in
         float radius = cp.footprint;
         float approx = p.l2c[(size_t)last] * kPi * radius * radius * vc.num_paths;
         CorrelRatios cr;
-        cr.init(p, cp.dist0, false);
+        cr.init(p, cp.dist0, ph.from_background);
         if (!cfg.disable_correl_aware_mis) approx *= cr[last];
         if (approx == 0.0f) {
             mis = 0.0f;
@@ -3141,7 +3384,13 @@ Rgb vcm_camera_path(VcmCtx& vc, uint32_t row, uint32_t col, uint32_t iteration) 
     for (int depth = 1; depth < cfg.max_depth + 1;) {
         HitRec h = trace_closest(s, ray, false);
         vc.c->rays++;
-        if (h.tri < 0) break;
+        if (h.tri < 0) {
+            if (s.has_env) {  // CameraRandomWalk.OnInvalidHit (BidirBase.Camera.cs:604-612)
+                cp.v.push_back({pdf_dir, 0.0f});
+                L = L + prefix * vcm_background_hit(vc, ray.d, cp);
+            }
+            break;
+        }
         vc.c->hits++;
         SurfacePoint hit = make_point(s, h.tri, {h.u, h.v}, h.t);
         float pdf_from = pdf_dir * area_to_solid_angle(prev, hit);
@@ -3188,7 +3437,6 @@ Rgb vcm_camera_path(VcmCtx& vc, uint32_t row, uint32_t col, uint32_t iteration) 
 uint64_t or_render_vcm(const or_scene* s, const sspt_vcm_settings* cfg, float cell_size, uint32_t first,
                        uint32_t num, float* image, sspt_ray_stats* stats, int32_t threads) {
     const int64_t W = s->width, H = s->height;
-    if (s->has_env || s->emitters.empty()) return UINT64_MAX;
     if (cfg->algorithm != SSPT_VERTEX_CACHE_BIDIR && cfg->algorithm != SSPT_VCM) return UINT64_MAX;
     if (cfg->algorithm == SSPT_VCM && (cfg->max_num_photons < 1 || cfg->max_num_photons > 32)) return UINT64_MAX;
     const uint32_t paths = cfg->num_light_paths < 0 ? (uint32_t)(W * H) : (uint32_t)cfg->num_light_paths;
