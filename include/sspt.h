@@ -462,7 +462,9 @@ sspt_status sspt_render_debug(sspt_ctx* ctx, sspt_scene* scene, uint32_t index, 
  * unspecified order (fp32 atomics); non-finite splats are counted
  * (sspt_framebuffer_nan_count). stats count the light paths' closest-hit
  * traces and their camera connections' occlusion tests. Scenes with a
- * background or without emitters, and batches, are SSPT_ERR_UNSUPPORTED. */
+ * background (whose hits the camera paths of this configuration would add:
+ * render it with sspt_render_vcm, SSPT_VERTEX_CACHE_BIDIR) or without
+ * emitters, and batches, are SSPT_ERR_UNSUPPORTED. */
 typedef struct sspt_lt_settings {
     uint32_t base_seed_light;  /* BaseSeedLight, default 0x13C0FEFE */
     int32_t num_light_paths;   /* NumLightPaths, < 0 (default) = one per pixel */
@@ -495,8 +497,16 @@ sspt_status sspt_render_light_tracer(sspt_ctx* ctx, sspt_scene* scene, const ssp
  * add every finite value / n, the light tracer's in an unspecified order);
  * non-finite values are counted. The light-path cache holds every vertex (the
  * reference's PathCache of min(MaxDepth + 1, 10) vertices per path drops whole
- * paths only on overflow, which cannot happen for MaxDepth <= 9). Scenes with
- * a background or without emitters, and batches, are SSPT_ERR_UNSUPPORTED;
+ * paths only on overflow, which cannot happen for MaxDepth <= 9). With a
+ * background (EnvironmentMap), SelectLight picks it with probability
+ * 1 / (1 + emitters) and the light path starts from EnvironmentMap.SampleRay
+ * (a map direction and a point on the disc of radius SceneRadius about the
+ * scene's vertex mean, BidirBase.Light.cs:103-133,174-189,283-299,
+ * RandomWalk.cs:89-128); next-event estimation picks it with the same
+ * probability and camera paths that leave the scene add its radiance under
+ * EmitterHitMis (BidirBase.Camera.cs:338-418,532-561), as the reference does
+ * (no MinDepth test for these hits; the background's next-event pdf without
+ * the NumShadowRays factor). Batches are SSPT_ERR_UNSUPPORTED;
  * num_light_paths other than -1 or the pixel count is SSPT_ERR_INVALID
  * (ClassicBidir.cs:72-75). */
 typedef struct sspt_bidir_settings {
@@ -538,8 +548,9 @@ sspt_status sspt_render_bidir(sspt_ctx* ctx, sspt_scene* scene, const sspt_bidir
  * the reference's sources). All techniques are combined by the balance
  * heuristic, for SSPT_VCM with the correlation-aware merge weights (Grittmann
  * et al. 2021, CorrelAwareRatios / MergeMis :580-697) unless disable_correl_aware_mis.
- * Scenes with a background or without emitters, batches, and
- * max_num_photons outside 1..32 are rejected. */
+ * Backgrounds as for sspt_render_bidir (hits added whatever enable_hitting,
+ * CorrelAwareRatios with fromBackground). Batches and max_num_photons outside
+ * 1..32 are rejected. */
 typedef enum sspt_vcm_algorithm {
     SSPT_VERTEX_CACHE_BIDIR = 0,  /* VertexCacheBidir */
     SSPT_VCM = 1                  /* VertexConnectionAndMerging */

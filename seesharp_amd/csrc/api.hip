@@ -512,6 +512,27 @@ sspt_status add_scene(sspt_ctx* ctx, WorldBuilder& W, const sspt_scene_desc* d) 
     // RegularGrid2d.Normalize + PiecewiseConstantPDF ctor (RegularGrid2d.cs:56-71,
     // PiecewiseConstant.cs:12-31)
     if (d->background) {
+        // Scene.Prepare (Scene.cs:114-141): the centre is the mean of every mesh vertex
+        // (summed in mesh and vertex order), the radius the distance to the furthest one
+        float sum[3] = {0.0f, 0.0f, 0.0f};
+        uint64_t total = 0;
+        for (uint32_t mi = 0; mi < d->num_meshes; ++mi) {
+            const sspt_mesh_desc& m = d->meshes[mi];
+            for (uint32_t v = 0; v < m.num_vertices; ++v)
+                for (int a = 0; a < 3; ++a) sum[a] = sum[a] + m.positions[v * 3 + a];
+            total += m.num_vertices;
+        }
+        for (int a = 0; a < 3; ++a) R.scene_center[a] = sum[a] / (float)total;
+        float r2 = 0.0f;
+        for (uint32_t mi = 0; mi < d->num_meshes; ++mi) {
+            const sspt_mesh_desc& m = d->meshes[mi];
+            for (uint32_t v = 0; v < m.num_vertices; ++v) {
+                const float* q = m.positions + (size_t)v * 3;
+                const float x = q[0] - R.scene_center[0], y = q[1] - R.scene_center[1], z = q[2] - R.scene_center[2];
+                r2 = std::max(x * x + y * y + z * z, r2);
+            }
+        }
+        R.scene_radius = std::sqrt(r2);
         int ew = (int)d->background->width, eh = (int)d->background->height;
         R.has_env = 1;
         R.env_w = ew;
@@ -1769,9 +1790,6 @@ sspt_status sspt_render_bidir(sspt_ctx* ctx, sspt_scene* sc, const sspt_bidir_se
     if (sc->recs.size() != 1)
         return fail(ctx, SSPT_ERR_UNSUPPORTED, "bidirectional rendering takes a single scene, not a batch or view set");
     const SceneRec& R = sc->recs[0];
-    if (R.has_env)
-        return fail(ctx, SSPT_ERR_UNSUPPORTED, "bidirectional rendering does not sample the background (scene has one)");
-    if (R.num_emitters == 0) return fail(ctx, SSPT_ERR_UNSUPPORTED, "bidirectional rendering needs emitters");
     if (cfg->max_depth < 0 || cfg->min_depth < 0 || cfg->num_shadow_rays < 0)
         return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "max_depth, min_depth and num_shadow_rays must be non-negative");
     const uint64_t paths = (uint64_t)R.num_pixels;
@@ -1782,8 +1800,9 @@ sspt_status sspt_render_bidir(sspt_ctx* ctx, sspt_scene* sc, const sspt_bidir_se
     if ((uint64_t)first_iteration + num_iterations > 0xFFFFFFFFull)
         return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "iteration index overflows");
     // light cache: max_depth levels of two float4 per path, the lengths and next-event pdfs;
-    // camera pdfs: max_depth levels of a float2 per pixel
-    const uint64_t levels = (uint64_t)std::max(cfg->max_depth, 1);
+    // camera pdfs: max_depth levels of a float2 per pixel. A path from the background
+    // stores its first hit whatever MaxDepth (RandomWalk.StartFromBackground): two levels.
+    const uint64_t levels = (uint64_t)std::max(cfg->max_depth, R.has_env ? 2 : 1);
     const uint64_t need = levels * paths * (2 * sizeof(float4) + sizeof(float2)) + paths * 8 + 256 * 4;
     if (need > (64ull << 30)) return fail(ctx, SSPT_ERR_OUT_OF_MEMORY, "bidirectional buffers exceed 64 GiB");
     HIP_TRY(ctx, hipSetDevice(ctx->device));
@@ -1851,9 +1870,6 @@ sspt_status sspt_render_vcm(sspt_ctx* ctx, sspt_scene* sc, const sspt_vcm_settin
     if (sc->recs.size() != 1)
         return fail(ctx, SSPT_ERR_UNSUPPORTED, "bidirectional rendering takes a single scene, not a batch or view set");
     const SceneRec& R = sc->recs[0];
-    if (R.has_env)
-        return fail(ctx, SSPT_ERR_UNSUPPORTED, "bidirectional rendering does not sample the background (scene has one)");
-    if (R.num_emitters == 0) return fail(ctx, SSPT_ERR_UNSUPPORTED, "bidirectional rendering needs emitters");
     if (cfg->algorithm != SSPT_VERTEX_CACHE_BIDIR && cfg->algorithm != SSPT_VCM)
         return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "unknown algorithm %d", cfg->algorithm);
     if (cfg->max_depth < 0 || cfg->min_depth < 0 || cfg->num_shadow_rays < 0 || cfg->num_connections < 0)
@@ -1864,7 +1880,7 @@ sspt_status sspt_render_vcm(sspt_ctx* ctx, sspt_scene* sc, const sspt_vcm_settin
         return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "max_num_photons must be in 1..%d", kMaxPhotons);
     const uint64_t pixels = (uint64_t)R.num_pixels;
     const uint64_t paths = cfg->num_light_paths < 0 ? pixels : (uint64_t)cfg->num_light_paths;
-    const uint64_t levels = (uint64_t)std::max(cfg->max_depth, 1);
+    const uint64_t levels = (uint64_t)std::max(cfg->max_depth, R.has_env ? 2 : 1);  // as in sspt_render_bidir
     if (paths * levels > 0x7FFFFFFFull)
         return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "too many light path vertices (paths x max_depth >= 2^31)");
     if ((uint64_t)first_iteration + num_iterations > 0xFFFFFFFFull)

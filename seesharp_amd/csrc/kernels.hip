@@ -1758,6 +1758,87 @@ __device__ __forceinline__ float pixel_jacobian(const DevCamera& c, f3 pos) {
     return d * d / fabsf(cosine);
 }
 
+// BidirBase.BackgroundProbability (BidirBase.Light.cs:132-133), which is also
+// ComputeNextEventBackgroundProbability (BidirBase.Camera.cs:354-355)
+__device__ __forceinline__ float background_probability(const SceneRec& R) {
+    return R.has_env ? 1.0f / (1.0f + (float)R.num_emitters) : 0.0f;
+}
+
+// SampleWarp.ToConcentricDisc (SampleWarp.cs:219-247): Shirley and Chiu's
+// concentric map of the unit square onto the unit disc.
+__device__ __forceinline__ f2 to_concentric_disc(f2 p) {
+    float phi, r;
+    const float a = 2.0f * p.x - 1.0f, b = 2.0f * p.y - 1.0f;
+    if (a > -b) {
+        if (a > b) {
+            r = a;
+            phi = (kPi * 0.25f) * sdiv(b, a);
+        } else {
+            r = b;
+            phi = (kPi * 0.25f) * (2.0f - sdiv(a, b));
+        }
+    } else if (a < b) {
+        r = -a;
+        phi = (kPi * 0.25f) * (4.0f + sdiv(b, a));
+    } else {
+        r = -b;
+        phi = b != 0.0f ? (kPi * 0.25f) * (6.0f - sdiv(a, b)) : 0.0f;
+    }
+    float sp, cp;
+    ssincos(phi, &sp, &cp);
+    return f2{r * cp, r * sp};
+}
+
+// EnvironmentMap.SampleRay (EnvironmentMap.cs:87-103): a direction of the map and
+// a point on the disc of radius SceneRadius about SceneCenter + SceneRadius *
+// direction, orthogonal to it; the ray runs back along the direction.
+struct EnvRay {
+    f3 o, d, weight;
+    float pdf;
+};
+__device__ __forceinline__ float background_pos_pdf(const SceneRec& R) {
+    return sdiv(1.0f / kPi, R.scene_radius * R.scene_radius);  // ToConcentricDiscJacobian / R^2
+}
+__device__ EnvRay env_sample_ray(const Env& E, const SceneRec& R, f2 primary_pos, f2 primary_dir) {
+    const EnvSample ds = env_sample(E, primary_dir);
+    const f2 disc = to_concentric_disc(primary_pos);
+    Frame f;
+    f.build(ds.dir);
+    const f3 centre = mk3(R.scene_center[0], R.scene_center[1], R.scene_center[2]);
+    EnvRay r;
+    r.o = centre + R.scene_radius * (ds.dir + f.t * disc.x + f.b * disc.y);
+    r.d = -ds.dir;
+    const float pos_pdf = background_pos_pdf(R);
+    r.weight = ds.weight / pos_pdf;
+    r.pdf = pos_pdf * ds.pdf;
+    return r;
+}
+// BidirBase.ComputeBackgroundPdf (BidirBase.Light.cs:174-178) over
+// EnvironmentMap.RayPdf (EnvironmentMap.cs:127-130): the pdf of the background
+// ray along `light_to_surface`, with the background's selection probability.
+__device__ __forceinline__ float background_pdf(const Env& E, const SceneRec& R, f3 light_to_surface) {
+    const float pdf = background_pos_pdf(R) * env_pdf(E, -light_to_surface);
+    return pdf * background_probability(R);
+}
+
+// Light vertex k of path p in the cache: a surface point, or the ray origin
+// (kept in lv_a, triangle -1) for vertex 0 of a path from the background.
+__device__ __forceinline__ Surf light_vertex(const DevScene& S, const BidirParams& P, int k, uint32_t p) {
+    const size_t at = (size_t)k * P.num_paths + p;
+    const float4 b = P.lv_b[at];
+    const int tri = __float_as_int(b.w);
+    if (tri < 0) {
+        Surf s{};
+        s.pos = f3_of(P.lv_a[at]);
+        return s;
+    }
+    return surface(S, tri, b.x, b.y);
+}
+// Whether light path p starts at the background (vertex 0 on no triangle).
+__device__ __forceinline__ bool from_background(const BidirParams& P, uint32_t p) {
+    return __float_as_int(P.lv_b[p].w) < 0;
+}
+
 struct BidirCtx {
     const DevScene& S;
     const SceneRec& R;
@@ -1766,20 +1847,23 @@ struct BidirCtx {
     __device__ __forceinline__ float4 va(int k) const { return P.lv_a[(size_t)k * P.num_paths + path]; }
     __device__ __forceinline__ float4 vb(int k) const { return P.lv_b[(size_t)k * P.num_paths + path]; }
     __device__ __forceinline__ float2 cam(int k) const { return P.cam[(size_t)k * P.num_pixels + path]; }
-    __device__ __forceinline__ Surf vertex(int k) const {
-        const float4 b = vb(k);
-        return surface(S, __float_as_int(b.w), b.x, b.y);
-    }
+    __device__ __forceinline__ Surf vertex(int k) const { return light_vertex(S, P, k, path); }
     // ClassicBidir.NextEventPdf (ClassicBidir.cs:42-44) over BidirBase.NextEventPdf
-    // (BidirBase.Camera.cs:338-347) of a point on emitter e, without a background
+    // (BidirBase.Camera.cs:338-347) of a point on emitter e
     __device__ __forceinline__ float next_event_pdf(const DevEmitter& e) const {
-        return e.inv_area * (1.0f / (float)R.num_emitters) * (1.0f - 0.0f) * (float)P.num_shadow_rays;
+        return e.inv_area * (1.0f / (float)R.num_emitters) * (1.0f - background_probability(R)) *
+               (float)P.num_shadow_rays;
+    }
+    // The same of the background point `to` seen from `from`: the map's pdf of the
+    // (unnormalised) direction between them
+    __device__ __forceinline__ float next_event_pdf_background(const Env& E, f3 from, f3 to) const {
+        return env_pdf(E, to - from) * background_probability(R) * (float)P.num_shadow_rays;
     }
     // BidirBase.ComputeEmitterPdf (BidirBase.Light.cs:160-166) with SelectLightPmf (:116-122)
     __device__ __forceinline__ float emitter_pdf(const DevEmitter& e, f3 ns, f3 light_to_surface, float jac) const {
         float pdf = emitter_pdf_ray(e, ns, light_to_surface);
         pdf *= jac;
-        pdf *= (1.0f - 0.0f) / (float)R.num_emitters;
+        pdf *= (1.0f - background_probability(R)) / (float)R.num_emitters;
         return pdf;
     }
     // ClassicBidir.CameraPathReciprocals (ClassicBidir.cs:164-176) over the
@@ -1821,10 +1905,12 @@ struct BidirCtx {
     }
 };
 
-// TraceLightPaths / TraceEmitterPath (BidirBase.Light.cs:197-282) with
-// RandomWalk.StartFromEmitter / ContinueWalk (RandomWalk.cs:82-90,150-217) and
-// LightPathWalk (BidirBase.Light.cs:324-410) storing the vertices; with the
-// light tracer on, every vertex is connected to the camera as it is stored
+// TraceLightPaths / TraceLightPath (BidirBase.Light.cs:197-219): SelectLight
+// (:103-111) picks the background with BackgroundProbability, else an emitter;
+// TraceEmitterPath / TraceBackgroundPath (:268-299) with RandomWalk.StartFromEmitter
+// / StartFromBackground / ContinueWalk (RandomWalk.cs:82-128,150-217) and
+// LightPathWalk (BidirBase.Light.cs:324-410) storing the vertices; with the light
+// tracer on, every vertex is connected to the camera as it is stored
 // (ConnectLightVertexToCamera, BidirBase.Light.cs:35-92, with
 // ClassicBidir.LightTracerMis, ClassicBidir.cs:131-134): the connection draws
 // no random numbers and reads only the vertices stored before it.
@@ -1833,34 +1919,135 @@ __global__ void __launch_bounds__(kTraceBlock) k_bidir_light(DevScene S, int sce
     __shared__ int stack_node[kStackNodeLds];
     __shared__ float stack_t[kStackLds * kTraceBlock];
     const SceneRec& R = S.recs[scene];
+    const Env E = env_of(S, R);
     const f3 cam = mk3(R.cam.pos[0], R.cam.pos[1], R.cam.pos[2]);
+    const float bg = background_probability(R);
     for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < P.num_paths; idx += gridDim.x * blockDim.x) {
         BidirCtx B{S, R, P, idx};
         Rng rng{hash_seed(P.seed_light, idx, P.iteration)};
-        // SelectLight (:103-110): no background, so nothing is drawn for it
-        const DevEmitter e = S.emitters[R.emitter_off + (int)rng.next_int((uint32_t)R.num_emitters)];
-        const float sel = (1.0f - 0.0f) / (float)R.num_emitters;
-        const f2 ppos = rng.next_float2();
-        const f2 pdir = rng.next_float2();
-        EmitterRay er = emitter_ray(S, e, ppos, pdir);
-        er.pdf *= sel;
-        const float ne = B.next_event_pdf(e);
-        P.lv_ne[idx] = ne;
-        if (er.pdf == 0.0f || is_zero(er.weight)) {
-            P.lv_len[idx] = 0;
-            continue;
+        // the light tracer's splat of vertex s (depth `depth`, ancestor `anc`, which is
+        // the background ray's origin, on no surface, for the first hit from the background)
+        auto splat = [&](const Surf& s, const Surf& anc, bool anc_on_surface, f3 prefix, int depth, float ne) {
+            uint32_t pix;
+            float jac;
+            if (!(P.mode == kModeClassic && P.light_tracer && depth + 1 >= P.min_depth &&
+                  camera_response(R.cam, s.pos, s.ng, &pix, &jac)))
+                return;
+            const f3 p0 = offset_point(s, cam - s.pos);
+            const bool occ = traverse<true>(S, R.root, R.abs_max, p0, cam - p0, 1e-5f, 1.0f - 1e-5f,
+                                            stack_node + threadIdx.x, stack_t + threadIdx.x)
+                                 .li >= 0;
+            wave_count(stats, kStatShadow, true);
+            wave_count(stats, kStatShadowTraced, true);
+            wave_count(stats, kStatOccluded, occ);
+            if (occ) return;
+            f3 to_cam = cam - s.pos;
+            to_cam = to_cam / length(to_cam);
+            const f3 to_anc = normalize(anc.pos - s.pos);
+            Shader sh;
+            make_shader(S, s, to_cam, &sh);  // the camera side: IsOnLightSubpath = false
+            f3 f = material_value(sh, to_anc);
+            if (is_zero(f)) return;
+            f = f * sdiv(fabsf(dot(s.ns, to_anc)), fabsf(dot(s.ng, to_anc)));
+            float pdf_rev, unused;
+            pdf_pair(sh, to_anc, &pdf_rev, &unused);
+            if (anc_on_surface) pdf_rev *= area_to_solid_angle(s.pos, anc.pos, anc.ng);
+            // PdfEmit = invLensArea * jacobian, then the vertex's own reverse pdf
+            const float mis =
+                srcp(1.0f + sdiv(B.light_reciprocals(0, depth, 1.0f * jac, pdf_rev, ne), P.num_paths_f));
+            const f3 w = prefix * f * jac / P.num_paths_f;
+            const f3 v = mis * w;
+            const bool bad = !isfinite(average(v));
+            wave_count(stats, kStatNan, bad);
+            if (!bad) {
+                const f3 add = div_exact(v, P.n);
+                float* px = image + 3ull * pix;
+                atomicAdd(px, add.x);
+                atomicAdd(px + 1, add.y);
+                atomicAdd(px + 2, add.z);
+            }
+        };
+        const bool from_bg = bg > 0.0f && rng.next_float() <= bg;
+        f3 prefix, approx = gray(1.0f);
+        Surf prev;
+        float pdf_dir, next_rev = 0.0f, ne = 0.0f;
+        f3 o, d;
+        float tmin;
+        int depth = 1, len = 1;
+        if (from_bg) {
+            const f2 ppos = rng.next_float2();  // SampleBackground: primaryPos, then primaryDir
+            const f2 pdir = rng.next_float2();
+            EnvRay er = env_sample_ray(E, R, ppos, pdir);
+            er.pdf *= bg;
+            er.weight = er.weight / bg;
+            if (er.pdf == 0.0f || is_zero(er.weight)) {
+                P.lv_len[idx] = 0;
+                continue;
+            }
+            P.lv_a[idx] = f4_of(er.o, 0.0f);
+            P.lv_b[idx] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
+            // StartFromBackground: the first hit samples its direction before it is
+            // stored; its pdfs stay per solid angle and its weight is the initial one
+            const HitOut h = traverse<false>(S, R.root, R.abs_max, er.o, er.d, 0.0f, INFINITY,
+                                             stack_node + threadIdx.x, stack_t + threadIdx.x);
+            wave_count(stats, kStatRays, true);
+            wave_count(stats, kStatHits, h.li >= 0);
+            if (h.li < 0) {
+                P.lv_len[idx] = 1;
+                continue;
+            }
+            const Surf s = surface(S, h.li, h.u, h.v);
+            Shader sh;
+            make_shader(S, s, -er.d, &sh);
+            sh.light = true;
+            const float pc = rng.next_float();
+            const f2 pd = rng.next_float2();
+            const BsdfSample bs = sample_bsdf<true>(sh, pc, pd);
+            approx = approx * bs.weight;
+            P.lv_a[P.num_paths + idx] = f4_of(er.weight, er.pdf);
+            P.lv_b[P.num_paths + idx] = make_float4(h.u, h.v, 0.0f, __int_as_float(h.li));
+            len = 2;
+            Surf origin{};
+            origin.pos = er.o;
+            ne = B.next_event_pdf_background(E, s.pos, er.o);
+            splat(s, origin, false, er.weight, 1, ne);
+            next_rev = bs.pdf_rev;  // OnContinue
+            if (P.max_depth <= 1 || bs.pdf == 0.0f || is_zero(bs.weight)) {
+                P.lv_ne[idx] = ne;
+                P.lv_len[idx] = len;
+                continue;
+            }
+            prefix = er.weight * bs.weight;
+            depth = 2;
+            pdf_dir = bs.pdf;
+            prev = s;
+            o = offset_point(s, bs.dir);
+            d = bs.dir;
+            tmin = s.err;
+        } else {
+            const DevEmitter e = S.emitters[R.emitter_off + (int)rng.next_int((uint32_t)R.num_emitters)];
+            const float sel = (1.0f - bg) / (float)R.num_emitters;
+            const f2 ppos = rng.next_float2();
+            const f2 pdir = rng.next_float2();
+            EmitterRay er = emitter_ray(S, e, ppos, pdir);
+            er.pdf *= sel;
+            ne = B.next_event_pdf(e);
+            if (er.pdf == 0.0f || is_zero(er.weight)) {
+                P.lv_len[idx] = 0;
+                continue;
+            }
+            const f2 b0 = to_uniform_triangle(ppos);
+            P.lv_a[idx] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            P.lv_b[idx] = make_float4(b0.x, b0.y, 0.0f, __int_as_float(e.tri));
+            prefix = er.weight / sel;
+            prev = er.p;
+            pdf_dir = er.pdf;
+            o = offset_point(er.p, er.dir);
+            d = er.dir;
+            tmin = er.p.err;
         }
-        const f2 b0 = to_uniform_triangle(ppos);
-        P.lv_a[idx] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        P.lv_b[idx] = make_float4(b0.x, b0.y, 0.0f, __int_as_float(e.tri));
-        int len = 1;
-        f3 prefix = er.weight / sel;
-        f3 approx = gray(1.0f);
-        Surf prev = er.p;
-        float pdf_dir = er.pdf, next_rev = 0.0f;
-        f3 o = offset_point(er.p, er.dir), d = er.dir;
-        float tmin = er.p.err;
-        for (int depth = 1; depth < P.max_depth;) {
+        P.lv_ne[idx] = ne;
+        for (; depth < P.max_depth;) {
             const HitOut h = traverse<false>(S, R.root, R.abs_max, o, d, tmin, INFINITY, stack_node + threadIdx.x,
                                              stack_t + threadIdx.x);
             wave_count(stats, kStatRays, true);
@@ -1873,46 +2060,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_bidir_light(DevScene S, int sce
             P.lv_a[at] = f4_of(prefix, pdf_from);
             P.lv_b[at] = make_float4(h.u, h.v, next_rev, __int_as_float(h.li));
             len = depth + 1;
-            uint32_t pix;
-            float jac;
-            if (P.mode == kModeClassic && P.light_tracer && depth + 1 >= P.min_depth &&
-                camera_response(R.cam, s.pos, s.ng, &pix, &jac)) {
-                const f3 p0 = offset_point(s, cam - s.pos);
-                const bool occ = traverse<true>(S, R.root, R.abs_max, p0, cam - p0, 1e-5f, 1.0f - 1e-5f,
-                                                stack_node + threadIdx.x, stack_t + threadIdx.x)
-                                     .li >= 0;
-                wave_count(stats, kStatShadow, true);
-                wave_count(stats, kStatShadowTraced, true);
-                wave_count(stats, kStatOccluded, occ);
-                if (!occ) {
-                    f3 to_cam = cam - s.pos;
-                    to_cam = to_cam / length(to_cam);
-                    const f3 to_anc = normalize(prev.pos - s.pos);
-                    Shader sh;
-                    make_shader(S, s, to_cam, &sh);  // the camera side: IsOnLightSubpath = false
-                    f3 f = material_value(sh, to_anc);
-                    if (!is_zero(f)) {
-                        f = f * sdiv(fabsf(dot(s.ns, to_anc)), fabsf(dot(s.ng, to_anc)));
-                        float pdf_rev, unused;
-                        pdf_pair(sh, to_anc, &pdf_rev, &unused);
-                        pdf_rev *= area_to_solid_angle(s.pos, prev.pos, prev.ng);
-                        // PdfEmit = invLensArea * jacobian, then the vertex's own reverse pdf
-                        const float mis = srcp(1.0f + sdiv(B.light_reciprocals(0, depth, 1.0f * jac, pdf_rev, ne),
-                                                           P.num_paths_f));
-                        const f3 w = prefix * f * jac / P.num_paths_f;
-                        const f3 v = mis * w;
-                        const bool bad = !isfinite(average(v));
-                        wave_count(stats, kStatNan, bad);
-                        if (!bad) {
-                            const f3 add = div_exact(v, P.n);
-                            float* px = image + 3ull * pix;
-                            atomicAdd(px, add.x);
-                            atomicAdd(px + 1, add.y);
-                            atomicAdd(px + 2, add.z);
-                        }
-                    }
-                }
-            }
+            splat(s, prev, true, prefix, depth, ne);
             if (depth + 1 >= P.max_depth) break;
             const float q = depth > 4 ? clampf(average(approx), 0.05f, 0.95f) : 1.0f;
             if (rng.next_float() > q) break;
@@ -1953,6 +2101,8 @@ __global__ void __launch_bounds__(kTraceBlock) k_bidir_camera(DevScene S, int sc
     __shared__ float stack_t[kStackLds * kTraceBlock];
     const SceneRec& R = S.recs[scene];
     const DevCamera& C = R.cam;
+    const Env E = env_of(S, R);
+    const float bg = background_probability(R);  // ComputeNextEventBackgroundProbability
     for (uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x; pix < P.num_paths; pix += gridDim.x * blockDim.x) {
         BidirCtx B{S, R, P, pix};
         const uint32_t row = pix / (uint32_t)C.width, col = pix % (uint32_t)C.width;
@@ -1967,13 +2117,31 @@ __global__ void __launch_bounds__(kTraceBlock) k_bidir_camera(DevScene S, int sc
         float pdf_dir = pixel_jacobian(C, o + raw);
         const int light_len = P.lv_len[pix];
         const float ne = P.lv_ne[pix];
+        const bool light_from_bg = light_len > 0 && from_background(P, pix);
         f3 L = gray(0.0f), prefix = gray(1.0f), approx = gray(1.0f);
         for (int depth = 1; depth < P.max_depth + 1;) {
             const HitOut h = traverse<false>(S, R.root, R.abs_max, o, d, tmin, INFINITY, stack_node + threadIdx.x,
                                              stack_t + threadIdx.x);
             wave_count(stats, kStatRays, true);
             wave_count(stats, kStatHits, h.li >= 0);
-            if (h.li < 0) break;  // OnBackgroundHit without a background adds nothing
+            if (h.li < 0) {
+                // CameraRandomWalk.OnInvalidHit (BidirBase.Camera.cs:604-612): the escaped
+                // direction is the last vertex, with its solid-angle pdf; OnBackgroundHit
+                // (:532-561) with ClassicBidir.EmitterHitMis, no MinDepth test
+                if (R.has_env) {
+                    const int c = depth;
+                    float mis = 1.0f;
+                    if (c > 1) {
+                        const float x_prev = c - 2 >= 1 ? B.cam(c - 2).y : 0.0f;
+                        float sum = 1.0f;
+                        sum += sdiv(env_pdf(E, d) * bg, pdf_dir);
+                        sum += sdiv(B.camera_reciprocals(c - 2, background_pdf(E, R, -d), x_prev), pdf_dir);
+                        mis = srcp(sum);
+                    }
+                    L = L + prefix * (mis * env_radiance(E, d));
+                }
+                break;
+            }
             const Surf s = surface(S, h.li, h.u, h.v);
             const float pdf_from = pdf_dir * area_to_solid_angle(prev_pos, s.pos, s.ng);
             if (pdf_from == 0.0f) break;
@@ -2029,7 +2197,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_bidir_camera(DevScene S, int sc
                             pcr *= rev_jac;
                             pcl *= area_to_solid_angle(s.pos, lv.pos, lv.ng);
                             pdf_pair(ls, -dir, &plc, &plr);
-                            plr *= area_to_solid_angle(lv.pos, anc.pos, anc.ng);
+                            if (k > 1 || !light_from_bg) plr *= area_to_solid_angle(lv.pos, anc.pos, anc.ng);
                             plc *= area_to_solid_angle(lv.pos, s.pos, s.ng);
                             const float dist2 = dot(dd, dd);
                             if (!is_zero(f_cam) && !is_zero(f_light) && pcl != 0.0f && dist2 != 0.0f) {
@@ -2048,8 +2216,33 @@ __global__ void __launch_bounds__(kTraceBlock) k_bidir_camera(DevScene S, int sc
             }
             if (depth < P.max_depth && depth + 1 >= P.min_depth) {
                 for (int r = 0; r < P.num_shadow_rays; ++r) {
-                    const float bg = 0.0f;  // ComputeNextEventBackgroundProbability without a background
-                    if (rng.next_float() < bg) continue;
+                    if (rng.next_float() < bg) {
+                        // the background branch (BidirBase.Camera.cs:381-418): its pdf carries
+                        // no NumShadowRays factor; ClassicBidir.NextEventMis (ClassicBidir.cs:147-153)
+                        EnvSample es = env_sample(E, rng.next_float2());
+                        es.pdf *= bg;
+                        es.weight = es.weight / bg;
+                        if (es.pdf == 0.0f) continue;
+                        const bool occ = traverse<true>(S, R.root, R.abs_max, offset_point(s, es.dir), es.dir, s.err,
+                                                        INFINITY, stack_node + threadIdx.x, stack_t + threadIdx.x)
+                                             .li >= 0;  // Raytracer.LeavesScene
+                        wave_count(stats, kStatShadow, true);
+                        wave_count(stats, kStatShadowTraced, true);
+                        wave_count(stats, kStatOccluded, occ);
+                        if (occ) continue;
+                        f3 bc;
+                        float unused, fwd, rev;
+                        eval_cos_pdf(sh, es.dir, &bc, &unused);
+                        pdf_pair(sh, es.dir, &fwd, &rev);
+                        rev *= rev_jac;
+                        if (fwd == 0.0f || rev == 0.0f || is_zero(bc)) continue;
+                        float sum = 1.0f;
+                        sum += sdiv(fwd, es.pdf);
+                        sum += sdiv(B.camera_reciprocals(c - 1, background_pdf(E, R, -es.dir), rev), es.pdf);
+                        value = value + prefix * (srcp(sum) * (es.weight * bc));
+                        continue;
+                    }
+                    if (R.num_emitters == 0) continue;
                     const DevEmitter e = S.emitters[R.emitter_off + (int)rng.next_int((uint32_t)R.num_emitters)];
                     const float prob = 1.0f / (float)R.num_emitters;
                     const f2 b = to_uniform_triangle(rng.next_float2());
@@ -2174,7 +2367,9 @@ struct PdfView {
 struct Correl {
     const PdfView* v;
     float accept_area;
-    __device__ __forceinline__ Correl(const PdfView& view, float dist_to_cam) : v(&view) {
+    bool from_bg;  // the light end is the background: its last density stays per solid angle
+    __device__ __forceinline__ Correl(const PdfView& view, float dist_to_cam, bool from_background)
+        : v(&view), from_bg(from_background) {
         const float radius = dist_to_cam * 0.0174550649f;
         accept_area = radius * radius * kPi;
     }
@@ -2186,7 +2381,7 @@ struct Correl {
         float light = 1.0f;
         for (int i = nsv - 1; i >= idx; --i) {
             float next = v->l2c(i) * accept_area;
-            if (i == nsv - 1) next *= accept_area;  // not from the background
+            if (i == nsv - 1 && !from_bg) next *= accept_area;
             light *= fminf(next, 1.0f);
         }
         if (cam == 0.0f && light == 0.0f) return 1.0f;
@@ -2398,12 +2593,12 @@ __global__ void __launch_bounds__(kTraceBlock) k_vcm_splat(DevScene S, int scene
         const int len = P.lv_len[idx];
         if (len < 2) continue;
         const float ne = P.lv_ne[idx];
-        float4 b = P.lv_b[idx];
-        Surf prev = surface(S, __float_as_int(b.w), b.x, b.y);
+        const bool from_bg = from_background(P, idx);
+        Surf prev = light_vertex(S, P, 0, idx);
         for (int depth = 1; depth < len; ++depth) {
             const size_t at = (size_t)depth * P.num_paths + idx;
             const float4 a = P.lv_a[at];
-            b = P.lv_b[at];
+            const float4 b = P.lv_b[at];
             const Surf s = surface(S, __float_as_int(b.w), b.x, b.y);
             uint32_t pix;
             float jac;
@@ -2427,20 +2622,20 @@ __global__ void __launch_bounds__(kTraceBlock) k_vcm_splat(DevScene S, int scene
                         f = f * sdiv(fabsf(dot(s.ns, to_anc)), fabsf(dot(s.ng, to_anc)));
                         float pdf_rev, unused;
                         pdf_pair(sh, to_anc, &pdf_rev, &unused);
-                        pdf_rev *= area_to_solid_angle(s.pos, prev.pos, prev.ng);
+                        if (depth > 1 || !from_bg) pdf_rev *= area_to_solid_angle(s.pos, prev.pos, prev.ng);
                         PdfView v{&P, 0u, idx, depth + 1, 0, ne};
                         v.set_c2l(0, 1.0f * jac);
                         v.set_c2l(1, pdf_rev);
                         float mis;
                         if (P.mode == kModeVcm) {
-                            const Correl cr(v, dist);
+                            const Correl cr(v, dist, from_bg);
                             const float footprint = sqrtf(1.0f / v.c2l(0));
                             float sum = vcm_light_reciprocals(P, v, -1, footprint, cr, density);
                             sum = sdiv(sum, P.num_paths_f);
                             sum += 1.0f;
                             mis = srcp(sum);
                         } else {
-                            const Correl cr(v, dist);
+                            const Correl cr(v, dist, from_bg);
                             mis = srcp(1.0f + sdiv(vcm_light_reciprocals(P, v, -1, 0.0f, cr, density),
                                                    P.num_paths_f));
                         }
@@ -2479,7 +2674,9 @@ __global__ void __launch_bounds__(kTraceBlock) k_vcm_camera(DevScene S, int scen
     const bool vcm = P.mode == kModeVcm;
     const int num_vertices = P.num_paths > 0 ? P.cum[P.num_paths - 1] : 0;
     const float density = select_density(P, num_vertices);
-    const float pmf_light = (1.0f - 0.0f) / (float)R.num_emitters;
+    const Env E = env_of(S, R);
+    const float bg = background_probability(R);
+    const float pmf_light = (1.0f - bg) / (float)R.num_emitters;  // unused without emitters
     for (uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x; pix < P.num_pixels; pix += gridDim.x * blockDim.x) {
         const uint32_t row = pix / (uint32_t)C.width, col = pix % (uint32_t)C.width;
         Rng rng{hash_seed(P.seed_camera, pix, P.iteration)};
@@ -2497,7 +2694,27 @@ __global__ void __launch_bounds__(kTraceBlock) k_vcm_camera(DevScene S, int scen
                                              stack_t + threadIdx.x);
             wave_count(stats, kStatRays, true);
             wave_count(stats, kStatHits, h.li >= 0);
-            if (h.li < 0) break;
+            if (h.li < 0) {
+                // OnInvalidHit / OnBackgroundHit (BidirBase.Camera.cs:532-561,604-612) with
+                // EmitterHitMis (VertexCacheBidir.cs:171-180, VertexConnectionAndMerging.cs:700-732),
+                // whatever EnableHitting and MinDepth say
+                if (R.has_env) {
+                    const int c = depth;
+                    P.cam[(size_t)(c - 1) * P.num_pixels + pix] = make_float2(pdf_dir, 0.0f);
+                    float mis = 1.0f;
+                    if (c > 1) {
+                        PdfView v{&P, pix, 0u, c, c, env_pdf(E, d) * bg};
+                        v.set_l2c(c - 2, background_pdf(E, R, -d));
+                        const Correl cr(v, dist0, true);
+                        float sum = 1.0f;
+                        sum += sdiv(v.ne, pdf_dir);
+                        sum += sdiv(vcm_camera_reciprocals(P, v, c - 2, footprint, cr, density), pdf_dir);
+                        mis = srcp(sum);
+                    }
+                    L = L + prefix * (mis * env_radiance(E, d));
+                }
+                break;
+            }
             const Surf s = surface(S, h.li, h.u, h.v);
             const float pdf_from = pdf_dir * area_to_solid_angle(prev_pos, s.pos, s.ng);
             if (pdf_from == 0.0f) break;
@@ -2519,10 +2736,10 @@ __global__ void __launch_bounds__(kTraceBlock) k_vcm_camera(DevScene S, int scen
                 const f3 emission = emitted(e, s.ns, -d);
                 float mis = 1.0f;
                 if (c > 1) {
-                    PdfView v{&P, pix, 0u, c, c, e.inv_area * (1.0f / (float)R.num_emitters) * (1.0f - 0.0f) *
+                    PdfView v{&P, pix, 0u, c, c, e.inv_area * (1.0f / (float)R.num_emitters) * (1.0f - bg) *
                                                      (float)P.num_shadow_rays};
                     v.set_l2c(c - 2, emitter_pdf_ray(e, s.ns, -d) * rev_jac * pmf_light);
-                    const Correl cr(v, dist0);
+                    const Correl cr(v, dist0, false);
                     const float pdf_this = pdf_from;
                     float sum = 1.0f;
                     sum += sdiv(v.ne, pdf_this);
@@ -2544,9 +2761,9 @@ __global__ void __launch_bounds__(kTraceBlock) k_vcm_camera(DevScene S, int scen
                     if (total > P.max_depth || total < P.min_depth) continue;
                     const float4 la = P.lv_a[(size_t)k * P.num_paths + lp];
                     const float4 lb = P.lv_b[(size_t)k * P.num_paths + lp];
-                    const float4 ab = P.lv_b[(size_t)(k - 1) * P.num_paths + lp];
                     const Surf lv = surface(S, __float_as_int(lb.w), lb.x, lb.y);
-                    const Surf anc = surface(S, __float_as_int(ab.w), ab.x, ab.y);
+                    const Surf anc = light_vertex(S, P, k - 1, lp);
+                    const bool lp_bg = from_background(P, lp);
                     const f3 dd = s.pos - lv.pos;
                     const f3 p0 = offset_point(lv, dd), p1 = offset_point(s, -dd);
                     const bool occ = traverse<true>(S, R.root, R.abs_max, p0, p1 - p0, 1e-5f, 1.0f - 1e-5f,
@@ -2571,7 +2788,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_vcm_camera(DevScene S, int scen
                     pcr *= rev_jac;
                     pcl *= area_to_solid_angle(s.pos, lv.pos, lv.ng);
                     pdf_pair(ls, -dir, &plc, &plr);
-                    plr *= area_to_solid_angle(lv.pos, anc.pos, anc.ng);
+                    if (k > 1 || !lp_bg) plr *= area_to_solid_angle(lv.pos, anc.pos, anc.ng);
                     plc *= area_to_solid_angle(lv.pos, s.pos, s.ng);
                     const float dist2 = dot(dd, dd);
                     if (is_zero(f_cam) || is_zero(f_light) || pcl == 0.0f || dist2 == 0.0f) continue;
@@ -2580,7 +2797,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_vcm_camera(DevScene S, int scen
                     v.set_l2c(c - 1, plc);
                     v.set_c2l(c, pcl);
                     v.set_c2l(c + 1, plr);
-                    const Correl cr(v, dist0);
+                    const Correl cr(v, dist0, lp_bg);
                     float sum = 1.0f;
                     sum += sdiv(vcm_camera_reciprocals(P, v, c - 1, footprint, cr, density), density);
                     sum += sdiv(vcm_light_reciprocals(P, v, c - 1, footprint, cr, density), density);
@@ -2607,10 +2824,11 @@ __global__ void __launch_bounds__(kTraceBlock) k_vcm_camera(DevScene S, int scen
                         if (total > P.max_depth || total < P.min_depth) continue;
                         const float4 la = P.lv_a[(size_t)k * P.num_paths + lp];
                         const float4 lb = P.lv_b[(size_t)k * P.num_paths + lp];
-                        const float4 ab = P.lv_b[(size_t)(k - 1) * P.num_paths + lp];
                         const Surf ph = surface(S, __float_as_int(lb.w), lb.x, lb.y);
                         if (fabsf(dot(s.ng, ph.ng)) < 0.4f) continue;
-                        const Surf anc = surface(S, __float_as_int(ab.w), ab.x, ab.y);
+                        const Surf anc = light_vertex(S, P, k - 1, lp);
+                        const bool lp_bg = from_background(P, lp);
+                        const bool first_from_bg = k == 1 && lp_bg;
                         const f3 to_anc = normalize(anc.pos - s.pos);
                         f3 bsdf = material_value(sh, to_anc);
                         bsdf = bsdf * sdiv(fabsf(dot(s.ns, to_anc)), fabsf(dot(ph.ng, to_anc)));
@@ -2620,12 +2838,16 @@ __global__ void __launch_bounds__(kTraceBlock) k_vcm_camera(DevScene S, int scen
                         float pdf_light_rev, pdf_cam_rev;
                         pdf_pair(sh, to_anc, &pdf_light_rev, &pdf_cam_rev);
                         pdf_cam_rev *= rev_jac;
-                        pdf_light_rev *= area_to_solid_angle(s.pos, anc.pos, anc.ng);
+                        // at the first hit from the background the pdf stays per solid angle
+                        if (!first_from_bg) pdf_light_rev *= area_to_solid_angle(s.pos, anc.pos, anc.ng);
                         if (pdf_cam_rev == 0.0f || pdf_light_rev == 0.0f) continue;
-                        PdfView v{&P, pix, lp, c + k, c, P.lv_ne[lp]};
+                        // NextEventPdf(shader point, ancestor) for a photon of depth 1
+                        const float ne_m = first_from_bg ? env_pdf(E, anc.pos - s.pos) * bg * (float)P.num_shadow_rays
+                                                         : P.lv_ne[lp];
+                        PdfView v{&P, pix, lp, c + k, c, ne_m};
                         if (c > 1) v.set_l2c(c - 2, pdf_cam_rev);
                         v.set_c2l(c, pdf_light_rev);
-                        const Correl cr(v, dist0);
+                        const Correl cr(v, dist0, lp_bg);
                         // MergeMis (:648-697)
                         float approx_m = v.l2c(c - 1) * kPi * footprint * footprint * P.num_paths_f;
                         if (P.correl) approx_m *= cr[c - 1];
@@ -2645,8 +2867,38 @@ __global__ void __launch_bounds__(kTraceBlock) k_vcm_camera(DevScene S, int scen
             }
             if (depth < P.max_depth && depth + 1 >= P.min_depth) {
                 for (int r = 0; r < P.num_shadow_rays; ++r) {
-                    const float bg = 0.0f;
-                    if (rng.next_float() < bg) continue;
+                    if (rng.next_float() < bg) {
+                        // the background branch (BidirBase.Camera.cs:381-418), NextEventMis with
+                        // isBackground (VertexCacheBidir.cs:200-207, VertexConnectionAndMerging.cs:802-836)
+                        EnvSample es = env_sample(E, rng.next_float2());
+                        es.pdf *= bg;
+                        es.weight = es.weight / bg;
+                        if (es.pdf == 0.0f) continue;
+                        const bool occ = traverse<true>(S, R.root, R.abs_max, offset_point(s, es.dir), es.dir, s.err,
+                                                        INFINITY, stack_node + threadIdx.x, stack_t + threadIdx.x)
+                                             .li >= 0;  // Raytracer.LeavesScene
+                        wave_count(stats, kStatShadow, true);
+                        wave_count(stats, kStatShadowTraced, true);
+                        wave_count(stats, kStatOccluded, occ);
+                        if (occ) continue;
+                        f3 bc;
+                        float unused, fwd, rev;
+                        eval_cos_pdf(sh, es.dir, &bc, &unused);
+                        pdf_pair(sh, es.dir, &fwd, &rev);
+                        rev *= rev_jac;
+                        if (fwd == 0.0f || rev == 0.0f || is_zero(bc)) continue;
+                        PdfView v{&P, pix, 0u, c + 1, c, es.pdf};
+                        if (c > 1) v.set_l2c(c - 2, rev);
+                        v.set_l2c(c - 1, background_pdf(E, R, -es.dir));
+                        v.set_c2l(c, fwd);
+                        const Correl cr(v, dist0, true);
+                        float sum = 1.0f;
+                        if (P.hitting) sum += sdiv(fwd, es.pdf);
+                        sum += sdiv(vcm_camera_reciprocals(P, v, c - 1, footprint, cr, density), es.pdf);
+                        value = value + prefix * (srcp(sum) * (es.weight * bc));
+                        continue;
+                    }
+                    if (R.num_emitters == 0) continue;
                     const DevEmitter e = S.emitters[R.emitter_off + (int)rng.next_int((uint32_t)R.num_emitters)];
                     const float prob = 1.0f / (float)R.num_emitters;
                     const f2 b = to_uniform_triangle(rng.next_float2());
@@ -2681,7 +2933,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_vcm_camera(DevScene S, int scen
                     v.set_l2c(c - 1, emitter_pdf_ray(e, lp.ns, l2s) * area_to_solid_angle(lp.pos, s.pos, s.ng) *
                                          pmf_light);
                     v.set_c2l(c, fwd);
-                    const Correl cr(v, dist0);
+                    const Correl cr(v, dist0, false);
                     float sum = 1.0f;
                     if (P.hitting) sum += sdiv(fwd, lpdf);
                     sum += sdiv(vcm_camera_reciprocals(P, v, c - 1, footprint, cr, density), lpdf);

@@ -140,6 +140,7 @@ struct SceneRec {
     uint32_t env_guide_row_off;  // int index into the guide pool: env_guide_row_g + 1 entries
     uint32_t env_guide_col_off;  // env_h tables of env_guide_col_g + 1 entries
     int env_guide_row_g, env_guide_col_g;  // guide buckets (powers of two)
+    float scene_center[3], scene_radius;   // Background.SceneCenter / SceneRadius (bidirectional light paths)
     float abs_max;         // largest vertex coordinate magnitude (ray slack scale)
     uint32_t num_pixels;
     // settings (PathTracerBase fields)

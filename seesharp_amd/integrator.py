@@ -668,8 +668,10 @@ class ClassicBidir(Integrator):
     the camera (EnableLightTracer), and traces one camera path per pixel that
     adds emitter hits, NumShadowRays next-event samples and, with
     EnableConnections, connections to every vertex of its pixel's light path,
-    all combined by the balance heuristic. Scenes with a background are not
-    supported."""
+    all combined by the balance heuristic. A background is a light of its own:
+    light paths start from it (EnvironmentMap.SampleRay) with probability
+    1 / (1 + emitters), next events sample it, camera paths that leave the
+    scene hit it."""
 
     def __init__(self, device: Device | None = None):
         super().__init__()
@@ -716,7 +718,7 @@ class VertexCacheBidir(Integrator):
     camera (EnableLightTracer), and one camera path per pixel with emitter hits
     (EnableHitting), NumConnections connections to vertices drawn from the whole
     cache and NumShadowRays next-event samples, combined by the balance
-    heuristic. Scenes with a background are not supported."""
+    heuristic. Backgrounds as for ClassicBidir."""
 
     algorithm = abi.VERTEX_CACHE_BIDIR
 
