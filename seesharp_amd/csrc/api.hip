@@ -1836,6 +1836,7 @@ sspt_status sspt_render_bidir(sspt_ctx* ctx, sspt_scene* sc, const sspt_bidir_se
     P.num_paths = (uint32_t)paths;
     P.num_pixels = (uint32_t)paths;
     P.mode = kModeClassic;
+    P.env = R.has_env;
     P.max_depth = cfg->max_depth;
     P.min_depth = cfg->min_depth;
     P.num_shadow_rays = cfg->num_shadow_rays;
@@ -1936,6 +1937,7 @@ sspt_status sspt_render_vcm(sspt_ctx* ctx, sspt_scene* sc, const sspt_vcm_settin
     P.light_tracer = cfg->enable_light_tracer ? 1 : 0;
     P.num_paths_f = (float)paths;
     P.mode = vcm ? kModeVcm : kModeVertexCache;
+    P.env = R.has_env;
     P.num_connections = cfg->num_connections;
     P.hitting = cfg->enable_hitting ? 1 : 0;
     P.merging = merging ? 1 : 0;

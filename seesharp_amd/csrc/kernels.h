@@ -138,6 +138,7 @@ struct BidirParams {
     // PdfFromAncestor, PdfToAncestor.
     float2* cam;
     uint32_t num_pixels;  // camera paths (ClassicBidir: num_paths)
+    int env;              // the scene has a background (the kernels' instantiation with its code)
     // VertexCacheBidir / VertexConnectionAndMerging
     int mode;             // kModeClassic, kModeVertexCache, kModeVcm
     int num_connections, hitting, merging, merge_primary, max_photons, correl;

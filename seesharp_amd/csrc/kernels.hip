@@ -1822,12 +1822,15 @@ __device__ __forceinline__ float background_pdf(const Env& E, const SceneRec& R,
 }
 
 // Light vertex k of path p in the cache: a surface point, or the ray origin
-// (kept in lv_a, triangle -1) for vertex 0 of a path from the background.
+// (kept in lv_a, triangle -1) for vertex 0 of a path from the background. The
+// bidirectional kernels are instantiated with and without a background (ENV),
+// so scenes without one compile none of its code.
+template <bool ENV>
 __device__ __forceinline__ Surf light_vertex(const DevScene& S, const BidirParams& P, int k, uint32_t p) {
     const size_t at = (size_t)k * P.num_paths + p;
     const float4 b = P.lv_b[at];
     const int tri = __float_as_int(b.w);
-    if (tri < 0) {
+    if (ENV && tri < 0) {
         Surf s{};
         s.pos = f3_of(P.lv_a[at]);
         return s;
@@ -1835,35 +1838,37 @@ __device__ __forceinline__ Surf light_vertex(const DevScene& S, const BidirParam
     return surface(S, tri, b.x, b.y);
 }
 // Whether light path p starts at the background (vertex 0 on no triangle).
+template <bool ENV>
 __device__ __forceinline__ bool from_background(const BidirParams& P, uint32_t p) {
-    return __float_as_int(P.lv_b[p].w) < 0;
+    return ENV && __float_as_int(P.lv_b[p].w) < 0;
 }
 
+template <bool ENV>
 struct BidirCtx {
     const DevScene& S;
     const SceneRec& R;
     const BidirParams& P;
     uint32_t path;  // the light path / pixel
+    float bg;       // BackgroundProbability (0 without a background)
     __device__ __forceinline__ float4 va(int k) const { return P.lv_a[(size_t)k * P.num_paths + path]; }
     __device__ __forceinline__ float4 vb(int k) const { return P.lv_b[(size_t)k * P.num_paths + path]; }
     __device__ __forceinline__ float2 cam(int k) const { return P.cam[(size_t)k * P.num_pixels + path]; }
-    __device__ __forceinline__ Surf vertex(int k) const { return light_vertex(S, P, k, path); }
+    __device__ __forceinline__ Surf vertex(int k) const { return light_vertex<ENV>(S, P, k, path); }
     // ClassicBidir.NextEventPdf (ClassicBidir.cs:42-44) over BidirBase.NextEventPdf
     // (BidirBase.Camera.cs:338-347) of a point on emitter e
     __device__ __forceinline__ float next_event_pdf(const DevEmitter& e) const {
-        return e.inv_area * (1.0f / (float)R.num_emitters) * (1.0f - background_probability(R)) *
-               (float)P.num_shadow_rays;
+        return e.inv_area * (1.0f / (float)R.num_emitters) * (1.0f - bg) * (float)P.num_shadow_rays;
     }
     // The same of the background point `to` seen from `from`: the map's pdf of the
     // (unnormalised) direction between them
     __device__ __forceinline__ float next_event_pdf_background(const Env& E, f3 from, f3 to) const {
-        return env_pdf(E, to - from) * background_probability(R) * (float)P.num_shadow_rays;
+        return env_pdf(E, to - from) * bg * (float)P.num_shadow_rays;
     }
     // BidirBase.ComputeEmitterPdf (BidirBase.Light.cs:160-166) with SelectLightPmf (:116-122)
     __device__ __forceinline__ float emitter_pdf(const DevEmitter& e, f3 ns, f3 light_to_surface, float jac) const {
         float pdf = emitter_pdf_ray(e, ns, light_to_surface);
         pdf *= jac;
-        pdf *= (1.0f - background_probability(R)) / (float)R.num_emitters;
+        pdf *= (1.0f - bg) / (float)R.num_emitters;
         return pdf;
     }
     // ClassicBidir.CameraPathReciprocals (ClassicBidir.cs:164-176) over the
@@ -1914,66 +1919,24 @@ struct BidirCtx {
 // (ConnectLightVertexToCamera, BidirBase.Light.cs:35-92, with
 // ClassicBidir.LightTracerMis, ClassicBidir.cs:131-134): the connection draws
 // no random numbers and reads only the vertices stored before it.
-__global__ void __launch_bounds__(kTraceBlock) k_bidir_light(DevScene S, int scene, BidirParams P, float* image,
+template <bool ENV>
+__global__ void __launch_bounds__(kTraceBlock, 3) k_bidir_light(DevScene S, int scene, BidirParams P, float* image,
                                                              unsigned long long* stats) {
     __shared__ int stack_node[kStackNodeLds];
     __shared__ float stack_t[kStackLds * kTraceBlock];
     const SceneRec& R = S.recs[scene];
     const Env E = env_of(S, R);
     const f3 cam = mk3(R.cam.pos[0], R.cam.pos[1], R.cam.pos[2]);
-    const float bg = background_probability(R);
+    const float bg = ENV ? background_probability(R) : 0.0f;
     for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < P.num_paths; idx += gridDim.x * blockDim.x) {
-        BidirCtx B{S, R, P, idx};
+        BidirCtx<ENV> B{S, R, P, idx, bg};
         Rng rng{hash_seed(P.seed_light, idx, P.iteration)};
-        // the light tracer's splat of vertex s (depth `depth`, ancestor `anc`, which is
-        // the background ray's origin, on no surface, for the first hit from the background)
-        auto splat = [&](const Surf& s, const Surf& anc, bool anc_on_surface, f3 prefix, int depth, float ne) {
-            uint32_t pix;
-            float jac;
-            if (!(P.mode == kModeClassic && P.light_tracer && depth + 1 >= P.min_depth &&
-                  camera_response(R.cam, s.pos, s.ng, &pix, &jac)))
-                return;
-            const f3 p0 = offset_point(s, cam - s.pos);
-            const bool occ = traverse<true>(S, R.root, R.abs_max, p0, cam - p0, 1e-5f, 1.0f - 1e-5f,
-                                            stack_node + threadIdx.x, stack_t + threadIdx.x)
-                                 .li >= 0;
-            wave_count(stats, kStatShadow, true);
-            wave_count(stats, kStatShadowTraced, true);
-            wave_count(stats, kStatOccluded, occ);
-            if (occ) return;
-            f3 to_cam = cam - s.pos;
-            to_cam = to_cam / length(to_cam);
-            const f3 to_anc = normalize(anc.pos - s.pos);
-            Shader sh;
-            make_shader(S, s, to_cam, &sh);  // the camera side: IsOnLightSubpath = false
-            f3 f = material_value(sh, to_anc);
-            if (is_zero(f)) return;
-            f = f * sdiv(fabsf(dot(s.ns, to_anc)), fabsf(dot(s.ng, to_anc)));
-            float pdf_rev, unused;
-            pdf_pair(sh, to_anc, &pdf_rev, &unused);
-            if (anc_on_surface) pdf_rev *= area_to_solid_angle(s.pos, anc.pos, anc.ng);
-            // PdfEmit = invLensArea * jacobian, then the vertex's own reverse pdf
-            const float mis =
-                srcp(1.0f + sdiv(B.light_reciprocals(0, depth, 1.0f * jac, pdf_rev, ne), P.num_paths_f));
-            const f3 w = prefix * f * jac / P.num_paths_f;
-            const f3 v = mis * w;
-            const bool bad = !isfinite(average(v));
-            wave_count(stats, kStatNan, bad);
-            if (!bad) {
-                const f3 add = div_exact(v, P.n);
-                float* px = image + 3ull * pix;
-                atomicAdd(px, add.x);
-                atomicAdd(px + 1, add.y);
-                atomicAdd(px + 2, add.z);
-            }
-        };
         const bool from_bg = bg > 0.0f && rng.next_float() <= bg;
         f3 prefix, approx = gray(1.0f);
-        Surf prev;
+        Surf prev{};  // the ancestor; for a background path first the ray origin, on no surface
         float pdf_dir, next_rev = 0.0f, ne = 0.0f;
         f3 o, d;
         float tmin;
-        int depth = 1, len = 1;
         if (from_bg) {
             const f2 ppos = rng.next_float2();  // SampleBackground: primaryPos, then primaryDir
             const f2 pdir = rng.next_float2();
@@ -1986,44 +1949,12 @@ __global__ void __launch_bounds__(kTraceBlock) k_bidir_light(DevScene S, int sce
             }
             P.lv_a[idx] = f4_of(er.o, 0.0f);
             P.lv_b[idx] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
-            // StartFromBackground: the first hit samples its direction before it is
-            // stored; its pdfs stay per solid angle and its weight is the initial one
-            const HitOut h = traverse<false>(S, R.root, R.abs_max, er.o, er.d, 0.0f, INFINITY,
-                                             stack_node + threadIdx.x, stack_t + threadIdx.x);
-            wave_count(stats, kStatRays, true);
-            wave_count(stats, kStatHits, h.li >= 0);
-            if (h.li < 0) {
-                P.lv_len[idx] = 1;
-                continue;
-            }
-            const Surf s = surface(S, h.li, h.u, h.v);
-            Shader sh;
-            make_shader(S, s, -er.d, &sh);
-            sh.light = true;
-            const float pc = rng.next_float();
-            const f2 pd = rng.next_float2();
-            const BsdfSample bs = sample_bsdf<true>(sh, pc, pd);
-            approx = approx * bs.weight;
-            P.lv_a[P.num_paths + idx] = f4_of(er.weight, er.pdf);
-            P.lv_b[P.num_paths + idx] = make_float4(h.u, h.v, 0.0f, __int_as_float(h.li));
-            len = 2;
-            Surf origin{};
-            origin.pos = er.o;
-            ne = B.next_event_pdf_background(E, s.pos, er.o);
-            splat(s, origin, false, er.weight, 1, ne);
-            next_rev = bs.pdf_rev;  // OnContinue
-            if (P.max_depth <= 1 || bs.pdf == 0.0f || is_zero(bs.weight)) {
-                P.lv_ne[idx] = ne;
-                P.lv_len[idx] = len;
-                continue;
-            }
-            prefix = er.weight * bs.weight;
-            depth = 2;
-            pdf_dir = bs.pdf;
-            prev = s;
-            o = offset_point(s, bs.dir);
-            d = bs.dir;
-            tmin = s.err;
+            prefix = er.weight;
+            prev.pos = er.o;
+            pdf_dir = er.pdf;  // the first hit's pdf stays the ray's (per solid angle)
+            o = er.o;
+            d = er.d;
+            tmin = 0.0f;
         } else {
             const DevEmitter e = S.emitters[R.emitter_off + (int)rng.next_int((uint32_t)R.num_emitters)];
             const float sel = (1.0f - bg) / (float)R.num_emitters;
@@ -2047,23 +1978,75 @@ __global__ void __launch_bounds__(kTraceBlock) k_bidir_light(DevScene S, int sce
             tmin = er.p.err;
         }
         P.lv_ne[idx] = ne;
-        for (; depth < P.max_depth;) {
+        int len = 1;
+        // StartFromBackground traces and stores the first hit whatever MaxDepth, samples
+        // its continuation without roulette, keeps its pdfs per solid angle and continues
+        // with the BSDF weight alone (RandomWalk.cs:89-128); ContinueWalk then as for
+        // emitter paths
+        for (int depth = 1; depth < P.max_depth || (from_bg && depth == 1);) {
+            const bool first_bg = from_bg && depth == 1;
             const HitOut h = traverse<false>(S, R.root, R.abs_max, o, d, tmin, INFINITY, stack_node + threadIdx.x,
                                              stack_t + threadIdx.x);
             wave_count(stats, kStatRays, true);
             wave_count(stats, kStatHits, h.li >= 0);
             if (h.li < 0) break;
             const Surf s = surface(S, h.li, h.u, h.v);
-            const float pdf_from = pdf_dir * area_to_solid_angle(prev.pos, s.pos, s.ng);
+            const float pdf_from = first_bg ? pdf_dir : pdf_dir * area_to_solid_angle(prev.pos, s.pos, s.ng);
             if (pdf_from == 0.0f) break;
             const size_t at = (size_t)depth * P.num_paths + idx;
             P.lv_a[at] = f4_of(prefix, pdf_from);
             P.lv_b[at] = make_float4(h.u, h.v, next_rev, __int_as_float(h.li));
             len = depth + 1;
-            splat(s, prev, true, prefix, depth, ne);
-            if (depth + 1 >= P.max_depth) break;
-            const float q = depth > 4 ? clampf(average(approx), 0.05f, 0.95f) : 1.0f;
-            if (rng.next_float() > q) break;
+            if (first_bg) {  // NextEventPdf of the first edge: the map's pdf towards the origin
+                ne = B.next_event_pdf_background(E, s.pos, prev.pos);
+                P.lv_ne[idx] = ne;
+            }
+            uint32_t pix;
+            float jac;
+            if (P.mode == kModeClassic && P.light_tracer && depth + 1 >= P.min_depth &&
+                camera_response(R.cam, s.pos, s.ng, &pix, &jac)) {
+                const f3 p0 = offset_point(s, cam - s.pos);
+                const bool occ = traverse<true>(S, R.root, R.abs_max, p0, cam - p0, 1e-5f, 1.0f - 1e-5f,
+                                                stack_node + threadIdx.x, stack_t + threadIdx.x)
+                                     .li >= 0;
+                wave_count(stats, kStatShadow, true);
+                wave_count(stats, kStatShadowTraced, true);
+                wave_count(stats, kStatOccluded, occ);
+                if (!occ) {
+                    f3 to_cam = cam - s.pos;
+                    to_cam = to_cam / length(to_cam);
+                    const f3 to_anc = normalize(prev.pos - s.pos);
+                    Shader sh;
+                    make_shader(S, s, to_cam, &sh);  // the camera side: IsOnLightSubpath = false
+                    f3 f = material_value(sh, to_anc);
+                    if (!is_zero(f)) {
+                        f = f * sdiv(fabsf(dot(s.ns, to_anc)), fabsf(dot(s.ng, to_anc)));
+                        float pdf_rev, unused;
+                        pdf_pair(sh, to_anc, &pdf_rev, &unused);
+                        if (!first_bg) pdf_rev *= area_to_solid_angle(s.pos, prev.pos, prev.ng);
+                        // PdfEmit = invLensArea * jacobian, then the vertex's own reverse pdf
+                        const float mis = srcp(
+                            1.0f + sdiv(B.light_reciprocals(0, depth, 1.0f * jac, pdf_rev, ne), P.num_paths_f));
+                        const f3 w = prefix * f * jac / P.num_paths_f;
+                        const f3 v = mis * w;
+                        const bool bad = !isfinite(average(v));
+                        wave_count(stats, kStatNan, bad);
+                        if (!bad) {
+                            const f3 add = div_exact(v, P.n);
+                            float* px = image + 3ull * pix;
+                            atomicAdd(px, add.x);
+                            atomicAdd(px + 1, add.y);
+                            atomicAdd(px + 2, add.z);
+                        }
+                    }
+                }
+            }
+            float q = 1.0f;
+            if (!first_bg) {
+                if (depth + 1 >= P.max_depth) break;
+                q = depth > 4 ? clampf(average(approx), 0.05f, 0.95f) : 1.0f;
+                if (rng.next_float() > q) break;
+            }
             Shader sh;
             make_shader(S, s, -d, &sh);
             sh.light = true;
@@ -2071,10 +2054,17 @@ __global__ void __launch_bounds__(kTraceBlock) k_bidir_light(DevScene S, int sce
             const f2 pd = rng.next_float2();
             const BsdfSample bs = sample_bsdf<true>(sh, pc, pd);
             approx = approx * (bs.weight / q);
-            next_rev = bs.pdf_rev * area_to_solid_angle(s.pos, prev.pos, prev.ng);  // OnContinue
+            // OnContinue
+            next_rev = first_bg ? bs.pdf_rev : bs.pdf_rev * area_to_solid_angle(s.pos, prev.pos, prev.ng);
+            if (first_bg && P.max_depth <= 1) break;
             if (bs.pdf == 0.0f || is_zero(bs.weight)) break;
-            f3 w = bs.weight * sdiv(fabsf(dot(s.ng, bs.dir)), fabsf(dot(s.ns, bs.dir)));
-            w = w * sdiv(fabsf(dot(s.ns, -d)), fabsf(dot(s.ng, -d)));
+            f3 w = bs.weight;
+            if (!first_bg) {
+                // RandomWalk.cs:193-204: the geometric instead of the shading cosine, and the
+                // rendering equation's cosine ratio at this vertex of the light subpath
+                w = w * sdiv(fabsf(dot(s.ng, bs.dir)), fabsf(dot(s.ns, bs.dir)));
+                w = w * sdiv(fabsf(dot(s.ns, -d)), fabsf(dot(s.ng, -d)));
+            }
             prefix = prefix * (w / q);
             ++depth;
             pdf_dir = bs.pdf;
@@ -2095,16 +2085,17 @@ __global__ void __launch_bounds__(kTraceBlock) k_bidir_light(DevScene S, int sce
 // NumShadowRays next-event samples (PerformNextEventEstimation, :368-484), each
 // weighted by the balance heuristic (ClassicBidir.cs:119-153). The pixel value
 // is added to the running mean after the light tracer's splats.
-__global__ void __launch_bounds__(kTraceBlock) k_bidir_camera(DevScene S, int scene, BidirParams P, float* image,
+template <bool ENV>
+__global__ void __launch_bounds__(kTraceBlock, 2) k_bidir_camera(DevScene S, int scene, BidirParams P, float* image,
                                                               unsigned long long* stats) {
     __shared__ int stack_node[kStackNodeLds];
     __shared__ float stack_t[kStackLds * kTraceBlock];
     const SceneRec& R = S.recs[scene];
     const DevCamera& C = R.cam;
     const Env E = env_of(S, R);
-    const float bg = background_probability(R);  // ComputeNextEventBackgroundProbability
+    const float bg = ENV ? background_probability(R) : 0.0f;  // ComputeNextEventBackgroundProbability
     for (uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x; pix < P.num_paths; pix += gridDim.x * blockDim.x) {
-        BidirCtx B{S, R, P, pix};
+        BidirCtx<ENV> B{S, R, P, pix, bg};
         const uint32_t row = pix / (uint32_t)C.width, col = pix % (uint32_t)C.width;
         Rng rng{hash_seed(P.seed_camera, pix, P.iteration)};
         const f2 off = rng.next_float2();
@@ -2117,7 +2108,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_bidir_camera(DevScene S, int sc
         float pdf_dir = pixel_jacobian(C, o + raw);
         const int light_len = P.lv_len[pix];
         const float ne = P.lv_ne[pix];
-        const bool light_from_bg = light_len > 0 && from_background(P, pix);
+        const bool light_from_bg = light_len > 0 && from_background<ENV>(P, pix);
         f3 L = gray(0.0f), prefix = gray(1.0f), approx = gray(1.0f);
         for (int depth = 1; depth < P.max_depth + 1;) {
             const HitOut h = traverse<false>(S, R.root, R.abs_max, o, d, tmin, INFINITY, stack_node + threadIdx.x,
@@ -2128,7 +2119,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_bidir_camera(DevScene S, int sc
                 // CameraRandomWalk.OnInvalidHit (BidirBase.Camera.cs:604-612): the escaped
                 // direction is the last vertex, with its solid-angle pdf; OnBackgroundHit
                 // (:532-561) with ClassicBidir.EmitterHitMis, no MinDepth test
-                if (R.has_env) {
+                if (ENV) {
                     const int c = depth;
                     float mis = 1.0f;
                     if (c > 1) {
@@ -2216,7 +2207,8 @@ __global__ void __launch_bounds__(kTraceBlock) k_bidir_camera(DevScene S, int sc
             }
             if (depth < P.max_depth && depth + 1 >= P.min_depth) {
                 for (int r = 0; r < P.num_shadow_rays; ++r) {
-                    if (rng.next_float() < bg) {
+                    const float u_bg = rng.next_float();  // drawn with or without a background
+                    if (ENV && u_bg < bg) {
                         // the background branch (BidirBase.Camera.cs:381-418): its pdf carries
                         // no NumShadowRays factor; ClassicBidir.NextEventMis (ClassicBidir.cs:147-153)
                         EnvSample es = env_sample(E, rng.next_float2());
@@ -2581,6 +2573,7 @@ __device__ int nearest_photons(const BidirParams& P, f3 q, float radius, float r
 // the cache is numbered: the light tracer's weight counts the connections,
 // whose density needs the vertex count (LightTracerMis, VertexCacheBidir.cs:183-186,
 // VertexConnectionAndMerging.cs:735-766).
+template <bool ENV>
 __global__ void __launch_bounds__(kTraceBlock) k_vcm_splat(DevScene S, int scene, BidirParams P, float* image,
                                                            unsigned long long* stats) {
     __shared__ int stack_node[kStackNodeLds];
@@ -2593,8 +2586,8 @@ __global__ void __launch_bounds__(kTraceBlock) k_vcm_splat(DevScene S, int scene
         const int len = P.lv_len[idx];
         if (len < 2) continue;
         const float ne = P.lv_ne[idx];
-        const bool from_bg = from_background(P, idx);
-        Surf prev = light_vertex(S, P, 0, idx);
+        const bool from_bg = from_background<ENV>(P, idx);
+        Surf prev = light_vertex<ENV>(S, P, 0, idx);
         for (int depth = 1; depth < len; ++depth) {
             const size_t at = (size_t)depth * P.num_paths + idx;
             const float4 a = P.lv_a[at];
@@ -2665,7 +2658,8 @@ __global__ void __launch_bounds__(kTraceBlock) k_vcm_splat(DevScene S, int scene
 // merging with the nearest photons (VCM) and NumShadowRays next-event samples,
 // each weighted against all enabled techniques (VertexCacheBidir.cs:171-237,
 // VertexConnectionAndMerging.cs:648-929).
-__global__ void __launch_bounds__(kTraceBlock) k_vcm_camera(DevScene S, int scene, BidirParams P, float* image,
+template <bool ENV>
+__global__ void __launch_bounds__(kTraceBlock, 2) k_vcm_camera(DevScene S, int scene, BidirParams P, float* image,
                                                             unsigned long long* stats) {
     __shared__ int stack_node[kStackNodeLds];
     __shared__ float stack_t[kStackLds * kTraceBlock];
@@ -2675,7 +2669,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_vcm_camera(DevScene S, int scen
     const int num_vertices = P.num_paths > 0 ? P.cum[P.num_paths - 1] : 0;
     const float density = select_density(P, num_vertices);
     const Env E = env_of(S, R);
-    const float bg = background_probability(R);
+    const float bg = ENV ? background_probability(R) : 0.0f;
     const float pmf_light = (1.0f - bg) / (float)R.num_emitters;  // unused without emitters
     for (uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x; pix < P.num_pixels; pix += gridDim.x * blockDim.x) {
         const uint32_t row = pix / (uint32_t)C.width, col = pix % (uint32_t)C.width;
@@ -2698,7 +2692,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_vcm_camera(DevScene S, int scen
                 // OnInvalidHit / OnBackgroundHit (BidirBase.Camera.cs:532-561,604-612) with
                 // EmitterHitMis (VertexCacheBidir.cs:171-180, VertexConnectionAndMerging.cs:700-732),
                 // whatever EnableHitting and MinDepth say
-                if (R.has_env) {
+                if (ENV) {
                     const int c = depth;
                     P.cam[(size_t)(c - 1) * P.num_pixels + pix] = make_float2(pdf_dir, 0.0f);
                     float mis = 1.0f;
@@ -2762,8 +2756,8 @@ __global__ void __launch_bounds__(kTraceBlock) k_vcm_camera(DevScene S, int scen
                     const float4 la = P.lv_a[(size_t)k * P.num_paths + lp];
                     const float4 lb = P.lv_b[(size_t)k * P.num_paths + lp];
                     const Surf lv = surface(S, __float_as_int(lb.w), lb.x, lb.y);
-                    const Surf anc = light_vertex(S, P, k - 1, lp);
-                    const bool lp_bg = from_background(P, lp);
+                    const Surf anc = light_vertex<ENV>(S, P, k - 1, lp);
+                    const bool lp_bg = from_background<ENV>(P, lp);
                     const f3 dd = s.pos - lv.pos;
                     const f3 p0 = offset_point(lv, dd), p1 = offset_point(s, -dd);
                     const bool occ = traverse<true>(S, R.root, R.abs_max, p0, p1 - p0, 1e-5f, 1.0f - 1e-5f,
@@ -2826,8 +2820,8 @@ __global__ void __launch_bounds__(kTraceBlock) k_vcm_camera(DevScene S, int scen
                         const float4 lb = P.lv_b[(size_t)k * P.num_paths + lp];
                         const Surf ph = surface(S, __float_as_int(lb.w), lb.x, lb.y);
                         if (fabsf(dot(s.ng, ph.ng)) < 0.4f) continue;
-                        const Surf anc = light_vertex(S, P, k - 1, lp);
-                        const bool lp_bg = from_background(P, lp);
+                        const Surf anc = light_vertex<ENV>(S, P, k - 1, lp);
+                        const bool lp_bg = from_background<ENV>(P, lp);
                         const bool first_from_bg = k == 1 && lp_bg;
                         const f3 to_anc = normalize(anc.pos - s.pos);
                         f3 bsdf = material_value(sh, to_anc);
@@ -2867,7 +2861,8 @@ __global__ void __launch_bounds__(kTraceBlock) k_vcm_camera(DevScene S, int scen
             }
             if (depth < P.max_depth && depth + 1 >= P.min_depth) {
                 for (int r = 0; r < P.num_shadow_rays; ++r) {
-                    if (rng.next_float() < bg) {
+                    const float u_bg = rng.next_float();  // drawn with or without a background
+                    if (ENV && u_bg < bg) {
                         // the background branch (BidirBase.Camera.cs:381-418), NextEventMis with
                         // isBackground (VertexCacheBidir.cs:200-207, VertexConnectionAndMerging.cs:802-836)
                         EnvSample es = env_sample(E, rng.next_float2());
@@ -3443,14 +3438,14 @@ void launch_light_paths(hipStream_t s, const DevScene& sc, int scene, const Ligh
 
 void launch_bidir_light(hipStream_t s, const DevScene& sc, int scene, const BidirParams& p, float* image,
                         unsigned long long* stats) {
-    hipLaunchKernelGGL(k_bidir_light, dim3(blocks(p.num_paths, kTraceBlock)), dim3(kTraceBlock), 0, s, sc, scene, p,
-                       image, stats);
+    hipLaunchKernelGGL(p.env ? k_bidir_light<true> : k_bidir_light<false>, dim3(blocks(p.num_paths, kTraceBlock)),
+                       dim3(kTraceBlock), 0, s, sc, scene, p, image, stats);
 }
 
 void launch_bidir_camera(hipStream_t s, const DevScene& sc, int scene, const BidirParams& p, float* image,
                          unsigned long long* stats) {
-    hipLaunchKernelGGL(k_bidir_camera, dim3(blocks(p.num_paths, kTraceBlock)), dim3(kTraceBlock), 0, s, sc, scene, p,
-                       image, stats);
+    hipLaunchKernelGGL(p.env ? k_bidir_camera<true> : k_bidir_camera<false>, dim3(blocks(p.num_paths, kTraceBlock)),
+                       dim3(kTraceBlock), 0, s, sc, scene, p, image, stats);
 }
 
 void launch_vcm_prepare(hipStream_t s, const DevScene& sc, int scene, const BidirParams& p, int* scan_tmp) {
@@ -3479,10 +3474,10 @@ void launch_vcm_prepare(hipStream_t s, const DevScene& sc, int scene, const Bidi
 void launch_vcm_camera(hipStream_t s, const DevScene& sc, int scene, const BidirParams& p, float* image,
                        unsigned long long* stats) {
     if (p.light_tracer && p.num_paths > 0)
-        hipLaunchKernelGGL(k_vcm_splat, dim3(blocks(p.num_paths, kTraceBlock)), dim3(kTraceBlock), 0, s, sc, scene, p,
-                           image, stats);
-    hipLaunchKernelGGL(k_vcm_camera, dim3(blocks(p.num_pixels, kTraceBlock)), dim3(kTraceBlock), 0, s, sc, scene, p,
-                       image, stats);
+        hipLaunchKernelGGL(p.env ? k_vcm_splat<true> : k_vcm_splat<false>, dim3(blocks(p.num_paths, kTraceBlock)),
+                           dim3(kTraceBlock), 0, s, sc, scene, p, image, stats);
+    hipLaunchKernelGGL(p.env ? k_vcm_camera<true> : k_vcm_camera<false>, dim3(blocks(p.num_pixels, kTraceBlock)),
+                       dim3(kTraceBlock), 0, s, sc, scene, p, image, stats);
 }
 
 void launch_debug_visualize(hipStream_t s, const DevScene& sc, int scene, uint32_t num_pixels, uint32_t base_seed,
