@@ -2542,30 +2542,62 @@ __device__ __forceinline__ void consider_photon(float4 ph, f3 q, float r2, int k
     d2s[at] = d2;
     ids[at] = id;
 }
+// The query visits the cells of its box in shells around q's own cell, nearest
+// first: shell s holds the cells whose index differs from q's by s on some
+// axis. A photon of shell s lies more than (s - 3) * h from q (one cell for q's
+// place in its cell, one for each rounded cell index), so once k photons are
+// found closer than that no later shell can change the result, ties included;
+// the search stops there instead of visiting the whole box (a grazing primary
+// hit has a footprint of many cells). A box still unfinished after
+// kMaxShellCells cells, or wider than 4096 cells, scans every photon instead.
+constexpr int kMaxShellCells = 32768;
 __device__ int nearest_photons(const BidirParams& P, f3 q, float radius, float r2, int kmax, float* d2s, int* ids) {
     int found = 0;
     const float h = P.cell;
-    const float span = 2.0f * radius / h + 2.0f;  // cells per axis the query box can touch
-    if (!(span * span * span <= (float)(P.table_mask + 1))) {
-        // a query box of more cells than buckets (or a non-finite radius): every bucket once
+    auto every_photon = [&]() {
+        found = 0;
         const uint32_t end = P.bucket_start[P.table_mask + 1];
         for (uint32_t e = 0; e < end; ++e) consider_photon(P.photons[e], q, r2, kmax, d2s, ids, &found);
         return found;
-    }
+    };
+    const float span = 2.0f * radius / h + 2.0f;  // cells per axis the query box can touch
+    if (!(span <= 4096.0f)) return every_photon();  // also a non-finite radius
     const int3 lo = grid_cell(mk3(q.x - radius, q.y - radius, q.z - radius), h);
     const int3 hi = grid_cell(mk3(q.x + radius, q.y + radius, q.z + radius), h);
-    for (int x = lo.x; x <= hi.x; ++x)
-        for (int y = lo.y; y <= hi.y; ++y)
-            for (int z = lo.z; z <= hi.z; ++z) {
-                const uint32_t b = cell_bucket(make_int3(x, y, z), P.table_mask);
-                const uint32_t end = P.bucket_start[b + 1];
-                for (uint32_t e = P.bucket_start[b]; e < end; ++e) {
-                    const float4 ph = P.photons[e];
-                    const int3 pc = grid_cell(f3_of(ph), h);
-                    if (pc.x != x || pc.y != y || pc.z != z) continue;  // another cell in the same bucket
-                    consider_photon(ph, q, r2, kmax, d2s, ids, &found);
+    const int3 c0 = grid_cell(q, h);
+    const int smax = max(max(max(c0.x - lo.x, hi.x - c0.x), max(c0.y - lo.y, hi.y - c0.y)),
+                         max(c0.z - lo.z, hi.z - c0.z));
+    auto visit = [&](int x, int y, int z) {
+        const uint32_t b = cell_bucket(make_int3(x, y, z), P.table_mask);
+        const uint32_t end = P.bucket_start[b + 1];
+        for (uint32_t e = P.bucket_start[b]; e < end; ++e) {
+            const float4 ph = P.photons[e];
+            const int3 pc = grid_cell(f3_of(ph), h);
+            if (pc.x != x || pc.y != y || pc.z != z) continue;  // another cell in the same bucket
+            consider_photon(ph, q, r2, kmax, d2s, ids, &found);
+        }
+    };
+    int visited = 0;
+    for (int sh = 0; sh <= smax; ++sh) {
+        if (found == kmax && sh >= 4) {
+            const float bound = (float)(sh - 3) * h;
+            if (d2s[kmax - 1] < bound * bound * 0.99999f) break;
+        }
+        if (visited > kMaxShellCells) return every_photon();
+        const int x0 = max(lo.x, c0.x - sh), x1 = min(hi.x, c0.x + sh);
+        const int y0 = max(lo.y, c0.y - sh), y1 = min(hi.y, c0.y + sh);
+        const int z0 = max(lo.z, c0.z - sh), z1 = min(hi.z, c0.z + sh);
+        for (int x = x0; x <= x1; ++x)
+            for (int y = y0; y <= y1; ++y) {
+                if (abs(x - c0.x) == sh || abs(y - c0.y) == sh) {  // a face of the shell: its whole z range
+                    for (int z = z0; z <= z1; ++z) visit(x, y, z);
+                    visited += z1 - z0 + 1;
+                } else {  // inside in x and y: the shell's two z caps
+                    if (c0.z - sh >= lo.z) visit(x, y, c0.z - sh), ++visited;
+                    if (c0.z + sh <= hi.z) visit(x, y, c0.z + sh), ++visited;
                 }
             }
+    }
     return found;
 }
 
