@@ -476,8 +476,9 @@ def bidir_main(args):
     sky (its area lights only), 1024^2, MaxDepth 8: ClassicBidir with every
     technique on (one light path and one camera path per pixel and iteration,
     NumShadowRays 1), `spp` iterations per step; the light tracer alone
-    (VertexCacheBidir as a pure light tracer) beside it. Kernel times from HIP
-    events; the oracle's ClassicBidir timed on one iteration of the same scene."""
+    (VertexCacheBidir as a pure light tracer), VertexCacheBidir and VCM (one
+    connection, 8 photons per merge) beside it. Kernel times from HIP events; the
+    oracle's ClassicBidir timed on one iteration of the same scene."""
     from seesharp_amd import Device, GpuScene, abi, scenes
 
     dev = Device(0)
@@ -505,6 +506,23 @@ def bidir_main(args):
     g.clear()
     lst = g.render_light_tracer(lt, 0, iters)
     lt_ms = dev.kernel_timing("light_paths")
+    # VertexCacheBidir and VCM (one connection, one shadow ray, 8 photons per merge) on the same scene
+    npx = args.res * args.res
+    vc = {}
+    for name, algo in (("vertex_cache_bidir", abi.VERTEX_CACHE_BIDIR), ("vcm", abi.VCM)):
+        vcfg = abi.VcmSettings(algo, 0xC030114, 0x13C0FEFE, args.max_depth, 1, -1, 1, 1, 1, 1, 1, 0, 8, 0)
+        g.clear()
+        g.render_vcm(vcfg, 0, 1)
+        dev.reset_kernel_timing()
+        g.clear()
+        t0 = time.perf_counter()
+        vst = g.render_vcm(vcfg, 0, iters)
+        v_ms = (time.perf_counter() - t0) * 1000.0 / iters
+        vc[name] = {"ms_per_iteration": round(v_ms, 3), "msamples_per_s": round(npx / v_ms / 1e3, 2),
+                    "kernel_ms_per_iteration": {k: round(v[1] / max(1, v[0]), 3) for k, v in
+                                                ((k, dev.kernel_timing(k)) for k in
+                                                 ("vcm_light", "vcm_prepare", "vcm_camera"))},
+                    "ray_stats_per_step": vst}
     dev.set_kernel_timing(False)
     npx = args.res * args.res
     step_ms = float(np.mean(ms))
@@ -535,6 +553,7 @@ def bidir_main(args):
         "light_tracer": {"ms_per_iteration": round(lt_ms[1] / max(1, lt_ms[0]), 3),
                          "mpaths_per_s": round(npx * lt_ms[0] / lt_ms[1] / 1e3, 2) if lt_ms[1] else None,
                          "ray_stats_per_step": lst},
+        **vc,
     }
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))

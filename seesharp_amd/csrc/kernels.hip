@@ -1636,7 +1636,7 @@ __device__ __forceinline__ f3 material_value(const Shader& sh, f3 in_world) {
 // connection draws no random numbers, so this equals splatting the cached
 // vertices afterwards. Splats are added to the running-mean image with fp32
 // atomics (FrameBuffer.Splat: value / CurIteration; non-finite ones counted).
-__global__ void __launch_bounds__(kTraceBlock) k_light_paths(DevScene S, int scene, LightParams P, float* image,
+__global__ void __launch_bounds__(kTraceBlock, 4) k_light_paths(DevScene S, int scene, LightParams P, float* image,
                                                              unsigned long long* stats) {
     __shared__ int stack_node[kStackNodeLds];
     __shared__ float stack_t[kStackLds * kTraceBlock];
@@ -1920,7 +1920,7 @@ struct BidirCtx {
 // ClassicBidir.LightTracerMis, ClassicBidir.cs:131-134): the connection draws
 // no random numbers and reads only the vertices stored before it.
 template <bool ENV>
-__global__ void __launch_bounds__(kTraceBlock, 3) k_bidir_light(DevScene S, int scene, BidirParams P, float* image,
+__global__ void __launch_bounds__(kTraceBlock, 4) k_bidir_light(DevScene S, int scene, BidirParams P, float* image,
                                                              unsigned long long* stats) {
     __shared__ int stack_node[kStackNodeLds];
     __shared__ float stack_t[kStackLds * kTraceBlock];
@@ -2086,7 +2086,7 @@ __global__ void __launch_bounds__(kTraceBlock, 3) k_bidir_light(DevScene S, int 
 // weighted by the balance heuristic (ClassicBidir.cs:119-153). The pixel value
 // is added to the running mean after the light tracer's splats.
 template <bool ENV>
-__global__ void __launch_bounds__(kTraceBlock, 2) k_bidir_camera(DevScene S, int scene, BidirParams P, float* image,
+__global__ void __launch_bounds__(kTraceBlock, 4) k_bidir_camera(DevScene S, int scene, BidirParams P, float* image,
                                                               unsigned long long* stats) {
     __shared__ int stack_node[kStackNodeLds];
     __shared__ float stack_t[kStackLds * kTraceBlock];
@@ -2354,27 +2354,52 @@ struct PdfView {
     }
 };
 
-// VertexConnectionAndMerging.CorrelAwareRatios (VertexConnectionAndMerging.cs:580-646),
-// evaluated per index from the same prefix / suffix products.
+// VertexConnectionAndMerging.CorrelAwareRatios (VertexConnectionAndMerging.cs:580-646):
+// the camera-side prefix products and light-side suffix products of one full
+// path. Computed on the first lookup (the weights without correlation-aware
+// merging never read them) into per-lane LDS (`lds`, stride kTraceBlock) for
+// paths of up to kCorrelMax surface vertices, in the reference's order, so each
+// lookup is two LDS reads; longer paths recompute the products per lookup.
+constexpr int kCorrelMax = 16;
 struct Correl {
     const PdfView* v;
     float accept_area;
     bool from_bg;  // the light end is the background: its last density stays per solid angle
-    __device__ __forceinline__ Correl(const PdfView& view, float dist_to_cam, bool from_background)
-        : v(&view), from_bg(from_background) {
+    float* lds;    // cam[i] at lds[2 i * kTraceBlock], light[i] at lds[(2 i + 1) * kTraceBlock], or null
+    mutable bool ready = false;
+    __device__ __forceinline__ Correl(const PdfView& view, float dist_to_cam, bool from_background, float* lds_lane)
+        : v(&view), from_bg(from_background), lds(lds_lane) {
         const float radius = dist_to_cam * 0.0174550649f;
         accept_area = radius * radius * kPi;
+    }
+    __device__ __forceinline__ float light_factor(int i, int nsv) const {
+        float next = v->l2c(i) * accept_area;
+        if (i == nsv - 1 && !from_bg) next *= accept_area;
+        return fminf(next, 1.0f);
     }
     __device__ float operator[](int idx) const {
         if (idx == 0) return 1.0f;
         const int nsv = v->n - 1;
-        float cam = 1.0f;
-        for (int i = 0; i <= idx; ++i) cam *= fminf(v->c2l(i) * accept_area, 1.0f);
-        float light = 1.0f;
-        for (int i = nsv - 1; i >= idx; --i) {
-            float next = v->l2c(i) * accept_area;
-            if (i == nsv - 1 && !from_bg) next *= accept_area;
-            light *= fminf(next, 1.0f);
+        float cam = 1.0f, light = 1.0f;
+        if (lds && nsv <= kCorrelMax) {
+            if (!ready) {
+                float product = 1.0f;
+                for (int i = 0; i < nsv; ++i) {
+                    product *= fminf(v->c2l(i) * accept_area, 1.0f);
+                    lds[2 * i * kTraceBlock] = product;
+                }
+                product = 1.0f;
+                for (int i = nsv - 1; i >= 0; --i) {
+                    product *= light_factor(i, nsv);
+                    lds[(2 * i + 1) * kTraceBlock] = product;
+                }
+                ready = true;
+            }
+            cam = lds[2 * idx * kTraceBlock];
+            light = lds[(2 * idx + 1) * kTraceBlock];
+        } else {
+            for (int i = 0; i <= idx; ++i) cam *= fminf(v->c2l(i) * accept_area, 1.0f);
+            for (int i = nsv - 1; i >= idx; --i) light *= light_factor(i, nsv);
         }
         if (cam == 0.0f && light == 0.0f) return 1.0f;
         return sdiv(cam, cam + light - cam * light);
@@ -2610,6 +2635,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_vcm_splat(DevScene S, int scene
                                                            unsigned long long* stats) {
     __shared__ int stack_node[kStackNodeLds];
     __shared__ float stack_t[kStackLds * kTraceBlock];
+    __shared__ float correl_lds[2 * kCorrelMax * kTraceBlock];
     const SceneRec& R = S.recs[scene];
     const f3 cam = mk3(R.cam.pos[0], R.cam.pos[1], R.cam.pos[2]);
     const int num_vertices = P.num_paths > 0 ? P.cum[P.num_paths - 1] : 0;
@@ -2653,14 +2679,14 @@ __global__ void __launch_bounds__(kTraceBlock) k_vcm_splat(DevScene S, int scene
                         v.set_c2l(1, pdf_rev);
                         float mis;
                         if (P.mode == kModeVcm) {
-                            const Correl cr(v, dist, from_bg);
+                            const Correl cr(v, dist, from_bg, correl_lds + threadIdx.x);
                             const float footprint = sqrtf(1.0f / v.c2l(0));
                             float sum = vcm_light_reciprocals(P, v, -1, footprint, cr, density);
                             sum = sdiv(sum, P.num_paths_f);
                             sum += 1.0f;
                             mis = srcp(sum);
                         } else {
-                            const Correl cr(v, dist, from_bg);
+                            const Correl cr(v, dist, from_bg, correl_lds + threadIdx.x);
                             mis = srcp(1.0f + sdiv(vcm_light_reciprocals(P, v, -1, 0.0f, cr, density),
                                                    P.num_paths_f));
                         }
@@ -2691,10 +2717,11 @@ __global__ void __launch_bounds__(kTraceBlock) k_vcm_splat(DevScene S, int scene
 // each weighted against all enabled techniques (VertexCacheBidir.cs:171-237,
 // VertexConnectionAndMerging.cs:648-929).
 template <bool ENV>
-__global__ void __launch_bounds__(kTraceBlock, 2) k_vcm_camera(DevScene S, int scene, BidirParams P, float* image,
+__global__ void __launch_bounds__(kTraceBlock, 3) k_vcm_camera(DevScene S, int scene, BidirParams P, float* image,
                                                             unsigned long long* stats) {
     __shared__ int stack_node[kStackNodeLds];
     __shared__ float stack_t[kStackLds * kTraceBlock];
+    __shared__ float correl_lds[2 * kCorrelMax * kTraceBlock];
     const SceneRec& R = S.recs[scene];
     const DevCamera& C = R.cam;
     const bool vcm = P.mode == kModeVcm;
@@ -2731,7 +2758,7 @@ __global__ void __launch_bounds__(kTraceBlock, 2) k_vcm_camera(DevScene S, int s
                     if (c > 1) {
                         PdfView v{&P, pix, 0u, c, c, env_pdf(E, d) * bg};
                         v.set_l2c(c - 2, background_pdf(E, R, -d));
-                        const Correl cr(v, dist0, true);
+                        const Correl cr(v, dist0, true, correl_lds + threadIdx.x);
                         float sum = 1.0f;
                         sum += sdiv(v.ne, pdf_dir);
                         sum += sdiv(vcm_camera_reciprocals(P, v, c - 2, footprint, cr, density), pdf_dir);
@@ -2765,7 +2792,7 @@ __global__ void __launch_bounds__(kTraceBlock, 2) k_vcm_camera(DevScene S, int s
                     PdfView v{&P, pix, 0u, c, c, e.inv_area * (1.0f / (float)R.num_emitters) * (1.0f - bg) *
                                                      (float)P.num_shadow_rays};
                     v.set_l2c(c - 2, emitter_pdf_ray(e, s.ns, -d) * rev_jac * pmf_light);
-                    const Correl cr(v, dist0, false);
+                    const Correl cr(v, dist0, false, correl_lds + threadIdx.x);
                     const float pdf_this = pdf_from;
                     float sum = 1.0f;
                     sum += sdiv(v.ne, pdf_this);
@@ -2823,7 +2850,7 @@ __global__ void __launch_bounds__(kTraceBlock, 2) k_vcm_camera(DevScene S, int s
                     v.set_l2c(c - 1, plc);
                     v.set_c2l(c, pcl);
                     v.set_c2l(c + 1, plr);
-                    const Correl cr(v, dist0, lp_bg);
+                    const Correl cr(v, dist0, lp_bg, correl_lds + threadIdx.x);
                     float sum = 1.0f;
                     sum += sdiv(vcm_camera_reciprocals(P, v, c - 1, footprint, cr, density), density);
                     sum += sdiv(vcm_light_reciprocals(P, v, c - 1, footprint, cr, density), density);
@@ -2873,7 +2900,7 @@ __global__ void __launch_bounds__(kTraceBlock, 2) k_vcm_camera(DevScene S, int s
                         PdfView v{&P, pix, lp, c + k, c, ne_m};
                         if (c > 1) v.set_l2c(c - 2, pdf_cam_rev);
                         v.set_c2l(c, pdf_light_rev);
-                        const Correl cr(v, dist0, lp_bg);
+                        const Correl cr(v, dist0, lp_bg, correl_lds + threadIdx.x);
                         // MergeMis (:648-697)
                         float approx_m = v.l2c(c - 1) * kPi * footprint * footprint * P.num_paths_f;
                         if (P.correl) approx_m *= cr[c - 1];
@@ -2918,7 +2945,7 @@ __global__ void __launch_bounds__(kTraceBlock, 2) k_vcm_camera(DevScene S, int s
                         if (c > 1) v.set_l2c(c - 2, rev);
                         v.set_l2c(c - 1, background_pdf(E, R, -es.dir));
                         v.set_c2l(c, fwd);
-                        const Correl cr(v, dist0, true);
+                        const Correl cr(v, dist0, true, correl_lds + threadIdx.x);
                         float sum = 1.0f;
                         if (P.hitting) sum += sdiv(fwd, es.pdf);
                         sum += sdiv(vcm_camera_reciprocals(P, v, c - 1, footprint, cr, density), es.pdf);
@@ -2960,7 +2987,7 @@ __global__ void __launch_bounds__(kTraceBlock, 2) k_vcm_camera(DevScene S, int s
                     v.set_l2c(c - 1, emitter_pdf_ray(e, lp.ns, l2s) * area_to_solid_angle(lp.pos, s.pos, s.ng) *
                                          pmf_light);
                     v.set_c2l(c, fwd);
-                    const Correl cr(v, dist0, false);
+                    const Correl cr(v, dist0, false, correl_lds + threadIdx.x);
                     float sum = 1.0f;
                     if (P.hitting) sum += sdiv(fwd, lpdf);
                     sum += sdiv(vcm_camera_reciprocals(P, v, c - 1, footprint, cr, density), lpdf);
