@@ -78,6 +78,43 @@ def test_vcm_settings_match_oracle(device, kw):
     compare(GpuScene(device, s), ol.OracleScene(s), ol.vcm_settings(**kw), 2, 1e-7, str(kw))
 
 
+def grazing_floor(width=64, height=64):
+    """A 40 x 40 diffuse floor seen from 5 cm above it towards the horizon, a glossy
+    box on it, lit by an area light: the primary hits near the horizon have pixel
+    footprints of a few up to millions of photon-grid cells, so the photon query
+    takes its shell search, its early stop and its scan of every photon."""
+    from seesharp_amd.scene import (DiffuseEmitter, DiffuseMaterial, FrameBuffer, GenericMaterial, Mesh,
+                                    PerspectiveCamera, Scene, create_look_at)
+
+    s = Scene()
+
+    def add(v, idx, mat):
+        m = Mesh(v, idx)
+        m.Material = mat
+        s.Meshes.append(m)
+        return m
+
+    add(*scenes.quad((-20, 0, -20), (-20, 0, 20), (20, 0, 20), (20, 0, -20)),
+        DiffuseMaterial(DiffuseMaterial.Parameters(BaseColor=(0.7, 0.7, 0.7))))
+    add(*scenes.box_mesh((0.0, 0.25, -3.0), (0.5, 0.5, 0.5), 30.0),
+        GenericMaterial(GenericMaterial.Parameters(BaseColor=(0.8, 0.5, 0.3), Roughness=0.3, Metallic=0.5,
+                                                   SpecularTintStrength=1.0, Anisotropic=0.0,
+                                                   SpecularTransmittance=0.0, IndexOfRefraction=1.5)))
+    light = add(*scenes.light_quad(0.0, 3.0, -2.0, 1.5), DiffuseMaterial(DiffuseMaterial.Parameters(
+        BaseColor=(0.0, 0.0, 0.0))))
+    s.Emitters.extend(DiffuseEmitter.MakeFromMesh(light, (20.0, 20.0, 20.0)))
+    s.Camera = PerspectiveCamera(create_look_at((0.0, 0.05, 4.0), (0.0, 0.05, -10.0), (0, 1, 0)), 50.0)
+    s.FrameBuffer = FrameBuffer(width, height)
+    return s
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(merge_primary=True), dict(merge_primary=True, max_num_photons=32)],
+                         ids=["default", "merge_primary", "merge_primary-32_photons"])
+def test_vcm_grazing_footprints_match_oracle(device, kw):
+    s = grazing_floor()
+    compare(GpuScene(device, s), ol.OracleScene(s), ol.vcm_settings(max_depth=5, **kw), 2, 1e-7, str(kw))
+
+
 def test_vertex_cache_bidir_reduces_to_light_tracer(device):
     s = spheres_box(48, 48)
     g = GpuScene(device, s)
