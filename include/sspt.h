@@ -461,10 +461,13 @@ sspt_status sspt_render_debug(sspt_ctx* ctx, sspt_scene* scene, uint32_t index, 
  * the image by (n - 1) / n, then adds every finite splat / n in an
  * unspecified order (fp32 atomics); non-finite splats are counted
  * (sspt_framebuffer_nan_count). stats count the light paths' closest-hit
- * traces and their camera connections' occlusion tests. Scenes with a
- * background (whose hits the camera paths of this configuration would add:
- * render it with sspt_render_vcm, SSPT_VERTEX_CACHE_BIDIR) or without
- * emitters, and batches, are SSPT_ERR_UNSUPPORTED. */
+ * traces and their camera connections' occlusion tests. With a background
+ * (EnvironmentMap) this configuration also traces camera paths, which add the
+ * background's radiance under EmitterHitMis, and light paths start from it as
+ * in sspt_render_bidir: the call is then sspt_render_vcm with
+ * SSPT_VERTEX_CACHE_BIDIR, no connections, next events or emitter hits, and
+ * BidirBase's default base_seed_camera 0xC030114. Scenes without emitters or
+ * background, and batches, are SSPT_ERR_UNSUPPORTED. */
 typedef struct sspt_lt_settings {
     uint32_t base_seed_light;  /* BaseSeedLight, default 0x13C0FEFE */
     int32_t num_light_paths;   /* NumLightPaths, < 0 (default) = one per pixel */

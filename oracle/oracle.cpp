@@ -2225,7 +2225,22 @@ void light_path(const LightCtx& lc, uint32_t idx, uint32_t iteration, std::vecto
 
 uint64_t or_render_light_tracer(const or_scene* s, const sspt_lt_settings* cfg, uint32_t first, uint32_t num,
                                 float* image, sspt_ray_stats* stats, int32_t threads) {
-    if (s->has_env || s->emitters.empty()) return UINT64_MAX;
+    if (s->has_env) {
+        // VertexCacheBidir with no connections, next events or emitter hits: with a
+        // background its camera paths (BidirBase's default BaseSeedCamera) hit it under
+        // EmitterHitMis (BidirBase.Camera.cs:532-561)
+        sspt_vcm_settings v{};
+        v.algorithm = SSPT_VERTEX_CACHE_BIDIR;
+        v.base_seed_camera = 0xC030114u;
+        v.base_seed_light = cfg->base_seed_light;
+        v.max_depth = cfg->max_depth;
+        v.min_depth = cfg->min_depth;
+        v.num_light_paths = cfg->num_light_paths;
+        v.enable_light_tracer = 1;
+        v.max_num_photons = 8;
+        return or_render_vcm(s, &v, 1.0f, first, num, image, stats, threads);
+    }
+    if (s->emitters.empty()) return UINT64_MAX;
     const int64_t W = s->width, H = s->height;
     const uint32_t paths = cfg->num_light_paths < 0 ? (uint32_t)(W * H) : (uint32_t)cfg->num_light_paths;
     std::atomic<uint64_t> rays{0}, shadow{0}, occl{0}, hits{0};

@@ -1755,8 +1755,26 @@ sspt_status sspt_render_light_tracer(sspt_ctx* ctx, sspt_scene* sc, const sspt_l
     if (sc->recs.size() != 1)
         return fail(ctx, SSPT_ERR_UNSUPPORTED, "the light tracer renders a single scene, not a batch or view set");
     const SceneRec& R = sc->recs[0];
-    if (R.has_env)
-        return fail(ctx, SSPT_ERR_UNSUPPORTED, "the light tracer does not sample the background (scene has one)");
+    if (R.has_env) {
+        // With a background this configuration of VertexCacheBidir also traces camera
+        // paths, which hit the background under EmitterHitMis (BidirBase.Camera.cs:532-561):
+        // it is sspt_render_vcm's VertexCacheBidir with no connections, next events or
+        // emitter hits, its camera paths seeded with BidirBase's default BaseSeedCamera.
+        sspt_vcm_settings v{};
+        v.algorithm = SSPT_VERTEX_CACHE_BIDIR;
+        v.base_seed_camera = 0xC030114u;
+        v.base_seed_light = cfg->base_seed_light;
+        v.max_depth = cfg->max_depth;
+        v.min_depth = cfg->min_depth;
+        v.num_light_paths = cfg->num_light_paths;
+        v.num_connections = 0;
+        v.num_shadow_rays = 0;
+        v.enable_light_tracer = 1;
+        v.enable_hitting = 0;
+        v.enable_merging = 0;
+        v.max_num_photons = 8;
+        return sspt_render_vcm(ctx, sc, &v, first_iteration, num_iterations, stats);
+    }
     if (R.num_emitters == 0) return fail(ctx, SSPT_ERR_UNSUPPORTED, "the light tracer needs emitters");
     if (cfg->max_depth < 0 || cfg->min_depth < 0)
         return fail(ctx, SSPT_ERR_INVALID_ARGUMENT, "max_depth and min_depth must be non-negative");

@@ -626,7 +626,8 @@ class VertexCacheBidirLightTracer(Integrator):
     EnableLightTracer = true, rendered by BidirBase.Render
     (SeeSharp/Integrators/Bidir/BidirBase.cs:84-146). Every iteration traces
     NumLightPaths paths from the emitters and splats each vertex onto the camera
-    (BidirBase.Light.cs:35-277). Scenes with a background are not supported."""
+    (BidirBase.Light.cs:35-277). With a background, light paths start from it too and
+    camera paths add its radiance where they leave the scene (BidirBase.OnBackgroundHit)."""
 
     def __init__(self, device: Device | None = None):
         super().__init__()

@@ -167,7 +167,7 @@ class OracleScene:
         stats = abi.RayStats()
         nans = lib().or_render_light_tracer(self.ptr, C.byref(cfg), first, num, fp(image), C.byref(stats), threads)
         if nans == 2 ** 64 - 1:
-            raise ValueError("the light tracer needs emitters and no background")
+            raise ValueError("the light tracer needs emitters or a background")
         return image, stats, int(nans)
 
     def render_bidir(self, cfg, first=0, num=1, image=None, threads=0):

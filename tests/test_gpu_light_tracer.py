@@ -18,7 +18,6 @@ import pytest
 
 import oracle_lib as ol
 from seesharp_amd import GpuBatch, GpuScene, VertexCacheBidirLightTracer, abi, scenes
-from seesharp_amd.scene import EnvironmentMap
 from test_oracle_light_tracer import spheres_box, tilted_floor_box
 
 pytestmark = pytest.mark.gpu
@@ -131,11 +130,29 @@ def test_light_tracer_integrator(device):
     assert relmse(fb.Image, img_o) <= 1e-9
 
 
+def test_light_tracer_with_background_matches_oracle(device):
+    # light paths from the sky and the camera paths' background hits (VertexCacheBidir)
+    from test_oracle_bidir_background import open_box
+
+    s = open_box(48, 48, spheres=True)
+    g = GpuScene(device, s)
+    o = ol.OracleScene(s)
+    for cfg in (ol.lt_settings(max_depth=6), ol.lt_settings(max_depth=1), ol.lt_settings(max_depth=8, min_depth=3)):
+        g.clear()
+        sg = g.render_light_tracer(cfg, 0, 2)
+        img_o, so, _ = o.render_light_tracer(cfg, 0, 2)
+        img_g = g.read_image()
+        err = relmse(img_g, img_o)
+        div = float(np.mean(np.any(~np.isclose(img_g, img_o, rtol=1e-3, atol=1e-5), axis=-1)))
+        print(f"depth {cfg.max_depth} min {cfg.min_depth}: relMSE {err:.2e} diverted {div:.2e} "
+              f"rays {sg['num_rays']} / {so.num_rays}")
+        # observed 8.9e-6: a pixel or two whose path a last-ulp difference diverts
+        assert img_o.mean() > 0 and err <= 1e-4 and div <= 0.01
+        assert abs(img_g.mean() - img_o.mean()) <= 1e-3 * img_o.mean()
+        assert abs(sg["num_rays"] - so.num_rays) <= 1e-3 * so.num_rays
+
+
 def test_light_tracer_unsupported(device):
-    s = scenes.box_scene(16, 16)
-    s.Background = EnvironmentMap(np.ones((8, 16, 3), np.float32))
-    with pytest.raises(abi.SsptError):
-        GpuScene(device, s).render_light_tracer(ol.lt_settings(), 0, 1)
     batch = GpuBatch(device, [scenes.box_scene(16, 16), scenes.box_scene(16, 16)])
     with pytest.raises(abi.SsptError):
         device.check(device.lib.sspt_render_light_tracer(device.ptr, batch.ptr, ol.lt_settings(), 0, 1, None))

@@ -18,8 +18,8 @@ import numpy as np
 import pytest
 
 import oracle_lib as ol
-from seesharp_amd import scenes
-from seesharp_amd.scene import (EnvironmentMap, GenericMaterial, GlossyEmitter, Mesh)
+from seesharp_amd import abi, scenes
+from seesharp_amd.scene import (GenericMaterial, GlossyEmitter, Mesh)
 
 BATCHES = 8
 
@@ -141,8 +141,16 @@ def test_light_tracer_running_mean_split():
     assert np.array_equal(full, part)
 
 
-def test_light_tracer_rejects_background():
-    s = scenes.box_scene(16, 16)
-    s.Background = EnvironmentMap(np.ones((8, 16, 3), np.float32))
-    with pytest.raises(ValueError):
-        ol.OracleScene(s).render_light_tracer(ol.lt_settings(), 0, 1)
+def test_light_tracer_with_background_is_vertex_cache_bidir():
+    # with a background the configuration also traces camera paths that hit it
+    # (BidirBase.OnBackgroundHit): VertexCacheBidir without connections, next
+    # events or emitter hits, camera paths from the default BaseSeedCamera
+    from test_oracle_bidir_background import open_box
+
+    o = ol.OracleScene(open_box(24, 24))
+    lt, st_lt, _ = o.render_light_tracer(ol.lt_settings(max_depth=5), 0, 2)
+    vcb, st_v, _ = o.render_vcm(ol.vcm_settings(algorithm=abi.VERTEX_CACHE_BIDIR, base_seed_camera=0xC030114,
+                                                max_depth=5, num_connections=0, num_shadow_rays=0,
+                                                enable_hitting=False), 0, 2)
+    assert lt.mean() > 0 and st_lt.num_rays == st_v.num_rays
+    np.testing.assert_array_equal(lt, vcb)
