@@ -1618,6 +1618,27 @@ __device__ __forceinline__ EmitterRay emitter_ray(const DevScene& S, const DevEm
     return r;
 }
 
+// SurfaceShader.Evaluate and Pdf(inDir) of one direction in one pass: the BSDF
+// value without the cosine, the forward and the reverse pdf (the bidirectional
+// weights need all three; two calls would evaluate the material twice).
+__device__ __forceinline__ f3 value_pdf_pair(const Shader& sh, f3 in_world, float* fwd, float* rev) {
+    f3 v;
+    if (sh.m.kind == 1) {
+        const f3 in = sh.f.to_local(in_world);
+        generic_value_pdf(sh, in, &v, fwd);
+        *rev = generic_pdf_rev(sh, in);
+    } else {
+        diffuse_value_pdf(sh, in_world, &v, fwd);
+        *rev = diffuse_pdf_rev(sh, in_world);
+    }
+    return v;
+}
+// The same with EvaluateWithCosine's |cos| of the shading frame.
+__device__ __forceinline__ f3 value_cos_pdf_pair(const Shader& sh, f3 in_world, float* fwd, float* rev) {
+    const f3 v = value_pdf_pair(sh, in_world, fwd, rev);
+    return v * fabsf(sh.f.to_local(in_world).z);
+}
+
 // SurfaceShader.Evaluate: the BSDF value without the cosine.
 __device__ __forceinline__ f3 material_value(const Shader& sh, f3 in_world) {
     f3 v;
@@ -2018,11 +2039,10 @@ __global__ void __launch_bounds__(kTraceBlock, 4) k_bidir_light(DevScene S, int 
                     const f3 to_anc = normalize(prev.pos - s.pos);
                     Shader sh;
                     make_shader(S, s, to_cam, &sh);  // the camera side: IsOnLightSubpath = false
-                    f3 f = material_value(sh, to_anc);
+                    float pdf_rev, unused;
+                    f3 f = value_pdf_pair(sh, to_anc, &pdf_rev, &unused);
                     if (!is_zero(f)) {
                         f = f * sdiv(fabsf(dot(s.ns, to_anc)), fabsf(dot(s.ng, to_anc)));
-                        float pdf_rev, unused;
-                        pdf_pair(sh, to_anc, &pdf_rev, &unused);
                         if (!first_bg) pdf_rev *= area_to_solid_angle(s.pos, prev.pos, prev.ng);
                         // PdfEmit = invLensArea * jacobian, then the vertex's own reverse pdf
                         const float mis = srcp(
@@ -2178,16 +2198,12 @@ __global__ void __launch_bounds__(kTraceBlock, 4) k_bidir_camera(DevScene S, int
                             Shader ls;
                             make_shader(S, lv, to_anc, &ls);
                             ls.light = true;
-                            f3 f_light = material_value(ls, -dir) * fabsf(dot(lv.ng, -dir));
-                            f_light = f_light * sdiv(fabsf(dot(lv.ns, to_anc)), fabsf(dot(lv.ng, to_anc)));
-                            f3 f_cam;
-                            float unused;
-                            eval_cos_pdf(sh, dir, &f_cam, &unused);
                             float pcl, pcr, plc, plr;
-                            pdf_pair(sh, dir, &pcl, &pcr);
+                            f3 f_light = value_pdf_pair(ls, -dir, &plc, &plr) * fabsf(dot(lv.ng, -dir));
+                            f_light = f_light * sdiv(fabsf(dot(lv.ns, to_anc)), fabsf(dot(lv.ng, to_anc)));
+                            const f3 f_cam = value_cos_pdf_pair(sh, dir, &pcl, &pcr);
                             pcr *= rev_jac;
                             pcl *= area_to_solid_angle(s.pos, lv.pos, lv.ng);
-                            pdf_pair(ls, -dir, &plc, &plr);
                             if (k > 1 || !light_from_bg) plr *= area_to_solid_angle(lv.pos, anc.pos, anc.ng);
                             plc *= area_to_solid_angle(lv.pos, s.pos, s.ng);
                             const float dist2 = dot(dd, dd);
@@ -2222,10 +2238,8 @@ __global__ void __launch_bounds__(kTraceBlock, 4) k_bidir_camera(DevScene S, int
                         wave_count(stats, kStatShadowTraced, true);
                         wave_count(stats, kStatOccluded, occ);
                         if (occ) continue;
-                        f3 bc;
-                        float unused, fwd, rev;
-                        eval_cos_pdf(sh, es.dir, &bc, &unused);
-                        pdf_pair(sh, es.dir, &fwd, &rev);
+                        float fwd, rev;
+                        const f3 bc = value_cos_pdf_pair(sh, es.dir, &fwd, &rev);
                         rev *= rev_jac;
                         if (fwd == 0.0f || rev == 0.0f || is_zero(bc)) continue;
                         float sum = 1.0f;
@@ -2255,12 +2269,9 @@ __global__ void __launch_bounds__(kTraceBlock, 4) k_bidir_camera(DevScene S, int
                     if (occ) continue;
                     const f3 l2s = normalize(s.pos - lp.pos);
                     const f3 emission = emitted(e, lp.ns, l2s);
-                    f3 bc;
-                    float unused;
-                    eval_cos_pdf(sh, -l2s, &bc, &unused);
-                    const float jac = area_to_solid_angle(s.pos, lp.pos, lp.ng);
                     float fwd, rev;
-                    pdf_pair(sh, -l2s, &fwd, &rev);
+                    const f3 bc = value_cos_pdf_pair(sh, -l2s, &fwd, &rev);
+                    const float jac = area_to_solid_angle(s.pos, lp.pos, lp.ng);
                     fwd *= jac;
                     rev *= rev_jac;
                     if (is_zero(emission) || is_zero(bc) || jac == 0.0f || fwd == 0.0f || rev == 0.0f) continue;
@@ -2668,11 +2679,10 @@ __global__ void __launch_bounds__(kTraceBlock) k_vcm_splat(DevScene S, int scene
                     const f3 to_anc = normalize(prev.pos - s.pos);
                     Shader sh;
                     make_shader(S, s, to_cam, &sh);
-                    f3 f = material_value(sh, to_anc);
+                    float pdf_rev, unused;
+                    f3 f = value_pdf_pair(sh, to_anc, &pdf_rev, &unused);
                     if (!is_zero(f)) {
                         f = f * sdiv(fabsf(dot(s.ns, to_anc)), fabsf(dot(s.ng, to_anc)));
-                        float pdf_rev, unused;
-                        pdf_pair(sh, to_anc, &pdf_rev, &unused);
                         if (depth > 1 || !from_bg) pdf_rev *= area_to_solid_angle(s.pos, prev.pos, prev.ng);
                         PdfView v{&P, 0u, idx, depth + 1, 0, ne};
                         v.set_c2l(0, 1.0f * jac);
@@ -2831,16 +2841,12 @@ __global__ void __launch_bounds__(kTraceBlock, 3) k_vcm_camera(DevScene S, int s
                     Shader ls;
                     make_shader(S, lv, to_anc, &ls);
                     ls.light = true;
-                    f3 f_light = material_value(ls, -dir) * fabsf(dot(lv.ng, -dir));
-                    f_light = f_light * sdiv(fabsf(dot(lv.ns, to_anc)), fabsf(dot(lv.ng, to_anc)));
-                    f3 f_cam;
-                    float unused;
-                    eval_cos_pdf(sh, dir, &f_cam, &unused);
                     float pcl, pcr, plc, plr;
-                    pdf_pair(sh, dir, &pcl, &pcr);
+                    f3 f_light = value_pdf_pair(ls, -dir, &plc, &plr) * fabsf(dot(lv.ng, -dir));
+                    f_light = f_light * sdiv(fabsf(dot(lv.ns, to_anc)), fabsf(dot(lv.ng, to_anc)));
+                    const f3 f_cam = value_cos_pdf_pair(sh, dir, &pcl, &pcr);
                     pcr *= rev_jac;
                     pcl *= area_to_solid_angle(s.pos, lv.pos, lv.ng);
-                    pdf_pair(ls, -dir, &plc, &plr);
                     if (k > 1 || !lp_bg) plr *= area_to_solid_angle(lv.pos, anc.pos, anc.ng);
                     plc *= area_to_solid_angle(lv.pos, s.pos, s.ng);
                     const float dist2 = dot(dd, dd);
@@ -2883,13 +2889,12 @@ __global__ void __launch_bounds__(kTraceBlock, 3) k_vcm_camera(DevScene S, int s
                         const bool lp_bg = from_background<ENV>(P, lp);
                         const bool first_from_bg = k == 1 && lp_bg;
                         const f3 to_anc = normalize(anc.pos - s.pos);
-                        f3 bsdf = material_value(sh, to_anc);
+                        float pdf_light_rev, pdf_cam_rev;
+                        f3 bsdf = value_pdf_pair(sh, to_anc, &pdf_light_rev, &pdf_cam_rev);
                         bsdf = bsdf * sdiv(fabsf(dot(s.ns, to_anc)), fabsf(dot(ph.ng, to_anc)));
                         const f3 contrib = f3_of(la) * bsdf / P.num_paths_f;
                         if (is_zero(contrib)) continue;
                         if (fabsf(dot(to_anc, s.ng)) < 1e-4f) continue;
-                        float pdf_light_rev, pdf_cam_rev;
-                        pdf_pair(sh, to_anc, &pdf_light_rev, &pdf_cam_rev);
                         pdf_cam_rev *= rev_jac;
                         // at the first hit from the background the pdf stays per solid angle
                         if (!first_from_bg) pdf_light_rev *= area_to_solid_angle(s.pos, anc.pos, anc.ng);
@@ -2935,10 +2940,8 @@ __global__ void __launch_bounds__(kTraceBlock, 3) k_vcm_camera(DevScene S, int s
                         wave_count(stats, kStatShadowTraced, true);
                         wave_count(stats, kStatOccluded, occ);
                         if (occ) continue;
-                        f3 bc;
-                        float unused, fwd, rev;
-                        eval_cos_pdf(sh, es.dir, &bc, &unused);
-                        pdf_pair(sh, es.dir, &fwd, &rev);
+                        float fwd, rev;
+                        const f3 bc = value_cos_pdf_pair(sh, es.dir, &fwd, &rev);
                         rev *= rev_jac;
                         if (fwd == 0.0f || rev == 0.0f || is_zero(bc)) continue;
                         PdfView v{&P, pix, 0u, c + 1, c, es.pdf};
@@ -2973,12 +2976,9 @@ __global__ void __launch_bounds__(kTraceBlock, 3) k_vcm_camera(DevScene S, int s
                     if (occ) continue;
                     const f3 l2s = normalize(s.pos - lp.pos);
                     const f3 emission = emitted(e, lp.ns, l2s);
-                    f3 bc;
-                    float unused;
-                    eval_cos_pdf(sh, -l2s, &bc, &unused);
-                    const float jac = area_to_solid_angle(s.pos, lp.pos, lp.ng);
                     float fwd, rev;
-                    pdf_pair(sh, -l2s, &fwd, &rev);
+                    const f3 bc = value_cos_pdf_pair(sh, -l2s, &fwd, &rev);
+                    const float jac = area_to_solid_angle(s.pos, lp.pos, lp.ng);
                     fwd *= jac;
                     rev *= rev_jac;
                     if (is_zero(emission) || is_zero(bc) || jac == 0.0f || fwd == 0.0f || rev == 0.0f) continue;
