@@ -96,3 +96,24 @@ def test_background_vcm_settings_match_oracle(device, kw, algorithm):
     s = open_box(48, 48, spheres=True)
     compare(GpuScene(device, s), ol.OracleScene(s), "render_vcm", ol.vcm_settings(algorithm=algorithm, **kw), 2,
             1e-5, ident(kw))
+
+
+def textured_glossy_open_box(width=48, height=48):
+    """The open box under the sky with textured back wall and floor (albedo and
+    roughness textures), a glossy ceiling emitter and the flat-shaded spheres."""
+    from seesharp_amd.scene import GlossyEmitter
+
+    s = open_box(width, height, spheres=True)
+    back, floor = scenes.textured_materials(32, 5, None)
+    s.Meshes[0].Material = floor
+    s.Meshes[1].Material = back  # the ceiling was removed: index 1 is the back wall
+    light = s.Emitters[0].Mesh
+    s.Emitters = GlossyEmitter.MakeFromMesh(light, (15.0, 15.0, 15.0), 6.0)
+    return s
+
+
+@pytest.mark.parametrize("render", list(RENDERS))
+def test_background_with_textures_and_glossy_emitter_matches_oracle(device, render):
+    kind, settings, kw = RENDERS[render]
+    s = textured_glossy_open_box()
+    compare(GpuScene(device, s), ol.OracleScene(s), kind, settings(max_depth=6, **kw), 2, 1e-5, "textured glossy")
