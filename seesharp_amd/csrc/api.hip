@@ -1821,7 +1821,13 @@ sspt_status sspt_render_bidir(sspt_ctx* ctx, sspt_scene* sc, const sspt_bidir_se
     // camera pdfs: max_depth levels of a float2 per pixel. A path from the background
     // stores its first hit whatever MaxDepth (RandomWalk.StartFromBackground): two levels.
     const uint64_t levels = (uint64_t)std::max(cfg->max_depth, R.has_env ? 2 : 1);
-    const uint64_t need = levels * paths * (2 * sizeof(float4) + sizeof(float2)) + paths * 8 + 256 * 4;
+    // the camera wavefront: per pixel its state (10 float4 and the RNG word), two list
+    // entries, and per bounce max_depth - 1 connection and num_shadow_rays next-event
+    // records of three float4 and a list entry each
+    const uint64_t recs = (uint64_t)std::max(cfg->max_depth - 1, 0) + (uint64_t)cfg->num_shadow_rays;
+    const uint64_t wave_bytes = paths * (10 * sizeof(float4) + 3 * 4) + paths * std::max<uint64_t>(recs, 1) * 52 +
+                                32 * 256;  // and each of the 18 carves' alignment
+    const uint64_t need = levels * paths * (2 * sizeof(float4) + sizeof(float2)) + paths * 8 + 256 * 4 + wave_bytes;
     if (need > (64ull << 30)) return fail(ctx, SSPT_ERR_OUT_OF_MEMORY, "bidirectional buffers exceed 64 GiB");
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     if (sc->bidir_bytes < need) {
@@ -1849,6 +1855,18 @@ sspt_status sspt_render_bidir(sspt_ctx* ctx, sspt_scene* sc, const sspt_bidir_se
     P.cam = reinterpret_cast<float2*>(carve(&off, levels * paths * sizeof(float2)));
     P.lv_len = reinterpret_cast<int*>(carve(&off, paths * sizeof(int)));
     P.lv_ne = reinterpret_cast<float*>(carve(&off, paths * sizeof(float)));
+    const uint64_t nrec = paths * std::max<uint64_t>(recs, 1);
+    P.bc_rng = reinterpret_cast<uint32_t*>(carve(&off, paths * 4));
+    float4** state[] = {&P.bc_a, &P.bc_b, &P.bc_o, &P.bc_d, &P.bc_pp, &P.bc_pn, &P.bc_L, &P.bc_hit, &P.bc_v0, &P.bc_v1};
+    for (float4** f : state) *f = reinterpret_cast<float4*>(carve(&off, paths * sizeof(float4)));
+    P.bc_r0 = reinterpret_cast<float4*>(carve(&off, nrec * sizeof(float4)));
+    P.bc_r1 = reinterpret_cast<float4*>(carve(&off, nrec * sizeof(float4)));
+    P.bc_rc = reinterpret_cast<float4*>(carve(&off, nrec * sizeof(float4)));
+    P.bc_list0 = reinterpret_cast<uint32_t*>(carve(&off, paths * 4));
+    P.bc_list1 = reinterpret_cast<uint32_t*>(carve(&off, paths * 4));
+    P.bc_shadow = reinterpret_cast<uint32_t*>(carve(&off, nrec * 4));
+    P.bc_count = reinterpret_cast<uint32_t*>(carve(&off, 16));
+    P.bc_rec = (int)recs;
     P.seed_camera = cfg->base_seed_camera;
     P.seed_light = cfg->base_seed_light;
     P.num_paths = (uint32_t)paths;
@@ -1874,7 +1892,7 @@ sspt_status sspt_render_bidir(sspt_ctx* ctx, sspt_scene* sc, const sspt_bidir_se
         if (paths > 0) {
             timed(ctx, "bidir_light", [&] { launch_bidir_light(ctx->stream, sc->dev, 0, P, image, ctx->lt_stats); });
             timed(ctx, "bidir_camera",
-                  [&] { launch_bidir_camera(ctx->stream, sc->dev, 0, P, image, ctx->lt_stats); });
+                  [&] { launch_bidir_camera_wave(ctx->stream, sc->dev, 0, P, image, ctx->lt_stats); });
         }
         HIP_TRY(ctx, hipGetLastError());
     }

@@ -139,6 +139,27 @@ struct BidirParams {
     float2* cam;
     uint32_t num_pixels;  // camera paths (ClassicBidir: num_paths)
     int env;              // the scene has a background (the kernels' instantiation with its code)
+    // ClassicBidir's camera paths as a wavefront (per pixel p; records at [p * bc_rec + r]):
+    // live paths' state, hit, the vertex's emitter term and prefix, shadow records
+    uint32_t* bc_rng;
+    float4* bc_a;         // prefix rgb, pdf of the last direction
+    float4* bc_b;         // approximate throughput rgb, depth (int bits)
+    float4* bc_o;         // ray origin, tmin
+    float4* bc_d;         // ray direction
+    float4* bc_pp;        // previous vertex position
+    float4* bc_pn;        // previous vertex normal
+    float4* bc_L;         // the path's sum
+    float4* bc_hit;       // u, v, t, triangle (int bits; -1 = miss)
+    float4* bc_v0;        // the vertex's prefix-weighted emitter / background term, connections (bits)
+    float4* bc_v1;        // the vertex's prefix, next-event records (bits)
+    float4* bc_r0;        // record: segment start (or ray origin), tmin of a ray to infinity
+    float4* bc_r1;        // record: segment end (or direction), w = 1 for a ray to infinity
+    float4* bc_rc;        // record: unweighted term (mis * w), visibility
+    uint32_t* bc_list0;   // live paths of even bounces
+    uint32_t* bc_list1;   // of odd bounces
+    uint32_t* bc_shadow;  // record slots to test
+    uint32_t* bc_count;   // [0], [1]: list sizes, [2]: records to test
+    int bc_rec;           // records per path and bounce: max_depth - 1 connections + num_shadow_rays
     // VertexCacheBidir / VertexConnectionAndMerging
     int mode;             // kModeClassic, kModeVertexCache, kModeVcm
     int num_connections, hitting, merging, merge_primary, max_photons, correl;
@@ -156,8 +177,10 @@ constexpr int kMaxPhotons = 32;  // largest MaxNumPhotons of a merge
 // TraceLightPaths (+ SplatLightVertices if light_tracer), then the camera paths.
 void launch_bidir_light(hipStream_t s, const DevScene& sc, int scene, const BidirParams& p, float* image,
                         unsigned long long* stats);
-void launch_bidir_camera(hipStream_t s, const DevScene& sc, int scene, const BidirParams& p, float* image,
-                         unsigned long long* stats);
+// The camera paths as a wavefront: per bounce a trace, a shade, a shadow and a
+// gather kernel.
+void launch_bidir_camera_wave(hipStream_t s, const DevScene& sc, int scene, const BidirParams& p, float* image,
+                              unsigned long long* stats);
 // VertexCacheBidir / VCM: the light paths' global vertex indices (p.cum), the
 // photon grid (VCM with merging) and the camera paths. `scan_tmp` holds
 // scan_tmp_words(max(num_paths, table size)) ints.

@@ -1747,14 +1747,14 @@ __global__ void __launch_bounds__(kTraceBlock, 4) k_light_paths(DevScene S, int 
 }
 
 // ---------------------------------------------- bidirectional path tracing -
-// ClassicBidir (ClassicBidir.cs) in two kernels per iteration: k_bidir_light
-// traces one light path per pixel into a cache in HBM, one level per vertex
-// depth so that the lanes of a wave (neighbouring paths) touch neighbouring
-// entries, and splats its vertices onto the camera; k_bidir_camera then
-// follows one camera path per pixel and combines its emitter hits, next-event
-// samples and connections to the vertices of its pixel's light path. One thread
-// per path: the MIS weights read the pdfs along both subpaths from the cache and
-// the per-pixel camera pdf levels instead of per-thread arrays.
+// ClassicBidir (ClassicBidir.cs) per iteration: k_bidir_light traces one light
+// path per pixel into a cache in HBM, one level per vertex depth so that the
+// lanes of a wave (neighbouring paths) touch neighbouring entries, and splats
+// its vertices onto the camera; the camera wavefront (k_bc_*) then follows one
+// camera path per pixel and combines its emitter hits, next-event samples and
+// connections to the vertices of its pixel's light path. The MIS weights read
+// the pdfs along both subpaths from the cache and the per-pixel camera pdf
+// levels instead of per-thread arrays.
 
 // Emitter.PdfRay: DiffuseEmitter.cs:39-42, GlossyEmitter.cs:42-45 (with
 // SampleWarp.ToCosineLobeJacobian, SampleWarp.cs:192-194).
@@ -2097,212 +2097,336 @@ __global__ void __launch_bounds__(kTraceBlock, 4) k_bidir_light(DevScene S, int 
     }
 }
 
+// ------------------------------------- ClassicBidir camera paths, wavefront -
 // BidirBase.RenderPixel (BidirBase.Camera.cs:97-106) and the camera's RandomWalk
 // (RandomWalk.cs:73-80,150-217, CameraRandomWalk BidirBase.Camera.cs:588-653) of
 // MaxDepth + 1 vertices with ClassicBidir.OnCameraHit (ClassicBidir.cs:94-116):
-// OnEmitterHit (BidirBase.Camera.cs:500-524), the connections to every vertex
-// of the pixel's light path (BidirConnections / Connect, :192-307) and
-// NumShadowRays next-event samples (PerformNextEventEstimation, :368-484), each
-// weighted by the balance heuristic (ClassicBidir.cs:119-153). The pixel value
-// is added to the running mean after the light tracer's splats.
+// OnEmitterHit / OnBackgroundHit (BidirBase.Camera.cs:500-561), the connections
+// to every vertex of the pixel's light path (BidirConnections / Connect,
+// :192-307) and NumShadowRays next-event samples (PerformNextEventEstimation,
+// :368-484), each weighted by the balance heuristic (ClassicBidir.cs:119-153).
+// Per bounce k_bc_trace finds the live paths' closest hits, k_bc_shade shades
+// them (the hit's emitter or background term, connections and next events
+// written as shadow records that carry their weighted terms, the
+// continuation), k_bc_shadow tests the records' segments and k_bc_gather adds
+// each path's vertex value to its sum in the reference's order (emitter term,
+// prefix times the visible connections' sum, each visible next event). The
+// shadow rays run in a traversal kernel at the traversal's occupancy instead of
+// inside the shading (camera pass 8.0 -> 5.35 ms per large-scene iteration),
+// records whose term is zero are counted but not traced, and the live paths are
+// compacted every bounce. The pixel value is added to the running mean after
+// the light tracer's splats.
+__device__ __forceinline__ uint32_t* bc_list(const BidirParams& P, int which) {
+    return which ? P.bc_list1 : P.bc_list0;
+}
+
 template <bool ENV>
-__global__ void __launch_bounds__(kTraceBlock, 4) k_bidir_camera(DevScene S, int scene, BidirParams P, float* image,
-                                                              unsigned long long* stats) {
-    __shared__ int stack_node[kStackNodeLds];
-    __shared__ float stack_t[kStackLds * kTraceBlock];
+__global__ void __launch_bounds__(256) k_bc_init(DevScene S, int scene, BidirParams P) {
     const SceneRec& R = S.recs[scene];
     const DevCamera& C = R.cam;
-    const Env E = env_of(S, R);
-    const float bg = ENV ? background_probability(R) : 0.0f;  // ComputeNextEventBackgroundProbability
-    for (uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x; pix < P.num_paths; pix += gridDim.x * blockDim.x) {
-        BidirCtx<ENV> B{S, R, P, pix, bg};
+    for (uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x; pix < P.num_pixels; pix += gridDim.x * blockDim.x) {
         const uint32_t row = pix / (uint32_t)C.width, col = pix % (uint32_t)C.width;
         Rng rng{hash_seed(P.seed_camera, pix, P.iteration)};
         const f2 off = rng.next_float2();
         f3 o, d, raw;
         camera_ray(C, (float)col + off.x, (float)row + off.y, &o, &d, &raw);
-        float tmin = 0.0f;
         // CameraRaySample.Point (position, normal = Camera.Direction) and PdfRay
-        f3 prev_pos = o;
-        f3 prev_n = mk3(-C.c2w[8], -C.c2w[9], -C.c2w[10]);
-        float pdf_dir = pixel_jacobian(C, o + raw);
-        const int light_len = P.lv_len[pix];
-        const float ne = P.lv_ne[pix];
-        const bool light_from_bg = light_len > 0 && from_background<ENV>(P, pix);
-        f3 L = gray(0.0f), prefix = gray(1.0f), approx = gray(1.0f);
-        for (int depth = 1; depth < P.max_depth + 1;) {
-            const HitOut h = traverse<false>(S, R.root, R.abs_max, o, d, tmin, INFINITY, stack_node + threadIdx.x,
-                                             stack_t + threadIdx.x);
-            wave_count(stats, kStatRays, true);
-            wave_count(stats, kStatHits, h.li >= 0);
-            if (h.li < 0) {
-                // CameraRandomWalk.OnInvalidHit (BidirBase.Camera.cs:604-612): the escaped
-                // direction is the last vertex, with its solid-angle pdf; OnBackgroundHit
-                // (:532-561) with ClassicBidir.EmitterHitMis, no MinDepth test
-                if (ENV) {
-                    const int c = depth;
-                    float mis = 1.0f;
-                    if (c > 1) {
-                        const float x_prev = c - 2 >= 1 ? B.cam(c - 2).y : 0.0f;
-                        float sum = 1.0f;
-                        sum += sdiv(env_pdf(E, d) * bg, pdf_dir);
-                        sum += sdiv(B.camera_reciprocals(c - 2, background_pdf(E, R, -d), x_prev), pdf_dir);
-                        mis = srcp(sum);
-                    }
-                    L = L + prefix * (mis * env_radiance(E, d));
-                }
-                break;
-            }
-            const Surf s = surface(S, h.li, h.u, h.v);
-            const float pdf_from = pdf_dir * area_to_solid_angle(prev_pos, s.pos, s.ng);
-            if (pdf_from == 0.0f) break;
-            const float rev_jac = area_to_solid_angle(s.pos, prev_pos, prev_n);
-            Shader sh;
-            make_shader(S, s, -d, &sh);
-            const int c = depth;  // camera vertices, this one included
-            P.cam[(size_t)(c - 1) * P.num_pixels + pix] = make_float2(pdf_from, 0.0f);
-            f3 value = gray(0.0f);
-            const int ei = S.tri_emitter[h.li];
-            if (ei >= 0 && depth >= P.min_depth) {
-                const DevEmitter e = S.emitters[R.emitter_off + ei];
-                const f3 emission = emitted(e, s.ns, -d);
+        P.bc_rng[pix] = rng.state;
+        P.bc_a[pix] = make_float4(1.0f, 1.0f, 1.0f, pixel_jacobian(C, o + raw));
+        P.bc_b[pix] = make_float4(1.0f, 1.0f, 1.0f, __int_as_float(1));
+        P.bc_o[pix] = f4_of(o, 0.0f);
+        P.bc_d[pix] = f4_of(d, 0.0f);
+        P.bc_pp[pix] = f4_of(o, 0.0f);
+        P.bc_pn[pix] = make_float4(-C.c2w[8], -C.c2w[9], -C.c2w[10], 0.0f);
+        P.bc_L[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        P.bc_list0[pix] = pix;
+    }
+}
+
+__global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(SSPT_TRACE_WAVES)))
+k_bc_trace(DevScene S, int scene, BidirParams P, int in, unsigned long long* stats) {
+    __shared__ int stack_node[kStackNodeLds];
+    __shared__ float stack_t[kStackLds * kTraceBlock];
+    const SceneRec& R = S.recs[scene];
+    const uint32_t n = P.bc_count[in];
+    const uint32_t* list = bc_list(P, in);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t pix = list[i];
+        const float4 o = P.bc_o[pix], d = P.bc_d[pix];
+        const HitOut h = traverse<false>(S, R.root, R.abs_max, f3_of(o), f3_of(d), o.w, INFINITY,
+                                         stack_node + threadIdx.x, stack_t + threadIdx.x);
+        wave_count(stats, kStatRays, true);
+        wave_count(stats, kStatHits, h.li >= 0);
+        P.bc_hit[pix] = make_float4(h.u, h.v, h.t, __int_as_float(h.li));
+    }
+}
+
+template <bool ENV>
+__global__ void __launch_bounds__(kTraceBlock, 2) k_bc_shade(DevScene S, int scene, BidirParams P, int in,
+                                                             unsigned long long* stats) {
+    const SceneRec& R = S.recs[scene];
+    const Env E = env_of(S, R);
+    const float bg = ENV ? background_probability(R) : 0.0f;
+    const uint32_t n = P.bc_count[in];
+    const uint32_t* list = bc_list(P, in);
+    uint32_t* next = bc_list(P, 1 - in);
+    const int conn_slots = max(P.max_depth - 1, 0);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t pix = list[i];
+        BidirCtx<ENV> B{S, R, P, pix, bg};
+        Rng rng{P.bc_rng[pix]};
+        const float4 a = P.bc_a[pix], b = P.bc_b[pix];
+        f3 prefix = f3_of(a), approx = f3_of(b);
+        const float pdf_dir = a.w;
+        int depth = __float_as_int(b.w);
+        const f3 d = f3_of(P.bc_d[pix]);
+        const f3 prev_pos = f3_of(P.bc_pp[pix]), prev_n = f3_of(P.bc_pn[pix]);
+        const float4 hit = P.bc_hit[pix];
+        const int li = __float_as_int(hit.w);
+        const size_t rec0 = (size_t)pix * P.bc_rec;
+        f3 value = gray(0.0f);
+        int nconn = 0, nnee = 0;
+        bool conn_block = false;
+        if (li < 0) {
+            // CameraRandomWalk.OnInvalidHit / OnBackgroundHit with ClassicBidir.EmitterHitMis
+            if (ENV) {
+                const int c = depth;
                 float mis = 1.0f;
                 if (c > 1) {
-                    const float pdf_emit = B.emitter_pdf(e, s.ns, -d, rev_jac);
                     const float x_prev = c - 2 >= 1 ? B.cam(c - 2).y : 0.0f;
                     float sum = 1.0f;
-                    sum += sdiv(B.next_event_pdf(e), pdf_from);
-                    sum += sdiv(B.camera_reciprocals(c - 2, pdf_emit, x_prev), pdf_from);
+                    sum += sdiv(env_pdf(E, d) * bg, pdf_dir);
+                    sum += sdiv(B.camera_reciprocals(c - 2, background_pdf(E, R, -d), x_prev), pdf_dir);
                     mis = srcp(sum);
                 }
-                value = value + prefix * (mis * emission);
+                value = prefix * (mis * env_radiance(E, d));
             }
-            if (depth < P.max_depth && P.connections && light_len > 1) {
-                f3 conn = gray(0.0f);
-                Surf anc = B.vertex(0);
-                for (int k = 1; k < light_len; ++k) {
-                    const Surf lv = B.vertex(k);
-                    const int total = k + c + 1;
-                    if (total <= P.max_depth && total >= P.min_depth) {
-                        const f3 dd = s.pos - lv.pos;
-                        const f3 p0 = offset_point(lv, dd), p1 = offset_point(s, -dd);
-                        const bool occ = traverse<true>(S, R.root, R.abs_max, p0, p1 - p0, 1e-5f, 1.0f - 1e-5f,
-                                                        stack_node + threadIdx.x, stack_t + threadIdx.x)
-                                             .li >= 0;
-                        wave_count(stats, kStatShadow, true);
-                        wave_count(stats, kStatShadowTraced, true);
-                        wave_count(stats, kStatOccluded, occ);
-                        if (!occ) {
-                            const f3 to_anc = normalize(anc.pos - lv.pos);
-                            const f3 dir = normalize(lv.pos - s.pos);  // camera vertex -> light vertex
-                            Shader ls;
-                            make_shader(S, lv, to_anc, &ls);
-                            ls.light = true;
-                            float pcl, pcr, plc, plr;
-                            f3 f_light = value_pdf_pair(ls, -dir, &plc, &plr) * fabsf(dot(lv.ng, -dir));
-                            f_light = f_light * sdiv(fabsf(dot(lv.ns, to_anc)), fabsf(dot(lv.ng, to_anc)));
-                            const f3 f_cam = value_cos_pdf_pair(sh, dir, &pcl, &pcr);
-                            pcr *= rev_jac;
-                            pcl *= area_to_solid_angle(s.pos, lv.pos, lv.ng);
-                            if (k > 1 || !light_from_bg) plr *= area_to_solid_angle(lv.pos, anc.pos, anc.ng);
-                            plc *= area_to_solid_angle(lv.pos, s.pos, s.ng);
-                            const float dist2 = dot(dd, dd);
-                            if (!is_zero(f_cam) && !is_zero(f_light) && pcl != 0.0f && dist2 != 0.0f) {
-                                float sum = 1.0f;
-                                sum += B.camera_reciprocals(c - 1, plc, pcr);
-                                sum += B.light_reciprocals(c, k, pcl, plr, ne);
-                                const float mis = srcp(sum);
-                                const f3 w = f3_of(B.va(k)) * f_light * f_cam / dist2 / 1.0f;
-                                conn = conn + mis * w;
-                            }
-                        }
+            P.bc_v0[pix] = f4_of(value, __int_as_float(0));
+            P.bc_v1[pix] = f4_of(prefix, __int_as_float(0));
+            continue;
+        }
+        const Surf s = surface(S, li, hit.x, hit.y);
+        const float pdf_from = pdf_dir * area_to_solid_angle(prev_pos, s.pos, s.ng);
+        if (pdf_from == 0.0f) {
+            P.bc_v0[pix] = f4_of(value, __int_as_float(0));
+            P.bc_v1[pix] = f4_of(prefix, __int_as_float(0));
+            continue;
+        }
+        const float rev_jac = area_to_solid_angle(s.pos, prev_pos, prev_n);
+        Shader sh;
+        make_shader(S, s, -d, &sh);
+        const int c = depth;  // camera vertices, this one included
+        P.cam[(size_t)(c - 1) * P.num_pixels + pix] = make_float2(pdf_from, 0.0f);
+        const int ei = S.tri_emitter[li];
+        if (ei >= 0 && depth >= P.min_depth) {
+            const DevEmitter e = S.emitters[R.emitter_off + ei];
+            const f3 emission = emitted(e, s.ns, -d);
+            float mis = 1.0f;
+            if (c > 1) {
+                const float pdf_emit = B.emitter_pdf(e, s.ns, -d, rev_jac);
+                const float x_prev = c - 2 >= 1 ? B.cam(c - 2).y : 0.0f;
+                float sum = 1.0f;
+                sum += sdiv(B.next_event_pdf(e), pdf_from);
+                sum += sdiv(B.camera_reciprocals(c - 2, pdf_emit, x_prev), pdf_from);
+                mis = srcp(sum);
+            }
+            value = value + prefix * (mis * emission);
+        }
+        const int light_len = P.lv_len[pix];
+        if (depth < P.max_depth && P.connections && light_len > 1) {
+            conn_block = true;
+            const float ne = P.lv_ne[pix];
+            const bool light_from_bg = from_background<ENV>(P, pix);
+            Surf anc = B.vertex(0);
+            for (int k = 1; k < light_len; ++k) {
+                const Surf lv = B.vertex(k);
+                const int total = k + c + 1;
+                if (total <= P.max_depth && total >= P.min_depth) {
+                    const size_t slot = rec0 + nconn++;
+                    wave_count(stats, kStatShadow, true);
+                    const f3 dd = s.pos - lv.pos;
+                    const f3 to_anc = normalize(anc.pos - lv.pos);
+                    const f3 dir = normalize(lv.pos - s.pos);  // camera vertex -> light vertex
+                    Shader ls;
+                    make_shader(S, lv, to_anc, &ls);
+                    ls.light = true;
+                    float pcl, pcr, plc, plr;
+                    f3 f_light = value_pdf_pair(ls, -dir, &plc, &plr) * fabsf(dot(lv.ng, -dir));
+                    f_light = f_light * sdiv(fabsf(dot(lv.ns, to_anc)), fabsf(dot(lv.ng, to_anc)));
+                    const f3 f_cam = value_cos_pdf_pair(sh, dir, &pcl, &pcr);
+                    pcr *= rev_jac;
+                    pcl *= area_to_solid_angle(s.pos, lv.pos, lv.ng);
+                    if (k > 1 || !light_from_bg) plr *= area_to_solid_angle(lv.pos, anc.pos, anc.ng);
+                    plc *= area_to_solid_angle(lv.pos, s.pos, s.ng);
+                    const float dist2 = dot(dd, dd);
+                    f3 term = gray(0.0f);
+                    if (!is_zero(f_cam) && !is_zero(f_light) && pcl != 0.0f && dist2 != 0.0f) {
+                        float sum = 1.0f;
+                        sum += B.camera_reciprocals(c - 1, plc, pcr);
+                        sum += B.light_reciprocals(c, k, pcl, plr, ne);
+                        const float mis = srcp(sum);
+                        const f3 w = f3_of(B.va(k)) * f_light * f_cam / dist2 / 1.0f;
+                        term = mis * w;
                     }
-                    anc = lv;
+                    P.bc_rc[slot] = f4_of(term, 0.0f);
+                    if (!is_zero(term)) {
+                        const f3 p0 = offset_point(lv, dd), p1 = offset_point(s, -dd);
+                        P.bc_r0[slot] = f4_of(p0, 0.0f);
+                        P.bc_r1[slot] = f4_of(p1, 0.0f);
+                        P.bc_shadow[atomicAdd(P.bc_count + 2, 1u)] = (uint32_t)slot;
+                    }
                 }
-                value = value + prefix * conn;
+                anc = lv;
             }
-            if (depth < P.max_depth && depth + 1 >= P.min_depth) {
-                for (int r = 0; r < P.num_shadow_rays; ++r) {
-                    const float u_bg = rng.next_float();  // drawn with or without a background
-                    if (ENV && u_bg < bg) {
-                        // the background branch (BidirBase.Camera.cs:381-418): its pdf carries
-                        // no NumShadowRays factor; ClassicBidir.NextEventMis (ClassicBidir.cs:147-153)
-                        EnvSample es = env_sample(E, rng.next_float2());
-                        es.pdf *= bg;
-                        es.weight = es.weight / bg;
-                        if (es.pdf == 0.0f) continue;
-                        const bool occ = traverse<true>(S, R.root, R.abs_max, offset_point(s, es.dir), es.dir, s.err,
-                                                        INFINITY, stack_node + threadIdx.x, stack_t + threadIdx.x)
-                                             .li >= 0;  // Raytracer.LeavesScene
-                        wave_count(stats, kStatShadow, true);
-                        wave_count(stats, kStatShadowTraced, true);
-                        wave_count(stats, kStatOccluded, occ);
-                        if (occ) continue;
-                        float fwd, rev;
-                        const f3 bc = value_cos_pdf_pair(sh, es.dir, &fwd, &rev);
-                        rev *= rev_jac;
-                        if (fwd == 0.0f || rev == 0.0f || is_zero(bc)) continue;
+        }
+        if (depth < P.max_depth && depth + 1 >= P.min_depth) {
+            for (int r = 0; r < P.num_shadow_rays; ++r) {
+                const float u_bg = rng.next_float();  // drawn with or without a background
+                if (ENV && u_bg < bg) {
+                    EnvSample es = env_sample(E, rng.next_float2());
+                    es.pdf *= bg;
+                    es.weight = es.weight / bg;
+                    if (es.pdf == 0.0f) continue;
+                    const size_t slot = rec0 + conn_slots + nnee++;
+                    wave_count(stats, kStatShadow, true);
+                    float fwd, rev;
+                    const f3 bc = value_cos_pdf_pair(sh, es.dir, &fwd, &rev);
+                    rev *= rev_jac;
+                    f3 term = gray(0.0f);
+                    if (!(fwd == 0.0f || rev == 0.0f || is_zero(bc))) {
                         float sum = 1.0f;
                         sum += sdiv(fwd, es.pdf);
                         sum += sdiv(B.camera_reciprocals(c - 1, background_pdf(E, R, -es.dir), rev), es.pdf);
-                        value = value + prefix * (srcp(sum) * (es.weight * bc));
-                        continue;
+                        term = srcp(sum) * (es.weight * bc);
                     }
-                    if (R.num_emitters == 0) continue;
-                    const DevEmitter e = S.emitters[R.emitter_off + (int)rng.next_int((uint32_t)R.num_emitters)];
-                    const float prob = 1.0f / (float)R.num_emitters;
-                    const f2 b = to_uniform_triangle(rng.next_float2());
-                    const Surf lp = surface(S, e.tri, b.x, b.y);
-                    float lpdf = e.inv_area;
-                    lpdf *= prob;
-                    lpdf *= (float)P.num_shadow_rays;
-                    lpdf *= (1.0f - bg);
-                    if (lpdf == 0.0f) continue;
-                    const f3 dd = lp.pos - s.pos;
-                    const f3 p0 = offset_point(s, dd), p1 = offset_point(lp, -dd);
-                    const bool occ = traverse<true>(S, R.root, R.abs_max, p0, p1 - p0, 1e-5f, 1.0f - 1e-5f,
-                                                    stack_node + threadIdx.x, stack_t + threadIdx.x)
-                                         .li >= 0;
-                    wave_count(stats, kStatShadow, true);
-                    wave_count(stats, kStatShadowTraced, true);
-                    wave_count(stats, kStatOccluded, occ);
-                    if (occ) continue;
-                    const f3 l2s = normalize(s.pos - lp.pos);
-                    const f3 emission = emitted(e, lp.ns, l2s);
-                    float fwd, rev;
-                    const f3 bc = value_cos_pdf_pair(sh, -l2s, &fwd, &rev);
-                    const float jac = area_to_solid_angle(s.pos, lp.pos, lp.ng);
-                    fwd *= jac;
-                    rev *= rev_jac;
-                    if (is_zero(emission) || is_zero(bc) || jac == 0.0f || fwd == 0.0f || rev == 0.0f) continue;
+                    P.bc_rc[slot] = f4_of(term, 0.0f);
+                    if (!is_zero(term)) {  // Raytracer.LeavesScene
+                        P.bc_r0[slot] = f4_of(offset_point(s, es.dir), s.err);
+                        P.bc_r1[slot] = f4_of(es.dir, 1.0f);
+                        P.bc_shadow[atomicAdd(P.bc_count + 2, 1u)] = (uint32_t)slot;
+                    }
+                    continue;
+                }
+                if (R.num_emitters == 0) continue;
+                const DevEmitter e = S.emitters[R.emitter_off + (int)rng.next_int((uint32_t)R.num_emitters)];
+                const float prob = 1.0f / (float)R.num_emitters;
+                const f2 bb = to_uniform_triangle(rng.next_float2());
+                const Surf lp = surface(S, e.tri, bb.x, bb.y);
+                float lpdf = e.inv_area;
+                lpdf *= prob;
+                lpdf *= (float)P.num_shadow_rays;
+                lpdf *= (1.0f - bg);
+                if (lpdf == 0.0f) continue;
+                const size_t slot = rec0 + conn_slots + nnee++;
+                wave_count(stats, kStatShadow, true);
+                const f3 dd = lp.pos - s.pos;
+                const f3 l2s = normalize(s.pos - lp.pos);
+                const f3 emission = emitted(e, lp.ns, l2s);
+                float fwd, rev;
+                const f3 bc = value_cos_pdf_pair(sh, -l2s, &fwd, &rev);
+                const float jac = area_to_solid_angle(s.pos, lp.pos, lp.ng);
+                fwd *= jac;
+                rev *= rev_jac;
+                f3 term = gray(0.0f);
+                if (!(is_zero(emission) || is_zero(bc) || jac == 0.0f || fwd == 0.0f || rev == 0.0f)) {
                     const float pdf_emit = B.emitter_pdf(e, lp.ns, l2s, area_to_solid_angle(lp.pos, s.pos, s.ng));
                     float sum = 1.0f;
                     sum += sdiv(fwd, lpdf);
                     sum += sdiv(B.camera_reciprocals(c - 1, pdf_emit, rev), lpdf);
-                    const float mis = srcp(sum);
-                    const f3 w = emission * bc * sdiv(jac, lpdf);
-                    value = value + prefix * (mis * w);
+                    term = srcp(sum) * (emission * bc * sdiv(jac, lpdf));
+                }
+                P.bc_rc[slot] = f4_of(term, 0.0f);
+                if (!is_zero(term)) {
+                    P.bc_r0[slot] = f4_of(offset_point(s, dd), 0.0f);
+                    P.bc_r1[slot] = f4_of(offset_point(lp, -dd), 0.0f);
+                    P.bc_shadow[atomicAdd(P.bc_count + 2, 1u)] = (uint32_t)slot;
                 }
             }
-            L = L + value;
-            if (depth + 1 >= P.max_depth + 1) break;
-            const float q = depth > 4 ? clampf(average(approx), 0.05f, 0.95f) : 1.0f;
-            if (rng.next_float() > q) break;
-            const float pc = rng.next_float();
-            const f2 pd = rng.next_float2();
-            const BsdfSample bs = sample_bsdf<true>(sh, pc, pd);
-            approx = approx * (bs.weight / q);
-            P.cam[(size_t)(c - 1) * P.num_pixels + pix].y = bs.pdf_rev * rev_jac;  // OnContinue
-            if (bs.pdf == 0.0f || is_zero(bs.weight)) break;
-            prefix = prefix * (bs.weight / q);
-            ++depth;
-            pdf_dir = bs.pdf;
-            prev_pos = s.pos;
-            prev_n = s.ng;
-            o = offset_point(s, bs.dir);
-            d = bs.dir;
-            tmin = s.err;
         }
+        P.bc_v0[pix] = f4_of(value, __int_as_float(nconn | (conn_block ? 0x10000 : 0)));
+        P.bc_v1[pix] = f4_of(prefix, __int_as_float(nnee));
+        // the continuation (RandomWalk.ContinueWalk)
+        bool cont = depth + 1 < P.max_depth + 1;
+        if (cont) {
+            const float q = depth > 4 ? clampf(average(approx), 0.05f, 0.95f) : 1.0f;
+            cont = !(rng.next_float() > q);
+            if (cont) {
+                const float pc = rng.next_float();
+                const f2 pd = rng.next_float2();
+                const BsdfSample bs = sample_bsdf<true>(sh, pc, pd);
+                approx = approx * (bs.weight / q);
+                P.cam[(size_t)(c - 1) * P.num_pixels + pix].y = bs.pdf_rev * rev_jac;  // OnContinue
+                cont = !(bs.pdf == 0.0f || is_zero(bs.weight));
+                if (cont) {
+                    prefix = prefix * (bs.weight / q);
+                    ++depth;
+                    P.bc_a[pix] = f4_of(prefix, bs.pdf);
+                    P.bc_b[pix] = f4_of(approx, __int_as_float(depth));
+                    P.bc_o[pix] = f4_of(offset_point(s, bs.dir), s.err);
+                    P.bc_d[pix] = f4_of(bs.dir, 0.0f);
+                    P.bc_pp[pix] = f4_of(s.pos, 0.0f);
+                    P.bc_pn[pix] = f4_of(s.ng, 0.0f);
+                    P.bc_rng[pix] = rng.state;
+                    next[atomicAdd(P.bc_count + (1 - in), 1u)] = pix;
+                }
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(SSPT_TRACE_WAVES)))
+k_bc_shadow(DevScene S, int scene, BidirParams P, unsigned long long* stats) {
+    __shared__ int stack_node[kStackNodeLds];
+    __shared__ float stack_t[kStackLds * kTraceBlock];
+    const SceneRec& R = S.recs[scene];
+    const uint32_t n = P.bc_count[2];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t slot = P.bc_shadow[i];
+        const float4 r0 = P.bc_r0[slot], r1 = P.bc_r1[slot];
+        const f3 p0 = f3_of(r0), p1 = f3_of(r1);
+        const bool to_infinity = r1.w != 0.0f;
+        const bool occ = traverse<true>(S, R.root, R.abs_max, p0, to_infinity ? p1 : p1 - p0,
+                                        to_infinity ? r0.w : 1e-5f, to_infinity ? INFINITY : 1.0f - 1e-5f,
+                                        stack_node + threadIdx.x, stack_t + threadIdx.x)
+                             .li >= 0;
+        wave_count(stats, kStatShadowTraced, true);
+        wave_count(stats, kStatOccluded, occ);
+        P.bc_rc[slot].w = occ ? 0.0f : 1.0f;
+    }
+}
+
+// The vertex's value in ClassicBidir.OnCameraHit's order: its emitter term,
+// prefix times the sum of the visible connections, then each visible next event.
+__global__ void __launch_bounds__(256) k_bc_gather(BidirParams P, int in) {
+    const uint32_t n = P.bc_count[in];
+    const uint32_t* list = bc_list(P, in);
+    const int conn_slots = max(P.max_depth - 1, 0);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t pix = list[i];
+        const float4 v0 = P.bc_v0[pix], v1 = P.bc_v1[pix];
+        const int cw = __float_as_int(v0.w), nconn = cw & 0xFFFF, nnee = __float_as_int(v1.w);
+        const f3 prefix = f3_of(v1);
+        const size_t rec0 = (size_t)pix * P.bc_rec;
+        f3 value = f3_of(v0);
+        if (cw & 0x10000) {
+            f3 conn = gray(0.0f);
+            for (int r = 0; r < nconn; ++r) {
+                const float4 rc = P.bc_rc[rec0 + r];
+                if (rc.w != 0.0f) conn = conn + f3_of(rc);
+            }
+            value = value + prefix * conn;
+        }
+        for (int r = 0; r < nnee; ++r) {
+            const float4 rc = P.bc_rc[rec0 + conn_slots + r];
+            if (rc.w != 0.0f) value = value + prefix * f3_of(rc);
+        }
+        P.bc_L[pix] = f4_of(f3_of(P.bc_L[pix]) + value, 0.0f);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_bc_finish(BidirParams P, float* image, unsigned long long* stats) {
+    for (uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x; pix < P.num_pixels; pix += gridDim.x * blockDim.x) {
+        const f3 L = f3_of(P.bc_L[pix]);
         const bool bad = !isfinite(average(L));
         wave_count(stats, kStatNan, bad);
         if (!bad) {
@@ -3501,10 +3625,25 @@ void launch_bidir_light(hipStream_t s, const DevScene& sc, int scene, const Bidi
                        dim3(kTraceBlock), 0, s, sc, scene, p, image, stats);
 }
 
-void launch_bidir_camera(hipStream_t s, const DevScene& sc, int scene, const BidirParams& p, float* image,
-                         unsigned long long* stats) {
-    hipLaunchKernelGGL(p.env ? k_bidir_camera<true> : k_bidir_camera<false>, dim3(blocks(p.num_paths, kTraceBlock)),
-                       dim3(kTraceBlock), 0, s, sc, scene, p, image, stats);
+void launch_bidir_camera_wave(hipStream_t s, const DevScene& sc, int scene, const BidirParams& p, float* image,
+                              unsigned long long* stats) {
+    const uint32_t n = p.num_pixels;
+    const uint32_t grid = blocks(n, 256);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(1), 0, s, p.bc_count, (uint64_t)1, n);
+    hipLaunchKernelGGL(p.env ? k_bc_init<true> : k_bc_init<false>, dim3(grid), dim3(256), 0, s, sc, scene, p);
+    int in = 0;
+    for (int depth = 1; depth < p.max_depth + 1; ++depth, in = 1 - in) {
+        // the next bounce's list and the record list start empty
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(1), 0, s, p.bc_count + (1 - in), (uint64_t)1, 0u);
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(1), 0, s, p.bc_count + 2, (uint64_t)1, 0u);
+        hipLaunchKernelGGL(k_bc_trace, dim3(blocks(n, kTraceBlock)), dim3(kTraceBlock), 0, s, sc, scene, p, in, stats);
+        hipLaunchKernelGGL(p.env ? k_bc_shade<true> : k_bc_shade<false>, dim3(blocks(n, kTraceBlock)),
+                           dim3(kTraceBlock), 0, s, sc, scene, p, in, stats);
+        hipLaunchKernelGGL(k_bc_shadow, dim3(blocks((uint64_t)n * std::max(p.bc_rec, 1), kTraceBlock)),
+                           dim3(kTraceBlock), 0, s, sc, scene, p, stats);
+        hipLaunchKernelGGL(k_bc_gather, dim3(grid), dim3(256), 0, s, p, in);
+    }
+    hipLaunchKernelGGL(k_bc_finish, dim3(grid), dim3(256), 0, s, p, image, stats);
 }
 
 void launch_vcm_prepare(hipStream_t s, const DevScene& sc, int scene, const BidirParams& p, int* scan_tmp) {
